@@ -1,0 +1,185 @@
+/* meep_amd.h — C-ABI of the MI355X FDTD engine, the drop-in for Meep's time-step path.
+ *
+ * Every entry point replaces one method of the reference C++ API (cited as file:line in
+ * NanoComp/meep).  The reference exposes that API to Python through SWIG (python/meep.i);
+ * a ctypes / SWIG binding of the functions below is the same shape.  The C++ classes in
+ * meep_amd.hpp (namespace meep) are a thin layer over these functions, so a C++ caller of
+ * meep::fields::step() links against this library unchanged.
+ *
+ * Conventions
+ *  - Enumerations carry the reference's numeric values (src/meep/vec.hpp:28-79,
+ *    src/meep.hpp:1594), so component/direction integers pass through unchanged.
+ *  - Field arrays are exchanged in the reference layout: one array of ntot = prod(n_d + 1)
+ *    values per component and real/imaginary part, Z fastest in 3D, Y fastest in 2D
+ *    (src/vec.cpp:262-296,482-494).  Values cross the ABI as double whatever the engine's
+ *    arithmetic type.
+ *  - Functions returning int return 0 on success and a negative code on error; the message
+ *    is in mp_last_error().  The C++ layer turns a failure into meep::abort (a
+ *    std::runtime_error "meep: ..."), matching src/mympi.cpp:238-256.
+ *  - A handle is not thread-safe; one host thread drives one fields object (as in the
+ *    reference, src/meep.hpp:1621-1622).
+ */
+#ifndef MEEP_AMD_H
+#define MEEP_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* component (src/meep/vec.hpp:31-54) */
+enum {
+  MP_Ex = 0, MP_Ey = 1, MP_Ez = 4, MP_Hx = 5, MP_Hy = 6, MP_Hz = 9,
+  MP_Dx = 10, MP_Dy = 11, MP_Dz = 14, MP_Bx = 15, MP_By = 16, MP_Bz = 19
+};
+/* direction (src/meep/vec.hpp:78) and boundary_side (:77) */
+enum { MP_X = 0, MP_Y = 1, MP_Z = 2 };
+enum { MP_High = 0, MP_Low = 1 };
+/* boundary_condition (src/meep.hpp:1594) */
+enum { MP_Periodic = 0, MP_Metallic = 1, MP_Magnetic = 2, MP_None = 3 };
+/* state arrays reachable through mp_fields_get_array / mp_fields_set_array */
+enum { MP_ARR_F = 0, MP_ARR_F_U = 1, MP_ARR_F_W = 2, MP_ARR_P = 3, MP_ARR_P_PREV = 4 };
+/* DFT accumulator storage: the reference's complex<realnum> (src/meep.hpp:1138), FP64, or
+   compensated (Kahan) complex<float>. */
+enum { MP_DFT_REALNUM = 0, MP_DFT_F64 = 1, MP_DFT_KAHAN32 = 2 };
+
+typedef struct mp_structure mp_structure;
+typedef struct mp_fields mp_fields;
+typedef struct mp_dft_flux mp_dft_flux;
+
+/* r points at (x, y, z) in the reference's coordinates (origin at the cell's low corner). */
+typedef double (*mp_scalar_func)(const double *r, void *user);
+typedef void (*mp_complex_func)(const double *r, void *user, double *re, double *im);
+
+/* Time dependence of a source: gaussian_src_time(freq, width, start, end)
+   (src/sources.cpp:85-96) when kind == 0, continuous_src_time(freq, width, start, end,
+   slowness) (src/meep.hpp:1040-1044) when kind == 1.  is_integrated as src_time
+   (src/meep.hpp:944-957). */
+typedef struct {
+  int kind;
+  double freq;
+  double width;
+  double start_time;
+  double end_time;
+  double slowness;
+  int is_integrated;
+} mp_src_time;
+
+/* Message of the last failed call on this thread. */
+const char *mp_last_error(void);
+
+/* Name of the engine behind this library ("hip-gfx950" for the product). */
+const char *mp_engine_name(void);
+
+/* ---- structure -------------------------------------------------------------------- */
+
+/* structure(grid_volume, double eps(const vec&), boundary_region, identity(), 0, Courant,
+   use_anisotropic_averaging = false)  — src/meep.hpp:835-839, src/structure.cpp:48-63.
+   dim is 1, 2 or 3 (vol1d/vol2d/vol3d, src/vec.cpp:904-930; sizes of absent directions are
+   ignored).  eps may be NULL (vacuum).  The boundary region is the sum of n_pml
+   pml(thickness, direction, side, R) terms (src/structure.cpp:252-273).  single_precision
+   selects float as realnum (the reference's --enable-single, src/meep.hpp:43-46). */
+mp_structure *mp_structure_new(int dim, double xsize, double ysize, double zsize, double a,
+                               mp_scalar_func eps, void *eps_user, int n_pml,
+                               const int *pml_direction, const int *pml_side,
+                               const double *pml_thickness, const double *pml_R,
+                               double courant, int single_precision);
+void mp_structure_delete(mp_structure *s);
+/* grid_volume::ntot (src/meep/vec.hpp:1056) and num_direction (src/meep/vec.hpp:1028). */
+size_t mp_structure_ntot(const mp_structure *s);
+int mp_structure_num(const mp_structure *s, int *n_xyz);
+double mp_structure_dt(const mp_structure *s);
+/* Overwrite the diagonal chi1inv[c][dir(c)] of electric component c on all ntot points
+   (structure_chunk::chi1inv, src/meep.hpp:597; filled by set_chi1inv,
+   src/anisotropic_averaging.cpp:211-328). */
+int mp_structure_set_chi1inv(mp_structure *s, int c, const double *chi1inv);
+int mp_structure_get_chi1inv(const mp_structure *s, int c, double *chi1inv);
+/* structure::add_susceptibility(sigma, E_stuff, lorentzian_susceptibility(omega_0, gamma,
+   no_omega_0_denominator)) — src/meep.hpp:248,864-865, src/structure.cpp:471-507.
+   Returns the susceptibility's index (0, 1, ...) or a negative code. */
+int mp_structure_add_susceptibility(mp_structure *s, double omega_0, double gamma,
+                                    int no_omega_0_denominator, mp_scalar_func sigma,
+                                    void *sigma_user);
+/* Overwrite sigma[c][dir(c)] of susceptibility isus on all ntot points. */
+int mp_structure_set_sigma(mp_structure *s, int isus, int c, const double *sigma);
+
+/* ---- fields ----------------------------------------------------------------------- */
+
+/* fields::fields(structure*, m = 0, beta = 0) — src/meep.hpp:1740-1742, src/fields.cpp:32-86.
+   Fields are complex until mp_fields_use_real_fields. */
+mp_fields *mp_fields_new(mp_structure *s);
+void mp_fields_delete(mp_fields *f);
+/* fields::use_real_fields — src/meep.hpp:1746, src/fields.cpp:144-156 */
+int mp_fields_use_real_fields(mp_fields *f);
+/* fields::use_bloch(const vec &k) — src/meep.hpp:1763, src/boundaries.cpp:113-119.
+   k holds one real entry per grid direction in X, Y, Z order. */
+int mp_fields_use_bloch(mp_fields *f, const double *k);
+/* fields::use_bloch(direction, complex k) — src/meep.hpp:1762, src/boundaries.cpp:90-111 */
+int mp_fields_use_bloch_dir(mp_fields *f, int d, double k_re, double k_im);
+/* fields::set_boundary — src/meep.hpp:1761, src/boundaries.cpp:78-88 */
+int mp_fields_set_boundary(mp_fields *f, int side, int d, int condition);
+/* Deprecated fields::add_point_source(c, freq, width, peaktime, cutoff, p, amp, is_c)
+   — src/meep.hpp:1888-1889, src/sources.cpp:193-213. */
+int mp_fields_add_point_source_legacy(mp_fields *f, int c, double freq, double width,
+                                      double peaktime, double cutoff, const double *p,
+                                      double amp_re, double amp_im, int is_continuous);
+/* fields::add_volume_source(c, src, volume(vmin, vmax), A, amp)
+   — src/meep.hpp:1893-1897, src/sources.cpp:462-490.  A may be NULL (A = 1). */
+int mp_fields_add_volume_source(mp_fields *f, int c, const mp_src_time *src,
+                                const double *vmin, const double *vmax, double amp_re,
+                                double amp_im, mp_complex_func A, void *A_user);
+/* fields::step() — src/meep.hpp:1871, src/step.cpp:35-139.  Returns an error (the
+   reference aborts) when the fields are not finite. */
+int mp_fields_step(mp_fields *f);
+/* n calls of fields::step() issued back to back with no host synchronisation between
+   them; the finiteness check runs on the device and is read once at the end. */
+int mp_fields_step_n(mp_fields *f, long n);
+/* fields::time(), round_time() — src/meep.hpp:1876-1877; fields::t and dt. */
+double mp_fields_time(const mp_fields *f);
+double mp_fields_round_time(const mp_fields *f);
+long mp_fields_t(const mp_fields *f);
+double mp_fields_dt(const mp_fields *f);
+/* fields::last_source_time — src/meep.hpp:1887 */
+double mp_fields_last_source_time(mp_fields *f);
+/* fields::get_field(component, const vec&) — src/meep.hpp:2183-2185, src/monitor.cpp:93-108 */
+int mp_fields_get_field(mp_fields *f, int c, const double *p, double *re, double *im);
+/* Copy one state array (reference layout, ntot doubles) to the host: kind MP_ARR_F is
+   fields_chunk::f, MP_ARR_F_U f_u, MP_ARR_F_W f_w, MP_ARR_P / MP_ARR_P_PREV the
+   polarisation of susceptibility ipol (lorentzian_data, src/susceptibility.cpp:98-139).
+   Returns 1 (and zeros) when the array is not allocated. */
+int mp_fields_get_array(mp_fields *f, int c, int cmp, int kind, int ipol, double *out);
+/* Overwrite one state array (checkpoint restore, adjoint seeding). */
+int mp_fields_set_array(mp_fields *f, int c, int cmp, int kind, int ipol, const double *in);
+/* Whether component c is allocated (fields::have_component, src/fields.cpp:158-162). */
+int mp_fields_have_component(mp_fields *f, int c);
+/* Per-phase wall time in seconds under the reference's sink names (time_sink,
+   src/meep.hpp:1595-1618): 0 FieldUpdateB, 1 FieldUpdateH, 2 FieldUpdateD, 3 FieldUpdateE,
+   4 BoundarySteppingB..E (sum), 5 FourierTransforming, 6 Stepping (whole steps). */
+int mp_fields_get_times(mp_fields *f, double *times7);
+/* DFT accumulator storage for monitors added later (MP_DFT_*). */
+int mp_fields_set_dft_accumulator(mp_fields *f, int kind);
+
+/* ---- DFT flux ----------------------------------------------------------------------- */
+
+/* fields::add_dft_flux(direction, volume, const double *freq, Nfreq, use_symmetry,
+   centered_grid = true, decimation_factor) — src/meep.hpp:2056-2058, src/dft.cpp:602-650,835-842.
+   decimation_factor 0 picks the reference's automatic factor (src/dft.cpp:196-217). */
+mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin,
+                                    const double *vmax, const double *freq, int nfreq,
+                                    int decimation_factor);
+void mp_dft_flux_delete(mp_dft_flux *fl);
+/* dft_flux::flux() — src/dft.cpp:542-556.  out has nfreq entries. */
+int mp_dft_flux_flux(mp_dft_flux *fl, double *out);
+/* Number of dft_chunks in the E (which = 0) or H (which = 1) list and the points in chunk i. */
+int mp_dft_flux_num_chunks(mp_dft_flux *fl, int which);
+long mp_dft_flux_chunk_points(mp_dft_flux *fl, int which, int i);
+int mp_dft_flux_decimation(mp_dft_flux *fl);
+/* dft_chunk::dft of chunk i, interleaved (re, im), frequency fastest (src/meep.hpp:1138). */
+int mp_dft_flux_get_dft(mp_dft_flux *fl, int which, int i, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEEP_AMD_H */

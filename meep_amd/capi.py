@@ -1,0 +1,385 @@
+"""ctypes mirror of the C-ABI in include/meep_amd.h.
+
+The classes below follow the reference's C++ API names (``structure``, ``fields``,
+``dft_flux``; src/meep.hpp) so scripts read like the reference's own tests
+(tests/known_results.cpp).  Each class takes the ``ctypes.CDLL`` it drives; the package
+entry point :func:`meep_amd.load_library` returns the MI355X engine library.
+"""
+
+import ctypes as C
+
+import numpy as np
+
+# reference enums (src/meep/vec.hpp:31-79, src/meep.hpp:1594)
+Ex, Ey, Ez, Hx, Hy, Hz = 0, 1, 4, 5, 6, 9
+Dx, Dy, Dz, Bx, By, Bz = 10, 11, 14, 15, 16, 19
+X, Y, Z = 0, 1, 2
+High, Low = 0, 1
+Periodic, Metallic, Magnetic, NoBoundary = 0, 1, 2, 3
+ARR_F, ARR_F_U, ARR_F_W, ARR_P, ARR_P_PREV = 0, 1, 2, 3, 4
+DFT_REALNUM, DFT_F64, DFT_KAHAN32 = 0, 1, 2
+
+component_name = {Ex: "ex", Ey: "ey", Ez: "ez", Hx: "hx", Hy: "hy", Hz: "hz",
+                  Dx: "dx", Dy: "dy", Dz: "dz", Bx: "bx", By: "by", Bz: "bz"}
+
+SCALAR_FUNC = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_void_p)
+COMPLEX_FUNC = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_void_p,
+                           C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+class SrcTime(C.Structure):
+    """mp_src_time: gaussian_src_time / continuous_src_time (src/sources.cpp:67-145)."""
+    _fields_ = [("kind", C.c_int), ("freq", C.c_double), ("width", C.c_double),
+                ("start_time", C.c_double), ("end_time", C.c_double),
+                ("slowness", C.c_double), ("is_integrated", C.c_int)]
+
+
+def gaussian_src_time(freq, fwidth, start_time=0.0, cutoff=5.0, is_integrated=False):
+    """Python GaussianSource semantics (python/source.py:326-336)."""
+    width = 1.0 / fwidth
+    return SrcTime(0, freq, width, start_time, start_time + 2 * width * cutoff, 3.0,
+                   int(is_integrated))
+
+
+def continuous_src_time(freq, width=0.0, start_time=0.0, end_time=1e20, slowness=3.0,
+                        is_integrated=False):
+    return SrcTime(1, freq, width, start_time, end_time, slowness, int(is_integrated))
+
+
+_SIGNATURES = {
+    "mp_last_error": (C.c_char_p, []),
+    "mp_engine_name": (C.c_char_p, []),
+    "mp_structure_new": (C.c_void_p, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      SCALAR_FUNC, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.c_double, C.c_int]),
+    "mp_structure_delete": (None, [C.c_void_p]),
+    "mp_structure_ntot": (C.c_size_t, [C.c_void_p]),
+    "mp_structure_num": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "mp_structure_dt": (C.c_double, [C.c_void_p]),
+    "mp_structure_set_chi1inv": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_structure_get_chi1inv": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_structure_add_susceptibility": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                                  SCALAR_FUNC, C.c_void_p]),
+    "mp_structure_set_sigma": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mp_fields_new": (C.c_void_p, [C.c_void_p]),
+    "mp_fields_delete": (None, [C.c_void_p]),
+    "mp_fields_use_real_fields": (C.c_int, [C.c_void_p]),
+    "mp_fields_use_bloch": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_fields_use_bloch_dir": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
+    "mp_fields_set_boundary": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "mp_fields_add_point_source_legacy": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                                    C.c_double, C.c_double,
+                                                    C.POINTER(C.c_double), C.c_double,
+                                                    C.c_double, C.c_int]),
+    "mp_fields_add_volume_source": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SrcTime),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                              C.c_double, C.c_double, COMPLEX_FUNC,
+                                              C.c_void_p]),
+    "mp_fields_step": (C.c_int, [C.c_void_p]),
+    "mp_fields_step_n": (C.c_int, [C.c_void_p, C.c_long]),
+    "mp_fields_time": (C.c_double, [C.c_void_p]),
+    "mp_fields_round_time": (C.c_double, [C.c_void_p]),
+    "mp_fields_t": (C.c_long, [C.c_void_p]),
+    "mp_fields_dt": (C.c_double, [C.c_void_p]),
+    "mp_fields_last_source_time": (C.c_double, [C.c_void_p]),
+    "mp_fields_get_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mp_fields_get_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.c_double)]),
+    "mp_fields_set_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.c_double)]),
+    "mp_fields_have_component": (C.c_int, [C.c_void_p, C.c_int]),
+    "mp_fields_get_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_fields_set_dft_accumulator": (C.c_int, [C.c_void_p, C.c_int]),
+    "mp_fields_add_dft_flux": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.c_int, C.c_int]),
+    "mp_dft_flux_delete": (None, [C.c_void_p]),
+    "mp_dft_flux_flux": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_dft_flux_num_chunks": (C.c_int, [C.c_void_p, C.c_int]),
+    "mp_dft_flux_chunk_points": (C.c_long, [C.c_void_p, C.c_int, C.c_int]),
+    "mp_dft_flux_decimation": (C.c_int, [C.c_void_p]),
+    "mp_dft_flux_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def bind(lib):
+    """Attach argtypes/restype to every entry point of ``lib``; return ``lib``."""
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+class MeepError(RuntimeError):
+    """Raised where the reference calls meep::abort (src/mympi.cpp:238-256)."""
+
+
+def _check(lib, rc):
+    if rc is not None and rc < 0:
+        raise MeepError(lib.mp_last_error().decode())
+    return rc
+
+
+def _dvec(vals):
+    arr = (C.c_double * 3)(*(list(vals) + [0.0] * (3 - len(vals))))
+    return arr
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class structure:
+    """meep::structure (src/meep.hpp:815-930) with eps given as a function of (x, y, z)."""
+
+    def __init__(self, lib, dim, size, a, eps=None, pml=(), courant=0.5, single=False):
+        self.lib = lib
+        self._keep = []
+        size = list(size) + [0.0] * (3 - len(size))
+        if dim == 1:
+            size = [0.0, 0.0, size[0]] if size[2] == 0 else size
+        eps_cb = SCALAR_FUNC(0) if eps is None else SCALAR_FUNC(
+            lambda r, _u: float(eps(r[0], r[1], r[2])))
+        self._keep.append(eps_cb)
+        n = len(pml)
+        dirs = (C.c_int * max(n, 1))(*[p[0] for p in pml])
+        sides = (C.c_int * max(n, 1))(*[p[1] for p in pml])
+        thick = (C.c_double * max(n, 1))(*[p[2] for p in pml])
+        R = (C.c_double * max(n, 1))(*[(p[3] if len(p) > 3 else 1e-15) for p in pml])
+        self.ptr = lib.mp_structure_new(dim, size[0], size[1], size[2], a, eps_cb, None, n,
+                                        dirs, sides, thick, R, courant, int(single))
+        if not self.ptr:
+            raise MeepError(lib.mp_last_error().decode())
+        self.dim = dim
+        self.single = single
+        self.a = a
+
+    @property
+    def ntot(self):
+        return self.lib.mp_structure_ntot(self.ptr)
+
+    @property
+    def num(self):
+        n = (C.c_int * 3)()
+        self.lib.mp_structure_num(self.ptr, n)
+        return tuple(n)
+
+    @property
+    def shape(self):
+        """Array shape of a field in the reference layout (slowest direction first)."""
+        n = self.num
+        if self.dim == 1:
+            return (n[2] + 1,)
+        if self.dim == 2:
+            return (n[0] + 1, n[1] + 1)
+        return (n[0] + 1, n[1] + 1, n[2] + 1)
+
+    @property
+    def dt(self):
+        return self.lib.mp_structure_dt(self.ptr)
+
+    def set_chi1inv(self, c, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        assert arr.size == self.ntot
+        _check(self.lib, self.lib.mp_structure_set_chi1inv(self.ptr, c, _dptr(arr)))
+
+    def get_chi1inv(self, c):
+        out = np.empty(self.ntot)
+        _check(self.lib, self.lib.mp_structure_get_chi1inv(self.ptr, c, _dptr(out)))
+        return out.reshape(self.shape)
+
+    def add_susceptibility(self, sigma, omega_0, gamma, drude=False):
+        """add_susceptibility(sigma, E_stuff, lorentzian_susceptibility(omega_0, gamma, drude))."""
+        cb = SCALAR_FUNC(lambda r, _u: float(sigma(r[0], r[1], r[2])))
+        self._keep.append(cb)
+        return _check(self.lib, self.lib.mp_structure_add_susceptibility(
+            self.ptr, omega_0, gamma, int(drude), cb, None))
+
+    def set_sigma(self, isus, c, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        _check(self.lib, self.lib.mp_structure_set_sigma(self.ptr, isus, c, _dptr(arr)))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.lib.mp_structure_delete(self.ptr)
+            self.ptr = None
+
+
+class dft_flux:
+    """meep::dft_flux (src/meep.hpp:1209-1255)."""
+
+    def __init__(self, fields, ptr, nfreq):
+        self.f = fields
+        self.lib = fields.lib
+        self.ptr = ptr
+        self.nfreq = nfreq
+
+    def flux(self):
+        out = np.empty(self.nfreq)
+        _check(self.lib, self.lib.mp_dft_flux_flux(self.ptr, _dptr(out)))
+        return out
+
+    def num_chunks(self, which):
+        return self.lib.mp_dft_flux_num_chunks(self.ptr, which)
+
+    @property
+    def decimation(self):
+        return self.lib.mp_dft_flux_decimation(self.ptr)
+
+    def dft(self, which, i=0):
+        n = self.lib.mp_dft_flux_chunk_points(self.ptr, which, i)
+        out = np.empty(2 * n * self.nfreq)
+        _check(self.lib, self.lib.mp_dft_flux_get_dft(self.ptr, which, i, _dptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
+
+
+class fields:
+    """meep::fields (src/meep.hpp:1716-2320)."""
+
+    def __init__(self, s):
+        self.lib = s.lib
+        self.s = s
+        self._keep = []
+        self.ptr = self.lib.mp_fields_new(s.ptr)
+        if not self.ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        self._fluxes = []
+
+    def use_real_fields(self):
+        _check(self.lib, self.lib.mp_fields_use_real_fields(self.ptr))
+
+    def use_bloch(self, k):
+        kk = (C.c_double * 3)(*(list(k) + [0.0] * (3 - len(k))))
+        _check(self.lib, self.lib.mp_fields_use_bloch(self.ptr, kk))
+
+    def use_bloch_dir(self, d, k):
+        k = complex(k)
+        _check(self.lib, self.lib.mp_fields_use_bloch_dir(self.ptr, d, k.real, k.imag))
+
+    def set_boundary(self, side, d, cond):
+        _check(self.lib, self.lib.mp_fields_set_boundary(self.ptr, side, d, cond))
+
+    def add_point_source(self, c, freq, width, peaktime, cutoff, p, amp=1.0, is_continuous=0):
+        """Deprecated C++ add_point_source (src/sources.cpp:193-213)."""
+        amp = complex(amp)
+        _check(self.lib, self.lib.mp_fields_add_point_source_legacy(
+            self.ptr, c, freq, width, peaktime, cutoff, _dvec(p), amp.real, amp.imag,
+            int(is_continuous)))
+
+    def add_volume_source(self, c, src, vmin, vmax, amp=1.0, amp_func=None):
+        amp = complex(amp)
+        if amp_func is None:
+            cb = COMPLEX_FUNC(0)
+        else:
+            def _cb(r, _u, re, im):
+                v = complex(amp_func(r[0], r[1], r[2]))
+                re[0] = v.real
+                im[0] = v.imag
+            cb = COMPLEX_FUNC(_cb)
+        self._keep.append(cb)
+        _check(self.lib, self.lib.mp_fields_add_volume_source(
+            self.ptr, c, C.byref(src), _dvec(vmin), _dvec(vmax), amp.real, amp.imag, cb, None))
+
+    def add_point_source_src(self, c, src, p, amp=1.0):
+        self.add_volume_source(c, src, p, p, amp)
+
+    def step(self):
+        _check(self.lib, self.lib.mp_fields_step(self.ptr))
+
+    def step_n(self, n):
+        _check(self.lib, self.lib.mp_fields_step_n(self.ptr, int(n)))
+
+    def time(self):
+        return self.lib.mp_fields_time(self.ptr)
+
+    def round_time(self):
+        return self.lib.mp_fields_round_time(self.ptr)
+
+    @property
+    def t(self):
+        return self.lib.mp_fields_t(self.ptr)
+
+    @property
+    def dt(self):
+        return self.lib.mp_fields_dt(self.ptr)
+
+    def last_source_time(self):
+        return self.lib.mp_fields_last_source_time(self.ptr)
+
+    def get_field(self, c, p):
+        re, im = C.c_double(), C.c_double()
+        _check(self.lib, self.lib.mp_fields_get_field(self.ptr, c, _dvec(p), C.byref(re),
+                                                      C.byref(im)))
+        return complex(re.value, im.value)
+
+    def have_component(self, c):
+        return bool(self.lib.mp_fields_have_component(self.ptr, c))
+
+    def get_array(self, c, cmp=0, kind=ARR_F, ipol=0):
+        out = np.empty(self.s.ntot)
+        rc = _check(self.lib, self.lib.mp_fields_get_array(self.ptr, c, cmp, kind, ipol,
+                                                           _dptr(out)))
+        return out.reshape(self.s.shape), rc == 0
+
+    def set_array(self, c, cmp, kind, arr, ipol=0):
+        arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        _check(self.lib, self.lib.mp_fields_set_array(self.ptr, c, cmp, kind, ipol, _dptr(arr)))
+
+    def times(self):
+        t = (C.c_double * 7)()
+        self.lib.mp_fields_get_times(self.ptr, t)
+        return list(t)
+
+    def set_dft_accumulator(self, kind):
+        _check(self.lib, self.lib.mp_fields_set_dft_accumulator(self.ptr, kind))
+
+    def add_dft_flux(self, d, vmin, vmax, freqs, decimation_factor=0):
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        ptr = self.lib.mp_fields_add_dft_flux(self.ptr, d, _dvec(vmin), _dvec(vmax),
+                                              _dptr(freqs), len(freqs), decimation_factor)
+        if not ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        fl = dft_flux(self, ptr, len(freqs))
+        self._fluxes.append(fl)
+        return fl
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            for fl in self._fluxes:
+                if fl.ptr:
+                    self.lib.mp_dft_flux_delete(fl.ptr)
+                    fl.ptr = None
+            self.lib.mp_fields_delete(self.ptr)
+            self.ptr = None
+
+
+def linspace(fmin, fmax, n):
+    """meep::linspace (src/dft.cpp:30-40)."""
+    if n <= 1:
+        return np.array([(fmin + fmax) * 0.5])
+    df = (fmax - fmin) / (n - 1)
+    return np.array([fmin + i * df for i in range(n)])
+
+
+def volone(zsize, a):
+    return dict(dim=1, size=(0.0, 0.0, zsize), a=a)
+
+
+def voltwo(xsize, ysize, a):
+    return dict(dim=2, size=(xsize, ysize, 0.0), a=a)
+
+
+def vol3d(xsize, ysize, zsize, a):
+    return dict(dim=3, size=(xsize, ysize, zsize), a=a)
+
+
+def pml(thickness, direction=None, side=None, R=1e-15):
+    """boundary_region pml(...) terms (src/structure.cpp:252-273)."""
+    dirs = [X, Y, Z] if direction is None else [direction]
+    sides = [Low, High] if side is None else [side]
+    return [(d, s, thickness, R) for d in dirs for s in sides]
