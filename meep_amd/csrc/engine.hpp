@@ -1,0 +1,117 @@
+// engine.hpp — parameter blocks shared by the host plan (fields.cpp) and the CDNA4 kernels
+// (kernels.hip).  All field arrays use the reference layout: one array of ntot values per
+// component and real/imaginary part (src/vec.cpp:293-296,482-494); a launch covers every
+// array element with one thread and each thread handles all components of its cell.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mx {
+
+constexpr int MAXPOLE = 4;  // Lorentz/Drude poles per field type
+constexpr int SRC_TAB = 12; // sparse source entries carried in the kernel arguments
+
+// Geometry of one chunk.  Loop axis k (0 slowest, 2 stride-1) walks direction axdir[k]
+// (-1 when the grid has no such direction); this is the reference's yucky_direction order
+// (src/vec.cpp:262-270): X,Y,Z in 3D, Z(empty),X,Y in 2D, Z in 1D.
+struct Grid {
+  uint32_t ntot;
+  int ext[3];    // array extent along each loop axis (n+1, or 1 for an absent axis)
+  int axdir[3];  // direction of each loop axis
+  int num[3];    // cells per direction X,Y,Z (0 when absent)
+  int hi_metal;  // bit d set: the High wall normal to d is Metallic (src/boundaries.cpp:191-205)
+  int lo_magnetic; // bit d set: the Low wall normal to d is Magnetic
+};
+
+template <class R> struct PmlProfile {
+  const R *sig[3], *kap[3], *siginv[3]; // per direction; null when that direction has no PML
+};
+
+// One B or D component of the curl update (step_curl, src/step_generic.cpp:65-249).
+template <class R> struct CurlComp {
+  R *f;
+  const R *g1, *g2; // curl partners after the reference's swap (g1 never null unless both are)
+  int64_t s1, s2;   // partner strides (negated for the D update, src/step_db.cpp:81-84)
+  R dtdx;
+  R *fu;     // PML auxiliary (dsigu); null when absent
+  int dsig;  // PML direction acting on f, -1 if none
+  int dsigu; // PML direction acting through fu, -1 if none
+  int lo[3], hi[3]; // owned index range per direction (inclusive)
+  int ysh[3];       // Yee shift per direction (PML index k = 2*j + ysh)
+};
+
+// One E or H component of update_eh (src/update_eh.cpp:66-215, src/step_generic.cpp:566-785)
+// plus its Lorentzian polarisations (src/susceptibility.cpp:188-281).
+template <class R> struct UpdComp {
+  R *f;       // E (or H)
+  const R *g; // D (or B)
+  const R *u; // diagonal chi1inv; null means 1
+  R *fw;      // PML auxiliary W; null when this direction has no PML
+  int dsigw;  // -1 when none
+  int lo[3], hi[3], ysh[3];
+  R *P[MAXPOLE], *Pp[MAXPOLE];
+  const R *sigma[MAXPOLE];
+};
+
+template <class R> struct PoleCoef {
+  R gamma1inv, gamma1, omega0dtsqr, omega0dtsqr_denom;
+};
+
+// Sparse per-step source values: f[idx] of component comp loses val (already the real or
+// imaginary part of amp * current * dt, or of amp * dipole), subtracted in double as the
+// reference does (src/step.cpp:310-315, src/update_eh.cpp:128-138).
+struct SrcTab {
+  int n;
+  int comp[SRC_TAB];
+  int64_t idx[SRC_TAB];
+  double val[SRC_TAB];
+};
+
+template <class R> struct StepArgs {
+  Grid g;
+  PmlProfile<R> pml;
+  int nc;
+  CurlComp<R> cc[3];
+  UpdComp<R> uc[3];
+  int npole;
+  PoleCoef<R> pole[MAXPOLE];
+  SrcTab cur; // currents, after the curl
+  SrcTab dip; // integrated dipoles, inside f_minus_p
+};
+
+// Halo connection (connect_the_chunks / process_incoming_chunk_data,
+// src/boundaries.cpp:368-600, src/step.cpp:172-223).  slot indexes a table of array pairs.
+struct Conn {
+  int64_t dst, src;
+  int32_t slot, cls; // cls: 0 phase, 1 negate, 2 copy
+  double ph_re, ph_im;
+};
+
+// DFT accumulation (dft_chunk::update_dft, src/dft.cpp:266-308) over npts points.
+// The per-frequency phase exp(i omega t) * scale is formed on the device (src/dft.cpp:270-271).
+template <class R> struct DftArgs {
+  const R *f0, *f1; // real / imaginary field arrays (f1 null for real fields)
+  const int64_t *idx;
+  const double *w;
+  const double *omega;
+  double time, scale_re, scale_im;
+  int64_t avg1, avg2;
+  int npts, nfreq;
+  void *acc;    // accumulators, frequency fastest
+  void *comp;   // Kahan compensation (float2), or null
+  int acc_kind; // MP_DFT_* of meep_amd.h
+};
+
+// launchers (kernels.hip)
+template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
+template <class R>
+void launch_conn(const Conn *conns, int n, R *const *slot_ptrs, int ncmp, hipStream_t s);
+template <class R> void launch_src(R *f, const int64_t *idx, const double *val, int n, hipStream_t s);
+template <class R> void launch_dft(const DftArgs<R> &a, hipStream_t s);
+template <class R>
+void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int n, int *flag,
+                  hipStream_t s);
+
+} // namespace mx

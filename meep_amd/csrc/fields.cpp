@@ -1,0 +1,1728 @@
+// fields.cpp — host side of the MI355X engine: structure setup, the per-step launch plan and
+// the C-ABI of include/meep_amd.h.
+//
+// One fields object is one chunk on one GPU with one HIP stream.  State arrays live in HBM in
+// the reference layout; the host keeps only the plan (component descriptors, halo tables,
+// sparse sources, DFT point lists) and never touches field values during a step.  There is
+// no CPU path: every entry point that advances or reads fields goes through the kernels in
+// kernels.hip and fails with an error when no GPU is usable.
+#include "../../include/meep_amd.h"
+#include "engine.hpp"
+#include "grid.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mx {
+
+typedef std::complex<double> cdouble;
+const double pi = 3.141592653589793238462643383279502884197;
+enum { E_ft = 0, H_ft = 1, D_ft = 2, B_ft = 3 };
+enum { Periodic = 0, Metallic = 1, Magnetic = 2, None = 3 };
+enum { High = 0, Low = 1 };
+
+thread_local std::string g_last_error;
+
+struct mx_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+[[noreturn]] static void fail(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw mx_error(std::string("meep: ") + buf);
+}
+#define HIPCHECK(x)                                                                                \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) fail("HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+bool Geom::interpolate(int c, const V3 &pc, IV locs[8], double w[8]) const {
+  const double SMALL = 1e-13;
+  V3 p;
+  for (int d = 0; d < 3; ++d)
+    if (has(d)) p.t[d] = (pc.t[d] - ysh(c, d) * 0.5 * inva) * a;
+  IV mid;
+  for (int d = 0; d < 3; ++d)
+    if (has(d)) mid.t[d] = int(std::floor(p.t[d])) * 2 + 1 + ysh(c, d);
+  V3 midv = loc(mid), dv;
+  for (int d = 0; d < 3; ++d)
+    if (has(d)) dv.t[d] = (pc.t[d] - midv.t[d]) * (2 * a);
+  int have = 1;
+  for (int i = 0; i < 8; ++i) {
+    locs[i] = round_vec(midv);
+    w[i] = 1.0;
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (!has(d)) continue;
+    for (int i = 0; i < have; ++i) {
+      locs[have + i] = locs[i];
+      w[have + i] = w[i];
+      locs[i].t[d] = mid.t[d] - 1;
+      w[i] *= 0.5 * (1.0 - dv.t[d]);
+      locs[have + i].t[d] = mid.t[d] + 1;
+      w[have + i] *= 0.5 * (1.0 + dv.t[d]);
+    }
+    have *= 2;
+  }
+  for (int i = have; i < 8; ++i) w[i] = 0.0;
+  double tw = 0;
+  for (int i = 0; i < have; ++i) tw += w[i];
+  for (int i = 0; i < have; ++i) w[i] += (1.0 - tw) * (1.0 / have);
+  for (int i = 0; i < have; ++i) {
+    if (w[i] < 0.0) {
+      if (-w[i] >= SMALL * 1e5) fail("large negative interpolation weight");
+      w[i] = 0.0;
+    }
+    else if (w[i] < SMALL)
+      w[i] = 0.0;
+  }
+  // compact the non-zero weights to the front (src/vec.cpp:511-526)
+  {
+    IV *l = locs;
+    double *ww = w;
+    int n = have;
+    while (n) {
+      if (std::fabs(ww[0]) < 2e-15) {
+        ww[0] = ww[n - 1];
+        l[0] = l[n - 1];
+        ww[n - 1] = 0.0;
+        l[n - 1] = IV();
+      }
+      else {
+        ++ww;
+        ++l;
+      }
+      --n;
+    }
+  }
+  bool same = true;
+  for (int i = 0; i < 8 && w[i]; ++i)
+    if (w[i] != w[0]) same = false;
+  if (same) {
+    int nw = 0;
+    for (int i = 0; i < 8 && w[i]; ++i) ++nw;
+    for (int i = 0; i < 8 && w[i]; ++i) w[i] = 1.0 / nw;
+  }
+  return true;
+}
+
+// ---- host structure -------------------------------------------------------------------
+struct HostSus {
+  double omega_0, gamma; // stored as realnum by the reference (src/meep.hpp:276)
+  bool drude;
+  std::vector<double> sigma[NC]; // diagonal sigma[c][dir(c)], empty when trivial
+};
+
+struct HostStructure {
+  Geom gv;
+  bool single = false;
+  double courant = 0.5, dt = 0.5;
+  std::vector<double> chi1inv[NC]; // diagonal, empty when trivial
+  // uPML profile in double; each side's profile written into one array (one chunk holds both)
+  std::vector<double> sig[3], kap[3];
+  bool pml_dir[3] = {false, false, false};
+  std::vector<HostSus> sus;
+};
+
+// structure_chunk::use_pml (src/structure.cpp:625-691) with the quadratic profile of pml()
+static void host_use_pml(HostStructure &s, int d, double dx, int side, double Rasym) {
+  if (dx <= 0.0) return;
+  const Geom &gv = s.gv;
+  const double a = gv.a;
+  const double prefac = (-std::log(Rasym)) / (4 * dx * (1. / 3.));
+  const double bloc = side == High ? gv.num[d] * gv.inva : 0.0;
+  auto pml_x = [&](int i) {
+    double here = i * 0.5 / a;
+    return 0.5 / a * ((int)(dx * (2 * a) + 0.5) - (int)(std::fabs(bloc - here) * (2 * a) + 0.5));
+  };
+  bool found = false;
+  for (int i = 0; i <= 2 * gv.num[d] + 1; ++i)
+    if (pml_x(i) > 0) found = true;
+  if (!found) return;
+  if (!s.pml_dir[d]) {
+    s.pml_dir[d] = true;
+    s.sig[d].assign(2 * gv.num[d] + 2, 0.0);
+    s.kap[d].assign(2 * gv.num[d] + 2, 1.0);
+  }
+  for (int i = 0; i <= 2 * gv.num[d] + 1; ++i) {
+    double x = pml_x(i);
+    if (x > 0) {
+      double sp = (x / dx) * (x / dx);
+      s.sig[d][i] = 0.5 * s.dt * prefac * sp;
+      s.kap[d][i] = 1.0; // 1 + (mean_stretch - 1) / (1/4) * sp * (x / dx), mean_stretch = 1
+    }
+  }
+}
+
+// ---- sources ----------------------------------------------------------------------------
+struct SrcTime {
+  int kind = 0;
+  double freq = 0, width = 0, peak_time = 0, cutoff = 0, fwidth = 0;
+  double start_time = 0, end_time = 0, slowness = 3.0;
+  bool is_integrated = true;
+
+  static SrcTime gaussian(double f, double w, double st, double et) {
+    // gaussian_src_time(f, w, start, end) — src/sources.cpp:85-96
+    SrcTime s;
+    s.freq = f;
+    s.width = w;
+    s.peak_time = 0.5 * (st + et);
+    s.cutoff = (et - st) * 0.5;
+    s.fwidth = std::sqrt(-2.0 * std::log(1e-7)) / (w * pi);
+    while (std::exp(-s.cutoff * s.cutoff / (2 * w * w)) < 1e-100) s.cutoff *= 0.9;
+    s.cutoff = float(s.cutoff);
+    return s;
+  }
+  static SrcTime continuous(double f, double w, double st, double et, double slow) {
+    SrcTime s;
+    s.kind = 1;
+    s.freq = f;
+    s.width = w;
+    s.start_time = float(st);
+    s.end_time = float(et);
+    s.slowness = slow;
+    return s;
+  }
+  cdouble dipole(double time) const {
+    if (kind == 0) { // src/sources.cpp:98-108
+      double tt = time - peak_time;
+      if (float(std::fabs(tt)) > cutoff) return 0.0;
+      cdouble amp = 1.0 / cdouble(0, -2 * pi * freq);
+      return std::exp(-tt * tt / (2 * width * width)) * std::polar(1.0, -2 * pi * freq * tt) * amp;
+    }
+    float rtime = float(time); // src/sources.cpp:128-145
+    if (rtime < start_time || rtime > end_time) return 0.0;
+    cdouble amp = 1.0 / (cdouble(0, -1.0) * (2 * pi) * freq);
+    if (width == 0.0) return std::exp(cdouble(0, -1.0) * (2 * pi) * freq * time) * amp;
+    double ts = (time - start_time) / width - slowness;
+    double te = (end_time - time) / width - slowness;
+    return std::exp(cdouble(0, -1.0) * (2 * pi) * freq * time) * amp * (1.0 + std::tanh(ts)) *
+           (1.0 + std::tanh(te)) * 0.25;
+  }
+  // src_time::current (src/meep.hpp:989-991)
+  cdouble current(double time, double dt) const { return (dipole(time + dt) - dipole(time)) / dt; }
+  double last_time() const { return kind == 0 ? float(peak_time + cutoff) : end_time; }
+  double get_fwidth() const { return kind == 0 ? fwidth : 0.0; }
+  bool equal(const SrcTime &o) const {
+    if (kind != o.kind || is_integrated != o.is_integrated) return false;
+    if (kind == 0) return freq == o.freq && width == o.width && peak_time == o.peak_time && cutoff == o.cutoff;
+    return freq == o.freq && width == o.width && start_time == o.start_time && end_time == o.end_time &&
+           slowness == o.slowness;
+  }
+};
+
+struct SrcVol { // src_vol (src/meep_internals.hpp:49-82)
+  int c;        // E or H component
+  int src;      // index into the source-time list
+  std::vector<int64_t> index;
+  std::vector<cdouble> amp;
+  // device copies for lists longer than the kernel-argument table
+  int64_t *d_idx = nullptr;
+  double *d_val = nullptr;
+};
+
+// ---- device buffers -----------------------------------------------------------------------
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = n;
+    if (n) HIPCHECK(hipMalloc(&p, n));
+  }
+};
+
+static void ensure_gpu() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    fail("no usable GPU: the MI355X engine has no CPU path (hipGetDeviceCount: %s)", hipGetErrorString(e));
+}
+
+struct FieldsBase {
+  virtual ~FieldsBase() {}
+  HostStructure *s = nullptr;
+  virtual void use_real_fields() = 0;
+  virtual void use_bloch(int d, cdouble k) = 0;
+  virtual void set_boundary(int side, int d, int cond) = 0;
+  virtual void add_volume_source(int c, const SrcTime &st, V3 wmin, V3 wmax, cdouble amp, mp_complex_func A, void *Au) = 0;
+  virtual void step_n(long n, bool check_each) = 0;
+  virtual long t_now() const = 0;
+  virtual double dt_now() const = 0;
+  virtual double last_source_time() const = 0;
+  virtual cdouble get_field(int c, const V3 &p) = 0;
+  virtual int get_array(int c, int cmp, int kind, int ipol, double *out) = 0;
+  virtual void set_array(int c, int cmp, int kind, int ipol, const double *in) = 0;
+  virtual bool have_component(int c) const = 0;
+  virtual void times(double *t7) const = 0;
+  virtual void set_dft_acc(int kind) = 0;
+  virtual void *add_dft_flux(int d, V3 mn, V3 mx, const std::vector<double> &freq, int dec, int *dec_out) = 0;
+  virtual void flux(void *fl, double *out) = 0;
+  virtual int flux_chunks(void *fl, int which) = 0;
+  virtual long flux_points(void *fl, int which, int i) = 0;
+  virtual void flux_dft(void *fl, int which, int i, double *out) = 0;
+  virtual void flux_delete(void *fl) = 0;
+};
+
+template <class R> struct DftChunk {
+  int c;
+  int npts = 0, nfreq = 0;
+  int decimation = 1;
+  cdouble scale;
+  int64_t avg1 = 0, avg2 = 0;
+  int acc_kind = MP_DFT_REALNUM;
+  DevBuf idx, w, omega, acc, comp;
+};
+template <class R> struct Flux {
+  std::vector<DftChunk<R> *> E, H;
+  int nfreq;
+};
+
+template <class R> struct Fields : FieldsBase {
+  Geom gv;
+  double a, dt, courant;
+  long t = 0;
+  bool is_real = false;
+  bool components_allocated = false;
+  hipStream_t stream = nullptr;
+
+  R *f[NC][2] = {}, *f_u[NC][2] = {}, *f_w[NC][2] = {};
+  std::vector<std::unique_ptr<DevBuf>> bufs;
+  // materials and uPML on the device
+  R *d_chi1inv[NC] = {};
+  R *d_sig[3] = {}, *d_kap[3] = {}, *d_siginv[3] = {};
+  struct Pol {
+    int isus;
+    bool allocated = false;
+    R *P[NC][2] = {}, *Pp[NC][2] = {};
+    R *sigma[NC] = {};
+    PoleCoef<R> coef;
+  };
+  std::vector<Pol> pols; // the reference's list order: newest susceptibility first
+
+  int bc[2][3];
+  cdouble k[3], eikna[3];
+  std::vector<SrcTime> srcs;
+  std::vector<SrcVol> sources[4];
+  // curl plan (figure_out_step_plan, src/fields.cpp:428-456)
+  int plus_c[NC], minus_c[NC], plus_d[NC], minus_d[NC];
+  bool have_p[NC], have_m[NC];
+
+  // halo tables, rebuilt when allocations or boundary conditions change
+  bool plan_valid = false;
+  DevBuf d_conn[8];
+  int n_conn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  DevBuf d_slots;
+  std::vector<R *> h_slots;
+  // finiteness probe at the cell centre (src/step.cpp:137-138)
+  DevBuf d_chk_ptrs, d_chk_idx, d_chk_w, d_flag;
+  int n_chk = 0;
+  DevBuf d_scratch; // saved f_minus_p points for long integrated-source lists
+
+  std::vector<std::unique_ptr<DftChunk<R>>> dft_chunks;
+  std::vector<std::unique_ptr<Flux<R>>> fluxes;
+  int dft_acc = MP_DFT_REALNUM;
+  double timers[7] = {0, 0, 0, 0, 0, 0, 0};
+
+  R *dalloc(const R *copy_from) {
+    bufs.emplace_back(new DevBuf);
+    bufs.back()->alloc(sizeof(R) * gv.ntot);
+    R *p = static_cast<R *>(bufs.back()->p);
+    if (copy_from)
+      HIPCHECK(hipMemcpyAsync(p, copy_from, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+    else
+      HIPCHECK(hipMemsetAsync(p, 0, sizeof(R) * gv.ntot, stream));
+    return p;
+  }
+  template <class T> T *upload(const std::vector<T> &v) {
+    bufs.emplace_back(new DevBuf);
+    bufs.back()->alloc(sizeof(T) * v.size());
+    HIPCHECK(hipMemcpy(bufs.back()->p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return static_cast<T *>(bufs.back()->p);
+  }
+  int ncmp() const { return is_real ? 1 : 2; }
+
+  explicit Fields(HostStructure *st) {
+    ensure_gpu();
+    s = st;
+    gv = st->gv;
+    a = gv.a;
+    dt = st->dt;
+    courant = st->courant;
+    if (gv.ntot >= (size_t(1) << 32)) fail("chunk too large for 32-bit cell indexing (%zu points)", gv.ntot);
+    HIPCHECK(hipStreamCreate(&stream));
+    for (int d = 0; d < 3; ++d) {
+      k[d] = 0.0;
+      eikna[d] = 1.0;
+      for (int b = 0; b < 2; ++b) bc[b][d] = gv.has(d) ? Metallic : None;
+    }
+    for (int c = 0; c < NC; ++c) have_p[c] = have_m[c] = false;
+    // materials
+    for (int c = 0; c < 5; ++c)
+      if (!s->chi1inv[c].empty()) {
+        std::vector<R> v(gv.ntot);
+        for (size_t i = 0; i < gv.ntot; ++i) v[i] = R(s->chi1inv[c][i]);
+        d_chi1inv[c] = upload(v);
+      }
+    for (int d = 0; d < 3; ++d)
+      if (s->pml_dir[d]) {
+        size_t n = s->sig[d].size();
+        std::vector<R> sg(n), kp(n), si(n);
+        for (size_t i = 0; i < n; ++i) {
+          sg[i] = R(s->sig[d][i]);
+          kp[i] = R(s->kap[d][i]);
+          si[i] = R(1) / (kp[i] + sg[i]);
+        }
+        d_sig[d] = upload(sg);
+        d_kap[d] = upload(kp);
+        d_siginv[d] = upload(si);
+      }
+    if (int(s->sus.size()) > MAXPOLE) fail("at most %d susceptibilities are supported", MAXPOLE);
+    for (int i = int(s->sus.size()) - 1; i >= 0; --i) {
+      Pol p;
+      p.isus = i;
+      const HostSus &su = s->sus[i];
+      for (int c = 0; c < 5; ++c)
+        if (!su.sigma[c].empty()) {
+          std::vector<R> v(gv.ntot);
+          for (size_t q = 0; q < gv.ntot; ++q) v[q] = R(su.sigma[c][q]);
+          p.sigma[c] = upload(v);
+        }
+      // lorentzian_susceptibility::update_P coefficients in realnum (src/susceptibility.cpp:192-196)
+      const R om = R(su.omega_0), ga = R(su.gamma), rdt = R(dt);
+      const R omega2pi = R(2 * pi * om), g2pi = R(ga * 2 * pi);
+      p.coef.omega0dtsqr = omega2pi * omega2pi * rdt * rdt;
+      p.coef.gamma1inv = R(1) / (R(1) + g2pi * rdt / R(2));
+      p.coef.gamma1 = R(1) - g2pi * rdt / R(2);
+      p.coef.omega0dtsqr_denom = su.drude ? R(0) : p.coef.omega0dtsqr;
+      pols.push_back(p);
+    }
+    d_flag.alloc(sizeof(int));
+    HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
+    // unit directions are periodic (src/fields.cpp:80-85)
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d) && gv.num[d] == 1) use_bloch(d, 0.0);
+  }
+  ~Fields() override {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      for (int ft = 0; ft < 4; ++ft)
+        for (auto &sv : sources[ft]) {
+          if (sv.d_idx) (void)hipFree(sv.d_idx);
+          if (sv.d_val) (void)hipFree(sv.d_val);
+        }
+      (void)hipStreamDestroy(stream);
+    }
+  }
+
+  bool has_field(int c) const {
+    if (gv.dim == 1) return c == 0 || c == 6 || c == 10 || c == 16;
+    return comp_dir(c) != NODIR;
+  }
+  static bool is_tm(int c) { return c == 5 || c == 6 || c == 15 || c == 16 || c == 4 || c == 14; }
+
+  void figure_out_step_plan() {
+    auto cross = [](int a, int b) { return (3 + 2 * a - b) % 3; };
+    auto cross_negative = [](int a, int b) { return ((3 + b - a) % 3) == 2; };
+    for (int c = 0; c < NC; ++c) have_p[c] = have_m[c] = false;
+    for (int c1 = 0; c1 < NC; ++c1) {
+      if (!f[c1][0]) continue;
+      int dc1 = comp_dir(c1);
+      for (int c2 = 0; c2 < NC; ++c2) {
+        bool pair = ((is_electric(c1) || is_D(c1)) && is_magnetic(c2)) || ((is_magnetic(c1) || is_B(c1)) && is_electric(c2));
+        if (!pair) continue;
+        int dc2 = comp_dir(c2);
+        if (dc2 == NODIR || dc1 == dc2 || !has_field(c1) || !has_field(c2)) continue;
+        int dd = cross(dc1, dc2);
+        if (!gv.has(dd)) continue;
+        if (cross_negative(dc2, dc1)) {
+          minus_c[c1] = c2;
+          minus_d[c1] = dd;
+          have_m[c1] = true;
+        }
+        else {
+          plus_c[c1] = c2;
+          plus_d[c1] = dd;
+          have_p[c1] = true;
+        }
+      }
+    }
+  }
+  // require_component / alloc_f (src/fields.cpp:480-504,548-574)
+  void require_component(int c) {
+    if (!has_field(c)) fail("cannot require a component absent from this grid");
+    components_allocated = true;
+    bool changed = false;
+    for (int ca = 0; ca < NC; ++ca) {
+      if (!has_field(ca)) continue;
+      if (gv.dim == 2 && (is_tm(c) ^ is_tm(ca))) continue;
+      for (int cmp = 0; cmp < ncmp(); ++cmp)
+        if (!f[ca][cmp]) {
+          changed = true;
+          if (is_magnetic(ca)) {
+            int bcmp = dir_comp(15, comp_dir(ca));
+            if (!f[bcmp][cmp]) f[bcmp][cmp] = dalloc(nullptr);
+            f[ca][cmp] = f[bcmp][cmp];
+          }
+          else
+            f[ca][cmp] = dalloc(nullptr);
+        }
+    }
+    if (changed) {
+      figure_out_step_plan();
+      plan_valid = false;
+    }
+  }
+
+  void use_real_fields() override {
+    for (int d = 0; d < 3; ++d)
+      if (bc[High][d] == Periodic && k[d] != 0.0) fail("Can't use real fields with bloch boundary conditions!");
+    is_real = true;
+    for (int c = 0; c < NC; ++c) f[c][1] = f_u[c][1] = f_w[c][1] = nullptr;
+    plan_valid = false;
+  }
+  void set_boundary(int b, int d, int cond) override {
+    if (bc[b][d] != cond) {
+      bc[b][d] = cond;
+      if (cond == Periodic || bc[1 - b][d] == Periodic) bc[1 - b][d] = cond;
+      plan_valid = false;
+    }
+  }
+  void use_bloch(int d, cdouble kk) override {
+    k[d] = kk;
+    for (int b = 0; b < 2; ++b) set_boundary(b, d, Periodic);
+    if (std::real(kk) * gv.num[d] == 0.5 * a)
+      eikna[d] = -std::exp(-std::imag(kk) * ((2 * pi / a) * gv.num[d]));
+    else
+      eikna[d] = std::exp(cdouble(0, 1) * kk * ((2 * pi / a) * gv.num[d]));
+    if (is_real && kk != 0.0) fail("Can't use real fields with bloch boundary conditions!");
+    plan_valid = false;
+  }
+
+  // ---- halo plan (find_metals + connect_the_chunks for one chunk) ----
+  bool on_metal_boundary(const IV &here) const {
+    for (int d = 0; d < 3; ++d) {
+      if (!gv.has(d)) continue;
+      if (here.t[d] == 2 * gv.num[d] && bc[High][d] == Metallic) return true;
+      if (bc[Low][d] == Magnetic && here.t[d] == 1) return true;
+      if (bc[Low][d] == Metallic && here.t[d] == 0) return true;
+    }
+    return false;
+  }
+  bool locate(IV *there, cdouble *ph) const {
+    if (!gv.owns(*there))
+      for (int d = 0; d < 3; ++d) {
+        if (!gv.has(d) || bc[High][d] != Periodic) continue;
+        const int L = 2 * gv.num[d];
+        if (there->t[d] <= 0)
+          while (there->t[d] <= 0) {
+            there->t[d] += L;
+            *ph *= std::conj(eikna[d]);
+          }
+        else
+          while (there->t[d] - L > 0) {
+            there->t[d] -= L;
+            *ph *= eikna[d];
+          }
+      }
+    return gv.owns(*there);
+  }
+  // slots: 0..19 the field arrays, 20 + 20*p + c the polarisation P of pole p
+  void rebuild_plan() {
+    HIPCHECK(hipStreamSynchronize(stream));
+    const int nslot = NC + NC * int(pols.size());
+    h_slots.assign(2 * nslot, nullptr);
+    for (int c = 0; c < NC; ++c)
+      for (int cmp = 0; cmp < 2; ++cmp) h_slots[2 * c + cmp] = f[c][cmp];
+    for (size_t p = 0; p < pols.size(); ++p)
+      for (int c = 0; c < NC; ++c)
+        for (int cmp = 0; cmp < 2; ++cmp) h_slots[2 * (NC + NC * p + c) + cmp] = pols[p].P[c][cmp];
+    d_slots.alloc(sizeof(R *) * h_slots.size());
+    HIPCHECK(hipMemcpy(d_slots.p, h_slots.data(), sizeof(R *) * h_slots.size(), hipMemcpyHostToDevice));
+
+    std::vector<Conn> lists[8];
+    bool B_redundant[5];
+    for (int q = 0; q < 5; ++q) B_redundant[q] = f[5 + q][0] == f[15 + q][0];
+    auto phase_class = [](cdouble ph) {
+      auto close = [](cdouble p, double r) { return std::fabs(p.imag()) < 1e-13 && std::fabs(p.real() - r) < 1e-13; };
+      return close(ph, 1.0) ? 2 : close(ph, -1.0) ? 1 : 0;
+    };
+    for (int corig = 0; corig < NC; ++corig) {
+      if (!f[corig][0]) continue;
+      if (is_B(corig) && B_redundant[corig - 15]) continue;
+      // the non-owned planes: index 0 (no Yee shift) or n (Yee shift) along each direction
+      // (get_boundary_icorners, src/vec.cpp:395-430), as disjoint boxes
+      int lo[3], hi[3];
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = 0;
+        hi[d] = gv.has(d) ? gv.num[d] : 0;
+      }
+      for (int d = 0; d < 3; ++d) {
+        if (!gv.has(d)) continue;
+        const int plane = gv.ysh(corig, d) ? gv.num[d] : 0;
+        int blo[3], bhi[3];
+        for (int q = 0; q < 3; ++q) {
+          blo[q] = lo[q];
+          bhi[q] = hi[q];
+        }
+        blo[d] = bhi[d] = plane;
+        for (int j0 = blo[0]; j0 <= bhi[0]; ++j0)
+          for (int j1 = blo[1]; j1 <= bhi[1]; ++j1)
+            for (int j2 = blo[2]; j2 <= bhi[2]; ++j2) {
+              int j[3] = {j0, j1, j2};
+              IV here = gv.iloc(corig, j);
+              const int64_t n = gv.index(corig, here);
+              cdouble ph = 1.0;
+              if (!locate(&here, &ph) || on_metal_boundary(here)) continue;
+              Conn cn;
+              cn.dst = n;
+              cn.src = gv.index(corig, here);
+              cn.slot = corig;
+              cn.cls = phase_class(ph);
+              cn.ph_re = ph.real();
+              cn.ph_im = ph.imag();
+              lists[field_type(corig)].push_back(cn);
+              if (is_electric(corig) || is_magnetic(corig))
+                for (size_t p = 0; p < pols.size(); ++p)
+                  if (pols[p].allocated && pols[p].P[corig][0]) {
+                    Conn cp = cn;
+                    cp.slot = NC + NC * int(p) + corig;
+                    lists[is_electric(corig) ? 4 : 5].push_back(cp);
+                  }
+            }
+        // later boxes exclude this plane (the boxes are disjoint)
+        if (plane == 0)
+          lo[d] = 1;
+        else
+          hi[d] = plane - 1;
+      }
+    }
+    for (int ft = 0; ft < 8; ++ft)
+      for (const Conn &cn : lists[ft])
+        if (cn.dst < 0 || cn.src < 0 || size_t(cn.dst) >= gv.ntot || size_t(cn.src) >= gv.ntot || cn.slot < 0 || cn.slot >= nslot)
+          fail("internal: halo connection outside the array");
+    for (int ft = 0; ft < 8; ++ft) {
+      n_conn[ft] = int(lists[ft].size());
+      d_conn[ft].alloc(sizeof(Conn) * lists[ft].size());
+      if (n_conn[ft])
+        HIPCHECK(hipMemcpy(d_conn[ft].p, lists[ft].data(), sizeof(Conn) * lists[ft].size(), hipMemcpyHostToDevice));
+    }
+    // centre probe: E and D components at the interpolation points of the centre
+    std::vector<const R *> cp;
+    std::vector<int64_t> ci;
+    std::vector<double> cw;
+    V3 cen = gv.center();
+    for (int c = 0; c < 5; ++c) {
+      if (!has_field(c) || !f[c][0]) continue;
+      for (int which = 0; which < 2; ++which) {
+        int cc = which ? c + 10 : c;
+        IV locs[8];
+        double w[8];
+        gv.interpolate(cc, cen, locs, w);
+        for (int q = 0; q < 8 && w[q]; ++q) {
+          IV il = locs[q];
+          cdouble ph = 1.0;
+          locate(&il, &ph);
+          if (!gv.owns(il)) continue;
+          for (int cmp = 0; cmp < ncmp(); ++cmp)
+            if (f[cc][cmp]) {
+              cp.push_back(f[cc][cmp]);
+              ci.push_back(gv.index(cc, il));
+              cw.push_back(w[q]);
+            }
+        }
+      }
+    }
+    n_chk = int(cp.size());
+    d_chk_ptrs.alloc(sizeof(R *) * cp.size());
+    d_chk_idx.alloc(sizeof(int64_t) * ci.size());
+    d_chk_w.alloc(sizeof(double) * cw.size());
+    if (n_chk) {
+      HIPCHECK(hipMemcpy(d_chk_ptrs.p, cp.data(), sizeof(R *) * cp.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(d_chk_idx.p, ci.data(), sizeof(int64_t) * ci.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(d_chk_w.p, cw.data(), sizeof(double) * cw.size(), hipMemcpyHostToDevice));
+    }
+    plan_valid = true;
+  }
+  void boundaries(int ft) {
+    if (n_conn[ft]) launch_conn<R>(static_cast<const Conn *>(d_conn[ft].p), n_conn[ft], static_cast<R *const *>(d_slots.p), ncmp(), stream);
+  }
+
+  // ---- per-step argument blocks ----
+  Grid grid() const {
+    Grid g;
+    g.ntot = uint32_t(gv.ntot);
+    for (int kx = 0; kx < 3; ++kx) {
+      int d = gv.ydir(kx);
+      g.axdir[kx] = gv.has(d) ? d : -1;
+      g.ext[kx] = gv.has(d) ? gv.num[d] + 1 : 1;
+    }
+    for (int d = 0; d < 3; ++d) g.num[d] = gv.num[d];
+    g.hi_metal = g.lo_magnetic = 0;
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d)) {
+        if (bc[High][d] == Metallic) g.hi_metal |= 1 << d;
+        if (bc[Low][d] == Magnetic) g.lo_magnetic |= 1 << d;
+      }
+    return g;
+  }
+  bool pml_in(int d) const { return d >= 0 && d < 3 && s->pml_dir[d]; }
+
+  // step_db's per-component setup (src/step_db.cpp:47-84); allocates f_u lazily as a copy of f
+  int curl_comps(int ft, int cmp, CurlComp<R> *cc, int *which) {
+    int n = 0;
+    const int c0 = ft == B_ft ? 15 : 10;
+    for (int cc_ = c0; cc_ < c0 + 5; ++cc_) {
+      if (!f[cc_][cmp]) continue;
+      CurlComp<R> &k = cc[n];
+      const int dc = comp_dir(cc_);
+      const int dsig0 = cycle_dir(dc, 1), dsigu0 = cycle_dir(dc, 2);
+      k.dsig = pml_in(dsig0) ? dsig0 : -1;
+      k.dsigu = pml_in(dsigu0) ? dsigu0 : -1;
+      if (k.dsigu >= 0 && !f_u[cc_][cmp]) f_u[cc_][cmp] = dalloc(f[cc_][cmp]);
+      k.f = f[cc_][cmp];
+      k.fu = f_u[cc_][cmp];
+      const R *g1 = have_p[cc_] ? f[plus_c[cc_]][cmp] : nullptr;
+      const R *g2 = have_m[cc_] ? f[minus_c[cc_]][cmp] : nullptr;
+      int64_t s1 = have_p[cc_] ? gv.st[plus_d[cc_]] : 0, s2 = have_m[cc_] ? gv.st[minus_d[cc_]] : 0;
+      if (ft == D_ft) {
+        s1 = -s1;
+        s2 = -s2;
+      }
+      R dtdx = R(courant);
+      if (!g1) {
+        std::swap(g1, g2);
+        std::swap(s1, s2);
+        dtdx = -dtdx;
+      }
+      k.g1 = g1;
+      k.g2 = g2;
+      k.s1 = s1;
+      k.s2 = s2;
+      k.dtdx = dtdx;
+      for (int d = 0; d < 3; ++d) {
+        k.lo[d] = gv.own_lo(cc_, d);
+        k.hi[d] = gv.own_hi(cc_, d);
+        k.ysh[d] = gv.ysh(cc_, d);
+      }
+      // the owned range keeps i + s1 and i + s2 inside the array (checked, not assumed)
+      for (int which_s = 0; which_s < 2; ++which_s) {
+        const bool used = which_s == 0 ? (g1 != nullptr) : (g2 != nullptr);
+        if (!used) continue;
+        const int64_t sv = which_s == 0 ? s1 : s2;
+        int dd = -1;
+        for (int d = 0; d < 3; ++d)
+          if (gv.has(d) && (gv.st[d] == sv || gv.st[d] == -sv)) dd = d;
+        if (dd < 0) fail("internal: curl stride matches no direction");
+        if ((sv > 0 && k.hi[dd] + 1 > gv.num[dd]) || (sv < 0 && k.lo[dd] - 1 < 0))
+          fail("internal: curl neighbour outside the array");
+      }
+      which[n] = cc_;
+      ++n;
+    }
+    return n;
+  }
+  // update_eh's per-component setup (src/update_eh.cpp:148-195), aligned with the curl
+  // components in src (entry q updates the E/H partner of src[q]).  Allocates H (a copy of B)
+  // and W (a copy of E/H) lazily, as the reference does on its first call; an entry whose
+  // E/H still aliases D/B has f == nullptr and is skipped.  Returns how many entries update.
+  int upd_comps(int ft, int cmp, const int *src, int n, UpdComp<R> *uc, bool have_fmp) {
+    int active = 0;
+    for (int q = 0; q < n; ++q) {
+      const int dc = src[q], ec = dc - 10;
+      UpdComp<R> &e = uc[q];
+      memset(&e, 0, sizeof e);
+      for (int d = 0; d < 3; ++d) {
+        e.lo[d] = gv.own_lo(ec, d);
+        e.hi[d] = gv.own_hi(ec, d);
+        e.ysh[d] = gv.ysh(ec, d);
+      }
+      e.dsigw = -1;
+      if (!f[ec][cmp]) continue;
+      const int dsigw = pml_in(comp_dir(ec)) ? comp_dir(ec) : -1;
+      const R *u = d_chi1inv[ec];
+      if (f[ec][cmp] == f[dc][cmp] && (u || have_fmp || dsigw >= 0)) {
+        f[ec][cmp] = dalloc(f[dc][cmp]);
+        plan_valid = false;
+      }
+      if (dsigw >= 0 && !f_w[ec][cmp]) f_w[ec][cmp] = dalloc(f[ec][cmp]);
+      if (f[ec][cmp] == f[dc][cmp]) continue;
+      e.f = f[ec][cmp];
+      e.g = f[dc][cmp];
+      e.u = u;
+      e.fw = f_w[ec][cmp];
+      e.dsigw = dsigw;
+      for (int p = 0; p < MAXPOLE; ++p)
+        if (ft == E_ft && p < int(pols.size()) && pols[p].allocated) {
+          e.P[p] = pols[p].P[ec][cmp];
+          e.Pp[p] = pols[p].Pp[ec][cmp];
+          e.sigma[p] = pols[p].sigma[ec];
+        }
+      ++active;
+    }
+    return active;
+  }
+  // whether update_eh(ft) would allocate something now (then the curl runs on its own first)
+  bool upd_pending(int cmp, const int *src, int n, bool have_fmp) const {
+    for (int q = 0; q < n; ++q) {
+      const int dc = src[q], ec = dc - 10;
+      if (!f[ec][cmp]) continue;
+      const int dsigw = pml_in(comp_dir(ec)) ? comp_dir(ec) : -1;
+      if (f[ec][cmp] == f[dc][cmp] && (d_chi1inv[ec] || have_fmp || dsigw >= 0)) return true;
+      if (dsigw >= 0 && !f_w[ec][cmp]) return true;
+    }
+    return false;
+  }
+
+  bool needs_P(const Pol &p, int c, int cmp) const {
+    return is_electric(c) && p.sigma[c] && f[c][cmp];
+  }
+  void alloc_pols() {
+    for (auto &p : pols)
+      if (!p.allocated) {
+        for (int c = 0; c < 5; ++c)
+          for (int cmp = 0; cmp < ncmp(); ++cmp)
+            if (needs_P(p, c, cmp)) {
+              p.P[c][cmp] = dalloc(nullptr);
+              p.Pp[c][cmp] = dalloc(nullptr);
+            }
+        p.allocated = true;
+        plan_valid = false;
+      }
+  }
+
+  // sparse source values of field type ft2 (D or B) at this step
+  void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, double time, SrcTab &tab, std::vector<const SrcVol *> &big) {
+    tab.n = 0;
+    for (const SrcVol &sv : sources[ft2]) {
+      const SrcTime &st = srcs[sv.src];
+      if (st.is_integrated != integrated) continue;
+      const cdouble scal = integrated ? st.dipole(time) : st.current(time, dt);
+      const double mult = integrated ? 1.0 : dt;
+      const int c2 = dir_comp(ft2 == D_ft ? 10 : 15, comp_dir(sv.c));
+      int slot = -1;
+      for (int q = 0; q < nc; ++q)
+        if (which[q] == (integrated ? sv.c : c2)) slot = q;
+      if (slot < 0) continue;
+      if (scal == 0.0) continue; // exact zero contributes nothing (the reference subtracts 0)
+      if (tab.n + int(sv.index.size()) > SRC_TAB) {
+        big.push_back(&sv);
+        continue;
+      }
+      for (size_t j = 0; j < sv.index.size(); ++j) {
+        const cdouble A = (sv.amp[j] * scal) * mult;
+        tab.comp[tab.n] = slot;
+        tab.idx[tab.n] = sv.index[j];
+        tab.val[tab.n] = cmp ? A.imag() : A.real();
+        ++tab.n;
+      }
+    }
+  }
+  void apply_big(const SrcVol &sv, R *fbase, int cmp, cdouble scal, double mult) {
+    std::vector<double> vals(sv.index.size());
+    for (size_t j = 0; j < sv.index.size(); ++j) {
+      const cdouble A = (sv.amp[j] * scal) * mult;
+      vals[j] = cmp ? A.imag() : A.real();
+    }
+    // values staged through a per-source device buffer; the copy is ordered on the stream
+    SrcVol &m = const_cast<SrcVol &>(sv);
+    if (!m.d_val) {
+      HIPCHECK(hipMalloc(&m.d_val, sizeof(double) * vals.size() * 2));
+      HIPCHECK(hipMalloc(&m.d_idx, sizeof(int64_t) * vals.size()));
+      HIPCHECK(hipMemcpy(m.d_idx, sv.index.data(), sizeof(int64_t) * vals.size(), hipMemcpyHostToDevice));
+    }
+    double *dv = m.d_val + cmp * vals.size();
+    HIPCHECK(hipMemcpyAsync(dv, vals.data(), sizeof(double) * vals.size(), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream)); // vals lives on the host stack
+    launch_src<R>(fbase, m.d_idx, dv, int(vals.size()), stream);
+  }
+
+  // One fields::step() (src/step.cpp:35-139).
+  void one_step() {
+    const double tB = t * dt, tH = t * dt + 0.5 * dt, tE = t * dt + dt;
+    bool int_B = false, int_D = false;
+    for (auto &sv : sources[B_ft])
+      if (srcs[sv.src].is_integrated) int_B = true;
+    for (auto &sv : sources[D_ft])
+      if (srcs[sv.src].is_integrated) int_D = true;
+    if (int_B) fail("integrated magnetic sources are not supported by the GPU engine");
+    bool have_fmp_E = int_D;
+    for (auto &p : pols)
+      for (int c = 0; c < 5; ++c)
+        for (int cmp = 0; cmp < ncmp(); ++cmp)
+          if (needs_P(p, c, cmp)) have_fmp_E = true;
+
+    StepArgs<R> A;
+    memset(&A, 0, sizeof A);
+    A.g = grid();
+    for (int d = 0; d < 3; ++d) {
+      A.pml.sig[d] = d_sig[d];
+      A.pml.kap[d] = d_kap[d];
+      A.pml.siginv[d] = d_siginv[d];
+    }
+    // --- B, then H = mu^-1 B where H is its own array ---
+    for (int cmp = 0; cmp < ncmp(); ++cmp) {
+      int which[3];
+      A.nc = curl_comps(B_ft, cmp, A.cc, which);
+      std::vector<const SrcVol *> big;
+      fill_tab(B_ft, cmp, which, A.nc, false, tB, A.cur, big);
+      A.dip.n = 0;
+      A.npole = 0;
+      if (big.empty() && !upd_pending(cmp, which, A.nc, false)) {
+        const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
+        launch_step<R>(A, true, nu > 0, stream);
+        continue;
+      }
+      launch_step<R>(A, true, false, stream);
+      for (const SrcVol *sv : big) apply_big(*sv, f[dir_comp(15, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].current(tB, dt), dt);
+      StepArgs<R> U = A;
+      U.cur.n = 0;
+      if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) launch_step<R>(U, false, true, stream);
+    }
+    if (!plan_valid) rebuild_plan();
+    boundaries(B_ft);
+    boundaries(H_ft);
+    // --- D, then E = chi1inv (D - P - dipoles) and the polarisations ---
+    alloc_pols();
+    if (!plan_valid) rebuild_plan();
+    for (int cmp = 0; cmp < ncmp(); ++cmp) {
+      int which[3], whiche[3];
+      A.nc = curl_comps(D_ft, cmp, A.cc, which);
+      std::vector<const SrcVol *> bigc, bigd;
+      fill_tab(D_ft, cmp, which, A.nc, false, tH, A.cur, bigc);
+      upd_comps(E_ft, cmp, which, A.nc, A.uc, have_fmp_E);
+      for (int q = 0; q < A.nc; ++q) whiche[q] = which[q] - 10;
+      fill_tab(D_ft, cmp, whiche, A.nc, true, tE, A.dip, bigd);
+      A.npole = int(pols.size());
+      for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
+      if (bigc.empty() && bigd.empty()) {
+        launch_step<R>(A, true, true, stream);
+        continue;
+      }
+      launch_step<R>(A, true, false, stream);
+      for (const SrcVol *sv : bigc) apply_big(*sv, f[dir_comp(10, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].current(tH, dt), dt);
+      // long integrated lists: f_minus_p formed in place in D around the update, then D restored
+      std::vector<std::pair<R *, int64_t>> saved;
+      for (const SrcVol *sv : bigd) {
+        R *D = f[dir_comp(10, comp_dir(sv->c))][cmp];
+        for (int64_t ix : sv->index) saved.push_back({D, ix});
+      }
+      if (!saved.empty()) {
+        d_scratch.alloc(sizeof(R) * saved.size());
+        for (size_t q = 0; q < saved.size(); ++q)
+          HIPCHECK(hipMemcpyAsync(static_cast<R *>(d_scratch.p) + q, saved[q].first + saved[q].second, sizeof(R), hipMemcpyDeviceToDevice, stream));
+        for (const SrcVol *sv : bigd) apply_big(*sv, f[dir_comp(10, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].dipole(tE), 1.0);
+      }
+      StepArgs<R> U = A;
+      U.cur.n = 0;
+      launch_step<R>(U, false, true, stream);
+      for (size_t q = 0; q < saved.size(); ++q)
+        HIPCHECK(hipMemcpyAsync(saved[q].first + saved[q].second, static_cast<R *>(d_scratch.p) + q, sizeof(R), hipMemcpyDeviceToDevice, stream));
+    }
+    if (!plan_valid) rebuild_plan();
+    boundaries(D_ft);
+    boundaries(4); // PE
+    boundaries(E_ft);
+    t += 1;
+    update_dfts();
+    if (n_chk)
+      launch_check<R>(static_cast<const R *const *>(d_chk_ptrs.p), static_cast<const int64_t *>(d_chk_idx.p),
+                      static_cast<const double *>(d_chk_w.p), n_chk, static_cast<int *>(d_flag.p), stream);
+  }
+
+  void step_n(long n, bool) override {
+    if (!plan_valid) rebuild_plan();
+    auto t0 = std::chrono::steady_clock::now();
+    for (long q = 0; q < n; ++q) one_step();
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+    timers[6] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int flag = 0;
+    HIPCHECK(hipMemcpy(&flag, d_flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) fail("simulation fields are NaN or Inf");
+  }
+  long t_now() const override { return t; }
+  double dt_now() const override { return dt; }
+  double last_source_time() const override {
+    double after = 0;
+    for (auto &st : srcs) after = std::max(after, st.last_time());
+    return after;
+  }
+
+  // ---- DFT ----
+  void update_dfts() {
+    for (auto &cp : dft_chunks) {
+      DftChunk<R> &ch = *cp;
+      if (t % ch.decimation) continue;
+      if (!f[ch.c][0]) continue;
+      DftArgs<R> da;
+      da.f0 = f[ch.c][0];
+      da.f1 = f[ch.c][1];
+      da.idx = static_cast<const int64_t *>(ch.idx.p);
+      da.w = static_cast<const double *>(ch.w.p);
+      da.omega = static_cast<const double *>(ch.omega.p);
+      da.time = (is_magnetic(ch.c) || is_B(ch.c)) ? t * dt - 0.5 * dt : t * dt;
+      da.scale_re = ch.scale.real();
+      da.scale_im = ch.scale.imag();
+      da.avg1 = ch.avg1;
+      da.avg2 = ch.avg2;
+      da.npts = ch.npts;
+      da.nfreq = ch.nfreq;
+      da.acc = ch.acc.p;
+      da.comp = ch.comp.p;
+      da.acc_kind = ch.acc_kind;
+      launch_dft<R>(da, stream);
+    }
+  }
+
+  // loop_in_chunks for one chunk and the identity symmetry (src/loop_in_chunks.cpp:232-540)
+  template <class CB> void loop_in_chunks(const V3 &wmin, const V3 &wmax, int cgrid, CB chunkloop) {
+    IV iyee_c;
+    for (int d = 0; d < 3; ++d) iyee_c.t[d] = gv.ysh(20, d) - gv.ysh(cgrid, d);
+    V3 yc = gv.loc(iyee_c);
+    IV is, ie;
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d)) {
+        is.t[d] = 1 + 2 * int(std::floor((wmin.t[d] + yc.t[d]) * a - .5)) - iyee_c.t[d];
+        ie.t[d] = 1 + 2 * int(std::ceil((wmax.t[d] + yc.t[d]) * a - .5)) - iyee_c.t[d];
+      }
+    V3 s0, s1, e0, e1;
+    for (int d = 0; d < 3; ++d) {
+      if (!gv.has(d)) continue;
+      double w0 = 1. - wmin.t[d] * a + 0.5 * is.t[d];
+      double w1 = 1. + wmax.t[d] * a - 0.5 * ie.t[d];
+      if (ie.t[d] >= is.t[d] + 3 * 2) {
+        s0.t[d] = w0 * w0 / 2;
+        s1.t[d] = 1 - (1 - w0) * (1 - w0) / 2;
+        e0.t[d] = w1 * w1 / 2;
+        e1.t[d] = 1 - (1 - w1) * (1 - w1) / 2;
+      }
+      else if (ie.t[d] == is.t[d] + 2 * 2) {
+        s0.t[d] = w0 * w0 / 2;
+        s1.t[d] = 1 - (1 - w0) * (1 - w0) / 2 - (1 - w1) * (1 - w1) / 2;
+        e0.t[d] = w1 * w1 / 2;
+        e1.t[d] = s1.t[d];
+      }
+      else if (wmin.t[d] == wmax.t[d]) {
+        s0.t[d] = w0;
+        s1.t[d] = w1;
+        e0.t[d] = w1;
+        e1.t[d] = w0;
+      }
+      else if (ie.t[d] == is.t[d] + 1 * 2) {
+        s0.t[d] = w0 * w0 / 2 - (1 - w1) * (1 - w1) / 2;
+        e0.t[d] = w1 * w1 / 2 - (1 - w0) * (1 - w0) / 2;
+        s1.t[d] = e0.t[d];
+        e1.t[d] = s0.t[d];
+      }
+      else
+        fail("bug: impossible looping boundaries");
+    }
+    IV mins, maxs;
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d) && bc[High][d] == Periodic) {
+        const double L = gv.num[d] * gv.inva;
+        mins.t[d] = int(std::floor((wmin.t[d] - gv.num[d] * gv.inva) / L));
+        maxs.t[d] = int(std::ceil((wmax.t[d] - 0.0) / L));
+      }
+    IV sh = mins;
+    IV isco, ieco;
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d)) {
+        isco.t[d] = 2 - gv.ysh(cgrid, d);
+        ieco.t[d] = 2 * gv.num[d] - gv.ysh(cgrid, d);
+      }
+    while (true) {
+      cdouble ph = 1.0;
+      IV shifti;
+      for (int d = 0; d < 3; ++d)
+        if (gv.has(d)) {
+          shifti.t[d] = 2 * gv.num[d] * sh.t[d];
+          ph *= std::pow(eikna[d], sh.t[d]);
+        }
+      IV iscS, iecS;
+      bool nonempty = true;
+      for (int d = 0; d < 3; ++d)
+        if (gv.has(d)) {
+          iscS.t[d] = std::max(is.t[d] - shifti.t[d], isco.t[d]);
+          iecS.t[d] = std::min(ie.t[d] - shifti.t[d], ieco.t[d]);
+          if (iscS.t[d] > iecS.t[d]) nonempty = false;
+        }
+      if (nonempty) {
+        V3 s0c, s1c, e0c, e1c;
+        for (int d = 0; d < 3; ++d) s0c.t[d] = s1c.t[d] = e0c.t[d] = e1c.t[d] = 1.0;
+        for (int d = 0; d < 3; ++d) {
+          if (!gv.has(d)) continue;
+          const int a0 = iscS.t[d] + shifti.t[d], b0 = iecS.t[d] + shifti.t[d];
+          if (a0 == is.t[d]) {
+            s0c.t[d] = s0.t[d];
+            s1c.t[d] = s1.t[d];
+          }
+          else if (a0 == is.t[d] + 2)
+            s0c.t[d] = s1.t[d];
+          if (b0 == ie.t[d]) {
+            e0c.t[d] = e0.t[d];
+            e1c.t[d] = e1.t[d];
+          }
+          else if (b0 == ie.t[d] - 2)
+            e0c.t[d] = e1.t[d];
+          if (b0 == a0) {
+            double w = std::min(s0c.t[d], e0c.t[d]);
+            s0c.t[d] = e0c.t[d] = s1c.t[d] = e1c.t[d] = w;
+          }
+          else if (b0 == a0 + 2) {
+            double w = std::min(s0c.t[d], e1c.t[d]);
+            s0c.t[d] = e1c.t[d] = w;
+            w = std::min(s1c.t[d], e0c.t[d]);
+            s1c.t[d] = e0c.t[d] = w;
+          }
+          else if (b0 == a0 + 4) {
+            double w = std::min(s1c.t[d], e1c.t[d]);
+            s1c.t[d] = e1c.t[d] = w;
+          }
+        }
+        double dV0 = 1.0;
+        for (int d = 0; d < 3; ++d)
+          if (gv.has(d) && wmax.t[d] - wmin.t[d] > 0.0) dV0 *= gv.inva;
+        chunkloop(iscS, iecS, s0c, s1c, e0c, e1c, dV0, shifti, ph);
+      }
+      bool wrapped = true;
+      for (int d = 0; d < 3; ++d) {
+        if (!gv.has(d)) continue;
+        if (sh.t[d] + 1 <= maxs.t[d]) {
+          sh.t[d] += 1;
+          wrapped = false;
+          break;
+        }
+        sh.t[d] = mins.t[d];
+      }
+      if (wrapped) break;
+    }
+  }
+  // Visit the points of box [is, ie] (half-pixel coordinates, step 2) in loop order, giving
+  // the array index, the counter and the integration weight (IVEC_LOOP_WEIGHT,
+  // src/meep/vec.hpp:338-350).
+  template <class CB>
+  void visit_box(const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV, CB cb) const {
+    int n[3], d_[3];
+    for (int kx = 0; kx < 3; ++kx) {
+      int d = gv.ydir(kx);
+      d_[kx] = gv.has(d) ? d : -1;
+      n[kx] = gv.has(d) ? (ie.t[d] - is.t[d]) / 2 + 1 : 1;
+    }
+    auto w1 = [&](int i, int nn, int d) {
+      if (d < 0) return 1.0;
+      return (i > 1 && i < nn - 2) ? 1.0 : (i == 0 ? s0.t[d] : (i == 1 ? s1.t[d] : i == nn - 1 ? e0.t[d] : (i == nn - 2 ? e1.t[d] : 1.0)));
+    };
+    int64_t counter = 0;
+    for (int i1 = 0; i1 < n[0]; ++i1)
+      for (int i2 = 0; i2 < n[1]; ++i2)
+        for (int i3 = 0; i3 < n[2]; ++i3, ++counter) {
+          IV p;
+          int ii[3] = {i1, i2, i3};
+          for (int kx = 0; kx < 3; ++kx)
+            if (d_[kx] >= 0) p.t[d_[kx]] = is.t[d_[kx]] + 2 * ii[kx];
+          int64_t idx = 0;
+          for (int d = 0; d < 3; ++d)
+            if (gv.has(d)) idx += p.t[d] / 2 * gv.st[d];
+          double w = w1(i3, n[2], d_[2]) * (w1(i2, n[1], d_[1]) * (dV * w1(i1, n[0], d_[0])));
+          cb(idx, counter, p, w, ii);
+        }
+  }
+
+  int add_src_time(const SrcTime &st) {
+    for (size_t q = 0; q < srcs.size(); ++q)
+      if (srcs[q].equal(st)) return int(q);
+    srcs.push_back(st);
+    return int(srcs.size()) - 1;
+  }
+  // fields::add_volume_source + src_vol_chunkloop (src/sources.cpp:243-312,462-490)
+  void add_volume_source(int c, const SrcTime &st, V3 wmin, V3 wmax, cdouble amp, mp_complex_func A, void *Au) override {
+    if (c < 0 || c >= NC || comp_dir(c) == NODIR || !has_field(c)) fail("invalid source component %d", c);
+    for (int d = 0; d < 3; ++d) {
+      if (!gv.has(d)) continue;
+      const double w = gv.num[d] * gv.inva, wd = wmax.t[d] - wmin.t[d];
+      if (wd > w + gv.inva)
+        fail("Source width > cell width");
+      else if (wd > w) {
+        wmin.t[d] -= (wd - w) * 0.5;
+        wmax.t[d] = wmin.t[d] + w;
+      }
+    }
+    cdouble damp = amp;
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d) && wmax.t[d] - wmin.t[d] == 0.0) damp *= a;
+    const int isrc = add_src_time(st);
+    V3 cen;
+    for (int d = 0; d < 3; ++d) cen.t[d] = (wmin.t[d] + wmax.t[d]) * 0.5;
+    loop_in_chunks(wmin, wmax, c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double, const IV &shift, cdouble sph) {
+      std::vector<int64_t> index;
+      std::vector<cdouble> amps;
+      const cdouble am = damp * std::conj(sph);
+      visit_box(is, ie, s0, s1, e0, e1, 1.0, [&](int64_t idx, int64_t, const IV &p, double w, const int *) {
+        if (!gv.owns(p)) return;
+        double rel[3];
+        for (int d = 0; d < 3; ++d) rel[d] = gv.has(d) ? (p.t[d] * 0.5 * gv.inva + shift.t[d] * (0.5 * gv.inva)) - cen.t[d] : -cen.t[d];
+        cdouble Av = 1.0;
+        if (A) {
+          double re = 1, im = 0;
+          A(rel, Au, &re, &im);
+          Av = cdouble(re, im);
+        }
+        amps.push_back(w * am * Av);
+        index.push_back(idx);
+      });
+      for (int64_t i : index)
+        if (i < 0 || size_t(i) >= gv.ntot) fail("internal: source point outside the array");
+      const int ft = (is_magnetic(c) || is_B(c)) ? B_ft : D_ft;
+      const int cc = (is_D(c) || is_B(c)) ? c - 10 : c;
+      for (SrcVol &o : sources[ft])
+        if (o.c == cc && o.src == isrc && o.index == index) {
+          for (size_t q = 0; q < o.amp.size(); ++q) o.amp[q] += amps[q];
+          return;
+        }
+      SrcVol sv;
+      sv.c = cc;
+      sv.src = isrc;
+      sv.index = std::move(index);
+      sv.amp = std::move(amps);
+      sources[ft].push_back(std::move(sv));
+    });
+    require_component(c);
+  }
+
+  int auto_decimation(const std::vector<double> &freq) const {
+    double src_freq_max = 0;
+    for (const SrcTime &st : srcs)
+      if (st.get_fwidth() != 0) src_freq_max = std::max(src_freq_max, std::abs(st.freq) + 0.5 * st.get_fwidth());
+    double freq_max = 0;
+    for (double fr : freq) freq_max = std::max(freq_max, std::abs(fr));
+    if (freq_max > 0 && src_freq_max > 0) return std::max(1, int(std::floor(1 / (dt * (freq_max + src_freq_max)))));
+    return 1;
+  }
+  // fields::add_dft for one component (src/dft.cpp:175-235)
+  void add_dft(std::vector<DftChunk<R> *> &out, int c, const V3 &mn, const V3 &mxv, const std::vector<double> &freq, bool include_w,
+               cdouble stored_weight, int decimation) {
+    loop_in_chunks(mn, mxv, 20, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &,
+                                   cdouble sph) {
+      if (!f[c][0]) return;
+      std::unique_ptr<DftChunk<R>> ch(new DftChunk<R>);
+      ch->c = c;
+      gv.yee2cent(c, ch->avg1, ch->avg2);
+      ch->scale = stored_weight * sph * (dt / std::sqrt(2.0 * pi) * decimation);
+      ch->decimation = decimation;
+      std::vector<int64_t> idx;
+      std::vector<double> w;
+      visit_box(is, ie, s0, s1, e0, e1, dV0, [&](int64_t i, int64_t, const IV &, double ww, const int *) {
+        idx.push_back(i);
+        w.push_back(include_w ? ww : 1.0);
+      });
+      for (int64_t i : idx)
+        if (i < 0 || size_t(i + ch->avg1 + ch->avg2) >= gv.ntot) fail("internal: DFT point outside the array");
+      ch->npts = int(idx.size());
+      ch->nfreq = int(freq.size());
+      std::vector<double> om;
+      for (double fr : freq) om.push_back(2 * pi * fr);
+      ch->idx.alloc(sizeof(int64_t) * idx.size());
+      ch->w.alloc(sizeof(double) * w.size());
+      ch->omega.alloc(sizeof(double) * om.size());
+      HIPCHECK(hipMemcpy(ch->idx.p, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(ch->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(ch->omega.p, om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice));
+      ch->acc_kind = dft_acc;
+      const size_t nacc = size_t(ch->npts) * ch->nfreq;
+      const size_t esz = dft_acc == MP_DFT_F64 ? 16 : dft_acc == MP_DFT_KAHAN32 ? 8 : 2 * sizeof(R);
+      ch->acc.alloc(esz * nacc);
+      HIPCHECK(hipMemset(ch->acc.p, 0, esz * nacc));
+      if (dft_acc == MP_DFT_KAHAN32) {
+        ch->comp.alloc(8 * nacc);
+        HIPCHECK(hipMemset(ch->comp.p, 0, 8 * nacc));
+      }
+      out.push_back(ch.get());
+      dft_chunks.push_back(std::move(ch));
+    });
+  }
+  void *add_dft_flux(int d, V3 mn, V3 mxv, const std::vector<double> &freq, int dec, int *dec_out) override {
+    if (!components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+    const int decim = dec > 0 ? dec : auto_decimation(freq);
+    *dec_out = decim;
+    int cE[2], cH[2];
+    if (d == 0) {
+      cE[0] = 1, cE[1] = 4, cH[0] = 9, cH[1] = 6;
+    }
+    else if (d == 1) {
+      cE[0] = 4, cE[1] = 0, cH[0] = 5, cH[1] = 9;
+    }
+    else {
+      cE[0] = 0, cE[1] = 1, cH[0] = 6, cH[1] = 5;
+    }
+    std::unique_ptr<Flux<R>> fl(new Flux<R>);
+    fl->nfreq = int(freq.size());
+    std::vector<DftChunk<R> *> El[2], Hl[2];
+    for (int q = 0; q < 2; ++q) {
+      add_dft(El[q], cE[q], mn, mxv, freq, true, cdouble(double(1 - 2 * q), 0), decim);
+      add_dft(Hl[q], cH[q], mn, mxv, freq, false, cdouble(1.0, 0), decim);
+    }
+    // the reference prepends each chunk to its list (src/dft.cpp:120-129,623-641)
+    for (int q = 1; q >= 0; --q) {
+      for (int j = int(El[q].size()) - 1; j >= 0; --j) fl->E.push_back(El[q][j]);
+      for (int j = int(Hl[q].size()) - 1; j >= 0; --j) fl->H.push_back(Hl[q][j]);
+    }
+    fluxes.push_back(std::move(fl));
+    return fluxes.back().get();
+  }
+  std::vector<cdouble> read_dft(const DftChunk<R> &ch) {
+    HIPCHECK(hipStreamSynchronize(stream));
+    const size_t n = size_t(ch.npts) * ch.nfreq;
+    std::vector<cdouble> out(n);
+    if (ch.acc_kind == MP_DFT_F64) {
+      HIPCHECK(hipMemcpy(out.data(), ch.acc.p, 16 * n, hipMemcpyDeviceToHost));
+    }
+    else if (ch.acc_kind == MP_DFT_KAHAN32) {
+      std::vector<float> v(2 * n);
+      HIPCHECK(hipMemcpy(v.data(), ch.acc.p, 8 * n, hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < n; ++q) out[q] = cdouble(v[2 * q], v[2 * q + 1]);
+    }
+    else {
+      std::vector<R> v(2 * n);
+      HIPCHECK(hipMemcpy(v.data(), ch.acc.p, 2 * sizeof(R) * n, hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < n; ++q) out[q] = cdouble(v[2 * q], v[2 * q + 1]);
+    }
+    return out;
+  }
+  // dft_flux::flux (src/dft.cpp:542-556): the product is formed in the accumulator's type
+  void flux(void *flp, double *out) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    const int Nf = fl->nfreq;
+    for (int q = 0; q < Nf; ++q) out[q] = 0;
+    for (size_t q = 0; q < fl->E.size() && q < fl->H.size(); ++q) {
+      std::vector<cdouble> e = read_dft(*fl->E[q]), h = read_dft(*fl->H[q]);
+      const bool narrow = fl->E[q]->acc_kind == MP_DFT_REALNUM;
+      for (int kk = 0; kk < fl->E[q]->npts; ++kk)
+        for (int i = 0; i < Nf; ++i) {
+          const size_t o = size_t(kk) * Nf + i;
+          if (narrow) {
+            std::complex<R> ee(R(e[o].real()), R(e[o].imag())), hh(R(h[o].real()), R(h[o].imag()));
+            out[i] += double(std::real(ee * std::conj(hh)));
+          }
+          else
+            out[i] += std::real(e[o] * std::conj(h[o]));
+        }
+    }
+  }
+  int flux_chunks(void *flp, int which) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    return int(which ? fl->H.size() : fl->E.size());
+  }
+  long flux_points(void *flp, int which, int i) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    auto &l = which ? fl->H : fl->E;
+    if (i < 0 || i >= int(l.size())) fail("no such dft chunk");
+    return l[i]->npts;
+  }
+  void flux_dft(void *flp, int which, int i, double *out) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    auto &l = which ? fl->H : fl->E;
+    if (i < 0 || i >= int(l.size())) fail("no such dft chunk");
+    std::vector<cdouble> v = read_dft(*l[i]);
+    for (size_t q = 0; q < v.size(); ++q) {
+      out[2 * q] = v[q].real();
+      out[2 * q + 1] = v[q].imag();
+    }
+  }
+  void flux_delete(void *flp) override {
+    for (size_t q = 0; q < fluxes.size(); ++q)
+      if (fluxes[q].get() == flp) {
+        Flux<R> *fl = fluxes[q].get();
+        for (auto *lst : {&fl->E, &fl->H})
+          for (DftChunk<R> *ch : *lst)
+            for (size_t z = 0; z < dft_chunks.size(); ++z)
+              if (dft_chunks[z].get() == ch) {
+                dft_chunks.erase(dft_chunks.begin() + z);
+                break;
+              }
+        fluxes.erase(fluxes.begin() + q);
+        return;
+      }
+  }
+
+  // ---- readout ----
+  R read1(const R *p, int64_t i) {
+    R v;
+    HIPCHECK(hipMemcpy(&v, p + i, sizeof(R), hipMemcpyDeviceToHost));
+    return v;
+  }
+  cdouble get_field(int c, const V3 &p) override {
+    if (c < 0 || c >= NC || comp_dir(c) == NODIR) fail("invalid component %d", c);
+    HIPCHECK(hipStreamSynchronize(stream));
+    IV locs[8];
+    double w[8];
+    gv.interpolate(c, p, locs, w);
+    cdouble res = 0.0;
+    for (int q = 0; q < 8 && w[q]; ++q) {
+      IV il = locs[q];
+      cdouble kph = 1.0;
+      locate(&il, &kph);
+      cdouble v = 0.0;
+      if (gv.owns(il) && f[c][0]) {
+        const int64_t i = gv.index(c, il);
+        v = f[c][1] ? cdouble(read1(f[c][0], i), read1(f[c][1], i)) : cdouble(read1(f[c][0], i), 0);
+        v *= kph;
+      }
+      res += w[q] * v;
+    }
+    return res;
+  }
+  R *array_ptr(int c, int cmp, int kind, int ipol) {
+    if (c < 0 || c >= NC || cmp < 0 || cmp > 1) fail("invalid component");
+    if (kind == MP_ARR_F) return f[c][cmp];
+    if (kind == MP_ARR_F_U) return f_u[c][cmp];
+    if (kind == MP_ARR_F_W) return f_w[c][cmp];
+    for (auto &p : pols)
+      if (p.isus == ipol && p.allocated) return kind == MP_ARR_P ? p.P[c][cmp] : p.Pp[c][cmp];
+    return nullptr;
+  }
+  int get_array(int c, int cmp, int kind, int ipol, double *out) override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    R *p = array_ptr(c, cmp, kind, ipol);
+    if (!p) {
+      for (size_t i = 0; i < gv.ntot; ++i) out[i] = 0;
+      return 1;
+    }
+    std::vector<R> v(gv.ntot);
+    HIPCHECK(hipMemcpy(v.data(), p, sizeof(R) * gv.ntot, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < gv.ntot; ++i) out[i] = double(v[i]);
+    return 0;
+  }
+  void set_array(int c, int cmp, int kind, int ipol, const double *in) override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    R *p = array_ptr(c, cmp, kind, ipol);
+    if (!p) fail("array not allocated");
+    std::vector<R> v(gv.ntot);
+    for (size_t i = 0; i < gv.ntot; ++i) v[i] = R(in[i]);
+    HIPCHECK(hipMemcpy(p, v.data(), sizeof(R) * gv.ntot, hipMemcpyHostToDevice));
+  }
+  bool have_component(int c) const override { return c >= 0 && c < NC && f[c][0]; }
+  void times(double *t7) const override {
+    for (int q = 0; q < 7; ++q) t7[q] = timers[q];
+  }
+  void set_dft_acc(int kind) override {
+    if (kind < 0 || kind > 2) fail("unknown DFT accumulator kind");
+    if (kind == MP_DFT_KAHAN32 && sizeof(R) != sizeof(float)) fail("compensated FP32 accumulators need FP32 fields");
+    dft_acc = kind;
+  }
+};
+
+template <class F> int guard(F fn) {
+  try {
+    fn();
+    return 0;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return -1;
+  }
+}
+
+static V3 to_v3(const double *p) {
+  V3 v;
+  for (int d = 0; d < 3; ++d) v.t[d] = p ? p[d] : 0;
+  return v;
+}
+
+struct FluxHandle {
+  FieldsBase *owner;
+  void *fl;
+  int nfreq, decimation;
+};
+
+} // namespace mx
+
+using namespace mx;
+
+extern "C" {
+
+const char *mp_last_error(void) { return g_last_error.c_str(); }
+const char *mp_engine_name(void) { return "hip-gfx950"; }
+
+mp_structure *mp_structure_new(int dim, double xs, double ys, double zs, double a, mp_scalar_func eps, void *eu, int n_pml,
+                               const int *pd, const int *ps, const double *pt, const double *pR, double courant, int single) {
+  HostStructure *s = new HostStructure;
+  int rc = guard([&] {
+    auto np = [&](double sz) { return sz == 0 ? 1 : int(sz * a + 0.5); };
+    if (dim == 1)
+      s->gv.init(1, a, 0, 0, int(zs * a + 0.5));
+    else if (dim == 2)
+      s->gv.init(2, a, np(xs), np(ys), 0);
+    else if (dim == 3)
+      s->gv.init(3, a, np(xs), np(ys), np(zs));
+    else
+      fail("dim must be 1, 2 or 3");
+    s->single = single != 0;
+    s->courant = courant;
+    s->dt = courant / a;
+    const Geom &gv = s->gv;
+    for (int i = 0; i < n_pml; ++i) {
+      const int d = pd[i];
+      if (d < 0 || d > 2 || !gv.has(d) || gv.num[d] <= 1) continue;
+      host_use_pml(*s, d, pt[i], ps[i], pR && pR[i] > 0 ? pR[i] : 1e-15);
+    }
+    if (eps) // set_chi1inv without subpixel averaging (src/anisotropic_averaging.cpp:90-98)
+      for (int c = 0; c < 5; ++c) {
+        if (comp_dir(c) == NODIR || (gv.dim == 1 && c != 0)) continue;
+        std::vector<double> v(gv.ntot, 1.0);
+        bool trivial = true;
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = 0;
+          hi[d] = gv.has(d) ? gv.num[d] : 0;
+        }
+        for (int j0 = lo[0]; j0 <= hi[0]; ++j0)
+          for (int j1 = lo[1]; j1 <= hi[1]; ++j1)
+            for (int j2 = lo[2]; j2 <= hi[2]; ++j2) {
+              int j[3] = {j0, j1, j2};
+              IV here = gv.iloc(c, j);
+              V3 h = gv.loc(here);
+              double r[3];
+              for (int d = 0; d < 3; ++d) r[d] = gv.has(d) ? ((h.t[d] - 0.5 * gv.inva) + (h.t[d] + 0.5 * gv.inva)) * 0.5 : 0.0;
+              const int64_t idx = gv.index(c, here);
+              const double e = eps(r, eu);
+              const double val = single ? double(float(1 / e)) : 1 / e;
+              v[idx] = val;
+              if (val != 1.0) trivial = false;
+            }
+        if (!trivial) s->chi1inv[c] = std::move(v);
+      }
+  });
+  if (rc) {
+    delete s;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_structure *>(s);
+}
+void mp_structure_delete(mp_structure *s) { delete reinterpret_cast<HostStructure *>(s); }
+size_t mp_structure_ntot(const mp_structure *s) { return reinterpret_cast<const HostStructure *>(s)->gv.ntot; }
+int mp_structure_num(const mp_structure *s, int *n) {
+  for (int d = 0; d < 3; ++d) n[d] = reinterpret_cast<const HostStructure *>(s)->gv.num[d];
+  return 0;
+}
+double mp_structure_dt(const mp_structure *s) { return reinterpret_cast<const HostStructure *>(s)->dt; }
+int mp_structure_set_chi1inv(mp_structure *sp, int c, const double *arr) {
+  return guard([&] {
+    HostStructure *s = reinterpret_cast<HostStructure *>(sp);
+    if (c < 0 || c >= 5 || comp_dir(c) == NODIR) fail("chi1inv: electric component expected");
+    s->chi1inv[c].assign(arr, arr + s->gv.ntot);
+  });
+}
+int mp_structure_get_chi1inv(const mp_structure *sp, int c, double *arr) {
+  return guard([&] {
+    const HostStructure *s = reinterpret_cast<const HostStructure *>(sp);
+    if (c < 0 || c >= 5) fail("chi1inv: electric component expected");
+    for (size_t i = 0; i < s->gv.ntot; ++i) arr[i] = s->chi1inv[c].empty() ? 1.0 : s->chi1inv[c][i];
+  });
+}
+int mp_structure_add_susceptibility(mp_structure *sp, double omega_0, double gamma, int drude, mp_scalar_func sigma, void *su) {
+  int idx = -1;
+  int rc = guard([&] {
+    HostStructure *s = reinterpret_cast<HostStructure *>(sp);
+    const Geom &gv = s->gv;
+    HostSus sus;
+    sus.omega_0 = s->single ? double(float(omega_0)) : omega_0;
+    sus.gamma = s->single ? double(float(gamma)) : gamma;
+    sus.drude = drude != 0;
+    for (int c = 0; c < 5; ++c) {
+      if (comp_dir(c) == NODIR || (gv.dim == 1 && c != 0)) continue;
+      std::vector<double> v(gv.ntot, 0.0);
+      bool trivial = true;
+      for (int j0 = 0; j0 <= (gv.has(0) ? gv.num[0] : 0); ++j0)
+        for (int j1 = 0; j1 <= (gv.has(1) ? gv.num[1] : 0); ++j1)
+          for (int j2 = 0; j2 <= (gv.has(2) ? gv.num[2] : 0); ++j2) {
+            int j[3] = {j0, j1, j2};
+            IV here = gv.iloc(c, j);
+            V3 h = gv.loc(here);
+            double r[3] = {h.t[0], h.t[1], h.t[2]};
+            const double val = sigma ? sigma(r, su) : 0.0;
+            v[gv.index(c, here)] = val;
+            if (val != 0) trivial = false;
+          }
+      if (!trivial) sus.sigma[c] = std::move(v);
+    }
+    s->sus.push_back(std::move(sus));
+    idx = int(s->sus.size()) - 1;
+  });
+  return rc ? rc : idx;
+}
+int mp_structure_set_sigma(mp_structure *sp, int isus, int c, const double *sigma) {
+  return guard([&] {
+    HostStructure *s = reinterpret_cast<HostStructure *>(sp);
+    if (isus < 0 || isus >= int(s->sus.size())) fail("no such susceptibility");
+    if (c < 0 || c >= 5) fail("sigma: electric component expected");
+    bool trivial = true;
+    for (size_t i = 0; i < s->gv.ntot; ++i)
+      if (sigma[i] != 0) trivial = false;
+    if (trivial)
+      s->sus[isus].sigma[c].clear();
+    else
+      s->sus[isus].sigma[c].assign(sigma, sigma + s->gv.ntot);
+  });
+}
+
+mp_fields *mp_fields_new(mp_structure *sp) {
+  FieldsBase *f = nullptr;
+  int rc = guard([&] {
+    HostStructure *s = reinterpret_cast<HostStructure *>(sp);
+    if (s->single)
+      f = new Fields<float>(s);
+    else
+      f = new Fields<double>(s);
+  });
+  return rc ? nullptr : reinterpret_cast<mp_fields *>(f);
+}
+void mp_fields_delete(mp_fields *f) { delete reinterpret_cast<FieldsBase *>(f); }
+#define F_ reinterpret_cast<FieldsBase *>(f)
+int mp_fields_use_real_fields(mp_fields *f) {
+  return guard([&] { F_->use_real_fields(); });
+}
+int mp_fields_use_bloch(mp_fields *f, const double *k) {
+  return guard([&] {
+    int j = 0;
+    for (int d = 0; d < 3; ++d)
+      if (F_->s->gv.has(d)) F_->use_bloch(d, k[j++]);
+  });
+}
+int mp_fields_use_bloch_dir(mp_fields *f, int d, double kr, double ki) {
+  return guard([&] { F_->use_bloch(d, cdouble(kr, ki)); });
+}
+int mp_fields_set_boundary(mp_fields *f, int side, int d, int cond) {
+  return guard([&] { F_->set_boundary(side, d, cond); });
+}
+int mp_fields_add_point_source_legacy(mp_fields *f, int c, double freq, double width, double peaktime, double cutoff, const double *p,
+                                      double ar, double ai, int is_c) {
+  return guard([&] {
+    // src/sources.cpp:193-213
+    const double dt = F_->dt_now(), now = F_->t_now() * dt;
+    width /= freq;
+    SrcTime st;
+    if (is_c)
+      st = SrcTime::continuous(freq, width, now, 1e300, cutoff);
+    else {
+      cutoff = F_->s->gv.inva + cutoff * width;
+      if (peaktime <= 0.0) peaktime = now + cutoff;
+      peaktime += is_magnetic(c) ? -dt * 0.5 : dt;
+      st = SrcTime::gaussian(freq, width, peaktime - cutoff, peaktime + cutoff);
+    }
+    st.is_integrated = !is_magnetic(c);
+    V3 pv = to_v3(p);
+    F_->add_volume_source(c, st, pv, pv, cdouble(ar, ai), nullptr, nullptr);
+  });
+}
+int mp_fields_add_volume_source(mp_fields *f, int c, const mp_src_time *src, const double *vmin, const double *vmax, double ar, double ai,
+                                mp_complex_func A, void *Au) {
+  return guard([&] {
+    SrcTime st = src->kind == 0 ? SrcTime::gaussian(src->freq, src->width, src->start_time, src->end_time)
+                                : SrcTime::continuous(src->freq, src->width, src->start_time, src->end_time, src->slowness);
+    st.is_integrated = src->is_integrated != 0;
+    F_->add_volume_source(c, st, to_v3(vmin), to_v3(vmax), cdouble(ar, ai), A, Au);
+  });
+}
+int mp_fields_step(mp_fields *f) {
+  return guard([&] { F_->step_n(1, true); });
+}
+int mp_fields_step_n(mp_fields *f, long n) {
+  return guard([&] { F_->step_n(n, false); });
+}
+double mp_fields_time(const mp_fields *f) {
+  auto *b = reinterpret_cast<const FieldsBase *>(f);
+  return b->t_now() * b->dt_now();
+}
+double mp_fields_round_time(const mp_fields *f) {
+  auto *b = reinterpret_cast<const FieldsBase *>(f);
+  return double(float(b->t_now() * b->dt_now()));
+}
+long mp_fields_t(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->t_now(); }
+double mp_fields_dt(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->dt_now(); }
+double mp_fields_last_source_time(mp_fields *f) { return F_->last_source_time(); }
+int mp_fields_get_field(mp_fields *f, int c, const double *p, double *re, double *im) {
+  return guard([&] {
+    cdouble v = F_->get_field(c, to_v3(p));
+    *re = v.real();
+    *im = v.imag();
+  });
+}
+int mp_fields_get_array(mp_fields *f, int c, int cmp, int kind, int ipol, double *out) {
+  int missing = 0;
+  int rc = guard([&] { missing = F_->get_array(c, cmp, kind, ipol, out); });
+  return rc ? rc : missing;
+}
+int mp_fields_set_array(mp_fields *f, int c, int cmp, int kind, int ipol, const double *in) {
+  return guard([&] { F_->set_array(c, cmp, kind, ipol, in); });
+}
+int mp_fields_have_component(mp_fields *f, int c) { return F_->have_component(c) ? 1 : 0; }
+int mp_fields_get_times(mp_fields *f, double *t7) {
+  F_->times(t7);
+  return 0;
+}
+int mp_fields_set_dft_accumulator(mp_fields *f, int kind) {
+  return guard([&] { F_->set_dft_acc(kind); });
+}
+mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin, const double *vmax, const double *freq, int nfreq, int dec) {
+  FluxHandle *h = new FluxHandle;
+  int rc = guard([&] {
+    std::vector<double> fr(freq, freq + nfreq);
+    h->owner = F_;
+    h->nfreq = nfreq;
+    h->fl = F_->add_dft_flux(d, to_v3(vmin), to_v3(vmax), fr, dec, &h->decimation);
+  });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_dft_flux *>(h);
+}
+void mp_dft_flux_delete(mp_dft_flux *flp) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
+  if (!h) return;
+  h->owner->flux_delete(h->fl);
+  delete h;
+}
+int mp_dft_flux_flux(mp_dft_flux *flp, double *out) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
+  return guard([&] { h->owner->flux(h->fl, out); });
+}
+int mp_dft_flux_num_chunks(mp_dft_flux *flp, int which) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
+  return h->owner->flux_chunks(h->fl, which);
+}
+long mp_dft_flux_chunk_points(mp_dft_flux *flp, int which, int i) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
+  long n = -1;
+  guard([&] { n = h->owner->flux_points(h->fl, which, i); });
+  return n;
+}
+int mp_dft_flux_decimation(mp_dft_flux *flp) { return reinterpret_cast<FluxHandle *>(flp)->decimation; }
+int mp_dft_flux_get_dft(mp_dft_flux *flp, int which, int i, double *out) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
+  return guard([&] { h->owner->flux_dft(h->fl, which, i, out); });
+}
+
+} // extern "C"

@@ -1,0 +1,131 @@
+"""Scenarios run through the C-ABI on any library (the CPU oracle or the MI355X engine).
+
+Each builder returns (structure, fields, fluxes) after adding sources and monitors; the
+parity tests step both libraries identically and compare every state array.
+"""
+
+import math
+
+import numpy as np
+
+from meep_amd.capi import (Bx, By, Bz, Dx, Dy, Dz, Ex, Ey, Ez, Hx, Hy, Hz, High, Low, X, Y, Z,
+                           Magnetic, Metallic, continuous_src_time, fields, gaussian_src_time,
+                           linspace, pml, structure)
+
+
+def blocks_eps(seed, n_blocks, lo=1.0, hi=12.0, size=(1.0, 1.0, 1.0)):
+    """Piecewise-constant eps on a seeded coarse grid of blocks (a function of position)."""
+    rng = np.random.default_rng(seed)
+    vals = rng.uniform(lo, hi, size=(n_blocks,) * 3)
+
+    def eps(x, y, z):
+        i = min(int(x / size[0] * n_blocks), n_blocks - 1) if size[0] > 0 else 0
+        j = min(int(y / size[1] * n_blocks), n_blocks - 1) if size[1] > 0 else 0
+        k = min(int(z / size[2] * n_blocks), n_blocks - 1) if size[2] > 0 else 0
+        return float(vals[max(i, 0), max(j, 0), max(k, 0)])
+    return eps
+
+
+def waveguide_2d(lib, single=False, a=10, real=True, nfreq=20):
+    """Scaled-down straight waveguide (the shape of python/examples/straight-waveguide.py):
+    eps=12 strip, PML, Gaussian Ez line-free point source (Python-style, not integrated),
+    transverse flux line."""
+    sx, sy, dpml = 6.0, 4.0, 1.0
+
+    def eps(x, y, z):
+        return 12.0 if abs(y - sy / 2) <= 0.5 else 1.0
+    s = structure(lib, dim=2, size=(sx, sy), a=a, eps=eps, pml=pml(dpml), single=single)
+    f = fields(s)
+    if real:
+        f.use_real_fields()
+    src = gaussian_src_time(0.15, 0.1)
+    f.add_volume_source(Ez, src, (dpml + 0.5, sy / 2, 0), (dpml + 0.5, sy / 2, 0))
+    fl = f.add_dft_flux(X, (sx - dpml - 0.5, sy / 2 - 1.5, 0), (sx - dpml - 0.5, sy / 2 + 1.5, 0),
+                        linspace(0.1, 0.2, nfreq))
+    return s, f, [fl]
+
+
+def box_3d_pml_lorentz(lib, single=False, a=8, real=True):
+    """Small 3D cell: PML on every face, lower half a Lorentz + Drude medium, blocks above,
+    Ex plane-wave current (a z-plane longer than the kernel's sparse table), flux plane."""
+    L = 2.0
+    eps_blk = blocks_eps(3, 4, 2.0, 6.0, (L, L, L))
+
+    def eps(x, y, z):
+        return 2.0 if z < L / 2 else eps_blk(x, y, z)
+
+    def lower(x, y, z):
+        return 1.0 if z < L / 2 else 0.0
+    s = structure(lib, dim=3, size=(L, L, L), a=a, eps=eps, pml=pml(0.375), single=single)
+    s.add_susceptibility(lambda x, y, z: 1.5 * lower(x, y, z), 1.0, 0.05)
+    s.add_susceptibility(lambda x, y, z: 3.0 * lower(x, y, z), 1.0, 0.1, drude=True)
+    f = fields(s)
+    if real:
+        f.use_real_fields()
+    src = gaussian_src_time(0.8, 0.4)
+    f.add_volume_source(Ex, src, (0.375, 0.375, 1.5), (L - 0.375, L - 0.375, 1.5))
+    fl = f.add_dft_flux(Z, (0.4, 0.4, 1.25), (L - 0.4, L - 0.4, 1.25), linspace(0.5, 1.1, 8))
+    return s, f, [fl]
+
+
+def bloch_3d(lib, single=False, a=6, k=(0.3, 0.5, 0.8)):
+    """Fully periodic complex fields with a Bloch vector (CONNECT_PHASE), blocks eps,
+    integrated point dipole (the legacy C++ source) and a continuous magnetic source."""
+    s = structure(lib, dim=3, size=(1.5, 1.0, 2.0), a=a, eps=blocks_eps(5, 3, 1.0, 12.0, (1.5, 1.0, 2.0)),
+                  single=single)
+    f = fields(s)
+    f.add_point_source(Ey, 0.4, 3.0, 0.0, 2.0, (0.7, 0.5, 1.0), complex(0, -2 * math.pi * 0.4))
+    f.add_volume_source(Hz, continuous_src_time(0.5, 2.0), (0.2, 0.3, 0.4), (0.2, 0.3, 0.4), 0.3)
+    f.use_bloch(k)
+    return s, f, []
+
+
+def metal_magnetic_2d_te(lib, single=False, a=12):
+    """2D TE (Hz, Ex, Ey) with a magnetic low-x wall, metallic elsewhere, y-PML, and an
+    integrated line source longer than the kernel's sparse table."""
+    s = structure(lib, dim=2, size=(2.0, 3.0), a=a, eps=blocks_eps(7, 5, 1.0, 8.0, (2.0, 3.0, 1.0)),
+                  pml=pml(0.5, Y), single=single)
+    f = fields(s)
+    f.set_boundary(Low, X, Magnetic)
+    src = gaussian_src_time(0.6, 0.3, is_integrated=True)
+    f.add_volume_source(Ex, src, (0.3, 0.8, 0), (1.7, 0.8, 0), amp_func=lambda x, y, z: 1 + x)
+    fl = f.add_dft_flux(Y, (0.2, 2.2, 0), (1.8, 2.2, 0), linspace(0.4, 0.8, 5))
+    return s, f, [fl]
+
+
+def polariton_1d(lib, single=False):
+    """1D Lorentzian medium with the legacy integrated source (known_results polariton)."""
+    s = structure(lib, dim=1, size=(0, 0, 1.0), a=10, eps=lambda x, y, z: 1.0, single=single)
+    s.add_susceptibility(lambda x, y, z: 7.63, 0.3, 0.1)
+    f = fields(s)
+    f.add_point_source(Ex, 0.2, 3.0, 0.0, 2.0, (0, 0, 0.5), complex(0, -2 * math.pi * 0.2))
+    return s, f, []
+
+
+SCENARIOS = {
+    "waveguide_2d": (waveguide_2d, 1600),
+    "box_3d_pml_lorentz": (box_3d_pml_lorentz, 120),
+    "bloch_3d": (bloch_3d, 80),
+    "metal_magnetic_2d_te": (metal_magnetic_2d_te, 200),
+    "polariton_1d": (polariton_1d, 200),
+}
+
+FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)
+
+
+def state(f, s, npol=0):
+    """Every allocated field and polarisation array, keyed (name, component, cmp)."""
+    out = {}
+    for c in FIELD_COMPONENTS:
+        if not f.have_component(c):
+            continue
+        for cmp in (0, 1):
+            arr, ok = f.get_array(c, cmp)
+            if ok:
+                out[("f", c, cmp)] = arr
+            for ip in range(npol):
+                for kind, name in ((3, "P"), (4, "Pp")):
+                    arr, ok = f.get_array(c, cmp, kind, ip)
+                    if ok:
+                        out[(name, c, cmp, ip)] = arr
+    return out
