@@ -17,6 +17,7 @@ X, Y, Z = 0, 1, 2
 High, Low = 0, 1
 Periodic, Metallic, Magnetic, NoBoundary = 0, 1, 2, 3
 ARR_F, ARR_F_U, ARR_F_W, ARR_P, ARR_P_PREV = 0, 1, 2, 3, 4
+K_STEP_BH, K_STEP_DE, K_HALO, K_DFT, K_SRC = 0, 1, 2, 3, 4
 DFT_REALNUM, DFT_F64, DFT_KAHAN32 = 0, 1, 2
 
 component_name = {Ex: "ex", Ey: "ey", Ez: "ez", Hx: "hx", Hy: "hy", Hz: "hz",
@@ -92,6 +93,10 @@ _SIGNATURES = {
     "mp_fields_have_component": (C.c_int, [C.c_void_p, C.c_int]),
     "mp_fields_get_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mp_fields_set_dft_accumulator": (C.c_int, [C.c_void_p, C.c_int]),
+    "mp_fields_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "mp_fields_get_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_long)]),
+    "mp_fields_reset_kernel_timing": (C.c_int, [C.c_void_p]),
     "mp_fields_add_dft_flux": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.c_int, C.c_int]),
@@ -334,6 +339,19 @@ class fields:
         t = (C.c_double * 7)()
         self.lib.mp_fields_get_times(self.ptr, t)
         return list(t)
+
+    def set_kernel_timing(self, on=True):
+        _check(self.lib, self.lib.mp_fields_set_kernel_timing(self.ptr, int(on)))
+
+    def kernel_timing(self, which):
+        """(total milliseconds, launches) of one kernel class since the last reset."""
+        ms, n = C.c_double(), C.c_long()
+        _check(self.lib, self.lib.mp_fields_get_kernel_timing(self.ptr, which, C.byref(ms),
+                                                              C.byref(n)))
+        return ms.value, n.value
+
+    def reset_kernel_timing(self):
+        _check(self.lib, self.lib.mp_fields_reset_kernel_timing(self.ptr))
 
     def set_dft_accumulator(self, kind):
         _check(self.lib, self.lib.mp_fields_set_dft_accumulator(self.ptr, kind))

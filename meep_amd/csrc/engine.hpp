@@ -13,16 +13,19 @@ namespace mx {
 constexpr int MAXPOLE = 4;  // Lorentz/Drude poles per field type
 constexpr int SRC_TAB = 12; // sparse source entries carried in the kernel arguments
 
-// Geometry of one chunk.  Loop axis k (0 slowest, 2 stride-1) walks direction axdir[k]
-// (-1 when the grid has no such direction); this is the reference's yucky_direction order
-// (src/vec.cpp:262-270): X,Y,Z in 3D, Z(empty),X,Y in 2D, Z in 1D.
+// Geometry of one chunk in the device layout, plus the box a launch covers.  Loop axis k
+// (0 slowest, 2 stride-1) walks direction axdir[k] (-1 when the grid has no such direction);
+// this is the reference's yucky_direction order (src/vec.cpp:262-270): X,Y,Z in 3D,
+// Z(empty),X,Y in 2D, Z in 1D.  The stride-1 axis is padded to a multiple of 4 points so
+// every row starts 16-byte aligned; padding points belong to no component.
 struct Grid {
-  uint32_t ntot;
-  int ext[3];    // array extent along each loop axis (n+1, or 1 for an absent axis)
-  int axdir[3];  // direction of each loop axis
-  int num[3];    // cells per direction X,Y,Z (0 when absent)
-  int hi_metal;  // bit d set: the High wall normal to d is Metallic (src/boundaries.cpp:191-205)
+  int64_t st[3];   // device strides per direction X,Y,Z (0 when absent)
+  int axdir[3];    // direction of each loop axis
+  int num[3];      // cells per direction X,Y,Z (0 when absent)
+  int hi_metal;    // bit d set: the High wall normal to d is Metallic (src/boundaries.cpp:191-205)
   int lo_magnetic; // bit d set: the Low wall normal to d is Magnetic
+  int blo[3];      // launch box: lowest index per direction
+  int bhi[3];      // launch box: highest index per direction
 };
 
 template <class R> struct PmlProfile {
@@ -64,6 +67,7 @@ template <class R> struct PoleCoef {
 // reference does (src/step.cpp:310-315, src/update_eh.cpp:128-138).
 struct SrcTab {
   int n;
+  int64_t lo, hi; // index range of the entries (cells outside it skip the table)
   int comp[SRC_TAB];
   int64_t idx[SRC_TAB];
   double val[SRC_TAB];
@@ -79,6 +83,15 @@ template <class R> struct StepArgs {
   PoleCoef<R> pole[MAXPOLE];
   SrcTab cur; // currents, after the curl
   SrcTab dip; // integrated dipoles, inside f_minus_p
+  // Slots are by direction: cc[d] / uc[d] is the component along d (f == nullptr when
+  // absent).  For the vectorised kernel the curl partners are given by direction too: F[d]
+  // is the partner field's component along d; slot d's g1 is F[(d+2)%3] differenced along
+  // d+1 and its g2 is F[(d+1)%3] along d+2 (the Cartesian plan of src/fields.cpp:428-456),
+  // when hasp[d] / hasm[d].  sgn is +1 for forward (B) and -1 for backward (D) differences.
+  const R *F[3];
+  int hasp[3], hasm[3];
+  int sgn;
+  R dtdx; // Courant number (the vectorised kernel applies the partner signs itself)
 };
 
 // Halo connection (connect_the_chunks / process_incoming_chunk_data,
@@ -105,7 +118,10 @@ template <class R> struct DftArgs {
 };
 
 // launchers (kernels.hip)
+// generic scalar kernel over the box a.g.blo..bhi (any cells, uPML included)
 template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
+// vectorised kernel over a box in which no uPML profile is active (16-byte accesses)
+template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
 template <class R>
 void launch_conn(const Conn *conns, int n, R *const *slot_ptrs, int ncmp, hipStream_t s);
 template <class R> void launch_src(R *f, const int64_t *idx, const double *val, int n, hipStream_t s);

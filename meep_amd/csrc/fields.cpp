@@ -17,6 +17,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -277,6 +278,9 @@ struct FieldsBase {
   virtual bool have_component(int c) const = 0;
   virtual void times(double *t7) const = 0;
   virtual void set_dft_acc(int kind) = 0;
+  virtual void set_ktiming(bool on) = 0;
+  virtual void get_ktiming(int which, double *ms, long *n) = 0;
+  virtual void reset_ktiming() = 0;
   virtual void *add_dft_flux(int d, V3 mn, V3 mx, const std::vector<double> &freq, int dec, int *dec_out) = 0;
   virtual void flux(void *fl, double *out) = 0;
   virtual int flux_chunks(void *fl, int which) = 0;
@@ -345,6 +349,69 @@ template <class R> struct Fields : FieldsBase {
   int dft_acc = MP_DFT_REALNUM;
   double timers[7] = {0, 0, 0, 0, 0, 0, 0};
 
+  // vectorised kernel on the interior box unless MEEP_AMD_GENERIC_KERNEL is set (both kernels
+  // are held to the oracle by the tests)
+  bool use_fast = getenv("MEEP_AMD_GENERIC_KERNEL") == nullptr;
+  // kernel timing: event pairs recorded around launches, resolved after each synchronisation
+  bool ktiming = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  double k_ms[5] = {0, 0, 0, 0, 0};
+  long k_n[5] = {0, 0, 0, 0, 0};
+  int open_kind = -1;
+  hipEvent_t open_ev = nullptr;
+  hipEvent_t get_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  void kbegin(int kind) {
+    if (!ktiming) return;
+    open_kind = kind;
+    open_ev = get_event();
+    HIPCHECK(hipEventRecord(open_ev, stream));
+  }
+  void kend() {
+    if (!ktiming || open_kind < 0) return;
+    hipEvent_t b = get_event();
+    HIPCHECK(hipEventRecord(b, stream));
+    pending.push_back({open_kind, open_ev, b});
+    open_kind = -1;
+  }
+  void resolve_events() {
+    for (const Pending &p : pending) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
+      k_ms[p.kind] += ms;
+      k_n[p.kind] += 1;
+      ev_pool.push_back(p.a);
+      ev_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void set_ktiming(bool on) override { ktiming = on; }
+  void get_ktiming(int which, double *ms, long *n) override {
+    if (which < 0 || which > 4) fail("unknown kernel class %d", which);
+    HIPCHECK(hipStreamSynchronize(stream));
+    resolve_events();
+    *ms = k_ms[which];
+    *n = k_n[which];
+  }
+  void reset_ktiming() override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    resolve_events();
+    for (int q = 0; q < 5; ++q) k_ms[q] = 0, k_n[q] = 0;
+  }
+
   R *dalloc(const R *copy_from) {
     bufs.emplace_back(new DevBuf);
     bufs.back()->alloc(sizeof(R) * gv.ntot);
@@ -362,11 +429,18 @@ template <class R> struct Fields : FieldsBase {
     return static_cast<T *>(bufs.back()->p);
   }
   int ncmp() const { return is_real ? 1 : 2; }
+  // a reference-layout host array in the device layout (padding filled with pad)
+  std::vector<R> to_device(const std::vector<double> &ref, R pad) const {
+    std::vector<R> v(gv.ntot, pad);
+    gv.for_each_point([&](int64_t r, int64_t d) { v[d] = R(ref[r]); });
+    return v;
+  }
 
   explicit Fields(HostStructure *st) {
     ensure_gpu();
     s = st;
     gv = st->gv;
+    gv.pad_rows(4); // 16-byte aligned rows for both FP32 (4 points) and FP64 (2 points)
     a = gv.a;
     dt = st->dt;
     courant = st->courant;
@@ -380,11 +454,7 @@ template <class R> struct Fields : FieldsBase {
     for (int c = 0; c < NC; ++c) have_p[c] = have_m[c] = false;
     // materials
     for (int c = 0; c < 5; ++c)
-      if (!s->chi1inv[c].empty()) {
-        std::vector<R> v(gv.ntot);
-        for (size_t i = 0; i < gv.ntot; ++i) v[i] = R(s->chi1inv[c][i]);
-        d_chi1inv[c] = upload(v);
-      }
+      if (!s->chi1inv[c].empty()) d_chi1inv[c] = upload(to_device(s->chi1inv[c], R(1)));
     for (int d = 0; d < 3; ++d)
       if (s->pml_dir[d]) {
         size_t n = s->sig[d].size();
@@ -404,11 +474,7 @@ template <class R> struct Fields : FieldsBase {
       p.isus = i;
       const HostSus &su = s->sus[i];
       for (int c = 0; c < 5; ++c)
-        if (!su.sigma[c].empty()) {
-          std::vector<R> v(gv.ntot);
-          for (size_t q = 0; q < gv.ntot; ++q) v[q] = R(su.sigma[c][q]);
-          p.sigma[c] = upload(v);
-        }
+        if (!su.sigma[c].empty()) p.sigma[c] = upload(to_device(su.sigma[c], R(0)));
       // lorentzian_susceptibility::update_P coefficients in realnum (src/susceptibility.cpp:192-196)
       const R om = R(su.omega_0), ga = R(su.gamma), rdt = R(dt);
       const R omega2pi = R(2 * pi * om), g2pi = R(ga * 2 * pi);
@@ -432,6 +498,11 @@ template <class R> struct Fields : FieldsBase {
           if (sv.d_idx) (void)hipFree(sv.d_idx);
           if (sv.d_val) (void)hipFree(sv.d_val);
         }
+      for (const Pending &p : pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+      }
+      for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream);
     }
   }
@@ -666,19 +737,74 @@ template <class R> struct Fields : FieldsBase {
     plan_valid = true;
   }
   void boundaries(int ft) {
-    if (n_conn[ft]) launch_conn<R>(static_cast<const Conn *>(d_conn[ft].p), n_conn[ft], static_cast<R *const *>(d_slots.p), ncmp(), stream);
+    if (!n_conn[ft]) return;
+    kbegin(MP_K_HALO);
+    launch_conn<R>(static_cast<const Conn *>(d_conn[ft].p), n_conn[ft], static_cast<R *const *>(d_slots.p), ncmp(), stream);
+    kend();
   }
-
+  int cur_cmp = 0; // real (0) or imaginary (1) part being stepped
+  // Every pointer a step kernel can dereference for the slots it will visit is checked on
+  // the host first, so a planning mistake raises an error instead of faulting the GPU.
+  void check_args(const StepArgs<R> &a, bool curl, bool upd, bool fast) const {
+    for (int c = 0; c < a.nc; ++c) {
+      if (curl && a.cc[c].f) {
+        const CurlComp<R> &k = a.cc[c];
+        if (k.g2 && !k.g1) fail("internal: curl slot %d has g2 without g1", c);
+        if (k.dsigu >= 0 && !k.fu) fail("internal: curl slot %d lacks its PML auxiliary", c);
+        if ((k.dsig >= 0 || k.dsigu >= 0) && (!a.pml.sig[k.dsig >= 0 ? k.dsig : k.dsigu])) fail("internal: missing PML profile");
+        if (fast) {
+          if (a.hasp[c] && !a.F[(c + 2) % 3]) fail("internal: fast slot %d lacks its plus partner", c);
+          if (a.hasm[c] && !a.F[(c + 1) % 3]) fail("internal: fast slot %d lacks its minus partner", c);
+        }
+      }
+      if (upd && a.uc[c].f) {
+        const UpdComp<R> &u = a.uc[c];
+        if (!u.g) fail("internal: update slot %d has no source array", c);
+        if (u.dsigw >= 0 && (!u.fw || !a.pml.sig[u.dsigw])) fail("internal: update slot %d lacks its PML auxiliary", c);
+        for (int p = 0; p < MAXPOLE; ++p)
+          if (p < a.npole && u.P[p] && !u.Pp[p]) fail("internal: polarisation without its previous value");
+      }
+    }
+  }
+  void step_kernel(const StepArgs<R> &a, bool curl, bool upd, int kind) {
+    if (!boxes_valid) compute_boxes();
+    check_args(a, curl, upd, false);
+    kbegin(kind);
+    StepArgs<R> b = a;
+    if (!interior_empty) {
+      for (int d = 0; d < 3; ++d) {
+        b.g.blo[d] = interior.lo[d];
+        b.g.bhi[d] = interior.hi[d];
+      }
+      if (use_fast && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b))) {
+        check_args(b, curl, upd, true);
+        launch_fast<R>(b, curl, upd, stream);
+      }
+      else
+        launch_step<R>(b, curl, upd, stream);
+    }
+    for (const Box &bx : slabs) {
+      for (int d = 0; d < 3; ++d) {
+        b.g.blo[d] = bx.lo[d];
+        b.g.bhi[d] = bx.hi[d];
+      }
+      launch_step<R>(b, curl, upd, stream);
+    }
+    kend();
+  }
   // ---- per-step argument blocks ----
   Grid grid() const {
     Grid g;
-    g.ntot = uint32_t(gv.ntot);
     for (int kx = 0; kx < 3; ++kx) {
-      int d = gv.ydir(kx);
+      const int d = gv.ydir(kx);
       g.axdir[kx] = gv.has(d) ? d : -1;
-      g.ext[kx] = gv.has(d) ? gv.num[d] + 1 : 1;
     }
-    for (int d = 0; d < 3; ++d) g.num[d] = gv.num[d];
+    for (int d = 0; d < 3; ++d) {
+      g.st[d] = gv.st[d];
+      g.num[d] = gv.num[d];
+      g.blo[d] = 0;
+      g.bhi[d] = gv.has(d) ? gv.num[d] : 0;
+    }
     g.hi_metal = g.lo_magnetic = 0;
     for (int d = 0; d < 3; ++d)
       if (gv.has(d)) {
@@ -687,15 +813,80 @@ template <class R> struct Fields : FieldsBase {
       }
     return g;
   }
+  // The interior box: array indices j along each direction where no uPML profile is active
+  // for either Yee shift (so every component there updates with the plain formulas), and
+  // the slabs that cover the rest of the grid without overlap.
+  struct Box {
+    int lo[3], hi[3];
+  };
+  bool boxes_valid = false;
+  Box interior;
+  bool interior_empty = false;
+  std::vector<Box> slabs;
+  void compute_boxes() {
+    int ilo[3], ihi[3], full_hi[3];
+    interior_empty = false;
+    for (int d = 0; d < 3; ++d) {
+      full_hi[d] = gv.has(d) ? gv.num[d] : 0;
+      ilo[d] = 0;
+      ihi[d] = full_hi[d];
+      if (!gv.has(d) || !s->pml_dir[d]) continue;
+      auto quiet = [&](int j) {
+        for (int k = 2 * j; k <= 2 * j + 1; ++k)
+          if (s->sig[d][k] != 0.0 || s->kap[d][k] != 1.0) return false;
+        return true;
+      };
+      while (ilo[d] <= full_hi[d] && !quiet(ilo[d])) ++ilo[d];
+      while (ihi[d] >= ilo[d] && !quiet(ihi[d])) --ihi[d];
+      for (int j = ilo[d]; j <= ihi[d]; ++j)
+        if (!quiet(j)) fail("uPML profile active inside the interior of direction %d", d);
+      if (ilo[d] > ihi[d]) interior_empty = true;
+    }
+    slabs.clear();
+    if (interior_empty) {
+      Box b;
+      for (int d = 0; d < 3; ++d) b.lo[d] = 0, b.hi[d] = full_hi[d];
+      slabs.push_back(b);
+    }
+    else {
+      for (int d = 0; d < 3; ++d) interior.lo[d] = ilo[d], interior.hi[d] = ihi[d];
+      // slabs: along X over everything, then Y within the X-interior, then Z within both
+      int clo[3], chi[3];
+      for (int d = 0; d < 3; ++d) clo[d] = 0, chi[d] = full_hi[d];
+      for (int d = 0; d < 3; ++d) {
+        if (!gv.has(d)) continue;
+        if (ilo[d] > 0) {
+          Box b;
+          for (int q = 0; q < 3; ++q) b.lo[q] = clo[q], b.hi[q] = chi[q];
+          b.lo[d] = 0;
+          b.hi[d] = ilo[d] - 1;
+          slabs.push_back(b);
+        }
+        if (ihi[d] < full_hi[d]) {
+          Box b;
+          for (int q = 0; q < 3; ++q) b.lo[q] = clo[q], b.hi[q] = chi[q];
+          b.lo[d] = ihi[d] + 1;
+          b.hi[d] = full_hi[d];
+          slabs.push_back(b);
+        }
+        clo[d] = ilo[d];
+        chi[d] = ihi[d];
+      }
+    }
+    boxes_valid = true;
+  }
   bool pml_in(int d) const { return d >= 0 && d < 3 && s->pml_dir[d]; }
 
   // step_db's per-component setup (src/step_db.cpp:47-84); allocates f_u lazily as a copy of f
   int curl_comps(int ft, int cmp, CurlComp<R> *cc, int *which) {
-    int n = 0;
     const int c0 = ft == B_ft ? 15 : 10;
-    for (int cc_ = c0; cc_ < c0 + 5; ++cc_) {
-      if (!f[cc_][cmp]) continue;
-      CurlComp<R> &k = cc[n];
+    for (int d = 0; d < 3; ++d) {
+      CurlComp<R> &k = cc[d];
+      memset(&k, 0, sizeof k);
+      k.dsig = k.dsigu = -1;
+      which[d] = -1;
+      const int cc_ = dir_comp(c0, d);
+      if (!has_field(cc_) || !f[cc_][cmp]) continue;
       const int dc = comp_dir(cc_);
       const int dsig0 = cycle_dir(dc, 1), dsigu0 = cycle_dir(dc, 2);
       k.dsig = pml_in(dsig0) ? dsig0 : -1;
@@ -721,10 +912,10 @@ template <class R> struct Fields : FieldsBase {
       k.s1 = s1;
       k.s2 = s2;
       k.dtdx = dtdx;
-      for (int d = 0; d < 3; ++d) {
-        k.lo[d] = gv.own_lo(cc_, d);
-        k.hi[d] = gv.own_hi(cc_, d);
-        k.ysh[d] = gv.ysh(cc_, d);
+      for (int q = 0; q < 3; ++q) {
+        k.lo[q] = gv.own_lo(cc_, q);
+        k.hi[q] = gv.own_hi(cc_, q);
+        k.ysh[q] = gv.ysh(cc_, q);
       }
       // the owned range keeps i + s1 and i + s2 inside the array (checked, not assumed)
       for (int which_s = 0; which_s < 2; ++which_s) {
@@ -732,16 +923,38 @@ template <class R> struct Fields : FieldsBase {
         if (!used) continue;
         const int64_t sv = which_s == 0 ? s1 : s2;
         int dd = -1;
-        for (int d = 0; d < 3; ++d)
-          if (gv.has(d) && (gv.st[d] == sv || gv.st[d] == -sv)) dd = d;
+        for (int q = 0; q < 3; ++q)
+          if (gv.has(q) && (gv.st[q] == sv || gv.st[q] == -sv)) dd = q;
         if (dd < 0) fail("internal: curl stride matches no direction");
         if ((sv > 0 && k.hi[dd] + 1 > gv.num[dd]) || (sv < 0 && k.lo[dd] - 1 < 0))
           fail("internal: curl neighbour outside the array");
       }
-      which[n] = cc_;
-      ++n;
+      which[d] = cc_;
     }
-    return n;
+    return 3;
+  }
+  // Partner description for the vectorised kernel: F[d] is the partner component along d and
+  // slot d's plus / minus partners must be F[(d+2)%3] along d+1 and F[(d+1)%3] along d+2.
+  // Returns false when the plan has another shape (then the generic kernel runs everywhere).
+  bool fast_partners(int ft, int cmp, StepArgs<R> &a) const {
+    const int p0 = ft == B_ft ? 0 : 5, c0 = ft == B_ft ? 15 : 10;
+    for (int d = 0; d < 3; ++d) a.F[d] = has_field(dir_comp(p0, d)) ? f[dir_comp(p0, d)][cmp] : nullptr;
+    for (int d = 0; d < 3; ++d) {
+      a.hasp[d] = a.hasm[d] = 0;
+      const int cc_ = dir_comp(c0, d);
+      if (!has_field(cc_) || !f[cc_][cmp]) continue;
+      if (have_p[cc_] && f[plus_c[cc_]][cmp]) {
+        if (comp_dir(plus_c[cc_]) != (d + 2) % 3 || plus_d[cc_] != (d + 1) % 3) return false;
+        a.hasp[d] = 1;
+      }
+      if (have_m[cc_] && f[minus_c[cc_]][cmp]) {
+        if (comp_dir(minus_c[cc_]) != (d + 1) % 3 || minus_d[cc_] != (d + 2) % 3) return false;
+        a.hasm[d] = 1;
+      }
+    }
+    a.sgn = ft == B_ft ? 1 : -1;
+    a.dtdx = R(courant);
+    return true;
   }
   // update_eh's per-component setup (src/update_eh.cpp:148-195), aligned with the curl
   // components in src (entry q updates the E/H partner of src[q]).  Allocates H (a copy of B)
@@ -750,9 +963,11 @@ template <class R> struct Fields : FieldsBase {
   int upd_comps(int ft, int cmp, const int *src, int n, UpdComp<R> *uc, bool have_fmp) {
     int active = 0;
     for (int q = 0; q < n; ++q) {
-      const int dc = src[q], ec = dc - 10;
       UpdComp<R> &e = uc[q];
       memset(&e, 0, sizeof e);
+      e.dsigw = -1;
+      if (src[q] < 0) continue;
+      const int dc = src[q], ec = dc - 10;
       for (int d = 0; d < 3; ++d) {
         e.lo[d] = gv.own_lo(ec, d);
         e.hi[d] = gv.own_hi(ec, d);
@@ -786,6 +1001,7 @@ template <class R> struct Fields : FieldsBase {
   // whether update_eh(ft) would allocate something now (then the curl runs on its own first)
   bool upd_pending(int cmp, const int *src, int n, bool have_fmp) const {
     for (int q = 0; q < n; ++q) {
+      if (src[q] < 0) continue;
       const int dc = src[q], ec = dc - 10;
       if (!f[ec][cmp]) continue;
       const int dsigw = pml_in(comp_dir(ec)) ? comp_dir(ec) : -1;
@@ -815,6 +1031,8 @@ template <class R> struct Fields : FieldsBase {
   // sparse source values of field type ft2 (D or B) at this step
   void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, double time, SrcTab &tab, std::vector<const SrcVol *> &big) {
     tab.n = 0;
+    tab.lo = INT64_MAX;
+    tab.hi = -1;
     for (const SrcVol &sv : sources[ft2]) {
       const SrcTime &st = srcs[sv.src];
       if (st.is_integrated != integrated) continue;
@@ -835,6 +1053,8 @@ template <class R> struct Fields : FieldsBase {
         tab.comp[tab.n] = slot;
         tab.idx[tab.n] = sv.index[j];
         tab.val[tab.n] = cmp ? A.imag() : A.real();
+        tab.lo = std::min(tab.lo, sv.index[j]);
+        tab.hi = std::max(tab.hi, sv.index[j]);
         ++tab.n;
       }
     }
@@ -855,7 +1075,9 @@ template <class R> struct Fields : FieldsBase {
     double *dv = m.d_val + cmp * vals.size();
     HIPCHECK(hipMemcpyAsync(dv, vals.data(), sizeof(double) * vals.size(), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipStreamSynchronize(stream)); // vals lives on the host stack
+    kbegin(MP_K_SRC);
     launch_src<R>(fbase, m.d_idx, dv, int(vals.size()), stream);
+    kend();
   }
 
   // One fields::step() (src/step.cpp:35-139).
@@ -883,6 +1105,7 @@ template <class R> struct Fields : FieldsBase {
     }
     // --- B, then H = mu^-1 B where H is its own array ---
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
+      cur_cmp = cmp;
       int which[3];
       A.nc = curl_comps(B_ft, cmp, A.cc, which);
       std::vector<const SrcVol *> big;
@@ -891,14 +1114,14 @@ template <class R> struct Fields : FieldsBase {
       A.npole = 0;
       if (big.empty() && !upd_pending(cmp, which, A.nc, false)) {
         const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
-        launch_step<R>(A, true, nu > 0, stream);
+        step_kernel(A, true, nu > 0, MP_K_STEP_BH);
         continue;
       }
-      launch_step<R>(A, true, false, stream);
+      step_kernel(A, true, false, MP_K_STEP_BH);
       for (const SrcVol *sv : big) apply_big(*sv, f[dir_comp(15, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].current(tB, dt), dt);
       StepArgs<R> U = A;
       U.cur.n = 0;
-      if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) launch_step<R>(U, false, true, stream);
+      if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) step_kernel(U, false, true, MP_K_STEP_BH);
     }
     if (!plan_valid) rebuild_plan();
     boundaries(B_ft);
@@ -907,20 +1130,21 @@ template <class R> struct Fields : FieldsBase {
     alloc_pols();
     if (!plan_valid) rebuild_plan();
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
+      cur_cmp = cmp;
       int which[3], whiche[3];
       A.nc = curl_comps(D_ft, cmp, A.cc, which);
       std::vector<const SrcVol *> bigc, bigd;
       fill_tab(D_ft, cmp, which, A.nc, false, tH, A.cur, bigc);
       upd_comps(E_ft, cmp, which, A.nc, A.uc, have_fmp_E);
-      for (int q = 0; q < A.nc; ++q) whiche[q] = which[q] - 10;
+      for (int q = 0; q < A.nc; ++q) whiche[q] = which[q] < 0 ? -1 : which[q] - 10;
       fill_tab(D_ft, cmp, whiche, A.nc, true, tE, A.dip, bigd);
       A.npole = int(pols.size());
       for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
       if (bigc.empty() && bigd.empty()) {
-        launch_step<R>(A, true, true, stream);
+        step_kernel(A, true, true, MP_K_STEP_DE);
         continue;
       }
-      launch_step<R>(A, true, false, stream);
+      step_kernel(A, true, false, MP_K_STEP_DE);
       for (const SrcVol *sv : bigc) apply_big(*sv, f[dir_comp(10, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].current(tH, dt), dt);
       // long integrated lists: f_minus_p formed in place in D around the update, then D restored
       std::vector<std::pair<R *, int64_t>> saved;
@@ -936,7 +1160,7 @@ template <class R> struct Fields : FieldsBase {
       }
       StepArgs<R> U = A;
       U.cur.n = 0;
-      launch_step<R>(U, false, true, stream);
+      step_kernel(U, false, true, MP_K_STEP_DE);
       for (size_t q = 0; q < saved.size(); ++q)
         HIPCHECK(hipMemcpyAsync(saved[q].first + saved[q].second, static_cast<R *>(d_scratch.p) + q, sizeof(R), hipMemcpyDeviceToDevice, stream));
     }
@@ -958,6 +1182,13 @@ template <class R> struct Fields : FieldsBase {
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipGetLastError());
     timers[6] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (ktiming) {
+      resolve_events();
+      timers[0] = k_ms[MP_K_STEP_BH] * 1e-3;
+      timers[2] = k_ms[MP_K_STEP_DE] * 1e-3;
+      timers[4] = k_ms[MP_K_HALO] * 1e-3;
+      timers[5] = k_ms[MP_K_DFT] * 1e-3;
+    }
     int flag = 0;
     HIPCHECK(hipMemcpy(&flag, d_flag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) fail("simulation fields are NaN or Inf");
@@ -992,7 +1223,9 @@ template <class R> struct Fields : FieldsBase {
       da.acc = ch.acc.p;
       da.comp = ch.comp.p;
       da.acc_kind = ch.acc_kind;
+      kbegin(MP_K_DFT);
       launch_dft<R>(da, stream);
+      kend();
     }
   }
 
@@ -1407,12 +1640,12 @@ template <class R> struct Fields : FieldsBase {
     HIPCHECK(hipStreamSynchronize(stream));
     R *p = array_ptr(c, cmp, kind, ipol);
     if (!p) {
-      for (size_t i = 0; i < gv.ntot; ++i) out[i] = 0;
+      for (size_t i = 0; i < gv.rntot; ++i) out[i] = 0;
       return 1;
     }
     std::vector<R> v(gv.ntot);
     HIPCHECK(hipMemcpy(v.data(), p, sizeof(R) * gv.ntot, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < gv.ntot; ++i) out[i] = double(v[i]);
+    gv.for_each_point([&](int64_t r, int64_t d) { out[r] = double(v[d]); });
     return 0;
   }
   void set_array(int c, int cmp, int kind, int ipol, const double *in) override {
@@ -1420,7 +1653,8 @@ template <class R> struct Fields : FieldsBase {
     R *p = array_ptr(c, cmp, kind, ipol);
     if (!p) fail("array not allocated");
     std::vector<R> v(gv.ntot);
-    for (size_t i = 0; i < gv.ntot; ++i) v[i] = R(in[i]);
+    HIPCHECK(hipMemcpy(v.data(), p, sizeof(R) * gv.ntot, hipMemcpyDeviceToHost));
+    gv.for_each_point([&](int64_t r, int64_t d) { v[d] = R(in[r]); });
     HIPCHECK(hipMemcpy(p, v.data(), sizeof(R) * gv.ntot, hipMemcpyHostToDevice));
   }
   bool have_component(int c) const override { return c >= 0 && c < NC && f[c][0]; }
@@ -1681,6 +1915,15 @@ int mp_fields_have_component(mp_fields *f, int c) { return F_->have_component(c)
 int mp_fields_get_times(mp_fields *f, double *t7) {
   F_->times(t7);
   return 0;
+}
+int mp_fields_set_kernel_timing(mp_fields *f, int on) {
+  return guard([&] { F_->set_ktiming(on != 0); });
+}
+int mp_fields_get_kernel_timing(mp_fields *f, int which, double *ms, long *n) {
+  return guard([&] { F_->get_ktiming(which, ms, n); });
+}
+int mp_fields_reset_kernel_timing(mp_fields *f) {
+  return guard([&] { F_->reset_ktiming(); });
 }
 int mp_fields_set_dft_accumulator(mp_fields *f, int kind) {
   return guard([&] { F_->set_dft_acc(kind); });

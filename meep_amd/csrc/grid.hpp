@@ -41,8 +41,11 @@ struct Geom {
   int dim = 3;
   double a = 1, inva = 1;
   int num[3] = {0, 0, 0};
-  int64_t st[3] = {0, 0, 0};
-  size_t ntot = 1;
+  int64_t st[3] = {0, 0, 0}; // strides of this layout
+  size_t ntot = 1;           // points of this layout (padding included)
+  int64_t rst[3] = {0, 0, 0}; // strides of the reference layout
+  size_t rntot = 1;           // points of the reference layout
+  int fast = 2;               // the stride-1 direction
 
   bool has(int d) const { return dim == 1 ? d == 2 : (dim == 2 ? d != 2 : true); }
   int ydir(int k) const { return dim == 2 ? (k + 2) % 3 : k; }
@@ -60,7 +63,37 @@ struct Geom {
     st[0] = has(0) ? int64_t(num[2] + 1) * (num[1] + 1) : 0;
     st[1] = has(1) ? int64_t(num[2] + 1) : 0;
     st[2] = has(2) ? 1 : 0;
-    if (dim == 2) st[0] = num[1] + 1, st[1] = 1;
+    fast = dim == 2 ? 1 : 2;
+    for (int q = 0; q < 3; ++q) rst[q] = st[q];
+    rntot = ntot;
+  }
+  // Device layout: the stride-1 direction's rows padded to a multiple of `mult` points
+  // (the padding belongs to no component).  Index arithmetic is unchanged otherwise.
+  void pad_rows(int mult) {
+    const int64_t row = (int64_t(num[fast] + 1) + mult - 1) / mult * mult;
+    int64_t s = 1;
+    st[0] = st[1] = st[2] = 0;
+    st[fast] = 1;
+    s = row;
+    // remaining directions, fastest first (3D: Y then X; 2D: X)
+    for (int d = 2; d >= 0; --d)
+      if (has(d) && d != fast) {
+        st[d] = s;
+        s *= num[d] + 1;
+      }
+    ntot = size_t(s);
+  }
+  // reference-layout index of the point with array coordinates j
+  int64_t ref_of(const int j[3]) const { return j[0] * rst[0] + j[1] * rst[1] + j[2] * rst[2]; }
+  int64_t dev_of(const int j[3]) const { return j[0] * st[0] + j[1] * st[1] + j[2] * st[2]; }
+  // visit every point as (reference index, device index)
+  template <class F> void for_each_point(F fn) const {
+    int hi[3];
+    for (int d = 0; d < 3; ++d) hi[d] = has(d) ? num[d] : 0;
+    int j[3];
+    for (j[0] = 0; j[0] <= hi[0]; ++j[0])
+      for (j[1] = 0; j[1] <= hi[1]; ++j[1])
+        for (j[2] = 0; j[2] <= hi[2]; ++j[2]) fn(ref_of(j), dev_of(j));
   }
   int ysh(int c, int d) const {
     if (!has(d)) return 0;

@@ -1763,6 +1763,14 @@ int mp_fields_get_times(mp_fields *f, double *t7) {
     return 0;
   });
 }
+// the oracle has no kernels to time
+int mp_fields_set_kernel_timing(mp_fields *, int) { return 0; }
+int mp_fields_get_kernel_timing(mp_fields *, int, double *ms, long *n) {
+  *ms = 0;
+  *n = 0;
+  return 0;
+}
+int mp_fields_reset_kernel_timing(mp_fields *) { return 0; }
 int mp_fields_set_dft_accumulator(mp_fields *f, int kind) {
   return guard([&] {
     if (kind != MP_DFT_REALNUM && kind != MP_DFT_F64) fail("oracle supports realnum and FP64 DFT accumulators");
