@@ -92,14 +92,22 @@ template <class R> struct StepArgs {
   int hasp[3], hasm[3];
   int sgn;
   R dtdx; // Courant number (the vectorised kernel applies the partner signs itself)
+  // Periodic wrap along the stride-1 direction folded into the kernels (real Bloch phase
+  // zsign = +-1): a component with no Yee shift along it copies its owned point n into the
+  // halo point 0, one with a shift copies point 0 into the halo point n (f, E/H and P).
+  int zwrap;
+  int zsign;
+  int rev; // visit logical blocks in descending order (backward differences)
 };
 
 // Halo connection (connect_the_chunks / process_incoming_chunk_data,
-// src/boundaries.cpp:368-600, src/step.cpp:172-223).  slot indexes a table of array pairs.
+// src/boundaries.cpp:368-600, src/step.cpp:172-223): non-owned point dst of array slot
+// `slot` takes the owned point src, copied (cls 2), negated (cls 1) or multiplied by the
+// Bloch phase ph[entry] (cls 0).  12 bytes per entry; phases live in a side array.
 struct Conn {
-  int64_t dst, src;
-  int32_t slot, cls; // cls: 0 phase, 1 negate, 2 copy
-  double ph_re, ph_im;
+  uint32_t dst, src;
+  uint16_t slot;
+  uint16_t cls;
 };
 
 // DFT accumulation (dft_chunk::update_dft, src/dft.cpp:266-308) over npts points.
@@ -123,7 +131,7 @@ template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, h
 // vectorised kernel over a box in which no uPML profile is active (16-byte accesses)
 template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
 template <class R>
-void launch_conn(const Conn *conns, int n, R *const *slot_ptrs, int ncmp, hipStream_t s);
+void launch_conn(const Conn *conns, const double2 *phases, int n, R *const *slot_ptrs, int ncmp, hipStream_t s);
 template <class R> void launch_src(R *f, const int64_t *idx, const double *val, int n, hipStream_t s);
 template <class R> void launch_dft(const DftArgs<R> &a, hipStream_t s);
 template <class R>

@@ -335,8 +335,10 @@ template <class R> struct Fields : FieldsBase {
 
   // halo tables, rebuilt when allocations or boundary conditions change
   bool plan_valid = false;
-  DevBuf d_conn[8];
-  int n_conn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // two merged halo lists: 0 = B, H and their polarisations (after the B phase),
+  // 1 = D, E and P_E (after the D phase)
+  DevBuf d_conn[2], d_phase[2];
+  int n_conn[2] = {0, 0};
   DevBuf d_slots;
   std::vector<R *> h_slots;
   // finiteness probe at the cell centre (src/step.cpp:137-138)
@@ -619,6 +621,10 @@ template <class R> struct Fields : FieldsBase {
       }
     return gv.owns(*there);
   }
+  // Periodic wrap along the stride-1 direction is folded into the update kernels when its
+  // Bloch phase is real (+1 or -1); those halo points then leave the connection lists.
+  bool zwrap_ok = false;
+  int zsign = 1;
   // slots: 0..19 the field arrays, 20 + 20*p + c the polarisation P of pole p
   void rebuild_plan() {
     HIPCHECK(hipStreamSynchronize(stream));
@@ -632,13 +638,23 @@ template <class R> struct Fields : FieldsBase {
     d_slots.alloc(sizeof(R *) * h_slots.size());
     HIPCHECK(hipMemcpy(d_slots.p, h_slots.data(), sizeof(R *) * h_slots.size(), hipMemcpyHostToDevice));
 
-    std::vector<Conn> lists[8];
+    std::vector<Conn> lists[2];
+    std::vector<double2> phs[2];
     bool B_redundant[5];
     for (int q = 0; q < 5; ++q) B_redundant[q] = f[5 + q][0] == f[15 + q][0];
     auto phase_class = [](cdouble ph) {
       auto close = [](cdouble p, double r) { return std::fabs(p.imag()) < 1e-13 && std::fabs(p.real() - r) < 1e-13; };
       return close(ph, 1.0) ? 2 : close(ph, -1.0) ? 1 : 0;
     };
+    {
+      const int d = gv.fast;
+      const cdouble e = eikna[d];
+      zwrap_ok = false;
+      if (gv.has(d) && bc[High][d] == Periodic && std::fabs(e.imag()) < 1e-13) {
+        if (std::fabs(e.real() - 1.0) < 1e-13) zwrap_ok = true, zsign = 1;
+        if (std::fabs(e.real() + 1.0) < 1e-13) zwrap_ok = true, zsign = -1;
+      }
+    }
     for (int corig = 0; corig < NC; ++corig) {
       if (!f[corig][0]) continue;
       if (is_B(corig) && B_redundant[corig - 15]) continue;
@@ -658,7 +674,9 @@ template <class R> struct Fields : FieldsBase {
           bhi[q] = hi[q];
         }
         blo[d] = bhi[d] = plane;
-        for (int j0 = blo[0]; j0 <= bhi[0]; ++j0)
+        // the last box (along the stride-1 direction) is written by the kernels themselves
+        const bool folded = zwrap_ok && d == gv.fast;
+        for (int j0 = blo[0]; j0 <= (folded ? blo[0] - 1 : bhi[0]); ++j0)
           for (int j1 = blo[1]; j1 <= bhi[1]; ++j1)
             for (int j2 = blo[2]; j2 <= bhi[2]; ++j2) {
               int j[3] = {j0, j1, j2};
@@ -667,19 +685,22 @@ template <class R> struct Fields : FieldsBase {
               cdouble ph = 1.0;
               if (!locate(&here, &ph) || on_metal_boundary(here)) continue;
               Conn cn;
-              cn.dst = n;
-              cn.src = gv.index(corig, here);
-              cn.slot = corig;
-              cn.cls = phase_class(ph);
-              cn.ph_re = ph.real();
-              cn.ph_im = ph.imag();
-              lists[field_type(corig)].push_back(cn);
+              cn.dst = uint32_t(n);
+              cn.src = uint32_t(gv.index(corig, here));
+              cn.slot = uint16_t(corig);
+              cn.cls = uint16_t(phase_class(ph));
+              const double2 phv = make_double2(ph.real(), ph.imag());
+              const int ft = field_type(corig);
+              const int grp = (ft == 1 || ft == 3) ? 0 : 1;
+              lists[grp].push_back(cn);
+              phs[grp].push_back(phv);
               if (is_electric(corig) || is_magnetic(corig))
                 for (size_t p = 0; p < pols.size(); ++p)
                   if (pols[p].allocated && pols[p].P[corig][0]) {
                     Conn cp = cn;
-                    cp.slot = NC + NC * int(p) + corig;
-                    lists[is_electric(corig) ? 4 : 5].push_back(cp);
+                    cp.slot = uint16_t(NC + NC * int(p) + corig);
+                    lists[grp].push_back(cp);
+                    phs[grp].push_back(phv);
                   }
             }
         // later boxes exclude this plane (the boxes are disjoint)
@@ -689,15 +710,21 @@ template <class R> struct Fields : FieldsBase {
           hi[d] = plane - 1;
       }
     }
-    for (int ft = 0; ft < 8; ++ft)
-      for (const Conn &cn : lists[ft])
-        if (cn.dst < 0 || cn.src < 0 || size_t(cn.dst) >= gv.ntot || size_t(cn.src) >= gv.ntot || cn.slot < 0 || cn.slot >= nslot)
+    for (int q = 0; q < 2; ++q)
+      for (const Conn &cn : lists[q]) {
+        if (size_t(cn.dst) >= gv.ntot || size_t(cn.src) >= gv.ntot || cn.slot >= nslot)
           fail("internal: halo connection outside the array");
-    for (int ft = 0; ft < 8; ++ft) {
-      n_conn[ft] = int(lists[ft].size());
-      d_conn[ft].alloc(sizeof(Conn) * lists[ft].size());
-      if (n_conn[ft])
-        HIPCHECK(hipMemcpy(d_conn[ft].p, lists[ft].data(), sizeof(Conn) * lists[ft].size(), hipMemcpyHostToDevice));
+        if (cn.cls == 0 && (!h_slots[2 * cn.slot + 1] || ncmp() < 2)) fail("internal: Bloch phase on real fields");
+        if (!h_slots[2 * cn.slot]) fail("internal: halo connection to an unallocated array");
+      }
+    for (int q = 0; q < 2; ++q) {
+      n_conn[q] = int(lists[q].size());
+      d_conn[q].alloc(sizeof(Conn) * lists[q].size());
+      bool any_phase = false;
+      for (const Conn &cn : lists[q]) any_phase = any_phase || cn.cls == 0;
+      d_phase[q].alloc(any_phase ? sizeof(double2) * phs[q].size() : 0);
+      if (n_conn[q]) HIPCHECK(hipMemcpy(d_conn[q].p, lists[q].data(), sizeof(Conn) * lists[q].size(), hipMemcpyHostToDevice));
+      if (any_phase) HIPCHECK(hipMemcpy(d_phase[q].p, phs[q].data(), sizeof(double2) * phs[q].size(), hipMemcpyHostToDevice));
     }
     // centre probe: E and D components at the interpolation points of the centre
     std::vector<const R *> cp;
@@ -736,10 +763,12 @@ template <class R> struct Fields : FieldsBase {
     }
     plan_valid = true;
   }
-  void boundaries(int ft) {
-    if (!n_conn[ft]) return;
+  // step_boundaries for a merged group of field types (see d_conn)
+  void boundaries(int grp) {
+    if (!n_conn[grp]) return;
     kbegin(MP_K_HALO);
-    launch_conn<R>(static_cast<const Conn *>(d_conn[ft].p), n_conn[ft], static_cast<R *const *>(d_slots.p), ncmp(), stream);
+    launch_conn<R>(static_cast<const Conn *>(d_conn[grp].p), static_cast<const double2 *>(d_phase[grp].p), n_conn[grp],
+                   static_cast<R *const *>(d_slots.p), ncmp(), stream);
     kend();
   }
   int cur_cmp = 0; // real (0) or imaginary (1) part being stepped
@@ -777,6 +806,9 @@ template <class R> struct Fields : FieldsBase {
         b.g.bhi[d] = interior.hi[d];
       }
       if (use_fast && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b))) {
+        // backward differences read the previous plane: walk the planes downwards so it was
+        // fetched just before (forward differences prefetch their neighbour naturally)
+        b.rev = kind == MP_K_STEP_DE ? 1 : 0;
         check_args(b, curl, upd, true);
         launch_fast<R>(b, curl, upd, stream);
       }
@@ -1080,8 +1112,8 @@ template <class R> struct Fields : FieldsBase {
     kend();
   }
 
-  // One fields::step() (src/step.cpp:35-139).
-  void one_step() {
+  // One fields::step() (src/step.cpp:35-139); the finiteness probe is launched when probe.
+  void one_step(bool probe) {
     const double tB = t * dt, tH = t * dt + 0.5 * dt, tE = t * dt + dt;
     bool int_B = false, int_D = false;
     for (auto &sv : sources[B_ft])
@@ -1095,9 +1127,12 @@ template <class R> struct Fields : FieldsBase {
         for (int cmp = 0; cmp < ncmp(); ++cmp)
           if (needs_P(p, c, cmp)) have_fmp_E = true;
 
+    if (!plan_valid) rebuild_plan();
     StepArgs<R> A;
     memset(&A, 0, sizeof A);
     A.g = grid();
+    A.zwrap = zwrap_ok ? 1 : 0;
+    A.zsign = zsign;
     for (int d = 0; d < 3; ++d) {
       A.pml.sig[d] = d_sig[d];
       A.pml.kap[d] = d_kap[d];
@@ -1124,11 +1159,12 @@ template <class R> struct Fields : FieldsBase {
       if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) step_kernel(U, false, true, MP_K_STEP_BH);
     }
     if (!plan_valid) rebuild_plan();
-    boundaries(B_ft);
-    boundaries(H_ft);
+    boundaries(0); // B, H (step_boundaries B_stuff .. H_stuff)
     // --- D, then E = chi1inv (D - P - dipoles) and the polarisations ---
     alloc_pols();
     if (!plan_valid) rebuild_plan();
+    A.zwrap = zwrap_ok ? 1 : 0;
+    A.zsign = zsign;
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
       cur_cmp = cmp;
       int which[3], whiche[3];
@@ -1165,12 +1201,10 @@ template <class R> struct Fields : FieldsBase {
         HIPCHECK(hipMemcpyAsync(saved[q].first + saved[q].second, static_cast<R *>(d_scratch.p) + q, sizeof(R), hipMemcpyDeviceToDevice, stream));
     }
     if (!plan_valid) rebuild_plan();
-    boundaries(D_ft);
-    boundaries(4); // PE
-    boundaries(E_ft);
+    boundaries(1); // D, P_E, E (step_boundaries D_stuff .. E_stuff)
     t += 1;
     update_dfts();
-    if (n_chk)
+    if (probe && n_chk)
       launch_check<R>(static_cast<const R *const *>(d_chk_ptrs.p), static_cast<const int64_t *>(d_chk_idx.p),
                       static_cast<const double *>(d_chk_w.p), n_chk, static_cast<int *>(d_flag.p), stream);
   }
@@ -1178,7 +1212,8 @@ template <class R> struct Fields : FieldsBase {
   void step_n(long n, bool) override {
     if (!plan_valid) rebuild_plan();
     auto t0 = std::chrono::steady_clock::now();
-    for (long q = 0; q < n; ++q) one_step();
+    // the reference checks after every step (src/step.cpp:137-138); a batch probes its last step
+    for (long q = 0; q < n; ++q) one_step(q == n - 1);
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipGetLastError());
     timers[6] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

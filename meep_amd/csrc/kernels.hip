@@ -26,10 +26,19 @@ namespace mx {
 
 constexpr int BLOCK = 256;
 
-// logical block index, dealt to XCDs in contiguous runs (gridDim.x is a multiple of 8)
-__device__ __forceinline__ uint32_t xcd_block() {
+// logical block index, dealt to XCDs in contiguous runs (gridDim.x is a multiple of 8);
+// rev walks them in descending order
+__device__ __forceinline__ uint32_t xcd_block(int rev = 0) {
   const uint32_t nb = gridDim.x, b = blockIdx.x;
-  return (b & 7u) * (nb >> 3) + (b >> 3);
+  const uint32_t lb = (b & 7u) * (nb >> 3) + (b >> 3);
+  return rev ? nb - 1 - lb : lb;
+}
+
+// halo point along the stride-1 direction fed by the owned point z of a component with Yee
+// shift ysh there; -1 when z is not the wrapped end
+__device__ __forceinline__ int zwrap_dst(int ysh, int z, int n) {
+  if (ysh == 0) return z == n ? 0 : -1;
+  return z == 0 ? n : -1;
 }
 
 __device__ __forceinline__ int jsel(const int j[3], int d) { return d == 0 ? j[0] : d == 1 ? j[1] : j[2]; }
@@ -99,6 +108,11 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, int c, uint32_
     if (in_tab(a.cur, i)) fv = apply_tab(a.cur, c, i, fv);
     if (on_metal(a.g, j, k.ysh)) fv = R(0);
     k.f[i] = fv;
+    if (a.zwrap) {
+      const int d2 = a.g.axdir[2], z = jsel(j, d2);
+      const int zd = zwrap_dst(ysel(k.ysh, d2), z, a.g.num[d2]);
+      if (zd >= 0) k.f[i - z + zd] = a.zsign > 0 ? fv : -fv;
+    }
   }
   if (UPD && a.uc[c].f) {
     const UpdComp<R> &u = a.uc[c];
@@ -126,12 +140,23 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, int c, uint32_
       if (p < a.npole && u.P[p] && u.sigma[p]) {
         const PoleCoef<R> &q = a.pole[p];
         const R pcur = u.P[p][i];
-        u.P[p][i] = q.gamma1inv * (pcur * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * u.Pp[p][i] +
-                                   q.omega0dtsqr * (u.sigma[p][i] * wv));
+        const R pn = q.gamma1inv * (pcur * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * u.Pp[p][i] +
+                                    q.omega0dtsqr * (u.sigma[p][i] * wv));
+        u.P[p][i] = pn;
         u.Pp[p][i] = pcur;
+        if (a.zwrap) {
+          const int d2 = a.g.axdir[2], z = jsel(j, d2);
+          const int zd = zwrap_dst(ysel(u.ysh, d2), z, a.g.num[d2]);
+          if (zd >= 0) u.P[p][i - z + zd] = a.zsign > 0 ? pn : -pn;
+        }
       }
     if (on_metal(a.g, j, u.ysh)) ev = R(0);
     u.f[i] = ev;
+    if (a.zwrap) {
+      const int d2 = a.g.axdir[2], z = jsel(j, d2);
+      const int zd = zwrap_dst(ysel(u.ysh, d2), z, a.g.num[d2]);
+      if (zd >= 0) u.f[i - z + zd] = a.zsign > 0 ? ev : -ev;
+    }
   }
 }
 
@@ -217,6 +242,17 @@ __device__ __forceinline__ Vec<R, V> neighbour(const R *g, uint32_t i, int64_t s
   return vload<R, V>(g + (int64_t(i) + s));
 }
 
+// the periodic image of an owned end point of the row, written into the row's halo point
+template <class R, int V>
+__device__ __forceinline__ void wrap_store(R *f, uint32_t i, int z0, int ysh, int n, int sign, const Vec<R, V> &x, unsigned mask) {
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+    if ((mask >> k) & 1) {
+      const int zd = zwrap_dst(ysh, z0 + k, n);
+      if (zd >= 0) f[i - z0 + zd] = sign > 0 ? x.v[k] : -x.v[k];
+    }
+}
+
 // metal points of a component along the row of a group (the uniform directions d0, d1 give
 // all or nothing)
 __device__ __forceinline__ unsigned metal_mask(const Grid &g, const int j[3], const int ysh[3], int d0, int d1, int d2, int z0, int V) {
@@ -232,7 +268,7 @@ __device__ __forceinline__ unsigned metal_mask(const Grid &g, const int j[3], co
 
 template <class R, int V, bool CURL, bool UPD>
 __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t count, int g0, int ngroups) {
-  const uint32_t t = xcd_block() * uint32_t(BLOCK) + threadIdx.x;
+  const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
   const Grid &g = a.g;
   const int d0 = g.axdir[0], d1 = g.axdir[1], d2 = g.axdir[2];
@@ -320,6 +356,7 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
           if ((metal >> k) & 1) fv.v[k] = R(0);
       }
       vstore<R, V>(kc.f + i, fv, mask);
+      if (a.zwrap) wrap_store<R, V>(kc.f, i, z0, ysh[d2], g.num[d2], a.zsign, fv, mask);
     }
     if (UPD && a.uc[c].f) {
       const UpdComp<R> &u = a.uc[c];
@@ -352,6 +389,7 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
             Pn.v[k] = q.gamma1inv * (P.v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Pp.v[k] + q.omega0dtsqr * (sg.v[k] * ev.v[k]));
           vstore<R, V>(u.P[p] + i, Pn, mask);
           vstore<R, V>(u.Pp[p] + i, P, mask);
+          if (a.zwrap) wrap_store<R, V>(u.P[p], i, z0, ysh[d2], g.num[d2], a.zsign, Pn, mask);
         }
       if (metal) {
 #pragma unroll
@@ -359,6 +397,7 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
           if ((metal >> k) & 1) ev.v[k] = R(0);
       }
       vstore<R, V>(u.f + i, ev, mask);
+      if (a.zwrap) wrap_store<R, V>(u.f, i, z0, ysh[d2], g.num[d2], a.zsign, ev, mask);
     }
   }
 }
@@ -366,14 +405,15 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
 // ---------------------------------------------------------------------------------------
 
 template <class R>
-__global__ __launch_bounds__(BLOCK) void k_conn(const Conn *conns, int n, R *const *slots, int ncmp) {
+__global__ __launch_bounds__(BLOCK) void k_conn(const Conn *conns, const double2 *phases, int n, R *const *slots, int ncmp) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= n) return;
   const Conn e = conns[t];
   R *a0 = slots[2 * e.slot], *a1 = slots[2 * e.slot + 1];
   if (e.cls == 0) { // CONNECT_PHASE: complex<realnum> product
+    const double2 ph = phases[t];
     const R re = a0[e.src], im = a1[e.src];
-    const R pr = R(e.ph_re), pi = R(e.ph_im);
+    const R pr = R(ph.x), pi = R(ph.y);
     a0[e.dst] = pr * re - pi * im;
     a1[e.dst] = pr * im + pi * re;
   }
@@ -493,8 +533,9 @@ template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, h
     hipLaunchKernelGGL((k_fast<R, V, false, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
 }
 
-template <class R> void launch_conn(const Conn *conns, int n, R *const *slots, int ncmp, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_conn<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, conns, n, slots, ncmp);
+template <class R>
+void launch_conn(const Conn *conns, const double2 *phases, int n, R *const *slots, int ncmp, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_conn<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, conns, phases, n, slots, ncmp);
 }
 
 template <class R> void launch_src(R *f, const int64_t *idx, const double *val, int n, hipStream_t s) {
@@ -514,7 +555,7 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
 #define INSTANTIATE(R)                                                                             \
   template void launch_step<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_fast<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
-  template void launch_conn<R>(const Conn *, int, R *const *, int, hipStream_t);                   \
+  template void launch_conn<R>(const Conn *, const double2 *, int, R *const *, int, hipStream_t);  \
   template void launch_src<R>(R *, const int64_t *, const double *, int, hipStream_t);            \
   template void launch_dft<R>(const DftArgs<R> &, hipStream_t);                                    \
   template void launch_check<R>(const R *const *, const int64_t *, const double *, int, int *,     \

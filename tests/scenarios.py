@@ -102,12 +102,56 @@ def polariton_1d(lib, single=False):
     return s, f, []
 
 
+def periodic_3d_k0(lib, single=False, a=8):
+    """k = 0 Bloch box with real fields (the bandwidth workload's boundary conditions) and a
+    Lorentzian medium in part of it: the periodic images along the stride-1 axis, including
+    those of P, are written by the update kernels."""
+    L = (1.5, 1.25, 2.0)
+    s = structure(lib, dim=3, size=L, a=a, eps=blocks_eps(11, 3, 1.0, 12.0, L), single=single)
+    s.add_susceptibility(lambda x, y, z: 2.0 if x < 0.75 else 0.0, 0.8, 0.1)
+    f = fields(s)
+    f.use_bloch((0.0, 0.0, 0.0))
+    f.use_real_fields()
+    f.add_volume_source(Ex, gaussian_src_time(0.6, 0.3), (0.7, 0.6, 1.0), (0.7, 0.6, 1.0))
+    f.add_volume_source(Ez, gaussian_src_time(0.5, 0.3), (0.2, 0.2, 0.0), (0.2, 0.2, 0.0))
+    return s, f, []
+
+
+def zone_edge_2d(lib, single=False, a=10):
+    """2D TM with the Bloch vector on the zone edge along y (phase -1, CONNECT_NEGATE) and a
+    general phase along x: complex fields."""
+    s = structure(lib, dim=2, size=(1.5, 2.0), a=a, eps=blocks_eps(13, 4, 1.0, 6.0, (1.5, 2.0, 1.0)),
+                  single=single)
+    f = fields(s)
+    f.add_point_source(Ez, 0.5, 3.0, 0.0, 2.0, (0.6, 1.0, 0), complex(0, 1.0))
+    # zone edge along y: real(k) * n_y == a / 2 with n_y = 20 cells (src/boundaries.cpp:98-99)
+    f.use_bloch((0.2, 0.5 * a / 20))
+    return s, f, []
+
+
+def xpml_periodic_3d(lib, single=False, a=8):
+    """x-PML with periodic y and z (k = 0, real fields): the slabs run the generic kernel,
+    which writes the periodic images along the stride-1 axis as well."""
+    L = (3.0, 1.0, 1.5)
+    s = structure(lib, dim=3, size=L, a=a, eps=blocks_eps(17, 3, 1.0, 4.0, L), pml=pml(0.75, X),
+                  single=single)
+    f = fields(s)
+    f.use_bloch((0.0, 0.0, 0.0))
+    f.use_real_fields()
+    f.add_volume_source(Ey, gaussian_src_time(0.7, 0.3), (1.5, 0.5, 0.7), (1.5, 0.5, 0.7))
+    fl = f.add_dft_flux(X, (2.1, 0.0, 0.0), (2.1, 1.0, 1.5), linspace(0.5, 0.9, 6))
+    return s, f, [fl]
+
+
 SCENARIOS = {
     "waveguide_2d": (waveguide_2d, 1600),
     "box_3d_pml_lorentz": (box_3d_pml_lorentz, 120),
     "bloch_3d": (bloch_3d, 80),
     "metal_magnetic_2d_te": (metal_magnetic_2d_te, 200),
     "polariton_1d": (polariton_1d, 200),
+    "periodic_3d_k0": (periodic_3d_k0, 150),
+    "zone_edge_2d": (zone_edge_2d, 200),
+    "xpml_periodic_3d": (xpml_periodic_3d, 150),
 }
 
 FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)

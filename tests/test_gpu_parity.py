@@ -34,12 +34,10 @@ def _run(lib, builder, nsteps, single):
 @pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
 @pytest.mark.parametrize("name", sorted(SCENARIOS))
 def test_state_parity(gpu_lib, oracle_lib, name, single, kernel, monkeypatch):
-    """kernel='generic' forces the one-thread-per-element kernel where the 3D x-marching
-    kernel would otherwise run, so both kernels are held to the oracle."""
+    """kernel='generic' forces the one-point-per-thread kernel on the interior box too, where
+    the vectorised kernel would otherwise run, so both kernels are held to the oracle."""
     builder, nsteps = SCENARIOS[name]
     if kernel == "generic":
-        if "3d" not in name:
-            pytest.skip("the generic kernel is the only one for 1D/2D")
         monkeypatch.setenv("MEEP_AMD_GENERIC_KERNEL", "1")
     print(f"[parity] {name} single={single} steps={nsteps} kernel={kernel}", flush=True)
     sg, fg, flg = _run(gpu_lib, builder, nsteps, single)
