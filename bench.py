@@ -9,11 +9,14 @@ fields::step() over the whole box.  Inputs are resident in HBM before the timed 
 
 Roofline: the dominant kernel's launches are timed with HIP events on the engine's stream
 during the timed region; its algorithmic bytes per launch are the reference-state words it
-owns per cell times the cells, so `achieved` is GB/s of algorithmic traffic.
-`traffic` is read from the committed rocprofv3 counter summary for this workload when present.
+moves per cell (D phase: D read + write, chi1inv read, E write = 12 words; B phase: B read +
+write, E read = 9 words) times the cells, so `achieved` is GB/s of algorithmic traffic.
+`traffic` (HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE counters, scaled by a
+device-copy calibration) is read from profiles/traffic_step_de_box256_<precision>.json.
 
-CPU baseline: the oracle (a single-threaded C++ restatement of the reference's loops) timed
-on this host on a bounded sample of the same workload: the first steps of the same 256^3 box.
+CPU baseline: the oracle (a C++ restatement of the reference's loops, OpenMP over the outer
+two loop counters as the reference's collapse(2) loops) timed on this host on a bounded
+sample of the same workload: the first steps of the same cell.
 """
 
 import argparse
@@ -40,7 +43,8 @@ METRIC = "Gcell-updates/s + % HBM roofline; adjoint-gradient time / forward time
 # algorithmic reference-state words per cell owned by each kernel (E, D, B=H, chi1inv: 21/step)
 WORDS = {capi.K_STEP_BH: 9,   # B read + write, E read
          capi.K_STEP_DE: 12}  # D read + write, chi1inv read, E write
-KERNEL_NAME = {capi.K_STEP_BH: "k_step<curl B, H>", capi.K_STEP_DE: "k_step<curl D, E update>"}
+KERNEL_NAME = {capi.K_STEP_BH: "B phase: k_fast<R, V, curl, no update>",
+               capi.K_STEP_DE: "D phase: k_fast<R, V, curl, E update>"}
 
 
 def eps_blocks(seed=2, p12=0.3, n=N, block=BLOCK):
@@ -72,6 +76,99 @@ def build(lib, single, eps_b):
     return s, f
 
 
+# ---- the 3D PML + Lorentz-Drude + DFT workload -----------------------------------------
+C3_CELLS = (384, 384, 192)
+C3_PML = 16          # cells of uPML on every face
+C3_A = 10.0          # resolution
+C3_NFREQ = 32
+
+
+def c3_materials(n=C3_CELLS, seed=3, nspheres=150):
+    """eps (per E component, reference layout) and the dispersive mask for the workload:
+    lower half (z < n_z/2 cells) eps_inf = 2 with the two poles; upper half vacuum with seeded
+    dielectric spheres (radius 4-12 cells, eps 2-6), sampled at each component's Yee point."""
+    nx, ny, nz = n
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform((0, 0, nz / 2), (nx, ny, nz), size=(nspheres, 3))
+    rad = rng.uniform(4, 12, size=nspheres)
+    val = rng.uniform(2, 6, size=nspheres)
+    eps, lower = {}, {}
+    for c, sh in ((capi.Ex, (0.5, 0, 0)), (capi.Ey, (0, 0.5, 0)), (capi.Ez, (0, 0, 0.5))):
+        x = np.arange(nx + 1) + sh[0]
+        y = np.arange(ny + 1) + sh[1]
+        z = np.arange(nz + 1) + sh[2]
+        e = np.ones((nx + 1, ny + 1, nz + 1))
+        for (cx, cy, cz), r, v in zip(cen, rad, val):
+            ix = np.nonzero(np.abs(x - cx) <= r)[0]
+            iy = np.nonzero(np.abs(y - cy) <= r)[0]
+            iz = np.nonzero(np.abs(z - cz) <= r)[0]
+            if not (ix.size and iy.size and iz.size):
+                continue
+            d2 = ((x[ix, None, None] - cx) ** 2 + (y[None, iy, None] - cy) ** 2 +
+                  (z[None, None, iz] - cz) ** 2)
+            sub = e[ix[0]:ix[-1] + 1, iy[0]:iy[-1] + 1, iz[0]:iz[-1] + 1]
+            sub[d2 <= r * r] = v
+        low = (z < nz / 2)[None, None, :]
+        e = np.where(low, 2.0, e)
+        eps[c] = e
+        lower[c] = np.broadcast_to(low, e.shape).astype(np.float64)
+    return eps, lower
+
+
+def build_c3(lib, single, n=C3_CELLS, npml=C3_PML, a=C3_A, nfreq=C3_NFREQ, dft_kind=None,
+             fcen=0.7, df=0.6):
+    """3D cell of n cells with npml cells of uPML on every face; lower half a 2-pole
+    Lorentz-Drude medium (Lorentz f0 = 1.0, gamma 0.05, sigma 1.5; Drude f = 1.0, gamma 0.1,
+    sigma 3, i.e. lorentzian_susceptibility(1.0, 0.1, no_omega_0_denominator = true)) over
+    eps_inf = 2, seeded spheres above; a z-plane of Ex current at 5/6 of the height (Gaussian
+    fcen, df, uniform amplitude); DFT of Ex, Ey, Ez on the z-plane just above the interface at
+    nfreq frequencies across the pulse, decimation 1."""
+    L = [c / a for c in n]
+    s = capi.structure(lib, dim=3, size=L, a=a, eps=None, pml=capi.pml(npml / a), single=single)
+    eps, lower = c3_materials(n)
+    for c in (capi.Ex, capi.Ey, capi.Ez):
+        s.set_chi1inv(c, 1.0 / eps[c])
+    del eps
+    s.add_susceptibility(None, 1.0, 0.05)
+    s.add_susceptibility(None, 1.0, 0.1, drude=True)
+    for c in (capi.Ex, capi.Ey, capi.Ez):
+        s.set_sigma(0, c, 1.5 * lower[c])
+        s.set_sigma(1, c, 3.0 * lower[c])
+    del lower
+    f = capi.fields(s)
+    f.use_real_fields()
+    if dft_kind is not None:
+        f.set_dft_accumulator(dft_kind)
+    zsrc = round(n[2] * 5 / 6) / a
+    f.add_volume_source(capi.Ex, capi.gaussian_src_time(fcen, df), (0, 0, zsrc), (L[0], L[1], zsrc))
+    zdft = (n[2] // 2 + 4) / a
+    freqs = capi.linspace(fcen - df / 2, fcen + df / 2, nfreq)
+    dft = f.add_dft_fields([capi.Ex, capi.Ey, capi.Ez], (0, 0, zdft), (L[0], L[1], zdft), freqs,
+                           decimation_factor=1)
+    return s, f, dft
+
+
+def c3_algorithmic_bytes(n=C3_CELLS, npml=C3_PML, word=4, nfreq=C3_NFREQ, acc_bytes=16):
+    """Reference-state bytes per step: 21 words per cell, +10 per PML face cell, +20 per edge
+    cell, +30 per corner cell, +15 per pole over dispersive cells, plus each DFT accumulator
+    read and written once."""
+    cnt = np.zeros(4)
+    regions = []
+    for d in range(3):
+        m = np.zeros(n[d], dtype=int)
+        m[:npml] = 1
+        m[n[d] - npml:] = 1
+        regions.append(m)
+    k = regions[0][:, None, None] + regions[1][None, :, None] + regions[2][None, None, :]
+    for q in range(4):
+        cnt[q] = np.count_nonzero(k == q)
+    cells = np.prod(n)
+    disp = n[0] * n[1] * (n[2] // 2)
+    words = 21 * cells + 10 * cnt[1] + 20 * cnt[2] + 30 * cnt[3] + 2 * 15 * disp
+    dft = 3 * (n[0] + 1) * (n[1] + 1) * nfreq * 2 * acc_bytes
+    return words * word + dft
+
+
 def load_traffic(precision):
     path = os.path.join(ROOT, "profiles", f"traffic_step_de_box256_{precision}.json")
     if not os.path.exists(path):
@@ -80,25 +177,143 @@ def load_traffic(precision):
         return json.load(fh).get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(single, eps_b, seconds=15.0):
+def oracle_lib():
     import subprocess
     opath = os.path.join(ROOT, "oracle", "_build", "libmeep_oracle.so")
     if not os.path.exists(opath):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
-    lib = capi.bind(ctypes.CDLL(opath))
-    s, f = build(lib, single, eps_b)
-    f.step()  # first step builds the halo tables
+    return capi.bind(ctypes.CDLL(opath))
+
+
+def cpu_baseline(make, cells, what, seconds=15.0, max_steps=20):
+    """The oracle (its per-point loops over OpenMP threads) on the first steps of the same
+    workload on this host."""
+    f = make(oracle_lib())
+    f.step()  # the first step builds the halo tables and allocates lazily
     steps, t0 = 0, time.perf_counter()
     while True:
         f.step()
         steps += 1
         el = time.perf_counter() - t0
-        if el > seconds or steps >= 20:
+        if el > seconds or steps >= max_steps:
             break
-    rate = N ** 3 * steps / el / 1e9
-    return {"value": rate, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (1 thread) on the same 256^3 box, {steps} timed steps after 1 "
-                      f"untimed, {el:.1f} s, host {platform.processor() or platform.machine()}"}
+    rate = cells * steps / el / 1e9
+    threads = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()
+    return {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle ({threads} OpenMP threads) on {what}, {steps} timed steps after "
+                      f"1 untimed, {el:.1f} s, host {platform.processor() or platform.machine()} "
+                      f"with {os.cpu_count()} logical CPUs"}
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.synchronize
+    except Exception:  # torch is only used for the device barrier
+        pass
+    return lambda: None
+
+
+def timed_run(f, steps, warmup):
+    f.step_n(warmup)
+    sync = device_sync()
+    f.set_kernel_timing(True)
+    f.reset_kernel_timing()
+    sync()
+    t0 = time.perf_counter()
+    f.step_n(steps)
+    sync()
+    el = time.perf_counter() - t0
+    stats = {k: f.kernel_timing(k) for k in (capi.K_STEP_BH, capi.K_STEP_DE, capi.K_HALO,
+                                              capi.K_DFT, capi.K_SRC)}
+    return el, stats
+
+
+def line_base(args, value, el, config):
+    return {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": config,
+    }
+
+
+def run_box256(args, lib):
+    single = args.precision == "f32"
+    eps_b = eps_blocks()
+    s, f = build(lib, single, eps_b)
+    el, stats = timed_run(f, args.steps, args.warmup)
+    cells = N ** 3
+    value = cells * args.steps / el / 1e9
+    dom = max((capi.K_STEP_BH, capi.K_STEP_DE), key=lambda k: stats[k][0])
+    ms, nl = stats[dom]
+    avg_s = ms / max(nl, 1) / 1e3
+    wbytes = 4 if single else 8
+    alg_bytes = WORDS[dom] * wbytes * cells
+    achieved = alg_bytes / avg_s / 1e9
+    step_alg = 21 * wbytes * cells
+    print(f"[bench] {args.steps} steps in {el:.3f} s; per-kernel ms: "
+          + ", ".join(f"{k}:{v[0]:.1f}/{v[1]}" for k, v in stats.items())
+          + f"; whole-step algorithmic rate {step_alg * args.steps / el / 1e9:.0f} GB/s",
+          file=sys.stderr)
+    line = line_base(args, value, el, {
+        "workload": "3D Bloch-periodic box 256^3 cells, eps 8^3 blocks {1,12} p=0.3 seed 2, "
+                    "Ex Gaussian dipole at centre, no PML, real fields",
+        "cells": cells, "resolution": A, "courant": 0.5, "parallelism": "none"})
+    line["roofline"] = {"bound": "hbm", "kernel": KERNEL_NAME[dom],
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": load_traffic(args.precision),
+                        "alg_bytes_per_launch": alg_bytes,
+                        "avg_launch_us": round(avg_s * 1e6, 2)}
+    line["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(lambda L: build(L, single, eps_b)[1], cells,
+                                            "the same 256^3 box")
+    return line
+
+
+def run_c3(args, lib):
+    """The PML + Lorentz-Drude + DFT workload; the roofline is the whole step's algorithmic
+    bytes (every launch of the step, DFT included) over the step time."""
+    single = args.precision == "f32"
+    kind = capi.DFT_KAHAN32 if single else capi.DFT_F64
+    s, f, dft = build_c3(lib, single, dft_kind=kind)
+    el, stats = timed_run(f, args.steps, args.warmup)
+    cells = int(np.prod(C3_CELLS))
+    value = cells * args.steps / el / 1e9
+    wbytes = 4 if single else 8
+    alg = c3_algorithmic_bytes(word=wbytes)
+    achieved = alg * args.steps / el / 1e9
+    print(f"[bench] c3 {args.steps} steps in {el:.3f} s; per-kernel ms: "
+          + ", ".join(f"{k}:{v[0]:.1f}/{v[1]}" for k, v in stats.items()), file=sys.stderr)
+    line = line_base(args, value, el, {
+        "workload": "3D 384x384x192 cells, 16-cell uPML on all faces, lower half eps_inf 2 + "
+                    "Lorentz (1.0, 0.05, sigma 1.5) + Drude (1.0, 0.1, sigma 3), 150 seeded "
+                    "spheres above, Ex plane current, DFT Ex/Ey/Ez on a 384x384 plane at 32 "
+                    "frequencies (" + ("Kahan FP32" if single else "FP64") + " accumulators)",
+        "cells": cells, "resolution": C3_A, "courant": 0.5, "parallelism": "none"})
+    line["roofline"] = {"bound": "hbm", "kernel": "whole step (all launches)",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "alg_bytes_per_step": alg,
+                        "phase_ms": {str(k): round(v[0] / args.steps, 4) for k, v in stats.items()}}
+    line["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(lambda L: build_c3(L, single)[1], cells,
+                                            "the same 384x384x192 cell (realnum DFT)",
+                                            seconds=20.0, max_steps=8)
+    return line
 
 
 def main():
@@ -107,6 +322,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--workload", choices=["box256", "pml_lorentz_dft"], default="box256",
+                    help="box256: 3D Bloch-periodic 256^3 box (the headline); "
+                         "pml_lorentz_dft: 384x384x192 with PML, two poles and a DFT plane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy 5 times (the reference "
@@ -123,67 +341,12 @@ def main():
         print(json.dumps({"copy_bytes_per_launch": n}))
         return
     if args.gpus != 1:
-        print("bench.py: this engine runs one chunk on one GPU; multi-GPU runs are not part "
-              "of this build (see DESIGN.md for how the step would shard).", file=sys.stderr)
+        print("bench.py: this engine runs one chunk on one GPU; runs on more than one GPU are "
+              "not supported (the step would shard as X slabs with one halo-plane exchange "
+              "per half-step).", file=sys.stderr)
         sys.exit(2)
-    single = args.precision == "f32"
     lib = meep_amd.load_library()
-    eps_b = eps_blocks()
-    s, f = build(lib, single, eps_b)
-    f.step_n(args.warmup)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is only used for the device barrier
-        sync = lambda: None  # noqa: E731
-    f.set_kernel_timing(True)
-    f.reset_kernel_timing()
-    sync()
-    t0 = time.perf_counter()
-    f.step_n(args.steps)
-    sync()
-    el = time.perf_counter() - t0
-    cells = N ** 3
-    value = cells * args.steps / el / 1e9
-    stats = {k: f.kernel_timing(k) for k in (capi.K_STEP_BH, capi.K_STEP_DE, capi.K_HALO,
-                                              capi.K_DFT, capi.K_SRC)}
-    dom = max((capi.K_STEP_BH, capi.K_STEP_DE), key=lambda k: stats[k][0])
-    ms, nl = stats[dom]
-    avg_s = ms / max(nl, 1) / 1e3
-    wbytes = 4 if single else 8
-    alg_bytes = WORDS[dom] * wbytes * cells
-    achieved = alg_bytes / avg_s / 1e9
-    step_alg = 21 * wbytes * cells
-    print(f"[bench] {args.steps} steps in {el:.3f} s; per-kernel ms: "
-          + ", ".join(f"{k}:{v[0]:.1f}/{v[1]}" for k, v in stats.items())
-          + f"; whole-step algorithmic rate {step_alg * args.steps / el / 1e9:.0f} GB/s",
-          file=sys.stderr)
-    line = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "Gcell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(el / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": args.precision,
-        "data": "synthetic",
-        "config": {"workload": "3D Bloch-periodic box 256^3 cells, eps 8^3 blocks {1,12} "
-                               "p=0.3 seed 2, Ex Gaussian dipole at centre, no PML, real fields",
-                   "cells": cells, "resolution": A, "courant": 0.5, "parallelism": "none"},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAME[dom],
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.precision),
-                     "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_us": round(avg_s * 1e6, 2)},
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(single, eps_b)
+    line = run_box256(args, lib) if args.workload == "box256" else run_c3(args, lib)
     print(json.dumps(line))
 
 

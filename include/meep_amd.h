@@ -47,6 +47,7 @@ enum { MP_DFT_REALNUM = 0, MP_DFT_F64 = 1, MP_DFT_KAHAN32 = 2 };
 typedef struct mp_structure mp_structure;
 typedef struct mp_fields mp_fields;
 typedef struct mp_dft_flux mp_dft_flux;
+typedef struct mp_dft_fields mp_dft_fields;
 
 /* r points at (x, y, z) in the reference's coordinates (origin at the cell's low corner). */
 typedef double (*mp_scalar_func)(const double *r, void *user);
@@ -142,6 +143,10 @@ long mp_fields_t(const mp_fields *f);
 double mp_fields_dt(const mp_fields *f);
 /* fields::last_source_time — src/meep.hpp:1887 */
 double mp_fields_last_source_time(mp_fields *f);
+/* fields::field_energy() — src/energy_and_flux.cpp:48,54-62: the electric energy plus the
+   magnetic energy of B and H advanced half a step and averaged to time t
+   (synchronize_magnetic_fields, :146-178); every magnetic array is restored afterwards. */
+int mp_fields_field_energy(mp_fields *f, double *energy);
 /* fields::get_field(component, const vec&) — src/meep.hpp:2183-2185, src/monitor.cpp:93-108 */
 int mp_fields_get_field(mp_fields *f, int c, const double *p, double *re, double *im);
 /* Copy one state array (reference layout, ntot doubles) to the host: kind MP_ARR_F is
@@ -185,6 +190,23 @@ long mp_dft_flux_chunk_points(mp_dft_flux *fl, int which, int i);
 int mp_dft_flux_decimation(mp_dft_flux *fl);
 /* dft_chunk::dft of chunk i, interleaved (re, im), frequency fastest (src/meep.hpp:1138). */
 int mp_dft_flux_get_dft(mp_dft_flux *fl, int which, int i, double *out);
+
+/* ---- DFT fields ---------------------------------------------------------------------- */
+
+/* fields::add_dft_fields(components, num_components, volume, const double *freq, Nfreq,
+   use_centered_grid = true, decimation_factor) — src/meep.hpp:2085-2090, src/dft.cpp:913-927:
+   per component, accumulators on the centred grid without integration weights.  Chunks are
+   listed in the reference's order (each component's chunks prepended). */
+mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *components, int num_components,
+                                        const double *vmin, const double *vmax,
+                                        const double *freq, int nfreq, int decimation_factor);
+void mp_dft_fields_delete(mp_dft_fields *df);
+int mp_dft_fields_num_chunks(mp_dft_fields *df);
+long mp_dft_fields_chunk_points(mp_dft_fields *df, int i);
+/* the field component accumulated by chunk i (dft_chunk::c) */
+int mp_dft_fields_chunk_component(mp_dft_fields *df, int i);
+/* dft_chunk::dft of chunk i, interleaved (re, im), frequency fastest. */
+int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out);
 
 #ifdef __cplusplus
 }

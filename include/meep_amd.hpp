@@ -1,0 +1,342 @@
+// meep_amd.hpp — the subset of Meep's C++ API (namespace meep) whose fields::step() runs on
+// the MI355X engine.  Header-only: every method forwards to the C-ABI in meep_amd.h.
+//
+// A program written against the reference's classes for this path — grid_volume from
+// vol1d/vol2d/vol3d, structure(gv, eps, pml(...)), fields(&s), add_point_source,
+// add_volume_source, use_bloch, step(), get_field, add_dft_flux / flux() — compiles against
+// this header and links -lmeep_amd.  Names, argument meaning and the error behaviour
+// (meep::abort -> std::runtime_error("meep: ...")) follow src/meep.hpp of NanoComp/meep; the
+// line cited on each member is the declaration it mirrors.
+#pragma once
+
+#include "meep_amd.h"
+
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace meep {
+
+typedef std::complex<double> cdouble;
+
+// src/meep/vec.hpp:31-79 and src/meep.hpp:1594 (same numeric values)
+enum component { Ex = 0, Ey, Er, Ep, Ez, Hx, Hy, Hr, Hp, Hz, Dx, Dy, Dr, Dp, Dz, Bx, By, Br, Bp, Bz };
+enum direction { X = 0, Y, Z, R, P, NO_DIRECTION };
+enum boundary_side { High = 0, Low };
+enum boundary_condition { Periodic = 0, Metallic, Magnetic, None };
+enum ndim { D1 = 0, D2, D3, Dcyl };
+
+[[noreturn]] inline void abort(const char *fmt, ...) { // src/mympi.cpp:238-256
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw std::runtime_error(std::string("meep: ") + buf);
+}
+inline void check(int rc) {
+  if (rc < 0) throw std::runtime_error(mp_last_error());
+}
+
+// vec (src/meep/vec.hpp:621-750), Cartesian only
+class vec {
+public:
+  vec() : dim(D1) { t[0] = t[1] = t[2] = 0; }
+  explicit vec(double zz) : dim(D1) { t[0] = t[1] = 0, t[2] = zz; }
+  vec(double xx, double yy) : dim(D2) { t[0] = xx, t[1] = yy, t[2] = 0; }
+  vec(double xx, double yy, double zz) : dim(D3) { t[0] = xx, t[1] = yy, t[2] = zz; }
+  double x() const { return t[0]; }
+  double y() const { return t[1]; }
+  double z() const { return t[2]; }
+  double in_direction(direction d) const { return t[d]; }
+  void set_direction(direction d, double v) { t[d] = v; }
+  const double *data() const { return t; }
+  ndim dim;
+
+private:
+  double t[3];
+};
+
+// grid_volume (src/meep/vec.hpp:1014-1177): size and resolution of a cell with its low
+// corner at the origin
+class grid_volume {
+public:
+  ndim dim;
+  double a;
+  double size[3];
+  vec center() const { // grid_volume::center (src/vec.cpp:1089-1103)
+    double c[3];
+    for (int d = 0; d < 3; ++d) {
+      int n = int(size[d] * a + 0.5);
+      c[d] = (n - n % 2) * 0.5 / a;
+    }
+    if (dim == D1) return vec(c[2]);
+    if (dim == D2) return vec(c[0], c[1]);
+    return vec(c[0], c[1], c[2]);
+  }
+  double inva() const { return 1.0 / a; }
+};
+inline grid_volume volone(double zsize, double a) { return grid_volume{D1, a, {0, 0, zsize}}; }
+inline grid_volume vol1d(double zsize, double a) { return volone(zsize, a); }
+inline grid_volume voltwo(double xsize, double ysize, double a) { return grid_volume{D2, a, {xsize, ysize, 0}}; }
+inline grid_volume vol2d(double xsize, double ysize, double a) { return voltwo(xsize, ysize, a); }
+inline grid_volume vol3d(double xsize, double ysize, double zsize, double a) {
+  return grid_volume{D3, a, {xsize, ysize, zsize}};
+}
+
+// volume (src/meep/vec.hpp:948-1004): a box between two corners
+class volume {
+public:
+  volume(const vec &a, const vec &b) : lo(a), hi(b) {}
+  explicit volume(const vec &p) : lo(p), hi(p) {}
+  vec get_min_corner() const { return lo; }
+  vec get_max_corner() const { return hi; }
+
+private:
+  vec lo, hi;
+};
+
+// boundary_region / pml() (src/meep.hpp:660-736, src/structure.cpp:252-273)
+class boundary_region {
+public:
+  struct term {
+    int d, side;
+    double thickness, R;
+  };
+  std::vector<term> terms;
+  boundary_region operator+(const boundary_region &o) const {
+    boundary_region r(*this);
+    r.terms.insert(r.terms.end(), o.terms.begin(), o.terms.end());
+    return r;
+  }
+};
+inline boundary_region pml(double thickness, direction d, boundary_side side, double Rasymptotic = 1e-15) {
+  boundary_region r;
+  r.terms.push_back({int(d), int(side), thickness, Rasymptotic});
+  return r;
+}
+inline boundary_region pml(double thickness, direction d, double Rasymptotic = 1e-15) {
+  return pml(thickness, d, Low, Rasymptotic) + pml(thickness, d, High, Rasymptotic);
+}
+inline boundary_region pml(double thickness, double Rasymptotic = 1e-15) {
+  return pml(thickness, X, Rasymptotic) + pml(thickness, Y, Rasymptotic) + pml(thickness, Z, Rasymptotic);
+}
+
+// lorentzian_susceptibility (src/meep.hpp:246-280)
+class lorentzian_susceptibility {
+public:
+  lorentzian_susceptibility(double omega_0, double gamma, bool no_omega_0_denominator = false)
+      : omega_0(omega_0), gamma(gamma), drude(no_omega_0_denominator) {}
+  double omega_0, gamma;
+  bool drude;
+};
+
+// src_time classes (src/meep.hpp:943-1080); is_integrated defaults to true as in C++
+class src_time {
+public:
+  mp_src_time d{};
+  bool is_integrated = true;
+  mp_src_time desc() const {
+    mp_src_time r = d;
+    r.is_integrated = is_integrated ? 1 : 0;
+    return r;
+  }
+};
+class gaussian_src_time : public src_time {
+public:
+  gaussian_src_time(double f, double fwidth, double s = 5.0) { // src/sources.cpp:72-83
+    d.kind = 0;
+    d.freq = f;
+    d.width = 1.0 / fwidth;
+    d.start_time = 0;
+    d.end_time = 2 * d.width * s;
+  }
+  gaussian_src_time(double f, double w, double st, double et) { // src/sources.cpp:85-96
+    d.kind = 0;
+    d.freq = f;
+    d.width = w;
+    d.start_time = st;
+    d.end_time = et;
+  }
+};
+class continuous_src_time : public src_time {
+public:
+  continuous_src_time(double f, double w = 0.0, double st = 0.0, double et = 1e300, double s = 3.0) {
+    d.kind = 1;
+    d.freq = f;
+    d.width = w;
+    d.start_time = st;
+    d.end_time = et;
+    d.slowness = s;
+  }
+};
+
+namespace detail {
+inline double eps_trampoline(const double *r, void *u) {
+  typedef double (*fn)(const vec &);
+  return reinterpret_cast<fn>(u)(vec(r[0], r[1], r[2]));
+}
+} // namespace detail
+
+// realnum (src/meep.hpp:43-46) is float when the reference is configured --enable-single.
+// Here the choice is made per structure at run time; this flag is the default that every
+// structure created afterwards takes (initialised from MEEP_AMD_SINGLE at compile time).
+inline bool &realnum_is_float() {
+#ifdef MEEP_AMD_SINGLE
+  static bool f = true;
+#else
+  static bool f = false;
+#endif
+  return f;
+}
+
+// symmetry (src/meep.hpp:745-812): only the identity has a GPU path.
+class symmetry {
+public:
+  bool is_identity = true;
+};
+inline symmetry identity() { return symmetry(); }
+
+// structure (src/meep.hpp:815-930)
+class structure {
+public:
+  // structure(gv, double eps(const vec&), boundary_region, symmetry, num_chunks, Courant,
+  // use_anisotropic_averaging, tol, maxeval) (src/meep.hpp:835-839); eps sees (x, y, z)
+  // with absent coordinates 0.  One chunk always: num_chunks is accepted and ignored.
+  structure(const grid_volume &gv, double eps(const vec &), const boundary_region &br = boundary_region(),
+            const symmetry &sym = identity(), int num_chunks = 0, double Courant = 0.5,
+            bool use_anisotropic_averaging = false, double tol = 1e-4, int maxeval = 100000)
+      : gv_(gv) {
+    (void)num_chunks, (void)tol, (void)maxeval;
+    if (!sym.is_identity) meep::abort("only the identity symmetry is supported by the GPU engine");
+    if (use_anisotropic_averaging) meep::abort("subpixel averaging is not supported by the GPU engine");
+    std::vector<int> dirs, sides;
+    std::vector<double> th, R;
+    for (const auto &t : br.terms) {
+      dirs.push_back(t.d);
+      sides.push_back(t.side);
+      th.push_back(t.thickness);
+      R.push_back(t.R);
+    }
+    int dim = gv.dim == D1 ? 1 : gv.dim == D2 ? 2 : 3;
+    p = mp_structure_new(dim, gv.size[0], gv.size[1], gv.size[2], gv.a, eps ? detail::eps_trampoline : nullptr,
+                         reinterpret_cast<void *>(eps), int(dirs.size()), dirs.data(), sides.data(), th.data(), R.data(),
+                         Courant, realnum_is_float() ? 1 : 0);
+    if (!p) throw std::runtime_error(mp_last_error());
+  }
+  ~structure() { mp_structure_delete(p); }
+  structure(const structure &) = delete;
+  structure &operator=(const structure &) = delete;
+  // add_susceptibility(sigma, E_stuff, sus) (src/meep.hpp:864)
+  void add_susceptibility(double sigma(const vec &), int /* field_type: E_stuff */, const lorentzian_susceptibility &sus) {
+    check(mp_structure_add_susceptibility(p, sus.omega_0, sus.gamma, sus.drude, detail::eps_trampoline,
+                                          reinterpret_cast<void *>(sigma)));
+  }
+  const grid_volume &gv() const { return gv_; }
+  mp_structure *handle() { return p; }
+
+private:
+  grid_volume gv_;
+  mp_structure *p;
+};
+constexpr int E_stuff = 0;
+
+// dft_flux (src/meep.hpp:1209-1255)
+class dft_flux {
+public:
+  dft_flux(mp_dft_flux *h, int n) : h(h), n(n) {}
+  std::vector<double> flux_vec() const {
+    std::vector<double> out(n);
+    check(mp_dft_flux_flux(h, out.data()));
+    return out;
+  }
+  // flux() returns a new[]-allocated array as the reference does (src/dft.cpp:542-556)
+  double *flux() const {
+    double *out = new double[n];
+    check(mp_dft_flux_flux(h, out));
+    return out;
+  }
+  int Nfreq() const { return n; }
+
+private:
+  mp_dft_flux *h;
+  int n;
+};
+
+// fields (src/meep.hpp:1716-2320)
+class fields {
+public:
+  explicit fields(structure *s, double m = 0, double beta = 0) : gv(s->gv()) {
+    if (m != 0 || beta != 0) meep::abort("only m = 0, beta = 0 are supported by the GPU engine");
+    p = mp_fields_new(s->handle());
+    if (!p) throw std::runtime_error(mp_last_error());
+  }
+  ~fields() {
+    for (mp_dft_flux *h : fluxes) mp_dft_flux_delete(h);
+    mp_fields_delete(p);
+  }
+  fields(const fields &) = delete;
+  fields &operator=(const fields &) = delete;
+
+  void use_real_fields() { check(mp_fields_use_real_fields(p)); }                       // :1746
+  void use_bloch(direction d, cdouble k) { check(mp_fields_use_bloch_dir(p, d, k.real(), k.imag())); } // :1762
+  void use_bloch(const vec &k) { // :1763
+    double kk[3];
+    int j = 0;
+    if (gv.dim == D1)
+      kk[j++] = k.z();
+    else {
+      kk[j++] = k.x();
+      kk[j++] = k.y();
+      if (gv.dim == D3) kk[j++] = k.z();
+    }
+    check(mp_fields_use_bloch(p, kk));
+  }
+  void set_boundary(boundary_side b, direction d, boundary_condition c) { check(mp_fields_set_boundary(p, b, d, c)); } // :1761
+  // deprecated add_point_source (src/meep.hpp:1888-1889)
+  void add_point_source(component c, double freq, double width, double peaktime, double cutoff, const vec &pos,
+                        cdouble amp = 1.0, int is_continuous = 0) {
+    check(mp_fields_add_point_source_legacy(p, c, freq, width, peaktime, cutoff, pos.data(), amp.real(), amp.imag(),
+                                            is_continuous));
+  }
+  void add_point_source(component c, const src_time &src, const vec &pos, cdouble amp = 1.0) { // :1890
+    mp_src_time d = src.desc();
+    check(mp_fields_add_volume_source(p, c, &d, pos.data(), pos.data(), amp.real(), amp.imag(), nullptr, nullptr));
+  }
+  void add_volume_source(component c, const src_time &src, const volume &where, cdouble amp = 1.0) { // :1893
+    mp_src_time d = src.desc();
+    check(mp_fields_add_volume_source(p, c, &d, where.get_min_corner().data(), where.get_max_corner().data(), amp.real(),
+                                      amp.imag(), nullptr, nullptr));
+  }
+  void step() { check(mp_fields_step(p)); }                // :1871
+  double time() const { return mp_fields_time(p); }        // :1877
+  double round_time() const { return mp_fields_round_time(p); } // :1876
+  double last_source_time() { return mp_fields_last_source_time(p); } // :1887
+  cdouble get_field(component c, const vec &loc) const {   // :2183
+    double re = 0, im = 0;
+    check(mp_fields_get_field(p, c, loc.data(), &re, &im));
+    return cdouble(re, im);
+  }
+  // add_dft_flux(direction, volume, freq_min, freq_max, Nfreq) (src/meep.hpp:2044-2049)
+  dft_flux add_dft_flux(direction d, const volume &where, double freq_min, double freq_max, int Nfreq) {
+    std::vector<double> fr(Nfreq);
+    double df = Nfreq <= 1 ? 0.0 : (freq_max - freq_min) / (Nfreq - 1);
+    for (int i = 0; i < Nfreq; ++i) fr[i] = Nfreq <= 1 ? (freq_min + freq_max) * 0.5 : freq_min + i * df;
+    mp_dft_flux *h = mp_fields_add_dft_flux(p, d, where.get_min_corner().data(), where.get_max_corner().data(), fr.data(),
+                                            Nfreq, 0);
+    if (!h) throw std::runtime_error(mp_last_error());
+    fluxes.push_back(h);
+    return dft_flux(h, Nfreq);
+  }
+  mp_fields *handle() { return p; }
+
+private:
+  grid_volume gv;
+  mp_fields *p;
+  std::vector<mp_dft_flux *> fluxes;
+};
+
+} // namespace meep

@@ -40,5 +40,20 @@ def build(verbose=False):
     return LIB
 
 
+def build_cpp_program(src, out, lib_dir, lib_name, verbose=False):
+    """Compile a C++ program written against include/meep_amd.hpp and link it to one
+    implementation of the C-ABI (``lib<lib_name>.so`` in ``lib_dir``), found at run time
+    through an rpath relative to the program."""
+    root = os.path.dirname(HERE)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    rel = os.path.relpath(lib_dir, os.path.dirname(out))
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"), src,
+           "-L", lib_dir, "-l" + lib_name, "-Wl,-rpath,$ORIGIN/" + rel, "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -106,6 +106,15 @@ _SIGNATURES = {
     "mp_dft_flux_chunk_points": (C.c_long, [C.c_void_p, C.c_int, C.c_int]),
     "mp_dft_flux_decimation": (C.c_int, [C.c_void_p]),
     "mp_dft_flux_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mp_fields_field_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_fields_add_dft_fields": (C.c_void_p, [C.c_void_p, C.POINTER(C.c_int), C.c_int,
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double), C.c_int, C.c_int]),
+    "mp_dft_fields_delete": (None, [C.c_void_p]),
+    "mp_dft_fields_num_chunks": (C.c_int, [C.c_void_p]),
+    "mp_dft_fields_chunk_points": (C.c_long, [C.c_void_p, C.c_int]),
+    "mp_dft_fields_chunk_component": (C.c_int, [C.c_void_p, C.c_int]),
+    "mp_dft_fields_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
@@ -199,8 +208,10 @@ class structure:
         return out.reshape(self.shape)
 
     def add_susceptibility(self, sigma, omega_0, gamma, drude=False):
-        """add_susceptibility(sigma, E_stuff, lorentzian_susceptibility(omega_0, gamma, drude))."""
-        cb = SCALAR_FUNC(lambda r, _u: float(sigma(r[0], r[1], r[2])))
+        """add_susceptibility(sigma, E_stuff, lorentzian_susceptibility(omega_0, gamma, drude)).
+        sigma None leaves sigma zero everywhere (fill it with set_sigma)."""
+        cb = SCALAR_FUNC(0) if sigma is None else SCALAR_FUNC(
+            lambda r, _u: float(sigma(r[0], r[1], r[2])))
         self._keep.append(cb)
         return _check(self.lib, self.lib.mp_structure_add_susceptibility(
             self.ptr, omega_0, gamma, int(drude), cb, None))
@@ -240,6 +251,31 @@ class dft_flux:
         n = self.lib.mp_dft_flux_chunk_points(self.ptr, which, i)
         out = np.empty(2 * n * self.nfreq)
         _check(self.lib, self.lib.mp_dft_flux_get_dft(self.ptr, which, i, _dptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
+
+
+class dft_fields:
+    """meep::dft_fields (src/meep.hpp:1420-1441): accumulators per chunk and component."""
+
+    def __init__(self, fields, ptr, nfreq):
+        self.f = fields
+        self.lib = fields.lib
+        self.ptr = ptr
+        self.nfreq = nfreq
+
+    def num_chunks(self):
+        return self.lib.mp_dft_fields_num_chunks(self.ptr)
+
+    def component(self, i):
+        c = self.lib.mp_dft_fields_chunk_component(self.ptr, i)
+        if c < 0:
+            raise MeepError(self.lib.mp_last_error().decode())
+        return c
+
+    def dft(self, i):
+        n = self.lib.mp_dft_fields_chunk_points(self.ptr, i)
+        out = np.empty(2 * n * self.nfreq)
+        _check(self.lib, self.lib.mp_dft_fields_get_dft(self.ptr, i, _dptr(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
 
 
@@ -316,6 +352,12 @@ class fields:
     def last_source_time(self):
         return self.lib.mp_fields_last_source_time(self.ptr)
 
+    def field_energy(self):
+        """fields::field_energy() (src/energy_and_flux.cpp:48)."""
+        e = C.c_double()
+        _check(self.lib, self.lib.mp_fields_field_energy(self.ptr, C.byref(e)))
+        return e.value
+
     def get_field(self, c, p):
         re, im = C.c_double(), C.c_double()
         _check(self.lib, self.lib.mp_fields_get_field(self.ptr, c, _dvec(p), C.byref(re),
@@ -366,11 +408,26 @@ class fields:
         self._fluxes.append(fl)
         return fl
 
+    def add_dft_fields(self, components, vmin, vmax, freqs, decimation_factor=0):
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        comps = (C.c_int * len(components))(*components)
+        ptr = self.lib.mp_fields_add_dft_fields(self.ptr, comps, len(components), _dvec(vmin),
+                                                _dvec(vmax), _dptr(freqs), len(freqs),
+                                                decimation_factor)
+        if not ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        df = dft_fields(self, ptr, len(freqs))
+        self._fluxes.append(df)
+        return df
+
     def __del__(self):
         if getattr(self, "ptr", None):
             for fl in self._fluxes:
                 if fl.ptr:
-                    self.lib.mp_dft_flux_delete(fl.ptr)
+                    if isinstance(fl, dft_fields):
+                        self.lib.mp_dft_fields_delete(fl.ptr)
+                    else:
+                        self.lib.mp_dft_flux_delete(fl.ptr)
                     fl.ptr = None
             self.lib.mp_fields_delete(self.ptr)
             self.ptr = None

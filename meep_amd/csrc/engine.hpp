@@ -1,7 +1,9 @@
 // engine.hpp — parameter blocks shared by the host plan (fields.cpp) and the CDNA4 kernels
-// (kernels.hip).  All field arrays use the reference layout: one array of ntot values per
-// component and real/imaginary part (src/vec.cpp:293-296,482-494); a launch covers every
-// array element with one thread and each thread handles all components of its cell.
+// (kernels.hip).  Field arrays keep the reference's organisation — one array per component
+// and real/imaginary part over the (n+1)^d points of the chunk, in the reference's loop order
+// (src/vec.cpp:293-296,482-494) — with the stride-1 rows padded (see Grid).  A thread of the
+// generic kernel handles every component of one point; the interior kernel handles a short
+// run of points along the stride-1 axis.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,6 +28,29 @@ struct Grid {
   int lo_magnetic; // bit d set: the Low wall normal to d is Magnetic
   int blo[3];      // launch box: lowest index per direction
   int bhi[3];      // launch box: highest index per direction
+  // The launch box by loop axis, derived from the above just before a launch (finalize_args
+  // in fields.cpp).  Kernels index only these and the per-slot loop-axis data, with constant
+  // indices: a run-time index into the kernel's parameter block makes the compiler copy the
+  // whole block into scratch memory.
+  int alo[3];      // lowest index along loop axis k (0 when absent)
+  int an[3];       // points along loop axis k (1 when absent)
+  int64_t ast[3];  // device stride along loop axis k (0 when absent)
+  int nax[3];      // cells along loop axis k
+  // Periodic halos folded into the kernels along loop axis k when the Bloch phase there is
+  // real: +1 or -1 (the phase), 0 when not folded.  A component with no Yee shift along the
+  // axis copies its owned point n into the halo point 0, one with a shift copies point 0
+  // into the halo point n (the reference's connections, src/boundaries.cpp:368-600), and a
+  // point that wraps along several folded axes writes every combination.
+  int wrap[3];
+};
+
+// Per-slot data by loop axis (derived by finalize_args): owned index range, and the index of
+// the plane zeroed by a High metallic wall (mhi) or a Low magnetic wall (mlo), -1 for none
+// (on_metal_boundary, src/boundaries.cpp:191-205), and the Yee shift along each loop axis.
+struct AxisSlot {
+  int lo[3], hi[3];
+  int mhi[3], mlo[3];
+  int ysh[3];
 };
 
 template <class R> struct PmlProfile {
@@ -43,6 +68,11 @@ template <class R> struct CurlComp {
   int dsigu; // PML direction acting through fu, -1 if none
   int lo[3], hi[3]; // owned index range per direction (inclusive)
   int ysh[3];       // Yee shift per direction (PML index k = 2*j + ysh)
+  // derived by finalize_args: loop axis of dsig / dsigu (-1 if none), the Yee shift along it,
+  // and that direction's uPML profiles
+  AxisSlot ax;
+  int ksig, ksigu, ysig, ysigu;
+  const R *sig, *kap, *siginv, *sigu, *kapu, *siginvu;
 };
 
 // One E or H component of update_eh (src/update_eh.cpp:66-215, src/step_generic.cpp:566-785)
@@ -56,6 +86,10 @@ template <class R> struct UpdComp {
   int lo[3], hi[3], ysh[3];
   R *P[MAXPOLE], *Pp[MAXPOLE];
   const R *sigma[MAXPOLE];
+  // derived by finalize_args, as for CurlComp
+  AxisSlot ax;
+  int ksigw, ysigw;
+  const R *sigw, *kapw;
 };
 
 template <class R> struct PoleCoef {
@@ -73,6 +107,23 @@ struct SrcTab {
   double val[SRC_TAB];
 };
 
+// Sources too long for a table, carried as dense index boxes: source s covers array indices
+// lo[s][k]..hi[s][k] along loop axis k of component comp[s], with one complex amplitude per
+// box point in amp[s] (box-local, strides bst[s], zero where the source has no point).  The value subtracted
+// from the field at a point is the real (cmp 0) or imaginary (cmp 1) part of
+// (amp * scal) * mult, formed in double as the reference forms sv.current(j) * dt
+// (src/step.cpp:307-315) and sv.dipole(j) (src/update_eh.cpp:128-138).
+constexpr int SRC_BOX = 2;
+struct SrcBoxes {
+  int n;
+  int comp[SRC_BOX];
+  int lo[SRC_BOX][3], hi[SRC_BOX][3];
+  int64_t bst[SRC_BOX][3];
+  const double2 *amp[SRC_BOX];
+  double2 scal[SRC_BOX];
+  double mult[SRC_BOX];
+};
+
 template <class R> struct StepArgs {
   Grid g;
   PmlProfile<R> pml;
@@ -83,6 +134,8 @@ template <class R> struct StepArgs {
   PoleCoef<R> pole[MAXPOLE];
   SrcTab cur; // currents, after the curl
   SrcTab dip; // integrated dipoles, inside f_minus_p
+  SrcBoxes curb, dipb; // the same for long source lists
+  int cmp;             // 0: real parts, 1: imaginary parts of the fields this launch moves
   // Slots are by direction: cc[d] / uc[d] is the component along d (f == nullptr when
   // absent).  For the vectorised kernel the curl partners are given by direction too: F[d]
   // is the partner field's component along d; slot d's g1 is F[(d+2)%3] differenced along
@@ -92,11 +145,6 @@ template <class R> struct StepArgs {
   int hasp[3], hasm[3];
   int sgn;
   R dtdx; // Courant number (the vectorised kernel applies the partner signs itself)
-  // Periodic wrap along the stride-1 direction folded into the kernels (real Bloch phase
-  // zsign = +-1): a component with no Yee shift along it copies its owned point n into the
-  // halo point 0, one with a shift copies point 0 into the halo point n (f, E/H and P).
-  int zwrap;
-  int zsign;
   int rev; // visit logical blocks in descending order (backward differences)
 };
 
@@ -132,7 +180,15 @@ template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, h
 template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
 template <class R>
 void launch_conn(const Conn *conns, const double2 *phases, int n, R *const *slot_ptrs, int ncmp, hipStream_t s);
-template <class R> void launch_src(R *f, const int64_t *idx, const double *val, int n, hipStream_t s);
+// f[idx[j]] -= Re or Im of (amp[j] * scal) * mult for the n points of one source
+template <class R>
+void launch_src(R *f, const int64_t *idx, const double2 *amp, int n, double2 scal, double mult, int cmp,
+                hipStream_t s);
+// f[i] = (f[i] + b[i]) / 2 over n points (fields_chunk::average_with_backup)
+template <class R> void launch_average(R *f, const R *b, size_t n, hipStream_t s);
+// dst[j] = f[idx[j]] and f[idx[j]] = src[j]
+template <class R> void launch_gather(R *dst, const R *f, const int64_t *idx, int n, hipStream_t s);
+template <class R> void launch_scatter(R *f, const int64_t *idx, const R *src, int n, hipStream_t s);
 template <class R> void launch_dft(const DftArgs<R> &a, hipStream_t s);
 template <class R>
 void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int n, int *flag,

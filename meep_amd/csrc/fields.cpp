@@ -231,9 +231,16 @@ struct SrcVol { // src_vol (src/meep_internals.hpp:49-82)
   int src;      // index into the source-time list
   std::vector<int64_t> index;
   std::vector<cdouble> amp;
-  // device copies for lists longer than the kernel-argument table
+  std::vector<int> jpos; // array indices X, Y, Z of each point (0 for absent directions)
+  // device copies for lists longer than the kernel-argument table, made on first use and
+  // refreshed when the amplitudes change
   int64_t *d_idx = nullptr;
-  double *d_val = nullptr;
+  double2 *d_amp = nullptr; // list order
+  double2 *d_box = nullptr; // dense over the index box blo..bhi (zero off the list)
+  int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+  int64_t bst[3] = {0, 0, 0};
+  bool use_box = false;
+  bool dirty = true;
 };
 
 // ---- device buffers -----------------------------------------------------------------------
@@ -273,6 +280,7 @@ struct FieldsBase {
   virtual double dt_now() const = 0;
   virtual double last_source_time() const = 0;
   virtual cdouble get_field(int c, const V3 &p) = 0;
+  virtual double field_energy() = 0;
   virtual int get_array(int c, int cmp, int kind, int ipol, double *out) = 0;
   virtual void set_array(int c, int cmp, int kind, int ipol, const double *in) = 0;
   virtual bool have_component(int c) const = 0;
@@ -282,6 +290,8 @@ struct FieldsBase {
   virtual void get_ktiming(int which, double *ms, long *n) = 0;
   virtual void reset_ktiming() = 0;
   virtual void *add_dft_flux(int d, V3 mn, V3 mx, const std::vector<double> &freq, int dec, int *dec_out) = 0;
+  virtual void *add_dft_fields(const int *comps, int ncomp, V3 mn, V3 mx, const std::vector<double> &freq, int dec, int *dec_out) = 0;
+  virtual int flux_chunk_component(void *fl, int which, int i) = 0;
   virtual void flux(void *fl, double *out) = 0;
   virtual int flux_chunks(void *fl, int which) = 0;
   virtual long flux_points(void *fl, int which, int i) = 0;
@@ -498,7 +508,8 @@ template <class R> struct Fields : FieldsBase {
       for (int ft = 0; ft < 4; ++ft)
         for (auto &sv : sources[ft]) {
           if (sv.d_idx) (void)hipFree(sv.d_idx);
-          if (sv.d_val) (void)hipFree(sv.d_val);
+          if (sv.d_amp) (void)hipFree(sv.d_amp);
+          if (sv.d_box) (void)hipFree(sv.d_box);
         }
       for (const Pending &p : pending) {
         (void)hipEventDestroy(p.a);
@@ -623,8 +634,9 @@ template <class R> struct Fields : FieldsBase {
   }
   // Periodic wrap along the stride-1 direction is folded into the update kernels when its
   // Bloch phase is real (+1 or -1); those halo points then leave the connection lists.
-  bool zwrap_ok = false;
-  int zsign = 1;
+  // per direction: +1 / -1 when its periodic halo is folded into the kernels (real Bloch
+  // phase on both sides), 0 when the halo lists carry it
+  int wrap_dir[3] = {0, 0, 0};
   // slots: 0..19 the field arrays, 20 + 20*p + c the polarisation P of pole p
   void rebuild_plan() {
     HIPCHECK(hipStreamSynchronize(stream));
@@ -646,15 +658,21 @@ template <class R> struct Fields : FieldsBase {
       auto close = [](cdouble p, double r) { return std::fabs(p.imag()) < 1e-13 && std::fabs(p.real() - r) < 1e-13; };
       return close(ph, 1.0) ? 2 : close(ph, -1.0) ? 1 : 0;
     };
-    {
-      const int d = gv.fast;
+    for (int d = 0; d < 3; ++d) {
       const cdouble e = eikna[d];
-      zwrap_ok = false;
-      if (gv.has(d) && bc[High][d] == Periodic && std::fabs(e.imag()) < 1e-13) {
-        if (std::fabs(e.real() - 1.0) < 1e-13) zwrap_ok = true, zsign = 1;
-        if (std::fabs(e.real() + 1.0) < 1e-13) zwrap_ok = true, zsign = -1;
+      wrap_dir[d] = 0;
+      if (gv.has(d) && bc[High][d] == Periodic && bc[Low][d] == Periodic && std::fabs(e.imag()) < 1e-13) {
+        if (std::fabs(e.real() - 1.0) < 1e-13) wrap_dir[d] = 1;
+        if (std::fabs(e.real() + 1.0) < 1e-13) wrap_dir[d] = -1;
       }
     }
+    // halo planes of directions the lists carry come first, so the folded planes' boxes hold
+    // only points that are halo points along folded directions alone
+    int order[3], no = 0;
+    for (int d = 0; d < 3; ++d)
+      if (!wrap_dir[d]) order[no++] = d;
+    for (int d = 0; d < 3; ++d)
+      if (wrap_dir[d]) order[no++] = d;
     for (int corig = 0; corig < NC; ++corig) {
       if (!f[corig][0]) continue;
       if (is_B(corig) && B_redundant[corig - 15]) continue;
@@ -665,7 +683,8 @@ template <class R> struct Fields : FieldsBase {
         lo[d] = 0;
         hi[d] = gv.has(d) ? gv.num[d] : 0;
       }
-      for (int d = 0; d < 3; ++d) {
+      for (int od = 0; od < 3; ++od) {
+        const int d = order[od];
         if (!gv.has(d)) continue;
         const int plane = gv.ysh(corig, d) ? gv.num[d] : 0;
         int blo[3], bhi[3];
@@ -674,8 +693,8 @@ template <class R> struct Fields : FieldsBase {
           bhi[q] = hi[q];
         }
         blo[d] = bhi[d] = plane;
-        // the last box (along the stride-1 direction) is written by the kernels themselves
-        const bool folded = zwrap_ok && d == gv.fast;
+        // folded planes are written by the kernels themselves
+        const bool folded = wrap_dir[d] != 0;
         for (int j0 = blo[0]; j0 <= (folded ? blo[0] - 1 : bhi[0]); ++j0)
           for (int j1 = blo[1]; j1 <= bhi[1]; ++j1)
             for (int j2 = blo[2]; j2 <= bhi[2]; ++j2) {
@@ -794,34 +813,124 @@ template <class R> struct Fields : FieldsBase {
           if (p < a.npole && u.P[p] && !u.Pp[p]) fail("internal: polarisation without its previous value");
       }
     }
+    // source boxes: inside the arrays, with amplitudes, for a present slot
+    const SrcBoxes *lists[2] = {&a.curb, &a.dipb};
+    for (int l = 0; l < 2; ++l) {
+      const SrcBoxes &b = *lists[l];
+      if (b.n < 0 || b.n > SRC_BOX) fail("internal: bad source box count");
+      for (int q = 0; q < b.n; ++q) {
+        if (!b.amp[q] || b.comp[q] < 0 || b.comp[q] >= a.nc) fail("internal: bad source box %d", q);
+        for (int k = 0; k < 3; ++k) {
+          const int d = gv.ydir(k);
+          const int hi = gv.has(d) ? gv.num[d] : 0;
+          if (b.lo[q][k] < 0 || b.hi[q][k] > hi || b.lo[q][k] > b.hi[q][k]) fail("internal: source box outside the grid");
+        }
+      }
+    }
+    // loop-axis view: launch box inside the grid, slots' ranges inside the arrays
+    const Grid &g = a.g;
+    for (int k = 0; k < 3; ++k) {
+      const int d = gv.ydir(k);
+      const int n = gv.has(d) ? gv.num[d] : 0;
+      if (g.alo[k] < 0 || g.an[k] < 1 || g.alo[k] + g.an[k] - 1 > n) fail("internal: launch box outside the grid");
+      if (g.ast[k] != (gv.has(d) ? gv.st[d] : 0)) fail("internal: launch strides do not match the layout");
+      if (g.wrap[k] < -1 || g.wrap[k] > 1 || (g.wrap[k] && (!gv.has(d) || g.nax[k] != n))) fail("internal: bad folded wrap");
+      for (int c = 0; c < a.nc; ++c) {
+        const AxisSlot &x = curl ? a.cc[c].ax : a.uc[c].ax;
+        if ((curl ? a.cc[c].f : a.uc[c].f) && (x.lo[k] < 0 || x.hi[k] > n)) fail("internal: slot %d range outside the grid", c);
+      }
+    }
+    for (int c = 0; c < a.nc; ++c) {
+      if (curl && a.cc[c].f) {
+        const CurlComp<R> &k = a.cc[c];
+        if ((k.ksig >= 0) != (k.sig != nullptr) || (k.ksigu >= 0) != (k.sigu != nullptr)) fail("internal: slot %d PML axes", c);
+      }
+      if (upd && a.uc[c].f && (a.uc[c].ksigw >= 0) != (a.uc[c].sigw != nullptr)) fail("internal: slot %d PML W axis", c);
+    }
+  }
+  // Loop-axis view of an argument block (the only form the kernels read): the launch box and
+  // each slot's owned range, metal planes and Yee shift by loop axis, and the loop axis, Yee
+  // shift and profiles of each slot's uPML directions.
+  void axis_slot(AxisSlot &ax, const int *lo, const int *hi, const int *ysh) const {
+    for (int k = 0; k < 3; ++k) {
+      const int d = gv.ydir(k);
+      ax.mhi[k] = ax.mlo[k] = -1;
+      if (!gv.has(d)) {
+        ax.lo[k] = ax.hi[k] = 0;
+        continue;
+      }
+      ax.lo[k] = lo[d];
+      ax.hi[k] = hi[d];
+      if (bc[High][d] == Metallic && ysh[d] == 0) ax.mhi[k] = gv.num[d];
+      if (bc[Low][d] == Magnetic && ysh[d] == 1) ax.mlo[k] = 0;
+    }
+    for (int k = 0; k < 3; ++k) ax.ysh[k] = gv.has(gv.ydir(k)) ? ysh[gv.ydir(k)] : 0;
+  }
+  int axis_of(int d) const {
+    if (d < 0) return -1;
+    for (int k = 0; k < 3; ++k)
+      if (gv.ydir(k) == d && gv.has(d)) return k;
+    fail("internal: direction %d has no loop axis", d);
+  }
+  void finalize_args(StepArgs<R> &a) const {
+    Grid &g = a.g;
+    for (int k = 0; k < 3; ++k) {
+      const int d = g.axdir[k];
+      g.alo[k] = d >= 0 ? g.blo[d] : 0;
+      g.an[k] = d >= 0 ? g.bhi[d] - g.blo[d] + 1 : 1;
+      g.ast[k] = d >= 0 ? g.st[d] : 0;
+    }
+    for (int k = 0; k < 3; ++k) {
+      const int d = g.axdir[k];
+      g.nax[k] = d >= 0 ? gv.num[d] : 0;
+      g.wrap[k] = d >= 0 ? wrap_dir[d] : 0;
+    }
+    for (int c = 0; c < 3; ++c) {
+      CurlComp<R> &k = a.cc[c];
+      axis_slot(k.ax, k.lo, k.hi, k.ysh);
+      k.ksig = axis_of(k.dsig);
+      k.ksigu = axis_of(k.dsigu);
+      k.ysig = k.dsig >= 0 ? k.ysh[k.dsig] : 0;
+      k.ysigu = k.dsigu >= 0 ? k.ysh[k.dsigu] : 0;
+      k.sig = k.dsig >= 0 ? a.pml.sig[k.dsig] : nullptr;
+      k.kap = k.dsig >= 0 ? a.pml.kap[k.dsig] : nullptr;
+      k.siginv = k.dsig >= 0 ? a.pml.siginv[k.dsig] : nullptr;
+      k.sigu = k.dsigu >= 0 ? a.pml.sig[k.dsigu] : nullptr;
+      k.kapu = k.dsigu >= 0 ? a.pml.kap[k.dsigu] : nullptr;
+      k.siginvu = k.dsigu >= 0 ? a.pml.siginv[k.dsigu] : nullptr;
+      UpdComp<R> &u = a.uc[c];
+      axis_slot(u.ax, u.lo, u.hi, u.ysh);
+      u.ksigw = axis_of(u.dsigw);
+      u.ysigw = u.dsigw >= 0 ? u.ysh[u.dsigw] : 0;
+      u.sigw = u.dsigw >= 0 ? a.pml.sig[u.dsigw] : nullptr;
+      u.kapw = u.dsigw >= 0 ? a.pml.kap[u.dsigw] : nullptr;
+    }
+  }
+  void launch_box(StepArgs<R> &b, const int *lo, const int *hi, bool curl, bool upd, bool fast) {
+    for (int d = 0; d < 3; ++d) {
+      b.g.blo[d] = lo[d];
+      b.g.bhi[d] = hi[d];
+    }
+    finalize_args(b);
+    check_args(b, curl, upd, fast);
+    if (fast)
+      launch_fast<R>(b, curl, upd, stream);
+    else
+      launch_step<R>(b, curl, upd, stream);
   }
   void step_kernel(const StepArgs<R> &a, bool curl, bool upd, int kind) {
     if (!boxes_valid) compute_boxes();
-    check_args(a, curl, upd, false);
     kbegin(kind);
     StepArgs<R> b = a;
     if (!interior_empty) {
-      for (int d = 0; d < 3; ++d) {
-        b.g.blo[d] = interior.lo[d];
-        b.g.bhi[d] = interior.hi[d];
-      }
-      if (use_fast && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b))) {
-        // backward differences read the previous plane: walk the planes downwards so it was
-        // fetched just before (forward differences prefetch their neighbour naturally)
-        b.rev = kind == MP_K_STEP_DE ? 1 : 0;
-        check_args(b, curl, upd, true);
-        launch_fast<R>(b, curl, upd, stream);
-      }
-      else
-        launch_step<R>(b, curl, upd, stream);
+      const bool fast = use_fast && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b));
+      // backward differences read the previous plane: walk the planes downwards so it was
+      // fetched just before (forward differences prefetch their neighbour naturally)
+      b.rev = fast && kind == MP_K_STEP_DE ? 1 : 0;
+      launch_box(b, interior.lo, interior.hi, curl, upd, fast);
     }
-    for (const Box &bx : slabs) {
-      for (int d = 0; d < 3; ++d) {
-        b.g.blo[d] = bx.lo[d];
-        b.g.bhi[d] = bx.hi[d];
-      }
-      launch_step<R>(b, curl, upd, stream);
-    }
+    b.rev = 0;
+    for (const Box &bx : slabs) launch_box(b, bx.lo, bx.hi, curl, upd, false);
     kend();
   }
   // ---- per-step argument blocks ----
@@ -1061,11 +1170,25 @@ template <class R> struct Fields : FieldsBase {
   }
 
   // sparse source values of field type ft2 (D or B) at this step
-  void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, double time, SrcTab &tab, std::vector<const SrcVol *> &big) {
+  // a source list too long for the argument table, with this step's scalar
+  struct LongSrc {
+    SrcVol *sv;
+    int slot;
+    cdouble scal;
+    double mult;
+  };
+  // Sources of field type ft2 whose integrated flag is `integrated`, for the components in
+  // slots `which`: short lists go into the argument table, long ones into dense index boxes
+  // while they fit (SRC_BOX, and a box at most 8x the list), the rest into `rest`, which the
+  // caller applies with k_src.  Values are (amp * current(t)) * dt for currents
+  // (src/step.cpp:307-315) and amp * dipole(t) for integrated sources (src/update_eh.cpp:128-138).
+  void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, double time, SrcTab &tab, SrcBoxes &bx,
+                std::vector<LongSrc> &rest) {
     tab.n = 0;
     tab.lo = INT64_MAX;
     tab.hi = -1;
-    for (const SrcVol &sv : sources[ft2]) {
+    bx.n = 0;
+    for (SrcVol &sv : sources[ft2]) {
       const SrcTime &st = srcs[sv.src];
       if (st.is_integrated != integrated) continue;
       const cdouble scal = integrated ? st.dipole(time) : st.current(time, dt);
@@ -1076,39 +1199,85 @@ template <class R> struct Fields : FieldsBase {
         if (which[q] == (integrated ? sv.c : c2)) slot = q;
       if (slot < 0) continue;
       if (scal == 0.0) continue; // exact zero contributes nothing (the reference subtracts 0)
-      if (tab.n + int(sv.index.size()) > SRC_TAB) {
-        big.push_back(&sv);
+      if (tab.n + int(sv.index.size()) <= SRC_TAB) {
+        for (size_t j = 0; j < sv.index.size(); ++j) {
+          const cdouble A = (sv.amp[j] * scal) * mult;
+          tab.comp[tab.n] = slot;
+          tab.idx[tab.n] = sv.index[j];
+          tab.val[tab.n] = cmp ? A.imag() : A.real();
+          tab.lo = std::min(tab.lo, sv.index[j]);
+          tab.hi = std::max(tab.hi, sv.index[j]);
+          ++tab.n;
+        }
         continue;
       }
-      for (size_t j = 0; j < sv.index.size(); ++j) {
-        const cdouble A = (sv.amp[j] * scal) * mult;
-        tab.comp[tab.n] = slot;
-        tab.idx[tab.n] = sv.index[j];
-        tab.val[tab.n] = cmp ? A.imag() : A.real();
-        tab.lo = std::min(tab.lo, sv.index[j]);
-        tab.hi = std::max(tab.hi, sv.index[j]);
-        ++tab.n;
+      prepare_src(sv);
+      if (sv.use_box && bx.n < SRC_BOX) {
+        const int b = bx.n++;
+        bx.comp[b] = slot;
+        for (int k = 0; k < 3; ++k) { // by loop axis
+          const int d = gv.ydir(k);
+          const bool has = gv.has(d);
+          bx.lo[b][k] = has ? sv.blo[d] : 0;
+          bx.hi[b][k] = has ? sv.bhi[d] : 0;
+          bx.bst[b][k] = has ? sv.bst[d] : 0;
+        }
+        bx.amp[b] = sv.d_box;
+        bx.scal[b] = make_double2(scal.real(), scal.imag());
+        bx.mult[b] = mult;
+        continue;
       }
+      rest.push_back({&sv, slot, scal, mult});
     }
   }
-  void apply_big(const SrcVol &sv, R *fbase, int cmp, cdouble scal, double mult) {
-    std::vector<double> vals(sv.index.size());
-    for (size_t j = 0; j < sv.index.size(); ++j) {
-      const cdouble A = (sv.amp[j] * scal) * mult;
-      vals[j] = cmp ? A.imag() : A.real();
+  // device copies of a long source list: indices, amplitudes, and the dense box when the
+  // list fills at least 1/8 of its index box
+  void prepare_src(SrcVol &sv) {
+    if (!sv.dirty) return;
+    const size_t n = sv.index.size();
+    if (!sv.d_idx) {
+      HIPCHECK(hipMalloc(&sv.d_idx, sizeof(int64_t) * n));
+      HIPCHECK(hipMalloc(&sv.d_amp, sizeof(double2) * n));
     }
-    // values staged through a per-source device buffer; the copy is ordered on the stream
-    SrcVol &m = const_cast<SrcVol &>(sv);
-    if (!m.d_val) {
-      HIPCHECK(hipMalloc(&m.d_val, sizeof(double) * vals.size() * 2));
-      HIPCHECK(hipMalloc(&m.d_idx, sizeof(int64_t) * vals.size()));
-      HIPCHECK(hipMemcpy(m.d_idx, sv.index.data(), sizeof(int64_t) * vals.size(), hipMemcpyHostToDevice));
+    std::vector<double2> amp(n);
+    for (size_t j = 0; j < n; ++j) amp[j] = make_double2(sv.amp[j].real(), sv.amp[j].imag());
+    HIPCHECK(hipMemcpy(sv.d_idx, sv.index.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(sv.d_amp, amp.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; ++d) {
+      sv.blo[d] = INT32_MAX;
+      sv.bhi[d] = INT32_MIN;
     }
-    double *dv = m.d_val + cmp * vals.size();
-    HIPCHECK(hipMemcpyAsync(dv, vals.data(), sizeof(double) * vals.size(), hipMemcpyHostToDevice, stream));
-    HIPCHECK(hipStreamSynchronize(stream)); // vals lives on the host stack
+    for (size_t j = 0; j < n; ++j)
+      for (int d = 0; d < 3; ++d) {
+        sv.blo[d] = std::min(sv.blo[d], sv.jpos[3 * j + d]);
+        sv.bhi[d] = std::max(sv.bhi[d], sv.jpos[3 * j + d]);
+      }
+    size_t vol = 1;
+    for (int d = 2; d >= 0; --d) {
+      sv.bst[d] = int64_t(vol);
+      vol *= size_t(sv.bhi[d] - sv.blo[d] + 1);
+    }
+    sv.use_box = vol <= 8 * n;
+    if (sv.use_box) {
+      std::vector<double2> dense(vol, make_double2(0.0, 0.0));
+      std::vector<char> seen(vol, 0);
+      for (size_t j = 0; j < n; ++j) {
+        int64_t k = 0;
+        for (int d = 0; d < 3; ++d) k += int64_t(sv.jpos[3 * j + d] - sv.blo[d]) * sv.bst[d];
+        if (seen[k]) fail("internal: repeated point in a source list");
+        seen[k] = 1;
+        dense[k] = amp[j];
+      }
+      if (sv.d_box) (void)hipFree(sv.d_box);
+      HIPCHECK(hipMalloc(&sv.d_box, sizeof(double2) * vol));
+      HIPCHECK(hipMemcpy(sv.d_box, dense.data(), sizeof(double2) * vol, hipMemcpyHostToDevice));
+    }
+    sv.dirty = false;
+  }
+  void apply_long(const LongSrc &ls, R *fbase, int cmp) {
     kbegin(MP_K_SRC);
-    launch_src<R>(fbase, m.d_idx, dv, int(vals.size()), stream);
+    launch_src<R>(fbase, ls.sv->d_idx, ls.sv->d_amp, int(ls.sv->index.size()), make_double2(ls.scal.real(), ls.scal.imag()),
+                  ls.mult, cmp, stream);
     kend();
   }
 
@@ -1128,52 +1297,21 @@ template <class R> struct Fields : FieldsBase {
           if (needs_P(p, c, cmp)) have_fmp_E = true;
 
     if (!plan_valid) rebuild_plan();
-    StepArgs<R> A;
-    memset(&A, 0, sizeof A);
-    A.g = grid();
-    A.zwrap = zwrap_ok ? 1 : 0;
-    A.zsign = zsign;
-    for (int d = 0; d < 3; ++d) {
-      A.pml.sig[d] = d_sig[d];
-      A.pml.kap[d] = d_kap[d];
-      A.pml.siginv[d] = d_siginv[d];
-    }
-    // --- B, then H = mu^-1 B where H is its own array ---
-    for (int cmp = 0; cmp < ncmp(); ++cmp) {
-      cur_cmp = cmp;
-      int which[3];
-      A.nc = curl_comps(B_ft, cmp, A.cc, which);
-      std::vector<const SrcVol *> big;
-      fill_tab(B_ft, cmp, which, A.nc, false, tB, A.cur, big);
-      A.dip.n = 0;
-      A.npole = 0;
-      if (big.empty() && !upd_pending(cmp, which, A.nc, false)) {
-        const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
-        step_kernel(A, true, nu > 0, MP_K_STEP_BH);
-        continue;
-      }
-      step_kernel(A, true, false, MP_K_STEP_BH);
-      for (const SrcVol *sv : big) apply_big(*sv, f[dir_comp(15, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].current(tB, dt), dt);
-      StepArgs<R> U = A;
-      U.cur.n = 0;
-      if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) step_kernel(U, false, true, MP_K_STEP_BH);
-    }
-    if (!plan_valid) rebuild_plan();
-    boundaries(0); // B, H (step_boundaries B_stuff .. H_stuff)
+    StepArgs<R> A = base_args();
+    phase_B(A, tB);
     // --- D, then E = chi1inv (D - P - dipoles) and the polarisations ---
     alloc_pols();
     if (!plan_valid) rebuild_plan();
-    A.zwrap = zwrap_ok ? 1 : 0;
-    A.zsign = zsign;
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
       cur_cmp = cmp;
       int which[3], whiche[3];
       A.nc = curl_comps(D_ft, cmp, A.cc, which);
-      std::vector<const SrcVol *> bigc, bigd;
-      fill_tab(D_ft, cmp, which, A.nc, false, tH, A.cur, bigc);
+      std::vector<LongSrc> bigc, bigd;
+      A.cmp = cmp;
+      fill_tab(D_ft, cmp, which, A.nc, false, tH, A.cur, A.curb, bigc);
       upd_comps(E_ft, cmp, which, A.nc, A.uc, have_fmp_E);
       for (int q = 0; q < A.nc; ++q) whiche[q] = which[q] < 0 ? -1 : which[q] - 10;
-      fill_tab(D_ft, cmp, whiche, A.nc, true, tE, A.dip, bigd);
+      fill_tab(D_ft, cmp, whiche, A.nc, true, tE, A.dip, A.dipb, bigd);
       A.npole = int(pols.size());
       for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
       if (bigc.empty() && bigd.empty()) {
@@ -1181,24 +1319,30 @@ template <class R> struct Fields : FieldsBase {
         continue;
       }
       step_kernel(A, true, false, MP_K_STEP_DE);
-      for (const SrcVol *sv : bigc) apply_big(*sv, f[dir_comp(10, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].current(tH, dt), dt);
-      // long integrated lists: f_minus_p formed in place in D around the update, then D restored
-      std::vector<std::pair<R *, int64_t>> saved;
-      for (const SrcVol *sv : bigd) {
-        R *D = f[dir_comp(10, comp_dir(sv->c))][cmp];
-        for (int64_t ix : sv->index) saved.push_back({D, ix});
+      for (const LongSrc &ls : bigc) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp);
+      // integrated lists beyond the boxes: f_minus_p formed in place in D around the update,
+      // then those points of D restored.  Lists may share points (a periodic plane and its
+      // image), so every list is saved before any is applied.
+      size_t nsave = 0;
+      for (const LongSrc &ls : bigd) nsave += ls.sv->index.size();
+      if (nsave > d_scratch.bytes / sizeof(R)) d_scratch.alloc(sizeof(R) * nsave);
+      size_t off = 0;
+      for (const LongSrc &ls : bigd) {
+        R *D = f[dir_comp(10, comp_dir(ls.sv->c))][cmp];
+        launch_gather<R>(static_cast<R *>(d_scratch.p) + off, D, ls.sv->d_idx, int(ls.sv->index.size()), stream);
+        off += ls.sv->index.size();
       }
-      if (!saved.empty()) {
-        d_scratch.alloc(sizeof(R) * saved.size());
-        for (size_t q = 0; q < saved.size(); ++q)
-          HIPCHECK(hipMemcpyAsync(static_cast<R *>(d_scratch.p) + q, saved[q].first + saved[q].second, sizeof(R), hipMemcpyDeviceToDevice, stream));
-        for (const SrcVol *sv : bigd) apply_big(*sv, f[dir_comp(10, comp_dir(sv->c))][cmp], cmp, srcs[sv->src].dipole(tE), 1.0);
-      }
+      for (const LongSrc &ls : bigd) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp);
       StepArgs<R> U = A;
       U.cur.n = 0;
+      U.curb.n = 0;
       step_kernel(U, false, true, MP_K_STEP_DE);
-      for (size_t q = 0; q < saved.size(); ++q)
-        HIPCHECK(hipMemcpyAsync(saved[q].first + saved[q].second, static_cast<R *>(d_scratch.p) + q, sizeof(R), hipMemcpyDeviceToDevice, stream));
+      off = 0;
+      for (const LongSrc &ls : bigd) {
+        R *D = f[dir_comp(10, comp_dir(ls.sv->c))][cmp];
+        launch_scatter<R>(D, ls.sv->d_idx, static_cast<const R *>(d_scratch.p) + off, int(ls.sv->index.size()), stream);
+        off += ls.sv->index.size();
+      }
     }
     if (!plan_valid) rebuild_plan();
     boundaries(1); // D, P_E, E (step_boundaries D_stuff .. E_stuff)
@@ -1207,6 +1351,134 @@ template <class R> struct Fields : FieldsBase {
     if (probe && n_chk)
       launch_check<R>(static_cast<const R *const *>(d_chk_ptrs.p), static_cast<const int64_t *>(d_chk_idx.p),
                       static_cast<const double *>(d_chk_w.p), n_chk, static_cast<int *>(d_flag.p), stream);
+  }
+
+  StepArgs<R> base_args() const {
+    StepArgs<R> A;
+    memset(&A, 0, sizeof A);
+    A.g = grid();
+    for (int d = 0; d < 3; ++d) {
+      A.pml.sig[d] = d_sig[d];
+      A.pml.kap[d] = d_kap[d];
+      A.pml.siginv[d] = d_siginv[d];
+    }
+    return A;
+  }
+  // B = B - curl E (+ B sources at tB), then H = mu^-1 B where H is its own array, then the
+  // B and H halos: step_db(B_stuff), step_source(B_stuff), step_boundaries(B_stuff),
+  // update_eh(H_stuff), step_boundaries(H_stuff) (src/step.cpp:64-80)
+  void phase_B(StepArgs<R> &A, double tB) {
+    for (int cmp = 0; cmp < ncmp(); ++cmp) {
+      cur_cmp = cmp;
+      int which[3];
+      A.nc = curl_comps(B_ft, cmp, A.cc, which);
+      std::vector<LongSrc> big;
+      A.cmp = cmp;
+      fill_tab(B_ft, cmp, which, A.nc, false, tB, A.cur, A.curb, big);
+      A.dip.n = 0;
+      A.dipb.n = 0;
+      A.npole = 0;
+      if (big.empty() && !upd_pending(cmp, which, A.nc, false)) {
+        const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
+        step_kernel(A, true, nu > 0, MP_K_STEP_BH);
+        continue;
+      }
+      step_kernel(A, true, false, MP_K_STEP_BH);
+      for (const LongSrc &ls : big) apply_long(ls, f[dir_comp(15, comp_dir(ls.sv->c))][cmp], cmp);
+      StepArgs<R> U = A;
+      U.cur.n = 0;
+      U.curb.n = 0;
+      if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) step_kernel(U, false, true, MP_K_STEP_BH);
+    }
+    if (!plan_valid) rebuild_plan();
+    boundaries(0); // B, H (step_boundaries B_stuff .. H_stuff)
+  }
+
+  // ---- field energy (src/energy_and_flux.cpp:45-178, src/integrate.cpp:46-205) ----
+  // 0.5 * integral of Re(conj(F) G) over the cell for F = E (H) and G = D (B), per component,
+  // in FP64 on the host from read-back arrays: E and D share a grid, so each component is
+  // integrated on its own grid (the reference's four-point average has zero offsets there).
+  long double energy_of(int first) {
+    V3 wmin, wmax;
+    for (int d = 0; d < 3; ++d) wmax.t[d] = gv.has(d) ? gv.num[d] * gv.inva : 0.0;
+    long double sum = 0;
+    for (int c = first; c < first + 5; ++c) {
+      if (!has_field(c) || !f[c][0]) continue;
+      const int c2 = c + 10;
+      std::vector<R> h[2][2];
+      for (int q = 0; q < 2; ++q)
+        for (int k = 0; k < 2; ++k) {
+          const R *src = f[q ? c2 : c][k];
+          if (!src) continue;
+          h[q][k].resize(gv.ntot);
+          HIPCHECK(hipMemcpy(h[q][k].data(), src, sizeof(R) * gv.ntot, hipMemcpyDeviceToHost));
+        }
+      cdouble total = 0;
+      loop_in_chunks(wmin, wmax, c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1,
+                                        double dV0, const IV &, cdouble sph) {
+        std::complex<long double> part = 0;
+        visit_box(is, ie, s0, s1, e0, e1, dV0, [&](int64_t idx, int64_t, const IV &, double w, const int *) {
+          std::complex<R> fv[2];
+          for (int q = 0; q < 2; ++q) {
+            double fr[2] = {0, 0};
+            for (int k = 0; k < 2; ++k)
+              if (!h[q][k].empty()) {
+                const R a = h[q][k][idx];
+                fr[k] = 0.25 * (a + a + a + a);
+              }
+            fv[q] = std::complex<R>(cdouble(fr[0], fr[1]) * sph);
+          }
+          const cdouble integrand = std::real(std::conj(cdouble(fv[0])) * cdouble(fv[1]));
+          part += std::complex<long double>(integrand * w);
+        });
+        total += cdouble(part);
+      });
+      sum += std::real(total) * 0.5;
+    }
+    return sum;
+  }
+  // fields::field_energy with synchronize_magnetic_fields / restore_magnetic_fields
+  // (src/energy_and_flux.cpp:46-62,146-178): one more B phase, B and H averaged with their
+  // values half a step earlier, the magnetic energy taken, every magnetic array restored.
+  double field_energy() override {
+    if (!plan_valid) rebuild_plan();
+    HIPCHECK(hipStreamSynchronize(stream));
+    struct Saved {
+      R *p;
+      bool average;
+      std::unique_ptr<DevBuf> b;
+    };
+    std::vector<Saved> saved;
+    auto save = [&](R *p, bool average) {
+      if (!p) return;
+      Saved sv{p, average, std::unique_ptr<DevBuf>(new DevBuf)};
+      sv.b->alloc(sizeof(R) * gv.ntot);
+      HIPCHECK(hipMemcpyAsync(sv.b->p, p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+      saved.push_back(std::move(sv));
+    };
+    for (int cmp = 0; cmp < 2; ++cmp) {
+      for (int c = 15; c < 20; ++c) {
+        save(f[c][cmp], true);
+        save(f_u[c][cmp], false);
+        save(f_w[c][cmp], false);
+      }
+      for (int c = 5; c < 10; ++c)
+        if (f[c][cmp] && f[c][cmp] != f[c + 10][cmp]) {
+          save(f[c][cmp], true);
+          save(f_u[c][cmp], false);
+          save(f_w[c][cmp], false);
+        }
+    }
+    StepArgs<R> A = base_args();
+    phase_B(A, t * dt);
+    for (const Saved &sv : saved)
+      if (sv.average) launch_average<R>(sv.p, static_cast<const R *>(sv.b->p), gv.ntot, stream);
+    HIPCHECK(hipStreamSynchronize(stream));
+    const double magnetic = double(energy_of(5));
+    for (const Saved &sv : saved)
+      HIPCHECK(hipMemcpyAsync(sv.p, sv.b->p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    return double(energy_of(0)) + magnetic;
   }
 
   void step_n(long n, bool) override {
@@ -1447,6 +1719,7 @@ template <class R> struct Fields : FieldsBase {
     loop_in_chunks(wmin, wmax, c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double, const IV &shift, cdouble sph) {
       std::vector<int64_t> index;
       std::vector<cdouble> amps;
+      std::vector<int> jpos;
       const cdouble am = damp * std::conj(sph);
       visit_box(is, ie, s0, s1, e0, e1, 1.0, [&](int64_t idx, int64_t, const IV &p, double w, const int *) {
         if (!gv.owns(p)) return;
@@ -1460,6 +1733,7 @@ template <class R> struct Fields : FieldsBase {
         }
         amps.push_back(w * am * Av);
         index.push_back(idx);
+        for (int d = 0; d < 3; ++d) jpos.push_back(gv.has(d) ? (p.t[d] - gv.ysh(c, d)) / 2 : 0);
       });
       for (int64_t i : index)
         if (i < 0 || size_t(i) >= gv.ntot) fail("internal: source point outside the array");
@@ -1468,6 +1742,7 @@ template <class R> struct Fields : FieldsBase {
       for (SrcVol &o : sources[ft])
         if (o.c == cc && o.src == isrc && o.index == index) {
           for (size_t q = 0; q < o.amp.size(); ++q) o.amp[q] += amps[q];
+          o.dirty = true;
           return;
         }
       SrcVol sv;
@@ -1475,6 +1750,7 @@ template <class R> struct Fields : FieldsBase {
       sv.src = isrc;
       sv.index = std::move(index);
       sv.amp = std::move(amps);
+      sv.jpos = std::move(jpos);
       sources[ft].push_back(std::move(sv));
     });
     require_component(c);
@@ -1560,6 +1836,30 @@ template <class R> struct Fields : FieldsBase {
     fluxes.push_back(std::move(fl));
     return fluxes.back().get();
   }
+  // fields::add_dft_fields (src/dft.cpp:913-927): weights off, stored weight 1, centred grid;
+  // each component's chunks go to the front of the list as the reference prepends them
+  void *add_dft_fields(const int *comps, int ncomp, V3 mn, V3 mxv, const std::vector<double> &freq, int dec, int *dec_out) override {
+    if (!components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+    const int decim = dec > 0 ? dec : auto_decimation(freq);
+    *dec_out = decim;
+    std::unique_ptr<Flux<R>> fl(new Flux<R>);
+    fl->nfreq = int(freq.size());
+    for (int q = 0; q < ncomp; ++q) {
+      if (comps[q] < 0 || comps[q] >= NC) fail("invalid dft component %d", comps[q]);
+      std::vector<DftChunk<R> *> lst;
+      add_dft(lst, comps[q], mn, mxv, freq, false, cdouble(1.0, 0), decim);
+      std::vector<DftChunk<R> *> front(lst.rbegin(), lst.rend());
+      fl->E.insert(fl->E.begin(), front.begin(), front.end());
+    }
+    fluxes.push_back(std::move(fl));
+    return fluxes.back().get();
+  }
+  int flux_chunk_component(void *flp, int which, int i) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    auto &lst = which ? fl->H : fl->E;
+    if (i < 0 || i >= int(lst.size())) fail("no such dft chunk");
+    return lst[i]->c;
+  }
   std::vector<cdouble> read_dft(const DftChunk<R> &ch) {
     HIPCHECK(hipStreamSynchronize(stream));
     const size_t n = size_t(ch.npts) * ch.nfreq;
@@ -1568,9 +1868,12 @@ template <class R> struct Fields : FieldsBase {
       HIPCHECK(hipMemcpy(out.data(), ch.acc.p, 16 * n, hipMemcpyDeviceToHost));
     }
     else if (ch.acc_kind == MP_DFT_KAHAN32) {
-      std::vector<float> v(2 * n);
+      // compensated sum: the running sum less the carried compensation term
+      std::vector<float> v(2 * n), c(2 * n);
       HIPCHECK(hipMemcpy(v.data(), ch.acc.p, 8 * n, hipMemcpyDeviceToHost));
-      for (size_t q = 0; q < n; ++q) out[q] = cdouble(v[2 * q], v[2 * q + 1]);
+      HIPCHECK(hipMemcpy(c.data(), ch.comp.p, 8 * n, hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < n; ++q)
+        out[q] = cdouble(double(v[2 * q]) - double(c[2 * q]), double(v[2 * q + 1]) - double(c[2 * q + 1]));
     }
     else {
       std::vector<R> v(2 * n);
@@ -1931,6 +2234,9 @@ double mp_fields_round_time(const mp_fields *f) {
 long mp_fields_t(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->t_now(); }
 double mp_fields_dt(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->dt_now(); }
 double mp_fields_last_source_time(mp_fields *f) { return F_->last_source_time(); }
+int mp_fields_field_energy(mp_fields *f, double *energy) {
+  return guard([&] { *energy = F_->field_energy(); });
+}
 int mp_fields_get_field(mp_fields *f, int c, const double *p, double *re, double *im) {
   return guard([&] {
     cdouble v = F_->get_field(c, to_v3(p));
@@ -2001,6 +2307,34 @@ int mp_dft_flux_decimation(mp_dft_flux *flp) { return reinterpret_cast<FluxHandl
 int mp_dft_flux_get_dft(mp_dft_flux *flp, int which, int i, double *out) {
   FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
   return guard([&] { h->owner->flux_dft(h->fl, which, i, out); });
+}
+
+mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *comps, int ncomp, const double *vmin, const double *vmax,
+                                        const double *freq, int nfreq, int dec) {
+  FluxHandle *h = new FluxHandle;
+  h->owner = reinterpret_cast<FieldsBase *>(f);
+  h->nfreq = nfreq;
+  std::vector<double> fr(freq, freq + nfreq);
+  int rc = guard([&] { h->fl = h->owner->add_dft_fields(comps, ncomp, to_v3(vmin), to_v3(vmax), fr, dec, &h->decimation); });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_dft_fields *>(h);
+}
+void mp_dft_fields_delete(mp_dft_fields *df) { mp_dft_flux_delete(reinterpret_cast<mp_dft_flux *>(df)); }
+int mp_dft_fields_num_chunks(mp_dft_fields *df) { return mp_dft_flux_num_chunks(reinterpret_cast<mp_dft_flux *>(df), 0); }
+long mp_dft_fields_chunk_points(mp_dft_fields *df, int i) {
+  return mp_dft_flux_chunk_points(reinterpret_cast<mp_dft_flux *>(df), 0, i);
+}
+int mp_dft_fields_chunk_component(mp_dft_fields *df, int i) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(df);
+  int c = -1;
+  if (guard([&] { c = h->owner->flux_chunk_component(h->fl, 0, i); })) return -1;
+  return c;
+}
+int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out) {
+  return mp_dft_flux_get_dft(reinterpret_cast<mp_dft_flux *>(df), 0, i, out);
 }
 
 } // extern "C"

@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace mx {
 
 constexpr int BLOCK = 256;
@@ -34,36 +36,65 @@ __device__ __forceinline__ uint32_t xcd_block(int rev = 0) {
   return rev ? nb - 1 - lb : lb;
 }
 
-// halo point along the stride-1 direction fed by the owned point z of a component with Yee
-// shift ysh there; -1 when z is not the wrapped end
+// halo point along a folded axis fed by the owned point z of a component with Yee shift ysh
+// there; -1 when z is not the wrapped end
 __device__ __forceinline__ int zwrap_dst(int ysh, int z, int n) {
   if (ysh == 0) return z == n ? 0 : -1;
   return z == 0 ? n : -1;
 }
 
-__device__ __forceinline__ int jsel(const int j[3], int d) { return d == 0 ? j[0] : d == 1 ? j[1] : j[2]; }
-__device__ __forceinline__ int ysel(const int y[3], int d) { return d == 0 ? y[0] : d == 1 ? y[1] : y[2]; }
-template <class R> __device__ __forceinline__ const R *psel(const R *const p[3], int d) {
-  return d == 0 ? p[0] : d == 1 ? p[1] : p[2];
-}
-
-// on_metal_boundary for an owned point (src/boundaries.cpp:191-205): owned points reach the
-// High wall only with no Yee shift along d, and the Low magnetic plane only with one.
-__device__ __forceinline__ bool on_metal(const Grid &g, const int j[3], const int ysh[3]) {
+// Periodic images of the owned point (j0, j1, j2) along the folded axes: offset of each axis's
+// image (0 if none) and its sign.
+struct Images {
+  int64_t o[3];
+  int s[3];
+  bool any;
+};
+__device__ __forceinline__ Images images_of(const Grid &g, const AxisSlot &ax, int j0, int j1, int j2) {
+  Images m;
+  const int j[3] = {j0, j1, j2};
+  m.any = false;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    if (((g.hi_metal >> d) & 1) && ysh[d] == 0 && j[d] == g.num[d]) return true;
-    if (((g.lo_magnetic >> d) & 1) && ysh[d] == 1 && j[d] == 0) return true;
+  for (int k = 0; k < 3; ++k) {
+    const int d = g.wrap[k] ? zwrap_dst(ax.ysh[k], j[k], g.nax[k]) : -1;
+    m.o[k] = d >= 0 ? int64_t(d - j[k]) * g.ast[k] : 0;
+    m.s[k] = d >= 0 ? g.wrap[k] : 0;
+    m.any = m.any || d >= 0;
   }
-  return false;
+  return m;
+}
+template <class R> __device__ __forceinline__ R signed_value(R v, int s) { return s > 0 ? v : -v; }
+// write v at i and at every combination of its images
+template <class R> __device__ __forceinline__ void put(R *f, uint32_t i, R v, const Grid &g, const AxisSlot &ax, int j0, int j1, int j2) {
+  f[i] = v;
+  if (!(g.wrap[0] | g.wrap[1] | g.wrap[2])) return;
+  const Images m = images_of(g, ax, j0, j1, j2);
+  if (!m.any) return;
+#pragma unroll
+  for (int sub = 1; sub < 8; ++sub) {
+    int64_t off = 0;
+    int sg = 1;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if ((sub >> k) & 1) {
+        ok = ok && m.s[k] != 0;
+        off += m.o[k];
+        sg *= m.s[k];
+      }
+    if (ok) f[int64_t(i) + off] = signed_value(v, sg);
+  }
 }
 
-template <class R> __device__ __forceinline__ bool pml_active(const PmlProfile<R> &p, int d, int k) {
-  return psel(p.sig, d)[k] != R(0) || psel(p.kap, d)[k] != R(1);
-}
+// coordinate along loop axis k (a select over registers, never an indexed access)
+__device__ __forceinline__ int asel(int j0, int j1, int j2, int k) { return k == 0 ? j0 : (k == 1 ? j1 : j2); }
 
-__device__ __forceinline__ bool owned(const int j[3], const int lo[3], const int hi[3]) {
-  return j[0] >= lo[0] && j[0] <= hi[0] && j[1] >= lo[1] && j[1] <= hi[1] && j[2] >= lo[2] && j[2] <= hi[2];
+__device__ __forceinline__ bool slot_owned(const AxisSlot &s, int j0, int j1, int j2) {
+  return j0 >= s.lo[0] && j0 <= s.hi[0] && j1 >= s.lo[1] && j1 <= s.hi[1] && j2 >= s.lo[2] && j2 <= s.hi[2];
+}
+// on_metal_boundary for an owned point (src/boundaries.cpp:191-205)
+__device__ __forceinline__ bool slot_metal(const AxisSlot &s, int j0, int j1, int j2) {
+  return j0 == s.mhi[0] || j0 == s.mlo[0] || j1 == s.mhi[1] || j1 == s.mlo[1] || j2 == s.mhi[2] || j2 == s.mlo[2];
 }
 
 __device__ __forceinline__ bool in_tab(const SrcTab &t, uint32_t i) {
@@ -76,24 +107,49 @@ template <class R> __device__ __forceinline__ R apply_tab(const SrcTab &t, int c
   return v;
 }
 
+// value of box source S at loop-axis indices (j0, j1, j2); contraction is off so the complex
+// product rounds as the host's does
+template <int S> __device__ __forceinline__ double box_value(const SrcBoxes &b, int cmp, int j0, int j1, int j2) {
+#pragma clang fp contract(off)
+  const int64_t k = int64_t(j0 - b.lo[S][0]) * b.bst[S][0] + int64_t(j1 - b.lo[S][1]) * b.bst[S][1] +
+                    int64_t(j2 - b.lo[S][2]) * b.bst[S][2];
+  const double2 am = b.amp[S][k], sc = b.scal[S];
+  const double re = am.x * sc.x - am.y * sc.y, im = am.x * sc.y + am.y * sc.x;
+  return cmp ? im * b.mult[S] : re * b.mult[S];
+}
+template <int S> __device__ __forceinline__ bool in_box(const SrcBoxes &b, int c, int j0, int j1, int j2) {
+  return S < b.n && b.comp[S] == c && j0 >= b.lo[S][0] && j0 <= b.hi[S][0] && j1 >= b.lo[S][1] && j1 <= b.hi[S][1] &&
+         j2 >= b.lo[S][2] && j2 <= b.hi[S][2];
+}
+template <class R> __device__ __forceinline__ R apply_boxes(const SrcBoxes &b, int cmp, int c, int j0, int j1, int j2, R v) {
+  static_assert(SRC_BOX == 2, "one term per box");
+  if (in_box<0>(b, c, j0, j1, j2)) v = R(double(v) - box_value<0>(b, cmp, j0, j1, j2));
+  if (in_box<1>(b, c, j0, j1, j2)) v = R(double(v) - box_value<1>(b, cmp, j0, j1, j2));
+  return v;
+}
+
 // ---------------------------------------------------------------------------------------
 // generic kernel: one point per thread, full uPML logic, over the launch box
 
-template <class R, bool CURL, bool UPD>
-__device__ __forceinline__ void cell_update(const StepArgs<R> &a, int c, uint32_t i, const int j[3], R cu) {
+template <class R> __device__ __forceinline__ bool pml_on(const R *sig, const R *kap, int k) {
+  return sig && (sig[k] != R(0) || kap[k] != R(1));
+}
+
+template <class R, bool CURL, bool UPD, int C>
+__device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, int j0, int j1, int j2, R cu) {
   R fv = R(0);
   if (CURL) {
-    const CurlComp<R> &k = a.cc[c];
+    const CurlComp<R> &k = a.cc[C];
     fv = k.f[i];
-    const int kk = k.dsig >= 0 ? 2 * jsel(j, k.dsig) + ysel(k.ysh, k.dsig) : 0;
-    const int ku = k.dsigu >= 0 ? 2 * jsel(j, k.dsigu) + ysel(k.ysh, k.dsigu) : 0;
-    const bool pml = k.dsig >= 0 && pml_active(a.pml, k.dsig, kk);
-    const bool pmlu = k.dsigu >= 0 && pml_active(a.pml, k.dsigu, ku);
+    const int kk = k.ksig >= 0 ? 2 * asel(j0, j1, j2, k.ksig) + k.ysig : 0;
+    const int ku = k.ksigu >= 0 ? 2 * asel(j0, j1, j2, k.ksigu) + k.ysigu : 0;
+    const bool pml = pml_on(k.sig, k.kap, kk);
+    const bool pmlu = pml_on(k.sigu, k.kapu, ku);
     if (!pmlu) {
       if (!pml)
         fv -= k.dtdx * cu;
       else
-        fv = ((psel(a.pml.kap, k.dsig)[kk] - psel(a.pml.sig, k.dsig)[kk]) * fv - k.dtdx * cu) * psel(a.pml.siginv, k.dsig)[kk];
+        fv = ((k.kap[kk] - k.sig[kk]) * fv - k.dtdx * cu) * k.siginv[kk];
     }
     else {
       const R fprev = k.fu[i];
@@ -101,30 +157,27 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, int c, uint32_
       if (!pml)
         fun = fprev - k.dtdx * cu;
       else
-        fun = ((psel(a.pml.kap, k.dsig)[kk] - psel(a.pml.sig, k.dsig)[kk]) * fprev - k.dtdx * cu) * psel(a.pml.siginv, k.dsig)[kk];
+        fun = ((k.kap[kk] - k.sig[kk]) * fprev - k.dtdx * cu) * k.siginv[kk];
       k.fu[i] = fun;
-      fv = psel(a.pml.siginv, k.dsigu)[ku] * ((psel(a.pml.kap, k.dsigu)[ku] - psel(a.pml.sig, k.dsigu)[ku]) * fv + fun - fprev);
+      fv = k.siginvu[ku] * ((k.kapu[ku] - k.sigu[ku]) * fv + fun - fprev);
     }
-    if (in_tab(a.cur, i)) fv = apply_tab(a.cur, c, i, fv);
-    if (on_metal(a.g, j, k.ysh)) fv = R(0);
-    k.f[i] = fv;
-    if (a.zwrap) {
-      const int d2 = a.g.axdir[2], z = jsel(j, d2);
-      const int zd = zwrap_dst(ysel(k.ysh, d2), z, a.g.num[d2]);
-      if (zd >= 0) k.f[i - z + zd] = a.zsign > 0 ? fv : -fv;
-    }
+    if (in_tab(a.cur, i)) fv = apply_tab(a.cur, C, i, fv);
+    if (a.curb.n) fv = apply_boxes(a.curb, a.cmp, C, j0, j1, j2, fv);
+    if (slot_metal(k.ax, j0, j1, j2)) fv = R(0);
+    put(k.f, i, fv, a.g, k.ax, j0, j1, j2);
   }
-  if (UPD && a.uc[c].f) {
-    const UpdComp<R> &u = a.uc[c];
+  if (UPD && a.uc[C].f) {
+    const UpdComp<R> &u = a.uc[C];
     R gv = CURL ? fv : u.g[i];
 #pragma unroll
     for (int p = 0; p < MAXPOLE; ++p)
       if (p < a.npole && u.P[p]) gv -= u.P[p][i];
-    if (in_tab(a.dip, i)) gv = apply_tab(a.dip, c, i, gv);
+    if (in_tab(a.dip, i)) gv = apply_tab(a.dip, C, i, gv);
+    if (a.dipb.n) gv = apply_boxes(a.dipb, a.cmp, C, j0, j1, j2, gv);
     R ev, wv;
-    const int kw = u.dsigw >= 0 ? 2 * jsel(j, u.dsigw) + ysel(u.ysh, u.dsigw) : 0;
-    if (u.dsigw >= 0 && pml_active(a.pml, u.dsigw, kw)) {
-      const R kapw = psel(a.pml.kap, u.dsigw)[kw], sigw = psel(a.pml.sig, u.dsigw)[kw];
+    const int kw = u.ksigw >= 0 ? 2 * asel(j0, j1, j2, u.ksigw) + u.ysigw : 0;
+    if (pml_on(u.sigw, u.kapw, kw)) {
+      const R kapw = u.kapw[kw], sigw = u.sigw[kw];
       const R fwprev = u.fw[i];
       const R fwn = u.u ? gv * u.u[i] : gv;
       u.fw[i] = fwn;
@@ -142,22 +195,30 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, int c, uint32_
         const R pcur = u.P[p][i];
         const R pn = q.gamma1inv * (pcur * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * u.Pp[p][i] +
                                     q.omega0dtsqr * (u.sigma[p][i] * wv));
-        u.P[p][i] = pn;
+        put(u.P[p], i, pn, a.g, u.ax, j0, j1, j2);
         u.Pp[p][i] = pcur;
-        if (a.zwrap) {
-          const int d2 = a.g.axdir[2], z = jsel(j, d2);
-          const int zd = zwrap_dst(ysel(u.ysh, d2), z, a.g.num[d2]);
-          if (zd >= 0) u.P[p][i - z + zd] = a.zsign > 0 ? pn : -pn;
-        }
       }
-    if (on_metal(a.g, j, u.ysh)) ev = R(0);
-    u.f[i] = ev;
-    if (a.zwrap) {
-      const int d2 = a.g.axdir[2], z = jsel(j, d2);
-      const int zd = zwrap_dst(ysel(u.ysh, d2), z, a.g.num[d2]);
-      if (zd >= 0) u.f[i - z + zd] = a.zsign > 0 ? ev : -ev;
-    }
+    if (slot_metal(u.ax, j0, j1, j2)) ev = R(0);
+    put(u.f, i, ev, a.g, u.ax, j0, j1, j2);
   }
+}
+
+// one component slot C (a template argument, so every access to the slot's parameters has a
+// constant offset)
+template <class R, bool CURL, bool UPD, int C>
+__device__ __forceinline__ void step_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int j2) {
+  if (C >= a.nc) return;
+  if (CURL ? !a.cc[C].f : !a.uc[C].f) return; // no component along this direction
+  if (!slot_owned(CURL ? a.cc[C].ax : a.uc[C].ax, j0, j1, j2)) return;
+  R cu = R(0);
+  if (CURL) {
+    const CurlComp<R> &k = a.cc[C];
+    if (k.g2)
+      cu = k.g1[i + k.s1] - k.g1[i] + k.g2[i] - k.g2[i + k.s2];
+    else if (k.g1)
+      cu = k.g1[i + k.s1] - k.g1[i];
+  }
+  cell_update<R, CURL, UPD, C>(a, i, j0, j1, j2, cu);
 }
 
 template <class R, bool CURL, bool UPD>
@@ -165,34 +226,15 @@ __global__ __launch_bounds__(BLOCK) void k_step(const StepArgs<R> a, uint32_t co
   const uint32_t t = xcd_block() * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
   const Grid &g = a.g;
-  int e[3], j[3] = {0, 0, 0};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) e[k] = g.axdir[k] >= 0 ? g.bhi[g.axdir[k]] - g.blo[g.axdir[k]] + 1 : 1;
-  const uint32_t r = t / uint32_t(e[2]);
-  const int q2 = int(t - r * uint32_t(e[2]));
-  const int q0 = int(r / uint32_t(e[1]));
-  const int q1 = int(r - uint32_t(q0) * uint32_t(e[1]));
-  if (g.axdir[0] >= 0) j[g.axdir[0]] = g.blo[g.axdir[0]] + q0;
-  if (g.axdir[1] >= 0) j[g.axdir[1]] = g.blo[g.axdir[1]] + q1;
-  if (g.axdir[2] >= 0) j[g.axdir[2]] = g.blo[g.axdir[2]] + q2;
-  const uint32_t i = uint32_t(j[0] * g.st[0] + j[1] * g.st[1] + j[2] * g.st[2]);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (c >= a.nc) break;
-    if (CURL ? !a.cc[c].f : !a.uc[c].f) continue; // no component along this direction
-    const int *lo = CURL ? a.cc[c].lo : a.uc[c].lo;
-    const int *hi = CURL ? a.cc[c].hi : a.uc[c].hi;
-    if (!owned(j, lo, hi)) continue;
-    R cu = R(0);
-    if (CURL) {
-      const CurlComp<R> &k = a.cc[c];
-      if (k.g2)
-        cu = k.g1[i + k.s1] - k.g1[i] + k.g2[i] - k.g2[i + k.s2];
-      else if (k.g1)
-        cu = k.g1[i + k.s1] - k.g1[i];
-    }
-    cell_update<R, CURL, UPD>(a, c, i, j, cu);
-  }
+  const uint32_t r = t / uint32_t(g.an[2]);
+  const int q2 = int(t - r * uint32_t(g.an[2]));
+  const int q0 = int(r / uint32_t(g.an[1]));
+  const int q1 = int(r - uint32_t(q0) * uint32_t(g.an[1]));
+  const int j0 = g.alo[0] + q0, j1 = g.alo[1] + q1, j2 = g.alo[2] + q2;
+  const uint32_t i = uint32_t(j0 * g.ast[0] + j1 * g.ast[1] + j2 * g.ast[2]);
+  step_slot<R, CURL, UPD, 0>(a, i, j0, j1, j2);
+  step_slot<R, CURL, UPD, 1>(a, i, j0, j1, j2);
+  step_slot<R, CURL, UPD, 2>(a, i, j0, j1, j2);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -242,28 +284,156 @@ __device__ __forceinline__ Vec<R, V> neighbour(const R *g, uint32_t i, int64_t s
   return vload<R, V>(g + (int64_t(i) + s));
 }
 
-// the periodic image of an owned end point of the row, written into the row's halo point
+template <class R, int V> __device__ __forceinline__ Vec<R, V> signed_vec(const Vec<R, V> &x, int s) {
+  Vec<R, V> r;
+#pragma unroll
+  for (int k = 0; k < V; ++k) r.v[k] = signed_value(x.v[k], s);
+  return r;
+}
+
+// store the owned points of a group and their periodic images: whole-row images along loop
+// axes 0 and 1 (uniform over the row), single-point images along the row's own axis
 template <class R, int V>
-__device__ __forceinline__ void wrap_store(R *f, uint32_t i, int z0, int ysh, int n, int sign, const Vec<R, V> &x, unsigned mask) {
+__device__ __forceinline__ void put_row(R *f, uint32_t i, const Vec<R, V> &x, unsigned mask, const Grid &g, const AxisSlot &ax,
+                                        int j0, int j1, int z0) {
+  vstore<R, V>(f + i, x, mask);
+  if (!(g.wrap[0] | g.wrap[1] | g.wrap[2])) return;
+  const int e0 = g.wrap[0] ? zwrap_dst(ax.ysh[0], j0, g.nax[0]) : -1;
+  const int e1 = g.wrap[1] ? zwrap_dst(ax.ysh[1], j1, g.nax[1]) : -1;
+  const int64_t o0 = e0 >= 0 ? int64_t(e0 - j0) * g.ast[0] : 0, o1 = e1 >= 0 ? int64_t(e1 - j1) * g.ast[1] : 0;
+  if (e0 >= 0) vstore<R, V>(f + (int64_t(i) + o0), signed_vec(x, g.wrap[0]), mask);
+  if (e1 >= 0) vstore<R, V>(f + (int64_t(i) + o1), signed_vec(x, g.wrap[1]), mask);
+  if (e0 >= 0 && e1 >= 0) vstore<R, V>(f + (int64_t(i) + o0 + o1), signed_vec(x, g.wrap[0] * g.wrap[1]), mask);
+  if (!g.wrap[2]) return;
 #pragma unroll
   for (int k = 0; k < V; ++k)
     if ((mask >> k) & 1) {
-      const int zd = zwrap_dst(ysh, z0 + k, n);
-      if (zd >= 0) f[i - z0 + zd] = sign > 0 ? x.v[k] : -x.v[k];
+      const int zd = zwrap_dst(ax.ysh[2], z0 + k, g.nax[2]);
+      if (zd < 0) continue;
+      const int64_t b = int64_t(i) + k + (zd - (z0 + k));
+      const R v = x.v[k];
+      f[b] = signed_value(v, g.wrap[2]);
+      if (e0 >= 0) f[b + o0] = signed_value(v, g.wrap[2] * g.wrap[0]);
+      if (e1 >= 0) f[b + o1] = signed_value(v, g.wrap[2] * g.wrap[1]);
+      if (e0 >= 0 && e1 >= 0) f[b + o0 + o1] = signed_value(v, g.wrap[2] * g.wrap[0] * g.wrap[1]);
     }
 }
 
-// metal points of a component along the row of a group (the uniform directions d0, d1 give
-// all or nothing)
-__device__ __forceinline__ unsigned metal_mask(const Grid &g, const int j[3], const int ysh[3], int d0, int d1, int d2, int z0, int V) {
-  auto hit = [&](int d, int jd) {
-    return (((g.hi_metal >> d) & 1) && ysh[d] == 0 && jd == g.num[d]) || (((g.lo_magnetic >> d) & 1) && ysh[d] == 1 && jd == 0);
-  };
-  if ((d0 >= 0 && hit(d0, j[d0])) || (d1 >= 0 && hit(d1, j[d1]))) return (1u << V) - 1;
-  unsigned m = 0;
+// points of a group that are owned by a slot / on its metal planes
+template <int V> __device__ __forceinline__ unsigned group_owned(const AxisSlot &s, int j0, int j1, int z0, unsigned boxmask) {
+  if (j0 < s.lo[0] || j0 > s.hi[0] || j1 < s.lo[1] || j1 > s.hi[1]) return 0u;
+  unsigned m = boxmask;
+#pragma unroll
   for (int k = 0; k < V; ++k)
-    if (hit(d2, z0 + k)) m |= 1u << k;
+    if (z0 + k < s.lo[2] || z0 + k > s.hi[2]) m &= ~(1u << k);
   return m;
+}
+template <int V> __device__ __forceinline__ unsigned group_metal(const AxisSlot &s, int j0, int j1, int z0) {
+  if (j0 == s.mhi[0] || j0 == s.mlo[0] || j1 == s.mhi[1] || j1 == s.mlo[1]) return (1u << V) - 1;
+  unsigned m = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+    if (z0 + k == s.mhi[2] || z0 + k == s.mlo[2]) m |= 1u << k;
+  return m;
+}
+
+// slot C of the vectorised kernel; F0..F2 are the partner centre rows by direction
+template <class R, int V, bool CURL, bool UPD, int C>
+__device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int z0, unsigned mask,
+                                          const Vec<R, V> &F0, const Vec<R, V> &F1, const Vec<R, V> &F2) {
+  if (!mask) return;
+  const Grid &g = a.g;
+  const AxisSlot &ax = CURL ? a.cc[C].ax : a.uc[C].ax;
+  const unsigned metal = group_metal<V>(ax, j0, j1, z0);
+  Vec<R, V> fv;
+  if (CURL) {
+    const CurlComp<R> &kc = a.cc[C];
+    // slot C = direction C: g1 = F[(C+2)%3] along C+1, g2 = F[(C+1)%3] along C+2
+    constexpr int dp = (C + 1) % 3, dm = (C + 2) % 3;
+    const R *g1 = a.F[(C + 2) % 3], *g2 = a.F[(C + 1) % 3];
+    const Vec<R, V> &g1c = C == 0 ? F2 : (C == 1 ? F0 : F1);
+    const Vec<R, V> &g2c = C == 0 ? F1 : (C == 1 ? F2 : F0);
+    Vec<R, V> cu;
+#pragma unroll
+    for (int k = 0; k < V; ++k) cu.v[k] = R(0);
+    if (a.hasp[C]) {
+      const Vec<R, V> g1n = neighbour<R, V>(g1, i, a.sgn * g.st[dp], g1c, mask);
+#pragma unroll
+      for (int k = 0; k < V; ++k) cu.v[k] = g1n.v[k] - g1c.v[k];
+    }
+    if (a.hasm[C]) {
+      const Vec<R, V> g2n = neighbour<R, V>(g2, i, a.sgn * g.st[dm], g2c, mask);
+      if (a.hasp[C]) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) cu.v[k] = cu.v[k] + g2c.v[k] - g2n.v[k];
+      }
+      else {
+#pragma unroll
+        for (int k = 0; k < V; ++k) cu.v[k] = g2c.v[k] - g2n.v[k];
+      }
+    }
+    fv = vload<R, V>(kc.f + i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
+    if (a.cur.n && int64_t(i) + V - 1 >= a.cur.lo && int64_t(i) <= a.cur.hi) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) fv.v[k] = apply_tab(a.cur, C, i + k, fv.v[k]);
+    }
+    if (a.curb.n) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) fv.v[k] = apply_boxes(a.curb, a.cmp, C, j0, j1, z0 + k, fv.v[k]);
+    }
+    if (metal) {
+#pragma unroll
+      for (int k = 0; k < V; ++k)
+        if ((metal >> k) & 1) fv.v[k] = R(0);
+    }
+    put_row<R, V>(kc.f, i, fv, mask, g, ax, j0, j1, z0);
+  }
+  if (UPD && a.uc[C].f) {
+    const UpdComp<R> &u = a.uc[C];
+    Vec<R, V> gv = CURL ? fv : vload<R, V>(u.g + i);
+    Vec<R, V> Pc[MAXPOLE];
+#pragma unroll
+    for (int p = 0; p < MAXPOLE; ++p)
+      if (p < a.npole && u.P[p]) {
+        Pc[p] = vload<R, V>(u.P[p] + i);
+#pragma unroll
+        for (int k = 0; k < V; ++k) gv.v[k] -= Pc[p].v[k];
+      }
+    if (a.dip.n && int64_t(i) + V - 1 >= a.dip.lo && int64_t(i) <= a.dip.hi) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) gv.v[k] = apply_tab(a.dip, C, i + k, gv.v[k]);
+    }
+    if (a.dipb.n) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) gv.v[k] = apply_boxes(a.dipb, a.cmp, C, j0, j1, z0 + k, gv.v[k]);
+    }
+    Vec<R, V> ev = gv;
+    if (u.u) {
+      const Vec<R, V> uu = vload<R, V>(u.u + i);
+#pragma unroll
+      for (int k = 0; k < V; ++k) ev.v[k] = gv.v[k] * uu.v[k];
+    }
+#pragma unroll
+    for (int p = 0; p < MAXPOLE; ++p)
+      if (p < a.npole && u.P[p] && u.sigma[p]) {
+        const PoleCoef<R> &q = a.pole[p];
+        const Vec<R, V> Pp = vload<R, V>(u.Pp[p] + i), sg = vload<R, V>(u.sigma[p] + i);
+        Vec<R, V> Pn;
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+          Pn.v[k] = q.gamma1inv * (Pc[p].v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Pp.v[k] + q.omega0dtsqr * (sg.v[k] * ev.v[k]));
+        put_row<R, V>(u.P[p], i, Pn, mask, g, ax, j0, j1, z0);
+        vstore<R, V>(u.Pp[p] + i, Pc[p], mask);
+      }
+    if (metal) {
+#pragma unroll
+      for (int k = 0; k < V; ++k)
+        if ((metal >> k) & 1) ev.v[k] = R(0);
+    }
+    put_row<R, V>(u.f, i, ev, mask, g, ax, j0, j1, z0);
+  }
 }
 
 template <class R, int V, bool CURL, bool UPD>
@@ -271,38 +441,24 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
   const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
   const Grid &g = a.g;
-  const int d0 = g.axdir[0], d1 = g.axdir[1], d2 = g.axdir[2];
-  const int e1 = d1 >= 0 ? g.bhi[d1] - g.blo[d1] + 1 : 1;
   const uint32_t r = t / uint32_t(ngroups);
   const int grp = int(t - r * uint32_t(ngroups));
-  const int q0 = int(r / uint32_t(e1));
-  const int q1 = int(r - uint32_t(q0) * uint32_t(e1));
-  int j[3] = {0, 0, 0};
-  if (d0 >= 0) j[d0] = g.blo[d0] + q0;
-  if (d1 >= 0) j[d1] = g.blo[d1] + q1;
+  const int q0 = int(r / uint32_t(g.an[1]));
+  const int q1 = int(r - uint32_t(q0) * uint32_t(g.an[1]));
+  const int j0 = g.alo[0] + q0, j1 = g.alo[1] + q1;
   const int z0 = (g0 + grp) * V;
-  const uint32_t i = uint32_t((d0 >= 0 ? j[d0] * g.st[d0] : 0) + (d1 >= 0 ? j[d1] * g.st[d1] : 0) + z0);
+  const uint32_t i = uint32_t(j0 * g.ast[0] + j1 * g.ast[1] + z0);
   // points of the group inside the launch box along the row
   unsigned boxmask = 0;
 #pragma unroll
   for (int k = 0; k < V; ++k)
-    if (z0 + k >= g.blo[d2] && z0 + k <= g.bhi[d2]) boxmask |= 1u << k;
-  // ownership of each component in this group
-  unsigned cmask[3] = {0, 0, 0}, any = 0;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (CURL ? !a.cc[c].f : !a.uc[c].f) continue;
-    const int *lo = CURL ? a.cc[c].lo : a.uc[c].lo;
-    const int *hi = CURL ? a.cc[c].hi : a.uc[c].hi;
-    if ((d0 >= 0 && (j[d0] < lo[d0] || j[d0] > hi[d0])) || (d1 >= 0 && (j[d1] < lo[d1] || j[d1] > hi[d1]))) continue;
-    unsigned m = boxmask;
-#pragma unroll
-    for (int k = 0; k < V; ++k)
-      if (z0 + k < lo[d2] || z0 + k > hi[d2]) m &= ~(1u << k);
-    cmask[c] = m;
-    any |= m;
-  }
-  if (!any) return;
+    if (z0 + k >= g.alo[2] && z0 + k < g.alo[2] + g.an[2]) boxmask |= 1u << k;
+  // ownership of each component slot in this group
+  unsigned m0 = 0, m1 = 0, m2 = 0;
+  if (CURL ? a.cc[0].f : a.uc[0].f) m0 = group_owned<V>(CURL ? a.cc[0].ax : a.uc[0].ax, j0, j1, z0, boxmask);
+  if (CURL ? a.cc[1].f : a.uc[1].f) m1 = group_owned<V>(CURL ? a.cc[1].ax : a.uc[1].ax, j0, j1, z0, boxmask);
+  if (CURL ? a.cc[2].f : a.uc[2].f) m2 = group_owned<V>(CURL ? a.cc[2].ax : a.uc[2].ax, j0, j1, z0, boxmask);
+  if (!(m0 | m1 | m2)) return;
   // centre rows of the partner components (by direction), loaded once
   Vec<R, V> F0, F1, F2;
   if (CURL) {
@@ -310,96 +466,9 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
     if (a.F[1]) F1 = vload<R, V>(a.F[1] + i);
     if (a.F[2]) F2 = vload<R, V>(a.F[2] + i);
   }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const unsigned mask = cmask[c];
-    if (!mask) continue;
-    const int *ysh = CURL ? a.cc[c].ysh : a.uc[c].ysh;
-    const unsigned metal = (g.hi_metal | g.lo_magnetic) ? metal_mask(g, j, ysh, d0, d1, d2, z0, V) : 0u;
-    Vec<R, V> fv;
-    if (CURL) {
-      const CurlComp<R> &kc = a.cc[c];
-      // slot c = direction c: g1 = F[(c+2)%3] along c+1, g2 = F[(c+1)%3] along c+2
-      const int dp = (c + 1) % 3, dm = (c + 2) % 3;
-      const R *g1 = a.F[(c + 2) % 3], *g2 = a.F[(c + 1) % 3];
-      const Vec<R, V> &g1c = c == 0 ? F2 : (c == 1 ? F0 : F1);
-      const Vec<R, V> &g2c = c == 0 ? F1 : (c == 1 ? F2 : F0);
-      Vec<R, V> cu;
-#pragma unroll
-      for (int k = 0; k < V; ++k) cu.v[k] = R(0);
-      if (a.hasp[c]) {
-        const Vec<R, V> g1n = neighbour<R, V>(g1, i, a.sgn * g.st[dp], g1c, mask);
-#pragma unroll
-        for (int k = 0; k < V; ++k) cu.v[k] = g1n.v[k] - g1c.v[k];
-      }
-      if (a.hasm[c]) {
-        const Vec<R, V> g2n = neighbour<R, V>(g2, i, a.sgn * g.st[dm], g2c, mask);
-        if (a.hasp[c]) {
-#pragma unroll
-          for (int k = 0; k < V; ++k) cu.v[k] = cu.v[k] + g2c.v[k] - g2n.v[k];
-        }
-        else {
-#pragma unroll
-          for (int k = 0; k < V; ++k) cu.v[k] = g2c.v[k] - g2n.v[k];
-        }
-      }
-      fv = vload<R, V>(kc.f + i);
-#pragma unroll
-      for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
-      if (a.cur.n && int64_t(i) + V - 1 >= a.cur.lo && int64_t(i) <= a.cur.hi) {
-#pragma unroll
-        for (int k = 0; k < V; ++k) fv.v[k] = apply_tab(a.cur, c, i + k, fv.v[k]);
-      }
-      if (metal) {
-#pragma unroll
-        for (int k = 0; k < V; ++k)
-          if ((metal >> k) & 1) fv.v[k] = R(0);
-      }
-      vstore<R, V>(kc.f + i, fv, mask);
-      if (a.zwrap) wrap_store<R, V>(kc.f, i, z0, ysh[d2], g.num[d2], a.zsign, fv, mask);
-    }
-    if (UPD && a.uc[c].f) {
-      const UpdComp<R> &u = a.uc[c];
-      Vec<R, V> gv = CURL ? fv : vload<R, V>(u.g + i);
-#pragma unroll
-      for (int p = 0; p < MAXPOLE; ++p)
-        if (p < a.npole && u.P[p]) {
-          const Vec<R, V> P = vload<R, V>(u.P[p] + i);
-#pragma unroll
-          for (int k = 0; k < V; ++k) gv.v[k] -= P.v[k];
-        }
-      if (a.dip.n && int64_t(i) + V - 1 >= a.dip.lo && int64_t(i) <= a.dip.hi) {
-#pragma unroll
-        for (int k = 0; k < V; ++k) gv.v[k] = apply_tab(a.dip, c, i + k, gv.v[k]);
-      }
-      Vec<R, V> ev = gv;
-      if (u.u) {
-        const Vec<R, V> uu = vload<R, V>(u.u + i);
-#pragma unroll
-        for (int k = 0; k < V; ++k) ev.v[k] = gv.v[k] * uu.v[k];
-      }
-#pragma unroll
-      for (int p = 0; p < MAXPOLE; ++p)
-        if (p < a.npole && u.P[p] && u.sigma[p]) {
-          const PoleCoef<R> &q = a.pole[p];
-          const Vec<R, V> P = vload<R, V>(u.P[p] + i), Pp = vload<R, V>(u.Pp[p] + i), sg = vload<R, V>(u.sigma[p] + i);
-          Vec<R, V> Pn;
-#pragma unroll
-          for (int k = 0; k < V; ++k)
-            Pn.v[k] = q.gamma1inv * (P.v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Pp.v[k] + q.omega0dtsqr * (sg.v[k] * ev.v[k]));
-          vstore<R, V>(u.P[p] + i, Pn, mask);
-          vstore<R, V>(u.Pp[p] + i, P, mask);
-          if (a.zwrap) wrap_store<R, V>(u.P[p], i, z0, ysh[d2], g.num[d2], a.zsign, Pn, mask);
-        }
-      if (metal) {
-#pragma unroll
-        for (int k = 0; k < V; ++k)
-          if ((metal >> k) & 1) ev.v[k] = R(0);
-      }
-      vstore<R, V>(u.f + i, ev, mask);
-      if (a.zwrap) wrap_store<R, V>(u.f, i, z0, ysh[d2], g.num[d2], a.zsign, ev, mask);
-    }
-  }
+  fast_slot<R, V, CURL, UPD, 0>(a, i, j0, j1, z0, m0, F0, F1, F2);
+  fast_slot<R, V, CURL, UPD, 1>(a, i, j0, j1, z0, m1, F0, F1, F2);
+  fast_slot<R, V, CURL, UPD, 2>(a, i, j0, j1, z0, m2, F0, F1, F2);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -428,10 +497,13 @@ __global__ __launch_bounds__(BLOCK) void k_conn(const Conn *conns, const double2
 }
 
 template <class R>
-__global__ __launch_bounds__(BLOCK) void k_src(R *f, const int64_t *idx, const double *val, int n) {
+__global__ __launch_bounds__(BLOCK) void k_src(R *f, const int64_t *idx, const double2 *amp, int n, double2 scal, double mult, int cmp) {
+#pragma clang fp contract(off)
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   if (t >= n) return;
-  f[idx[t]] = R(double(f[idx[t]]) - val[t]);
+  const double2 am = amp[t];
+  const double re = am.x * scal.x - am.y * scal.y, im = am.x * scal.y + am.y * scal.x;
+  f[idx[t]] = R(double(f[idx[t]]) - (cmp ? im * mult : re * mult));
 }
 
 template <class R> __device__ __forceinline__ R avg_value(const R *F, int64_t idx, double w, int64_t a1, int64_t a2) {
@@ -440,54 +512,78 @@ template <class R> __device__ __forceinline__ R avg_value(const R *F, int64_t id
   return R(w * double(F[idx]));
 }
 
+// DFT accumulation: one thread per accumulator (frequency fastest, so a wavefront writes a
+// contiguous run), grid-stride.  Each block first forms the phases exp(i omega t) * scale of
+// all frequencies once into LDS (src/dft.cpp:270-271); lists longer than the table form
+// them per accumulator.
+constexpr int DFT_TAB = 512;
 template <class R> __global__ __launch_bounds__(BLOCK) void k_dft(const DftArgs<R> a) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  const int k = t / a.nfreq;
-  if (k >= a.npts) return;
-  const int fi = t - k * a.nfreq;
-  const int64_t idx = a.idx[k];
-  const double w = a.w[k];
-  const R f0 = avg_value(a.f0, idx, w, a.avg1, a.avg2);
-  const R f1 = a.f1 ? avg_value(a.f1, idx, w, a.avg1, a.avg2) : R(0);
-  double sn, cs;
-  sincos(a.omega[fi] * a.time, &sn, &cs);
-  const double pr = cs * a.scale_re - sn * a.scale_im, pim = cs * a.scale_im + sn * a.scale_re;
-  const size_t o = size_t(k) * a.nfreq + fi;
-  if (a.acc_kind == 1) { // FP64 accumulators
-    double2 *acc = reinterpret_cast<double2 *>(a.acc);
-    double2 v = acc[o];
-    v.x += pr * double(f0) - pim * double(f1);
-    v.y += pr * double(f1) + pim * double(f0);
-    acc[o] = v;
-    return;
+  __shared__ double2 ph[DFT_TAB];
+  const int nf = a.nfreq;
+  const bool tab = nf <= DFT_TAB;
+  if (tab) {
+    for (int q = threadIdx.x; q < nf; q += BLOCK) {
+      double sn, cs;
+      sincos(a.omega[q] * a.time, &sn, &cs);
+      ph[q] = make_double2(cs * a.scale_re - sn * a.scale_im, cs * a.scale_im + sn * a.scale_re);
+    }
   }
-  const R rr = R(pr), ri = R(pim);
-  R tr, ti;
-  if (a.f1) { // complex<realnum> product
-    tr = rr * f0 - ri * f1;
-    ti = rr * f1 + ri * f0;
+  __syncthreads();
+  const size_t n = size_t(a.npts) * size_t(nf);
+  for (size_t o = size_t(blockIdx.x) * BLOCK + threadIdx.x; o < n; o += size_t(gridDim.x) * BLOCK) {
+    const int k = int(o / size_t(nf));
+    const int fi = int(o - size_t(k) * size_t(nf));
+    const int64_t idx = a.idx[k];
+    const double w = a.w[k];
+    const R f0 = avg_value(a.f0, idx, w, a.avg1, a.avg2);
+    const R f1 = a.f1 ? avg_value(a.f1, idx, w, a.avg1, a.avg2) : R(0);
+    double pr, pim;
+    if (tab) {
+      pr = ph[fi].x;
+      pim = ph[fi].y;
+    }
+    else {
+      double sn, cs;
+      sincos(a.omega[fi] * a.time, &sn, &cs);
+      pr = cs * a.scale_re - sn * a.scale_im;
+      pim = cs * a.scale_im + sn * a.scale_re;
+    }
+    if (a.acc_kind == 1) { // FP64 accumulators
+      double2 *acc = reinterpret_cast<double2 *>(a.acc);
+      double2 v = acc[o];
+      v.x += pr * double(f0) - pim * double(f1);
+      v.y += pr * double(f1) + pim * double(f0);
+      acc[o] = v;
+      continue;
+    }
+    const R rr = R(pr), ri = R(pim);
+    R tr, ti;
+    if (a.f1) { // complex<realnum> product
+      tr = rr * f0 - ri * f1;
+      ti = rr * f1 + ri * f0;
+    }
+    else {
+      tr = f0 * rr;
+      ti = f0 * ri;
+    }
+    if (a.acc_kind == 2) { // Kahan-compensated complex<float>
+      float2 *acc = reinterpret_cast<float2 *>(a.acc);
+      float2 *cp = reinterpret_cast<float2 *>(a.comp);
+      float2 s = acc[o], c = cp[o];
+      const float yr = float(tr) - c.x, yi = float(ti) - c.y;
+      const float sr = s.x + yr, si = s.y + yi;
+      c.x = (sr - s.x) - yr;
+      c.y = (si - s.y) - yi;
+      s.x = sr;
+      s.y = si;
+      acc[o] = s;
+      cp[o] = c;
+      continue;
+    }
+    R *acc = reinterpret_cast<R *>(a.acc);
+    acc[2 * o] += tr;
+    acc[2 * o + 1] += ti;
   }
-  else {
-    tr = f0 * rr;
-    ti = f0 * ri;
-  }
-  if (a.acc_kind == 2) { // Kahan-compensated complex<float>
-    float2 *acc = reinterpret_cast<float2 *>(a.acc);
-    float2 *cp = reinterpret_cast<float2 *>(a.comp);
-    float2 s = acc[o], c = cp[o];
-    const float yr = float(tr) - c.x, yi = float(ti) - c.y;
-    const float sr = s.x + yr, si = s.y + yi;
-    c.x = (sr - s.x) - yr;
-    c.y = (si - s.y) - yi;
-    s.x = sr;
-    s.y = si;
-    acc[o] = s;
-    cp[o] = c;
-    return;
-  }
-  R *acc = reinterpret_cast<R *>(a.acc);
-  acc[2 * o] += tr;
-  acc[2 * o + 1] += ti;
 }
 
 template <class R>
@@ -502,9 +598,7 @@ static inline unsigned nblocks(size_t n) { return unsigned((n + BLOCK - 1) / BLO
 static inline unsigned nblocks8(size_t n) { return (nblocks(n) + 7u) / 8u * 8u; }
 
 template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
-  size_t count = 1;
-  for (int k = 0; k < 3; ++k)
-    if (a.g.axdir[k] >= 0) count *= size_t(a.g.bhi[a.g.axdir[k]] - a.g.blo[a.g.axdir[k]] + 1);
+  const size_t count = size_t(a.g.an[0]) * size_t(a.g.an[1]) * size_t(a.g.an[2]);
   const unsigned nb = nblocks8(count);
   const uint32_t c32 = uint32_t(count);
   if (curl && upd)
@@ -517,12 +611,9 @@ template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, h
 
 template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
   constexpr int V = 16 / sizeof(R);
-  const int d0 = a.g.axdir[0], d1 = a.g.axdir[1], d2 = a.g.axdir[2];
-  const int g0 = a.g.blo[d2] / V, g1 = a.g.bhi[d2] / V;
+  const int g0 = a.g.alo[2] / V, g1 = (a.g.alo[2] + a.g.an[2] - 1) / V;
   const int ngroups = g1 - g0 + 1;
-  size_t count = size_t(ngroups);
-  if (d0 >= 0) count *= size_t(a.g.bhi[d0] - a.g.blo[d0] + 1);
-  if (d1 >= 0) count *= size_t(a.g.bhi[d1] - a.g.blo[d1] + 1);
+  const size_t count = size_t(ngroups) * size_t(a.g.an[0]) * size_t(a.g.an[1]);
   const unsigned nb = nblocks8(count);
   const uint32_t c32 = uint32_t(count);
   if (curl && upd)
@@ -538,13 +629,38 @@ void launch_conn(const Conn *conns, const double2 *phases, int n, R *const *slot
   if (n > 0) hipLaunchKernelGGL(k_conn<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, conns, phases, n, slots, ncmp);
 }
 
-template <class R> void launch_src(R *f, const int64_t *idx, const double *val, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_src<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, idx, val, n);
+template <class R>
+void launch_src(R *f, const int64_t *idx, const double2 *amp, int n, double2 scal, double mult, int cmp, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_src<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, idx, amp, n, scal, mult, cmp);
+}
+
+template <class R> __global__ __launch_bounds__(BLOCK) void k_average(R *f, const R *b, size_t n) {
+  const size_t t = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (t < n) f[t] = R(0.5 * double(f[t] + b[t]));
+}
+template <class R> void launch_average(R *f, const R *b, size_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_average<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, b, n);
+}
+template <class R> __global__ __launch_bounds__(BLOCK) void k_gather(R *dst, const R *f, const int64_t *idx, int n) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t < n) dst[t] = f[idx[t]];
+}
+template <class R> __global__ __launch_bounds__(BLOCK) void k_scatter(R *f, const int64_t *idx, const R *src, int n) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t < n) f[idx[t]] = src[t];
+}
+template <class R> void launch_gather(R *dst, const R *f, const int64_t *idx, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, dst, f, idx, n);
+}
+template <class R> void launch_scatter(R *f, const int64_t *idx, const R *src, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, idx, src, n);
 }
 
 template <class R> void launch_dft(const DftArgs<R> &a, hipStream_t s) {
   const size_t n = size_t(a.npts) * a.nfreq;
-  if (n > 0) hipLaunchKernelGGL(k_dft<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, a);
+  // enough blocks to fill the chip several times over; each then walks a few strides
+  const unsigned nb = std::min<unsigned>(nblocks(n), 8192u);
+  if (n > 0) hipLaunchKernelGGL(k_dft<R>, dim3(nb), dim3(BLOCK), 0, s, a);
 }
 
 template <class R>
@@ -556,7 +672,11 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
   template void launch_step<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_fast<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_conn<R>(const Conn *, const double2 *, int, R *const *, int, hipStream_t);  \
-  template void launch_src<R>(R *, const int64_t *, const double *, int, hipStream_t);            \
+  template void launch_src<R>(R *, const int64_t *, const double2 *, int, double2, double, int,      \
+                              hipStream_t);                                                        \
+  template void launch_average<R>(R *, const R *, size_t, hipStream_t);                            \
+  template void launch_gather<R>(R *, const R *, const int64_t *, int, hipStream_t);               \
+  template void launch_scatter<R>(R *, const int64_t *, const R *, int, hipStream_t);              \
   template void launch_dft<R>(const DftArgs<R> &, hipStream_t);                                    \
   template void launch_check<R>(const R *const *, const int64_t *, const double *, int, int *,     \
                                 hipStream_t);

@@ -360,6 +360,18 @@ template <class F> inline void loop_ivecs(const GV &gv, const ivec &is, const iv
         body(idx, L, i1, i2, i3);
       }
 }
+// The same loop with the two outer counters shared out over OpenMP threads (the reference's
+// collapse(2) loops, src/meep/vec.hpp:201-230), for bodies that touch only their own point.
+template <class F> inline void loop_ivecs_par(const GV &gv, const ivec &is, const ivec &ie, F body) {
+  const Loop L = make_loop(gv, is, ie);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (ptrdiff_t i1 = 0; i1 < L.n[0]; ++i1)
+    for (ptrdiff_t i2 = 0; i2 < L.n[1]; ++i2)
+      for (ptrdiff_t i3 = 0; i3 < L.n[2]; ++i3) {
+        ptrdiff_t idx = L.idx0 + i1 * L.s[0] + i2 * L.s[1] + i3 * L.s[2];
+        body(idx, L, i1, i2, i3);
+      }
+}
 inline ivec loop_iloc(const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
   ivec r;
   ptrdiff_t ii[3] = {i1, i2, i3};
@@ -819,7 +831,7 @@ template <class R> struct Fields {
             *siginv = dsig != NODIR ? s->siginv[dsig].data() : nullptr;
     const R *sigu = dsigu != NODIR ? s->sig[dsigu].data() : nullptr, *kapu = dsigu != NODIR ? s->kap[dsigu].data() : nullptr,
             *siginvu = dsigu != NODIR ? s->siginv[dsigu].data() : nullptr;
-    loop_ivecs(gv, is, ie, [&](ptrdiff_t i, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
+    loop_ivecs_par(gv, is, ie, [&](ptrdiff_t i, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
       ivec here = loop_iloc(L, i1, i2, i3);
       R c = g2 ? (g1[i + s1] - g1[i] + g2[i] - g2[i + s2]) : (g1[i + s1] - g1[i]);
       if (dsig == NODIR) {
@@ -965,7 +977,7 @@ template <class R> struct Fields {
         R *fe = f[ec][cmp], *fw = f_w[ec][cmp];
         const R *sigw = dsigw != NODIR ? s->sig[dsigw].data() : nullptr, *kapw = dsigw != NODIR ? s->kap[dsigw].data() : nullptr;
         // step_update_EDHB diagonal branches (src/step_generic.cpp:682-701,774-782)
-        loop_ivecs(gv, gv.little_owned_corner(ec), gv.big_corner(), [&](ptrdiff_t i, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
+        loop_ivecs_par(gv, gv.little_owned_corner(ec), gv.big_corner(), [&](ptrdiff_t i, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
           if (dsigw != NODIR) {
             int kw = loop_iloc(L, i1, i2, i3).t[dsigw];
             R fwprev = fw[i], kapwkw = kapw[kw], sigwkw = sigw[kw];
@@ -1006,7 +1018,7 @@ template <class R> struct Fields {
           const R *sg = su.sigma[c].empty() ? nullptr : su.sigma[c].data();
           if (!w || !sg) continue;
           R *pp = p.P[c][cmp], *ppr = p.Pp[c][cmp];
-          loop_ivecs(gv, gv.little_owned_corner(c), gv.big_corner(), [&](ptrdiff_t i, const Loop &, ptrdiff_t, ptrdiff_t, ptrdiff_t) {
+          loop_ivecs_par(gv, gv.little_owned_corner(c), gv.big_corner(), [&](ptrdiff_t i, const Loop &, ptrdiff_t, ptrdiff_t, ptrdiff_t) {
             R pcur = pp[i];
             pp[i] = gamma1inv * (pcur * (2 - omega0dtsqr_denom) - gamma1 * ppr[i] + omega0dtsqr * (sg[i] * w[i]));
             ppr[i] = pcur;
@@ -1032,7 +1044,7 @@ template <class R> struct Fields {
         ph[i] = std::complex<R>(ph64[i]);
       }
       int numcmp = f[ch.c][1] ? 2 : 1;
-      loop_ivecs(gv, ch.is, ch.ie, [&](ptrdiff_t idx, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
+      loop_ivecs_par(gv, ch.is, ch.ie, [&](ptrdiff_t idx, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
         size_t idft = (i1 * L.n[1] + i2) * L.n[2] + i3;
         double w = ch.include_w ? loop_weight(gv, L, i1, i2, i3, ch.s0, ch.s1, ch.e0, ch.e1, ch.dV0 + ch.dV1 * i2) : 1.0;
         R fv[2] = {0, 0};
@@ -1111,6 +1123,79 @@ template <class R> struct Fields {
       if (has_field(c) && component_direction(c) != NODIR)
         sum += real(conj(get_field(c, cen)) * get_field(c + 10, cen));
     if (!std::isfinite(sum)) fail("simulation fields are NaN or Inf");
+  }
+
+  // ---- field energy (src/energy_and_flux.cpp:45-178, src/integrate.cpp:46-205) ----
+  // 0.5 * integral of Re(conj(F) G) over the cell for F = E (H) and G = D (B), per component;
+  // E and D share a grid, so the integration grid is the component's own (no averaging).
+  long double energy_of(int first) {
+    vec3 wmin, wmax;
+    for (int d = 0; d < 3; ++d) wmax.t[d] = gv.has(d) ? gv.num[d] * gv.inva : 0.0;
+    long double sum = 0;
+    for (int c = first; c < first + 5; ++c) {
+      if (!has_field(c) || !f[c][0]) continue;
+      const int c2 = c + 10;
+      cdouble total = 0;
+      loop_in_chunks(wmin, wmax, c, [&](const ivec &is, const ivec &ie, const vec3 &s0, const vec3 &s1, const vec3 &e0,
+                                        const vec3 &e1, double dV0, const ivec &, cdouble sph) {
+        std::complex<long double> part = 0;
+        loop_ivecs(gv, is, ie, [&](ptrdiff_t idx, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
+          std::complex<R> fv[2];
+          for (int q = 0; q < 2; ++q) {
+            const int cc = q ? c2 : c;
+            double fr[2] = {0, 0};
+            for (int k = 0; k < 2; ++k)
+              if (f[cc][k]) {
+                const R a = f[cc][k][idx];
+                fr[k] = 0.25 * (a + a + a + a); // the four-point average with zero offsets
+              }
+            fv[q] = std::complex<R>(cdouble(fr[0], fr[1]) * sph);
+          }
+          const cdouble integrand = real(conj(cdouble(fv[0])) * cdouble(fv[1]));
+          part += std::complex<long double>(integrand * loop_weight(gv, L, i1, i2, i3, s0, s1, e0, e1, dV0));
+        });
+        total += cdouble(part);
+      });
+      sum += real(total) * 0.5;
+    }
+    return sum;
+  }
+  // fields::field_energy with synchronize_magnetic_fields / restore_magnetic_fields
+  // (src/energy_and_flux.cpp:46-62,146-178): B and H advanced half a step and averaged with
+  // their values at t - dt/2, the magnetic energy taken, every magnetic array restored.
+  double field_energy() {
+    std::vector<std::pair<R *, std::vector<R>>> saved, saved_aux;
+    auto save = [&](std::vector<std::pair<R *, std::vector<R>>> &into, R *p) {
+      if (p) into.push_back({p, std::vector<R>(p, p + gv.ntot)});
+    };
+    for (int cmp = 0; cmp < 2; ++cmp) {
+      for (int c = 15; c < 20; ++c) {
+        save(saved, f[c][cmp]);
+        save(saved_aux, f_u[c][cmp]);
+        save(saved_aux, f_w[c][cmp]);
+      }
+      for (int c = 5; c < 10; ++c)
+        if (f[c][cmp] && f[c][cmp] != f[c + 10][cmp]) {
+          save(saved, f[c][cmp]);
+          save(saved_aux, f_u[c][cmp]);
+          save(saved_aux, f_w[c][cmp]);
+        }
+    }
+    calc_sources(t * dt);
+    if (step_db(B_stuff)) connections_valid = false;
+    step_source(B_stuff);
+    step_boundaries(B_stuff);
+    calc_sources(t * dt + 0.5 * dt);
+    if (update_eh(H_stuff)) connections_valid = false;
+    step_boundaries(H_stuff);
+    for (auto &sv : saved) {
+      R *fc = sv.first;
+      for (size_t i = 0; i < gv.ntot; ++i) fc[i] = 0.5 * (fc[i] + sv.second[i]);
+    }
+    const double magnetic = double(energy_of(5));
+    for (auto *lst : {&saved, &saved_aux})
+      for (auto &sv : *lst) std::copy(sv.second.begin(), sv.second.end(), sv.first);
+    return double(energy_of(0)) + magnetic;
   }
 
   // ---- loop_in_chunks for one chunk and the identity symmetry (src/loop_in_chunks.cpp) ----
@@ -1880,6 +1965,70 @@ int mp_dft_flux_get_dft(mp_dft_flux *flp, int which, int i, double *out) {
         out[2 * j] = v.real();
         out[2 * j + 1] = v.imag();
       }
+      return 0;
+    });
+  });
+}
+
+// fields::add_dft_fields (src/dft.cpp:913-927): one chunk list over all components, no
+// integration weights, stored weight 1, centred grid; each component's chunks are prepended.
+mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *comps, int ncomp, const double *vmin, const double *vmax,
+                                        const double *freq, int nfreq, int dec) {
+  AnyFields *af = reinterpret_cast<AnyFields *>(f);
+  AnyFlux *fl = new AnyFlux;
+  fl->owner = af;
+  fl->freq.assign(freq, freq + nfreq);
+  int rc = guard([&] {
+    with_fields(af, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      if (!F.components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+      int decim = dec > 0 ? dec : F.auto_decimation(fl->freq);
+      fl->decimation = decim;
+      vec3 mn = to_vec(vmin), mx = to_vec(vmax);
+      for (int q = 0; q < ncomp; ++q) {
+        std::vector<std::unique_ptr<DftChunk<R>>> lst;
+        F.add_dft(lst, comps[q], mn, mx, fl->freq, false, cdouble(1.0, 0), decim);
+        std::vector<void *> front;
+        for (int j = int(lst.size()) - 1; j >= 0; --j) {
+          front.push_back(lst[j].get());
+          F.dft_chunks.push_back(std::move(lst[j]));
+        }
+        fl->E.insert(fl->E.begin(), front.begin(), front.end());
+      }
+      return 0;
+    });
+  });
+  if (rc) {
+    delete fl;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_dft_fields *>(fl);
+}
+void mp_dft_fields_delete(mp_dft_fields *h) { delete reinterpret_cast<AnyFlux *>(h); }
+int mp_dft_fields_num_chunks(mp_dft_fields *h) { return mp_dft_flux_num_chunks(reinterpret_cast<mp_dft_flux *>(h), 0); }
+long mp_dft_fields_chunk_points(mp_dft_fields *h, int i) {
+  return mp_dft_flux_chunk_points(reinterpret_cast<mp_dft_flux *>(h), 0, i);
+}
+int mp_dft_fields_chunk_component(mp_dft_fields *h, int i) {
+  AnyFlux *fl = reinterpret_cast<AnyFlux *>(h);
+  if (i < 0 || i >= int(fl->E.size())) {
+    last_error = "no such dft chunk";
+    return -1;
+  }
+  return with_fields(fl->owner, [&](auto &F) {
+    typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+    return reinterpret_cast<DftChunk<R> *>(fl->E[i])->c;
+  });
+}
+int mp_dft_fields_get_dft(mp_dft_fields *h, int i, double *out) {
+  return mp_dft_flux_get_dft(reinterpret_cast<mp_dft_flux *>(h), 0, i, out);
+}
+
+// fields::field_energy (src/energy_and_flux.cpp:48)
+int mp_fields_field_energy(mp_fields *f, double *out) {
+  return guard([&] {
+    with_fields(reinterpret_cast<AnyFields *>(f), [&](auto &F) {
+      *out = F.field_energy();
       return 0;
     });
   });

@@ -105,11 +105,21 @@ def polariton_ex(lib, gv, eps, single=False):
     return f.get_field(Ex, center(s)).real
 
 
+def polariton_energy(lib, gv, eps, single=False):
+    s = structure(lib, eps=eps, single=single, **gv)
+    s.add_susceptibility(sigma_const, 0.3, 0.1)
+    f = fields(s)
+    _src(f, s, Ex)
+    _run(f, 10.0)
+    return f.field_energy()
+
+
 A = 10.0
 DPML = 1.0
 # (name, golden value, function, grid, eps) — tests/known_results.cpp:137-157
 CASES = [
     ("1D polariton", -0.0894851, polariton_ex, volone(1.0, A), one),
+    ("1D polariton energy", 0.0863443, polariton_energy, volone(1.0, A), one),
     ("1x1 metallic 2D TM", 5.20605, metallic_ez, voltwo(1.0, 1.0, A), one),
     ("1x1 PML 2D TM", 0.883776, using_pml_ez, voltwo(1.0 + 2 * DPML, 1.0 + 2 * DPML, A), one),
     ("1x1 X periodic 2D TM", 0.110425, x_periodic, voltwo(1.0, 1.0, A), one),

@@ -65,6 +65,30 @@ def box_3d_pml_lorentz(lib, single=False, a=8, real=True):
     src = gaussian_src_time(0.8, 0.4)
     f.add_volume_source(Ex, src, (0.375, 0.375, 1.5), (L - 0.375, L - 0.375, 1.5))
     fl = f.add_dft_flux(Z, (0.4, 0.4, 1.25), (L - 0.4, L - 0.4, 1.25), linspace(0.5, 1.1, 8))
+    df = f.add_dft_fields([Ex, Ey, Ez], (0.0, 0.0, 0.75), (L, L, 0.75), linspace(0.5, 1.1, 5))
+    return s, f, [fl, df]
+
+
+def long_sources_3d(lib, single=False, a=8, real=True):
+    """More long source lists than the kernels carry as dense boxes: three current planes
+    and three integrated (dipole) planes on one component each, so the box path and the
+    list fallback (k_src; gathered and restored D points for dipoles) both run, plus a
+    Lorentz medium so E needs f_minus_p.  PML on z only, periodic in x and y."""
+    Lx, Ly, Lz = 1.5, 1.0, 2.0
+    s = structure(lib, dim=3, size=(Lx, Ly, Lz), a=a, eps=blocks_eps(7, 3, 1.0, 6.0, (Lx, Ly, Lz)),
+                  pml=pml(0.375, Z), single=single)
+    s.add_susceptibility(lambda x, y, z: 0.5, 1.2, 0.1)
+    f = fields(s)
+    if real:
+        f.use_real_fields()
+    f.use_bloch((0.0, 0.0, 0.0))
+    for q, (c, zc) in enumerate(((Ex, 0.6), (Ex, 0.9), (Ey, 1.1), (Ex, 1.3))):
+        f.add_volume_source(c, gaussian_src_time(0.7 + 0.05 * q, 1.0), (0.0, 0.0, zc), (Lx, Ly, zc),
+                            1.0 + 0.25 * q)
+    for q, (c, zc) in enumerate(((Ez, 0.7), (Ez, 1.0), (Ey, 1.2))):
+        f.add_volume_source(c, gaussian_src_time(0.6 + 0.1 * q, 1.2, is_integrated=True),
+                            (0.0, 0.0, zc), (Lx, Ly, zc), 0.5 - 0.1 * q)
+    fl = f.add_dft_flux(Z, (0.0, 0.0, 1.5), (Lx, Ly, 1.5), linspace(0.5, 0.9, 4))
     return s, f, [fl]
 
 
@@ -143,6 +167,24 @@ def xpml_periodic_3d(lib, single=False, a=8):
     return s, f, [fl]
 
 
+def straight_waveguide(lib, single=False, real=True):
+    """The 2D straight-waveguide configuration at full size (python/examples/
+    straight-waveguide.py's geometry): 16x8 at resolution 20 (320x160 cells), eps = 12 for
+    |y - 4| <= 0.5 (no averaging), PML 1 on all sides, Gaussian Ez point current at (1, 4)
+    (fcen 0.15, df 0.1, not integrated), flux line x = 14.5, 1 <= y <= 7, 100 frequencies
+    in [0.1, 0.2], decimation 1.  Run 8000 steps (200 time units at Courant 0.5)."""
+    def eps(x, y, z):
+        return 12.0 if abs(y - 4.0) <= 0.5 else 1.0
+    s = structure(lib, dim=2, size=(16.0, 8.0), a=20, eps=eps, pml=pml(1.0), single=single)
+    f = fields(s)
+    if real:
+        f.use_real_fields()
+    f.add_volume_source(Ez, gaussian_src_time(0.15, 0.1), (1.0, 4.0, 0), (1.0, 4.0, 0))
+    fl = f.add_dft_flux(X, (14.5, 1.0, 0), (14.5, 7.0, 0), linspace(0.1, 0.2, 100),
+                        decimation_factor=1)
+    return s, f, [fl]
+
+
 SCENARIOS = {
     "waveguide_2d": (waveguide_2d, 1600),
     "box_3d_pml_lorentz": (box_3d_pml_lorentz, 120),
@@ -152,6 +194,7 @@ SCENARIOS = {
     "periodic_3d_k0": (periodic_3d_k0, 150),
     "zone_edge_2d": (zone_edge_2d, 200),
     "xpml_periodic_3d": (xpml_periodic_3d, 150),
+    "long_sources_3d": (long_sources_3d, 120),
 }
 
 FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)

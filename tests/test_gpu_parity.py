@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from known_results_cases import CASES, rel_tol
+from meep_amd.capi import dft_fields
 from scenarios import SCENARIOS, state
 
 pytestmark = pytest.mark.gpu
@@ -51,11 +52,19 @@ def test_state_parity(gpu_lib, oracle_lib, name, single, kernel, monkeypatch):
         err = np.abs(a[key] - b[key]).max()
         assert err <= tol * scale, f"{name} {key}: max err {err:.3e} vs scale {scale:.3e}"
     for x, y in zip(flg, flo):
+        if isinstance(y, dft_fields):
+            assert x.num_chunks() == y.num_chunks() > 0
+            for i in range(y.num_chunks()):
+                assert x.component(i) == y.component(i)
+                dx, dy = x.dft(i), y.dft(i)
+                sc = max(np.abs(dy).max(), 1e-30)
+                assert np.abs(dx - dy).max() <= tol * sc, f"{name} dft_fields chunk {i}"
+            continue
         fx, fy = x.flux(), y.flux()
         scale = max(np.abs(fy).max(), 1e-30)
-        assert np.abs(fx - fy).max() <= (1e-4 if single else 1e-10) * scale
+        assert np.abs(fx - fy).max() <= tol * scale
         for which in (0, 1):
             for i in range(y.num_chunks(which)):
                 dx, dy = x.dft(which, i), y.dft(which, i)
                 sc = max(np.abs(dy).max(), 1e-30)
-                assert np.abs(dx - dy).max() <= (1e-4 if single else 1e-10) * sc
+                assert np.abs(dx - dy).max() <= tol * sc
