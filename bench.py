@@ -169,6 +169,104 @@ def c3_algorithmic_bytes(n=C3_CELLS, npml=C3_PML, word=4, nfreq=C3_NFREQ, acc_by
     return words * word + dft
 
 
+# ---- the 2D differentiation workload -----------------------------------------------------
+C4_A = 50            # resolution: 600 x 400 cells in 12 x 8
+C4_SIZE = (12.0, 8.0)
+C4_DPML = 1.0
+C4_STEPS = 4000
+C4_DESIGN = ((4.0, 2.0, 0.0), (7.98, 5.98, 0.0))  # 200 x 200 Ez points
+
+
+def c4_rho(seed=4, n=200, blur=5.0):
+    """Seeded uniform density on the 200 x 200 design pixels, Gaussian-blurred (sigma 5 px,
+    periodic wrap, by FFT)."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(0.0, 1.0, size=(n, n))
+    k = np.fft.fftfreq(n)
+    g = np.exp(-2 * (np.pi * blur) ** 2 * (k[:, None] ** 2 + k[None, :] ** 2))
+    return np.real(np.fft.ifft2(np.fft.fft2(r) * g))
+
+
+def build_c4(lib, single, chi_ez=None, rho=None):
+    """TM fields in 12 x 8 at resolution 50 with PML 1; eps = 1 + 11 rho on the 200 x 200
+    design points (no averaging) and 1 elsewhere; an Ez line source at x = 2 with a Gaussian
+    amplitude profile along y (sigma 1) and a Gaussian pulse (fcen 1, df 0.4); the objective
+    monitor: DFT Ez on a 40-cell line at x = 10 centred on y = 4, 10 frequencies in
+    [0.8, 1.2], decimation 1."""
+    s = capi.structure(lib, dim=2, size=C4_SIZE, a=C4_A, eps=None, pml=capi.pml(C4_DPML),
+                       single=single)
+    if chi_ez is None:
+        rho = c4_rho() if rho is None else rho
+        n = s.shape
+        chi = np.ones(n)
+        x0, y0 = int(round(C4_DESIGN[0][0] * C4_A)), int(round(C4_DESIGN[0][1] * C4_A))
+        chi[x0:x0 + 200, y0:y0 + 200] = 1.0 / (1.0 + 11.0 * rho)
+        chi_ez = chi
+    s.set_chi1inv(capi.Ez, chi_ez)
+    f = capi.fields(s)
+    f.use_real_fields()
+    f.add_volume_source(capi.Ez, capi.gaussian_src_time(1.0, 0.4), (2.0, 1.5, 0), (2.0, 6.5, 0),
+                        amp_func=lambda x, y, z: float(np.exp(-0.5 * y * y)))
+    mon = f.add_dft_fields([capi.Ez], (10.0, 3.6, 0), (10.0, 4.38, 0),
+                           capi.linspace(0.8, 1.2, 10), decimation_factor=1)
+    return s, f, mon
+
+
+def run_c4(args, lib):
+    """Forward run with recording, then the reverse-mode gradient; value is the forward
+    Gcell-updates/s, and the line reports the gradient's cost relative to the forward run."""
+    from meep_amd import autograd as madj
+    single = args.precision == "f32"
+    interval = args.checkpoint_interval
+    prob = madj.Problem(lib, lambda L, chi: build_c4(L, single, chi), (capi.Ez,) + C4_DESIGN, C4_STEPS,
+                        checkpoint_interval=interval)
+    base, lo, shape = prob.base_chi()
+    chi_design = base[lo[0]:lo[0] + shape[0], lo[1]:lo[1] + shape[1]]
+    # warm-up run (compiles nothing, but pages in the library and the allocator)
+    J, h = prob.forward(chi_design)
+    prob.gradient(h)
+    J, h = prob.forward(chi_design)
+    g = prob.gradient(h)
+    tf, tg = prob.timing["forward_s"], prob.timing["gradient_s"]
+    cells = 600 * 400
+    line = line_base(args, cells * C4_STEPS / tf / 1e9, tf, {
+        "workload": "2D TM 600x400 cells at resolution 50, PML 50 cells, 200x200 design "
+                    "(eps = 1 + 11 rho, rho seeded uniform blurred 5 px), Gaussian-profile Ez "
+                    "line source, J = sum |DFT Ez|^2 on a 40-cell line at 10 frequencies, "
+                    f"4000 steps; reverse mode with checkpoint interval {prob.interval}",
+        "cells": cells, "resolution": C4_A, "courant": 0.5, "parallelism": "none"})
+    line["steps"] = C4_STEPS
+    line["ms_per_step"] = round(tf / C4_STEPS * 1e3, 4)
+    line["adjoint"] = {"forward_s": round(tf, 4), "gradient_s": round(tg, 4),
+                       "gradient_over_forward": round(tg / tf, 3), "J": J,
+                       "grad_max_abs": float(np.abs(g).max())}
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    if args.fd_pixels:
+        rng = np.random.default_rng(7)
+        out = []
+        chi_full = prob.full_chi(chi_design)
+        for _ in range(args.fd_pixels):
+            i, j = int(rng.integers(0, shape[0])), int(rng.integers(0, shape[1]))
+            h = 1e-3 * chi_full[lo[0] + i, lo[1] + j]
+            vals = []
+            for sg in (1, -1):
+                c2 = chi_full.copy()
+                c2[lo[0] + i, lo[1] + j] += sg * h
+                s, f, mon = build_c4(lib, False, c2)
+                f.step_n(C4_STEPS)
+                vals.append(sum(float(np.sum(np.abs(mon.dft(q)) ** 2)) for q in range(mon.num_chunks())))
+                del f, s
+            fd = (vals[0] - vals[1]) / (2 * h)
+            out.append((i, j, float(g[i, j]), fd))
+        gm = float(np.abs(g).max())
+        line["adjoint"]["fd_check"] = {
+            "pixels": len(out), "fd": "central differences of FP64 runs, step 1e-3 chi1inv",
+            "max_rel_err": max(abs(a - b) / max(abs(b), 1e-300) for _, _, a, b in out),
+            "max_err_over_max_grad": max(abs(a - b) for _, _, a, b in out) / gm}
+    return line
+
+
 def load_traffic(precision):
     path = os.path.join(ROOT, "profiles", f"traffic_step_de_box256_{precision}.json")
     if not os.path.exists(path):
@@ -322,9 +420,17 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--workload", choices=["box256", "pml_lorentz_dft"], default="box256",
+    ap.add_argument("--workload", choices=["box256", "pml_lorentz_dft", "adjoint2d"],
+                    default="box256",
                     help="box256: 3D Bloch-periodic 256^3 box (the headline); "
-                         "pml_lorentz_dft: 384x384x192 with PML, two poles and a DFT plane")
+                         "pml_lorentz_dft: 384x384x192 with PML, two poles and a DFT plane; "
+                         "adjoint2d: 600x400 forward run plus its reverse-mode gradient")
+    ap.add_argument("--checkpoint-interval", type=int, default=None,
+                    help="adjoint2d: steps between checkpoints (default sqrt(steps); 0 records "
+                         "the design history every step)")
+    ap.add_argument("--fd-pixels", type=int, default=0,
+                    help="adjoint2d: check the gradient on this many pixels by central "
+                         "differences of FP64 runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy 5 times (the reference "
@@ -346,7 +452,8 @@ def main():
               "per half-step).", file=sys.stderr)
         sys.exit(2)
     lib = meep_amd.load_library()
-    line = run_box256(args, lib) if args.workload == "box256" else run_c3(args, lib)
+    run = {"box256": run_box256, "pml_lorentz_dft": run_c3, "adjoint2d": run_c4}[args.workload]
+    line = run(args, lib)
     print(json.dumps(line))
 
 

@@ -48,6 +48,7 @@ typedef struct mp_structure mp_structure;
 typedef struct mp_fields mp_fields;
 typedef struct mp_dft_flux mp_dft_flux;
 typedef struct mp_dft_fields mp_dft_fields;
+typedef struct mp_adjoint mp_adjoint;
 
 /* r points at (x, y, z) in the reference's coordinates (origin at the cell's low corner). */
 typedef double (*mp_scalar_func)(const double *r, void *user);
@@ -207,6 +208,30 @@ long mp_dft_fields_chunk_points(mp_dft_fields *df, int i);
 int mp_dft_fields_chunk_component(mp_dft_fields *df, int i);
 /* dft_chunk::dft of chunk i, interleaved (re, im), frequency fastest. */
 int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out);
+
+/* ---- reverse mode (gradient of a DFT objective) ------------------------------------- */
+
+/* Differentiation of J = sum over the accumulators a of `objective` of |a|^2 with respect to
+   chi1inv of electric component c at its points inside [vmin, vmax] (the design region), by
+   the transposed step run backwards in time.  Its counterpart in the reference is the adjoint
+   solver's gradient (python/adjoint/optimization_problem.py, src/meepgeom.cpp:2835-3011),
+   which is formed from forward and adjoint DFT fields in the frequency domain; this one is
+   the exact gradient of the discrete time-stepped J.
+   Create it before the steps to be differentiated: they then record D at the design points,
+   every step when checkpoint_interval is 0, or the whole state every checkpoint_interval steps
+   (recording must then start at t = 0; the backward sweep recomputes each segment from its
+   checkpoint).  lo_xyz and n_xyz (either may be NULL) receive the design box's first array
+   index and size per direction (0 and 1 for absent directions).  Real fields, no dispersive materials, no integrated sources, real Bloch phases
+   only.  One per fields object. */
+mp_adjoint *mp_adjoint_new(mp_fields *f, mp_dft_fields *objective, int c, const double *vmin,
+                           const double *vmax, int checkpoint_interval, int *lo_xyz,
+                           int *n_xyz);
+void mp_adjoint_delete(mp_adjoint *a);
+/* J of the accumulators as they stand. */
+int mp_adjoint_objective(mp_adjoint *a, double *J);
+/* dJ / dchi1inv over the design box, in the reference layout order restricted to the box
+   (X slowest, the stride-1 direction fastest).  The fields end in the state they had. */
+int mp_adjoint_gradient(mp_adjoint *a, double *grad);
 
 #ifdef __cplusplus
 }

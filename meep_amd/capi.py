@@ -107,6 +107,12 @@ _SIGNATURES = {
     "mp_dft_flux_decimation": (C.c_int, [C.c_void_p]),
     "mp_dft_flux_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "mp_fields_field_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_adjoint_new": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]),
+    "mp_adjoint_delete": (None, [C.c_void_p]),
+    "mp_adjoint_objective": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_adjoint_gradient": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mp_fields_add_dft_fields": (C.c_void_p, [C.c_void_p, C.POINTER(C.c_int), C.c_int,
                                               C.POINTER(C.c_double), C.POINTER(C.c_double),
                                               C.POINTER(C.c_double), C.c_int, C.c_int]),
@@ -279,6 +285,46 @@ class dft_fields:
         return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
 
 
+class adjoint:
+    """Reverse-mode gradient of J = sum |a|^2 over a dft_fields monitor's accumulators with
+    respect to chi1inv of electric component c on the design box [vmin, vmax]; create it
+    before the steps to differentiate (see mp_adjoint_new)."""
+
+    def __init__(self, fields, monitor, c, vmin, vmax, checkpoint_interval=0):
+        self.f = fields
+        self.lib = fields.lib
+        lo, n = (C.c_int * 3)(), (C.c_int * 3)()
+        self.ptr = self.lib.mp_adjoint_new(fields.ptr, monitor.ptr, c, _dvec(vmin), _dvec(vmax),
+                                           int(checkpoint_interval), lo, n)
+        if not self.ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        self.box_lo = tuple(lo)
+        self.box_n = tuple(n)
+        fields._adjoints.append(self)
+
+    @property
+    def shape(self):
+        """Design box shape in the reference layout (absent directions dropped)."""
+        dim = self.f.s.dim
+        dirs = {1: (2,), 2: (0, 1), 3: (0, 1, 2)}[dim]
+        return tuple(self.box_n[d] for d in dirs)
+
+    def objective(self):
+        J = C.c_double()
+        _check(self.lib, self.lib.mp_adjoint_objective(self.ptr, C.byref(J)))
+        return J.value
+
+    def gradient(self):
+        out = np.empty(int(np.prod(self.box_n)))
+        _check(self.lib, self.lib.mp_adjoint_gradient(self.ptr, _dptr(out)))
+        return out.reshape(self.shape)
+
+    def close(self):
+        if self.ptr:
+            self.lib.mp_adjoint_delete(self.ptr)
+            self.ptr = None
+
+
 class fields:
     """meep::fields (src/meep.hpp:1716-2320)."""
 
@@ -290,6 +336,7 @@ class fields:
         if not self.ptr:
             raise MeepError(self.lib.mp_last_error().decode())
         self._fluxes = []
+        self._adjoints = []
 
     def use_real_fields(self):
         _check(self.lib, self.lib.mp_fields_use_real_fields(self.ptr))
@@ -422,6 +469,8 @@ class fields:
 
     def __del__(self):
         if getattr(self, "ptr", None):
+            for adj in self._adjoints:
+                adj.close()
             for fl in self._fluxes:
                 if fl.ptr:
                     if isinstance(fl, dft_fields):

@@ -105,6 +105,7 @@ struct SrcTab {
   int comp[SRC_TAB];
   int64_t idx[SRC_TAB];
   double val[SRC_TAB];
+  int vbase; // batched steps: entry s takes its value from the step's value row at vbase + s
 };
 
 // Sources too long for a table, carried as dense index boxes: source s covers array indices
@@ -122,6 +123,7 @@ struct SrcBoxes {
   const double2 *amp[SRC_BOX];
   double2 scal[SRC_BOX];
   double mult[SRC_BOX];
+  int vbase; // batched steps: box s takes scal from the step's value row at vbase + 2 s
 };
 
 template <class R> struct StepArgs {
@@ -173,6 +175,94 @@ template <class R> struct DftArgs {
   int acc_kind; // MP_DFT_* of meep_amd.h
 };
 
+// ---- batched steps (many chunks in one launch) ---------------------------------------------
+// One launch boxes of many independent chunks: item k is one launch of the single-chunk path
+// (its argument block, thread count and row groups), placed in device memory; a block finds
+// its item in blk_item and its first thread in blk_first.  Source values come from the step's
+// value row (SrcTab::vbase, SrcBoxes::vbase).
+template <class R> struct BatchItem {
+  StepArgs<R> a;
+  uint32_t count;
+  int g0, ngroups; // vectorised items: first row group and groups per row
+};
+// DFT accumulation of one chunk in a batch; the time of step s is
+// (t_start + s + 1) dt - half dt / 2, and steps where that index is not a multiple of the
+// decimation are skipped.
+template <class R> struct BatchDft {
+  DftArgs<R> a;
+  double dt;
+  int half;
+  int decimation;
+  int64_t t_start;
+};
+template <class R>
+void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, bool fast,
+                       bool curl, bool upd, int block, const double *vrow, hipStream_t s);
+template <class R>
+void launch_dft_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, long step,
+                      hipStream_t s);
+// halo lists of many chunks: item k is (conns, phases, n, slots, ncmp)
+struct BatchConn {
+  const Conn *conns;
+  const double2 *phases;
+  int n;
+  void *const *slots;
+  int ncmp;
+};
+template <class R>
+void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, hipStream_t s);
+constexpr int BATCH_BLOCKS[4] = {64, 128, 256, 512}; // block sizes the batch autotuner tries
+
+// ---- reverse-mode step (adjoint.hip) -------------------------------------------------------
+// The adjoint of one step applies the transpose of each forward update, in reverse order, to
+// cotangent arrays ("bar" arrays, one per forward state array, aliased as the forward arrays
+// are).  Slots mirror the forward StepArgs slots.
+template <class R> struct AdjSlots {
+  R *fb[3];   // cotangent of the slot's field (B or D for a curl slot, E or H for an update slot)
+  R *fub[3];  // curl: cotangent of f_u
+  R *gb[3];   // update: cotangent of the source field (D or B)
+  R *fwb[3];  // update: cotangent of f_w
+  R *cbar[3]; // curl: cotangent of the slot's curl term, zero off the owned points
+};
+// design box (loop-axis indices) of one update slot: the gradient of chi1inv there collects
+// (cotangent of u * D) * D over every step, with D read from the forward history
+template <class R> struct AdjDesign {
+  int slot; // -1: none
+  int lo[3], hi[3];
+  const R *hist; // D at the box's points for this step, box-local (loop order)
+  double *grad;  // box-local accumulator
+};
+// Gather of curl-term cotangents into one partner array: gb[j] += sum over entries e of
+// cbar_e[j - s_e] (when that point exists along axis ax_e) - cbar_e[j] for a "plus" entry,
+// and cbar_e[j] - cbar_e[j - s_e] for a "minus" entry.
+template <class R> struct AdjGather {
+  int n;
+  R *gb;
+  const R *cbar[4];
+  int64_t s[4];
+  int ax[4];   // loop axis of s
+  int step[4]; // +1 or -1: coordinate step of s along ax
+  int plus[4]; // 1: g1 (the i + s1 partner), 0: g2
+};
+
+// launchers (adjoint.hip)
+template <class R> void launch_adj_upd(const StepArgs<R> &a, const AdjSlots<R> &b, const AdjDesign<R> &dz, hipStream_t s);
+template <class R> void launch_adj_curl(const StepArgs<R> &a, const AdjSlots<R> &b, hipStream_t s);
+template <class R> void launch_adj_gather(const Grid &g, const AdjGather<R> &ga, hipStream_t s);
+// halo transpose: for each owned source point src[q], its halo images dst[first[q] .. first[q+1])
+// with signs sgn: bar[src] += sgn * bar[dst]; bar[dst] = 0 (slot pointers in bars)
+template <class R>
+void launch_adj_halo(const int *first, const uint32_t *src, const uint32_t *dst, const int8_t *sgn, const uint16_t *slot, int nsrc,
+                     R *const *bars, hipStream_t s);
+// DFT objective J = sum |a|^2: G[k] = w_k * sum_f 2 Re(conj(a_f(k)) ph_f) with ph_f =
+// exp(i omega_f t) * scale, then bar[idx_k + off] += coef * G[k]
+template <class R>
+void launch_adj_dft(const int64_t *idx, const double *w, const double2 *acc, const double *omega, int npts, int nfreq, double time,
+                    double scale_re, double scale_im, double *G, hipStream_t s);
+template <class R> void launch_adj_dft_scatter(R *bar, const int64_t *idx, const double *G, int npts, int64_t off, double coef, hipStream_t s);
+// copy D at the design box points (loop-axis box, device layout) into hist (box-local)
+template <class R> void launch_record(R *hist, const R *D, const Grid &g, const int *lo, const int *hi, hipStream_t s);
+
 // launchers (kernels.hip)
 // generic scalar kernel over the box a.g.blo..bhi (any cells, uPML included)
 template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
@@ -186,6 +276,9 @@ void launch_src(R *f, const int64_t *idx, const double2 *amp, int n, double2 sca
                 hipStream_t s);
 // f[i] = (f[i] + b[i]) / 2 over n points (fields_chunk::average_with_backup)
 template <class R> void launch_average(R *f, const R *b, size_t n, hipStream_t s);
+// f[dst[q]] = sgn[q] * f[idx[k[q]]]: the folded periodic images of source points
+template <class R>
+void launch_src_images(R *f, const int64_t *idx, const int *k, const int64_t *dst, const int8_t *sgn, int n, hipStream_t s);
 // dst[j] = f[idx[j]] and f[idx[j]] = src[j]
 template <class R> void launch_gather(R *dst, const R *f, const int64_t *idx, int n, hipStream_t s);
 template <class R> void launch_scatter(R *f, const int64_t *idx, const R *src, int n, hipStream_t s);

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <chrono>
 #include <complex>
 #include <cstdarg>
@@ -241,6 +242,13 @@ struct SrcVol { // src_vol (src/meep_internals.hpp:49-82)
   int64_t bst[3] = {0, 0, 0};
   bool use_box = false;
   bool dirty = true;
+  // folded periodic images of the list's points (point k, halo index, sign), for the plan
+  // version img_version
+  int n_img = 0;
+  long img_version = -1;
+  int *d_img_k = nullptr;
+  int64_t *d_img_dst = nullptr;
+  int8_t *d_img_sgn = nullptr;
 };
 
 // ---- device buffers -----------------------------------------------------------------------
@@ -268,6 +276,18 @@ static void ensure_gpu() {
     fail("no usable GPU: the MI355X engine has no CPU path (hipGetDeviceCount: %s)", hipGetErrorString(e));
 }
 
+// how a batched step computes one value of its source row: a table entry is the real (cmp 0)
+// or imaginary part of (amp * s(t)) * mult, a box entry (kind 1) the part cmp of s(t), with
+// s the current (or dipole, for integrated sources) of source-time src at t * dt + toff * dt
+struct SrcRecipe {
+  int sim, src;
+  cdouble amp;
+  double mult;
+  int cmp;
+  double toff;
+  int kind;
+};
+
 struct FieldsBase {
   virtual ~FieldsBase() {}
   HostStructure *s = nullptr;
@@ -281,6 +301,10 @@ struct FieldsBase {
   virtual double last_source_time() const = 0;
   virtual cdouble get_field(int c, const V3 &p) = 0;
   virtual double field_energy() = 0;
+  virtual void adj_begin(void *monitor, int c, V3 mn, V3 mx, int interval, int *box_lo, int *box_n) = 0;
+  virtual void adj_end() = 0;
+  virtual double adj_objective() = 0;
+  virtual void adj_gradient(double *out) = 0;
   virtual int get_array(int c, int cmp, int kind, int ipol, double *out) = 0;
   virtual void set_array(int c, int cmp, int kind, int ipol, const double *in) = 0;
   virtual bool have_component(int c) const = 0;
@@ -510,6 +534,9 @@ template <class R> struct Fields : FieldsBase {
           if (sv.d_idx) (void)hipFree(sv.d_idx);
           if (sv.d_amp) (void)hipFree(sv.d_amp);
           if (sv.d_box) (void)hipFree(sv.d_box);
+          if (sv.d_img_k) (void)hipFree(sv.d_img_k);
+          if (sv.d_img_dst) (void)hipFree(sv.d_img_dst);
+          if (sv.d_img_sgn) (void)hipFree(sv.d_img_sgn);
         }
       for (const Pending &p : pending) {
         (void)hipEventDestroy(p.a);
@@ -640,6 +667,7 @@ template <class R> struct Fields : FieldsBase {
   // slots: 0..19 the field arrays, 20 + 20*p + c the polarisation P of pole p
   void rebuild_plan() {
     HIPCHECK(hipStreamSynchronize(stream));
+    ++plan_version;
     const int nslot = NC + NC * int(pols.size());
     h_slots.assign(2 * nslot, nullptr);
     for (int c = 0; c < NC; ++c)
@@ -872,8 +900,7 @@ template <class R> struct Fields : FieldsBase {
       if (gv.ydir(k) == d && gv.has(d)) return k;
     fail("internal: direction %d has no loop axis", d);
   }
-  void finalize_args(StepArgs<R> &a) const {
-    Grid &g = a.g;
+  void finalize_grid(Grid &g) const {
     for (int k = 0; k < 3; ++k) {
       const int d = g.axdir[k];
       g.alo[k] = d >= 0 ? g.blo[d] : 0;
@@ -885,6 +912,9 @@ template <class R> struct Fields : FieldsBase {
       g.nax[k] = d >= 0 ? gv.num[d] : 0;
       g.wrap[k] = d >= 0 ? wrap_dir[d] : 0;
     }
+  }
+  void finalize_args(StepArgs<R> &a) const {
+    finalize_grid(a.g);
     for (int c = 0; c < 3; ++c) {
       CurlComp<R> &k = a.cc[c];
       axis_slot(k.ax, k.lo, k.hi, k.ysh);
@@ -936,6 +966,7 @@ template <class R> struct Fields : FieldsBase {
   // ---- per-step argument blocks ----
   Grid grid() const {
     Grid g;
+    memset(&g, 0, sizeof g);
     for (int kx = 0; kx < 3; ++kx) {
       const int d = gv.ydir(kx);
       g.axdir[kx] = gv.has(d) ? d : -1;
@@ -1183,11 +1214,14 @@ template <class R> struct Fields : FieldsBase {
   // caller applies with k_src.  Values are (amp * current(t)) * dt for currents
   // (src/step.cpp:307-315) and amp * dipole(t) for integrated sources (src/update_eh.cpp:128-138).
   void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, double time, SrcTab &tab, SrcBoxes &bx,
-                std::vector<LongSrc> &rest) {
+                std::vector<LongSrc> &rest, std::vector<SrcRecipe> *rec = nullptr, int sim = 0, double toff = 0) {
     tab.n = 0;
     tab.lo = INT64_MAX;
     tab.hi = -1;
     bx.n = 0;
+    // batched steps take every value from the step's row: the table's entries first, then
+    // two per box (scal), so the layout does not depend on the time
+    std::vector<SrcRecipe> trec, brec;
     for (SrcVol &sv : sources[ft2]) {
       const SrcTime &st = srcs[sv.src];
       if (st.is_integrated != integrated) continue;
@@ -1198,7 +1232,7 @@ template <class R> struct Fields : FieldsBase {
       for (int q = 0; q < nc; ++q)
         if (which[q] == (integrated ? sv.c : c2)) slot = q;
       if (slot < 0) continue;
-      if (scal == 0.0) continue; // exact zero contributes nothing (the reference subtracts 0)
+      if (!rec && scal == 0.0) continue; // exact zero contributes nothing (the reference subtracts 0)
       if (tab.n + int(sv.index.size()) <= SRC_TAB) {
         for (size_t j = 0; j < sv.index.size(); ++j) {
           const cdouble A = (sv.amp[j] * scal) * mult;
@@ -1208,6 +1242,7 @@ template <class R> struct Fields : FieldsBase {
           tab.lo = std::min(tab.lo, sv.index[j]);
           tab.hi = std::max(tab.hi, sv.index[j]);
           ++tab.n;
+          if (rec) trec.push_back({sim, sv.src, sv.amp[j], mult, cmp, toff, 0});
         }
         continue;
       }
@@ -1225,9 +1260,19 @@ template <class R> struct Fields : FieldsBase {
         bx.amp[b] = sv.d_box;
         bx.scal[b] = make_double2(scal.real(), scal.imag());
         bx.mult[b] = mult;
+        if (rec) {
+          brec.push_back({sim, sv.src, 1.0, 1.0, 0, toff, 1});
+          brec.push_back({sim, sv.src, 1.0, 1.0, 1, toff, 1});
+        }
         continue;
       }
       rest.push_back({&sv, slot, scal, mult});
+    }
+    if (rec) {
+      tab.vbase = int(rec->size());
+      rec->insert(rec->end(), trec.begin(), trec.end());
+      bx.vbase = int(rec->size());
+      rec->insert(rec->end(), brec.begin(), brec.end());
     }
   }
   // device copies of a long source list: indices, amplitudes, and the dense box when the
@@ -1274,11 +1319,72 @@ template <class R> struct Fields : FieldsBase {
     }
     sv.dirty = false;
   }
-  void apply_long(const LongSrc &ls, R *fbase, int cmp) {
+  // A long list applied after the curl kernel: the kernel has already written the folded
+  // periodic images of these points, so they are written again from the updated values.
+  // (Integrated lists restore D afterwards and need no images.)
+  void apply_long(const LongSrc &ls, R *fbase, int cmp, bool images = true) {
     kbegin(MP_K_SRC);
     launch_src<R>(fbase, ls.sv->d_idx, ls.sv->d_amp, int(ls.sv->index.size()), make_double2(ls.scal.real(), ls.scal.imag()),
                   ls.mult, cmp, stream);
+    if (images) {
+      prepare_images(*ls.sv);
+      launch_src_images<R>(fbase, ls.sv->d_idx, ls.sv->d_img_k, ls.sv->d_img_dst, ls.sv->d_img_sgn, ls.sv->n_img, stream);
+    }
     kend();
+  }
+  long plan_version = 0;
+  void prepare_images(SrcVol &sv) {
+    if (sv.img_version == plan_version) return;
+    std::vector<int> ks;
+    std::vector<int64_t> dsts;
+    std::vector<int8_t> sgs;
+    const int c = sv.c;
+    for (size_t k = 0; k < sv.index.size(); ++k) {
+      int dd[3], sg[3];
+      for (int d = 0; d < 3; ++d) {
+        dd[d] = -1;
+        sg[d] = 1;
+        if (!gv.has(d) || !wrap_dir[d]) continue;
+        const int j = sv.jpos[3 * k + d], y = gv.ysh(c, d);
+        const int dst = y == 0 ? (j == gv.num[d] ? 0 : -1) : (j == 0 ? gv.num[d] : -1);
+        if (dst >= 0) dd[d] = dst, sg[d] = wrap_dir[d];
+      }
+      for (int sub = 1; sub < 8; ++sub) {
+        int64_t off = 0;
+        int sign = 1;
+        bool ok = true;
+        for (int d = 0; d < 3; ++d)
+          if ((sub >> d) & 1) {
+            ok = ok && dd[d] >= 0;
+            if (dd[d] >= 0) {
+              off += int64_t(dd[d] - sv.jpos[3 * k + d]) * gv.st[d];
+              sign *= sg[d];
+            }
+          }
+        if (!ok) continue;
+        const int64_t dst = sv.index[k] + off;
+        if (dst < 0 || size_t(dst) >= gv.ntot) fail("internal: source image outside the array");
+        ks.push_back(int(k));
+        dsts.push_back(dst);
+        sgs.push_back(int8_t(sign));
+      }
+    }
+    if (sv.d_img_k) (void)hipFree(sv.d_img_k);
+    if (sv.d_img_dst) (void)hipFree(sv.d_img_dst);
+    if (sv.d_img_sgn) (void)hipFree(sv.d_img_sgn);
+    sv.d_img_k = nullptr;
+    sv.d_img_dst = nullptr;
+    sv.d_img_sgn = nullptr;
+    sv.n_img = int(ks.size());
+    if (sv.n_img) {
+      HIPCHECK(hipMalloc(&sv.d_img_k, sizeof(int) * ks.size()));
+      HIPCHECK(hipMalloc(&sv.d_img_dst, sizeof(int64_t) * dsts.size()));
+      HIPCHECK(hipMalloc(&sv.d_img_sgn, sgs.size()));
+      HIPCHECK(hipMemcpy(sv.d_img_k, ks.data(), sizeof(int) * ks.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(sv.d_img_dst, dsts.data(), sizeof(int64_t) * dsts.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(sv.d_img_sgn, sgs.data(), sgs.size(), hipMemcpyHostToDevice));
+    }
+    sv.img_version = plan_version;
   }
 
   // One fields::step() (src/step.cpp:35-139); the finiteness probe is launched when probe.
@@ -1332,7 +1438,7 @@ template <class R> struct Fields : FieldsBase {
         launch_gather<R>(static_cast<R *>(d_scratch.p) + off, D, ls.sv->d_idx, int(ls.sv->index.size()), stream);
         off += ls.sv->index.size();
       }
-      for (const LongSrc &ls : bigd) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp);
+      for (const LongSrc &ls : bigd) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp, false);
       StepArgs<R> U = A;
       U.cur.n = 0;
       U.curb.n = 0;
@@ -1347,7 +1453,8 @@ template <class R> struct Fields : FieldsBase {
     if (!plan_valid) rebuild_plan();
     boundaries(1); // D, P_E, E (step_boundaries D_stuff .. E_stuff)
     t += 1;
-    update_dfts();
+    if (adj && adj->recording) record_design();
+    if (!dft_off) update_dfts();
     if (probe && n_chk)
       launch_check<R>(static_cast<const R *const *>(d_chk_ptrs.p), static_cast<const int64_t *>(d_chk_idx.p),
                       static_cast<const double *>(d_chk_w.p), n_chk, static_cast<int *>(d_flag.p), stream);
@@ -1479,6 +1586,459 @@ template <class R> struct Fields : FieldsBase {
       HIPCHECK(hipMemcpyAsync(sv.p, sv.b->p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     return double(energy_of(0)) + magnetic;
+  }
+
+  // ---- reverse mode ----------------------------------------------------------------------
+  // Gradient of J = sum |a|^2 over the accumulators a of a dft_fields monitor with respect to
+  // chi1inv of one electric component on a design box, by the transposed step run backwards
+  // (adjoint.hip).  The forward steps record D at the design points: every step (interval 0)
+  // or, with checkpoints of the whole state every `interval` steps, recomputed per segment
+  // during the backward sweep.
+  struct Adj {
+    Flux<R> *mon = nullptr;
+    int c = -1;
+    int lo[3], hi[3]; // design box, loop-axis array indices of component c
+    size_t npts = 0;
+    int interval = 0;
+    long t0 = 0, nrec = 0;
+    bool recording = false;
+    std::vector<std::unique_ptr<DevBuf>> hist; // history mode: blocks of HBLK steps
+    struct Checkpoint {
+      long t;
+      std::vector<std::unique_ptr<DevBuf>> data; // one per state array, in state_arrays() order
+    };
+    std::vector<Checkpoint> cps;
+    DevBuf seg; // checkpoint mode: one segment of history
+  };
+  static constexpr long HBLK = 256;
+  std::unique_ptr<Adj> adj;
+  bool dft_off = false; // recomputation steps leave the DFT accumulators alone
+
+  // every distinct state array (real parts; the reverse mode needs real fields)
+  std::vector<R *> state_arrays() const {
+    std::vector<R *> v;
+    auto add = [&](R *p) {
+      if (p && std::find(v.begin(), v.end(), p) == v.end()) v.push_back(p);
+    };
+    for (int c = 0; c < NC; ++c) {
+      add(f[c][0]);
+      add(f_u[c][0]);
+      add(f_w[c][0]);
+    }
+    return v;
+  }
+  void adj_begin(void *monitor, int c, V3 mn, V3 mxv, int interval, int *box_lo, int *box_n) override {
+    if (!is_real) fail("the reverse mode needs real fields (use_real_fields)");
+    if (!pols.empty()) fail("the reverse mode does not support dispersive materials");
+    for (auto &lst : sources)
+      for (auto &sv : lst)
+        if (srcs[sv.src].is_integrated) fail("the reverse mode does not support integrated sources");
+    if (!is_electric(c) || !has_field(c)) fail("design component must be an electric field component");
+    if (!d_chi1inv[c]) fail("design component has no chi1inv array (give the structure a non-trivial eps)");
+    if (interval > 0 && t != 0) fail("checkpointed recording must start at t = 0");
+    Flux<R> *mon = nullptr;
+    for (auto &fl : fluxes)
+      if (fl.get() == monitor) mon = fl.get();
+    if (!mon || !mon->H.empty()) fail("the objective must be a dft_fields monitor of this fields object");
+    std::unique_ptr<Adj> A(new Adj);
+    A->mon = mon;
+    A->c = c;
+    // component c's points whose positions lie in [mn, mxv]
+    for (int k = 0; k < 3; ++k) {
+      const int d = gv.ydir(k);
+      if (!gv.has(d)) {
+        A->lo[k] = A->hi[k] = 0;
+        continue;
+      }
+      const int y = gv.ysh(c, d);
+      // position of index j: (2 j + y) / (2 a)
+      A->lo[k] = std::max(0, int(std::ceil((mn.t[d] * 2 * a - y) / 2 - 1e-9)));
+      A->hi[k] = std::min(gv.num[d], int(std::floor((mxv.t[d] * 2 * a - y) / 2 + 1e-9)));
+      if (A->hi[k] < A->lo[k]) fail("empty design region");
+      box_lo[d] = A->lo[k];
+      box_n[d] = A->hi[k] - A->lo[k] + 1;
+    }
+    A->npts = 1;
+    for (int k = 0; k < 3; ++k) A->npts *= size_t(A->hi[k] - A->lo[k] + 1);
+    A->interval = interval;
+    A->t0 = t;
+    A->recording = true;
+    adj = std::move(A);
+    if (interval > 0) {
+      // t = 0: every array is zero, including those still to be allocated lazily
+      adj->cps.emplace_back();
+      adj->cps.back().t = 0;
+    }
+  }
+  void adj_end() override { adj.reset(); }
+  void save_checkpoint() {
+    HIPCHECK(hipStreamSynchronize(stream));
+    typename Adj::Checkpoint cp;
+    cp.t = t;
+    for (R *p : state_arrays()) {
+      cp.data.emplace_back(new DevBuf);
+      cp.data.back()->alloc(sizeof(R) * gv.ntot);
+      HIPCHECK(hipMemcpyAsync(cp.data.back()->p, p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+    }
+    adj->cps.push_back(std::move(cp));
+  }
+  // the D at the design points after step (t - t0 - 1) -> history
+  void record_design() {
+    Adj &A = *adj;
+    const long n = A.nrec;
+    R *dst;
+    if (A.interval > 0) {
+      if (!seg_active) {
+        A.nrec += 1;
+        if ((t - A.t0) % A.interval == 0) save_checkpoint();
+        return;
+      }
+      dst = static_cast<R *>(A.seg.p) + size_t(n - seg_first) * A.npts;
+    }
+    else {
+      if (size_t(n / HBLK) >= A.hist.size()) {
+        A.hist.emplace_back(new DevBuf);
+        A.hist.back()->alloc(sizeof(R) * A.npts * HBLK);
+      }
+      dst = static_cast<R *>(A.hist[n / HBLK]->p) + size_t(n % HBLK) * A.npts;
+    }
+    const int dc = A.c + 10;
+    // the whole grid in its loop-axis view (the kernel indexes by loop axis)
+    Grid G = grid();
+    finalize_grid(G);
+    for (int k = 0; k < 3; ++k) {
+      const int n = gv.has(gv.ydir(k)) ? gv.num[gv.ydir(k)] : 0;
+      if (A.lo[k] < 0 || A.hi[k] > n || G.an[k] != n + 1 || G.ast[k] != (gv.has(gv.ydir(k)) ? gv.st[gv.ydir(k)] : 0))
+        fail("internal: design box outside the grid");
+    }
+    launch_record<R>(dst, f[dc][0], G, A.lo, A.hi, stream);
+    A.nrec += 1;
+  }
+  bool seg_active = false;
+  long seg_first = 0;
+  double adj_objective() override {
+    if (!adj) fail("no reverse-mode recording");
+    double J = 0;
+    for (DftChunk<R> *ch : adj->mon->E) {
+      const std::vector<cdouble> v = read_dft(*ch);
+      for (const cdouble &z : v) J += std::norm(z);
+    }
+    return J;
+  }
+  void adj_gradient(double *out) override {
+    if (!adj) fail("no reverse-mode recording");
+    Adj &A = *adj;
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (!plan_valid) rebuild_plan();
+    const long N = A.nrec;
+    // cotangent arrays, aliased as the forward arrays are
+    std::vector<R *> fwd = state_arrays();
+    std::vector<std::unique_ptr<DevBuf>> barbuf;
+    std::map<const R *, R *> bar;
+    for (R *p : fwd) {
+      barbuf.emplace_back(new DevBuf);
+      barbuf.back()->alloc(sizeof(R) * gv.ntot);
+      HIPCHECK(hipMemsetAsync(barbuf.back()->p, 0, sizeof(R) * gv.ntot, stream));
+      bar[p] = static_cast<R *>(barbuf.back()->p);
+    }
+    auto B = [&](const R *p) -> R * { return p ? bar.at(p) : nullptr; };
+    DevBuf cbar[3];
+    for (int q = 0; q < 3; ++q) {
+      cbar[q].alloc(sizeof(R) * gv.ntot);
+      HIPCHECK(hipMemsetAsync(cbar[q].p, 0, sizeof(R) * gv.ntot, stream));
+    }
+    // argument blocks of both phases over the whole grid
+    int whichB[3], whichD[3];
+    StepArgs<R> AB = base_args(), AD = base_args();
+    AB.nc = curl_comps(B_ft, 0, AB.cc, whichB);
+    upd_comps(H_ft, 0, whichB, AB.nc, AB.uc, false);
+    AD.nc = curl_comps(D_ft, 0, AD.cc, whichD);
+    upd_comps(E_ft, 0, whichD, AD.nc, AD.uc, false);
+    if (!plan_valid) fail("internal: the reverse sweep found arrays still to allocate");
+    int full_lo[3], full_hi[3];
+    for (int d = 0; d < 3; ++d) {
+      full_lo[d] = 0;
+      full_hi[d] = gv.has(d) ? gv.num[d] : 0;
+    }
+    for (StepArgs<R> *S : {&AB, &AD}) {
+      for (int d = 0; d < 3; ++d) {
+        S->g.blo[d] = full_lo[d];
+        S->g.bhi[d] = full_hi[d];
+      }
+      finalize_args(*S);
+    }
+    check_args(AB, true, true, false);
+    check_args(AD, true, true, false);
+    auto slots = [&](const StepArgs<R> &S) {
+      AdjSlots<R> b;
+      memset(&b, 0, sizeof b);
+      for (int q = 0; q < 3; ++q) {
+        b.fb[q] = S.cc[q].f ? B(S.cc[q].f) : nullptr;
+        b.fub[q] = B(S.cc[q].fu);
+        b.cbar[q] = static_cast<R *>(cbar[q].p);
+      }
+      return b;
+    };
+    auto uslots = [&](const StepArgs<R> &S) {
+      AdjSlots<R> b;
+      memset(&b, 0, sizeof b);
+      for (int q = 0; q < 3; ++q)
+        if (S.uc[q].f) {
+          b.fb[q] = B(S.uc[q].f);
+          b.gb[q] = B(S.uc[q].g);
+          b.fwb[q] = B(S.uc[q].fw);
+        }
+      return b;
+    };
+    const AdjSlots<R> cB = slots(AB), cD = slots(AD), uH = uslots(AB), uE = uslots(AD);
+    // partner gathers: the curl of slot q reads g1 at i + s1 and i, g2 at i and i + s2
+    auto gathers = [&](const StepArgs<R> &S) {
+      std::vector<AdjGather<R>> out;
+      auto entry = [&](const R *g, int q, int64_t s, int plus) {
+        AdjGather<R> *ga = nullptr;
+        for (auto &x : out)
+          if (x.gb == B(g)) ga = &x;
+        if (!ga) {
+          out.emplace_back();
+          memset(&out.back(), 0, sizeof(AdjGather<R>));
+          out.back().gb = B(g);
+          ga = &out.back();
+        }
+        if (ga->n >= 4) fail("internal: too many gather entries");
+        int ax = -1, step = 0;
+        for (int k = 0; k < 3; ++k) {
+          const int d = gv.ydir(k);
+          if (gv.has(d) && gv.st[d] == s) ax = k, step = 1;
+          if (gv.has(d) && gv.st[d] == -s) ax = k, step = -1;
+        }
+        if (ax < 0) fail("internal: curl stride matches no axis");
+        const int e = ga->n++;
+        ga->cbar[e] = static_cast<const R *>(cbar[q].p);
+        ga->s[e] = s;
+        ga->ax[e] = ax;
+        ga->step[e] = step;
+        ga->plus[e] = plus;
+      };
+      for (int q = 0; q < 3; ++q) {
+        const CurlComp<R> &k = S.cc[q];
+        if (!k.f) continue;
+        if (k.g1) entry(k.g1, q, k.s1, 1);
+        if (k.g2) entry(k.g2, q, k.s2, 0);
+      }
+      return out;
+    };
+    const std::vector<AdjGather<R>> gD = gathers(AD), gB = gathers(AB);
+    // halo transposes (every halo point, folded planes included), by field-type group
+    AdjHalo hal[2];
+    build_adj_halo(hal, bar);
+    // objective accumulators and per-chunk scratch
+    struct Obj {
+      DftChunk<R> *ch;
+      DevBuf acc, G;
+    };
+    std::vector<std::unique_ptr<Obj>> obj;
+    for (DftChunk<R> *ch : A.mon->E) {
+      std::unique_ptr<Obj> o(new Obj);
+      o->ch = ch;
+      const std::vector<cdouble> v = read_dft(*ch);
+      o->acc.alloc(sizeof(double2) * v.size());
+      HIPCHECK(hipMemcpy(o->acc.p, v.data(), sizeof(double2) * v.size(), hipMemcpyHostToDevice));
+      o->G.alloc(sizeof(double) * std::max(ch->npts, 1));
+      obj.push_back(std::move(o));
+    }
+    DevBuf grad;
+    grad.alloc(sizeof(double) * A.npts);
+    HIPCHECK(hipMemsetAsync(grad.p, 0, sizeof(double) * A.npts, stream));
+    Grid gfull = AD.g;
+    int dslot = -1;
+    for (int q = 0; q < 3; ++q)
+      if (whichD[q] == A.c + 10) dslot = q;
+    if (dslot < 0 || !AD.uc[dslot].f) fail("internal: design component has no E update");
+    const int64_t lo_dev = 0;
+    (void)lo_dev;
+    // one reverse step for forward step n (t before it: t0 + n); hist points at D after it
+    auto reverse_step = [&](long n, const R *hist) {
+      const long tafter = A.t0 + n + 1;
+      for (auto &o : obj) {
+        DftChunk<R> &ch = *o->ch;
+        if (tafter % ch.decimation) continue;
+        const double time = (is_magnetic(ch.c) || is_B(ch.c)) ? tafter * dt - 0.5 * dt : tafter * dt;
+        launch_adj_dft<R>(static_cast<const int64_t *>(ch.idx.p), static_cast<const double *>(ch.w.p),
+                          static_cast<const double2 *>(o->acc.p), static_cast<const double *>(ch.omega.p), ch.npts, ch.nfreq, time,
+                          ch.scale.real(), ch.scale.imag(), static_cast<double *>(o->G.p), stream);
+        R *eb = B(f[ch.c][0]);
+        const int64_t offs[4] = {0, ch.avg1, ch.avg2, ch.avg1 + ch.avg2};
+        const int nof = ch.avg2 ? 4 : ch.avg1 ? 2 : 1;
+        const double coef = nof == 4 ? 0.25 : nof == 2 ? 0.5 : 1.0;
+        for (int q = 0; q < nof; ++q)
+          launch_adj_dft_scatter<R>(eb, static_cast<const int64_t *>(ch.idx.p), static_cast<const double *>(o->G.p), ch.npts, offs[q],
+                                    coef, stream);
+      }
+      run_adj_halo(hal[1]);
+      AdjDesign<R> dz;
+      memset(&dz, 0, sizeof dz);
+      dz.slot = dslot;
+      for (int k = 0; k < 3; ++k) dz.lo[k] = A.lo[k], dz.hi[k] = A.hi[k];
+      dz.hist = hist;
+      dz.grad = static_cast<double *>(grad.p);
+      launch_adj_upd<R>(AD, uE, dz, stream);
+      launch_adj_curl<R>(AD, cD, stream);
+      for (const AdjGather<R> &ga : gD) launch_adj_gather<R>(gfull, ga, stream);
+      run_adj_halo(hal[0]);
+      bool any_h = false;
+      for (int q = 0; q < 3; ++q) any_h = any_h || AB.uc[q].f;
+      if (any_h) {
+        AdjDesign<R> none;
+        memset(&none, 0, sizeof none);
+        none.slot = -1;
+        launch_adj_upd<R>(AB, uH, none, stream);
+      }
+      launch_adj_curl<R>(AB, cB, stream);
+      for (const AdjGather<R> &ga : gB) launch_adj_gather<R>(gfull, ga, stream);
+    };
+    if (A.interval <= 0) {
+      for (long n = N - 1; n >= 0; --n)
+        reverse_step(n, static_cast<const R *>(A.hist[n / HBLK]->p) + size_t(n % HBLK) * A.npts);
+    }
+    else {
+      // keep the final state, recompute each segment from its checkpoint, sweep it backwards
+      HIPCHECK(hipStreamSynchronize(stream));
+      const long tend = t;
+      typename Adj::Checkpoint fin;
+      fin.t = t;
+      for (R *p : fwd) {
+        fin.data.emplace_back(new DevBuf);
+        fin.data.back()->alloc(sizeof(R) * gv.ntot);
+        HIPCHECK(hipMemcpyAsync(fin.data.back()->p, p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+      }
+      A.seg.alloc(sizeof(R) * A.npts * size_t(A.interval));
+      const long nrec_total = N;
+      for (long sidx = long(A.cps.size()) - 1; sidx >= 0; --sidx) {
+        const long first = A.cps[sidx].t - A.t0;
+        if (first >= nrec_total) continue;
+        const long last = std::min(nrec_total, first + A.interval); // steps first .. last-1
+        restore_state(A.cps[sidx], fwd);
+        t = A.cps[sidx].t;
+        seg_active = true;
+        seg_first = first;
+        A.nrec = first;
+        dft_off = true;
+        A.recording = true;
+        for (long q = first; q < last; ++q) one_step(false);
+        dft_off = false;
+        seg_active = false;
+        for (long n = last - 1; n >= first; --n)
+          reverse_step(n, static_cast<const R *>(A.seg.p) + size_t(n - first) * A.npts);
+      }
+      restore_state(fin, fwd);
+      t = tend;
+      A.nrec = nrec_total;
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpy(out, grad.p, sizeof(double) * A.npts, hipMemcpyDeviceToHost));
+  }
+  void restore_state(const typename Adj::Checkpoint &cp, const std::vector<R *> &arrays) {
+    if (cp.data.empty()) {
+      for (R *p : arrays) HIPCHECK(hipMemsetAsync(p, 0, sizeof(R) * gv.ntot, stream));
+      return;
+    }
+    if (cp.data.size() != arrays.size()) fail("internal: the state arrays changed since the checkpoint");
+    for (size_t q = 0; q < arrays.size(); ++q)
+      HIPCHECK(hipMemcpyAsync(arrays[q], cp.data[q]->p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+  }
+  // Halo transposes: every halo point of every allocated array (the kernels' folded planes
+  // included), grouped by owned source index so one thread owns each source.
+  struct AdjHalo {
+    int nsrc = 0;
+    DevBuf first, src, dst, sgn, slot, bars;
+  };
+  void build_adj_halo(AdjHalo hal[2], const std::map<const R *, R *> &bar) {
+    struct E {
+      uint32_t src, dst;
+      int8_t sgn;
+      uint16_t slot;
+    };
+    std::vector<E> lists[2];
+    std::vector<R *> bars;
+    std::map<R *, uint16_t> slot_of;
+    bool B_redundant[5];
+    for (int q = 0; q < 5; ++q) B_redundant[q] = f[5 + q][0] == f[15 + q][0];
+    for (int corig = 0; corig < NC; ++corig) {
+      if (!f[corig][0]) continue;
+      if (is_B(corig) && B_redundant[corig - 15]) continue;
+      R *bp = bar.at(f[corig][0]);
+      if (!slot_of.count(bp)) {
+        slot_of[bp] = uint16_t(bars.size());
+        bars.push_back(bp);
+      }
+      int lo[3], hi[3];
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = 0;
+        hi[d] = gv.has(d) ? gv.num[d] : 0;
+      }
+      for (int d = 0; d < 3; ++d) {
+        if (!gv.has(d)) continue;
+        const int plane = gv.ysh(corig, d) ? gv.num[d] : 0;
+        int blo[3], bhi[3];
+        for (int q = 0; q < 3; ++q) blo[q] = lo[q], bhi[q] = hi[q];
+        blo[d] = bhi[d] = plane;
+        for (int j0 = blo[0]; j0 <= bhi[0]; ++j0)
+          for (int j1 = blo[1]; j1 <= bhi[1]; ++j1)
+            for (int j2 = blo[2]; j2 <= bhi[2]; ++j2) {
+              int j[3] = {j0, j1, j2};
+              IV here = gv.iloc(corig, j);
+              const int64_t n = gv.index(corig, here);
+              cdouble ph = 1.0;
+              if (!locate(&here, &ph) || on_metal_boundary(here)) continue;
+              if (std::fabs(ph.imag()) > 1e-13 || std::fabs(std::fabs(ph.real()) - 1.0) > 1e-13)
+                fail("the reverse mode needs real Bloch phases");
+              const int ft = field_type(corig);
+              const int grp = (ft == 1 || ft == 3) ? 0 : 1;
+              lists[grp].push_back({uint32_t(gv.index(corig, here)), uint32_t(n), int8_t(ph.real() > 0 ? 1 : -1), slot_of[bp]});
+            }
+        if (plane == 0)
+          lo[d] = 1;
+        else
+          hi[d] = plane - 1;
+      }
+    }
+    for (int g = 0; g < 2; ++g) {
+      std::vector<E> &L = lists[g];
+      std::stable_sort(L.begin(), L.end(), [](const E &x, const E &y) { return x.src < y.src; });
+      std::vector<int> first;
+      std::vector<uint32_t> src, dst;
+      std::vector<int8_t> sg;
+      std::vector<uint16_t> sl;
+      for (size_t q = 0; q < L.size(); ++q) {
+        if (q == 0 || L[q].src != L[q - 1].src) {
+          first.push_back(int(q));
+          src.push_back(L[q].src);
+        }
+        dst.push_back(L[q].dst);
+        sg.push_back(L[q].sgn);
+        sl.push_back(L[q].slot);
+        if (size_t(L[q].dst) >= gv.ntot || size_t(L[q].src) >= gv.ntot) fail("internal: halo transpose outside the array");
+      }
+      first.push_back(int(L.size()));
+      AdjHalo &h = hal[g];
+      h.nsrc = int(src.size());
+      auto up = [&](DevBuf &b, const void *d, size_t n) {
+        b.alloc(n ? n : 1);
+        if (n) HIPCHECK(hipMemcpy(b.p, d, n, hipMemcpyHostToDevice));
+      };
+      up(h.first, first.data(), sizeof(int) * first.size());
+      up(h.src, src.data(), sizeof(uint32_t) * src.size());
+      up(h.dst, dst.data(), sizeof(uint32_t) * dst.size());
+      up(h.sgn, sg.data(), sg.size());
+      up(h.slot, sl.data(), sizeof(uint16_t) * sl.size());
+      up(h.bars, bars.data(), sizeof(R *) * bars.size());
+    }
+  }
+  void run_adj_halo(AdjHalo &h) {
+    launch_adj_halo<R>(static_cast<const int *>(h.first.p), static_cast<const uint32_t *>(h.src.p), static_cast<const uint32_t *>(h.dst.p),
+                       static_cast<const int8_t *>(h.sgn.p), static_cast<const uint16_t *>(h.slot.p), h.nsrc,
+                       static_cast<R *const *>(h.bars.p), stream);
   }
 
   void step_n(long n, bool) override {
@@ -2234,6 +2794,42 @@ double mp_fields_round_time(const mp_fields *f) {
 long mp_fields_t(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->t_now(); }
 double mp_fields_dt(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->dt_now(); }
 double mp_fields_last_source_time(mp_fields *f) { return F_->last_source_time(); }
+struct AdjHandle {
+  FieldsBase *owner;
+};
+mp_adjoint *mp_adjoint_new(mp_fields *f, mp_dft_fields *objective, int c, const double *vmin, const double *vmax, int interval,
+                           int *lo_xyz, int *n_xyz) {
+  AdjHandle *h = new AdjHandle{reinterpret_cast<FieldsBase *>(f)};
+  int lo[3] = {0, 0, 0}, box[3] = {1, 1, 1};
+  int rc = guard([&] {
+    FluxHandle *mon = reinterpret_cast<FluxHandle *>(objective);
+    if (!mon || mon->owner != h->owner) fail("the objective monitor belongs to another fields object");
+    h->owner->adj_begin(mon->fl, c, to_v3(vmin), to_v3(vmax), interval, lo, box);
+  });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (lo_xyz) lo_xyz[d] = lo[d];
+    if (n_xyz) n_xyz[d] = box[d];
+  }
+  return reinterpret_cast<mp_adjoint *>(h);
+}
+void mp_adjoint_delete(mp_adjoint *a) {
+  AdjHandle *h = reinterpret_cast<AdjHandle *>(a);
+  if (!h) return;
+  guard([&] { h->owner->adj_end(); });
+  delete h;
+}
+int mp_adjoint_objective(mp_adjoint *a, double *J) {
+  AdjHandle *h = reinterpret_cast<AdjHandle *>(a);
+  return guard([&] { *J = h->owner->adj_objective(); });
+}
+int mp_adjoint_gradient(mp_adjoint *a, double *grad) {
+  AdjHandle *h = reinterpret_cast<AdjHandle *>(a);
+  return guard([&] { h->owner->adj_gradient(grad); });
+}
 int mp_fields_field_energy(mp_fields *f, double *energy) {
   return guard([&] { *energy = F_->field_energy(); });
 }

@@ -19,29 +19,13 @@
 //   E = chi1inv f_minus_p  step_update_EDHB      src/step_generic.cpp:682-701,774-782
 //   Lorentz/Drude P        update_P              src/susceptibility.cpp:188-262
 #include "engine.hpp"
+#include "stencil.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 namespace mx {
-
-constexpr int BLOCK = 256;
-
-// logical block index, dealt to XCDs in contiguous runs (gridDim.x is a multiple of 8);
-// rev walks them in descending order
-__device__ __forceinline__ uint32_t xcd_block(int rev = 0) {
-  const uint32_t nb = gridDim.x, b = blockIdx.x;
-  const uint32_t lb = (b & 7u) * (nb >> 3) + (b >> 3);
-  return rev ? nb - 1 - lb : lb;
-}
-
-// halo point along a folded axis fed by the owned point z of a component with Yee shift ysh
-// there; -1 when z is not the wrapped end
-__device__ __forceinline__ int zwrap_dst(int ysh, int z, int n) {
-  if (ysh == 0) return z == n ? 0 : -1;
-  return z == 0 ? n : -1;
-}
 
 // Periodic images of the owned point (j0, j1, j2) along the folded axes: offset of each axis's
 // image (0 if none) and its sign.
@@ -86,34 +70,24 @@ template <class R> __device__ __forceinline__ void put(R *f, uint32_t i, R v, co
   }
 }
 
-// coordinate along loop axis k (a select over registers, never an indexed access)
-__device__ __forceinline__ int asel(int j0, int j1, int j2, int k) { return k == 0 ? j0 : (k == 1 ? j1 : j2); }
-
-__device__ __forceinline__ bool slot_owned(const AxisSlot &s, int j0, int j1, int j2) {
-  return j0 >= s.lo[0] && j0 <= s.hi[0] && j1 >= s.lo[1] && j1 <= s.hi[1] && j2 >= s.lo[2] && j2 <= s.hi[2];
-}
-// on_metal_boundary for an owned point (src/boundaries.cpp:191-205)
-__device__ __forceinline__ bool slot_metal(const AxisSlot &s, int j0, int j1, int j2) {
-  return j0 == s.mhi[0] || j0 == s.mlo[0] || j1 == s.mhi[1] || j1 == s.mlo[1] || j2 == s.mhi[2] || j2 == s.mlo[2];
-}
-
 __device__ __forceinline__ bool in_tab(const SrcTab &t, uint32_t i) {
   return t.n && int64_t(i) >= t.lo && int64_t(i) <= t.hi;
 }
-template <class R> __device__ __forceinline__ R apply_tab(const SrcTab &t, int c, uint32_t i, R v) {
+// vrow: the step's row of source values (batched steps), null when the values are in t
+template <class R> __device__ __forceinline__ R apply_tab(const SrcTab &t, int c, uint32_t i, R v, const double *vrow) {
 #pragma unroll
   for (int s = 0; s < SRC_TAB; ++s)
-    if (s < t.n && t.comp[s] == c && t.idx[s] == int64_t(i)) v = R(double(v) - t.val[s]);
+    if (s < t.n && t.comp[s] == c && t.idx[s] == int64_t(i)) v = R(double(v) - (vrow ? vrow[t.vbase + s] : t.val[s]));
   return v;
 }
 
 // value of box source S at loop-axis indices (j0, j1, j2); contraction is off so the complex
 // product rounds as the host's does
-template <int S> __device__ __forceinline__ double box_value(const SrcBoxes &b, int cmp, int j0, int j1, int j2) {
+template <int S> __device__ __forceinline__ double box_value(const SrcBoxes &b, int cmp, int j0, int j1, int j2, const double *vrow) {
 #pragma clang fp contract(off)
   const int64_t k = int64_t(j0 - b.lo[S][0]) * b.bst[S][0] + int64_t(j1 - b.lo[S][1]) * b.bst[S][1] +
                     int64_t(j2 - b.lo[S][2]) * b.bst[S][2];
-  const double2 am = b.amp[S][k], sc = b.scal[S];
+  const double2 am = b.amp[S][k], sc = vrow ? make_double2(vrow[b.vbase + 2 * S], vrow[b.vbase + 2 * S + 1]) : b.scal[S];
   const double re = am.x * sc.x - am.y * sc.y, im = am.x * sc.y + am.y * sc.x;
   return cmp ? im * b.mult[S] : re * b.mult[S];
 }
@@ -121,22 +95,19 @@ template <int S> __device__ __forceinline__ bool in_box(const SrcBoxes &b, int c
   return S < b.n && b.comp[S] == c && j0 >= b.lo[S][0] && j0 <= b.hi[S][0] && j1 >= b.lo[S][1] && j1 <= b.hi[S][1] &&
          j2 >= b.lo[S][2] && j2 <= b.hi[S][2];
 }
-template <class R> __device__ __forceinline__ R apply_boxes(const SrcBoxes &b, int cmp, int c, int j0, int j1, int j2, R v) {
+template <class R>
+__device__ __forceinline__ R apply_boxes(const SrcBoxes &b, int cmp, int c, int j0, int j1, int j2, R v, const double *vrow) {
   static_assert(SRC_BOX == 2, "one term per box");
-  if (in_box<0>(b, c, j0, j1, j2)) v = R(double(v) - box_value<0>(b, cmp, j0, j1, j2));
-  if (in_box<1>(b, c, j0, j1, j2)) v = R(double(v) - box_value<1>(b, cmp, j0, j1, j2));
+  if (in_box<0>(b, c, j0, j1, j2)) v = R(double(v) - box_value<0>(b, cmp, j0, j1, j2, vrow));
+  if (in_box<1>(b, c, j0, j1, j2)) v = R(double(v) - box_value<1>(b, cmp, j0, j1, j2, vrow));
   return v;
 }
 
 // ---------------------------------------------------------------------------------------
 // generic kernel: one point per thread, full uPML logic, over the launch box
 
-template <class R> __device__ __forceinline__ bool pml_on(const R *sig, const R *kap, int k) {
-  return sig && (sig[k] != R(0) || kap[k] != R(1));
-}
-
 template <class R, bool CURL, bool UPD, int C>
-__device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, int j0, int j1, int j2, R cu) {
+__device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, int j0, int j1, int j2, R cu, const double *vrow) {
   R fv = R(0);
   if (CURL) {
     const CurlComp<R> &k = a.cc[C];
@@ -161,8 +132,8 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
       k.fu[i] = fun;
       fv = k.siginvu[ku] * ((k.kapu[ku] - k.sigu[ku]) * fv + fun - fprev);
     }
-    if (in_tab(a.cur, i)) fv = apply_tab(a.cur, C, i, fv);
-    if (a.curb.n) fv = apply_boxes(a.curb, a.cmp, C, j0, j1, j2, fv);
+    if (in_tab(a.cur, i)) fv = apply_tab(a.cur, C, i, fv, vrow);
+    if (a.curb.n) fv = apply_boxes(a.curb, a.cmp, C, j0, j1, j2, fv, vrow);
     if (slot_metal(k.ax, j0, j1, j2)) fv = R(0);
     put(k.f, i, fv, a.g, k.ax, j0, j1, j2);
   }
@@ -172,8 +143,8 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
 #pragma unroll
     for (int p = 0; p < MAXPOLE; ++p)
       if (p < a.npole && u.P[p]) gv -= u.P[p][i];
-    if (in_tab(a.dip, i)) gv = apply_tab(a.dip, C, i, gv);
-    if (a.dipb.n) gv = apply_boxes(a.dipb, a.cmp, C, j0, j1, j2, gv);
+    if (in_tab(a.dip, i)) gv = apply_tab(a.dip, C, i, gv, vrow);
+    if (a.dipb.n) gv = apply_boxes(a.dipb, a.cmp, C, j0, j1, j2, gv, vrow);
     R ev, wv;
     const int kw = u.ksigw >= 0 ? 2 * asel(j0, j1, j2, u.ksigw) + u.ysigw : 0;
     if (pml_on(u.sigw, u.kapw, kw)) {
@@ -206,7 +177,7 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
 // one component slot C (a template argument, so every access to the slot's parameters has a
 // constant offset)
 template <class R, bool CURL, bool UPD, int C>
-__device__ __forceinline__ void step_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int j2) {
+__device__ __forceinline__ void step_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int j2, const double *vrow) {
   if (C >= a.nc) return;
   if (CURL ? !a.cc[C].f : !a.uc[C].f) return; // no component along this direction
   if (!slot_owned(CURL ? a.cc[C].ax : a.uc[C].ax, j0, j1, j2)) return;
@@ -218,13 +189,12 @@ __device__ __forceinline__ void step_slot(const StepArgs<R> &a, uint32_t i, int 
     else if (k.g1)
       cu = k.g1[i + k.s1] - k.g1[i];
   }
-  cell_update<R, CURL, UPD, C>(a, i, j0, j1, j2, cu);
+  cell_update<R, CURL, UPD, C>(a, i, j0, j1, j2, cu, vrow);
 }
 
+// point t of the launch box of a
 template <class R, bool CURL, bool UPD>
-__global__ __launch_bounds__(BLOCK) void k_step(const StepArgs<R> a, uint32_t count) {
-  const uint32_t t = xcd_block() * uint32_t(BLOCK) + threadIdx.x;
-  if (t >= count) return;
+__device__ __forceinline__ void step_body(const StepArgs<R> &a, uint32_t t, const double *vrow) {
   const Grid &g = a.g;
   const uint32_t r = t / uint32_t(g.an[2]);
   const int q2 = int(t - r * uint32_t(g.an[2]));
@@ -232,9 +202,16 @@ __global__ __launch_bounds__(BLOCK) void k_step(const StepArgs<R> a, uint32_t co
   const int q1 = int(r - uint32_t(q0) * uint32_t(g.an[1]));
   const int j0 = g.alo[0] + q0, j1 = g.alo[1] + q1, j2 = g.alo[2] + q2;
   const uint32_t i = uint32_t(j0 * g.ast[0] + j1 * g.ast[1] + j2 * g.ast[2]);
-  step_slot<R, CURL, UPD, 0>(a, i, j0, j1, j2);
-  step_slot<R, CURL, UPD, 1>(a, i, j0, j1, j2);
-  step_slot<R, CURL, UPD, 2>(a, i, j0, j1, j2);
+  step_slot<R, CURL, UPD, 0>(a, i, j0, j1, j2, vrow);
+  step_slot<R, CURL, UPD, 1>(a, i, j0, j1, j2, vrow);
+  step_slot<R, CURL, UPD, 2>(a, i, j0, j1, j2, vrow);
+}
+
+template <class R, bool CURL, bool UPD>
+__global__ __launch_bounds__(BLOCK) void k_step(const StepArgs<R> a, uint32_t count) {
+  const uint32_t t = xcd_block() * uint32_t(BLOCK) + threadIdx.x;
+  if (t >= count) return;
+  step_body<R, CURL, UPD>(a, t, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -340,7 +317,7 @@ template <int V> __device__ __forceinline__ unsigned group_metal(const AxisSlot 
 // slot C of the vectorised kernel; F0..F2 are the partner centre rows by direction
 template <class R, int V, bool CURL, bool UPD, int C>
 __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int z0, unsigned mask,
-                                          const Vec<R, V> &F0, const Vec<R, V> &F1, const Vec<R, V> &F2) {
+                                          const Vec<R, V> &F0, const Vec<R, V> &F1, const Vec<R, V> &F2, const double *vrow) {
   if (!mask) return;
   const Grid &g = a.g;
   const AxisSlot &ax = CURL ? a.cc[C].ax : a.uc[C].ax;
@@ -377,11 +354,11 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
     for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
     if (a.cur.n && int64_t(i) + V - 1 >= a.cur.lo && int64_t(i) <= a.cur.hi) {
 #pragma unroll
-      for (int k = 0; k < V; ++k) fv.v[k] = apply_tab(a.cur, C, i + k, fv.v[k]);
+      for (int k = 0; k < V; ++k) fv.v[k] = apply_tab(a.cur, C, i + k, fv.v[k], vrow);
     }
     if (a.curb.n) {
 #pragma unroll
-      for (int k = 0; k < V; ++k) fv.v[k] = apply_boxes(a.curb, a.cmp, C, j0, j1, z0 + k, fv.v[k]);
+      for (int k = 0; k < V; ++k) fv.v[k] = apply_boxes(a.curb, a.cmp, C, j0, j1, z0 + k, fv.v[k], vrow);
     }
     if (metal) {
 #pragma unroll
@@ -403,11 +380,11 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       }
     if (a.dip.n && int64_t(i) + V - 1 >= a.dip.lo && int64_t(i) <= a.dip.hi) {
 #pragma unroll
-      for (int k = 0; k < V; ++k) gv.v[k] = apply_tab(a.dip, C, i + k, gv.v[k]);
+      for (int k = 0; k < V; ++k) gv.v[k] = apply_tab(a.dip, C, i + k, gv.v[k], vrow);
     }
     if (a.dipb.n) {
 #pragma unroll
-      for (int k = 0; k < V; ++k) gv.v[k] = apply_boxes(a.dipb, a.cmp, C, j0, j1, z0 + k, gv.v[k]);
+      for (int k = 0; k < V; ++k) gv.v[k] = apply_boxes(a.dipb, a.cmp, C, j0, j1, z0 + k, gv.v[k], vrow);
     }
     Vec<R, V> ev = gv;
     if (u.u) {
@@ -436,10 +413,9 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
   }
 }
 
+// group t of the launch box of a (groups of V points along the rows, from group g0)
 template <class R, int V, bool CURL, bool UPD>
-__global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t count, int g0, int ngroups) {
-  const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
-  if (t >= count) return;
+__device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int g0, int ngroups, const double *vrow) {
   const Grid &g = a.g;
   const uint32_t r = t / uint32_t(ngroups);
   const int grp = int(t - r * uint32_t(ngroups));
@@ -466,17 +442,22 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
     if (a.F[1]) F1 = vload<R, V>(a.F[1] + i);
     if (a.F[2]) F2 = vload<R, V>(a.F[2] + i);
   }
-  fast_slot<R, V, CURL, UPD, 0>(a, i, j0, j1, z0, m0, F0, F1, F2);
-  fast_slot<R, V, CURL, UPD, 1>(a, i, j0, j1, z0, m1, F0, F1, F2);
-  fast_slot<R, V, CURL, UPD, 2>(a, i, j0, j1, z0, m2, F0, F1, F2);
+  fast_slot<R, V, CURL, UPD, 0>(a, i, j0, j1, z0, m0, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, 1>(a, i, j0, j1, z0, m1, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, 2>(a, i, j0, j1, z0, m2, F0, F1, F2, vrow);
+}
+
+
+template <class R, int V, bool CURL, bool UPD>
+__global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t count, int g0, int ngroups) {
+  const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
+  if (t >= count) return;
+  fast_body<R, V, CURL, UPD>(a, t, g0, ngroups, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------
 
-template <class R>
-__global__ __launch_bounds__(BLOCK) void k_conn(const Conn *conns, const double2 *phases, int n, R *const *slots, int ncmp) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= n) return;
+template <class R> __device__ __forceinline__ void conn_one(const Conn *conns, const double2 *phases, int t, R *const *slots, int ncmp) {
   const Conn e = conns[t];
   R *a0 = slots[2 * e.slot], *a1 = slots[2 * e.slot + 1];
   if (e.cls == 0) { // CONNECT_PHASE: complex<realnum> product
@@ -494,6 +475,12 @@ __global__ __launch_bounds__(BLOCK) void k_conn(const Conn *conns, const double2
     a0[e.dst] = a0[e.src];
     if (ncmp > 1) a1[e.dst] = a1[e.src];
   }
+}
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_conn(const Conn *conns, const double2 *phases, int n, R *const *slots, int ncmp) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= n) return;
+  conn_one<R>(conns, phases, t, slots, ncmp);
 }
 
 template <class R>
@@ -517,6 +504,52 @@ template <class R> __device__ __forceinline__ R avg_value(const R *F, int64_t id
 // all frequencies once into LDS (src/dft.cpp:270-271); lists longer than the table form
 // them per accumulator.
 constexpr int DFT_TAB = 512;
+// accumulator o of a DFT chunk with this frequency's phase exp(i omega t) * scale
+template <class R> __device__ __forceinline__ void dft_one(const DftArgs<R> &a, size_t o, double2 phv) {
+  const int nf = a.nfreq;
+  const int k = int(o / size_t(nf));
+  const int64_t idx = a.idx[k];
+  const double w = a.w[k];
+  const R f0 = avg_value(a.f0, idx, w, a.avg1, a.avg2);
+  const R f1 = a.f1 ? avg_value(a.f1, idx, w, a.avg1, a.avg2) : R(0);
+  const double pr = phv.x, pim = phv.y;
+  if (a.acc_kind == 1) { // FP64 accumulators
+    double2 *acc = reinterpret_cast<double2 *>(a.acc);
+    double2 v = acc[o];
+    v.x += pr * double(f0) - pim * double(f1);
+    v.y += pr * double(f1) + pim * double(f0);
+    acc[o] = v;
+    return;
+  }
+  const R rr = R(pr), ri = R(pim);
+  R tr, ti;
+  if (a.f1) { // complex<realnum> product
+    tr = rr * f0 - ri * f1;
+    ti = rr * f1 + ri * f0;
+  }
+  else {
+    tr = f0 * rr;
+    ti = f0 * ri;
+  }
+  if (a.acc_kind == 2) { // Kahan-compensated complex<float>
+    float2 *acc = reinterpret_cast<float2 *>(a.acc);
+    float2 *cp = reinterpret_cast<float2 *>(a.comp);
+    float2 s = acc[o], c = cp[o];
+    const float yr = float(tr) - c.x, yi = float(ti) - c.y;
+    const float sr = s.x + yr, si = s.y + yi;
+    c.x = (sr - s.x) - yr;
+    c.y = (si - s.y) - yi;
+    s.x = sr;
+    s.y = si;
+    acc[o] = s;
+    cp[o] = c;
+    return;
+  }
+  R *acc = reinterpret_cast<R *>(a.acc);
+  acc[2 * o] += tr;
+  acc[2 * o + 1] += ti;
+}
+
 template <class R> __global__ __launch_bounds__(BLOCK) void k_dft(const DftArgs<R> a) {
   __shared__ double2 ph[DFT_TAB];
   const int nf = a.nfreq;
@@ -531,58 +564,16 @@ template <class R> __global__ __launch_bounds__(BLOCK) void k_dft(const DftArgs<
   __syncthreads();
   const size_t n = size_t(a.npts) * size_t(nf);
   for (size_t o = size_t(blockIdx.x) * BLOCK + threadIdx.x; o < n; o += size_t(gridDim.x) * BLOCK) {
-    const int k = int(o / size_t(nf));
-    const int fi = int(o - size_t(k) * size_t(nf));
-    const int64_t idx = a.idx[k];
-    const double w = a.w[k];
-    const R f0 = avg_value(a.f0, idx, w, a.avg1, a.avg2);
-    const R f1 = a.f1 ? avg_value(a.f1, idx, w, a.avg1, a.avg2) : R(0);
-    double pr, pim;
-    if (tab) {
-      pr = ph[fi].x;
-      pim = ph[fi].y;
-    }
+    const int fi = int(o % size_t(nf));
+    double2 phv;
+    if (tab)
+      phv = ph[fi];
     else {
       double sn, cs;
       sincos(a.omega[fi] * a.time, &sn, &cs);
-      pr = cs * a.scale_re - sn * a.scale_im;
-      pim = cs * a.scale_im + sn * a.scale_re;
+      phv = make_double2(cs * a.scale_re - sn * a.scale_im, cs * a.scale_im + sn * a.scale_re);
     }
-    if (a.acc_kind == 1) { // FP64 accumulators
-      double2 *acc = reinterpret_cast<double2 *>(a.acc);
-      double2 v = acc[o];
-      v.x += pr * double(f0) - pim * double(f1);
-      v.y += pr * double(f1) + pim * double(f0);
-      acc[o] = v;
-      continue;
-    }
-    const R rr = R(pr), ri = R(pim);
-    R tr, ti;
-    if (a.f1) { // complex<realnum> product
-      tr = rr * f0 - ri * f1;
-      ti = rr * f1 + ri * f0;
-    }
-    else {
-      tr = f0 * rr;
-      ti = f0 * ri;
-    }
-    if (a.acc_kind == 2) { // Kahan-compensated complex<float>
-      float2 *acc = reinterpret_cast<float2 *>(a.acc);
-      float2 *cp = reinterpret_cast<float2 *>(a.comp);
-      float2 s = acc[o], c = cp[o];
-      const float yr = float(tr) - c.x, yi = float(ti) - c.y;
-      const float sr = s.x + yr, si = s.y + yi;
-      c.x = (sr - s.x) - yr;
-      c.y = (si - s.y) - yi;
-      s.x = sr;
-      s.y = si;
-      acc[o] = s;
-      cp[o] = c;
-      continue;
-    }
-    R *acc = reinterpret_cast<R *>(a.acc);
-    acc[2 * o] += tr;
-    acc[2 * o + 1] += ti;
+    dft_one<R>(a, o, phv);
   }
 }
 
@@ -594,8 +585,6 @@ __global__ void k_check(const R *const *ptrs, const int64_t *idx, const double *
   if (w[t] != 0.0 && !isfinite(double(v))) atomicOr(flag, 1);
 }
 
-static inline unsigned nblocks(size_t n) { return unsigned((n + BLOCK - 1) / BLOCK); }
-static inline unsigned nblocks8(size_t n) { return (nblocks(n) + 7u) / 8u * 8u; }
 
 template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
   const size_t count = size_t(a.g.an[0]) * size_t(a.g.an[1]) * size_t(a.g.an[2]);
@@ -634,6 +623,17 @@ void launch_src(R *f, const int64_t *idx, const double2 *amp, int n, double2 sca
   if (n > 0) hipLaunchKernelGGL(k_src<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, idx, amp, n, scal, mult, cmp);
 }
 
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_src_images(R *f, const int64_t *idx, const int *k, const int64_t *dst, const int8_t *sgn, int n) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= n) return;
+  const R v = f[idx[k[t]]];
+  f[dst[t]] = sgn[t] > 0 ? v : -v;
+}
+template <class R>
+void launch_src_images(R *f, const int64_t *idx, const int *k, const int64_t *dst, const int8_t *sgn, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_src_images<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, idx, k, dst, sgn, n);
+}
 template <class R> __global__ __launch_bounds__(BLOCK) void k_average(R *f, const R *b, size_t n) {
   const size_t t = size_t(blockIdx.x) * BLOCK + threadIdx.x;
   if (t < n) f[t] = R(0.5 * double(f[t] + b[t]));
@@ -668,13 +668,108 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
   if (n > 0) hipLaunchKernelGGL(k_check<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, ptrs, idx, w, n, flag);
 }
 
+// ---- batched launches -----------------------------------------------------------------
+// Logical blocks are dealt to XCDs in contiguous runs as in the single-chunk kernels (nblk is a
+// multiple of 8); each logical block belongs to one item.
+template <class R, int BS, bool FAST, bool CURL, bool UPD>
+__global__ __launch_bounds__(BS) void k_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first,
+                                                   const double *vrow) {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * (nb >> 3) + (b >> 3);
+  const uint32_t k = blk_item[lb];
+  const BatchItem<R> &it = items[k];
+  const uint32_t t = blk_first[lb] + threadIdx.x;
+  if (t >= it.count) return;
+  if (FAST)
+    fast_body<R, 16 / sizeof(R), CURL, UPD>(it.a, t, it.g0, it.ngroups, vrow);
+  else
+    step_body<R, CURL, UPD>(it.a, t, vrow);
+}
+
+template <class R, int BS>
+void launch_step_batch_bs(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, bool fast,
+                          bool curl, bool upd, const double *vrow, hipStream_t s) {
+  const dim3 grid(nblk), blk(BS);
+  if (fast) {
+    if (curl && upd)
+      hipLaunchKernelGGL((k_step_batch<R, BS, true, true, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+    else if (curl)
+      hipLaunchKernelGGL((k_step_batch<R, BS, true, true, false>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+    else
+      hipLaunchKernelGGL((k_step_batch<R, BS, true, false, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+  }
+  else {
+    if (curl && upd)
+      hipLaunchKernelGGL((k_step_batch<R, BS, false, true, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+    else if (curl)
+      hipLaunchKernelGGL((k_step_batch<R, BS, false, true, false>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+    else
+      hipLaunchKernelGGL((k_step_batch<R, BS, false, false, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+  }
+}
+template <class R>
+void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, bool fast,
+                       bool curl, bool upd, int block, const double *vrow, hipStream_t s) {
+  if (!nblk) return;
+  switch (block) {
+    case 64: launch_step_batch_bs<R, 64>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
+    case 128: launch_step_batch_bs<R, 128>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
+    case 256: launch_step_batch_bs<R, 256>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
+    default: launch_step_batch_bs<R, 512>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_dft_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, long step) {
+  __shared__ double2 ph[DFT_TAB];
+  const BatchDft<R> &it = items[blk_item[blockIdx.x]];
+  const int64_t tt = it.t_start + step + 1;
+  if (tt % it.decimation) return; // the whole block belongs to this chunk
+  const DftArgs<R> &a = it.a;
+  const double time = double(tt) * it.dt - (it.half ? 0.5 * it.dt : 0.0);
+  const int nf = a.nfreq;
+  if (nf > DFT_TAB) return; // the host keeps such chunks out of batches
+  for (int q = threadIdx.x; q < nf; q += BLOCK) {
+    double sn, cs;
+    sincos(a.omega[q] * time, &sn, &cs);
+    ph[q] = make_double2(cs * a.scale_re - sn * a.scale_im, cs * a.scale_im + sn * a.scale_re);
+  }
+  __syncthreads();
+  const size_t o = size_t(blk_first[blockIdx.x]) + threadIdx.x;
+  if (o >= size_t(a.npts) * size_t(nf)) return;
+  dft_one<R>(a, o, ph[int(o % size_t(nf))]);
+}
+template <class R>
+void launch_dft_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, long step,
+                      hipStream_t s) {
+  if (nblk) hipLaunchKernelGGL(k_dft_batch<R>, dim3(nblk), dim3(BLOCK), 0, s, items, blk_item, blk_first, step);
+}
+
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_conn_batch(const BatchConn *items, const uint32_t *blk_item, const uint32_t *blk_first) {
+  const BatchConn &it = items[blk_item[blockIdx.x]];
+  const int t = int(blk_first[blockIdx.x]) + int(threadIdx.x);
+  if (t >= it.n) return;
+  conn_one<R>(it.conns, it.phases, t, reinterpret_cast<R *const *>(it.slots), it.ncmp);
+}
+template <class R>
+void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, hipStream_t s) {
+  if (nblk) hipLaunchKernelGGL(k_conn_batch<R>, dim3(nblk), dim3(BLOCK), 0, s, items, blk_item, blk_first);
+}
+
 #define INSTANTIATE(R)                                                                             \
+  template void launch_step_batch<R>(const BatchItem<R> *, const uint32_t *, const uint32_t *, unsigned, bool, bool, bool, int, \
+                                     const double *, hipStream_t);                                 \
+  template void launch_dft_batch<R>(const BatchDft<R> *, const uint32_t *, const uint32_t *, unsigned, long, hipStream_t); \
+  template void launch_conn_batch<R>(const BatchConn *, const uint32_t *, const uint32_t *, unsigned, hipStream_t); \
   template void launch_step<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_fast<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_conn<R>(const Conn *, const double2 *, int, R *const *, int, hipStream_t);  \
   template void launch_src<R>(R *, const int64_t *, const double2 *, int, double2, double, int,      \
                               hipStream_t);                                                        \
   template void launch_average<R>(R *, const R *, size_t, hipStream_t);                            \
+  template void launch_src_images<R>(R *, const int64_t *, const int *, const int64_t *, const int8_t *, int, \
+                                     hipStream_t);                                                 \
   template void launch_gather<R>(R *, const R *, const int64_t *, int, hipStream_t);               \
   template void launch_scatter<R>(R *, const int64_t *, const R *, int, hipStream_t);              \
   template void launch_dft<R>(const DftArgs<R> &, hipStream_t);                                    \

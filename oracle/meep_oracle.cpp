@@ -364,7 +364,8 @@ template <class F> inline void loop_ivecs(const GV &gv, const ivec &is, const iv
 // collapse(2) loops, src/meep/vec.hpp:201-230), for bodies that touch only their own point.
 template <class F> inline void loop_ivecs_par(const GV &gv, const ivec &is, const ivec &ie, F body) {
   const Loop L = make_loop(gv, is, ie);
-#pragma omp parallel for collapse(2) schedule(static)
+  // small loops stay on one thread: waking the team costs more than they take
+#pragma omp parallel for collapse(2) schedule(static) if (L.n[0] * L.n[1] * L.n[2] >= (1 << 16))
   for (ptrdiff_t i1 = 0; i1 < L.n[0]; ++i1)
     for (ptrdiff_t i2 = 0; i2 < L.n[1]; ++i2)
       for (ptrdiff_t i3 = 0; i3 < L.n[2]; ++i3) {
@@ -2022,6 +2023,23 @@ int mp_dft_fields_chunk_component(mp_dft_fields *h, int i) {
 }
 int mp_dft_fields_get_dft(mp_dft_fields *h, int i, double *out) {
   return mp_dft_flux_get_dft(reinterpret_cast<mp_dft_flux *>(h), 0, i, out);
+}
+
+// The reverse mode is a product feature with no counterpart in the reference's C++ library;
+// its gradients are checked against finite differences of the engine itself, so the oracle
+// exports the entry points only to keep one ABI and reports that it does not provide them.
+mp_adjoint *mp_adjoint_new(mp_fields *, mp_dft_fields *, int, const double *, const double *, int, int *, int *) {
+  last_error = "the oracle has no reverse mode";
+  return nullptr;
+}
+void mp_adjoint_delete(mp_adjoint *) {}
+int mp_adjoint_objective(mp_adjoint *, double *) {
+  last_error = "the oracle has no reverse mode";
+  return -1;
+}
+int mp_adjoint_gradient(mp_adjoint *, double *) {
+  last_error = "the oracle has no reverse mode";
+  return -1;
 }
 
 // fields::field_energy (src/energy_and_flux.cpp:48)
