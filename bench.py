@@ -267,6 +267,106 @@ def run_c4(args, lib):
     return line
 
 
+# ---- the batched 2D workload ----------------------------------------------------------------
+C5_N = 64
+C5_A = 10.0
+C5_PML = 10   # cells
+C5_NFREQ = 16
+
+
+def c5_layout(seed=5, n=C5_N):
+    """Side lengths (cells, square cells from 128 to 512) and source frequencies."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(128, 513, size=n)
+    fcen = rng.uniform(0.1, 0.3, size=n)
+    return sides, fcen, rng
+
+
+def build_c5(lib, single, seed=5, n=C5_N):
+    """n independent 2D TM simulations: side_i x side_i cells at resolution 10, eps per 16x16
+    block drawn from U[1, 12], 10-cell uPML, Gaussian Ez point current (fcen_i from
+    U[0.1, 0.3], df = 0.1 fcen_i), flux line of 16 frequencies with FP64 accumulators."""
+    sides, fcen, rng = c5_layout(seed, n)
+    sims = []
+    for q in range(n):
+        side = int(sides[q])
+        L = side / C5_A
+        s = capi.structure(lib, dim=2, size=(L, L), a=C5_A, eps=None, pml=capi.pml(C5_PML / C5_A),
+                           single=single)
+        nb = (side + 15) // 16
+        blocks = rng.uniform(1.0, 12.0, size=(nb, nb))
+        b = np.arange(side + 1) // 16
+        b = np.minimum(b, nb - 1)
+        s.set_chi1inv(capi.Ez, 1.0 / blocks[np.ix_(b, b)])
+        f = capi.fields(s)
+        f.use_real_fields()
+        f.set_dft_accumulator(capi.DFT_F64)
+        fc = float(fcen[q])
+        f.add_volume_source(capi.Ez, capi.gaussian_src_time(fc, 0.1 * fc), (2.0, L / 2, 0), (2.0, L / 2, 0))
+        fl = f.add_dft_flux(capi.X, (L - 2.0, 1.5, 0), (L - 2.0, L - 1.5, 0),
+                            capi.linspace(0.95 * fc, 1.05 * fc, C5_NFREQ), decimation_factor=1)
+        sims.append((s, f, fl))
+    return sims, sides
+
+
+def c5_algorithmic_bytes(sides, word=4):
+    """Reference-state bytes per step of the batch: 9 words per 2D TM cell, +6 per PML strip
+    cell, +12 per PML corner cell, plus the flux line's 2 x 2 components' accumulators
+    (complex FP64, read and written)."""
+    total = 0
+    for side in sides:
+        side = int(side)
+        inner = side - 2 * C5_PML
+        strips = 4 * C5_PML * inner
+        corners = 4 * C5_PML * C5_PML
+        words = 9 * side * side + 6 * strips + 12 * corners
+        npts = (side - 30) + 1          # points of the flux line (|y - L/2| <= L/2 - 1.5)
+        total += words * word + 2 * npts * C5_NFREQ * 32
+    return total
+
+
+def run_c5(args, lib):
+    single = args.precision == "f32"
+    sims, sides = build_c5(lib, single)
+    b = capi.batch([f for _, f, _ in sims], use_graph=True, autotune=True)
+    b.step_n(args.warmup)  # planning, the chunks' own first step, block-size tuning
+    sync = device_sync()
+    sync()
+    t0 = time.perf_counter()
+    b.step_n(args.steps)
+    sync()
+    el = time.perf_counter() - t0
+    cells = int(np.sum(sides.astype(np.int64) ** 2))
+    value = cells * args.steps / el / 1e9
+    block, ms = b.stats()
+    alg = c5_algorithmic_bytes(sides, 4 if single else 8)
+    achieved = alg * args.steps / el / 1e9
+    line = line_base(args, value, el, {
+        "workload": f"{C5_N} independent 2D TM simulations, square sides 128-512 cells (seeded), "
+                    "resolution 10, eps 16x16 blocks in [1, 12], 10-cell uPML, Gaussian Ez point "
+                    "source fcen in [0.1, 0.3] df = 0.1 fcen, 16-frequency flux line with FP64 "
+                    "accumulators; one launch per launch group for all simulations, hipGraph step "
+                    f"loop, autotuned block size {block}",
+        "cells": cells, "simulations": C5_N, "resolution": C5_A, "courant": 0.5, "parallelism": "none"})
+    line["roofline"] = {"bound": "hbm", "kernel": "whole batched step (all launches)",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": alg}
+    line["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        # the oracle steps the simulations one after another
+        olib = oracle_lib()
+        osims, _ = build_c5(olib, single)
+        steps, t0 = 0, time.perf_counter()
+        for _, f, _ in osims:
+            f.step()
+        el0 = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": cells / el0 / 1e9, "unit": "Gcell-updates/s",
+                                "cores": ctypes.CDLL("libgomp.so.1").omp_get_max_threads(), "kind": "port",
+                                "sample": f"oracle, one step of each of the {C5_N} simulations in turn "
+                                          f"(the first, so it includes lazy allocation), {el0:.1f} s"}
+    return line
+
+
 def load_traffic(precision):
     path = os.path.join(ROOT, "profiles", f"traffic_step_de_box256_{precision}.json")
     if not os.path.exists(path):
@@ -420,11 +520,12 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--workload", choices=["box256", "pml_lorentz_dft", "adjoint2d"],
+    ap.add_argument("--workload", choices=["box256", "pml_lorentz_dft", "adjoint2d", "batch2d"],
                     default="box256",
                     help="box256: 3D Bloch-periodic 256^3 box (the headline); "
                          "pml_lorentz_dft: 384x384x192 with PML, two poles and a DFT plane; "
-                         "adjoint2d: 600x400 forward run plus its reverse-mode gradient")
+                         "adjoint2d: 600x400 forward run plus its reverse-mode gradient; "
+                         "batch2d: 64 uneven 2D simulations stepped together")
     ap.add_argument("--checkpoint-interval", type=int, default=None,
                     help="adjoint2d: steps between checkpoints (default sqrt(steps); 0 records "
                          "the design history every step)")
@@ -452,7 +553,8 @@ def main():
               "per half-step).", file=sys.stderr)
         sys.exit(2)
     lib = meep_amd.load_library()
-    run = {"box256": run_box256, "pml_lorentz_dft": run_c3, "adjoint2d": run_c4}[args.workload]
+    run = {"box256": run_box256, "pml_lorentz_dft": run_c3, "adjoint2d": run_c4,
+           "batch2d": run_c5}[args.workload]
     line = run(args, lib)
     print(json.dumps(line))
 

@@ -49,6 +49,7 @@ typedef struct mp_fields mp_fields;
 typedef struct mp_dft_flux mp_dft_flux;
 typedef struct mp_dft_fields mp_dft_fields;
 typedef struct mp_adjoint mp_adjoint;
+typedef struct mp_batch mp_batch;
 
 /* r points at (x, y, z) in the reference's coordinates (origin at the cell's low corner). */
 typedef double (*mp_scalar_func)(const double *r, void *user);
@@ -232,6 +233,27 @@ int mp_adjoint_objective(mp_adjoint *a, double *J);
 /* dJ / dchi1inv over the design box, in the reference layout order restricted to the box
    (X slowest, the stride-1 direction fastest).  The fields end in the state they had. */
 int mp_adjoint_gradient(mp_adjoint *a, double *grad);
+
+/* ---- many independent simulations stepped together -------------------------------- */
+
+/* A batch of n fields objects of one precision (each with its own structure, sources and
+   monitors, any sizes) that advance together: each launch group of a step (B phase, its
+   halos, D phase, its halos, DFT) covers every object in one kernel, source values for the
+   steps are computed on the host before the loop, and the loop is captured in a hipGraph.
+   No counterpart in the reference, which runs one simulation per process; the objects remain
+   usable on their own between batch calls.  Objects whose next step would allocate arrays
+   take that step on their own first (it counts towards n).  Long source lists that fall back
+   to separate launches are not supported in a batch. */
+mp_batch *mp_batch_new(mp_fields *const *fields, int n);
+void mp_batch_delete(mp_batch *b);
+/* n steps of every object. */
+int mp_batch_step_n(mp_batch *b, long n);
+/* use_graph: capture the step loop in a hipGraph (default 1); autotune: time the first steps
+   with each block size (64, 128, 256, 512) and keep the fastest (default 1); block: fix the
+   block size instead (0 keeps the current choice). */
+int mp_batch_set_options(mp_batch *b, int use_graph, int autotune, int block);
+/* the block size in use and the wall time per step of the last call. */
+int mp_batch_stats(mp_batch *b, int *block, double *ms_per_step);
 
 #ifdef __cplusplus
 }

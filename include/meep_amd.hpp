@@ -266,6 +266,33 @@ private:
   int n;
 };
 
+// dft_fields (src/meep.hpp:1420-1441): accumulators per chunk, as the engine holds them
+class dft_fields {
+public:
+  dft_fields(mp_dft_fields *h, int n) : h(h), n(n) {}
+  int num_chunks() const { return mp_dft_fields_num_chunks(h); }
+  component chunk_component(int i) const {
+    int c = mp_dft_fields_chunk_component(h, i);
+    if (c < 0) throw std::runtime_error(mp_last_error());
+    return component(c);
+  }
+  // dft_chunk::dft of chunk i: chunk_points(i) x Nfreq complex values, frequency fastest
+  std::vector<cdouble> chunk_dft(int i) const {
+    long np = mp_dft_fields_chunk_points(h, i);
+    std::vector<double> raw(size_t(2 * np * n));
+    check(mp_dft_fields_get_dft(h, i, raw.data()));
+    std::vector<cdouble> out(size_t(np * n));
+    for (size_t q = 0; q < out.size(); ++q) out[q] = cdouble(raw[2 * q], raw[2 * q + 1]);
+    return out;
+  }
+  int Nfreq() const { return n; }
+  mp_dft_fields *handle() const { return h; }
+
+private:
+  mp_dft_fields *h;
+  int n;
+};
+
 // fields (src/meep.hpp:1716-2320)
 class fields {
 public:
@@ -276,6 +303,7 @@ public:
   }
   ~fields() {
     for (mp_dft_flux *h : fluxes) mp_dft_flux_delete(h);
+    for (mp_dft_fields *h : dft_fields_list) mp_dft_fields_delete(h);
     mp_fields_delete(p);
   }
   fields(const fields &) = delete;
@@ -331,12 +359,39 @@ public:
     fluxes.push_back(h);
     return dft_flux(h, Nfreq);
   }
+  // add_dft_fields(components, num_components, volume, freq_min, freq_max, Nfreq)
+  // (src/meep.hpp:2078-2084)
+  dft_fields add_dft_fields(const component *cs, int num_components, const volume &where, double freq_min, double freq_max,
+                            int Nfreq) {
+    std::vector<double> fr(Nfreq);
+    double df = Nfreq <= 1 ? 0.0 : (freq_max - freq_min) / (Nfreq - 1);
+    for (int i = 0; i < Nfreq; ++i) fr[i] = Nfreq <= 1 ? (freq_min + freq_max) * 0.5 : freq_min + i * df;
+    std::vector<int> ci(cs, cs + num_components);
+    mp_dft_fields *h = mp_fields_add_dft_fields(p, ci.data(), num_components, where.get_min_corner().data(),
+                                                where.get_max_corner().data(), fr.data(), Nfreq, 0);
+    if (!h) throw std::runtime_error(mp_last_error());
+    dft_fields_list.push_back(h);
+    return dft_fields(h, Nfreq);
+  }
+  double field_energy() { // :2196
+    double e = 0;
+    check(mp_fields_field_energy(p, &e));
+    return e;
+  }
+  void print_times() { // :1879; the phases' wall times as the engine accounts them
+    double t7[7];
+    check(mp_fields_get_times(p, t7));
+    static const char *names[7] = {"stepping B", "", "stepping D", "", "boundaries", "dft", "total"};
+    for (int q = 0; q < 7; ++q)
+      if (names[q][0]) std::printf("    %-12s %g s\n", names[q], t7[q]);
+  }
   mp_fields *handle() { return p; }
 
 private:
   grid_volume gv;
   mp_fields *p;
   std::vector<mp_dft_flux *> fluxes;
+  std::vector<mp_dft_fields *> dft_fields_list;
 };
 
 } // namespace meep

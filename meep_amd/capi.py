@@ -111,6 +111,11 @@ _SIGNATURES = {
                                     C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int)]),
     "mp_adjoint_delete": (None, [C.c_void_p]),
+    "mp_batch_new": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int]),
+    "mp_batch_delete": (None, [C.c_void_p]),
+    "mp_batch_step_n": (C.c_int, [C.c_void_p, C.c_long]),
+    "mp_batch_set_options": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "mp_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "mp_adjoint_objective": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mp_adjoint_gradient": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mp_fields_add_dft_fields": (C.c_void_p, [C.c_void_p, C.POINTER(C.c_int), C.c_int,
@@ -322,6 +327,33 @@ class adjoint:
     def close(self):
         if self.ptr:
             self.lib.mp_adjoint_delete(self.ptr)
+            self.ptr = None
+
+
+class batch:
+    """Many fields objects of one precision stepped together (see mp_batch_new)."""
+
+    def __init__(self, field_objects, use_graph=True, autotune=True, block=0):
+        self.fields = list(field_objects)
+        self.lib = self.fields[0].lib
+        arr = (C.c_void_p * len(self.fields))(*[f.ptr for f in self.fields])
+        self.ptr = self.lib.mp_batch_new(arr, len(self.fields))
+        if not self.ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        _check(self.lib, self.lib.mp_batch_set_options(self.ptr, int(use_graph), int(autotune),
+                                                       int(block)))
+
+    def step_n(self, n):
+        _check(self.lib, self.lib.mp_batch_step_n(self.ptr, int(n)))
+
+    def stats(self):
+        b, ms = C.c_int(), C.c_double()
+        _check(self.lib, self.lib.mp_batch_stats(self.ptr, C.byref(b), C.byref(ms)))
+        return b.value, ms.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.lib.mp_batch_delete(self.ptr)
             self.ptr = None
 
 

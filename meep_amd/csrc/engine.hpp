@@ -42,6 +42,7 @@ struct Grid {
   // into the halo point n (the reference's connections, src/boundaries.cpp:368-600), and a
   // point that wraps along several folded axes writes every combination.
   int wrap[3];
+  int mhi_bits, mlo_bits; // bit k: loop axis k's High wall is Metallic / Low wall Magnetic
 };
 
 // Per-slot data by loop axis (derived by finalize_args): owned index range, and the index of
@@ -51,6 +52,7 @@ struct AxisSlot {
   int lo[3], hi[3];
   int mhi[3], mlo[3];
   int ysh[3];
+  int ybits; // bit k: ysh[k] (the vectorised kernel derives the rest from it and the Grid)
 };
 
 template <class R> struct PmlProfile {
@@ -94,6 +96,9 @@ template <class R> struct UpdComp {
 
 template <class R> struct PoleCoef {
   R gamma1inv, gamma1, omega0dtsqr, omega0dtsqr_denom;
+  // loop-axis box of the points where some component's sigma is non-zero; outside it P and
+  // P_prev stay exactly zero, so the kernels skip the pole there (empty when lo > hi)
+  int lo[3], hi[3];
 };
 
 // Sparse per-step source values: f[idx] of component comp loses val (already the real or
@@ -195,8 +200,9 @@ template <class R> struct BatchDft {
   int decimation;
   int64_t t_start;
 };
+// kind: 0 generic kernel, 1 vectorised, 2 vectorised with per-point uPML
 template <class R>
-void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, bool fast,
+void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, int kind,
                        bool curl, bool upd, int block, const double *vrow, hipStream_t s);
 template <class R>
 void launch_dft_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, long step,
@@ -266,8 +272,9 @@ template <class R> void launch_record(R *hist, const R *D, const Grid &g, const 
 // launchers (kernels.hip)
 // generic scalar kernel over the box a.g.blo..bhi (any cells, uPML included)
 template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
-// vectorised kernel over a box in which no uPML profile is active (16-byte accesses)
-template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
+// vectorised kernel (16-byte accesses); pml: the uPML branches per point (else the box must
+// have no active uPML profile)
+template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, bool pml, hipStream_t s);
 template <class R>
 void launch_conn(const Conn *conns, const double2 *phases, int n, R *const *slot_ptrs, int ncmp, hipStream_t s);
 // f[idx[j]] -= Re or Im of (amp[j] * scal) * mult for the n points of one source

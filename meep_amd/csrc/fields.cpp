@@ -411,13 +411,13 @@ template <class R> struct Fields : FieldsBase {
     return e;
   }
   void kbegin(int kind) {
-    if (!ktiming) return;
+    if (!ktiming || collector) return;
     open_kind = kind;
     open_ev = get_event();
     HIPCHECK(hipEventRecord(open_ev, stream));
   }
   void kend() {
-    if (!ktiming || open_kind < 0) return;
+    if (!ktiming || open_kind < 0 || collector) return;
     hipEvent_t b = get_event();
     HIPCHECK(hipEventRecord(b, stream));
     pending.push_back({open_kind, open_ev, b});
@@ -518,6 +518,28 @@ template <class R> struct Fields : FieldsBase {
       p.coef.gamma1inv = R(1) / (R(1) + g2pi * rdt / R(2));
       p.coef.gamma1 = R(1) - g2pi * rdt / R(2);
       p.coef.omega0dtsqr_denom = su.drude ? R(0) : p.coef.omega0dtsqr;
+      // where sigma is non-zero, by loop axis
+      int blo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, bhi[3] = {-1, -1, -1};
+      for (int c = 0; c < 5; ++c) {
+        if (su.sigma[c].empty()) continue;
+        int j[3], hi[3];
+        for (int d = 0; d < 3; ++d) hi[d] = gv.has(d) ? gv.num[d] : 0;
+        for (j[0] = 0; j[0] <= hi[0]; ++j[0])
+          for (j[1] = 0; j[1] <= hi[1]; ++j[1])
+            for (j[2] = 0; j[2] <= hi[2]; ++j[2])
+              if (su.sigma[c][gv.ref_of(j)] != 0.0)
+                for (int d = 0; d < 3; ++d) {
+                  blo[d] = std::min(blo[d], j[d]);
+                  bhi[d] = std::max(bhi[d], j[d]);
+                }
+      }
+      for (int k = 0; k < 3; ++k) {
+        const int d = gv.ydir(k);
+        const bool has = gv.has(d);
+        p.coef.lo[k] = has ? blo[d] : 0;
+        p.coef.hi[k] = has ? bhi[d] : 0;
+        if (bhi[d] < 0) p.coef.lo[k] = 1, p.coef.hi[k] = 0; // nowhere
+      }
       pols.push_back(p);
     }
     d_flag.alloc(sizeof(int));
@@ -813,6 +835,11 @@ template <class R> struct Fields : FieldsBase {
   // step_boundaries for a merged group of field types (see d_conn)
   void boundaries(int grp) {
     if (!n_conn[grp]) return;
+    if (collector) {
+      collector->conn[grp].push_back({static_cast<const Conn *>(d_conn[grp].p), static_cast<const double2 *>(d_phase[grp].p),
+                                      n_conn[grp], static_cast<void *const *>(d_slots.p), ncmp()});
+      return;
+    }
     kbegin(MP_K_HALO);
     launch_conn<R>(static_cast<const Conn *>(d_conn[grp].p), static_cast<const double2 *>(d_phase[grp].p), n_conn[grp],
                    static_cast<R *const *>(d_slots.p), ncmp(), stream);
@@ -869,6 +896,16 @@ template <class R> struct Fields : FieldsBase {
       }
     }
     for (int c = 0; c < a.nc; ++c) {
+      // the vectorised kernel derives owned ranges and metal planes from the Yee-shift bits
+      const AxisSlot &x = curl ? a.cc[c].ax : a.uc[c].ax;
+      if (curl ? a.cc[c].f : a.uc[c].f)
+        for (int k = 0; k < 3; ++k) {
+          const int n = g.nax[k], y = (x.ybits >> k) & 1;
+          const int lo = n ? 1 - y : 0, hi = n ? n - y : 0;
+          const int mhi = ((g.mhi_bits >> k) & 1) && y == 0 && n ? n : -1;
+          const int mlo = ((g.mlo_bits >> k) & 1) && y == 1 && n ? 0 : -1;
+          if (lo != x.lo[k] || hi != x.hi[k] || mhi != x.mhi[k] || mlo != x.mlo[k]) fail("internal: slot %d axis %d bits", c, k);
+        }
       if (curl && a.cc[c].f) {
         const CurlComp<R> &k = a.cc[c];
         if ((k.ksig >= 0) != (k.sig != nullptr) || (k.ksigu >= 0) != (k.sigu != nullptr)) fail("internal: slot %d PML axes", c);
@@ -892,7 +929,11 @@ template <class R> struct Fields : FieldsBase {
       if (bc[High][d] == Metallic && ysh[d] == 0) ax.mhi[k] = gv.num[d];
       if (bc[Low][d] == Magnetic && ysh[d] == 1) ax.mlo[k] = 0;
     }
-    for (int k = 0; k < 3; ++k) ax.ysh[k] = gv.has(gv.ydir(k)) ? ysh[gv.ydir(k)] : 0;
+    ax.ybits = 0;
+    for (int k = 0; k < 3; ++k) {
+      ax.ysh[k] = gv.has(gv.ydir(k)) ? ysh[gv.ydir(k)] : 0;
+      ax.ybits |= ax.ysh[k] << k;
+    }
   }
   int axis_of(int d) const {
     if (d < 0) return -1;
@@ -907,10 +948,13 @@ template <class R> struct Fields : FieldsBase {
       g.an[k] = d >= 0 ? g.bhi[d] - g.blo[d] + 1 : 1;
       g.ast[k] = d >= 0 ? g.st[d] : 0;
     }
+    g.mhi_bits = g.mlo_bits = 0;
     for (int k = 0; k < 3; ++k) {
       const int d = g.axdir[k];
       g.nax[k] = d >= 0 ? gv.num[d] : 0;
       g.wrap[k] = d >= 0 ? wrap_dir[d] : 0;
+      if (d >= 0 && ((g.hi_metal >> d) & 1)) g.mhi_bits |= 1 << k;
+      if (d >= 0 && ((g.lo_magnetic >> d) & 1)) g.mlo_bits |= 1 << k;
     }
   }
   void finalize_args(StepArgs<R> &a) const {
@@ -936,31 +980,155 @@ template <class R> struct Fields : FieldsBase {
       u.kapw = u.dsigw >= 0 ? a.pml.kap[u.dsigw] : nullptr;
     }
   }
-  void launch_box(StepArgs<R> &b, const int *lo, const int *hi, bool curl, bool upd, bool fast) {
+  void launch_box(StepArgs<R> &b, const int *lo, const int *hi, bool curl, bool upd, bool fast, bool pml) {
     for (int d = 0; d < 3; ++d) {
       b.g.blo[d] = lo[d];
       b.g.bhi[d] = hi[d];
     }
     finalize_args(b);
     check_args(b, curl, upd, fast);
+    if (collector) {
+      BatchItem<R> it;
+      it.a = b;
+      it.a.rev = 0;
+      if (fast) {
+        constexpr int V = 16 / sizeof(R);
+        it.g0 = b.g.alo[2] / V;
+        it.ngroups = (b.g.alo[2] + b.g.an[2] - 1) / V - it.g0 + 1;
+        it.count = uint32_t(it.ngroups) * uint32_t(b.g.an[0]) * uint32_t(b.g.an[1]);
+      }
+      else {
+        it.g0 = it.ngroups = 0;
+        it.count = uint32_t(b.g.an[0]) * uint32_t(b.g.an[1]) * uint32_t(b.g.an[2]);
+      }
+      const int kind = fast ? (pml ? 2 : 1) : 0;
+      collector->items[cur_phase][kind * 4 + (curl ? 2 : 0) + (upd ? 1 : 0)].push_back(it);
+      return;
+    }
     if (fast)
-      launch_fast<R>(b, curl, upd, stream);
+      launch_fast<R>(b, curl, upd, pml, stream);
     else
       launch_step<R>(b, curl, upd, stream);
   }
+  // ---- batched stepping: one step's launches recorded as items ---------------------------
+  struct Collect {
+    std::map<int, std::vector<BatchItem<R>>> items[2]; // phase (B, D) -> variant -> items
+    std::vector<BatchConn> conn[2];
+    std::vector<BatchDft<R>> dft;
+    std::vector<SrcRecipe> *rec = nullptr;
+    int sim = 0;
+  };
+  Collect *collector = nullptr;
+  int cur_phase = 0;
+  // Records the launches of the next step; false when that step would allocate arrays or
+  // needs the unfused source path (then it must run on its own first).
+  bool collect_step(Collect &C) {
+    for (auto &sv : sources[B_ft])
+      if (srcs[sv.src].is_integrated) fail("integrated magnetic sources are not supported by the GPU engine");
+    bool have_fmp_E = false;
+    for (auto &sv : sources[D_ft])
+      if (srcs[sv.src].is_integrated) have_fmp_E = true;
+    for (auto &p : pols)
+      for (int c = 0; c < 5; ++c)
+        for (int cmp = 0; cmp < ncmp(); ++cmp)
+          if (needs_P(p, c, cmp)) have_fmp_E = true;
+    if (!plan_valid) rebuild_plan();
+    struct Guard {
+      Fields *F;
+      ~Guard() { F->collector = nullptr; }
+    } guard{this};
+    collector = &C;
+    StepArgs<R> A = base_args();
+    cur_phase = 0;
+    for (int cmp = 0; cmp < ncmp(); ++cmp) {
+      cur_cmp = cmp;
+      int which[3];
+      A.nc = curl_comps(B_ft, cmp, A.cc, which);
+      std::vector<LongSrc> big;
+      A.cmp = cmp;
+      fill_tab(B_ft, cmp, which, A.nc, false, 0.0, A.cur, A.curb, big, C.rec, C.sim, 0.0);
+      A.dip.n = 0;
+      A.dipb.n = 0;
+      A.npole = 0;
+      if (!big.empty() || upd_pending(cmp, which, A.nc, false)) return false;
+      const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
+      step_kernel(A, true, nu > 0, MP_K_STEP_BH);
+    }
+    if (!plan_valid) return false;
+    boundaries(0);
+    alloc_pols();
+    if (!plan_valid) return false;
+    cur_phase = 1;
+    for (int cmp = 0; cmp < ncmp(); ++cmp) {
+      cur_cmp = cmp;
+      int which[3], whiche[3];
+      A.nc = curl_comps(D_ft, cmp, A.cc, which);
+      std::vector<LongSrc> bigc, bigd;
+      A.cmp = cmp;
+      fill_tab(D_ft, cmp, which, A.nc, false, 0.0, A.cur, A.curb, bigc, C.rec, C.sim, 0.5);
+      upd_comps(E_ft, cmp, which, A.nc, A.uc, have_fmp_E);
+      if (!plan_valid) return false;
+      for (int q = 0; q < A.nc; ++q) whiche[q] = which[q] < 0 ? -1 : which[q] - 10;
+      fill_tab(D_ft, cmp, whiche, A.nc, true, 0.0, A.dip, A.dipb, bigd, C.rec, C.sim, 1.0);
+      A.npole = int(pols.size());
+      for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
+      if (!bigc.empty() || !bigd.empty()) return false;
+      step_kernel(A, true, true, MP_K_STEP_DE);
+    }
+    boundaries(1);
+    for (auto &cp : dft_chunks) {
+      DftChunk<R> &ch = *cp;
+      if (!f[ch.c][0]) continue;
+      if (ch.nfreq > 512) return false; // beyond the batched kernel's phase table
+      BatchDft<R> bd;
+      memset(&bd, 0, sizeof bd);
+      bd.a.f0 = f[ch.c][0];
+      bd.a.f1 = f[ch.c][1];
+      bd.a.idx = static_cast<const int64_t *>(ch.idx.p);
+      bd.a.w = static_cast<const double *>(ch.w.p);
+      bd.a.omega = static_cast<const double *>(ch.omega.p);
+      bd.a.scale_re = ch.scale.real();
+      bd.a.scale_im = ch.scale.imag();
+      bd.a.avg1 = ch.avg1;
+      bd.a.avg2 = ch.avg2;
+      bd.a.npts = ch.npts;
+      bd.a.nfreq = ch.nfreq;
+      bd.a.acc = ch.acc.p;
+      bd.a.comp = ch.comp.p;
+      bd.a.acc_kind = ch.acc_kind;
+      bd.dt = dt;
+      bd.half = (is_magnetic(ch.c) || is_B(ch.c)) ? 1 : 0;
+      bd.decimation = ch.decimation;
+      bd.t_start = t;
+      C.dft.push_back(bd);
+    }
+    return plan_valid;
+  }
+  // the finiteness probe of the centre points on stream s, and its verdict
+  void probe_on(hipStream_t s) {
+    if (n_chk)
+      launch_check<R>(static_cast<const R *const *>(d_chk_ptrs.p), static_cast<const int64_t *>(d_chk_idx.p),
+                      static_cast<const double *>(d_chk_w.p), n_chk, static_cast<int *>(d_flag.p), s);
+  }
+  void check_flag() {
+    int flag = 0;
+    HIPCHECK(hipMemcpy(&flag, d_flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) fail("simulation fields are NaN or Inf");
+  }
   void step_kernel(const StepArgs<R> &a, bool curl, bool upd, int kind) {
     if (!boxes_valid) compute_boxes();
+    cur_phase = kind == MP_K_STEP_BH ? 0 : 1;
     kbegin(kind);
     StepArgs<R> b = a;
+    const bool fast = use_fast && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b));
     if (!interior_empty) {
-      const bool fast = use_fast && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b));
       // backward differences read the previous plane: walk the planes downwards so it was
       // fetched just before (forward differences prefetch their neighbour naturally)
       b.rev = fast && kind == MP_K_STEP_DE ? 1 : 0;
-      launch_box(b, interior.lo, interior.hi, curl, upd, fast);
+      launch_box(b, interior.lo, interior.hi, curl, upd, fast, interior_pml);
     }
     b.rev = 0;
-    for (const Box &bx : slabs) launch_box(b, bx.lo, bx.hi, curl, upd, false);
+    for (const Box &bx : slabs) launch_box(b, bx.lo, bx.hi, curl, upd, fast, true);
     kend();
   }
   // ---- per-step argument blocks ----
@@ -995,14 +1163,25 @@ template <class R> struct Fields : FieldsBase {
   Box interior;
   bool interior_empty = false;
   std::vector<Box> slabs;
+  // The launch boxes: the grid is split along the directions other than the stride-1 one
+  // into an interior box, where no uPML profile of those directions is active, and slabs;
+  // every box spans whole rows, and the kernels take the uPML of the stride-1 direction (and
+  // of the slabs' directions) point by point.
+  bool interior_pml = false; // some uPML profile is active along the stride-1 direction
   void compute_boxes() {
     int ilo[3], ihi[3], full_hi[3];
     interior_empty = false;
+    interior_pml = false;
     for (int d = 0; d < 3; ++d) {
       full_hi[d] = gv.has(d) ? gv.num[d] : 0;
       ilo[d] = 0;
       ihi[d] = full_hi[d];
       if (!gv.has(d) || !s->pml_dir[d]) continue;
+      if (d == gv.fast) {
+        for (size_t k = 0; k < s->sig[d].size(); ++k)
+          if (s->sig[d][k] != 0.0 || s->kap[d][k] != 1.0) interior_pml = true;
+        continue;
+      }
       auto quiet = [&](int j) {
         for (int k = 2 * j; k <= 2 * j + 1; ++k)
           if (s->sig[d][k] != 0.0 || s->kap[d][k] != 1.0) return false;
@@ -2566,6 +2745,275 @@ template <class R> struct Fields : FieldsBase {
   }
 };
 
+// ---- many independent chunks stepped together --------------------------------------------
+// Every launch group of a step (B phase, B halos, D phase, D halos, DFT) covers all chunks in
+// one kernel: the chunks' launch boxes become items in device memory, source values come from
+// per-step rows computed on the host before the loop, and the whole loop can be captured in a
+// hipGraph.  The block size is chosen by timing the first steps with each candidate (the
+// results do not depend on it).
+struct BatchBase {
+  virtual ~BatchBase() {}
+  virtual void step_n(long n) = 0;
+  virtual void options(int graph, int tune, int block) = 0;
+  virtual void stats(int *block, double *ms_per_step) const = 0;
+};
+
+template <class R> struct Batch : BatchBase {
+  std::vector<Fields<R> *> sims;
+  hipStream_t stream = nullptr;
+  bool use_graph = true, tune = true, tuned = false;
+  int block = 256;
+  double last_ms = 0;
+  // plan
+  bool planned = false;
+  std::vector<long> plan_versions, plan_t;
+  struct Group {
+    int phase;
+    int kind; // 0 generic, 1 vectorised, 2 vectorised with per-point uPML
+    bool curl, upd;
+    DevBuf items;
+    std::vector<BatchItem<R>> host;
+    DevBuf blk_item[4], blk_first[4];
+    unsigned nblk[4];
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  DevBuf conn_items[2], conn_item_tab[2], conn_first_tab[2];
+  unsigned conn_nblk[2] = {0, 0};
+  std::vector<BatchDft<R>> dft_host;
+  DevBuf dft_items, dft_item_tab, dft_first_tab;
+  unsigned dft_nblk = 0;
+  std::vector<SrcRecipe> rec;
+  DevBuf rows;
+
+  explicit Batch(std::vector<Fields<R> *> s) : sims(std::move(s)) {
+    if (sims.empty()) fail("a batch needs at least one fields object");
+    HIPCHECK(hipStreamCreate(&stream));
+  }
+  ~Batch() override {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+  void options(int graph, int t, int b) override {
+    use_graph = graph != 0;
+    tune = t != 0;
+    if (b) {
+      bool ok = false;
+      for (int c : BATCH_BLOCKS) ok = ok || c == b;
+      if (!ok) fail("batch block size must be 64, 128, 256 or 512");
+      block = b;
+      tuned = true;
+    }
+  }
+  void stats(int *b, double *ms) const override {
+    *b = block;
+    *ms = last_ms;
+  }
+  static int bs_index(int b) { return b == 64 ? 0 : b == 128 ? 1 : b == 256 ? 2 : 3; }
+  // block tables of a list of item thread counts for block size bs, padded to 8 blocks
+  static void tables(const std::vector<uint32_t> &counts, unsigned bs, std::vector<uint32_t> &item, std::vector<uint32_t> &first) {
+    item.clear();
+    first.clear();
+    for (size_t k = 0; k < counts.size(); ++k)
+      for (uint32_t f = 0; f < counts[k]; f += bs) {
+        item.push_back(uint32_t(k));
+        first.push_back(f);
+      }
+    while (item.size() % 8) {
+      item.push_back(0);
+      first.push_back(0x7fffffffu); // beyond every item's count: the block exits at once
+    }
+  }
+  template <class T> static void upload(DevBuf &b, const std::vector<T> &v) {
+    b.alloc(sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (!v.empty()) HIPCHECK(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  }
+  bool stale() const {
+    if (!planned) return true;
+    for (size_t q = 0; q < sims.size(); ++q)
+      if (sims[q]->plan_version != plan_versions[q] || !sims[q]->plan_valid) return true;
+    return false;
+  }
+  // plan from the chunks' next step; returns the steps taken on the way (0 or 1)
+  long plan() {
+    long taken = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      rec.clear();
+      std::vector<typename Fields<R>::Collect> cs(sims.size());
+      bool ok = true;
+      for (size_t q = 0; q < sims.size(); ++q) {
+        cs[q].rec = &rec;
+        cs[q].sim = int(q);
+        ok = sims[q]->collect_step(cs[q]) && ok;
+      }
+      if (!ok) {
+        if (attempt) fail("batch: a chunk keeps allocating arrays or needs the unfused source path");
+        // the first step of a chunk allocates lazily; every chunk takes it on its own
+        for (Fields<R> *f : sims) f->step_n(1, false);
+        taken = 1;
+        continue;
+      }
+      build(cs);
+      return taken;
+    }
+    return taken;
+  }
+  void build(std::vector<typename Fields<R>::Collect> &cs) {
+    groups.clear();
+    for (int phase = 0; phase < 2; ++phase)
+      for (int key = 0; key < 12; ++key) {
+        std::unique_ptr<Group> gp(new Group);
+        Group &g = *gp;
+        g.phase = phase;
+        g.kind = key / 4;
+        g.curl = key & 2;
+        g.upd = key & 1;
+        for (auto &c : cs) {
+          auto it = c.items[phase].find(key);
+          if (it != c.items[phase].end()) g.host.insert(g.host.end(), it->second.begin(), it->second.end());
+        }
+        if (g.host.empty()) continue;
+        upload(g.items, g.host);
+        std::vector<uint32_t> counts;
+        for (auto &it : g.host) counts.push_back(it.count);
+        for (int b = 0; b < 4; ++b) {
+          std::vector<uint32_t> item, first;
+          tables(counts, unsigned(BATCH_BLOCKS[b]), item, first);
+          upload(g.blk_item[b], item);
+          upload(g.blk_first[b], first);
+          g.nblk[b] = unsigned(item.size());
+        }
+        groups.push_back(std::move(gp));
+      }
+    for (int grp = 0; grp < 2; ++grp) {
+      std::vector<BatchConn> ci;
+      for (auto &c : cs) ci.insert(ci.end(), c.conn[grp].begin(), c.conn[grp].end());
+      std::vector<uint32_t> counts;
+      for (auto &c : ci) counts.push_back(uint32_t(c.n));
+      std::vector<uint32_t> item, first;
+      tables(counts, 256, item, first);
+      upload(conn_items[grp], ci);
+      upload(conn_item_tab[grp], item);
+      upload(conn_first_tab[grp], first);
+      conn_nblk[grp] = ci.empty() ? 0 : unsigned(item.size());
+    }
+    dft_host.clear();
+    for (auto &c : cs) dft_host.insert(dft_host.end(), c.dft.begin(), c.dft.end());
+    {
+      std::vector<uint32_t> counts;
+      for (auto &d : dft_host) counts.push_back(uint32_t(size_t(d.a.npts) * d.a.nfreq));
+      std::vector<uint32_t> item, first;
+      tables(counts, 256, item, first);
+      upload(dft_item_tab, item);
+      upload(dft_first_tab, first);
+      dft_nblk = dft_host.empty() ? 0 : unsigned(item.size());
+    }
+    plan_versions.clear();
+    for (Fields<R> *f : sims) plan_versions.push_back(f->plan_version);
+    planned = true;
+  }
+  void launch_step(long s, int bidx, const double *vrow) {
+    for (int phase = 0; phase < 2; ++phase) {
+      for (auto &gp : groups)
+        if (gp->phase == phase)
+          launch_step_batch<R>(static_cast<const BatchItem<R> *>(gp->items.p), static_cast<const uint32_t *>(gp->blk_item[bidx].p),
+                               static_cast<const uint32_t *>(gp->blk_first[bidx].p), gp->nblk[bidx], gp->kind, gp->curl, gp->upd,
+                               BATCH_BLOCKS[bidx], vrow, stream);
+      launch_conn_batch<R>(static_cast<const BatchConn *>(conn_items[phase].p), static_cast<const uint32_t *>(conn_item_tab[phase].p),
+                           static_cast<const uint32_t *>(conn_first_tab[phase].p), conn_nblk[phase], stream);
+    }
+    launch_dft_batch<R>(static_cast<const BatchDft<R> *>(dft_items.p), static_cast<const uint32_t *>(dft_item_tab.p),
+                        static_cast<const uint32_t *>(dft_first_tab.p), dft_nblk, s, stream);
+  }
+  void step_n(long n) override {
+    if (n <= 0) return;
+    for (Fields<R> *f : sims) HIPCHECK(hipStreamSynchronize(f->stream));
+    long done = 0;
+    if (stale()) done = plan();
+    const long m = n - done;
+    if (m <= 0) return;
+    for (Fields<R> *f : sims)
+      if (!f->plan_valid) fail("internal: a chunk's plan changed during batch planning");
+    // source rows for the m steps, and the DFT chunks' starting times
+    const size_t rl = std::max<size_t>(rec.size(), 1);
+    std::vector<double> hrows(rl * size_t(m), 0.0);
+    for (long st = 0; st < m; ++st)
+      for (size_t q = 0; q < rec.size(); ++q) {
+        const SrcRecipe &r = rec[q];
+        Fields<R> *F = sims[r.sim];
+        const SrcTime &src = F->srcs[r.src];
+        const double time = double(F->t + st) * F->dt + r.toff * F->dt;
+        const bool integrated = r.toff == 1.0;
+        const cdouble scal = integrated ? src.dipole(time) : src.current(time, F->dt);
+        double v;
+        if (r.kind == 0) {
+          const cdouble A = (r.amp * scal) * r.mult;
+          v = r.cmp ? A.imag() : A.real();
+        }
+        else
+          v = r.cmp ? scal.imag() : scal.real();
+        hrows[size_t(st) * rl + q] = v;
+      }
+    upload(rows, hrows);
+    {
+      size_t k = 0;
+      for (size_t q = 0; q < sims.size(); ++q)
+        for (auto &cp : sims[q]->dft_chunks)
+          if (sims[q]->f[cp->c][0]) dft_host[k++].t_start = sims[q]->t;
+      if (k != dft_host.size()) fail("internal: batch DFT chunks changed");
+      upload(dft_items, dft_host);
+    }
+    const double *drows = static_cast<const double *>(rows.p);
+    auto t0 = std::chrono::steady_clock::now();
+    long st = 0;
+    if (tune && !tuned) {
+      // two steps with each candidate block size, timed on the stream
+      double best = 1e300;
+      for (int b = 0; b < 4 && st + 2 <= m; ++b) {
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        HIPCHECK(hipEventRecord(e0, stream));
+        for (int r = 0; r < 2; ++r, ++st) launch_step(st, b, drows + size_t(st) * rl);
+        HIPCHECK(hipEventRecord(e1, stream));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms < best) best = ms, block = BATCH_BLOCKS[b];
+      }
+      tuned = st > 0;
+    }
+    const int bidx = bs_index(block);
+    if (st < m) {
+      if (use_graph) {
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (long q = st; q < m; ++q) launch_step(q, bidx, drows + size_t(q) * rl);
+        HIPCHECK(hipStreamEndCapture(stream, &graph));
+        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphLaunch(exec, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+      }
+      else
+        for (long q = st; q < m; ++q) launch_step(q, bidx, drows + size_t(q) * rl);
+    }
+    for (Fields<R> *f : sims) f->probe_on(stream);
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+    last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / double(m);
+    for (Fields<R> *f : sims) {
+      f->t += m;
+      f->check_flag();
+    }
+  }
+};
+
 template <class F> int guard(F fn) {
   try {
     fn();
@@ -2794,6 +3242,36 @@ double mp_fields_round_time(const mp_fields *f) {
 long mp_fields_t(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->t_now(); }
 double mp_fields_dt(const mp_fields *f) { return reinterpret_cast<const FieldsBase *>(f)->dt_now(); }
 double mp_fields_last_source_time(mp_fields *f) { return F_->last_source_time(); }
+mp_batch *mp_batch_new(mp_fields *const *fields, int n) {
+  BatchBase *b = nullptr;
+  int rc = guard([&] {
+    if (n <= 0 || !fields) fail("a batch needs at least one fields object");
+    std::vector<Fields<float> *> fs;
+    std::vector<Fields<double> *> fd;
+    for (int q = 0; q < n; ++q) {
+      FieldsBase *F = reinterpret_cast<FieldsBase *>(fields[q]);
+      if (auto *x = dynamic_cast<Fields<float> *>(F)) fs.push_back(x);
+      if (auto *x = dynamic_cast<Fields<double> *>(F)) fd.push_back(x);
+    }
+    if (int(fs.size()) == n)
+      b = new Batch<float>(fs);
+    else if (int(fd.size()) == n)
+      b = new Batch<double>(fd);
+    else
+      fail("the fields objects of a batch must share one precision");
+  });
+  return rc ? nullptr : reinterpret_cast<mp_batch *>(b);
+}
+void mp_batch_delete(mp_batch *b) { delete reinterpret_cast<BatchBase *>(b); }
+int mp_batch_step_n(mp_batch *b, long n) {
+  return guard([&] { reinterpret_cast<BatchBase *>(b)->step_n(n); });
+}
+int mp_batch_set_options(mp_batch *b, int use_graph, int autotune, int block) {
+  return guard([&] { reinterpret_cast<BatchBase *>(b)->options(use_graph, autotune, block); });
+}
+int mp_batch_stats(mp_batch *b, int *block, double *ms_per_step) {
+  return guard([&] { reinterpret_cast<const BatchBase *>(b)->stats(block, ms_per_step); });
+}
 struct AdjHandle {
   FieldsBase *owner;
 };

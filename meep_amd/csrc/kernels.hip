@@ -140,9 +140,13 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
   if (UPD && a.uc[C].f) {
     const UpdComp<R> &u = a.uc[C];
     R gv = CURL ? fv : u.g[i];
+    bool pin[MAXPOLE]; // inside pole p's box
 #pragma unroll
-    for (int p = 0; p < MAXPOLE; ++p)
-      if (p < a.npole && u.P[p]) gv -= u.P[p][i];
+    for (int p = 0; p < MAXPOLE; ++p) {
+      pin[p] = p < a.npole && u.P[p] && j0 >= a.pole[p].lo[0] && j0 <= a.pole[p].hi[0] && j1 >= a.pole[p].lo[1] &&
+               j1 <= a.pole[p].hi[1] && j2 >= a.pole[p].lo[2] && j2 <= a.pole[p].hi[2];
+      if (pin[p]) gv -= u.P[p][i];
+    }
     if (in_tab(a.dip, i)) gv = apply_tab(a.dip, C, i, gv, vrow);
     if (a.dipb.n) gv = apply_boxes(a.dipb, a.cmp, C, j0, j1, j2, gv, vrow);
     R ev, wv;
@@ -161,7 +165,7 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
     }
 #pragma unroll
     for (int p = 0; p < MAXPOLE; ++p)
-      if (p < a.npole && u.P[p] && u.sigma[p]) {
+      if (pin[p] && u.sigma[p]) {
         const PoleCoef<R> &q = a.pole[p];
         const R pcur = u.P[p][i];
         const R pn = q.gamma1inv * (pcur * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * u.Pp[p][i] +
@@ -271,12 +275,12 @@ template <class R, int V> __device__ __forceinline__ Vec<R, V> signed_vec(const 
 // store the owned points of a group and their periodic images: whole-row images along loop
 // axes 0 and 1 (uniform over the row), single-point images along the row's own axis
 template <class R, int V>
-__device__ __forceinline__ void put_row(R *f, uint32_t i, const Vec<R, V> &x, unsigned mask, const Grid &g, const AxisSlot &ax,
-                                        int j0, int j1, int z0) {
+__device__ __forceinline__ void put_row(R *f, uint32_t i, const Vec<R, V> &x, unsigned mask, const Grid &g, int ybits, int j0,
+                                        int j1, int z0) {
   vstore<R, V>(f + i, x, mask);
   if (!(g.wrap[0] | g.wrap[1] | g.wrap[2])) return;
-  const int e0 = g.wrap[0] ? zwrap_dst(ax.ysh[0], j0, g.nax[0]) : -1;
-  const int e1 = g.wrap[1] ? zwrap_dst(ax.ysh[1], j1, g.nax[1]) : -1;
+  const int e0 = g.wrap[0] ? zwrap_dst(ybits & 1, j0, g.nax[0]) : -1;
+  const int e1 = g.wrap[1] ? zwrap_dst((ybits >> 1) & 1, j1, g.nax[1]) : -1;
   const int64_t o0 = e0 >= 0 ? int64_t(e0 - j0) * g.ast[0] : 0, o1 = e1 >= 0 ? int64_t(e1 - j1) * g.ast[1] : 0;
   if (e0 >= 0) vstore<R, V>(f + (int64_t(i) + o0), signed_vec(x, g.wrap[0]), mask);
   if (e1 >= 0) vstore<R, V>(f + (int64_t(i) + o1), signed_vec(x, g.wrap[1]), mask);
@@ -285,7 +289,7 @@ __device__ __forceinline__ void put_row(R *f, uint32_t i, const Vec<R, V> &x, un
 #pragma unroll
   for (int k = 0; k < V; ++k)
     if ((mask >> k) & 1) {
-      const int zd = zwrap_dst(ax.ysh[2], z0 + k, g.nax[2]);
+      const int zd = zwrap_dst((ybits >> 2) & 1, z0 + k, g.nax[2]);
       if (zd < 0) continue;
       const int64_t b = int64_t(i) + k + (zd - (z0 + k));
       const R v = x.v[k];
@@ -296,32 +300,78 @@ __device__ __forceinline__ void put_row(R *f, uint32_t i, const Vec<R, V> &x, un
     }
 }
 
+// Owned range and metal planes of a slot along loop axis k, from its Yee-shift bits (the
+// reference's owned range [1 - ysh, n - ysh] and on_metal_boundary, src/vec.cpp:432-463,
+// src/boundaries.cpp:191-205); derived rather than loaded to keep the kernel's scalar
+// registers for the pointers.
+__device__ __forceinline__ bool owns_at(const Grid &g, int ybits, int k, int j) {
+  const int n = g.nax[k], y = (ybits >> k) & 1;
+  return n ? (j >= 1 - y && j <= n - y) : j == 0;
+}
+__device__ __forceinline__ bool metal_at(const Grid &g, int ybits, int k, int j) {
+  const int n = g.nax[k], y = (ybits >> k) & 1;
+  return n && ((((g.mhi_bits >> k) & 1) && y == 0 && j == n) || (((g.mlo_bits >> k) & 1) && y == 1 && j == 0));
+}
 // points of a group that are owned by a slot / on its metal planes
-template <int V> __device__ __forceinline__ unsigned group_owned(const AxisSlot &s, int j0, int j1, int z0, unsigned boxmask) {
-  if (j0 < s.lo[0] || j0 > s.hi[0] || j1 < s.lo[1] || j1 > s.hi[1]) return 0u;
+template <int V> __device__ __forceinline__ unsigned group_owned(const Grid &g, int ybits, int j0, int j1, int z0, unsigned boxmask) {
+  if (!owns_at(g, ybits, 0, j0) || !owns_at(g, ybits, 1, j1)) return 0u;
   unsigned m = boxmask;
 #pragma unroll
   for (int k = 0; k < V; ++k)
-    if (z0 + k < s.lo[2] || z0 + k > s.hi[2]) m &= ~(1u << k);
+    if (!owns_at(g, ybits, 2, z0 + k)) m &= ~(1u << k);
   return m;
 }
-template <int V> __device__ __forceinline__ unsigned group_metal(const AxisSlot &s, int j0, int j1, int z0) {
-  if (j0 == s.mhi[0] || j0 == s.mlo[0] || j1 == s.mhi[1] || j1 == s.mlo[1]) return (1u << V) - 1;
+template <int V> __device__ __forceinline__ unsigned group_metal(const Grid &g, int ybits, int j0, int j1, int z0) {
+  if (!(g.mhi_bits | g.mlo_bits)) return 0u;
+  if (metal_at(g, ybits, 0, j0) || metal_at(g, ybits, 1, j1)) return (1u << V) - 1;
   unsigned m = 0;
 #pragma unroll
   for (int k = 0; k < V; ++k)
-    if (z0 + k == s.mhi[2] || z0 + k == s.mlo[2]) m |= 1u << k;
+    if (metal_at(g, ybits, 2, z0 + k)) m |= 1u << k;
   return m;
 }
 
 // slot C of the vectorised kernel; F0..F2 are the partner centre rows by direction
-template <class R, int V, bool CURL, bool UPD, int C>
+// uPML coefficients of a group's V points along loop axis ks (-1: none): index 2 j + y with j
+// the point's coordinate along ks (the row's own axis varies per point, the others not);
+// bit k of `on` is set where the profile is active (sig != 0 or kap != 1)
+template <class R, int V> struct ProfV {
+  R sig[V], kap[V], sinv[V];
+  unsigned on;
+};
+template <class R, int V>
+__device__ __forceinline__ void load_prof(ProfV<R, V> &p, const R *sig, const R *kap, const R *siginv, int ks, int y, int j0, int j1,
+                                          int z0, int nz) {
+  p.on = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) p.sig[k] = R(0), p.kap[k] = R(1), p.sinv[k] = R(1);
+  if (!sig || ks < 0) return;
+  if (ks == 2) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const int idx = 2 * min(z0 + k, nz) + y; // points past the row's end are masked off
+      p.sig[k] = sig[idx];
+      p.kap[k] = kap[idx];
+      if (siginv) p.sinv[k] = siginv[idx];
+      if (p.sig[k] != R(0) || p.kap[k] != R(1)) p.on |= 1u << k;
+    }
+  }
+  else {
+    const int idx = 2 * (ks == 0 ? j0 : j1) + y;
+    const R sg = sig[idx], kp = kap[idx], si = siginv ? siginv[idx] : R(1);
+#pragma unroll
+    for (int k = 0; k < V; ++k) p.sig[k] = sg, p.kap[k] = kp, p.sinv[k] = si;
+    if (sg != R(0) || kp != R(1)) p.on = (1u << V) - 1;
+  }
+}
+
+template <class R, int V, bool CURL, bool UPD, bool PML, int C>
 __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int z0, unsigned mask,
                                           const Vec<R, V> &F0, const Vec<R, V> &F1, const Vec<R, V> &F2, const double *vrow) {
   if (!mask) return;
   const Grid &g = a.g;
-  const AxisSlot &ax = CURL ? a.cc[C].ax : a.uc[C].ax;
-  const unsigned metal = group_metal<V>(ax, j0, j1, z0);
+  const int ybits = CURL ? a.cc[C].ax.ybits : a.uc[C].ax.ybits;
+  const unsigned metal = group_metal<V>(g, ybits, j0, j1, z0);
   Vec<R, V> fv;
   if (CURL) {
     const CurlComp<R> &kc = a.cc[C];
@@ -350,8 +400,40 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       }
     }
     fv = vload<R, V>(kc.f + i);
+    if (!PML) {
 #pragma unroll
-    for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
+      for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
+    }
+    else {
+      // the four step_curl branches per point (src/step_generic.cpp:65-249)
+      ProfV<R, V> P, U;
+      load_prof<R, V>(P, kc.sig, kc.kap, kc.siginv, kc.ksig, kc.ysig, j0, j1, z0, g.nax[2]);
+      load_prof<R, V>(U, kc.sigu, kc.kapu, kc.siginvu, kc.ksigu, kc.ysigu, j0, j1, z0, g.nax[2]);
+      const unsigned um = U.on & mask;
+      Vec<R, V> fuv;
+      if (um) fuv = vload<R, V>(kc.fu + i);
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const bool pml = (P.on >> k) & 1, pmlu = (U.on >> k) & 1;
+        if (!pmlu) {
+          if (!pml)
+            fv.v[k] -= a.dtdx * cu.v[k];
+          else
+            fv.v[k] = ((P.kap[k] - P.sig[k]) * fv.v[k] - a.dtdx * cu.v[k]) * P.sinv[k];
+        }
+        else {
+          const R fprev = fuv.v[k];
+          R fun;
+          if (!pml)
+            fun = fprev - a.dtdx * cu.v[k];
+          else
+            fun = ((P.kap[k] - P.sig[k]) * fprev - a.dtdx * cu.v[k]) * P.sinv[k];
+          fuv.v[k] = fun;
+          fv.v[k] = U.sinv[k] * ((U.kap[k] - U.sig[k]) * fv.v[k] + fun - fprev);
+        }
+      }
+      if (um) vstore<R, V>(kc.fu + i, fuv, um);
+    }
     if (a.cur.n && int64_t(i) + V - 1 >= a.cur.lo && int64_t(i) <= a.cur.hi) {
 #pragma unroll
       for (int k = 0; k < V; ++k) fv.v[k] = apply_tab(a.cur, C, i + k, fv.v[k], vrow);
@@ -365,19 +447,23 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       for (int k = 0; k < V; ++k)
         if ((metal >> k) & 1) fv.v[k] = R(0);
     }
-    put_row<R, V>(kc.f, i, fv, mask, g, ax, j0, j1, z0);
+    put_row<R, V>(kc.f, i, fv, mask, g, ybits, j0, j1, z0);
   }
   if (UPD && a.uc[C].f) {
     const UpdComp<R> &u = a.uc[C];
     Vec<R, V> gv = CURL ? fv : vload<R, V>(u.g + i);
     Vec<R, V> Pc[MAXPOLE];
+    bool pin[MAXPOLE]; // the group meets pole p's box
 #pragma unroll
-    for (int p = 0; p < MAXPOLE; ++p)
-      if (p < a.npole && u.P[p]) {
+    for (int p = 0; p < MAXPOLE; ++p) {
+      pin[p] = p < a.npole && u.P[p] && j0 >= a.pole[p].lo[0] && j0 <= a.pole[p].hi[0] && j1 >= a.pole[p].lo[1] &&
+               j1 <= a.pole[p].hi[1] && z0 + V - 1 >= a.pole[p].lo[2] && z0 <= a.pole[p].hi[2];
+      if (pin[p]) {
         Pc[p] = vload<R, V>(u.P[p] + i);
 #pragma unroll
         for (int k = 0; k < V; ++k) gv.v[k] -= Pc[p].v[k];
       }
+    }
     if (a.dip.n && int64_t(i) + V - 1 >= a.dip.lo && int64_t(i) <= a.dip.hi) {
 #pragma unroll
       for (int k = 0; k < V; ++k) gv.v[k] = apply_tab(a.dip, C, i + k, gv.v[k], vrow);
@@ -392,16 +478,35 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
 #pragma unroll
       for (int k = 0; k < V; ++k) ev.v[k] = gv.v[k] * uu.v[k];
     }
+    Vec<R, V> wv = ev; // W of the polarisations: E, or f_w in the uPML
+    if (PML) {
+      // step_update_EDHB's uPML branch per point (src/step_generic.cpp:682-701)
+      ProfV<R, V> W;
+      load_prof<R, V>(W, u.sigw, u.kapw, nullptr, u.ksigw, u.ysigw, j0, j1, z0, g.nax[2]);
+      const unsigned wm = W.on & mask;
+      if (wm) {
+        Vec<R, V> fwv = vload<R, V>(u.fw + i);
+        const Vec<R, V> eold = vload<R, V>(u.f + i);
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+          if ((W.on >> k) & 1) {
+            const R fwprev = fwv.v[k], fwn = ev.v[k];
+            fwv.v[k] = fwn;
+            ev.v[k] = eold.v[k] + ((W.kap[k] + W.sig[k]) * fwn - (W.kap[k] - W.sig[k]) * fwprev);
+          }
+        vstore<R, V>(u.fw + i, fwv, wm);
+      }
+    }
 #pragma unroll
     for (int p = 0; p < MAXPOLE; ++p)
-      if (p < a.npole && u.P[p] && u.sigma[p]) {
+      if (pin[p] && u.sigma[p]) {
         const PoleCoef<R> &q = a.pole[p];
         const Vec<R, V> Pp = vload<R, V>(u.Pp[p] + i), sg = vload<R, V>(u.sigma[p] + i);
         Vec<R, V> Pn;
 #pragma unroll
         for (int k = 0; k < V; ++k)
-          Pn.v[k] = q.gamma1inv * (Pc[p].v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Pp.v[k] + q.omega0dtsqr * (sg.v[k] * ev.v[k]));
-        put_row<R, V>(u.P[p], i, Pn, mask, g, ax, j0, j1, z0);
+          Pn.v[k] = q.gamma1inv * (Pc[p].v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Pp.v[k] + q.omega0dtsqr * (sg.v[k] * wv.v[k]));
+        put_row<R, V>(u.P[p], i, Pn, mask, g, ybits, j0, j1, z0);
         vstore<R, V>(u.Pp[p] + i, Pc[p], mask);
       }
     if (metal) {
@@ -409,12 +514,12 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       for (int k = 0; k < V; ++k)
         if ((metal >> k) & 1) ev.v[k] = R(0);
     }
-    put_row<R, V>(u.f, i, ev, mask, g, ax, j0, j1, z0);
+    put_row<R, V>(u.f, i, ev, mask, g, ybits, j0, j1, z0);
   }
 }
 
 // group t of the launch box of a (groups of V points along the rows, from group g0)
-template <class R, int V, bool CURL, bool UPD>
+template <class R, int V, bool CURL, bool UPD, bool PML>
 __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int g0, int ngroups, const double *vrow) {
   const Grid &g = a.g;
   const uint32_t r = t / uint32_t(ngroups);
@@ -431,9 +536,9 @@ __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int 
     if (z0 + k >= g.alo[2] && z0 + k < g.alo[2] + g.an[2]) boxmask |= 1u << k;
   // ownership of each component slot in this group
   unsigned m0 = 0, m1 = 0, m2 = 0;
-  if (CURL ? a.cc[0].f : a.uc[0].f) m0 = group_owned<V>(CURL ? a.cc[0].ax : a.uc[0].ax, j0, j1, z0, boxmask);
-  if (CURL ? a.cc[1].f : a.uc[1].f) m1 = group_owned<V>(CURL ? a.cc[1].ax : a.uc[1].ax, j0, j1, z0, boxmask);
-  if (CURL ? a.cc[2].f : a.uc[2].f) m2 = group_owned<V>(CURL ? a.cc[2].ax : a.uc[2].ax, j0, j1, z0, boxmask);
+  if (CURL ? a.cc[0].f : a.uc[0].f) m0 = group_owned<V>(g, CURL ? a.cc[0].ax.ybits : a.uc[0].ax.ybits, j0, j1, z0, boxmask);
+  if (CURL ? a.cc[1].f : a.uc[1].f) m1 = group_owned<V>(g, CURL ? a.cc[1].ax.ybits : a.uc[1].ax.ybits, j0, j1, z0, boxmask);
+  if (CURL ? a.cc[2].f : a.uc[2].f) m2 = group_owned<V>(g, CURL ? a.cc[2].ax.ybits : a.uc[2].ax.ybits, j0, j1, z0, boxmask);
   if (!(m0 | m1 | m2)) return;
   // centre rows of the partner components (by direction), loaded once
   Vec<R, V> F0, F1, F2;
@@ -442,17 +547,17 @@ __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int 
     if (a.F[1]) F1 = vload<R, V>(a.F[1] + i);
     if (a.F[2]) F2 = vload<R, V>(a.F[2] + i);
   }
-  fast_slot<R, V, CURL, UPD, 0>(a, i, j0, j1, z0, m0, F0, F1, F2, vrow);
-  fast_slot<R, V, CURL, UPD, 1>(a, i, j0, j1, z0, m1, F0, F1, F2, vrow);
-  fast_slot<R, V, CURL, UPD, 2>(a, i, j0, j1, z0, m2, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, 0>(a, i, j0, j1, z0, m0, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, 1>(a, i, j0, j1, z0, m1, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, 2>(a, i, j0, j1, z0, m2, F0, F1, F2, vrow);
 }
 
 
-template <class R, int V, bool CURL, bool UPD>
+template <class R, int V, bool CURL, bool UPD, bool PML>
 __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t count, int g0, int ngroups) {
   const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
-  fast_body<R, V, CURL, UPD>(a, t, g0, ngroups, nullptr);
+  fast_body<R, V, CURL, UPD, PML>(a, t, g0, ngroups, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -598,7 +703,7 @@ template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, h
     hipLaunchKernelGGL((k_step<R, false, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32);
 }
 
-template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
+template <class R, bool PML> void launch_fast_t(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
   constexpr int V = 16 / sizeof(R);
   const int g0 = a.g.alo[2] / V, g1 = (a.g.alo[2] + a.g.an[2] - 1) / V;
   const int ngroups = g1 - g0 + 1;
@@ -606,11 +711,17 @@ template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, h
   const unsigned nb = nblocks8(count);
   const uint32_t c32 = uint32_t(count);
   if (curl && upd)
-    hipLaunchKernelGGL((k_fast<R, V, true, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+    hipLaunchKernelGGL((k_fast<R, V, true, true, PML>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
   else if (curl)
-    hipLaunchKernelGGL((k_fast<R, V, true, false>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+    hipLaunchKernelGGL((k_fast<R, V, true, false, PML>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
   else if (upd)
-    hipLaunchKernelGGL((k_fast<R, V, false, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+    hipLaunchKernelGGL((k_fast<R, V, false, true, PML>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+}
+template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, bool pml, hipStream_t s) {
+  if (pml)
+    launch_fast_t<R, true>(a, curl, upd, s);
+  else
+    launch_fast_t<R, false>(a, curl, upd, s);
 }
 
 template <class R>
@@ -671,7 +782,8 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
 // ---- batched launches -----------------------------------------------------------------
 // Logical blocks are dealt to XCDs in contiguous runs as in the single-chunk kernels (nblk is a
 // multiple of 8); each logical block belongs to one item.
-template <class R, int BS, bool FAST, bool CURL, bool UPD>
+// KIND: 0 generic, 1 vectorised, 2 vectorised with per-point uPML
+template <class R, int BS, int KIND, bool CURL, bool UPD>
 __global__ __launch_bounds__(BS) void k_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first,
                                                    const double *vrow) {
   const uint32_t nb = gridDim.x, b = blockIdx.x;
@@ -680,42 +792,44 @@ __global__ __launch_bounds__(BS) void k_step_batch(const BatchItem<R> *items, co
   const BatchItem<R> &it = items[k];
   const uint32_t t = blk_first[lb] + threadIdx.x;
   if (t >= it.count) return;
-  if (FAST)
-    fast_body<R, 16 / sizeof(R), CURL, UPD>(it.a, t, it.g0, it.ngroups, vrow);
+  if (KIND == 1)
+    fast_body<R, 16 / sizeof(R), CURL, UPD, false>(it.a, t, it.g0, it.ngroups, vrow);
+  else if (KIND == 2)
+    fast_body<R, 16 / sizeof(R), CURL, UPD, true>(it.a, t, it.g0, it.ngroups, vrow);
   else
     step_body<R, CURL, UPD>(it.a, t, vrow);
 }
 
-template <class R, int BS>
-void launch_step_batch_bs(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, bool fast,
-                          bool curl, bool upd, const double *vrow, hipStream_t s) {
+template <class R, int BS, int KIND>
+void launch_step_batch_k(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, bool curl,
+                         bool upd, const double *vrow, hipStream_t s) {
   const dim3 grid(nblk), blk(BS);
-  if (fast) {
-    if (curl && upd)
-      hipLaunchKernelGGL((k_step_batch<R, BS, true, true, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
-    else if (curl)
-      hipLaunchKernelGGL((k_step_batch<R, BS, true, true, false>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
-    else
-      hipLaunchKernelGGL((k_step_batch<R, BS, true, false, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
-  }
-  else {
-    if (curl && upd)
-      hipLaunchKernelGGL((k_step_batch<R, BS, false, true, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
-    else if (curl)
-      hipLaunchKernelGGL((k_step_batch<R, BS, false, true, false>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
-    else
-      hipLaunchKernelGGL((k_step_batch<R, BS, false, false, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
-  }
+  if (curl && upd)
+    hipLaunchKernelGGL((k_step_batch<R, BS, KIND, true, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+  else if (curl)
+    hipLaunchKernelGGL((k_step_batch<R, BS, KIND, true, false>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+  else
+    hipLaunchKernelGGL((k_step_batch<R, BS, KIND, false, true>), grid, blk, 0, s, items, blk_item, blk_first, vrow);
+}
+template <class R, int BS>
+void launch_step_batch_bs(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, int kind,
+                          bool curl, bool upd, const double *vrow, hipStream_t s) {
+  if (kind == 2)
+    launch_step_batch_k<R, BS, 2>(items, blk_item, blk_first, nblk, curl, upd, vrow, s);
+  else if (kind == 1)
+    launch_step_batch_k<R, BS, 1>(items, blk_item, blk_first, nblk, curl, upd, vrow, s);
+  else
+    launch_step_batch_k<R, BS, 0>(items, blk_item, blk_first, nblk, curl, upd, vrow, s);
 }
 template <class R>
-void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, bool fast,
+void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, int kind,
                        bool curl, bool upd, int block, const double *vrow, hipStream_t s) {
   if (!nblk) return;
   switch (block) {
-    case 64: launch_step_batch_bs<R, 64>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
-    case 128: launch_step_batch_bs<R, 128>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
-    case 256: launch_step_batch_bs<R, 256>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
-    default: launch_step_batch_bs<R, 512>(items, blk_item, blk_first, nblk, fast, curl, upd, vrow, s); break;
+    case 64: launch_step_batch_bs<R, 64>(items, blk_item, blk_first, nblk, kind, curl, upd, vrow, s); break;
+    case 128: launch_step_batch_bs<R, 128>(items, blk_item, blk_first, nblk, kind, curl, upd, vrow, s); break;
+    case 256: launch_step_batch_bs<R, 256>(items, blk_item, blk_first, nblk, kind, curl, upd, vrow, s); break;
+    default: launch_step_batch_bs<R, 512>(items, blk_item, blk_first, nblk, kind, curl, upd, vrow, s); break;
   }
 }
 
@@ -748,9 +862,9 @@ void launch_dft_batch(const BatchDft<R> *items, const uint32_t *blk_item, const 
 template <class R>
 __global__ __launch_bounds__(BLOCK) void k_conn_batch(const BatchConn *items, const uint32_t *blk_item, const uint32_t *blk_first) {
   const BatchConn &it = items[blk_item[blockIdx.x]];
-  const int t = int(blk_first[blockIdx.x]) + int(threadIdx.x);
-  if (t >= it.n) return;
-  conn_one<R>(it.conns, it.phases, t, reinterpret_cast<R *const *>(it.slots), it.ncmp);
+  const uint32_t t = blk_first[blockIdx.x] + threadIdx.x; // padding blocks start past every item
+  if (t >= uint32_t(it.n)) return;
+  conn_one<R>(it.conns, it.phases, int(t), reinterpret_cast<R *const *>(it.slots), it.ncmp);
 }
 template <class R>
 void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, hipStream_t s) {
@@ -758,12 +872,12 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
 }
 
 #define INSTANTIATE(R)                                                                             \
-  template void launch_step_batch<R>(const BatchItem<R> *, const uint32_t *, const uint32_t *, unsigned, bool, bool, bool, int, \
+  template void launch_step_batch<R>(const BatchItem<R> *, const uint32_t *, const uint32_t *, unsigned, int, bool, bool, int, \
                                      const double *, hipStream_t);                                 \
   template void launch_dft_batch<R>(const BatchDft<R> *, const uint32_t *, const uint32_t *, unsigned, long, hipStream_t); \
   template void launch_conn_batch<R>(const BatchConn *, const uint32_t *, const uint32_t *, unsigned, hipStream_t); \
   template void launch_step<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
-  template void launch_fast<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
+  template void launch_fast<R>(const StepArgs<R> &, bool, bool, bool, hipStream_t);                \
   template void launch_conn<R>(const Conn *, const double2 *, int, R *const *, int, hipStream_t);  \
   template void launch_src<R>(R *, const int64_t *, const double2 *, int, double2, double, int,      \
                               hipStream_t);                                                        \

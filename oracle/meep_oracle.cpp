@@ -2042,6 +2042,26 @@ int mp_adjoint_gradient(mp_adjoint *, double *) {
   return -1;
 }
 
+// Batches are a product feature with no counterpart in the reference; batched runs are checked
+// against the same objects stepped one at a time.
+mp_batch *mp_batch_new(mp_fields *const *, int) {
+  last_error = "the oracle has no batches";
+  return nullptr;
+}
+void mp_batch_delete(mp_batch *) {}
+int mp_batch_step_n(mp_batch *, long) {
+  last_error = "the oracle has no batches";
+  return -1;
+}
+int mp_batch_set_options(mp_batch *, int, int, int) {
+  last_error = "the oracle has no batches";
+  return -1;
+}
+int mp_batch_stats(mp_batch *, int *, double *) {
+  last_error = "the oracle has no batches";
+  return -1;
+}
+
 // fields::field_energy (src/energy_and_flux.cpp:48)
 int mp_fields_field_energy(mp_fields *f, double *out) {
   return guard([&] {

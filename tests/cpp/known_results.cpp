@@ -75,6 +75,15 @@ static double polariton_ex(const grid_volume &gv, double eps(const vec &)) {
   return real(f.get_field(Ex, gv.center()));
 }
 
+static double polariton_energy(const grid_volume &gv, double eps(const vec &)) {
+  structure s(gv, eps);
+  s.add_susceptibility(sigma_polariton, E_stuff, lorentzian_susceptibility(0.3, 0.1));
+  fields f(&s);
+  f.add_point_source(Ex, 0.2, 3.0, 0.0, 2.0, gv.center(), src_amp);
+  while (f.round_time() < 10.0) f.step();
+  return f.field_energy();
+}
+
 struct known_case {
   const char *name;
   double golden;
@@ -92,10 +101,10 @@ int main(int argc, char **argv) {
       filter = argv[i];
   }
   const double a = 10.0;
-  // golden values: tests/known_results.cpp:143-156 (the polariton energy case needs
-  // fields::field_energy, which this API layer does not expose)
+  // golden values: tests/known_results.cpp:143-156
   const known_case cases[] = {
       {"1D polariton", -0.0894851, polariton_ex, volone(1.0, a), one},
+      {"1D polariton energy", 0.0863443, polariton_energy, volone(1.0, a), one},
       {"1x1 metallic 2D TM", 5.20605, metallic_ez, voltwo(1.0, 1.0, a), one},
       {"1x1 PML 2D TM", 0.883776, using_pml_ez, voltwo(1.0 + 2 * dpml, 1.0 + 2 * dpml, a), one},
       {"1x1 X periodic 2D TM", 0.110425, x_periodic, voltwo(1.0, 1.0, a), one},

@@ -57,7 +57,7 @@ def test_cpp_api_known_results_on_oracle(tmp_path, oracle_lib):
     for args in ([], ["--single"]):
         r = subprocess.run([out] + args, capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stdout + r.stderr
-        assert r.stdout.count("Passed") == 12, r.stdout
+        assert r.stdout.count("Passed") == 13, r.stdout
 
 
 def test_cpp_api_errors_become_exceptions(tmp_path, oracle_lib):

@@ -21,4 +21,4 @@ def test_cpp_known_results_on_gpu(args):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("engine hip-gfx950")
-    assert r.stdout.count("Passed") == 12, r.stdout
+    assert r.stdout.count("Passed") == 13, r.stdout
