@@ -23,6 +23,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace mx {
@@ -1392,6 +1393,8 @@ template <class R> struct Fields : FieldsBase {
   // while they fit (SRC_BOX, and a box at most 8x the list), the rest into `rest`, which the
   // caller applies with k_src.  Values are (amp * current(t)) * dt for currents
   // (src/step.cpp:307-315) and amp * dipole(t) for integrated sources (src/update_eh.cpp:128-138).
+  // Batch planning (rec set) has no k_src pass: there the long lists of one source time and
+  // component share one merged box.
   void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, double time, SrcTab &tab, SrcBoxes &bx,
                 std::vector<LongSrc> &rest, std::vector<SrcRecipe> *rec = nullptr, int sim = 0, double toff = 0) {
     tab.n = 0;
@@ -1401,6 +1404,7 @@ template <class R> struct Fields : FieldsBase {
     // batched steps take every value from the step's row: the table's entries first, then
     // two per box (scal), so the layout does not depend on the time
     std::vector<SrcRecipe> trec, brec;
+    std::vector<LongSrc> longs; // batch planning: long lists, merged below
     for (SrcVol &sv : sources[ft2]) {
       const SrcTime &st = srcs[sv.src];
       if (st.is_integrated != integrated) continue;
@@ -1425,6 +1429,10 @@ template <class R> struct Fields : FieldsBase {
         }
         continue;
       }
+      if (rec) {
+        longs.push_back({&sv, slot, scal, mult});
+        continue;
+      }
       prepare_src(sv);
       if (sv.use_box && bx.n < SRC_BOX) {
         const int b = bx.n++;
@@ -1447,12 +1455,95 @@ template <class R> struct Fields : FieldsBase {
       }
       rest.push_back({&sv, slot, scal, mult});
     }
+    std::vector<char> taken(longs.size(), 0);
+    for (size_t q = 0; q < longs.size(); ++q) {
+      if (taken[q]) continue;
+      std::vector<SrcVol *> group;
+      for (size_t r = q; r < longs.size(); ++r)
+        if (!taken[r] && longs[r].sv->src == longs[q].sv->src && longs[r].slot == longs[q].slot) {
+          group.push_back(longs[r].sv);
+          taken[r] = 1;
+        }
+      const MergedBox *m = merged_box(ft2, group);
+      if (!m || bx.n >= SRC_BOX) {
+        for (SrcVol *sv : group) rest.push_back({sv, longs[q].slot, longs[q].scal, longs[q].mult});
+        continue;
+      }
+      const int b = bx.n++;
+      bx.comp[b] = longs[q].slot;
+      for (int k = 0; k < 3; ++k) {
+        const int d = gv.ydir(k);
+        const bool has = gv.has(d);
+        bx.lo[b][k] = has ? m->blo[d] : 0;
+        bx.hi[b][k] = has ? m->bhi[d] : 0;
+        bx.bst[b][k] = has ? m->bst[d] : 0;
+      }
+      bx.amp[b] = static_cast<const double2 *>(m->amp.p);
+      bx.scal[b] = make_double2(longs[q].scal.real(), longs[q].scal.imag());
+      bx.mult[b] = longs[q].mult;
+      brec.push_back({sim, longs[q].sv->src, 1.0, 1.0, 0, toff, 1});
+      brec.push_back({sim, longs[q].sv->src, 1.0, 1.0, 1, toff, 1});
+    }
     if (rec) {
       tab.vbase = int(rec->size());
       rec->insert(rec->end(), trec.begin(), trec.end());
       bx.vbase = int(rec->size());
       rec->insert(rec->end(), brec.begin(), brec.end());
     }
+  }
+  // One dense box holding the summed amplitudes of several source lists (one source time,
+  // one component) over their joint index box, for batched steps.  A volume that crosses a
+  // periodic boundary arrives as several lists that meet at the cell's edge points; the
+  // batch subtracts their sum once where the per-chunk path subtracts list by list, which
+  // differs only in rounding at those points.  Null when the box would exceed 8x its points.
+  struct MergedBox {
+    int blo[3], bhi[3];
+    int64_t bst[3];
+    DevBuf amp;
+    long version = -1;
+    bool ok = false;
+  };
+  std::map<std::tuple<int, int, int>, std::unique_ptr<MergedBox>> merged; // (ft, src, component)
+  long src_version = 0; // bumped whenever a source list is added or changed
+  const MergedBox *merged_box(int ft2, const std::vector<SrcVol *> &group) {
+    auto &slot = merged[std::make_tuple(ft2, group[0]->src, group[0]->c)];
+    if (!slot) slot.reset(new MergedBox);
+    MergedBox &m = *slot;
+    if (m.version == src_version) return m.ok ? &m : nullptr;
+    m.version = src_version;
+    m.ok = false;
+    for (int d = 0; d < 3; ++d) {
+      m.blo[d] = INT32_MAX;
+      m.bhi[d] = INT32_MIN;
+    }
+    for (const SrcVol *sv : group)
+      for (size_t j = 0; j < sv->index.size(); ++j)
+        for (int d = 0; d < 3; ++d) {
+          m.blo[d] = std::min(m.blo[d], sv->jpos[3 * j + d]);
+          m.bhi[d] = std::max(m.bhi[d], sv->jpos[3 * j + d]);
+        }
+    size_t vol = 1;
+    for (int d = 2; d >= 0; --d) {
+      m.bst[d] = int64_t(vol);
+      vol *= size_t(m.bhi[d] - m.blo[d] + 1);
+    }
+    std::vector<double2> dense(vol, make_double2(0.0, 0.0));
+    std::vector<char> seen(vol, 0);
+    size_t distinct = 0;
+    for (const SrcVol *sv : group)
+      for (size_t j = 0; j < sv->index.size(); ++j) {
+        int64_t k = 0;
+        for (int d = 0; d < 3; ++d) k += int64_t(sv->jpos[3 * j + d] - m.blo[d]) * m.bst[d];
+        distinct += !seen[k];
+        seen[k] = 1;
+        dense[k].x += sv->amp[j].real();
+        dense[k].y += sv->amp[j].imag();
+      }
+    if (vol > 8 * distinct) return nullptr;
+    m.amp.alloc(sizeof(double2) * vol);
+    HIPCHECK(hipMemcpy(m.amp.p, dense.data(), sizeof(double2) * vol, hipMemcpyHostToDevice));
+    m.ok = true;
+    return &m;
   }
   // device copies of a long source list: indices, amplitudes, and the dense box when the
   // list fills at least 1/8 of its index box
@@ -2478,6 +2569,7 @@ template <class R> struct Fields : FieldsBase {
         if (i < 0 || size_t(i) >= gv.ntot) fail("internal: source point outside the array");
       const int ft = (is_magnetic(c) || is_B(c)) ? B_ft : D_ft;
       const int cc = (is_D(c) || is_B(c)) ? c - 10 : c;
+      ++src_version;
       for (SrcVol &o : sources[ft])
         if (o.c == cc && o.src == isrc && o.index == index) {
           for (size_t q = 0; q < o.amp.size(); ++q) o.amp[q] += amps[q];
@@ -2766,7 +2858,7 @@ template <class R> struct Batch : BatchBase {
   double last_ms = 0;
   // plan
   bool planned = false;
-  std::vector<long> plan_versions, plan_t;
+  std::vector<long> plan_versions, src_versions, plan_t;
   struct Group {
     int phase;
     int kind; // 0 generic, 1 vectorised, 2 vectorised with per-point uPML
@@ -2832,7 +2924,8 @@ template <class R> struct Batch : BatchBase {
   bool stale() const {
     if (!planned) return true;
     for (size_t q = 0; q < sims.size(); ++q)
-      if (sims[q]->plan_version != plan_versions[q] || !sims[q]->plan_valid) return true;
+      if (sims[q]->plan_version != plan_versions[q] || sims[q]->src_version != src_versions[q] || !sims[q]->plan_valid)
+        return true;
     return false;
   }
   // plan from the chunks' next step; returns the steps taken on the way (0 or 1)
@@ -2910,7 +3003,11 @@ template <class R> struct Batch : BatchBase {
       dft_nblk = dft_host.empty() ? 0 : unsigned(item.size());
     }
     plan_versions.clear();
-    for (Fields<R> *f : sims) plan_versions.push_back(f->plan_version);
+    src_versions.clear();
+    for (Fields<R> *f : sims) {
+      plan_versions.push_back(f->plan_version);
+      src_versions.push_back(f->src_version);
+    }
     planned = true;
   }
   void launch_step(long s, int bidx, const double *vrow) {

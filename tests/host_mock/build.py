@@ -1,0 +1,54 @@
+"""Build the host-logic test library: meep_amd/csrc/fields.cpp compiled with g++ against the
+stand-in runtime in tests/host_mock/hip/, with empty kernel launchers generated from the
+launcher declarations in meep_amd/csrc/engine.hpp.  The result,
+tests/host_mock/_build/libmeep_host.so, exports the same C-ABI as the HIP library; stepping it
+runs every host decision and argument check but computes no fields.  Test infrastructure only.
+"""
+
+import os
+import re
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(ROOT, "meep_amd", "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT, "libmeep_host.so")
+
+
+def launcher_stubs():
+    """C++ source defining every `template <class R> void launch_*(...)` declared in
+    engine.hpp with an empty body, instantiated for float and double."""
+    src = open(os.path.join(CSRC, "engine.hpp")).read()
+    decls = re.findall(r"template <class R>\s*void (launch_\w+)\(([^;]*?)\);", src, re.S)
+    if not decls:
+        raise RuntimeError("no launcher declarations found in engine.hpp")
+    out = ['#include "engine.hpp"', "namespace mx {"]
+    out += [f"template <class R> void {name}({args}) {{}}" for name, args in decls]
+    for name, args in decls:
+        for real in ("float", "double"):
+            typed = re.sub(r"\bR\b", real, args)
+            out.append(f"template void {name}<{real}>({typed});")
+    out.append("}")
+    return "\n".join(out) + "\n"
+
+
+def build(force=False):
+    sources = [os.path.join(CSRC, n) for n in ("fields.cpp", "engine.hpp", "grid.hpp", "stencil.hpp")]
+    sources.append(os.path.join(HERE, "hip", "hip_runtime.h"))
+    if (not force and os.path.exists(LIB)
+            and os.path.getmtime(LIB) >= max(os.path.getmtime(p) for p in sources if os.path.exists(p))):
+        return LIB
+    os.makedirs(OUT, exist_ok=True)
+    stubs = os.path.join(OUT, "launchers.cpp")
+    with open(stubs, "w") as fh:
+        fh.write(launcher_stubs())
+    tmp = LIB + ".tmp"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-I", HERE, "-I", CSRC,
+                           stubs, os.path.join(CSRC, "fields.cpp"), "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(force=True))

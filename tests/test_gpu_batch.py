@@ -59,7 +59,7 @@ def test_batch_matches_individual_runs(gpu_lib, single, graph):
     assert block in (64, 128, 256, 512) and ms > 0
     tol = 1e-5 if single else 1e-11
     for (s1, f1, fl1), (s2, f2, fl2) in zip(solo, together):
-        assert f1.t() == f2.t() == nsteps
+        assert f1.t == f2.t == nsteps
         a, c = state(f1, s1), state(f2, s2)
         assert set(a) == set(c)
         for key in a:
