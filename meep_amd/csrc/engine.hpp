@@ -13,7 +13,7 @@
 namespace mx {
 
 constexpr int MAXPOLE = 4;  // Lorentz/Drude poles per field type
-constexpr int SRC_TAB = 12; // sparse source entries carried in the kernel arguments
+constexpr int SRC_TAB = 16; // entries of a short source list table
 
 // Geometry of one chunk in the device layout, plus the box a launch covers.  Loop axis k
 // (0 slowest, 2 stride-1) walks direction axdir[k] (-1 when the grid has no such direction);
@@ -43,6 +43,9 @@ struct Grid {
   // point that wraps along several folded axes writes every combination.
   int wrap[3];
   int mhi_bits, mlo_bits; // bit k: loop axis k's High wall is Metallic / Low wall Magnetic
+  // indices along loop axis k where no uPML profile of that axis is active for either Yee
+  // shift (empty, qlo > qhi, when not known): the kernels skip the profile loads there
+  int qlo[3], qhi[3];
 };
 
 // Per-slot data by loop axis (derived by finalize_args): owned index range, and the index of
@@ -101,24 +104,24 @@ template <class R> struct PoleCoef {
   int lo[3], hi[3];
 };
 
-// Sparse per-step source values: f[idx] of component comp loses val (already the real or
-// imaginary part of amp * current * dt, or of amp * dipole), subtracted in double as the
-// reference does (src/step.cpp:310-315, src/update_eh.cpp:128-138).
+// Short source lists: f[idx[s]] of slot comp[s] loses the value vrow[vbase + s] of the step's
+// value row (the real or imaginary part of amp * current * dt, or of amp * dipole, formed on
+// the host in double), subtracted in double as the reference does (src/step.cpp:310-315,
+// src/update_eh.cpp:128-138).
 struct SrcTab {
   int n;
   int64_t lo, hi; // index range of the entries (cells outside it skip the table)
   int comp[SRC_TAB];
   int64_t idx[SRC_TAB];
-  double val[SRC_TAB];
-  int vbase; // batched steps: entry s takes its value from the step's value row at vbase + s
+  int vbase;
 };
 
 // Sources too long for a table, carried as dense index boxes: source s covers array indices
-// lo[s][k]..hi[s][k] along loop axis k of component comp[s], with one complex amplitude per
-// box point in amp[s] (box-local, strides bst[s], zero where the source has no point).  The value subtracted
-// from the field at a point is the real (cmp 0) or imaginary (cmp 1) part of
-// (amp * scal) * mult, formed in double as the reference forms sv.current(j) * dt
-// (src/step.cpp:307-315) and sv.dipole(j) (src/update_eh.cpp:128-138).
+// lo[s][k]..hi[s][k] along loop axis k of slot comp[s], with one complex amplitude per box
+// point in amp[s] (box-local, strides bst[s], zero where the source has no point).  The value
+// subtracted at a point is the real (cmp 0) or imaginary (cmp 1) part of (amp * scal) * mult,
+// formed in double as the reference forms sv.current(j) * dt (src/step.cpp:307-315) and
+// sv.dipole(j) (src/update_eh.cpp:128-138); scal is (vrow[vbase + 2 s], vrow[vbase + 2 s + 1]).
 constexpr int SRC_BOX = 2;
 struct SrcBoxes {
   int n;
@@ -126,9 +129,17 @@ struct SrcBoxes {
   int lo[SRC_BOX][3], hi[SRC_BOX][3];
   int64_t bst[SRC_BOX][3];
   const double2 *amp[SRC_BOX];
-  double2 scal[SRC_BOX];
   double mult[SRC_BOX];
-  int vbase; // batched steps: box s takes scal from the step's value row at vbase + 2 s
+  int vbase;
+};
+
+// The sources of one launch, in device memory: the kernels read them only in the few groups
+// whose indices fall in [StepArgs::slo, StepArgs::shi], so the tables cost the other groups
+// no registers.
+struct SrcSet {
+  SrcTab cur;          // currents, after the curl
+  SrcTab dip;          // integrated dipoles, inside f_minus_p
+  SrcBoxes curb, dipb; // the same for long source lists
 };
 
 template <class R> struct StepArgs {
@@ -139,9 +150,10 @@ template <class R> struct StepArgs {
   UpdComp<R> uc[3];
   int npole;
   PoleCoef<R> pole[MAXPOLE];
-  SrcTab cur; // currents, after the curl
-  SrcTab dip; // integrated dipoles, inside f_minus_p
-  SrcBoxes curb, dipb; // the same for long source lists
+  const SrcSet *src;   // null: no sources in this launch
+  int64_t slo, shi;    // index range that holds every source point of src
+  const double *vrow;  // this step's source values (batched launches pass theirs per step)
+  int64_t vofs;        // batched launches: this chunk's offset in the step's value row
   int cmp;             // 0: real parts, 1: imaginary parts of the fields this launch moves
   // Slots are by direction: cc[d] / uc[d] is the component along d (f == nullptr when
   // absent).  For the vectorised kernel the curl partners are given by direction too: F[d]

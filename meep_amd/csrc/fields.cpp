@@ -234,14 +234,10 @@ struct SrcVol { // src_vol (src/meep_internals.hpp:49-82)
   std::vector<int64_t> index;
   std::vector<cdouble> amp;
   std::vector<int> jpos; // array indices X, Y, Z of each point (0 for absent directions)
-  // device copies for lists longer than the kernel-argument table, made on first use and
-  // refreshed when the amplitudes change
+  // device copies for lists applied by k_src, made on first use and refreshed when the
+  // amplitudes change
   int64_t *d_idx = nullptr;
   double2 *d_amp = nullptr; // list order
-  double2 *d_box = nullptr; // dense over the index box blo..bhi (zero off the list)
-  int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
-  int64_t bst[3] = {0, 0, 0};
-  bool use_box = false;
   bool dirty = true;
   // folded periodic images of the list's points (point k, halo index, sign), for the plan
   // version img_version
@@ -277,11 +273,12 @@ static void ensure_gpu() {
     fail("no usable GPU: the MI355X engine has no CPU path (hipGetDeviceCount: %s)", hipGetErrorString(e));
 }
 
-// how a batched step computes one value of its source row: a table entry is the real (cmp 0)
+// how a step computes one value of its source row: a table entry (kind 0) is the real (cmp 0)
 // or imaginary part of (amp * s(t)) * mult, a box entry (kind 1) the part cmp of s(t), with
-// s the current (or dipole, for integrated sources) of source-time src at t * dt + toff * dt
+// s the current (or dipole, for integrated sources, toff 1) of source-time src at
+// t * dt + toff * dt
 struct SrcRecipe {
-  int sim, src;
+  int src;
   cdouble amp;
   double mult;
   int cmp;
@@ -389,6 +386,14 @@ template <class R> struct Fields : FieldsBase {
   // vectorised kernel on the interior box unless MEEP_AMD_GENERIC_KERNEL is set (both kernels
   // are held to the oracle by the tests)
   bool use_fast = getenv("MEEP_AMD_GENERIC_KERNEL") == nullptr;
+  // Whole rows: the launch boxes span whole rows, with the stride-1 direction's uPML taken
+  // point by point inside the interior box (rows_pml), instead of thin end slabs whose short
+  // row pieces coalesce poorly.  Measured on MI355X: faster for 3D cells (384x384x192 with
+  // 16-cell uPML: 2.05 against 2.38 ms per step), slower for 2D ones (a batch of 128-512
+  // cell squares with 10-cell uPML: 0.204 against 0.194 ms).  MEEP_AMD_WHOLE_ROWS=0 or 1
+  // overrides the choice.
+  int whole_rows_env = getenv("MEEP_AMD_WHOLE_ROWS") ? atoi(getenv("MEEP_AMD_WHOLE_ROWS")) : -1;
+  bool rows_pml = false;
   // kernel timing: event pairs recorded around launches, resolved after each synchronisation
   bool ktiming = false;
   std::vector<hipEvent_t> ev_pool;
@@ -556,7 +561,6 @@ template <class R> struct Fields : FieldsBase {
         for (auto &sv : sources[ft]) {
           if (sv.d_idx) (void)hipFree(sv.d_idx);
           if (sv.d_amp) (void)hipFree(sv.d_amp);
-          if (sv.d_box) (void)hipFree(sv.d_box);
           if (sv.d_img_k) (void)hipFree(sv.d_img_k);
           if (sv.d_img_dst) (void)hipFree(sv.d_img_dst);
           if (sv.d_img_sgn) (void)hipFree(sv.d_img_sgn);
@@ -869,19 +873,12 @@ template <class R> struct Fields : FieldsBase {
           if (p < a.npole && u.P[p] && !u.Pp[p]) fail("internal: polarisation without its previous value");
       }
     }
-    // source boxes: inside the arrays, with amplitudes, for a present slot
-    const SrcBoxes *lists[2] = {&a.curb, &a.dipb};
-    for (int l = 0; l < 2; ++l) {
-      const SrcBoxes &b = *lists[l];
-      if (b.n < 0 || b.n > SRC_BOX) fail("internal: bad source box count");
-      for (int q = 0; q < b.n; ++q) {
-        if (!b.amp[q] || b.comp[q] < 0 || b.comp[q] >= a.nc) fail("internal: bad source box %d", q);
-        for (int k = 0; k < 3; ++k) {
-          const int d = gv.ydir(k);
-          const int hi = gv.has(d) ? gv.num[d] : 0;
-          if (b.lo[q][k] < 0 || b.hi[q][k] > hi || b.lo[q][k] > b.hi[q][k]) fail("internal: source box outside the grid");
-        }
-      }
+    // sources: one of this chunk's device source sets (validated when built), with its rows
+    if (a.src) {
+      const SrcSet *base = splan.dev;
+      if (!base || a.src < base || a.src >= base + 4 || a.slo > a.shi || a.slo < 0 || a.shi >= int64_t(gv.ntot))
+        fail("internal: bad source set");
+      if (!collector && !a.vrow) fail("internal: source set without its value row");
     }
     // loop-axis view: launch box inside the grid, slots' ranges inside the arrays
     const Grid &g = a.g;
@@ -956,6 +953,8 @@ template <class R> struct Fields : FieldsBase {
       g.wrap[k] = d >= 0 ? wrap_dir[d] : 0;
       if (d >= 0 && ((g.hi_metal >> d) & 1)) g.mhi_bits |= 1 << k;
       if (d >= 0 && ((g.lo_magnetic >> d) & 1)) g.mlo_bits |= 1 << k;
+      g.qlo[k] = d >= 0 && boxes_valid ? quiet_lo[d] : 1;
+      g.qhi[k] = d >= 0 && boxes_valid ? quiet_hi[d] : 0;
     }
   }
   void finalize_args(StepArgs<R> &a) const {
@@ -1016,8 +1015,7 @@ template <class R> struct Fields : FieldsBase {
     std::map<int, std::vector<BatchItem<R>>> items[2]; // phase (B, D) -> variant -> items
     std::vector<BatchConn> conn[2];
     std::vector<BatchDft<R>> dft;
-    std::vector<SrcRecipe> *rec = nullptr;
-    int sim = 0;
+    int64_t vofs = 0; // this chunk's offset in the batch's source row
   };
   Collect *collector = nullptr;
   int cur_phase = 0;
@@ -1045,13 +1043,10 @@ template <class R> struct Fields : FieldsBase {
       cur_cmp = cmp;
       int which[3];
       A.nc = curl_comps(B_ft, cmp, A.cc, which);
-      std::vector<LongSrc> big;
       A.cmp = cmp;
-      fill_tab(B_ft, cmp, which, A.nc, false, 0.0, A.cur, A.curb, big, C.rec, C.sim, 0.0);
-      A.dip.n = 0;
-      A.dipb.n = 0;
       A.npole = 0;
-      if (!big.empty() || upd_pending(cmp, which, A.nc, false)) return false;
+      set_batch_sources(A, 0, cmp, C.vofs);
+      if (!splan.L[0][cmp].rest_cur.empty() || upd_pending(cmp, which, A.nc, false)) return false;
       const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
       step_kernel(A, true, nu > 0, MP_K_STEP_BH);
     }
@@ -1062,18 +1057,15 @@ template <class R> struct Fields : FieldsBase {
     cur_phase = 1;
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
       cur_cmp = cmp;
-      int which[3], whiche[3];
+      int which[3];
       A.nc = curl_comps(D_ft, cmp, A.cc, which);
-      std::vector<LongSrc> bigc, bigd;
       A.cmp = cmp;
-      fill_tab(D_ft, cmp, which, A.nc, false, 0.0, A.cur, A.curb, bigc, C.rec, C.sim, 0.5);
       upd_comps(E_ft, cmp, which, A.nc, A.uc, have_fmp_E);
       if (!plan_valid) return false;
-      for (int q = 0; q < A.nc; ++q) whiche[q] = which[q] < 0 ? -1 : which[q] - 10;
-      fill_tab(D_ft, cmp, whiche, A.nc, true, 0.0, A.dip, A.dipb, bigd, C.rec, C.sim, 1.0);
+      set_batch_sources(A, 1, cmp, C.vofs);
       A.npole = int(pols.size());
       for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
-      if (!bigc.empty() || !bigd.empty()) return false;
+      if (!splan.L[1][cmp].rest_cur.empty() || !splan.L[1][cmp].rest_dip.empty()) return false;
       step_kernel(A, true, true, MP_K_STEP_DE);
     }
     boundaries(1);
@@ -1126,7 +1118,7 @@ template <class R> struct Fields : FieldsBase {
       // backward differences read the previous plane: walk the planes downwards so it was
       // fetched just before (forward differences prefetch their neighbour naturally)
       b.rev = fast && kind == MP_K_STEP_DE ? 1 : 0;
-      launch_box(b, interior.lo, interior.hi, curl, upd, fast, interior_pml);
+      launch_box(b, interior.lo, interior.hi, curl, upd, fast, rows_pml);
     }
     b.rev = 0;
     for (const Box &bx : slabs) launch_box(b, bx.lo, bx.hi, curl, upd, fast, true);
@@ -1161,28 +1153,23 @@ template <class R> struct Fields : FieldsBase {
     int lo[3], hi[3];
   };
   bool boxes_valid = false;
+  int quiet_lo[3] = {1, 1, 1}, quiet_hi[3] = {0, 0, 0}; // per direction: no uPML profile active
   Box interior;
   bool interior_empty = false;
   std::vector<Box> slabs;
-  // The launch boxes: the grid is split along the directions other than the stride-1 one
-  // into an interior box, where no uPML profile of those directions is active, and slabs;
-  // every box spans whole rows, and the kernels take the uPML of the stride-1 direction (and
-  // of the slabs' directions) point by point.
-  bool interior_pml = false; // some uPML profile is active along the stride-1 direction
+  // The launch boxes: the grid is split along every direction with uPML into an interior
+  // box, where no uPML profile is active (the plain kernels), and slabs, where the kernels
+  // take the uPML point by point.  Boxes along the stride-1 direction start and end inside
+  // 16-byte groups; a group shared by two boxes is updated lane by lane by each.
   void compute_boxes() {
     int ilo[3], ihi[3], full_hi[3];
     interior_empty = false;
-    interior_pml = false;
+    rows_pml = false;
     for (int d = 0; d < 3; ++d) {
       full_hi[d] = gv.has(d) ? gv.num[d] : 0;
       ilo[d] = 0;
       ihi[d] = full_hi[d];
       if (!gv.has(d) || !s->pml_dir[d]) continue;
-      if (d == gv.fast) {
-        for (size_t k = 0; k < s->sig[d].size(); ++k)
-          if (s->sig[d][k] != 0.0 || s->kap[d][k] != 1.0) interior_pml = true;
-        continue;
-      }
       auto quiet = [&](int j) {
         for (int k = 2 * j; k <= 2 * j + 1; ++k)
           if (s->sig[d][k] != 0.0 || s->kap[d][k] != 1.0) return false;
@@ -1193,6 +1180,17 @@ template <class R> struct Fields : FieldsBase {
       for (int j = ilo[d]; j <= ihi[d]; ++j)
         if (!quiet(j)) fail("uPML profile active inside the interior of direction %d", d);
       if (ilo[d] > ihi[d]) interior_empty = true;
+    }
+    for (int d = 0; d < 3; ++d) {
+      quiet_lo[d] = ilo[d];
+      quiet_hi[d] = ihi[d];
+    }
+    const bool whole_rows = whole_rows_env >= 0 ? whole_rows_env != 0 : gv.dim == 3;
+    if (whole_rows) { // the stride-1 direction's uPML stays inside whole-row boxes
+      const int d = gv.fast;
+      rows_pml = gv.has(d) && (ilo[d] > 0 || ihi[d] < full_hi[d]);
+      ilo[d] = 0;
+      ihi[d] = full_hi[d];
     }
     slabs.clear();
     if (interior_empty) {
@@ -1380,80 +1378,203 @@ template <class R> struct Fields : FieldsBase {
       }
   }
 
-  // sparse source values of field type ft2 (D or B) at this step
-  // a source list too long for the argument table, with this step's scalar
+  // ---- source plan ----
+  // Every launch's sources sit in a device-resident SrcSet; the values they subtract come
+  // from a row of doubles per step, formed on the host in FP64 for all the steps of a call
+  // (SrcRecipe: amp * current * dt or amp * dipole at the step's time) and uploaded once.
+  // Long lists that fit no box are applied after the curl with k_src (LongSrc).
   struct LongSrc {
     SrcVol *sv;
     int slot;
-    cdouble scal;
     double mult;
+    double toff; // time of the list within the step, in units of dt
+    bool integrated;
   };
+  struct SrcLaunch { // the sources of one (phase, cmp) launch
+    SrcSet set;      // host copy (vbase relative to the chunk's row)
+    int64_t lo = INT64_MAX, hi = -1;
+    std::vector<LongSrc> rest_cur, rest_dip;
+  };
+  struct SrcPlan {
+    long pv = -1, sv = -1;
+    SrcLaunch L[2][2]; // [B phase, D phase][cmp]
+    const SrcSet *dev = nullptr; // the four SrcSets, index 2 * phase + cmp
+    std::vector<SrcRecipe> rec;  // layout of the per-step row
+  } splan;
+  // The SrcSets and the rows live in set_buf and rows_buf.  A buffer replaced while launches
+  // that read it may still be queued moves to `keep`, which step_n empties after it has
+  // synchronised the stream; rows_free says that no queued launch reads rows_buf.
+  std::unique_ptr<DevBuf> set_buf, rows_buf;
+  std::vector<std::unique_ptr<DevBuf>> keep;
+  bool rows_free = true;
+  long rows_t0 = 0, rows_n = 0, rows_hint = 1;
+
+  // the scalar of a long list at step tt: current(time) (times dt in mult) or dipole(time)
+  cdouble long_scal(const LongSrc &ls, long tt) const {
+    const SrcTime &st = srcs[ls.sv->src];
+    const double time = double(tt) * dt + ls.toff * dt;
+    return ls.integrated ? st.dipole(time) : st.current(time, dt);
+  }
+  // the step's source row value of recipe r at step tt
+  double recipe_value(const SrcRecipe &r, long tt) const {
+    const SrcTime &src = srcs[r.src];
+    const double time = double(tt) * dt + r.toff * dt;
+    const cdouble scal = r.toff == 1.0 ? src.dipole(time) : src.current(time, dt);
+    if (r.kind == 0) {
+      const cdouble A = (r.amp * scal) * r.mult;
+      return r.cmp ? A.imag() : A.real();
+    }
+    return r.cmp ? scal.imag() : scal.real();
+  }
+  // which[d]: the component of slot d of field type ft (-1 when absent)
+  void slot_comps(int ft, int cmp, int *which) const {
+    const int c0 = ft == B_ft ? 15 : 10;
+    for (int d = 0; d < 3; ++d) {
+      const int c = dir_comp(c0, d);
+      which[d] = has_field(c) && f[c][cmp] ? c : -1;
+    }
+  }
+  void build_srcplan() {
+    splan.rec.clear();
+    for (int ph = 0; ph < 2; ++ph)
+      for (int cmp = 0; cmp < 2; ++cmp) {
+        SrcLaunch &L = splan.L[ph][cmp];
+        memset(&L.set, 0, sizeof L.set);
+        L.lo = INT64_MAX;
+        L.hi = -1;
+        L.rest_cur.clear();
+        L.rest_dip.clear();
+        if (cmp >= ncmp()) continue;
+        int which[3], whiche[3];
+        slot_comps(ph ? D_ft : B_ft, cmp, which);
+        fill_tab(ph ? D_ft : B_ft, cmp, which, 3, false, L.set.cur, L.set.curb, L.rest_cur, ph ? 0.5 : 0.0);
+        if (ph) {
+          for (int q = 0; q < 3; ++q) whiche[q] = which[q] < 0 ? -1 : which[q] - 10;
+          fill_tab(D_ft, cmp, whiche, 3, true, L.set.dip, L.set.dipb, L.rest_dip, 1.0);
+        }
+        for (const SrcTab *tb : {&L.set.cur, &L.set.dip})
+          if (tb->n) {
+            L.lo = std::min(L.lo, tb->lo);
+            L.hi = std::max(L.hi, tb->hi);
+          }
+        for (const SrcBoxes *bx : {&L.set.curb, &L.set.dipb}) {
+          if (bx->n < 0 || bx->n > SRC_BOX) fail("internal: bad source box count");
+          for (int q = 0; q < bx->n; ++q) {
+            if (!bx->amp[q] || bx->comp[q] < 0 || bx->comp[q] >= 3 || which[bx->comp[q]] < 0) fail("internal: bad source box %d", q);
+            int64_t first = 0, last = 0;
+            for (int k = 0; k < 3; ++k) {
+              const int d = gv.ydir(k);
+              const int hi = gv.has(d) ? gv.num[d] : 0;
+              if (bx->lo[q][k] < 0 || bx->hi[q][k] > hi || bx->lo[q][k] > bx->hi[q][k]) fail("internal: source box outside the grid");
+              const int64_t st = gv.has(d) ? gv.st[d] : 0;
+              first += bx->lo[q][k] * st;
+              last += bx->hi[q][k] * st;
+            }
+            L.lo = std::min(L.lo, first);
+            L.hi = std::max(L.hi, last);
+          }
+        }
+      }
+    std::vector<SrcSet> sets(4);
+    for (int ph = 0; ph < 2; ++ph)
+      for (int cmp = 0; cmp < 2; ++cmp) sets[2 * ph + cmp] = splan.L[ph][cmp].set;
+    if (set_buf) keep.push_back(std::move(set_buf));
+    set_buf.reset(new DevBuf);
+    set_buf->alloc(sizeof(SrcSet) * 4);
+    HIPCHECK(hipMemcpy(set_buf->p, sets.data(), sizeof(SrcSet) * 4, hipMemcpyHostToDevice));
+    splan.dev = static_cast<const SrcSet *>(set_buf->p);
+    splan.pv = plan_version;
+    splan.sv = src_version;
+    rows_n = 0;
+  }
+  bool srcplan_stale() const { return splan.pv != plan_version || splan.sv != src_version; }
+  void ensure_srcplan() {
+    if (!plan_valid) rebuild_plan();
+    if (srcplan_stale()) build_srcplan();
+  }
+  // the source row of step tt (rows for rows_hint steps from tt are formed and uploaded when
+  // tt is not covered); null when no launch has sources
+  const double *src_row(long tt) {
+    if (srcplan_stale()) build_srcplan();
+    if (splan.rec.empty()) return nullptr;
+    const size_t rl = splan.rec.size();
+    if (tt < rows_t0 || tt >= rows_t0 + rows_n) {
+      const long n = std::max<long>(rows_hint, 1);
+      std::vector<double> rows(rl * size_t(n));
+      for (long st = 0; st < n; ++st)
+        for (size_t q = 0; q < rl; ++q) rows[size_t(st) * rl + q] = recipe_value(splan.rec[q], tt + st);
+      const size_t need = sizeof(double) * rows.size();
+      if (!rows_buf || !rows_free || rows_buf->bytes < need) {
+        if (rows_buf) keep.push_back(std::move(rows_buf));
+        rows_buf.reset(new DevBuf);
+        rows_buf->alloc(need);
+      }
+      HIPCHECK(hipMemcpy(rows_buf->p, rows.data(), need, hipMemcpyHostToDevice));
+      rows_free = false;
+      rows_t0 = tt;
+      rows_n = n;
+    }
+    return static_cast<const double *>(rows_buf->p) + size_t(tt - rows_t0) * rl;
+  }
+  // the sources of launch (phase, cmp) of a batched step (values from the batch's row at vofs)
+  void set_batch_sources(StepArgs<R> &A, int ph, int cmp, int64_t vofs) {
+    if (srcplan_stale()) build_srcplan();
+    const SrcLaunch &L = splan.L[ph][cmp];
+    const bool any = L.lo <= L.hi;
+    A.src = any ? splan.dev + (2 * ph + cmp) : nullptr;
+    A.slo = any ? L.lo : 0;
+    A.shi = any ? L.hi : -1;
+    A.vrow = nullptr;
+    A.vofs = vofs;
+  }
+  // the sources of launch (phase, cmp) at step tt into A
+  void set_sources(StepArgs<R> &A, int ph, int cmp, long tt) {
+    const double *row = src_row(tt);
+    const SrcLaunch &L = splan.L[ph][cmp];
+    const bool any = L.lo <= L.hi;
+    A.src = any ? splan.dev + (2 * ph + cmp) : nullptr;
+    A.slo = any ? L.lo : 0;
+    A.shi = any ? L.hi : -1;
+    A.vrow = any ? row : nullptr;
+    A.vofs = 0;
+  }
+
   // Sources of field type ft2 whose integrated flag is `integrated`, for the components in
-  // slots `which`: short lists go into the argument table, long ones into dense index boxes
-  // while they fit (SRC_BOX, and a box at most 8x the list), the rest into `rest`, which the
-  // caller applies with k_src.  Values are (amp * current(t)) * dt for currents
-  // (src/step.cpp:307-315) and amp * dipole(t) for integrated sources (src/update_eh.cpp:128-138).
-  // Batch planning (rec set) has no k_src pass: there the long lists of one source time and
-  // component share one merged box.
-  void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, double time, SrcTab &tab, SrcBoxes &bx,
-                std::vector<LongSrc> &rest, std::vector<SrcRecipe> *rec = nullptr, int sim = 0, double toff = 0) {
+  // slots `which`: short lists go into the table, long ones into dense index boxes while they
+  // fit (SRC_BOX, and a box at most 8x its points; the long lists of one source time and
+  // component share one box), the rest into `rest`, which the caller applies with k_src.
+  // Each table entry and box appends its recipes to the row layout.  Values are
+  // (amp * current(t)) * dt for currents (src/step.cpp:307-315) and amp * dipole(t) for
+  // integrated sources (src/update_eh.cpp:128-138), t = (step + toff) dt.
+  void fill_tab(int ft2, int cmp, const int *which, int nc, bool integrated, SrcTab &tab, SrcBoxes &bx,
+                std::vector<LongSrc> &rest, double toff) {
     tab.n = 0;
     tab.lo = INT64_MAX;
     tab.hi = -1;
     bx.n = 0;
-    // batched steps take every value from the step's row: the table's entries first, then
-    // two per box (scal), so the layout does not depend on the time
     std::vector<SrcRecipe> trec, brec;
-    std::vector<LongSrc> longs; // batch planning: long lists, merged below
+    std::vector<LongSrc> longs;
     for (SrcVol &sv : sources[ft2]) {
       const SrcTime &st = srcs[sv.src];
       if (st.is_integrated != integrated) continue;
-      const cdouble scal = integrated ? st.dipole(time) : st.current(time, dt);
       const double mult = integrated ? 1.0 : dt;
       const int c2 = dir_comp(ft2 == D_ft ? 10 : 15, comp_dir(sv.c));
       int slot = -1;
       for (int q = 0; q < nc; ++q)
         if (which[q] == (integrated ? sv.c : c2)) slot = q;
       if (slot < 0) continue;
-      if (!rec && scal == 0.0) continue; // exact zero contributes nothing (the reference subtracts 0)
       if (tab.n + int(sv.index.size()) <= SRC_TAB) {
         for (size_t j = 0; j < sv.index.size(); ++j) {
-          const cdouble A = (sv.amp[j] * scal) * mult;
           tab.comp[tab.n] = slot;
           tab.idx[tab.n] = sv.index[j];
-          tab.val[tab.n] = cmp ? A.imag() : A.real();
           tab.lo = std::min(tab.lo, sv.index[j]);
           tab.hi = std::max(tab.hi, sv.index[j]);
           ++tab.n;
-          if (rec) trec.push_back({sim, sv.src, sv.amp[j], mult, cmp, toff, 0});
+          trec.push_back({sv.src, sv.amp[j], mult, cmp, toff, 0});
         }
         continue;
       }
-      if (rec) {
-        longs.push_back({&sv, slot, scal, mult});
-        continue;
-      }
-      prepare_src(sv);
-      if (sv.use_box && bx.n < SRC_BOX) {
-        const int b = bx.n++;
-        bx.comp[b] = slot;
-        for (int k = 0; k < 3; ++k) { // by loop axis
-          const int d = gv.ydir(k);
-          const bool has = gv.has(d);
-          bx.lo[b][k] = has ? sv.blo[d] : 0;
-          bx.hi[b][k] = has ? sv.bhi[d] : 0;
-          bx.bst[b][k] = has ? sv.bst[d] : 0;
-        }
-        bx.amp[b] = sv.d_box;
-        bx.scal[b] = make_double2(scal.real(), scal.imag());
-        bx.mult[b] = mult;
-        if (rec) {
-          brec.push_back({sim, sv.src, 1.0, 1.0, 0, toff, 1});
-          brec.push_back({sim, sv.src, 1.0, 1.0, 1, toff, 1});
-        }
-        continue;
-      }
-      rest.push_back({&sv, slot, scal, mult});
+      longs.push_back({&sv, slot, mult, toff, integrated});
     }
     std::vector<char> taken(longs.size(), 0);
     for (size_t q = 0; q < longs.size(); ++q) {
@@ -1466,7 +1587,10 @@ template <class R> struct Fields : FieldsBase {
         }
       const MergedBox *m = merged_box(ft2, group);
       if (!m || bx.n >= SRC_BOX) {
-        for (SrcVol *sv : group) rest.push_back({sv, longs[q].slot, longs[q].scal, longs[q].mult});
+        for (SrcVol *sv : group) {
+          prepare_src(*sv);
+          rest.push_back({sv, longs[q].slot, longs[q].mult, toff, integrated});
+        }
         continue;
       }
       const int b = bx.n++;
@@ -1479,17 +1603,14 @@ template <class R> struct Fields : FieldsBase {
         bx.bst[b][k] = has ? m->bst[d] : 0;
       }
       bx.amp[b] = static_cast<const double2 *>(m->amp.p);
-      bx.scal[b] = make_double2(longs[q].scal.real(), longs[q].scal.imag());
       bx.mult[b] = longs[q].mult;
-      brec.push_back({sim, longs[q].sv->src, 1.0, 1.0, 0, toff, 1});
-      brec.push_back({sim, longs[q].sv->src, 1.0, 1.0, 1, toff, 1});
+      brec.push_back({longs[q].sv->src, 1.0, 1.0, 0, toff, 1});
+      brec.push_back({longs[q].sv->src, 1.0, 1.0, 1, toff, 1});
     }
-    if (rec) {
-      tab.vbase = int(rec->size());
-      rec->insert(rec->end(), trec.begin(), trec.end());
-      bx.vbase = int(rec->size());
-      rec->insert(rec->end(), brec.begin(), brec.end());
-    }
+    tab.vbase = int(splan.rec.size());
+    splan.rec.insert(splan.rec.end(), trec.begin(), trec.end());
+    bx.vbase = int(splan.rec.size());
+    splan.rec.insert(splan.rec.end(), brec.begin(), brec.end());
   }
   // One dense box holding the summed amplitudes of several source lists (one source time,
   // one component) over their joint index box, for batched steps.  A volume that crosses a
@@ -1545,8 +1666,7 @@ template <class R> struct Fields : FieldsBase {
     m.ok = true;
     return &m;
   }
-  // device copies of a long source list: indices, amplitudes, and the dense box when the
-  // list fills at least 1/8 of its index box
+  // device copies of a long source list: indices and amplitudes
   void prepare_src(SrcVol &sv) {
     if (!sv.dirty) return;
     const size_t n = sv.index.size();
@@ -1558,43 +1678,15 @@ template <class R> struct Fields : FieldsBase {
     for (size_t j = 0; j < n; ++j) amp[j] = make_double2(sv.amp[j].real(), sv.amp[j].imag());
     HIPCHECK(hipMemcpy(sv.d_idx, sv.index.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(sv.d_amp, amp.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
-    for (int d = 0; d < 3; ++d) {
-      sv.blo[d] = INT32_MAX;
-      sv.bhi[d] = INT32_MIN;
-    }
-    for (size_t j = 0; j < n; ++j)
-      for (int d = 0; d < 3; ++d) {
-        sv.blo[d] = std::min(sv.blo[d], sv.jpos[3 * j + d]);
-        sv.bhi[d] = std::max(sv.bhi[d], sv.jpos[3 * j + d]);
-      }
-    size_t vol = 1;
-    for (int d = 2; d >= 0; --d) {
-      sv.bst[d] = int64_t(vol);
-      vol *= size_t(sv.bhi[d] - sv.blo[d] + 1);
-    }
-    sv.use_box = vol <= 8 * n;
-    if (sv.use_box) {
-      std::vector<double2> dense(vol, make_double2(0.0, 0.0));
-      std::vector<char> seen(vol, 0);
-      for (size_t j = 0; j < n; ++j) {
-        int64_t k = 0;
-        for (int d = 0; d < 3; ++d) k += int64_t(sv.jpos[3 * j + d] - sv.blo[d]) * sv.bst[d];
-        if (seen[k]) fail("internal: repeated point in a source list");
-        seen[k] = 1;
-        dense[k] = amp[j];
-      }
-      if (sv.d_box) (void)hipFree(sv.d_box);
-      HIPCHECK(hipMalloc(&sv.d_box, sizeof(double2) * vol));
-      HIPCHECK(hipMemcpy(sv.d_box, dense.data(), sizeof(double2) * vol, hipMemcpyHostToDevice));
-    }
     sv.dirty = false;
   }
   // A long list applied after the curl kernel: the kernel has already written the folded
   // periodic images of these points, so they are written again from the updated values.
   // (Integrated lists restore D afterwards and need no images.)
   void apply_long(const LongSrc &ls, R *fbase, int cmp, bool images = true) {
+    const cdouble scal = long_scal(ls, t);
     kbegin(MP_K_SRC);
-    launch_src<R>(fbase, ls.sv->d_idx, ls.sv->d_amp, int(ls.sv->index.size()), make_double2(ls.scal.real(), ls.scal.imag()),
+    launch_src<R>(fbase, ls.sv->d_idx, ls.sv->d_amp, int(ls.sv->index.size()), make_double2(scal.real(), scal.imag()),
                   ls.mult, cmp, stream);
     if (images) {
       prepare_images(*ls.sv);
@@ -1659,7 +1751,6 @@ template <class R> struct Fields : FieldsBase {
 
   // One fields::step() (src/step.cpp:35-139); the finiteness probe is launched when probe.
   void one_step(bool probe) {
-    const double tB = t * dt, tH = t * dt + 0.5 * dt, tE = t * dt + dt;
     bool int_B = false, int_D = false;
     for (auto &sv : sources[B_ft])
       if (srcs[sv.src].is_integrated) int_B = true;
@@ -1674,47 +1765,46 @@ template <class R> struct Fields : FieldsBase {
 
     if (!plan_valid) rebuild_plan();
     StepArgs<R> A = base_args();
-    phase_B(A, tB);
+    phase_B(A);
     // --- D, then E = chi1inv (D - P - dipoles) and the polarisations ---
     alloc_pols();
     if (!plan_valid) rebuild_plan();
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
       cur_cmp = cmp;
-      int which[3], whiche[3];
+      int which[3];
       A.nc = curl_comps(D_ft, cmp, A.cc, which);
-      std::vector<LongSrc> bigc, bigd;
       A.cmp = cmp;
-      fill_tab(D_ft, cmp, which, A.nc, false, tH, A.cur, A.curb, bigc);
       upd_comps(E_ft, cmp, which, A.nc, A.uc, have_fmp_E);
-      for (int q = 0; q < A.nc; ++q) whiche[q] = which[q] < 0 ? -1 : which[q] - 10;
-      fill_tab(D_ft, cmp, whiche, A.nc, true, tE, A.dip, A.dipb, bigd);
+      if (!plan_valid) rebuild_plan();
+      set_sources(A, 1, cmp, t);
       A.npole = int(pols.size());
       for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
-      if (bigc.empty() && bigd.empty()) {
+      const SrcLaunch &L = splan.L[1][cmp];
+      if (L.rest_cur.empty() && L.rest_dip.empty()) {
         step_kernel(A, true, true, MP_K_STEP_DE);
         continue;
       }
-      step_kernel(A, true, false, MP_K_STEP_DE);
-      for (const LongSrc &ls : bigc) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp);
+      // long lists beyond the boxes: the curl (with the in-kernel sources) on its own, the
+      // lists, then the update without the currents
+      StepArgs<R> C = A;
+      step_kernel(C, true, false, MP_K_STEP_DE);
+      for (const LongSrc &ls : L.rest_cur) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp);
       // integrated lists beyond the boxes: f_minus_p formed in place in D around the update,
       // then those points of D restored.  Lists may share points (a periodic plane and its
       // image), so every list is saved before any is applied.
       size_t nsave = 0;
-      for (const LongSrc &ls : bigd) nsave += ls.sv->index.size();
+      for (const LongSrc &ls : L.rest_dip) nsave += ls.sv->index.size();
       if (nsave > d_scratch.bytes / sizeof(R)) d_scratch.alloc(sizeof(R) * nsave);
       size_t off = 0;
-      for (const LongSrc &ls : bigd) {
+      for (const LongSrc &ls : L.rest_dip) {
         R *D = f[dir_comp(10, comp_dir(ls.sv->c))][cmp];
         launch_gather<R>(static_cast<R *>(d_scratch.p) + off, D, ls.sv->d_idx, int(ls.sv->index.size()), stream);
         off += ls.sv->index.size();
       }
-      for (const LongSrc &ls : bigd) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp, false);
-      StepArgs<R> U = A;
-      U.cur.n = 0;
-      U.curb.n = 0;
-      step_kernel(U, false, true, MP_K_STEP_DE);
+      for (const LongSrc &ls : L.rest_dip) apply_long(ls, f[dir_comp(10, comp_dir(ls.sv->c))][cmp], cmp, false);
+      step_kernel(A, false, true, MP_K_STEP_DE);
       off = 0;
-      for (const LongSrc &ls : bigd) {
+      for (const LongSrc &ls : L.rest_dip) {
         R *D = f[dir_comp(10, comp_dir(ls.sv->c))][cmp];
         launch_scatter<R>(D, ls.sv->d_idx, static_cast<const R *>(d_scratch.p) + off, int(ls.sv->index.size()), stream);
         off += ls.sv->index.size();
@@ -1741,30 +1831,27 @@ template <class R> struct Fields : FieldsBase {
     }
     return A;
   }
-  // B = B - curl E (+ B sources at tB), then H = mu^-1 B where H is its own array, then the
-  // B and H halos: step_db(B_stuff), step_source(B_stuff), step_boundaries(B_stuff),
+  // B = B - curl E (+ B sources), then H = mu^-1 B where H is its own array, then the B and
+  // H halos: step_db(B_stuff), step_source(B_stuff), step_boundaries(B_stuff),
   // update_eh(H_stuff), step_boundaries(H_stuff) (src/step.cpp:64-80)
-  void phase_B(StepArgs<R> &A, double tB) {
+  void phase_B(StepArgs<R> &A) {
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
       cur_cmp = cmp;
       int which[3];
       A.nc = curl_comps(B_ft, cmp, A.cc, which);
-      std::vector<LongSrc> big;
       A.cmp = cmp;
-      fill_tab(B_ft, cmp, which, A.nc, false, tB, A.cur, A.curb, big);
-      A.dip.n = 0;
-      A.dipb.n = 0;
       A.npole = 0;
-      if (big.empty() && !upd_pending(cmp, which, A.nc, false)) {
+      set_sources(A, 0, cmp, t);
+      const SrcLaunch &L = splan.L[0][cmp];
+      if (L.rest_cur.empty() && !upd_pending(cmp, which, A.nc, false)) {
         const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
         step_kernel(A, true, nu > 0, MP_K_STEP_BH);
         continue;
       }
       step_kernel(A, true, false, MP_K_STEP_BH);
-      for (const LongSrc &ls : big) apply_long(ls, f[dir_comp(15, comp_dir(ls.sv->c))][cmp], cmp);
+      for (const LongSrc &ls : L.rest_cur) apply_long(ls, f[dir_comp(15, comp_dir(ls.sv->c))][cmp], cmp);
       StepArgs<R> U = A;
-      U.cur.n = 0;
-      U.curb.n = 0;
+      U.src = nullptr;
       if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) step_kernel(U, false, true, MP_K_STEP_BH);
     }
     if (!plan_valid) rebuild_plan();
@@ -1847,7 +1934,8 @@ template <class R> struct Fields : FieldsBase {
         }
     }
     StepArgs<R> A = base_args();
-    phase_B(A, t * dt);
+    rows_hint = 1;
+    phase_B(A);
     for (const Saved &sv : saved)
       if (sv.average) launch_average<R>(sv.p, static_cast<const R *>(sv.b->p), gv.ntot, stream);
     HIPCHECK(hipStreamSynchronize(stream));
@@ -2194,7 +2282,10 @@ template <class R> struct Fields : FieldsBase {
         A.nrec = first;
         dft_off = true;
         A.recording = true;
-        for (long q = first; q < last; ++q) one_step(false);
+        for (long q = first; q < last; ++q) {
+          rows_hint = last - q;
+          one_step(false);
+        }
         dft_off = false;
         seg_active = false;
         for (long n = last - 1; n >= first; --n)
@@ -2314,8 +2405,14 @@ template <class R> struct Fields : FieldsBase {
   void step_n(long n, bool) override {
     if (!plan_valid) rebuild_plan();
     auto t0 = std::chrono::steady_clock::now();
+    HIPCHECK(hipStreamSynchronize(stream));
+    keep.clear(); // nothing queued reads them any more
+    rows_free = true;
     // the reference checks after every step (src/step.cpp:137-138); a batch probes its last step
-    for (long q = 0; q < n; ++q) one_step(q == n - 1);
+    for (long q = 0; q < n; ++q) {
+      rows_hint = n - q;
+      one_step(q == n - 1);
+    }
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipGetLastError());
     timers[6] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2874,7 +2971,8 @@ template <class R> struct Batch : BatchBase {
   std::vector<BatchDft<R>> dft_host;
   DevBuf dft_items, dft_item_tab, dft_first_tab;
   unsigned dft_nblk = 0;
-  std::vector<SrcRecipe> rec;
+  std::vector<size_t> row_off; // each chunk's offset in the step's source row
+  size_t row_len = 0;
   DevBuf rows;
 
   explicit Batch(std::vector<Fields<R> *> s) : sims(std::move(s)) {
@@ -2932,13 +3030,21 @@ template <class R> struct Batch : BatchBase {
   long plan() {
     long taken = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
-      rec.clear();
+      // each chunk's source row sits at row_off[q] of the batch's row
+      row_off.assign(sims.size(), 0);
+      row_len = 0;
+      for (size_t q = 0; q < sims.size(); ++q) {
+        sims[q]->ensure_srcplan();
+        row_off[q] = row_len;
+        row_len += sims[q]->splan.rec.size();
+      }
       std::vector<typename Fields<R>::Collect> cs(sims.size());
       bool ok = true;
       for (size_t q = 0; q < sims.size(); ++q) {
-        cs[q].rec = &rec;
-        cs[q].sim = int(q);
+        cs[q].vofs = int64_t(row_off[q]);
+        const long pv = sims[q]->splan.pv;
         ok = sims[q]->collect_step(cs[q]) && ok;
+        if (sims[q]->srcplan_stale() || sims[q]->splan.pv != pv) ok = false;
       }
       if (!ok) {
         if (attempt) fail("batch: a chunk keeps allocating arrays or needs the unfused source path");
@@ -3033,25 +3139,14 @@ template <class R> struct Batch : BatchBase {
     for (Fields<R> *f : sims)
       if (!f->plan_valid) fail("internal: a chunk's plan changed during batch planning");
     // source rows for the m steps, and the DFT chunks' starting times
-    const size_t rl = std::max<size_t>(rec.size(), 1);
+    const size_t rl = std::max<size_t>(row_len, 1);
     std::vector<double> hrows(rl * size_t(m), 0.0);
-    for (long st = 0; st < m; ++st)
-      for (size_t q = 0; q < rec.size(); ++q) {
-        const SrcRecipe &r = rec[q];
-        Fields<R> *F = sims[r.sim];
-        const SrcTime &src = F->srcs[r.src];
-        const double time = double(F->t + st) * F->dt + r.toff * F->dt;
-        const bool integrated = r.toff == 1.0;
-        const cdouble scal = integrated ? src.dipole(time) : src.current(time, F->dt);
-        double v;
-        if (r.kind == 0) {
-          const cdouble A = (r.amp * scal) * r.mult;
-          v = r.cmp ? A.imag() : A.real();
-        }
-        else
-          v = r.cmp ? scal.imag() : scal.real();
-        hrows[size_t(st) * rl + q] = v;
-      }
+    for (size_t q = 0; q < sims.size(); ++q) {
+      Fields<R> *F = sims[q];
+      const std::vector<SrcRecipe> &rec = F->splan.rec;
+      for (long st = 0; st < m; ++st)
+        for (size_t e = 0; e < rec.size(); ++e) hrows[size_t(st) * rl + row_off[q] + e] = F->recipe_value(rec[e], F->t + st);
+    }
     upload(rows, hrows);
     {
       size_t k = 0;

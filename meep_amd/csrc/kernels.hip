@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 
 namespace mx {
 
@@ -70,14 +71,11 @@ template <class R> __device__ __forceinline__ void put(R *f, uint32_t i, R v, co
   }
 }
 
-__device__ __forceinline__ bool in_tab(const SrcTab &t, uint32_t i) {
-  return t.n && int64_t(i) >= t.lo && int64_t(i) <= t.hi;
-}
-// vrow: the step's row of source values (batched steps), null when the values are in t
+// vrow: the step's row of source values
 template <class R> __device__ __forceinline__ R apply_tab(const SrcTab &t, int c, uint32_t i, R v, const double *vrow) {
 #pragma unroll
   for (int s = 0; s < SRC_TAB; ++s)
-    if (s < t.n && t.comp[s] == c && t.idx[s] == int64_t(i)) v = R(double(v) - (vrow ? vrow[t.vbase + s] : t.val[s]));
+    if (s < t.n && t.comp[s] == c && t.idx[s] == int64_t(i)) v = R(double(v) - vrow[t.vbase + s]);
   return v;
 }
 
@@ -87,8 +85,9 @@ template <int S> __device__ __forceinline__ double box_value(const SrcBoxes &b, 
 #pragma clang fp contract(off)
   const int64_t k = int64_t(j0 - b.lo[S][0]) * b.bst[S][0] + int64_t(j1 - b.lo[S][1]) * b.bst[S][1] +
                     int64_t(j2 - b.lo[S][2]) * b.bst[S][2];
-  const double2 am = b.amp[S][k], sc = vrow ? make_double2(vrow[b.vbase + 2 * S], vrow[b.vbase + 2 * S + 1]) : b.scal[S];
-  const double re = am.x * sc.x - am.y * sc.y, im = am.x * sc.y + am.y * sc.x;
+  const double2 am = b.amp[S][k];
+  const double sx = vrow[b.vbase + 2 * S], sy = vrow[b.vbase + 2 * S + 1];
+  const double re = am.x * sx - am.y * sy, im = am.x * sy + am.y * sx;
   return cmp ? im * b.mult[S] : re * b.mult[S];
 }
 template <int S> __device__ __forceinline__ bool in_box(const SrcBoxes &b, int c, int j0, int j1, int j2) {
@@ -100,6 +99,17 @@ __device__ __forceinline__ R apply_boxes(const SrcBoxes &b, int cmp, int c, int 
   static_assert(SRC_BOX == 2, "one term per box");
   if (in_box<0>(b, c, j0, j1, j2)) v = R(double(v) - box_value<0>(b, cmp, j0, j1, j2, vrow));
   if (in_box<1>(b, c, j0, j1, j2)) v = R(double(v) - box_value<1>(b, cmp, j0, j1, j2, vrow));
+  return v;
+}
+// currents (dip false) or integrated dipoles (dip true) of slot C at point i: the source set
+// is read from device memory only for indices inside the launch's source range
+template <class R, bool DIP, int C>
+__device__ __forceinline__ R apply_sources(const StepArgs<R> &a, uint32_t i, int j0, int j1, int j2, R v, const double *vrow) {
+  const SrcSet &S = *a.src;
+  const SrcTab &t = DIP ? S.dip : S.cur;
+  const SrcBoxes &b = DIP ? S.dipb : S.curb;
+  if (t.n && int64_t(i) >= t.lo && int64_t(i) <= t.hi) v = apply_tab(t, C, i, v, vrow);
+  if (b.n) v = apply_boxes(b, a.cmp, C, j0, j1, j2, v, vrow);
   return v;
 }
 
@@ -132,8 +142,7 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
       k.fu[i] = fun;
       fv = k.siginvu[ku] * ((k.kapu[ku] - k.sigu[ku]) * fv + fun - fprev);
     }
-    if (in_tab(a.cur, i)) fv = apply_tab(a.cur, C, i, fv, vrow);
-    if (a.curb.n) fv = apply_boxes(a.curb, a.cmp, C, j0, j1, j2, fv, vrow);
+    if (a.src && int64_t(i) >= a.slo && int64_t(i) <= a.shi) fv = apply_sources<R, false, C>(a, i, j0, j1, j2, fv, vrow);
     if (slot_metal(k.ax, j0, j1, j2)) fv = R(0);
     put(k.f, i, fv, a.g, k.ax, j0, j1, j2);
   }
@@ -147,8 +156,7 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
                j1 <= a.pole[p].hi[1] && j2 >= a.pole[p].lo[2] && j2 <= a.pole[p].hi[2];
       if (pin[p]) gv -= u.P[p][i];
     }
-    if (in_tab(a.dip, i)) gv = apply_tab(a.dip, C, i, gv, vrow);
-    if (a.dipb.n) gv = apply_boxes(a.dipb, a.cmp, C, j0, j1, j2, gv, vrow);
+    if (a.src && int64_t(i) >= a.slo && int64_t(i) <= a.shi) gv = apply_sources<R, true, C>(a, i, j0, j1, j2, gv, vrow);
     R ev, wv;
     const int kw = u.ksigw >= 0 ? 2 * asel(j0, j1, j2, u.ksigw) + u.ysigw : 0;
     if (pml_on(u.sigw, u.kapw, kw)) {
@@ -215,7 +223,7 @@ template <class R, bool CURL, bool UPD>
 __global__ __launch_bounds__(BLOCK) void k_step(const StepArgs<R> a, uint32_t count) {
   const uint32_t t = xcd_block() * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
-  step_body<R, CURL, UPD>(a, t, nullptr);
+  step_body<R, CURL, UPD>(a, t, a.vrow);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -340,13 +348,15 @@ template <class R, int V> struct ProfV {
   unsigned on;
 };
 template <class R, int V>
-__device__ __forceinline__ void load_prof(ProfV<R, V> &p, const R *sig, const R *kap, const R *siginv, int ks, int y, int j0, int j1,
-                                          int z0, int nz) {
+__device__ __forceinline__ void load_prof(ProfV<R, V> &p, const Grid &g, const R *sig, const R *kap, const R *siginv, int ks, int y,
+                                          int j0, int j1, int z0) {
   p.on = 0;
 #pragma unroll
   for (int k = 0; k < V; ++k) p.sig[k] = R(0), p.kap[k] = R(1), p.sinv[k] = R(1);
   if (!sig || ks < 0) return;
   if (ks == 2) {
+    if (z0 >= g.qlo[2] && z0 + V - 1 <= g.qhi[2]) return; // the whole group is quiet
+    const int nz = g.nax[2];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
       const int idx = 2 * min(z0 + k, nz) + y; // points past the row's end are masked off
@@ -357,7 +367,9 @@ __device__ __forceinline__ void load_prof(ProfV<R, V> &p, const R *sig, const R 
     }
   }
   else {
-    const int idx = 2 * (ks == 0 ? j0 : j1) + y;
+    const int j = ks == 0 ? j0 : j1;
+    if (j >= (ks == 0 ? g.qlo[0] : g.qlo[1]) && j <= (ks == 0 ? g.qhi[0] : g.qhi[1])) return;
+    const int idx = 2 * j + y;
     const R sg = sig[idx], kp = kap[idx], si = siginv ? siginv[idx] : R(1);
 #pragma unroll
     for (int k = 0; k < V; ++k) p.sig[k] = sg, p.kap[k] = kp, p.sinv[k] = si;
@@ -365,7 +377,9 @@ __device__ __forceinline__ void load_prof(ProfV<R, V> &p, const R *sig, const R 
   }
 }
 
-template <class R, int V, bool CURL, bool UPD, bool PML, int C>
+// POLES: the D-phase update handles Lorentz/Drude poles (a separate instantiation without
+// them keeps the pole-free kernel's register use down)
+template <class R, int V, bool CURL, bool UPD, bool PML, bool POLES, int C>
 __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int z0, unsigned mask,
                                           const Vec<R, V> &F0, const Vec<R, V> &F1, const Vec<R, V> &F2, const double *vrow) {
   if (!mask) return;
@@ -407,8 +421,8 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
     else {
       // the four step_curl branches per point (src/step_generic.cpp:65-249)
       ProfV<R, V> P, U;
-      load_prof<R, V>(P, kc.sig, kc.kap, kc.siginv, kc.ksig, kc.ysig, j0, j1, z0, g.nax[2]);
-      load_prof<R, V>(U, kc.sigu, kc.kapu, kc.siginvu, kc.ksigu, kc.ysigu, j0, j1, z0, g.nax[2]);
+      load_prof<R, V>(P, g, kc.sig, kc.kap, kc.siginv, kc.ksig, kc.ysig, j0, j1, z0);
+      load_prof<R, V>(U, g, kc.sigu, kc.kapu, kc.siginvu, kc.ksigu, kc.ysigu, j0, j1, z0);
       const unsigned um = U.on & mask;
       Vec<R, V> fuv;
       if (um) fuv = vload<R, V>(kc.fu + i);
@@ -434,13 +448,9 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       }
       if (um) vstore<R, V>(kc.fu + i, fuv, um);
     }
-    if (a.cur.n && int64_t(i) + V - 1 >= a.cur.lo && int64_t(i) <= a.cur.hi) {
+    if (a.src && int64_t(i) + V - 1 >= a.slo && int64_t(i) <= a.shi) {
 #pragma unroll
-      for (int k = 0; k < V; ++k) fv.v[k] = apply_tab(a.cur, C, i + k, fv.v[k], vrow);
-    }
-    if (a.curb.n) {
-#pragma unroll
-      for (int k = 0; k < V; ++k) fv.v[k] = apply_boxes(a.curb, a.cmp, C, j0, j1, z0 + k, fv.v[k], vrow);
+      for (int k = 0; k < V; ++k) fv.v[k] = apply_sources<R, false, C>(a, i + k, j0, j1, z0 + k, fv.v[k], vrow);
     }
     if (metal) {
 #pragma unroll
@@ -456,7 +466,7 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
     bool pin[MAXPOLE]; // the group meets pole p's box
 #pragma unroll
     for (int p = 0; p < MAXPOLE; ++p) {
-      pin[p] = p < a.npole && u.P[p] && j0 >= a.pole[p].lo[0] && j0 <= a.pole[p].hi[0] && j1 >= a.pole[p].lo[1] &&
+      pin[p] = POLES && p < a.npole && u.P[p] && j0 >= a.pole[p].lo[0] && j0 <= a.pole[p].hi[0] && j1 >= a.pole[p].lo[1] &&
                j1 <= a.pole[p].hi[1] && z0 + V - 1 >= a.pole[p].lo[2] && z0 <= a.pole[p].hi[2];
       if (pin[p]) {
         Pc[p] = vload<R, V>(u.P[p] + i);
@@ -464,13 +474,9 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
         for (int k = 0; k < V; ++k) gv.v[k] -= Pc[p].v[k];
       }
     }
-    if (a.dip.n && int64_t(i) + V - 1 >= a.dip.lo && int64_t(i) <= a.dip.hi) {
+    if (a.src && int64_t(i) + V - 1 >= a.slo && int64_t(i) <= a.shi) {
 #pragma unroll
-      for (int k = 0; k < V; ++k) gv.v[k] = apply_tab(a.dip, C, i + k, gv.v[k], vrow);
-    }
-    if (a.dipb.n) {
-#pragma unroll
-      for (int k = 0; k < V; ++k) gv.v[k] = apply_boxes(a.dipb, a.cmp, C, j0, j1, z0 + k, gv.v[k], vrow);
+      for (int k = 0; k < V; ++k) gv.v[k] = apply_sources<R, true, C>(a, i + k, j0, j1, z0 + k, gv.v[k], vrow);
     }
     Vec<R, V> ev = gv;
     if (u.u) {
@@ -482,7 +488,7 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
     if (PML) {
       // step_update_EDHB's uPML branch per point (src/step_generic.cpp:682-701)
       ProfV<R, V> W;
-      load_prof<R, V>(W, u.sigw, u.kapw, nullptr, u.ksigw, u.ysigw, j0, j1, z0, g.nax[2]);
+      load_prof<R, V>(W, g, u.sigw, u.kapw, nullptr, u.ksigw, u.ysigw, j0, j1, z0);
       const unsigned wm = W.on & mask;
       if (wm) {
         Vec<R, V> fwv = vload<R, V>(u.fw + i);
@@ -499,7 +505,7 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
     }
 #pragma unroll
     for (int p = 0; p < MAXPOLE; ++p)
-      if (pin[p] && u.sigma[p]) {
+      if (POLES && pin[p] && u.sigma[p]) {
         const PoleCoef<R> &q = a.pole[p];
         const Vec<R, V> Pp = vload<R, V>(u.Pp[p] + i), sg = vload<R, V>(u.sigma[p] + i);
         Vec<R, V> Pn;
@@ -519,7 +525,7 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
 }
 
 // group t of the launch box of a (groups of V points along the rows, from group g0)
-template <class R, int V, bool CURL, bool UPD, bool PML>
+template <class R, int V, bool CURL, bool UPD, bool PML, bool POLES>
 __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int g0, int ngroups, const double *vrow) {
   const Grid &g = a.g;
   const uint32_t r = t / uint32_t(ngroups);
@@ -547,17 +553,17 @@ __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int 
     if (a.F[1]) F1 = vload<R, V>(a.F[1] + i);
     if (a.F[2]) F2 = vload<R, V>(a.F[2] + i);
   }
-  fast_slot<R, V, CURL, UPD, PML, 0>(a, i, j0, j1, z0, m0, F0, F1, F2, vrow);
-  fast_slot<R, V, CURL, UPD, PML, 1>(a, i, j0, j1, z0, m1, F0, F1, F2, vrow);
-  fast_slot<R, V, CURL, UPD, PML, 2>(a, i, j0, j1, z0, m2, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, POLES, 0>(a, i, j0, j1, z0, m0, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, POLES, 1>(a, i, j0, j1, z0, m1, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, POLES, 2>(a, i, j0, j1, z0, m2, F0, F1, F2, vrow);
 }
 
 
-template <class R, int V, bool CURL, bool UPD, bool PML>
+template <class R, int V, bool CURL, bool UPD, bool PML, bool POLES>
 __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t count, int g0, int ngroups) {
   const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
-  fast_body<R, V, CURL, UPD, PML>(a, t, g0, ngroups, nullptr);
+  fast_body<R, V, CURL, UPD, PML, POLES>(a, t, g0, ngroups, a.vrow);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -604,19 +610,17 @@ template <class R> __device__ __forceinline__ R avg_value(const R *F, int64_t id
   return R(w * double(F[idx]));
 }
 
-// DFT accumulation: one thread per accumulator (frequency fastest, so a wavefront writes a
-// contiguous run), grid-stride.  Each block first forms the phases exp(i omega t) * scale of
-// all frequencies once into LDS (src/dft.cpp:270-271); lists longer than the table form
-// them per accumulator.
+// DFT accumulation (src/dft.cpp:270-306).  Each block first forms the phases
+// exp(i omega t) * scale of all frequencies once into LDS (src/dft.cpp:270-271); lists longer
+// than the table form them per accumulator.
 constexpr int DFT_TAB = 512;
-// accumulator o of a DFT chunk with this frequency's phase exp(i omega t) * scale
-template <class R> __device__ __forceinline__ void dft_one(const DftArgs<R> &a, size_t o, double2 phv) {
-  const int nf = a.nfreq;
-  const int k = int(o / size_t(nf));
-  const int64_t idx = a.idx[k];
-  const double w = a.w[k];
-  const R f0 = avg_value(a.f0, idx, w, a.avg1, a.avg2);
-  const R f1 = a.f1 ? avg_value(a.f1, idx, w, a.avg1, a.avg2) : R(0);
+__device__ __forceinline__ double2 dft_phase(double omega, double time, double scale_re, double scale_im) {
+  double sn, cs;
+  sincos(omega * time, &sn, &cs);
+  return make_double2(cs * scale_re - sn * scale_im, cs * scale_im + sn * scale_re);
+}
+// accumulator o += phase * (f0 + i f1), in the chunk's accumulator form
+template <class R> __device__ __forceinline__ void dft_acc(const DftArgs<R> &a, size_t o, double2 phv, R f0, R f1) {
   const double pr = phv.x, pim = phv.y;
   if (a.acc_kind == 1) { // FP64 accumulators
     double2 *acc = reinterpret_cast<double2 *>(a.acc);
@@ -654,31 +658,78 @@ template <class R> __device__ __forceinline__ void dft_one(const DftArgs<R> &a, 
   acc[2 * o] += tr;
   acc[2 * o + 1] += ti;
 }
+// the averaged, weighted field value of point k (src/dft.cpp:278-294)
+template <class R> __device__ __forceinline__ void dft_point(const DftArgs<R> &a, int k, R &f0, R &f1) {
+  const int64_t idx = a.idx[k];
+  const double w = a.w[k];
+  f0 = avg_value(a.f0, idx, w, a.avg1, a.avg2);
+  f1 = a.f1 ? avg_value(a.f1, idx, w, a.avg1, a.avg2) : R(0);
+}
+// accumulator o of a chunk (one thread per accumulator: the batched kernel)
+template <class R> __device__ __forceinline__ void dft_one(const DftArgs<R> &a, size_t o, double2 phv) {
+  const int k = int(o / size_t(a.nfreq));
+  R f0, f1;
+  dft_point(a, k, f0, f1);
+  dft_acc(a, o, phv, f0, f1);
+}
 
+// One thread per point and DFT_Q consecutive frequencies: the point's field values are read
+// once, and for Kahan accumulators with nfreq a multiple of DFT_Q the sums and compensations
+// move as 16-byte vectors.  Grid-stride over (point, frequency quad).
+constexpr int DFT_Q = 4;
 template <class R> __global__ __launch_bounds__(BLOCK) void k_dft(const DftArgs<R> a) {
   __shared__ double2 ph[DFT_TAB];
   const int nf = a.nfreq;
   const bool tab = nf <= DFT_TAB;
-  if (tab) {
-    for (int q = threadIdx.x; q < nf; q += BLOCK) {
-      double sn, cs;
-      sincos(a.omega[q] * a.time, &sn, &cs);
-      ph[q] = make_double2(cs * a.scale_re - sn * a.scale_im, cs * a.scale_im + sn * a.scale_re);
-    }
-  }
+  if (tab)
+    for (int q = threadIdx.x; q < nf; q += BLOCK) ph[q] = dft_phase(a.omega[q], a.time, a.scale_re, a.scale_im);
   __syncthreads();
-  const size_t n = size_t(a.npts) * size_t(nf);
-  for (size_t o = size_t(blockIdx.x) * BLOCK + threadIdx.x; o < n; o += size_t(gridDim.x) * BLOCK) {
-    const int fi = int(o % size_t(nf));
-    double2 phv;
-    if (tab)
-      phv = ph[fi];
-    else {
-      double sn, cs;
-      sincos(a.omega[fi] * a.time, &sn, &cs);
-      phv = make_double2(cs * a.scale_re - sn * a.scale_im, cs * a.scale_im + sn * a.scale_re);
+  const uint32_t nq = uint32_t(nf + DFT_Q - 1) / DFT_Q;
+  const uint32_t n = uint32_t(a.npts) * nq;
+  const bool vec = a.acc_kind == 2 && tab && nf % DFT_Q == 0;
+  for (uint32_t o = blockIdx.x * BLOCK + threadIdx.x; o < n; o += gridDim.x * BLOCK) {
+    const uint32_t k = o / nq;
+    const int fb = int(o - k * nq) * DFT_Q;
+    R f0, f1;
+    dft_point(a, int(k), f0, f1);
+    const size_t o0 = size_t(k) * size_t(nf) + size_t(fb);
+    if (vec) {
+      // four complex<float> sums and compensations: 32 bytes each, 32-byte aligned
+      float4 *acc = reinterpret_cast<float4 *>(reinterpret_cast<float2 *>(a.acc) + o0);
+      float4 *cp = reinterpret_cast<float4 *>(reinterpret_cast<float2 *>(a.comp) + o0);
+      float4 s[2] = {acc[0], acc[1]}, c[2] = {cp[0], cp[1]};
+      float *sv = reinterpret_cast<float *>(s), *cv = reinterpret_cast<float *>(c);
+#pragma unroll
+      for (int j = 0; j < DFT_Q; ++j) {
+        const double2 phv = ph[fb + j];
+        const R rr = R(phv.x), ri = R(phv.y);
+        R tr, ti;
+        if (a.f1) {
+          tr = rr * f0 - ri * f1;
+          ti = rr * f1 + ri * f0;
+        }
+        else {
+          tr = f0 * rr;
+          ti = f0 * ri;
+        }
+        const float yr = float(tr) - cv[2 * j], yi = float(ti) - cv[2 * j + 1];
+        const float sr = sv[2 * j] + yr, si = sv[2 * j + 1] + yi;
+        cv[2 * j] = (sr - sv[2 * j]) - yr;
+        cv[2 * j + 1] = (si - sv[2 * j + 1]) - yi;
+        sv[2 * j] = sr;
+        sv[2 * j + 1] = si;
+      }
+      acc[0] = s[0];
+      acc[1] = s[1];
+      cp[0] = c[0];
+      cp[1] = c[1];
+      continue;
     }
-    dft_one<R>(a, o, phv);
+    const int cnt = min(DFT_Q, nf - fb);
+    for (int j = 0; j < cnt; ++j) {
+      const double2 phv = tab ? ph[fb + j] : dft_phase(a.omega[fb + j], a.time, a.scale_re, a.scale_im);
+      dft_acc(a, o0 + size_t(j), phv, f0, f1);
+    }
   }
 }
 
@@ -710,12 +761,17 @@ template <class R, bool PML> void launch_fast_t(const StepArgs<R> &a, bool curl,
   const size_t count = size_t(ngroups) * size_t(a.g.an[0]) * size_t(a.g.an[1]);
   const unsigned nb = nblocks8(count);
   const uint32_t c32 = uint32_t(count);
-  if (curl && upd)
-    hipLaunchKernelGGL((k_fast<R, V, true, true, PML>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+  const bool poles = a.npole > 0;
+  if (curl && upd && poles)
+    hipLaunchKernelGGL((k_fast<R, V, true, true, PML, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+  else if (curl && upd)
+    hipLaunchKernelGGL((k_fast<R, V, true, true, PML, false>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
   else if (curl)
-    hipLaunchKernelGGL((k_fast<R, V, true, false, PML>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+    hipLaunchKernelGGL((k_fast<R, V, true, false, PML, false>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+  else if (upd && poles)
+    hipLaunchKernelGGL((k_fast<R, V, false, true, PML, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
   else if (upd)
-    hipLaunchKernelGGL((k_fast<R, V, false, true, PML>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+    hipLaunchKernelGGL((k_fast<R, V, false, true, PML, false>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
 }
 template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, bool pml, hipStream_t s) {
   if (pml)
@@ -768,9 +824,10 @@ template <class R> void launch_scatter(R *f, const int64_t *idx, const R *src, i
 }
 
 template <class R> void launch_dft(const DftArgs<R> &a, hipStream_t s) {
-  const size_t n = size_t(a.npts) * a.nfreq;
+  const size_t n = size_t(a.npts) * size_t((a.nfreq + DFT_Q - 1) / DFT_Q);
+  if (n >= (size_t(1) << 31)) throw std::runtime_error("meep: DFT chunk too large for one launch");
   // enough blocks to fill the chip several times over; each then walks a few strides
-  const unsigned nb = std::min<unsigned>(nblocks(n), 8192u);
+  const unsigned nb = std::min<unsigned>(nblocks(n), 4096u);
   if (n > 0) hipLaunchKernelGGL(k_dft<R>, dim3(nb), dim3(BLOCK), 0, s, a);
 }
 
@@ -792,12 +849,13 @@ __global__ __launch_bounds__(BS) void k_step_batch(const BatchItem<R> *items, co
   const BatchItem<R> &it = items[k];
   const uint32_t t = blk_first[lb] + threadIdx.x;
   if (t >= it.count) return;
+  const double *row = vrow + it.a.vofs;
   if (KIND == 1)
-    fast_body<R, 16 / sizeof(R), CURL, UPD, false>(it.a, t, it.g0, it.ngroups, vrow);
+    fast_body<R, 16 / sizeof(R), CURL, UPD, false, true>(it.a, t, it.g0, it.ngroups, row);
   else if (KIND == 2)
-    fast_body<R, 16 / sizeof(R), CURL, UPD, true>(it.a, t, it.g0, it.ngroups, vrow);
+    fast_body<R, 16 / sizeof(R), CURL, UPD, true, true>(it.a, t, it.g0, it.ngroups, row);
   else
-    step_body<R, CURL, UPD>(it.a, t, vrow);
+    step_body<R, CURL, UPD>(it.a, t, row);
 }
 
 template <class R, int BS, int KIND>
@@ -843,11 +901,7 @@ __global__ __launch_bounds__(BLOCK) void k_dft_batch(const BatchDft<R> *items, c
   const double time = double(tt) * it.dt - (it.half ? 0.5 * it.dt : 0.0);
   const int nf = a.nfreq;
   if (nf > DFT_TAB) return; // the host keeps such chunks out of batches
-  for (int q = threadIdx.x; q < nf; q += BLOCK) {
-    double sn, cs;
-    sincos(a.omega[q] * time, &sn, &cs);
-    ph[q] = make_double2(cs * a.scale_re - sn * a.scale_im, cs * a.scale_im + sn * a.scale_re);
-  }
+  for (int q = threadIdx.x; q < nf; q += BLOCK) ph[q] = dft_phase(a.omega[q], time, a.scale_re, a.scale_im);
   __syncthreads();
   const size_t o = size_t(blk_first[blockIdx.x]) + threadIdx.x;
   if (o >= size_t(a.npts) * size_t(nf)) return;

@@ -33,20 +33,31 @@ def launcher_stubs():
     return "\n".join(out) + "\n"
 
 
+def _fresh(sources):
+    return os.path.exists(LIB) and os.path.getmtime(LIB) >= max(
+        os.path.getmtime(p) for p in sources if os.path.exists(p))
+
+
 def build(force=False):
+    """Build the library unless it is newer than its sources; safe under parallel test
+    workers (one builds, the others wait on the lock)."""
+    import fcntl
     sources = [os.path.join(CSRC, n) for n in ("fields.cpp", "engine.hpp", "grid.hpp", "stencil.hpp")]
     sources.append(os.path.join(HERE, "hip", "hip_runtime.h"))
-    if (not force and os.path.exists(LIB)
-            and os.path.getmtime(LIB) >= max(os.path.getmtime(p) for p in sources if os.path.exists(p))):
+    if not force and _fresh(sources):
         return LIB
     os.makedirs(OUT, exist_ok=True)
-    stubs = os.path.join(OUT, "launchers.cpp")
-    with open(stubs, "w") as fh:
-        fh.write(launcher_stubs())
-    tmp = LIB + ".tmp"
-    subprocess.check_call(["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-I", HERE, "-I", CSRC,
-                           stubs, os.path.join(CSRC, "fields.cpp"), "-o", tmp])
-    os.replace(tmp, LIB)
+    with open(os.path.join(OUT, ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        if not force and _fresh(sources):
+            return LIB
+        stubs = os.path.join(OUT, "launchers.cpp")
+        with open(stubs, "w") as fh:
+            fh.write(launcher_stubs())
+        tmp = LIB + ".tmp"
+        subprocess.check_call(["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-I", HERE, "-I", CSRC,
+                               stubs, os.path.join(CSRC, "fields.cpp"), "-o", tmp])
+        os.replace(tmp, LIB)
     return LIB
 
 

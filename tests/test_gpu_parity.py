@@ -31,15 +31,18 @@ def _run(lib, builder, nsteps, single):
     return s, f, fluxes
 
 
-@pytest.mark.parametrize("kernel", ["auto", "generic"])
+@pytest.mark.parametrize("kernel", ["whole_rows", "split_rows", "generic"])
 @pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
 @pytest.mark.parametrize("name", sorted(SCENARIOS))
 def test_state_parity(gpu_lib, oracle_lib, name, single, kernel, monkeypatch):
-    """kernel='generic' forces the one-point-per-thread kernel on the interior box too, where
-    the vectorised kernel would otherwise run, so both kernels are held to the oracle."""
+    """Both launch-box layouts of the vectorised kernels (uPML along the rows taken point by
+    point in whole-row boxes, or split off into end slabs), and kernel='generic', which forces
+    the one-point-per-thread kernel on the interior box too, are each held to the oracle."""
     builder, nsteps = SCENARIOS[name]
     if kernel == "generic":
         monkeypatch.setenv("MEEP_AMD_GENERIC_KERNEL", "1")
+    else:
+        monkeypatch.setenv("MEEP_AMD_WHOLE_ROWS", "1" if kernel == "whole_rows" else "0")
     print(f"[parity] {name} single={single} steps={nsteps} kernel={kernel}", flush=True)
     sg, fg, flg = _run(gpu_lib, builder, nsteps, single)
     so, fo, flo = _run(oracle_lib, builder, nsteps, single)

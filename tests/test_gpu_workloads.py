@@ -3,11 +3,17 @@
 * 2D straight waveguide (320x160 cells, PML, 8000 steps, 100-frequency flux line): the
   engine's flux spectrum against the oracle's, FP64 at 1e-10 and FP32 at 1e-4 of the
   spectrum's maximum (the FP32 oracle performs the same FP32 operations).
+* The 3D uPML + Lorentz-Drude + DFT configuration at a size the oracle steps in seconds
+  (48x48x32 cells, 6-cell uPML, 400 steps, 32 frequencies): FP32 fields with compensated FP32
+  DFT accumulators against the FP64 oracle, DFT fields and a z-flux spectrum within 1e-5 of
+  their maxima (the precision form's contract; about 2e-6 is typical).
 """
 
 import numpy as np
 import pytest
 
+import bench
+from meep_amd import capi
 from scenarios import straight_waveguide
 
 pytestmark = pytest.mark.gpu
@@ -28,3 +34,23 @@ def test_straight_waveguide_flux(gpu_lib, oracle_lib, single):
     err = np.abs(g - o).max() / scale
     print(f"[workload] flux max rel err {err:.2e}", flush=True)
     assert err <= (1e-4 if single else 1e-10)
+
+
+def test_pml_lorentz_dft_fp32_precision(gpu_lib, oracle_lib):
+    n, npml, steps = (48, 48, 32), 6, 400
+    print(f"[workload] 3D uPML Lorentz-Drude DFT precision {n} {steps} steps", flush=True)
+    out = []
+    for lib, single, kind in ((gpu_lib, True, capi.DFT_KAHAN32), (oracle_lib, False, capi.DFT_F64)):
+        s, f, df = bench.build_c3(lib, single, n=n, npml=npml, nfreq=32, dft_kind=kind)
+        L = [c / 10.0 for c in n]
+        zf = (n[2] // 2 + 8) / 10.0
+        fl = f.add_dft_flux(capi.Z, (0, 0, zf), (L[0], L[1], zf), capi.linspace(0.4, 1.0, 32),
+                            decimation_factor=1)
+        f.step_n(steps)
+        out.append(([df.dft(i) for i in range(df.num_chunks())], fl.flux()))
+    (gd, gf), (od, of) = out
+    assert len(gd) == len(od) == 3
+    dft_err = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(gd, od))
+    flux_err = np.abs(gf - of).max() / np.abs(of).max()
+    print(f"[workload] dft max rel err {dft_err:.2e}, flux max rel err {flux_err:.2e}", flush=True)
+    assert dft_err <= 1e-5 and flux_err <= 1e-5

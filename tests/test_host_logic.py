@@ -23,13 +23,15 @@ def host_lib():
     return capi.bind(ctypes.CDLL(build.build()))
 
 
-@pytest.mark.parametrize("kernel", ["auto", "generic"])
+@pytest.mark.parametrize("kernel", ["whole_rows", "split_rows", "generic"])
 @pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
 @pytest.mark.parametrize("name", sorted(SCENARIOS))
 def test_scenario_plans_validate(host_lib, name, single, kernel, monkeypatch):
     builder, nsteps = SCENARIOS[name]
     if kernel == "generic":
         monkeypatch.setenv("MEEP_AMD_GENERIC_KERNEL", "1")
+    else:
+        monkeypatch.setenv("MEEP_AMD_WHOLE_ROWS", "1" if kernel == "whole_rows" else "0")
     s, f, _ = builder(host_lib, single=single)
     n = min(nsteps, 12)
     f.step_n(n)
