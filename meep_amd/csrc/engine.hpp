@@ -212,7 +212,8 @@ template <class R> struct BatchDft {
   int decimation;
   int64_t t_start;
 };
-// kind: 0 generic kernel, 1 vectorised, 2 vectorised with per-point uPML
+// kind: 0 generic kernel, 1 vectorised, 2 vectorised with per-point uPML, 3 and 4 the same two
+// updating Lorentz/Drude poles
 template <class R>
 void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, int kind,
                        bool curl, bool upd, int block, const double *vrow, hipStream_t s);

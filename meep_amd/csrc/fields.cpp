@@ -1001,7 +1001,7 @@ template <class R> struct Fields : FieldsBase {
         it.g0 = it.ngroups = 0;
         it.count = uint32_t(b.g.an[0]) * uint32_t(b.g.an[1]) * uint32_t(b.g.an[2]);
       }
-      const int kind = fast ? (pml ? 2 : 1) : 0;
+      const int kind = fast ? (pml ? 2 : 1) + (upd && b.npole > 0 ? 2 : 0) : 0;
       collector->items[cur_phase][kind * 4 + (curl ? 2 : 0) + (upd ? 1 : 0)].push_back(it);
       return;
     }
@@ -3061,7 +3061,7 @@ template <class R> struct Batch : BatchBase {
   void build(std::vector<typename Fields<R>::Collect> &cs) {
     groups.clear();
     for (int phase = 0; phase < 2; ++phase)
-      for (int key = 0; key < 12; ++key) {
+      for (int key = 0; key < 20; ++key) {
         std::unique_ptr<Group> gp(new Group);
         Group &g = *gp;
         g.phase = phase;

@@ -377,33 +377,92 @@ __device__ __forceinline__ void load_prof(ProfV<R, V> &p, const Grid &g, const R
   }
 }
 
-// POLES: the D-phase update handles Lorentz/Drude poles (a separate instantiation without
-// them keeps the pole-free kernel's register use down)
-template <class R, int V, bool CURL, bool UPD, bool PML, bool POLES, int C>
+// lanes of a group outside the quiet range of loop axis ks (where a uPML profile of that axis
+// may be active); decided from the launch's quiet ranges, without loading the profile
+template <int V> __device__ __forceinline__ unsigned prof_lanes(const Grid &g, int ks, int j0, int j1, int z0) {
+  if (ks < 0) return 0u;
+  if (ks == 2) {
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+      if (z0 + k < g.qlo[2] || z0 + k > g.qhi[2]) m |= 1u << k;
+    return m;
+  }
+  const int j = ks == 0 ? j0 : j1;
+  return (j >= (ks == 0 ? g.qlo[0] : g.qlo[1]) && j <= (ks == 0 ? g.qhi[0] : g.qhi[1])) ? 0u : (1u << V) - 1;
+}
+
+// One slot (component) of a group: curl (B or D) and/or update (H or E, and the poles).
+// NP: the poles the update handles (0, 1, 2 or MAXPOLE; registers are allocated for NP).
+// Every array load of the slot is issued before its first store: the compiler may not move a
+// load above a store it cannot prove independent, so a thread would otherwise wait for memory
+// several times per slot.  Profile loads (cached 1-D arrays) stay with the arithmetic.
+template <class R, int V, bool CURL, bool UPD, bool PML, int NP, int C>
 __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int j0, int j1, int z0, unsigned mask,
                                           const Vec<R, V> &F0, const Vec<R, V> &F1, const Vec<R, V> &F2, const double *vrow) {
   if (!mask) return;
   const Grid &g = a.g;
   const int ybits = CURL ? a.cc[C].ax.ybits : a.uc[C].ax.ybits;
   const unsigned metal = group_metal<V>(g, ybits, j0, j1, z0);
-  Vec<R, V> fv;
+  const bool upd = UPD && a.uc[C].f;
+  // slot C = direction C: g1 = F[(C+2)%3] along C+1, g2 = F[(C+1)%3] along C+2
+  constexpr int dp = (C + 1) % 3, dm = (C + 2) % 3;
+  const Vec<R, V> &g1c = C == 0 ? F2 : (C == 1 ? F0 : F1);
+  const Vec<R, V> &g2c = C == 0 ? F1 : (C == 1 ? F2 : F0);
+
+  // ---- loads ----
+  Vec<R, V> fv, g1n, g2n, fuv;
+  unsigned upot = 0; // lanes where the f_u profile may be active
   if (CURL) {
     const CurlComp<R> &kc = a.cc[C];
-    // slot C = direction C: g1 = F[(C+2)%3] along C+1, g2 = F[(C+1)%3] along C+2
-    constexpr int dp = (C + 1) % 3, dm = (C + 2) % 3;
-    const R *g1 = a.F[(C + 2) % 3], *g2 = a.F[(C + 1) % 3];
-    const Vec<R, V> &g1c = C == 0 ? F2 : (C == 1 ? F0 : F1);
-    const Vec<R, V> &g2c = C == 0 ? F1 : (C == 1 ? F2 : F0);
+    if (a.hasp[C]) g1n = neighbour<R, V>(a.F[(C + 2) % 3], i, a.sgn * g.st[dp], g1c, mask);
+    if (a.hasm[C]) g2n = neighbour<R, V>(a.F[(C + 1) % 3], i, a.sgn * g.st[dm], g2c, mask);
+    fv = vload<R, V>(kc.f + i);
+    if (PML && kc.sigu) {
+      upot = prof_lanes<V>(g, kc.ksigu, j0, j1, z0) & mask;
+      if (upot) fuv = vload<R, V>(kc.fu + i);
+    }
+  }
+  Vec<R, V> gv, uu, Pc[NP > 0 ? NP : 1], Ppv[NP > 0 ? NP : 1], sgv[NP > 0 ? NP : 1], fwv, eold;
+  bool pin[NP > 0 ? NP : 1]; // the group meets pole p's box
+  unsigned wpot = 0;         // lanes where the f_w profile may be active
+  if (upd) {
+    const UpdComp<R> &u = a.uc[C];
+    if (!CURL) gv = vload<R, V>(u.g + i);
+    if (u.u) uu = vload<R, V>(u.u + i);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      pin[p] = p < a.npole && u.P[p] && j0 >= a.pole[p].lo[0] && j0 <= a.pole[p].hi[0] && j1 >= a.pole[p].lo[1] &&
+               j1 <= a.pole[p].hi[1] && z0 + V - 1 >= a.pole[p].lo[2] && z0 <= a.pole[p].hi[2];
+      if (pin[p]) {
+        Pc[p] = vload<R, V>(u.P[p] + i);
+        if (u.sigma[p]) {
+          Ppv[p] = vload<R, V>(u.Pp[p] + i);
+          sgv[p] = vload<R, V>(u.sigma[p] + i);
+        }
+      }
+    }
+    if (PML && u.sigw) {
+      wpot = prof_lanes<V>(g, u.ksigw, j0, j1, z0) & mask;
+      if (wpot) {
+        fwv = vload<R, V>(u.fw + i);
+        eold = vload<R, V>(u.f + i);
+      }
+    }
+  }
+
+  // ---- arithmetic ----
+  unsigned um = 0;
+  if (CURL) {
+    const CurlComp<R> &kc = a.cc[C];
     Vec<R, V> cu;
 #pragma unroll
     for (int k = 0; k < V; ++k) cu.v[k] = R(0);
     if (a.hasp[C]) {
-      const Vec<R, V> g1n = neighbour<R, V>(g1, i, a.sgn * g.st[dp], g1c, mask);
 #pragma unroll
       for (int k = 0; k < V; ++k) cu.v[k] = g1n.v[k] - g1c.v[k];
     }
     if (a.hasm[C]) {
-      const Vec<R, V> g2n = neighbour<R, V>(g2, i, a.sgn * g.st[dm], g2c, mask);
       if (a.hasp[C]) {
 #pragma unroll
         for (int k = 0; k < V; ++k) cu.v[k] = cu.v[k] + g2c.v[k] - g2n.v[k];
@@ -413,7 +472,6 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
         for (int k = 0; k < V; ++k) cu.v[k] = g2c.v[k] - g2n.v[k];
       }
     }
-    fv = vload<R, V>(kc.f + i);
     if (!PML) {
 #pragma unroll
       for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
@@ -423,12 +481,10 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       ProfV<R, V> P, U;
       load_prof<R, V>(P, g, kc.sig, kc.kap, kc.siginv, kc.ksig, kc.ysig, j0, j1, z0);
       load_prof<R, V>(U, g, kc.sigu, kc.kapu, kc.siginvu, kc.ksigu, kc.ysigu, j0, j1, z0);
-      const unsigned um = U.on & mask;
-      Vec<R, V> fuv;
-      if (um) fuv = vload<R, V>(kc.fu + i);
+      um = U.on & mask & upot;
 #pragma unroll
       for (int k = 0; k < V; ++k) {
-        const bool pml = (P.on >> k) & 1, pmlu = (U.on >> k) & 1;
+        const bool pml = (P.on >> k) & 1, pmlu = (um >> k) & 1;
         if (!pmlu) {
           if (!pml)
             fv.v[k] -= a.dtdx * cu.v[k];
@@ -446,7 +502,6 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
           fv.v[k] = U.sinv[k] * ((U.kap[k] - U.sig[k]) * fv.v[k] + fun - fprev);
         }
       }
-      if (um) vstore<R, V>(kc.fu + i, fuv, um);
     }
     if (a.src && int64_t(i) + V - 1 >= a.slo && int64_t(i) <= a.shi) {
 #pragma unroll
@@ -457,75 +512,78 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       for (int k = 0; k < V; ++k)
         if ((metal >> k) & 1) fv.v[k] = R(0);
     }
-    put_row<R, V>(kc.f, i, fv, mask, g, ybits, j0, j1, z0);
   }
-  if (UPD && a.uc[C].f) {
+  Vec<R, V> ev, Pn[NP > 0 ? NP : 1];
+  unsigned wm = 0;
+  if (upd) {
     const UpdComp<R> &u = a.uc[C];
-    Vec<R, V> gv = CURL ? fv : vload<R, V>(u.g + i);
-    Vec<R, V> Pc[MAXPOLE];
-    bool pin[MAXPOLE]; // the group meets pole p's box
+    if (CURL) gv = fv;
 #pragma unroll
-    for (int p = 0; p < MAXPOLE; ++p) {
-      pin[p] = POLES && p < a.npole && u.P[p] && j0 >= a.pole[p].lo[0] && j0 <= a.pole[p].hi[0] && j1 >= a.pole[p].lo[1] &&
-               j1 <= a.pole[p].hi[1] && z0 + V - 1 >= a.pole[p].lo[2] && z0 <= a.pole[p].hi[2];
+    for (int p = 0; p < NP; ++p)
       if (pin[p]) {
-        Pc[p] = vload<R, V>(u.P[p] + i);
 #pragma unroll
         for (int k = 0; k < V; ++k) gv.v[k] -= Pc[p].v[k];
       }
-    }
     if (a.src && int64_t(i) + V - 1 >= a.slo && int64_t(i) <= a.shi) {
 #pragma unroll
       for (int k = 0; k < V; ++k) gv.v[k] = apply_sources<R, true, C>(a, i + k, j0, j1, z0 + k, gv.v[k], vrow);
     }
-    Vec<R, V> ev = gv;
+    ev = gv;
     if (u.u) {
-      const Vec<R, V> uu = vload<R, V>(u.u + i);
 #pragma unroll
       for (int k = 0; k < V; ++k) ev.v[k] = gv.v[k] * uu.v[k];
     }
-    Vec<R, V> wv = ev; // W of the polarisations: E, or f_w in the uPML
-    if (PML) {
+    const Vec<R, V> wv = ev; // W of the polarisations: E, or f_w in the uPML
+    if (PML && wpot) {
       // step_update_EDHB's uPML branch per point (src/step_generic.cpp:682-701)
       ProfV<R, V> W;
       load_prof<R, V>(W, g, u.sigw, u.kapw, nullptr, u.ksigw, u.ysigw, j0, j1, z0);
-      const unsigned wm = W.on & mask;
-      if (wm) {
-        Vec<R, V> fwv = vload<R, V>(u.fw + i);
-        const Vec<R, V> eold = vload<R, V>(u.f + i);
+      wm = W.on & mask & wpot;
 #pragma unroll
-        for (int k = 0; k < V; ++k)
-          if ((W.on >> k) & 1) {
-            const R fwprev = fwv.v[k], fwn = ev.v[k];
-            fwv.v[k] = fwn;
-            ev.v[k] = eold.v[k] + ((W.kap[k] + W.sig[k]) * fwn - (W.kap[k] - W.sig[k]) * fwprev);
-          }
-        vstore<R, V>(u.fw + i, fwv, wm);
-      }
+      for (int k = 0; k < V; ++k)
+        if ((wm >> k) & 1) {
+          const R fwprev = fwv.v[k], fwn = ev.v[k];
+          fwv.v[k] = fwn;
+          ev.v[k] = eold.v[k] + ((W.kap[k] + W.sig[k]) * fwn - (W.kap[k] - W.sig[k]) * fwprev);
+        }
     }
 #pragma unroll
-    for (int p = 0; p < MAXPOLE; ++p)
-      if (POLES && pin[p] && u.sigma[p]) {
+    for (int p = 0; p < NP; ++p)
+      if (pin[p] && u.sigma[p]) {
         const PoleCoef<R> &q = a.pole[p];
-        const Vec<R, V> Pp = vload<R, V>(u.Pp[p] + i), sg = vload<R, V>(u.sigma[p] + i);
-        Vec<R, V> Pn;
 #pragma unroll
         for (int k = 0; k < V; ++k)
-          Pn.v[k] = q.gamma1inv * (Pc[p].v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Pp.v[k] + q.omega0dtsqr * (sg.v[k] * wv.v[k]));
-        put_row<R, V>(u.P[p], i, Pn, mask, g, ybits, j0, j1, z0);
-        vstore<R, V>(u.Pp[p] + i, Pc[p], mask);
+          Pn[p].v[k] = q.gamma1inv * (Pc[p].v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Ppv[p].v[k] +
+                                      q.omega0dtsqr * (sgv[p].v[k] * wv.v[k]));
       }
     if (metal) {
 #pragma unroll
       for (int k = 0; k < V; ++k)
         if ((metal >> k) & 1) ev.v[k] = R(0);
     }
+  }
+
+  // ---- stores ----
+  if (CURL) {
+    const CurlComp<R> &kc = a.cc[C];
+    if (PML && um) vstore<R, V>(kc.fu + i, fuv, um);
+    put_row<R, V>(kc.f, i, fv, mask, g, ybits, j0, j1, z0);
+  }
+  if (upd) {
+    const UpdComp<R> &u = a.uc[C];
+    if (PML && wm) vstore<R, V>(u.fw + i, fwv, wm);
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (pin[p] && u.sigma[p]) {
+        put_row<R, V>(u.P[p], i, Pn[p], mask, g, ybits, j0, j1, z0);
+        vstore<R, V>(u.Pp[p] + i, Pc[p], mask);
+      }
     put_row<R, V>(u.f, i, ev, mask, g, ybits, j0, j1, z0);
   }
 }
 
 // group t of the launch box of a (groups of V points along the rows, from group g0)
-template <class R, int V, bool CURL, bool UPD, bool PML, bool POLES>
+template <class R, int V, bool CURL, bool UPD, bool PML, int NP>
 __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int g0, int ngroups, const double *vrow) {
   const Grid &g = a.g;
   const uint32_t r = t / uint32_t(ngroups);
@@ -553,17 +611,17 @@ __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int 
     if (a.F[1]) F1 = vload<R, V>(a.F[1] + i);
     if (a.F[2]) F2 = vload<R, V>(a.F[2] + i);
   }
-  fast_slot<R, V, CURL, UPD, PML, POLES, 0>(a, i, j0, j1, z0, m0, F0, F1, F2, vrow);
-  fast_slot<R, V, CURL, UPD, PML, POLES, 1>(a, i, j0, j1, z0, m1, F0, F1, F2, vrow);
-  fast_slot<R, V, CURL, UPD, PML, POLES, 2>(a, i, j0, j1, z0, m2, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, NP, 0>(a, i, j0, j1, z0, m0, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, NP, 1>(a, i, j0, j1, z0, m1, F0, F1, F2, vrow);
+  fast_slot<R, V, CURL, UPD, PML, NP, 2>(a, i, j0, j1, z0, m2, F0, F1, F2, vrow);
 }
 
 
-template <class R, int V, bool CURL, bool UPD, bool PML, bool POLES>
+template <class R, int V, bool CURL, bool UPD, bool PML, int NP>
 __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t count, int g0, int ngroups) {
   const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
-  fast_body<R, V, CURL, UPD, PML, POLES>(a, t, g0, ngroups, a.vrow);
+  fast_body<R, V, CURL, UPD, PML, NP>(a, t, g0, ngroups, a.vrow);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -754,24 +812,33 @@ template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, h
     hipLaunchKernelGGL((k_step<R, false, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32);
 }
 
+template <class R, bool PML, int NP>
+void launch_fast_np(const StepArgs<R> &a, bool curl, bool upd, unsigned nb, uint32_t c32, int g0, int ngroups, hipStream_t s) {
+  constexpr int V = 16 / sizeof(R);
+  if (curl && upd)
+    hipLaunchKernelGGL((k_fast<R, V, true, true, PML, NP>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+  else if (upd)
+    hipLaunchKernelGGL((k_fast<R, V, false, true, PML, NP>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+}
 template <class R, bool PML> void launch_fast_t(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
   constexpr int V = 16 / sizeof(R);
   const int g0 = a.g.alo[2] / V, g1 = (a.g.alo[2] + a.g.an[2] - 1) / V;
   const int ngroups = g1 - g0 + 1;
   const size_t count = size_t(ngroups) * size_t(a.g.an[0]) * size_t(a.g.an[1]);
+  if (count >= (size_t(1) << 32)) throw std::runtime_error("meep: launch box too large for one launch");
   const unsigned nb = nblocks8(count);
   const uint32_t c32 = uint32_t(count);
-  const bool poles = a.npole > 0;
-  if (curl && upd && poles)
-    hipLaunchKernelGGL((k_fast<R, V, true, true, PML, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
-  else if (curl && upd)
-    hipLaunchKernelGGL((k_fast<R, V, true, true, PML, false>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
-  else if (curl)
-    hipLaunchKernelGGL((k_fast<R, V, true, false, PML, false>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
-  else if (upd && poles)
-    hipLaunchKernelGGL((k_fast<R, V, false, true, PML, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
-  else if (upd)
-    hipLaunchKernelGGL((k_fast<R, V, false, true, PML, false>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+  if (curl && !upd) {
+    hipLaunchKernelGGL((k_fast<R, V, true, false, PML, 0>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+    return;
+  }
+  // registers for as many poles as the update handles
+  switch (a.npole) {
+    case 0: launch_fast_np<R, PML, 0>(a, curl, upd, nb, c32, g0, ngroups, s); break;
+    case 1: launch_fast_np<R, PML, 1>(a, curl, upd, nb, c32, g0, ngroups, s); break;
+    case 2: launch_fast_np<R, PML, 2>(a, curl, upd, nb, c32, g0, ngroups, s); break;
+    default: launch_fast_np<R, PML, MAXPOLE>(a, curl, upd, nb, c32, g0, ngroups, s); break;
+  }
 }
 template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, bool pml, hipStream_t s) {
   if (pml)
@@ -839,7 +906,8 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
 // ---- batched launches -----------------------------------------------------------------
 // Logical blocks are dealt to XCDs in contiguous runs as in the single-chunk kernels (nblk is a
 // multiple of 8); each logical block belongs to one item.
-// KIND: 0 generic, 1 vectorised, 2 vectorised with per-point uPML
+// KIND: 0 generic, 1 vectorised, 2 vectorised with per-point uPML, 3 and 4 the same two with
+// the Lorentz/Drude poles
 template <class R, int BS, int KIND, bool CURL, bool UPD>
 __global__ __launch_bounds__(BS) void k_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first,
                                                    const double *vrow) {
@@ -851,9 +919,13 @@ __global__ __launch_bounds__(BS) void k_step_batch(const BatchItem<R> *items, co
   if (t >= it.count) return;
   const double *row = vrow + it.a.vofs;
   if (KIND == 1)
-    fast_body<R, 16 / sizeof(R), CURL, UPD, false, true>(it.a, t, it.g0, it.ngroups, row);
+    fast_body<R, 16 / sizeof(R), CURL, UPD, false, 0>(it.a, t, it.g0, it.ngroups, row);
   else if (KIND == 2)
-    fast_body<R, 16 / sizeof(R), CURL, UPD, true, true>(it.a, t, it.g0, it.ngroups, row);
+    fast_body<R, 16 / sizeof(R), CURL, UPD, true, 0>(it.a, t, it.g0, it.ngroups, row);
+  else if (KIND == 3)
+    fast_body<R, 16 / sizeof(R), CURL, UPD, false, MAXPOLE>(it.a, t, it.g0, it.ngroups, row);
+  else if (KIND == 4)
+    fast_body<R, 16 / sizeof(R), CURL, UPD, true, MAXPOLE>(it.a, t, it.g0, it.ngroups, row);
   else
     step_body<R, CURL, UPD>(it.a, t, row);
 }
@@ -872,7 +944,11 @@ void launch_step_batch_k(const BatchItem<R> *items, const uint32_t *blk_item, co
 template <class R, int BS>
 void launch_step_batch_bs(const BatchItem<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, int kind,
                           bool curl, bool upd, const double *vrow, hipStream_t s) {
-  if (kind == 2)
+  if (kind == 4)
+    launch_step_batch_k<R, BS, 4>(items, blk_item, blk_first, nblk, curl, upd, vrow, s);
+  else if (kind == 3)
+    launch_step_batch_k<R, BS, 3>(items, blk_item, blk_first, nblk, curl, upd, vrow, s);
+  else if (kind == 2)
     launch_step_batch_k<R, BS, 2>(items, blk_item, blk_first, nblk, curl, upd, vrow, s);
   else if (kind == 1)
     launch_step_batch_k<R, BS, 1>(items, blk_item, blk_first, nblk, curl, upd, vrow, s);
