@@ -77,13 +77,13 @@ def build(lib, single, eps_b):
 
 
 # ---- the 3D PML + Lorentz-Drude + DFT workload -----------------------------------------
-C3_CELLS = (384, 384, 192)
-C3_PML = 16          # cells of uPML on every face
-C3_A = 10.0          # resolution
-C3_NFREQ = 32
+PLD_CELLS = (384, 384, 192)
+PLD_PML = 16          # cells of uPML on every face
+PLD_A = 10.0          # resolution
+PLD_NFREQ = 32
 
 
-def c3_materials(n=C3_CELLS, seed=3, nspheres=150):
+def pml_lorentz_dft_materials(n=PLD_CELLS, seed=3, nspheres=150):
     """eps (per E component, reference layout) and the dispersive mask for the workload:
     lower half (z < n_z/2 cells) eps_inf = 2 with the two poles; upper half vacuum with seeded
     dielectric spheres (radius 4-12 cells, eps 2-6), sampled at each component's Yee point."""
@@ -115,7 +115,7 @@ def c3_materials(n=C3_CELLS, seed=3, nspheres=150):
     return eps, lower
 
 
-def build_c3(lib, single, n=C3_CELLS, npml=C3_PML, a=C3_A, nfreq=C3_NFREQ, dft_kind=None,
+def build_pml_lorentz_dft(lib, single, n=PLD_CELLS, npml=PLD_PML, a=PLD_A, nfreq=PLD_NFREQ, dft_kind=None,
              fcen=0.7, df=0.6):
     """3D cell of n cells with npml cells of uPML on every face; lower half a 2-pole
     Lorentz-Drude medium (Lorentz f0 = 1.0, gamma 0.05, sigma 1.5; Drude f = 1.0, gamma 0.1,
@@ -125,7 +125,7 @@ def build_c3(lib, single, n=C3_CELLS, npml=C3_PML, a=C3_A, nfreq=C3_NFREQ, dft_k
     nfreq frequencies across the pulse, decimation 1."""
     L = [c / a for c in n]
     s = capi.structure(lib, dim=3, size=L, a=a, eps=None, pml=capi.pml(npml / a), single=single)
-    eps, lower = c3_materials(n)
+    eps, lower = pml_lorentz_dft_materials(n)
     for c in (capi.Ex, capi.Ey, capi.Ez):
         s.set_chi1inv(c, 1.0 / eps[c])
     del eps
@@ -148,10 +148,11 @@ def build_c3(lib, single, n=C3_CELLS, npml=C3_PML, a=C3_A, nfreq=C3_NFREQ, dft_k
     return s, f, dft
 
 
-def c3_algorithmic_bytes(n=C3_CELLS, npml=C3_PML, word=4, nfreq=C3_NFREQ, acc_bytes=16):
+def pml_lorentz_dft_bytes(n=PLD_CELLS, npml=PLD_PML, word=4, nfreq=PLD_NFREQ, acc_bytes=16, dft_points=None):
     """Reference-state bytes per step: 21 words per cell, +10 per PML face cell, +20 per edge
     cell, +30 per corner cell, +15 per pole over dispersive cells, plus each DFT accumulator
-    read and written once."""
+    read and written once (dft_points: the monitor's points over its chunks; a plane between
+    grid planes has two planes of points for the interpolation)."""
     cnt = np.zeros(4)
     regions = []
     for d in range(3):
@@ -165,19 +166,20 @@ def c3_algorithmic_bytes(n=C3_CELLS, npml=C3_PML, word=4, nfreq=C3_NFREQ, acc_by
     cells = np.prod(n)
     disp = n[0] * n[1] * (n[2] // 2)
     words = 21 * cells + 10 * cnt[1] + 20 * cnt[2] + 30 * cnt[3] + 2 * 15 * disp
-    dft = 3 * (n[0] + 1) * (n[1] + 1) * nfreq * 2 * acc_bytes
+    npts = sum(dft_points) if dft_points is not None else 3 * (n[0] + 1) * (n[1] + 1)
+    dft = npts * nfreq * 2 * acc_bytes
     return words * word + dft
 
 
 # ---- the 2D differentiation workload -----------------------------------------------------
-C4_A = 50            # resolution: 600 x 400 cells in 12 x 8
-C4_SIZE = (12.0, 8.0)
-C4_DPML = 1.0
-C4_STEPS = 4000
-C4_DESIGN = ((4.0, 2.0, 0.0), (7.98, 5.98, 0.0))  # 200 x 200 Ez points
+ADJ_A = 50            # resolution: 600 x 400 cells in 12 x 8
+ADJ_SIZE = (12.0, 8.0)
+ADJ_DPML = 1.0
+ADJ_STEPS = 4000
+ADJ_DESIGN = ((4.0, 2.0, 0.0), (7.98, 5.98, 0.0))  # 200 x 200 Ez points
 
 
-def c4_rho(seed=4, n=200, blur=5.0):
+def adjoint2d_rho(seed=4, n=200, blur=5.0):
     """Seeded uniform density on the 200 x 200 design pixels, Gaussian-blurred (sigma 5 px,
     periodic wrap, by FFT)."""
     rng = np.random.default_rng(seed)
@@ -187,19 +189,19 @@ def c4_rho(seed=4, n=200, blur=5.0):
     return np.real(np.fft.ifft2(np.fft.fft2(r) * g))
 
 
-def build_c4(lib, single, chi_ez=None, rho=None):
+def build_adjoint2d(lib, single, chi_ez=None, rho=None):
     """TM fields in 12 x 8 at resolution 50 with PML 1; eps = 1 + 11 rho on the 200 x 200
     design points (no averaging) and 1 elsewhere; an Ez line source at x = 2 with a Gaussian
     amplitude profile along y (sigma 1) and a Gaussian pulse (fcen 1, df 0.4); the objective
     monitor: DFT Ez on a 40-cell line at x = 10 centred on y = 4, 10 frequencies in
     [0.8, 1.2], decimation 1."""
-    s = capi.structure(lib, dim=2, size=C4_SIZE, a=C4_A, eps=None, pml=capi.pml(C4_DPML),
+    s = capi.structure(lib, dim=2, size=ADJ_SIZE, a=ADJ_A, eps=None, pml=capi.pml(ADJ_DPML),
                        single=single)
     if chi_ez is None:
-        rho = c4_rho() if rho is None else rho
+        rho = adjoint2d_rho() if rho is None else rho
         n = s.shape
         chi = np.ones(n)
-        x0, y0 = int(round(C4_DESIGN[0][0] * C4_A)), int(round(C4_DESIGN[0][1] * C4_A))
+        x0, y0 = int(round(ADJ_DESIGN[0][0] * ADJ_A)), int(round(ADJ_DESIGN[0][1] * ADJ_A))
         chi[x0:x0 + 200, y0:y0 + 200] = 1.0 / (1.0 + 11.0 * rho)
         chi_ez = chi
     s.set_chi1inv(capi.Ez, chi_ez)
@@ -212,13 +214,13 @@ def build_c4(lib, single, chi_ez=None, rho=None):
     return s, f, mon
 
 
-def run_c4(args, lib):
+def run_adjoint2d(args, lib):
     """Forward run with recording, then the reverse-mode gradient; value is the forward
     Gcell-updates/s, and the line reports the gradient's cost relative to the forward run."""
     from meep_amd import autograd as madj
     single = args.precision == "f32"
     interval = args.checkpoint_interval
-    prob = madj.Problem(lib, lambda L, chi: build_c4(L, single, chi), (capi.Ez,) + C4_DESIGN, C4_STEPS,
+    prob = madj.Problem(lib, lambda L, chi: build_adjoint2d(L, single, chi), (capi.Ez,) + ADJ_DESIGN, ADJ_STEPS,
                         checkpoint_interval=interval)
     base, lo, shape = prob.base_chi()
     chi_design = base[lo[0]:lo[0] + shape[0], lo[1]:lo[1] + shape[1]]
@@ -229,14 +231,14 @@ def run_c4(args, lib):
     g = prob.gradient(h)
     tf, tg = prob.timing["forward_s"], prob.timing["gradient_s"]
     cells = 600 * 400
-    line = line_base(args, cells * C4_STEPS / tf / 1e9, tf, {
+    line = line_base(args, cells * ADJ_STEPS / tf / 1e9, tf, {
         "workload": "2D TM 600x400 cells at resolution 50, PML 50 cells, 200x200 design "
                     "(eps = 1 + 11 rho, rho seeded uniform blurred 5 px), Gaussian-profile Ez "
                     "line source, J = sum |DFT Ez|^2 on a 40-cell line at 10 frequencies, "
                     f"4000 steps; reverse mode with checkpoint interval {prob.interval}",
-        "cells": cells, "resolution": C4_A, "courant": 0.5, "parallelism": "none"})
-    line["steps"] = C4_STEPS
-    line["ms_per_step"] = round(tf / C4_STEPS * 1e3, 4)
+        "cells": cells, "resolution": ADJ_A, "courant": 0.5, "parallelism": "none"})
+    line["steps"] = ADJ_STEPS
+    line["ms_per_step"] = round(tf / ADJ_STEPS * 1e3, 4)
     line["adjoint"] = {"forward_s": round(tf, 4), "gradient_s": round(tg, 4),
                        "gradient_over_forward": round(tg / tf, 3), "J": J,
                        "grad_max_abs": float(np.abs(g).max())}
@@ -253,8 +255,8 @@ def run_c4(args, lib):
             for sg in (1, -1):
                 c2 = chi_full.copy()
                 c2[lo[0] + i, lo[1] + j] += sg * h
-                s, f, mon = build_c4(lib, False, c2)
-                f.step_n(C4_STEPS)
+                s, f, mon = build_adjoint2d(lib, False, c2)
+                f.step_n(ADJ_STEPS)
                 vals.append(sum(float(np.sum(np.abs(mon.dft(q)) ** 2)) for q in range(mon.num_chunks())))
                 del f, s
             fd = (vals[0] - vals[1]) / (2 * h)
@@ -268,13 +270,13 @@ def run_c4(args, lib):
 
 
 # ---- the batched 2D workload ----------------------------------------------------------------
-C5_N = 64
-C5_A = 10.0
-C5_PML = 10   # cells
-C5_NFREQ = 16
+BATCH_N = 64
+BATCH_A = 10.0
+BATCH_PML = 10   # cells
+BATCH_NFREQ = 16
 
 
-def c5_layout(seed=5, n=C5_N):
+def batch2d_layout(seed=5, n=BATCH_N):
     """Side lengths (cells, square cells from 128 to 512) and source frequencies."""
     rng = np.random.default_rng(seed)
     sides = rng.integers(128, 513, size=n)
@@ -282,16 +284,16 @@ def c5_layout(seed=5, n=C5_N):
     return sides, fcen, rng
 
 
-def build_c5(lib, single, seed=5, n=C5_N):
+def build_batch2d(lib, single, seed=5, n=BATCH_N):
     """n independent 2D TM simulations: side_i x side_i cells at resolution 10, eps per 16x16
     block drawn from U[1, 12], 10-cell uPML, Gaussian Ez point current (fcen_i from
     U[0.1, 0.3], df = 0.1 fcen_i), flux line of 16 frequencies with FP64 accumulators."""
-    sides, fcen, rng = c5_layout(seed, n)
+    sides, fcen, rng = batch2d_layout(seed, n)
     sims = []
     for q in range(n):
         side = int(sides[q])
-        L = side / C5_A
-        s = capi.structure(lib, dim=2, size=(L, L), a=C5_A, eps=None, pml=capi.pml(C5_PML / C5_A),
+        L = side / BATCH_A
+        s = capi.structure(lib, dim=2, size=(L, L), a=BATCH_A, eps=None, pml=capi.pml(BATCH_PML / BATCH_A),
                            single=single)
         nb = (side + 15) // 16
         blocks = rng.uniform(1.0, 12.0, size=(nb, nb))
@@ -304,31 +306,31 @@ def build_c5(lib, single, seed=5, n=C5_N):
         fc = float(fcen[q])
         f.add_volume_source(capi.Ez, capi.gaussian_src_time(fc, 0.1 * fc), (2.0, L / 2, 0), (2.0, L / 2, 0))
         fl = f.add_dft_flux(capi.X, (L - 2.0, 1.5, 0), (L - 2.0, L - 1.5, 0),
-                            capi.linspace(0.95 * fc, 1.05 * fc, C5_NFREQ), decimation_factor=1)
+                            capi.linspace(0.95 * fc, 1.05 * fc, BATCH_NFREQ), decimation_factor=1)
         sims.append((s, f, fl))
     return sims, sides
 
 
-def c5_algorithmic_bytes(sides, word=4):
+def batch2d_bytes(sides, word=4):
     """Reference-state bytes per step of the batch: 9 words per 2D TM cell, +6 per PML strip
     cell, +12 per PML corner cell, plus the flux line's 2 x 2 components' accumulators
     (complex FP64, read and written)."""
     total = 0
     for side in sides:
         side = int(side)
-        inner = side - 2 * C5_PML
-        strips = 4 * C5_PML * inner
-        corners = 4 * C5_PML * C5_PML
+        inner = side - 2 * BATCH_PML
+        strips = 4 * BATCH_PML * inner
+        corners = 4 * BATCH_PML * BATCH_PML
         words = 9 * side * side + 6 * strips + 12 * corners
         npts = (side - 30) + 1          # points of the flux line (|y - L/2| <= L/2 - 1.5)
-        total += words * word + 2 * npts * C5_NFREQ * 32
+        total += words * word + 2 * npts * BATCH_NFREQ * 32
     return total
 
 
-def run_c5(args, lib):
+def run_batch2d(args, lib):
     single = args.precision == "f32"
-    sims, sides = build_c5(lib, single)
-    b = capi.batch([f for _, f, _ in sims], use_graph=True, autotune=True)
+    sims, sides = build_batch2d(lib, single)
+    b = capi.batch([f for _, f, _ in sims], use_graph=not args.batch_eager, autotune=True)
     b.step_n(args.warmup)  # planning, the chunks' own first step, block-size tuning
     sync = device_sync()
     sync()
@@ -339,15 +341,15 @@ def run_c5(args, lib):
     cells = int(np.sum(sides.astype(np.int64) ** 2))
     value = cells * args.steps / el / 1e9
     block, ms = b.stats()
-    alg = c5_algorithmic_bytes(sides, 4 if single else 8)
+    alg = batch2d_bytes(sides, 4 if single else 8)
     achieved = alg * args.steps / el / 1e9
     line = line_base(args, value, el, {
-        "workload": f"{C5_N} independent 2D TM simulations, square sides 128-512 cells (seeded), "
+        "workload": f"{BATCH_N} independent 2D TM simulations, square sides 128-512 cells (seeded), "
                     "resolution 10, eps 16x16 blocks in [1, 12], 10-cell uPML, Gaussian Ez point "
                     "source fcen in [0.1, 0.3] df = 0.1 fcen, 16-frequency flux line with FP64 "
-                    "accumulators; one launch per launch group for all simulations, hipGraph step "
-                    f"loop, autotuned block size {block}",
-        "cells": cells, "simulations": C5_N, "resolution": C5_A, "courant": 0.5, "parallelism": "none"})
+                    "accumulators; one launch per launch group for all simulations, "
+                    f"{'eager' if args.batch_eager else 'hipGraph'} step loop, autotuned block size {block}",
+        "cells": cells, "simulations": BATCH_N, "resolution": BATCH_A, "courant": 0.5, "parallelism": "none"})
     line["roofline"] = {"bound": "hbm", "kernel": "whole batched step (all launches)",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": alg}
@@ -355,14 +357,14 @@ def run_c5(args, lib):
     if not args.no_cpu_baseline:
         # the oracle steps the simulations one after another
         olib = oracle_lib()
-        osims, _ = build_c5(olib, single)
+        osims, _ = build_batch2d(olib, single)
         steps, t0 = 0, time.perf_counter()
         for _, f, _ in osims:
             f.step()
         el0 = time.perf_counter() - t0
         line["cpu_baseline"] = {"value": cells / el0 / 1e9, "unit": "Gcell-updates/s",
                                 "cores": ctypes.CDLL("libgomp.so.1").omp_get_max_threads(), "kind": "port",
-                                "sample": f"oracle, one step of each of the {C5_N} simulations in turn "
+                                "sample": f"oracle, one step of each of the {BATCH_N} simulations in turn "
                                           f"(the first, so it includes lazy allocation), {el0:.1f} s"}
     return line
 
@@ -481,26 +483,26 @@ def run_box256(args, lib):
     return line
 
 
-def run_c3(args, lib):
+def run_pml_lorentz_dft(args, lib):
     """The PML + Lorentz-Drude + DFT workload; the roofline is the whole step's algorithmic
     bytes (every launch of the step, DFT included) over the step time."""
     single = args.precision == "f32"
     kind = capi.DFT_KAHAN32 if single else capi.DFT_F64
-    s, f, dft = build_c3(lib, single, dft_kind=kind)
+    s, f, dft = build_pml_lorentz_dft(lib, single, dft_kind=kind)
     el, stats = timed_run(f, args.steps, args.warmup)
-    cells = int(np.prod(C3_CELLS))
+    cells = int(np.prod(PLD_CELLS))
     value = cells * args.steps / el / 1e9
     wbytes = 4 if single else 8
-    alg = c3_algorithmic_bytes(word=wbytes)
+    alg = pml_lorentz_dft_bytes(word=wbytes, dft_points=[dft.points(i) for i in range(dft.num_chunks())])
     achieved = alg * args.steps / el / 1e9
-    print(f"[bench] c3 {args.steps} steps in {el:.3f} s; per-kernel ms: "
+    print(f"[bench] pml_lorentz_dft {args.steps} steps in {el:.3f} s; per-kernel ms: "
           + ", ".join(f"{k}:{v[0]:.1f}/{v[1]}" for k, v in stats.items()), file=sys.stderr)
     line = line_base(args, value, el, {
         "workload": "3D 384x384x192 cells, 16-cell uPML on all faces, lower half eps_inf 2 + "
                     "Lorentz (1.0, 0.05, sigma 1.5) + Drude (1.0, 0.1, sigma 3), 150 seeded "
                     "spheres above, Ex plane current, DFT Ex/Ey/Ez on a 384x384 plane at 32 "
                     "frequencies (" + ("Kahan FP32" if single else "FP64") + " accumulators)",
-        "cells": cells, "resolution": C3_A, "courant": 0.5, "parallelism": "none"})
+        "cells": cells, "resolution": PLD_A, "courant": 0.5, "parallelism": "none"})
     line["roofline"] = {"bound": "hbm", "kernel": "whole step (all launches)",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -508,7 +510,7 @@ def run_c3(args, lib):
                         "phase_ms": {str(k): round(v[0] / args.steps, 4) for k, v in stats.items()}}
     line["cpu_baseline"] = None
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(lambda L: build_c3(L, single)[1], cells,
+        line["cpu_baseline"] = cpu_baseline(lambda L: build_pml_lorentz_dft(L, single)[1], cells,
                                             "the same 384x384x192 cell (realnum DFT)",
                                             seconds=20.0, max_steps=8)
     return line
@@ -533,6 +535,9 @@ def main():
                     help="adjoint2d: check the gradient on this many pixels by central "
                          "differences of FP64 runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="batch2d: launch the step loop eagerly instead of through a hipGraph "
+                         "(kernel traces do not list graph-launched kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy 5 times (the reference "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -553,8 +558,8 @@ def main():
               "per half-step).", file=sys.stderr)
         sys.exit(2)
     lib = meep_amd.load_library()
-    run = {"box256": run_box256, "pml_lorentz_dft": run_c3, "adjoint2d": run_c4,
-           "batch2d": run_c5}[args.workload]
+    run = {"box256": run_box256, "pml_lorentz_dft": run_pml_lorentz_dft, "adjoint2d": run_adjoint2d,
+           "batch2d": run_batch2d}[args.workload]
     line = run(args, lib)
     print(json.dumps(line))
 

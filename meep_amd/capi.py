@@ -283,6 +283,13 @@ class dft_fields:
             raise MeepError(self.lib.mp_last_error().decode())
         return c
 
+    def points(self, i):
+        """Points of chunk i (interpolation planes of the monitor volume included)."""
+        n = self.lib.mp_dft_fields_chunk_points(self.ptr, i)
+        if n < 0:
+            raise MeepError(self.lib.mp_last_error().decode())
+        return n
+
     def dft(self, i):
         n = self.lib.mp_dft_fields_chunk_points(self.ptr, i)
         out = np.empty(2 * n * self.nfreq)

@@ -232,18 +232,23 @@ __global__ __launch_bounds__(BLOCK) void k_step(const StepArgs<R> a, uint32_t co
 template <class R, int V> struct Vec {
   R v[V];
 };
+// one 16-byte (or, for 8-byte groups, 8-byte) access per group; groups are aligned to their size
+template <int B> struct VecWord;
+template <> struct VecWord<16> { using T = float4; };
+template <> struct VecWord<8> { using T = float2; };
 template <class R, int V> __device__ __forceinline__ Vec<R, V> vload(const R *p) {
-  static_assert(V * sizeof(R) == 16, "16-byte groups");
+  using W = typename VecWord<V * sizeof(R)>::T;
   Vec<R, V> r;
-  const float4 q = *reinterpret_cast<const float4 *>(p);
-  __builtin_memcpy(r.v, &q, 16);
+  const W q = *reinterpret_cast<const W *>(p);
+  __builtin_memcpy(r.v, &q, sizeof(W));
   return r;
 }
 template <class R, int V> __device__ __forceinline__ void vstore(R *p, const Vec<R, V> &x, unsigned mask) {
+  using W = typename VecWord<V * sizeof(R)>::T;
   if (mask == (1u << V) - 1) {
-    float4 q;
-    __builtin_memcpy(&q, x.v, 16);
-    *reinterpret_cast<float4 *>(p) = q;
+    W q;
+    __builtin_memcpy(&q, x.v, sizeof(W));
+    *reinterpret_cast<W *>(p) = q;
   }
   else {
 #pragma unroll
@@ -812,32 +817,38 @@ template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, h
     hipLaunchKernelGGL((k_step<R, false, true>), dim3(nb), dim3(BLOCK), 0, s, a, c32);
 }
 
-template <class R, bool PML, int NP>
-void launch_fast_np(const StepArgs<R> &a, bool curl, bool upd, unsigned nb, uint32_t c32, int g0, int ngroups, hipStream_t s) {
-  constexpr int V = 16 / sizeof(R);
+// groups of V points along the rows of a's launch box: first group, groups per row, threads
+template <int V> static inline void fast_groups(const Grid &g, int &g0, int &ngroups, uint32_t &count) {
+  g0 = g.alo[2] / V;
+  ngroups = (g.alo[2] + g.an[2] - 1) / V - g0 + 1;
+  const size_t n = size_t(ngroups) * size_t(g.an[0]) * size_t(g.an[1]);
+  if (n >= (size_t(1) << 32)) throw std::runtime_error("meep: launch box too large for one launch");
+  count = uint32_t(n);
+}
+template <class R, int V, bool PML, int NP> void launch_fast_np(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
+  int g0, ngroups;
+  uint32_t c32;
+  fast_groups<V>(a.g, g0, ngroups, c32);
+  const unsigned nb = nblocks8(c32);
   if (curl && upd)
     hipLaunchKernelGGL((k_fast<R, V, true, true, PML, NP>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+  else if (curl)
+    hipLaunchKernelGGL((k_fast<R, V, true, false, PML, NP>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
   else if (upd)
     hipLaunchKernelGGL((k_fast<R, V, false, true, PML, NP>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
 }
+// The pole-carrying updates hold P, P_prev and sigma of every pole in registers: in FP32 they
+// take 8-byte groups (V = 2) to keep the occupancy of the 16-byte form's plain kernels.
 template <class R, bool PML> void launch_fast_t(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
-  constexpr int V = 16 / sizeof(R);
-  const int g0 = a.g.alo[2] / V, g1 = (a.g.alo[2] + a.g.an[2] - 1) / V;
-  const int ngroups = g1 - g0 + 1;
-  const size_t count = size_t(ngroups) * size_t(a.g.an[0]) * size_t(a.g.an[1]);
-  if (count >= (size_t(1) << 32)) throw std::runtime_error("meep: launch box too large for one launch");
-  const unsigned nb = nblocks8(count);
-  const uint32_t c32 = uint32_t(count);
-  if (curl && !upd) {
-    hipLaunchKernelGGL((k_fast<R, V, true, false, PML, 0>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
+  constexpr int V = 16 / sizeof(R), VP = V > 2 ? 2 : V;
+  if (!upd || a.npole == 0) {
+    launch_fast_np<R, V, PML, 0>(a, curl, upd, s);
     return;
   }
-  // registers for as many poles as the update handles
   switch (a.npole) {
-    case 0: launch_fast_np<R, PML, 0>(a, curl, upd, nb, c32, g0, ngroups, s); break;
-    case 1: launch_fast_np<R, PML, 1>(a, curl, upd, nb, c32, g0, ngroups, s); break;
-    case 2: launch_fast_np<R, PML, 2>(a, curl, upd, nb, c32, g0, ngroups, s); break;
-    default: launch_fast_np<R, PML, MAXPOLE>(a, curl, upd, nb, c32, g0, ngroups, s); break;
+    case 1: launch_fast_np<R, VP, PML, 1>(a, curl, upd, s); break;
+    case 2: launch_fast_np<R, VP, PML, 2>(a, curl, upd, s); break;
+    default: launch_fast_np<R, VP, PML, MAXPOLE>(a, curl, upd, s); break;
   }
 }
 template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, bool pml, hipStream_t s) {

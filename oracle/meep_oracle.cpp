@@ -1,6 +1,6 @@
 /* meep_oracle.cpp — CPU restatement of Meep's FDTD time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker the MI355X engine is compared with; it
+ * TEST INFRASTRUCTURE ONLY.  This file is the CPU reference the MI355X engine is compared with; it
  * is never linked into, loaded by or called from the product library.  Only tests/, the
  * smoke check and bench.py's CPU-baseline leg use it.
  *
@@ -23,8 +23,17 @@
  * It implements the C-ABI of include/meep_amd.h so tests drive it and the engine with the
  * same calls.  realnum is a template parameter (double, or float for --enable-single).
  *
- * Parity status of each part is stated in the repository's design notes; the whole-step
- * behaviour is pinned by the reference's known answers (tests/known_results.cpp:137-157).
+ * Pinning (what holds this restatement to the reference, beyond reading it):
+ *   - the whole step (curl, uPML, E/H update, Lorentz poles, Bloch/metal/magnetic walls,
+ *     integrated legacy sources) and field_energy: the reference's 13 known answers
+ *     (tests/known_results.cpp:137-157), FP64 and FP32, in tests/test_oracle_known_results.py;
+ *   - DFT accumulation, flux, volume sources with loop_in_chunks weights and field_energy:
+ *     closed forms in tests/test_oracle_closed_forms.py (a lossless cavity conserves energy,
+ *     a lossless line carries the same flux through two planes, a current sheet radiates a
+ *     flux spectrum proportional to |J(omega)|^2);
+ *   - add_dft_fields shares the accumulation code the flux checks pin.
+ * The reverse mode and batched stepping entry points are not restated (they fail here); the
+ * engine's are checked against finite differences and against one-at-a-time runs.
  */
 #include "../include/meep_amd.h"
 

@@ -41,7 +41,7 @@ def test_pml_lorentz_dft_fp32_precision(gpu_lib, oracle_lib):
     print(f"[workload] 3D uPML Lorentz-Drude DFT precision {n} {steps} steps", flush=True)
     out = []
     for lib, single, kind in ((gpu_lib, True, capi.DFT_KAHAN32), (oracle_lib, False, capi.DFT_F64)):
-        s, f, df = bench.build_c3(lib, single, n=n, npml=npml, nfreq=32, dft_kind=kind)
+        s, f, df = bench.build_pml_lorentz_dft(lib, single, n=n, npml=npml, nfreq=32, dft_kind=kind)
         L = [c / 10.0 for c in n]
         zf = (n[2] // 2 + 8) / 10.0
         fl = f.add_dft_flux(capi.Z, (0, 0, zf), (L[0], L[1], zf), capi.linspace(0.4, 1.0, 32),

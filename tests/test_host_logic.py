@@ -69,7 +69,7 @@ def test_batch_replans_after_new_source(host_lib):
 
 def test_batch_bench_workload_plans(host_lib):
     import bench
-    sims, _ = bench.build_c5(host_lib, True, n=6)
+    sims, _ = bench.build_batch2d(host_lib, True, n=6)
     b = capi.batch([f for _, f, _ in sims], use_graph=True, autotune=True)
     b.step_n(25)
     assert all(f.t == 25 for _, f, _ in sims)
