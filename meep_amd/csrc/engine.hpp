@@ -134,7 +134,7 @@ struct SrcBoxes {
 };
 
 // The sources of one launch, in device memory: the kernels read them only in the few groups
-// whose indices fall in [StepArgs::slo, StepArgs::shi], so the tables cost the other groups
+// inside the box StepArgs::sbox_lo..sbox_hi (loop-axis indices), so the tables cost the other groups
 // no registers.
 struct SrcSet {
   SrcTab cur;          // currents, after the curl
@@ -151,7 +151,7 @@ template <class R> struct StepArgs {
   int npole;
   PoleCoef<R> pole[MAXPOLE];
   const SrcSet *src;   // null: no sources in this launch
-  int64_t slo, shi;    // index range that holds every source point of src
+  int sbox_lo[3], sbox_hi[3]; // loop-axis box holding every source point of src
   const double *vrow;  // this step's source values (batched launches pass theirs per step)
   int64_t vofs;        // batched launches: this chunk's offset in the step's value row
   int cmp;             // 0: real parts, 1: imaginary parts of the fields this launch moves

@@ -876,8 +876,9 @@ template <class R> struct Fields : FieldsBase {
     // sources: one of this chunk's device source sets (validated when built), with its rows
     if (a.src) {
       const SrcSet *base = splan.dev;
-      if (!base || a.src < base || a.src >= base + 4 || a.slo > a.shi || a.slo < 0 || a.shi >= int64_t(gv.ntot))
-        fail("internal: bad source set");
+      if (!base || a.src < base || a.src >= base + 4) fail("internal: bad source set");
+      for (int k = 0; k < 3; ++k)
+        if (a.sbox_lo[k] > a.sbox_hi[k]) fail("internal: empty source box");
       if (!collector && !a.vrow) fail("internal: source set without its value row");
     }
     // loop-axis view: launch box inside the grid, slots' ranges inside the arrays
@@ -1392,7 +1393,8 @@ template <class R> struct Fields : FieldsBase {
   };
   struct SrcLaunch { // the sources of one (phase, cmp) launch
     SrcSet set;      // host copy (vbase relative to the chunk's row)
-    int64_t lo = INT64_MAX, hi = -1;
+    int64_t lo = INT64_MAX, hi = -1;  // flat index range of the source points
+    int blo[3] = {0, 0, 0}, bhi[3] = {-1, -1, -1}; // their loop-axis box
     std::vector<LongSrc> rest_cur, rest_dip;
   };
   struct SrcPlan {
@@ -1442,6 +1444,10 @@ template <class R> struct Fields : FieldsBase {
         memset(&L.set, 0, sizeof L.set);
         L.lo = INT64_MAX;
         L.hi = -1;
+        for (int k = 0; k < 3; ++k) {
+          L.blo[k] = INT32_MAX;
+          L.bhi[k] = INT32_MIN;
+        }
         L.rest_cur.clear();
         L.rest_dip.clear();
         if (cmp >= ncmp()) continue;
@@ -1456,6 +1462,14 @@ template <class R> struct Fields : FieldsBase {
           if (tb->n) {
             L.lo = std::min(L.lo, tb->lo);
             L.hi = std::max(L.hi, tb->hi);
+            for (int e = 0; e < tb->n; ++e) {
+              int j[3];
+              index_to_axes(tb->idx[e], j);
+              for (int k = 0; k < 3; ++k) {
+                L.blo[k] = std::min(L.blo[k], j[k]);
+                L.bhi[k] = std::max(L.bhi[k], j[k]);
+              }
+            }
           }
         for (const SrcBoxes *bx : {&L.set.curb, &L.set.dipb}) {
           if (bx->n < 0 || bx->n > SRC_BOX) fail("internal: bad source box count");
@@ -1472,8 +1486,14 @@ template <class R> struct Fields : FieldsBase {
             }
             L.lo = std::min(L.lo, first);
             L.hi = std::max(L.hi, last);
+            for (int k = 0; k < 3; ++k) {
+              L.blo[k] = std::min(L.blo[k], bx->lo[q][k]);
+              L.bhi[k] = std::max(L.bhi[k], bx->hi[q][k]);
+            }
           }
         }
+        if (L.lo > L.hi)
+          for (int k = 0; k < 3; ++k) L.blo[k] = 1, L.bhi[k] = 0;
       }
     std::vector<SrcSet> sets(4);
     for (int ph = 0; ph < 2; ++ph)
@@ -1488,6 +1508,18 @@ template <class R> struct Fields : FieldsBase {
     rows_n = 0;
   }
   bool srcplan_stale() const { return splan.pv != plan_version || splan.sv != src_version; }
+  // loop-axis indices of array index i (0 along absent axes)
+  void index_to_axes(int64_t i, int *j) const {
+    for (int k = 0; k < 3; ++k) {
+      const int d = gv.ydir(k);
+      if (!gv.has(d)) {
+        j[k] = 0;
+        continue;
+      }
+      j[k] = int(i / gv.st[d]);
+      i -= int64_t(j[k]) * gv.st[d];
+    }
+  }
   void ensure_srcplan() {
     if (!plan_valid) rebuild_plan();
     if (srcplan_stale()) build_srcplan();
@@ -1522,8 +1554,10 @@ template <class R> struct Fields : FieldsBase {
     const SrcLaunch &L = splan.L[ph][cmp];
     const bool any = L.lo <= L.hi;
     A.src = any ? splan.dev + (2 * ph + cmp) : nullptr;
-    A.slo = any ? L.lo : 0;
-    A.shi = any ? L.hi : -1;
+    for (int k = 0; k < 3; ++k) {
+      A.sbox_lo[k] = any ? L.blo[k] : 1;
+      A.sbox_hi[k] = any ? L.bhi[k] : 0;
+    }
     A.vrow = nullptr;
     A.vofs = vofs;
   }
@@ -1533,8 +1567,10 @@ template <class R> struct Fields : FieldsBase {
     const SrcLaunch &L = splan.L[ph][cmp];
     const bool any = L.lo <= L.hi;
     A.src = any ? splan.dev + (2 * ph + cmp) : nullptr;
-    A.slo = any ? L.lo : 0;
-    A.shi = any ? L.hi : -1;
+    for (int k = 0; k < 3; ++k) {
+      A.sbox_lo[k] = any ? L.blo[k] : 1;
+      A.sbox_hi[k] = any ? L.bhi[k] : 0;
+    }
     A.vrow = any ? row : nullptr;
     A.vofs = 0;
   }

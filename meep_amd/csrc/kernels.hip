@@ -101,6 +101,11 @@ __device__ __forceinline__ R apply_boxes(const SrcBoxes &b, int cmp, int c, int 
   if (in_box<1>(b, c, j0, j1, j2)) v = R(double(v) - box_value<1>(b, cmp, j0, j1, j2, vrow));
   return v;
 }
+// whether the V points from (j0, j1, z0) along the row meet the launch's source box
+template <int V, class R> __device__ __forceinline__ bool in_src_box(const StepArgs<R> &a, int j0, int j1, int z0) {
+  return j0 >= a.sbox_lo[0] && j0 <= a.sbox_hi[0] && j1 >= a.sbox_lo[1] && j1 <= a.sbox_hi[1] && z0 + V - 1 >= a.sbox_lo[2] &&
+         z0 <= a.sbox_hi[2];
+}
 // currents (dip false) or integrated dipoles (dip true) of slot C at point i: the source set
 // is read from device memory only for indices inside the launch's source range
 template <class R, bool DIP, int C>
@@ -142,7 +147,7 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
       k.fu[i] = fun;
       fv = k.siginvu[ku] * ((k.kapu[ku] - k.sigu[ku]) * fv + fun - fprev);
     }
-    if (a.src && int64_t(i) >= a.slo && int64_t(i) <= a.shi) fv = apply_sources<R, false, C>(a, i, j0, j1, j2, fv, vrow);
+    if (a.src && in_src_box<1>(a, j0, j1, j2)) fv = apply_sources<R, false, C>(a, i, j0, j1, j2, fv, vrow);
     if (slot_metal(k.ax, j0, j1, j2)) fv = R(0);
     put(k.f, i, fv, a.g, k.ax, j0, j1, j2);
   }
@@ -156,7 +161,7 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
                j1 <= a.pole[p].hi[1] && j2 >= a.pole[p].lo[2] && j2 <= a.pole[p].hi[2];
       if (pin[p]) gv -= u.P[p][i];
     }
-    if (a.src && int64_t(i) >= a.slo && int64_t(i) <= a.shi) gv = apply_sources<R, true, C>(a, i, j0, j1, j2, gv, vrow);
+    if (a.src && in_src_box<1>(a, j0, j1, j2)) gv = apply_sources<R, true, C>(a, i, j0, j1, j2, gv, vrow);
     R ev, wv;
     const int kw = u.ksigw >= 0 ? 2 * asel(j0, j1, j2, u.ksigw) + u.ysigw : 0;
     if (pml_on(u.sigw, u.kapw, kw)) {
@@ -487,6 +492,13 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
       load_prof<R, V>(P, g, kc.sig, kc.kap, kc.siginv, kc.ksig, kc.ysig, j0, j1, z0);
       load_prof<R, V>(U, g, kc.sigu, kc.kapu, kc.siginvu, kc.ksigu, kc.ysigu, j0, j1, z0);
       um = U.on & mask & upot;
+      if (!(P.on | um)) {
+        // no active profile in the group: the plain update (a wave with no uPML point skips
+        // the per-point branches below entirely)
+#pragma unroll
+        for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
+      }
+      else
 #pragma unroll
       for (int k = 0; k < V; ++k) {
         const bool pml = (P.on >> k) & 1, pmlu = (um >> k) & 1;
@@ -508,7 +520,7 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
         }
       }
     }
-    if (a.src && int64_t(i) + V - 1 >= a.slo && int64_t(i) <= a.shi) {
+    if (a.src && in_src_box<V>(a, j0, j1, z0)) {
 #pragma unroll
       for (int k = 0; k < V; ++k) fv.v[k] = apply_sources<R, false, C>(a, i + k, j0, j1, z0 + k, fv.v[k], vrow);
     }
@@ -529,7 +541,7 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
 #pragma unroll
         for (int k = 0; k < V; ++k) gv.v[k] -= Pc[p].v[k];
       }
-    if (a.src && int64_t(i) + V - 1 >= a.slo && int64_t(i) <= a.shi) {
+    if (a.src && in_src_box<V>(a, j0, j1, z0)) {
 #pragma unroll
       for (int k = 0; k < V; ++k) gv.v[k] = apply_sources<R, true, C>(a, i + k, j0, j1, z0 + k, gv.v[k], vrow);
     }
