@@ -3,7 +3,8 @@ launch group for all of them, source values from precomputed rows, hipGraph-capt
 autotuned block size) give the same fields and DFT accumulators as the same simulations
 stepped one at a time (which the parity tests hold to the oracle).  The batched kernels run
 the same per-point code; the bound allows for the compiler contracting a multiply-add
-differently in the batched instantiation."""
+differently in the batched instantiation.  A second batch mixes dispersive poles, complex
+fields and Bloch-phase halos."""
 
 import numpy as np
 import pytest
@@ -68,3 +69,31 @@ def test_batch_matches_individual_runs(gpu_lib, single, graph):
         for x, y in zip(fl1, fl2):
             fx, fy = x.flux(), y.flux()
             assert np.abs(fx - fy).max() <= tol * max(np.abs(fx).max(), 1e-30)
+
+
+def mixed_physics(lib, single):
+    """A 3D uPML cell with Lorentz and Drude poles (the pole-carrying batched kernels), a 2D
+    cell with complex fields and a zone-edge Bloch vector, and a complex 3D Bloch cell (halo
+    lists with phases)."""
+    from scenarios import bloch_3d, box_3d_pml_lorentz, zone_edge_2d
+    return [box_3d_pml_lorentz(lib, single=single), zone_edge_2d(lib, single=single), bloch_3d(lib, single=single)]
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_batch_with_poles_and_bloch_phases(gpu_lib, single):
+    nsteps = 90
+    print(f"[batch] poles and Bloch phases single={single}", flush=True)
+    solo = mixed_physics(gpu_lib, single)
+    for s, f, _ in solo:
+        f.step_n(nsteps)
+    together = mixed_physics(gpu_lib, single)
+    b = batch([f for _, f, _ in together], use_graph=True, autotune=False)
+    b.step_n(nsteps)
+    tol = 1e-5 if single else 1e-11
+    for (s1, f1, fl1), (s2, f2, fl2) in zip(solo, together):
+        assert f1.t == f2.t == nsteps
+        a, c = state(f1, s1, npol=2), state(f2, s2, npol=2)
+        assert set(a) == set(c)
+        for key in a:
+            scale = max(np.abs(a[key]).max(), 1e-30)
+            assert np.abs(a[key] - c[key]).max() <= tol * scale, key

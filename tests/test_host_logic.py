@@ -79,3 +79,11 @@ def test_batch_bench_workload_plans(host_lib):
 def test_reverse_sweep_plans(host_lib, interval):
     J, g, lo, _ = AC.gradient(host_lib, AC.rho_grid(), single=False, interval=interval)
     assert g.shape == (11, 11) and np.all(np.isfinite(g))
+
+
+def test_batch_plan_poles_and_bloch_phases(host_lib):
+    import test_gpu_batch as TB
+    sims = TB.mixed_physics(host_lib, False)
+    b = capi.batch([f for _, f, _ in sims], use_graph=True, autotune=False)
+    b.step_n(5)
+    assert all(f.t == 5 for _, f, _ in sims)
