@@ -388,11 +388,14 @@ template <class R> struct Fields : FieldsBase {
   bool use_fast = getenv("MEEP_AMD_GENERIC_KERNEL") == nullptr;
   // Whole rows: the launch boxes span whole rows, with the stride-1 direction's uPML taken
   // point by point inside the interior box (rows_pml), instead of thin end slabs whose short
-  // row pieces coalesce poorly.  Measured on MI355X: faster for 3D cells (384x384x192 with
-  // 16-cell uPML: 2.05 against 2.38 ms per step), slower for 2D ones (a batch of 128-512
-  // cell squares with 10-cell uPML: 0.204 against 0.194 ms).  MEEP_AMD_WHOLE_ROWS=0 or 1
-  // overrides the choice.
+  // row pieces coalesce poorly and cost two more launches per phase.  Measured on MI355X:
+  // faster for a chunk stepped alone (384x384x192 with 16-cell uPML: 2.05 against 2.38 ms per
+  // step; 600x400 2D with 50-cell uPML: 0.068 against 0.098 ms), slower inside a batch, whose
+  // per-point-uPML group then carries most of the work (64 squares of 128-512 cells with
+  // 10-cell uPML: 0.220 against 0.186 ms); a batch sets batch_split on its chunks.
+  // MEEP_AMD_WHOLE_ROWS=0 or 1 overrides the choice.
   int whole_rows_env = getenv("MEEP_AMD_WHOLE_ROWS") ? atoi(getenv("MEEP_AMD_WHOLE_ROWS")) : -1;
+  bool batch_split = false;
   bool rows_pml = false;
   // kernel timing: event pairs recorded around launches, resolved after each synchronisation
   bool ktiming = false;
@@ -1186,7 +1189,7 @@ template <class R> struct Fields : FieldsBase {
       quiet_lo[d] = ilo[d];
       quiet_hi[d] = ihi[d];
     }
-    const bool whole_rows = whole_rows_env >= 0 ? whole_rows_env != 0 : gv.dim == 3;
+    const bool whole_rows = whole_rows_env >= 0 ? whole_rows_env != 0 : !batch_split;
     if (whole_rows) { // the stride-1 direction's uPML stays inside whole-row boxes
       const int d = gv.fast;
       rows_pml = gv.has(d) && (ilo[d] > 0 || ihi[d] < full_hi[d]);
@@ -2439,6 +2442,10 @@ template <class R> struct Fields : FieldsBase {
   }
 
   void step_n(long n, bool) override {
+    if (batch_split) { // stepped on its own again: the single-chunk layout
+      batch_split = false;
+      boxes_valid = false;
+    }
     if (!plan_valid) rebuild_plan();
     auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipStreamSynchronize(stream));
@@ -3021,6 +3028,15 @@ template <class R> struct Batch : BatchBase {
       (void)hipStreamDestroy(stream);
     }
   }
+  // the launch-box layout of the member chunks: split end slabs while batched (a chunk's own
+  // step_n restores its default)
+  void set_split(bool on) {
+    for (Fields<R> *f : sims)
+      if (f->batch_split != on) {
+        f->batch_split = on;
+        f->boxes_valid = false;
+      }
+  }
   void options(int graph, int t, int b) override {
     use_graph = graph != 0;
     tune = t != 0;
@@ -3086,6 +3102,7 @@ template <class R> struct Batch : BatchBase {
         if (attempt) fail("batch: a chunk keeps allocating arrays or needs the unfused source path");
         // the first step of a chunk allocates lazily; every chunk takes it on its own
         for (Fields<R> *f : sims) f->step_n(1, false);
+        set_split(true);
         taken = 1;
         continue;
       }
@@ -3169,6 +3186,7 @@ template <class R> struct Batch : BatchBase {
     if (n <= 0) return;
     for (Fields<R> *f : sims) HIPCHECK(hipStreamSynchronize(f->stream));
     long done = 0;
+    set_split(true);
     if (stale()) done = plan();
     const long m = n - done;
     if (m <= 0) return;
