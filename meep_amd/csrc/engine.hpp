@@ -165,6 +165,10 @@ template <class R> struct StepArgs {
   int sgn;
   R dtdx; // Courant number (the vectorised kernel applies the partner signs itself)
   int rev; // visit logical blocks in descending order (backward differences)
+  // pswap: the new P of each pole goes into the P_prev array (which then holds P, while the
+  // P array holds P_prev: the host swaps the two pointers after the step) instead of
+  // P_prev = P; P = new P, saving one store per pole and point
+  int pswap;
 };
 
 // Halo connection (connect_the_chunks / process_incoming_chunk_data,

@@ -694,6 +694,7 @@ template <class R> struct Fields : FieldsBase {
   // per direction: +1 / -1 when its periodic halo is folded into the kernels (real Bloch
   // phase on both sides), 0 when the halo lists carry it
   int wrap_dir[3] = {0, 0, 0};
+  bool p_conn = false; // some halo connection moves P (its pointers then stay in place)
   // slots: 0..19 the field arrays, 20 + 20*p + c the polarisation P of pole p
   void rebuild_plan() {
     HIPCHECK(hipStreamSynchronize(stream));
@@ -787,6 +788,10 @@ template <class R> struct Fields : FieldsBase {
           hi[d] = plane - 1;
       }
     }
+    p_conn = false; // halo lists that hold P slot pointers pin the P arrays
+    for (int q = 0; q < 2; ++q)
+      for (const Conn &cn : lists[q])
+        if (cn.slot >= NC) p_conn = true;
     for (int q = 0; q < 2; ++q)
       for (const Conn &cn : lists[q]) {
         if (size_t(cn.dst) >= gv.ntot || size_t(cn.src) >= gv.ntot || cn.slot >= nslot)
@@ -1808,6 +1813,9 @@ template <class R> struct Fields : FieldsBase {
     // --- D, then E = chi1inv (D - P - dipoles) and the polarisations ---
     alloc_pols();
     if (!plan_valid) rebuild_plan();
+    // P/P_prev ping-pong where P has no halo images: with folded or listed periodic halos the
+    // P array carries images while P_prev's halo points stay zero, as the reference keeps them
+    const bool pswap = !pols.empty() && !p_conn && !(wrap_dir[0] | wrap_dir[1] | wrap_dir[2]);
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
       cur_cmp = cmp;
       int which[3];
@@ -1818,6 +1826,7 @@ template <class R> struct Fields : FieldsBase {
       set_sources(A, 1, cmp, t);
       A.npole = int(pols.size());
       for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
+      A.pswap = pswap ? 1 : 0;
       const SrcLaunch &L = splan.L[1][cmp];
       if (L.rest_cur.empty() && L.rest_dip.empty()) {
         step_kernel(A, true, true, MP_K_STEP_DE);
@@ -1849,6 +1858,11 @@ template <class R> struct Fields : FieldsBase {
         off += ls.sv->index.size();
       }
     }
+    // the pole updates wrote the new P into the P_prev arrays
+    if (pswap)
+      for (auto &p : pols)
+        for (int c = 0; c < NC; ++c)
+          for (int cmp = 0; cmp < 2; ++cmp) std::swap(p.P[c][cmp], p.Pp[c][cmp]);
     if (!plan_valid) rebuild_plan();
     boundaries(1); // D, P_E, E (step_boundaries D_stuff .. E_stuff)
     t += 1;

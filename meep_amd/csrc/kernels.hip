@@ -183,8 +183,12 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
         const R pcur = u.P[p][i];
         const R pn = q.gamma1inv * (pcur * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * u.Pp[p][i] +
                                     q.omega0dtsqr * (u.sigma[p][i] * wv));
-        put(u.P[p], i, pn, a.g, u.ax, j0, j1, j2);
-        u.Pp[p][i] = pcur;
+        if (a.pswap)
+          put(u.Pp[p], i, pn, a.g, u.ax, j0, j1, j2);
+        else {
+          put(u.P[p], i, pn, a.g, u.ax, j0, j1, j2);
+          u.Pp[p][i] = pcur;
+        }
       }
     if (slot_metal(u.ax, j0, j1, j2)) ev = R(0);
     put(u.f, i, ev, a.g, u.ax, j0, j1, j2);
@@ -592,8 +596,12 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
 #pragma unroll
     for (int p = 0; p < NP; ++p)
       if (pin[p] && u.sigma[p]) {
-        put_row<R, V>(u.P[p], i, Pn[p], mask, g, ybits, j0, j1, z0);
-        vstore<R, V>(u.Pp[p] + i, Pc[p], mask);
+        if (a.pswap)
+          put_row<R, V>(u.Pp[p], i, Pn[p], mask, g, ybits, j0, j1, z0);
+        else {
+          put_row<R, V>(u.P[p], i, Pn[p], mask, g, ybits, j0, j1, z0);
+          vstore<R, V>(u.Pp[p] + i, Pc[p], mask);
+        }
       }
     put_row<R, V>(u.f, i, ev, mask, g, ybits, j0, j1, z0);
   }
