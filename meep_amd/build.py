@@ -20,17 +20,31 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
 
 
 def build(verbose=False):
+    """Compile every translation unit (the kernel files once per field precision) in parallel
+    processes, then link."""
     os.makedirs(OUT_DIR, exist_ok=True)
-    objs = []
+    jobs = []
     for src in SOURCES:
-        obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if src.endswith(".cpp"):
-            cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        stem = os.path.splitext(src)[0]
+        if src.endswith(".hip"):
+            parts = (1, 2) if src == "kernels.hip" else (0,)  # kernels.hip: single-chunk / batched launchers
+            for real in ("float", "double"):
+                for part in parts:
+                    obj = os.path.join(OUT_DIR, f"{stem}_{real}_{part}.o")
+                    jobs.append(([HIPCC] + FLAGS + [f"-DMX_REAL={real}", f"-DMX_PART={part}", "-c",
+                                                    os.path.join(CSRC, src), "-o", obj], obj))
+        else:
+            obj = os.path.join(OUT_DIR, stem + ".o")
+            jobs.append(([HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj], obj))
+    procs = []
+    for cmd, obj in jobs:
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
-        objs.append(obj)
+        procs.append((subprocess.Popen(cmd), cmd))
+    failed = [cmd for p, cmd in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+    objs = [obj for _, obj in jobs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

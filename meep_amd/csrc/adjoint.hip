@@ -253,7 +253,12 @@ template <class R> void launch_record(R *hist, const R *D, const Grid &g, const 
                                   double, double *, hipStream_t);                                                          \
   template void launch_adj_dft_scatter<R>(R *, const int64_t *, const double *, int, int64_t, double, hipStream_t);       \
   template void launch_record<R>(R *, const R *, const Grid &, const int *, const int *, hipStream_t);
+// The build compiles this file once per field precision (MX_REAL), in parallel.
+#ifdef MX_REAL
+INSTANTIATE_ADJ(MX_REAL)
+#else
 INSTANTIATE_ADJ(float)
 INSTANTIATE_ADJ(double)
+#endif
 
 } // namespace mx

@@ -1032,11 +1032,12 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
   if (nblk) hipLaunchKernelGGL(k_conn_batch<R>, dim3(nblk), dim3(BLOCK), 0, s, items, blk_item, blk_first);
 }
 
-#define INSTANTIATE(R)                                                                             \
+#define INSTANTIATE_BATCH(R)                                                                       \
   template void launch_step_batch<R>(const BatchItem<R> *, const uint32_t *, const uint32_t *, unsigned, int, bool, bool, int, \
                                      const double *, hipStream_t);                                 \
   template void launch_dft_batch<R>(const BatchDft<R> *, const uint32_t *, const uint32_t *, unsigned, long, hipStream_t); \
-  template void launch_conn_batch<R>(const BatchConn *, const uint32_t *, const uint32_t *, unsigned, hipStream_t); \
+  template void launch_conn_batch<R>(const BatchConn *, const uint32_t *, const uint32_t *, unsigned, hipStream_t);
+#define INSTANTIATE(R)                                                                             \
   template void launch_step<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_fast<R>(const StepArgs<R> &, bool, bool, bool, hipStream_t);                \
   template void launch_conn<R>(const Conn *, const double2 *, int, R *const *, int, hipStream_t);  \
@@ -1050,7 +1051,20 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
   template void launch_dft<R>(const DftArgs<R> &, hipStream_t);                                    \
   template void launch_check<R>(const R *const *, const int64_t *, const double *, int, int *,     \
                                 hipStream_t);
+// The build compiles this file once per field precision (MX_REAL) and part (MX_PART 1: the
+// single-chunk launchers, 2: the batched ones), in parallel.
+#ifdef MX_REAL
+#if MX_PART != 2
+INSTANTIATE(MX_REAL)
+#endif
+#if MX_PART != 1
+INSTANTIATE_BATCH(MX_REAL)
+#endif
+#else
 INSTANTIATE(float)
 INSTANTIATE(double)
+INSTANTIATE_BATCH(float)
+INSTANTIATE_BATCH(double)
+#endif
 
 } // namespace mx
