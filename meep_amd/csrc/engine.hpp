@@ -196,6 +196,18 @@ template <class R> struct DftArgs {
   int acc_kind; // MP_DFT_* of meep_amd.h
 };
 
+// Deferred DFT accumulation: the averaged, weighted field values of a chunk's points are kept
+// for up to DFT_DEFER accumulation steps (snap0/snap1, step-major), then one pass applies
+// every step's term to each accumulator, in step order and with the arithmetic of one step at
+// a time, so the accumulators are read and written once per DFT_DEFER steps.
+constexpr int DFT_DEFER = 8;
+constexpr int DFT_FLUSH_TAB = 2048; // phases (steps x frequencies) a flush keeps in LDS
+template <class R> struct DftDefer {
+  const R *snap0, *snap1; // [step][point]; snap1 null for real fields
+  int nsteps;
+  double time[DFT_DEFER]; // each step's time (as the per-step accumulation would use it)
+};
+
 // ---- batched steps (many chunks in one launch) ---------------------------------------------
 // One launch boxes of many independent chunks: item k is one launch of the single-chunk path
 // (its argument block, thread count and row groups), placed in device memory; a block finds
@@ -307,6 +319,10 @@ void launch_src_images(R *f, const int64_t *idx, const int *k, const int64_t *ds
 template <class R> void launch_gather(R *dst, const R *f, const int64_t *idx, int n, hipStream_t s);
 template <class R> void launch_scatter(R *f, const int64_t *idx, const R *src, int n, hipStream_t s);
 template <class R> void launch_dft(const DftArgs<R> &a, hipStream_t s);
+// the values of a's points at this step into snap0 / snap1 (one step's slot)
+template <class R> void launch_dft_snap(const DftArgs<R> &a, R *snap0, R *snap1, hipStream_t s);
+// the deferred steps of d into a's accumulators
+template <class R> void launch_dft_flush(const DftArgs<R> &a, const DftDefer<R> &d, hipStream_t s);
 template <class R>
 void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int n, int *flag,
                   hipStream_t s);

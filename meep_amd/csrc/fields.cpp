@@ -329,6 +329,10 @@ template <class R> struct DftChunk {
   int64_t avg1 = 0, avg2 = 0;
   int acc_kind = MP_DFT_REALNUM;
   DevBuf idx, w, omega, acc, comp;
+  // deferred accumulation (DftDefer): up to `defer` steps of point values wait in snap
+  int defer = 1, nsnap = 0;
+  double snap_time[DFT_DEFER] = {};
+  DevBuf snap;
 };
 template <class R> struct Flux {
   std::vector<DftChunk<R> *> E, H;
@@ -2466,10 +2470,13 @@ template <class R> struct Fields : FieldsBase {
     keep.clear(); // nothing queued reads them any more
     rows_free = true;
     // the reference checks after every step (src/step.cpp:137-138); a batch probes its last step
+    dft_defer_steps = n > 1;
     for (long q = 0; q < n; ++q) {
       rows_hint = n - q;
       one_step(q == n - 1);
     }
+    flush_dfts(); // every deferred DFT step reaches the accumulators before the call returns
+    dft_defer_steps = false;
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipGetLastError());
     timers[6] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2493,31 +2500,71 @@ template <class R> struct Fields : FieldsBase {
   }
 
   // ---- DFT ----
+  // the accumulation arguments of chunk ch at the current time
+  DftArgs<R> dft_args(const DftChunk<R> &ch) const {
+    DftArgs<R> da;
+    da.f0 = f[ch.c][0];
+    da.f1 = f[ch.c][1];
+    da.idx = static_cast<const int64_t *>(ch.idx.p);
+    da.w = static_cast<const double *>(ch.w.p);
+    da.omega = static_cast<const double *>(ch.omega.p);
+    da.time = (is_magnetic(ch.c) || is_B(ch.c)) ? t * dt - 0.5 * dt : t * dt;
+    da.scale_re = ch.scale.real();
+    da.scale_im = ch.scale.imag();
+    da.avg1 = ch.avg1;
+    da.avg2 = ch.avg2;
+    da.npts = ch.npts;
+    da.nfreq = ch.nfreq;
+    da.acc = ch.acc.p;
+    da.comp = ch.comp.p;
+    da.acc_kind = ch.acc_kind;
+    return da;
+  }
+  bool dft_defer_steps = false; // set by step_n for calls of several steps
+  // DFT accumulation of this step (dft_chunk::update_dft for every chunk, src/dft.cpp:250-308):
+  // chunks whose phases for DFT_DEFER steps fit the flush kernel's table keep this step's
+  // point values and accumulate every DFT_DEFER steps (and at the end of every step_n call)
   void update_dfts() {
     for (auto &cp : dft_chunks) {
       DftChunk<R> &ch = *cp;
       if (t % ch.decimation) continue;
       if (!f[ch.c][0]) continue;
-      DftArgs<R> da;
-      da.f0 = f[ch.c][0];
-      da.f1 = f[ch.c][1];
-      da.idx = static_cast<const int64_t *>(ch.idx.p);
-      da.w = static_cast<const double *>(ch.w.p);
-      da.omega = static_cast<const double *>(ch.omega.p);
-      da.time = (is_magnetic(ch.c) || is_B(ch.c)) ? t * dt - 0.5 * dt : t * dt;
-      da.scale_re = ch.scale.real();
-      da.scale_im = ch.scale.imag();
-      da.avg1 = ch.avg1;
-      da.avg2 = ch.avg2;
-      da.npts = ch.npts;
-      da.nfreq = ch.nfreq;
-      da.acc = ch.acc.p;
-      da.comp = ch.comp.p;
-      da.acc_kind = ch.acc_kind;
+      const DftArgs<R> da = dft_args(ch);
       kbegin(MP_K_DFT);
-      launch_dft<R>(da, stream);
+      if (ch.defer > 1 && dft_defer_steps) {
+        const size_t plane = size_t(ch.npts);
+        const size_t need = sizeof(R) * plane * DFT_DEFER * (da.f1 ? 2 : 1);
+        if (ch.snap.bytes < need) {
+          if (ch.nsnap) fail("internal: deferred DFT storage changed with steps pending");
+          ch.snap.alloc(need);
+        }
+        R *base = static_cast<R *>(ch.snap.p);
+        R *s0 = base + size_t(ch.nsnap) * plane;
+        R *s1 = da.f1 ? base + size_t(DFT_DEFER + ch.nsnap) * plane : nullptr;
+        launch_dft_snap<R>(da, s0, s1, stream);
+        ch.snap_time[ch.nsnap++] = da.time;
+        if (ch.nsnap == ch.defer) flush_dft(ch);
+      }
+      else
+        launch_dft<R>(da, stream);
       kend();
     }
+  }
+  void flush_dft(DftChunk<R> &ch) {
+    if (!ch.nsnap) return;
+    const DftArgs<R> da = dft_args(ch);
+    DftDefer<R> d;
+    memset(&d, 0, sizeof d);
+    const R *base = static_cast<const R *>(ch.snap.p);
+    d.snap0 = base;
+    d.snap1 = da.f1 ? base + size_t(DFT_DEFER) * size_t(ch.npts) : nullptr;
+    d.nsteps = ch.nsnap;
+    for (int j = 0; j < ch.nsnap; ++j) d.time[j] = ch.snap_time[j];
+    launch_dft_flush<R>(da, d, stream);
+    ch.nsnap = 0;
+  }
+  void flush_dfts() {
+    for (auto &cp : dft_chunks) flush_dft(*cp);
   }
 
   // loop_in_chunks for one chunk and the identity symmetry (src/loop_in_chunks.cpp:232-540)
@@ -2780,6 +2827,7 @@ template <class R> struct Fields : FieldsBase {
       HIPCHECK(hipMemcpy(ch->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(ch->omega.p, om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice));
       ch->acc_kind = dft_acc;
+      ch->defer = ch->nfreq * DFT_DEFER <= DFT_FLUSH_TAB ? DFT_DEFER : 1;
       const size_t nacc = size_t(ch->npts) * ch->nfreq;
       const size_t esz = dft_acc == MP_DFT_F64 ? 16 : dft_acc == MP_DFT_KAHAN32 ? 8 : 2 * sizeof(R);
       ch->acc.alloc(esz * nacc);

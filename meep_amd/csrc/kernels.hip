@@ -702,15 +702,43 @@ __device__ __forceinline__ double2 dft_phase(double omega, double time, double s
   sincos(omega * time, &sn, &cs);
   return make_double2(cs * scale_re - sn * scale_im, cs * scale_im + sn * scale_re);
 }
-// accumulator o += phase * (f0 + i f1), in the chunk's accumulator form
-template <class R> __device__ __forceinline__ void dft_acc(const DftArgs<R> &a, size_t o, double2 phv, R f0, R f1) {
+// One accumulator in registers, in the chunk's accumulator form: FP64 (d), Kahan-compensated
+// FP32 (s, c), or realnum (re, im).
+template <class R> struct DftAcc {
+  double2 d;
+  float2 s, c;
+  R re, im;
+};
+template <class R> __device__ __forceinline__ void dft_load(const DftArgs<R> &a, size_t o, DftAcc<R> &v) {
+  if (a.acc_kind == 1)
+    v.d = reinterpret_cast<const double2 *>(a.acc)[o];
+  else if (a.acc_kind == 2) {
+    v.s = reinterpret_cast<const float2 *>(a.acc)[o];
+    v.c = reinterpret_cast<const float2 *>(a.comp)[o];
+  }
+  else {
+    v.re = reinterpret_cast<const R *>(a.acc)[2 * o];
+    v.im = reinterpret_cast<const R *>(a.acc)[2 * o + 1];
+  }
+}
+template <class R> __device__ __forceinline__ void dft_store(const DftArgs<R> &a, size_t o, const DftAcc<R> &v) {
+  if (a.acc_kind == 1)
+    reinterpret_cast<double2 *>(a.acc)[o] = v.d;
+  else if (a.acc_kind == 2) {
+    reinterpret_cast<float2 *>(a.acc)[o] = v.s;
+    reinterpret_cast<float2 *>(a.comp)[o] = v.c;
+  }
+  else {
+    reinterpret_cast<R *>(a.acc)[2 * o] = v.re;
+    reinterpret_cast<R *>(a.acc)[2 * o + 1] = v.im;
+  }
+}
+// v += phase * (f0 + i f1) (src/dft.cpp:296-306)
+template <class R> __device__ __forceinline__ void dft_add(const DftArgs<R> &a, DftAcc<R> &v, double2 phv, R f0, R f1) {
   const double pr = phv.x, pim = phv.y;
-  if (a.acc_kind == 1) { // FP64 accumulators
-    double2 *acc = reinterpret_cast<double2 *>(a.acc);
-    double2 v = acc[o];
-    v.x += pr * double(f0) - pim * double(f1);
-    v.y += pr * double(f1) + pim * double(f0);
-    acc[o] = v;
+  if (a.acc_kind == 1) {
+    v.d.x += pr * double(f0) - pim * double(f1);
+    v.d.y += pr * double(f1) + pim * double(f0);
     return;
   }
   const R rr = R(pr), ri = R(pim);
@@ -724,22 +752,23 @@ template <class R> __device__ __forceinline__ void dft_acc(const DftArgs<R> &a, 
     ti = f0 * ri;
   }
   if (a.acc_kind == 2) { // Kahan-compensated complex<float>
-    float2 *acc = reinterpret_cast<float2 *>(a.acc);
-    float2 *cp = reinterpret_cast<float2 *>(a.comp);
-    float2 s = acc[o], c = cp[o];
-    const float yr = float(tr) - c.x, yi = float(ti) - c.y;
-    const float sr = s.x + yr, si = s.y + yi;
-    c.x = (sr - s.x) - yr;
-    c.y = (si - s.y) - yi;
-    s.x = sr;
-    s.y = si;
-    acc[o] = s;
-    cp[o] = c;
+    const float yr = float(tr) - v.c.x, yi = float(ti) - v.c.y;
+    const float sr = v.s.x + yr, si = v.s.y + yi;
+    v.c.x = (sr - v.s.x) - yr;
+    v.c.y = (si - v.s.y) - yi;
+    v.s.x = sr;
+    v.s.y = si;
     return;
   }
-  R *acc = reinterpret_cast<R *>(a.acc);
-  acc[2 * o] += tr;
-  acc[2 * o + 1] += ti;
+  v.re += tr;
+  v.im += ti;
+}
+// accumulator o += phase * (f0 + i f1), in the chunk's accumulator form
+template <class R> __device__ __forceinline__ void dft_acc(const DftArgs<R> &a, size_t o, double2 phv, R f0, R f1) {
+  DftAcc<R> v;
+  dft_load(a, o, v);
+  dft_add(a, v, phv, f0, f1);
+  dft_store(a, o, v);
 }
 // the averaged, weighted field value of point k (src/dft.cpp:278-294)
 template <class R> __device__ __forceinline__ void dft_point(const DftArgs<R> &a, int k, R &f0, R &f1) {
@@ -812,6 +841,56 @@ template <class R> __global__ __launch_bounds__(BLOCK) void k_dft(const DftArgs<
     for (int j = 0; j < cnt; ++j) {
       const double2 phv = tab ? ph[fb + j] : dft_phase(a.omega[fb + j], a.time, a.scale_re, a.scale_im);
       dft_acc(a, o0 + size_t(j), phv, f0, f1);
+    }
+  }
+}
+
+// one step's value of every point of a chunk, into its deferred-step slot
+template <class R> __global__ __launch_bounds__(BLOCK) void k_dft_snap(const DftArgs<R> a, R *s0, R *s1) {
+  const int k = int(blockIdx.x) * BLOCK + int(threadIdx.x);
+  if (k >= a.npts) return;
+  R f0, f1;
+  dft_point(a, k, f0, f1);
+  s0[k] = f0;
+  if (s1) s1[k] = f1;
+}
+// the deferred steps of a chunk: a thread per point and DFT_Q consecutive frequencies loads
+// each accumulator once, applies the steps' terms in step order, and stores it once
+template <class R> __global__ __launch_bounds__(BLOCK) void k_dft_flush(const DftArgs<R> a, const DftDefer<R> d) {
+  __shared__ double2 ph[DFT_FLUSH_TAB];
+  __shared__ double tt[DFT_DEFER];
+  const int nf = a.nfreq, ns = d.nsteps;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < DFT_DEFER; ++j) tt[j] = d.time[j];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < ns * nf; q += BLOCK) {
+    const int j = q / nf;
+    ph[q] = dft_phase(a.omega[q - j * nf], tt[j], a.scale_re, a.scale_im);
+  }
+  __syncthreads();
+  const uint32_t nq = uint32_t(nf + DFT_Q - 1) / DFT_Q;
+  const uint32_t n = uint32_t(a.npts) * nq;
+  for (uint32_t o = blockIdx.x * BLOCK + threadIdx.x; o < n; o += gridDim.x * BLOCK) {
+    const uint32_t k = o / nq;
+    const int fb = int(o - k * nq) * DFT_Q;
+    const int cnt = min(DFT_Q, nf - fb);
+    R f0[DFT_DEFER], f1[DFT_DEFER];
+#pragma unroll
+    for (int j = 0; j < DFT_DEFER; ++j)
+      if (j < ns) {
+        f0[j] = d.snap0[size_t(j) * a.npts + k];
+        f1[j] = d.snap1 ? d.snap1[size_t(j) * a.npts + k] : R(0);
+      }
+    const size_t o0 = size_t(k) * size_t(nf) + size_t(fb);
+    for (int m = 0; m < cnt; ++m) {
+      DftAcc<R> v;
+      dft_load(a, o0 + m, v);
+#pragma unroll
+      for (int j = 0; j < DFT_DEFER; ++j)
+        if (j < ns) dft_add(a, v, ph[j * nf + fb + m], f0[j], f1[j]);
+      dft_store(a, o0 + m, v);
     }
   }
 }
@@ -921,6 +1000,17 @@ template <class R> void launch_scatter(R *f, const int64_t *idx, const R *src, i
   if (n > 0) hipLaunchKernelGGL(k_scatter<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, idx, src, n);
 }
 
+template <class R> void launch_dft_snap(const DftArgs<R> &a, R *snap0, R *snap1, hipStream_t s) {
+  if (a.npts > 0) hipLaunchKernelGGL(k_dft_snap<R>, dim3(nblocks(size_t(a.npts))), dim3(BLOCK), 0, s, a, snap0, snap1);
+}
+template <class R> void launch_dft_flush(const DftArgs<R> &a, const DftDefer<R> &d, hipStream_t s) {
+  if (d.nsteps < 1 || d.nsteps > DFT_DEFER || d.nsteps * a.nfreq > DFT_FLUSH_TAB)
+    throw std::runtime_error("meep: internal: bad deferred DFT flush");
+  const size_t n = size_t(a.npts) * size_t((a.nfreq + DFT_Q - 1) / DFT_Q);
+  if (n >= (size_t(1) << 31)) throw std::runtime_error("meep: DFT chunk too large for one launch");
+  const unsigned nb = std::min<unsigned>(nblocks(n), 4096u);
+  if (n > 0) hipLaunchKernelGGL(k_dft_flush<R>, dim3(nb), dim3(BLOCK), 0, s, a, d);
+}
 template <class R> void launch_dft(const DftArgs<R> &a, hipStream_t s) {
   const size_t n = size_t(a.npts) * size_t((a.nfreq + DFT_Q - 1) / DFT_Q);
   if (n >= (size_t(1) << 31)) throw std::runtime_error("meep: DFT chunk too large for one launch");
@@ -1049,6 +1139,8 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
   template void launch_gather<R>(R *, const R *, const int64_t *, int, hipStream_t);               \
   template void launch_scatter<R>(R *, const int64_t *, const R *, int, hipStream_t);              \
   template void launch_dft<R>(const DftArgs<R> &, hipStream_t);                                    \
+  template void launch_dft_snap<R>(const DftArgs<R> &, R *, R *, hipStream_t);                     \
+  template void launch_dft_flush<R>(const DftArgs<R> &, const DftDefer<R> &, hipStream_t);         \
   template void launch_check<R>(const R *const *, const int64_t *, const double *, int, int *,     \
                                 hipStream_t);
 // The build compiles this file once per field precision (MX_REAL) and part (MX_PART 1: the

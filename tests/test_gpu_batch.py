@@ -97,3 +97,28 @@ def test_batch_with_poles_and_bloch_phases(gpu_lib, single):
         for key in a:
             scale = max(np.abs(a[key]).max(), 1e-30)
             assert np.abs(a[key] - c[key]).max() <= tol * scale, key
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_batched_chunks_continue_on_their_own(gpu_lib, single):
+    """Chunks stepped in a batch and then on their own (the launch-box layout switches back)
+    match chunks stepped on their own throughout."""
+    print(f"[batch] batch then alone single={single}", flush=True)
+    solo = build_all(gpu_lib, single)
+    for s, f, _ in solo:
+        f.step_n(120)
+    mixed = build_all(gpu_lib, single)
+    b = batch([f for _, f, _ in mixed], use_graph=True, autotune=False)
+    b.step_n(60)
+    for s, f, _ in mixed:
+        f.step_n(60)
+    tol = 1e-5 if single else 1e-11
+    for (s1, f1, fl1), (s2, f2, fl2) in zip(solo, mixed):
+        assert f1.t == f2.t == 120
+        a, c = state(f1, s1), state(f2, s2)
+        for key in a:
+            scale = max(np.abs(a[key]).max(), 1e-30)
+            assert np.abs(a[key] - c[key]).max() <= tol * scale, key
+        for x, y in zip(fl1, fl2):
+            fx, fy = x.flux(), y.flux()
+            assert np.abs(fx - fy).max() <= tol * max(np.abs(fx).max(), 1e-30)
