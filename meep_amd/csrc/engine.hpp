@@ -227,6 +227,7 @@ template <class R> struct BatchDft {
   int half;
   int decimation;
   int64_t t_start;
+  R *snap0, *snap1; // deferred accumulation (DftDefer): this chunk's point values by step slot
 };
 // kind: 0 generic kernel, 1 vectorised, 2 vectorised with per-point uPML, 3 and 4 the same two
 // updating Lorentz/Drude poles
@@ -236,6 +237,15 @@ void launch_step_batch(const BatchItem<R> *items, const uint32_t *blk_item, cons
 template <class R>
 void launch_dft_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, long step,
                       hipStream_t s);
+// deferred accumulation of a batch whose chunks all accumulate every step: step `step`'s point
+// values into slot `slot` (block tables over points), and the steps first .. first + nsteps - 1
+// into the accumulators (block tables over points x frequency quads)
+template <class R>
+void launch_dft_snap_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, int slot,
+                           hipStream_t s);
+template <class R>
+void launch_dft_flush_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, long first,
+                            int nsteps, hipStream_t s);
 // halo lists of many chunks: item k is (conns, phases, n, slots, ncmp)
 struct BatchConn {
   const Conn *conns;

@@ -1106,6 +1106,11 @@ template <class R> struct Fields : FieldsBase {
       bd.half = (is_magnetic(ch.c) || is_B(ch.c)) ? 1 : 0;
       bd.decimation = ch.decimation;
       bd.t_start = t;
+      if (ch.nsnap) fail("internal: deferred DFT steps pending at batch planning");
+      const size_t need = sizeof(R) * size_t(ch.npts) * DFT_DEFER * (f[ch.c][1] ? 2 : 1);
+      if (ch.snap.bytes < need) ch.snap.alloc(need);
+      bd.snap0 = static_cast<R *>(ch.snap.p);
+      bd.snap1 = f[ch.c][1] ? bd.snap0 + size_t(DFT_DEFER) * size_t(ch.npts) : nullptr;
       C.dft.push_back(bd);
     }
     return plan_valid;
@@ -3076,6 +3081,10 @@ template <class R> struct Batch : BatchBase {
   std::vector<BatchDft<R>> dft_host;
   DevBuf dft_items, dft_item_tab, dft_first_tab;
   unsigned dft_nblk = 0;
+  bool dft_defer = false; // deferred accumulation (DftDefer) over the chunks' snap rings
+  DevBuf dft_snap_item, dft_snap_first, dft_flush_item, dft_flush_first;
+  unsigned dft_snap_nblk = 0, dft_flush_nblk = 0;
+  long call_steps = 0; // steps of the current step_n call (the last one flushes)
   std::vector<size_t> row_off; // each chunk's offset in the step's source row
   size_t row_len = 0;
   DevBuf rows;
@@ -3223,6 +3232,25 @@ template <class R> struct Batch : BatchBase {
       upload(dft_first_tab, first);
       dft_nblk = dft_host.empty() ? 0 : unsigned(item.size());
     }
+    // deferred accumulation when every chunk accumulates every step and its phases for
+    // DFT_DEFER steps fit the flush kernel's table
+    dft_defer = !dft_host.empty();
+    for (auto &d : dft_host)
+      if (d.decimation != 1 || d.a.nfreq * DFT_DEFER > DFT_FLUSH_TAB || !d.snap0) dft_defer = false;
+    if (dft_defer) {
+      std::vector<uint32_t> counts, item, first;
+      for (auto &d : dft_host) counts.push_back(uint32_t(d.a.npts));
+      tables(counts, 256, item, first);
+      upload(dft_snap_item, item);
+      upload(dft_snap_first, first);
+      dft_snap_nblk = unsigned(item.size());
+      counts.clear();
+      for (auto &d : dft_host) counts.push_back(uint32_t(size_t(d.a.npts) * size_t((d.a.nfreq + 3) / 4)));
+      tables(counts, 256, item, first);
+      upload(dft_flush_item, item);
+      upload(dft_flush_first, first);
+      dft_flush_nblk = unsigned(item.size());
+    }
     plan_versions.clear();
     src_versions.clear();
     for (Fields<R> *f : sims) {
@@ -3241,8 +3269,18 @@ template <class R> struct Batch : BatchBase {
       launch_conn_batch<R>(static_cast<const BatchConn *>(conn_items[phase].p), static_cast<const uint32_t *>(conn_item_tab[phase].p),
                            static_cast<const uint32_t *>(conn_first_tab[phase].p), conn_nblk[phase], stream);
     }
-    launch_dft_batch<R>(static_cast<const BatchDft<R> *>(dft_items.p), static_cast<const uint32_t *>(dft_item_tab.p),
-                        static_cast<const uint32_t *>(dft_first_tab.p), dft_nblk, s, stream);
+    const BatchDft<R> *di = static_cast<const BatchDft<R> *>(dft_items.p);
+    if (!dft_defer) {
+      launch_dft_batch<R>(di, static_cast<const uint32_t *>(dft_item_tab.p), static_cast<const uint32_t *>(dft_first_tab.p),
+                          dft_nblk, s, stream);
+      return;
+    }
+    const int slot = int(s % DFT_DEFER);
+    launch_dft_snap_batch<R>(di, static_cast<const uint32_t *>(dft_snap_item.p), static_cast<const uint32_t *>(dft_snap_first.p),
+                             dft_snap_nblk, slot, stream);
+    if (slot == DFT_DEFER - 1 || s == call_steps - 1)
+      launch_dft_flush_batch<R>(di, static_cast<const uint32_t *>(dft_flush_item.p), static_cast<const uint32_t *>(dft_flush_first.p),
+                                dft_flush_nblk, s - slot, slot + 1, stream);
   }
   void step_n(long n) override {
     if (n <= 0) return;
@@ -3273,6 +3311,7 @@ template <class R> struct Batch : BatchBase {
       upload(dft_items, dft_host);
     }
     const double *drows = static_cast<const double *>(rows.p);
+    call_steps = m;
     auto t0 = std::chrono::steady_clock::now();
     long st = 0;
     if (tune && !tuned) {

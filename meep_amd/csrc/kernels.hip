@@ -1105,6 +1105,68 @@ __global__ __launch_bounds__(BLOCK) void k_dft_batch(const BatchDft<R> *items, c
   dft_one<R>(a, o, ph[int(o % size_t(nf))]);
 }
 template <class R>
+__global__ __launch_bounds__(BLOCK) void k_dft_snap_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first,
+                                                        int slot) {
+  const BatchDft<R> &it = items[blk_item[blockIdx.x]];
+  const uint32_t p = blk_first[blockIdx.x] + threadIdx.x; // padding blocks start past every item
+  if (p >= uint32_t(it.a.npts)) return;
+  R f0, f1;
+  dft_point(it.a, int(p), f0, f1);
+  const size_t o = size_t(slot) * size_t(it.a.npts) + p;
+  it.snap0[o] = f0;
+  if (it.snap1) it.snap1[o] = f1;
+}
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_dft_flush_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first,
+                                                         long first, int ns) {
+  __shared__ double2 ph[DFT_FLUSH_TAB];
+  const BatchDft<R> &it = items[blk_item[blockIdx.x]];
+  const DftArgs<R> &a = it.a;
+  const int nf = a.nfreq;
+  if (nf * ns > DFT_FLUSH_TAB) return; // the host keeps such chunks out of deferral
+  for (int q = threadIdx.x; q < ns * nf; q += BLOCK) {
+    const int j = q / nf;
+    const int64_t tt = it.t_start + first + j + 1;
+    const double time = double(tt) * it.dt - (it.half ? 0.5 * it.dt : 0.0);
+    ph[q] = dft_phase(a.omega[q - j * nf], time, a.scale_re, a.scale_im);
+  }
+  __syncthreads();
+  const uint32_t nq = uint32_t(nf + DFT_Q - 1) / DFT_Q;
+  const uint32_t o = blk_first[blockIdx.x] + threadIdx.x;
+  if (o >= uint32_t(a.npts) * nq) return;
+  const uint32_t k = o / nq;
+  const int fb = int(o - k * nq) * DFT_Q;
+  const int cnt = min(DFT_Q, nf - fb);
+  R f0[DFT_DEFER], f1[DFT_DEFER];
+#pragma unroll
+  for (int j = 0; j < DFT_DEFER; ++j)
+    if (j < ns) {
+      f0[j] = it.snap0[size_t(j) * a.npts + k];
+      f1[j] = it.snap1 ? it.snap1[size_t(j) * a.npts + k] : R(0);
+    }
+  const size_t o0 = size_t(k) * size_t(nf) + size_t(fb);
+  for (int m = 0; m < cnt; ++m) {
+    DftAcc<R> v;
+    dft_load(a, o0 + m, v);
+#pragma unroll
+    for (int j = 0; j < DFT_DEFER; ++j)
+      if (j < ns) dft_add(a, v, ph[j * nf + fb + m], f0[j], f1[j]);
+    dft_store(a, o0 + m, v);
+  }
+}
+template <class R>
+void launch_dft_snap_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, int slot,
+                           hipStream_t s) {
+  if (slot < 0 || slot >= DFT_DEFER) throw std::runtime_error("meep: internal: bad deferred DFT slot");
+  if (nblk) hipLaunchKernelGGL(k_dft_snap_batch<R>, dim3(nblk), dim3(BLOCK), 0, s, items, blk_item, blk_first, slot);
+}
+template <class R>
+void launch_dft_flush_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, long first,
+                            int nsteps, hipStream_t s) {
+  if (nsteps < 1 || nsteps > DFT_DEFER) throw std::runtime_error("meep: internal: bad deferred DFT flush");
+  if (nblk) hipLaunchKernelGGL(k_dft_flush_batch<R>, dim3(nblk), dim3(BLOCK), 0, s, items, blk_item, blk_first, first, nsteps);
+}
+template <class R>
 void launch_dft_batch(const BatchDft<R> *items, const uint32_t *blk_item, const uint32_t *blk_first, unsigned nblk, long step,
                       hipStream_t s) {
   if (nblk) hipLaunchKernelGGL(k_dft_batch<R>, dim3(nblk), dim3(BLOCK), 0, s, items, blk_item, blk_first, step);
@@ -1126,6 +1188,8 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
   template void launch_step_batch<R>(const BatchItem<R> *, const uint32_t *, const uint32_t *, unsigned, int, bool, bool, int, \
                                      const double *, hipStream_t);                                 \
   template void launch_dft_batch<R>(const BatchDft<R> *, const uint32_t *, const uint32_t *, unsigned, long, hipStream_t); \
+  template void launch_dft_snap_batch<R>(const BatchDft<R> *, const uint32_t *, const uint32_t *, unsigned, int, hipStream_t); \
+  template void launch_dft_flush_batch<R>(const BatchDft<R> *, const uint32_t *, const uint32_t *, unsigned, long, int, hipStream_t); \
   template void launch_conn_batch<R>(const BatchConn *, const uint32_t *, const uint32_t *, unsigned, hipStream_t);
 #define INSTANTIATE(R)                                                                             \
   template void launch_step<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
