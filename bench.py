@@ -148,11 +148,14 @@ def build_pml_lorentz_dft(lib, single, n=PLD_CELLS, npml=PLD_PML, a=PLD_A, nfreq
     return s, f, dft
 
 
-def pml_lorentz_dft_bytes(n=PLD_CELLS, npml=PLD_PML, word=4, nfreq=PLD_NFREQ, acc_bytes=16, dft_points=None):
+def pml_lorentz_dft_bytes(n=PLD_CELLS, npml=PLD_PML, word=4, nfreq=PLD_NFREQ, acc_bytes=16, dft_points=None,
+                          sigma_arrays=False):
     """Reference-state bytes per step: 21 words per cell, +10 per PML face cell, +20 per edge
-    cell, +30 per corner cell, +15 per pole over dispersive cells, plus each DFT accumulator
-    read and written once (dft_points: the monitor's points over its chunks; a plane between
-    grid planes has two planes of points for the interpolation)."""
+    cell, +30 per corner cell, +12 per pole over dispersive cells (P and P_prev read and
+    written), +3 more where sigma is read as an array (sigma_arrays; the workload's sigma is
+    one value over the lower half, which the kernels take as a scalar), plus each DFT
+    accumulator read and written once (dft_points: the monitor's points over its chunks; a
+    plane between grid planes has two planes of points for the interpolation)."""
     cnt = np.zeros(4)
     regions = []
     for d in range(3):
@@ -165,7 +168,7 @@ def pml_lorentz_dft_bytes(n=PLD_CELLS, npml=PLD_PML, word=4, nfreq=PLD_NFREQ, ac
         cnt[q] = np.count_nonzero(k == q)
     cells = np.prod(n)
     disp = n[0] * n[1] * (n[2] // 2)
-    words = 21 * cells + 10 * cnt[1] + 20 * cnt[2] + 30 * cnt[3] + 2 * 15 * disp
+    words = 21 * cells + 10 * cnt[1] + 20 * cnt[2] + 30 * cnt[3] + 2 * (15 if sigma_arrays else 12) * disp
     npts = sum(dft_points) if dft_points is not None else 3 * (n[0] + 1) * (n[1] + 1)
     dft = npts * nfreq * 2 * acc_bytes
     return words * word + dft

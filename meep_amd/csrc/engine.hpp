@@ -91,6 +91,10 @@ template <class R> struct UpdComp {
   int lo[3], hi[3], ysh[3];
   R *P[MAXPOLE], *Pp[MAXPOLE];
   const R *sigma[MAXPOLE];
+  // bit p: sigma of pole p equals sigval[p] at every point of the pole's box (and is zero
+  // outside it), so the vectorised kernels take the value instead of loading the array
+  int sigone;
+  R sigval[MAXPOLE];
   // derived by finalize_args, as for CurlComp
   AxisSlot ax;
   int ksigw, ysigw;

@@ -357,6 +357,8 @@ template <class R> struct Fields : FieldsBase {
     bool allocated = false;
     R *P[NC][2] = {}, *Pp[NC][2] = {};
     R *sigma[NC] = {};
+    bool sig_uni[NC] = {}; // sigma of the component is sig_val over the whole pole box
+    R sig_val[NC] = {};
     PoleCoef<R> coef;
   };
   std::vector<Pol> pols; // the reference's list order: newest susceptibility first
@@ -546,6 +548,24 @@ template <class R> struct Fields : FieldsBase {
                   bhi[d] = std::max(bhi[d], j[d]);
                 }
       }
+      // components whose sigma (in realnum) takes one value over the whole box
+      if (bhi[0] >= 0 || bhi[1] >= 0 || bhi[2] >= 0)
+        for (int c = 0; c < 5; ++c) {
+          if (su.sigma[c].empty()) continue;
+          int j[3], lo[3], hi[3];
+          for (int d = 0; d < 3; ++d) {
+            lo[d] = gv.has(d) ? blo[d] : 0;
+            hi[d] = gv.has(d) ? bhi[d] : 0;
+          }
+          const R v0 = R(su.sigma[c][gv.ref_of(lo)]);
+          bool uni = true;
+          for (j[0] = lo[0]; uni && j[0] <= hi[0]; ++j[0])
+            for (j[1] = lo[1]; uni && j[1] <= hi[1]; ++j[1])
+              for (j[2] = lo[2]; uni && j[2] <= hi[2]; ++j[2])
+                if (R(su.sigma[c][gv.ref_of(j)]) != v0) uni = false;
+          p.sig_uni[c] = uni;
+          p.sig_val[c] = v0;
+        }
       for (int k = 0; k < 3; ++k) {
         const int d = gv.ydir(k);
         const bool has = gv.has(d);
@@ -1361,6 +1381,10 @@ template <class R> struct Fields : FieldsBase {
           e.P[p] = pols[p].P[ec][cmp];
           e.Pp[p] = pols[p].Pp[ec][cmp];
           e.sigma[p] = pols[p].sigma[ec];
+          if (e.sigma[p] && pols[p].sig_uni[ec]) {
+            e.sigone |= 1 << p;
+            e.sigval[p] = pols[p].sig_val[ec];
+          }
         }
       ++active;
     }

@@ -452,7 +452,13 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
         Pc[p] = vload<R, V>(u.P[p] + i);
         if (u.sigma[p]) {
           Ppv[p] = vload<R, V>(u.Pp[p] + i);
-          sgv[p] = vload<R, V>(u.sigma[p] + i);
+          if ((u.sigone >> p) & 1) { // one value over the pole's box, zero outside it
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+              sgv[p].v[k] = z0 + k >= a.pole[p].lo[2] && z0 + k <= a.pole[p].hi[2] ? u.sigval[p] : R(0);
+          }
+          else
+            sgv[p] = vload<R, V>(u.sigma[p] + i);
         }
       }
     }

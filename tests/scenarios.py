@@ -47,7 +47,9 @@ def waveguide_2d(lib, single=False, a=10, real=True, nfreq=20):
 
 def box_3d_pml_lorentz(lib, single=False, a=8, real=True):
     """Small 3D cell: PML on every face, lower half a Lorentz + Drude medium, blocks above,
-    Ex plane-wave current (a z-plane longer than the kernel's sparse table), flux plane."""
+    Ex plane-wave current (a z-plane longer than the kernel's sparse table), flux plane.  The
+    Lorentz sigma is uniform over the lower half and the Drude sigma graded along x, so the
+    kernels' single-value and per-point sigma forms run side by side."""
     L = 2.0
     eps_blk = blocks_eps(3, 4, 2.0, 6.0, (L, L, L))
 
@@ -58,7 +60,7 @@ def box_3d_pml_lorentz(lib, single=False, a=8, real=True):
         return 1.0 if z < L / 2 else 0.0
     s = structure(lib, dim=3, size=(L, L, L), a=a, eps=eps, pml=pml(0.375), single=single)
     s.add_susceptibility(lambda x, y, z: 1.5 * lower(x, y, z), 1.0, 0.05)
-    s.add_susceptibility(lambda x, y, z: 3.0 * lower(x, y, z), 1.0, 0.1, drude=True)
+    s.add_susceptibility(lambda x, y, z: (2.5 + 0.5 * x) * lower(x, y, z), 1.0, 0.1, drude=True)
     f = fields(s)
     if real:
         f.use_real_fields()
