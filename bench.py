@@ -61,18 +61,19 @@ def chi1inv_array(eps_b, n=N, block=BLOCK):
     return np.ascontiguousarray(1.0 / eps_b[np.ix_(b, b, b)])
 
 
-def build(lib, single, eps_b):
-    L = N / A
+def build(lib, single, eps_b, n=N, fcen=0.15, fwidth=0.1, start_time=0.0):
+    """The Bloch box of n^3 cells with eps on 8^3 blocks eps_b (n a multiple of 8)."""
+    L = n / A
     s = capi.structure(lib, dim=3, size=(L, L, L), a=A, eps=None, single=single)
-    chi = chi1inv_array(eps_b)
+    chi = chi1inv_array(eps_b, n)
     for c in (capi.Ex, capi.Ey, capi.Ez):
         s.set_chi1inv(c, chi)
     del chi
     f = capi.fields(s)
     f.use_bloch((0.0, 0.0, 0.0))
     f.use_real_fields()
-    cen = (N // 2) / A
-    f.add_point_source_src(capi.Ex, capi.gaussian_src_time(0.15, 0.1), (cen, cen, cen))
+    cen = (n // 2) / A
+    f.add_point_source_src(capi.Ex, capi.gaussian_src_time(fcen, fwidth, start_time=start_time), (cen, cen, cen))
     return s, f
 
 
@@ -287,10 +288,12 @@ def batch2d_layout(seed=5, n=BATCH_N):
     return sides, fcen, rng
 
 
-def build_batch2d(lib, single, seed=5, n=BATCH_N):
+def build_batch2d(lib, single, seed=5, n=BATCH_N, t_peak=None):
     """n independent 2D TM simulations: side_i x side_i cells at resolution 10, eps per 16x16
     block drawn from U[1, 12], 10-cell uPML, Gaussian Ez point current (fcen_i from
-    U[0.1, 0.3], df = 0.1 fcen_i), flux line of 16 frequencies with FP64 accumulators."""
+    U[0.1, 0.3], df = 0.1 fcen_i), flux line of 16 frequencies with FP64 accumulators.
+    t_peak: start each pulse early enough that it peaks at this time (the workload's pulses
+    peak at five widths, 170-500 time units in)."""
     sides, fcen, rng = batch2d_layout(seed, n)
     sims = []
     for q in range(n):
@@ -307,7 +310,9 @@ def build_batch2d(lib, single, seed=5, n=BATCH_N):
         f.use_real_fields()
         f.set_dft_accumulator(capi.DFT_F64)
         fc = float(fcen[q])
-        f.add_volume_source(capi.Ez, capi.gaussian_src_time(fc, 0.1 * fc), (2.0, L / 2, 0), (2.0, L / 2, 0))
+        t0 = 0.0 if t_peak is None else t_peak - 5.0 / (0.1 * fc)
+        f.add_volume_source(capi.Ez, capi.gaussian_src_time(fc, 0.1 * fc, start_time=t0), (2.0, L / 2, 0),
+                            (2.0, L / 2, 0))
         fl = f.add_dft_flux(capi.X, (L - 2.0, 1.5, 0), (L - 2.0, L - 1.5, 0),
                             capi.linspace(0.95 * fc, 1.05 * fc, BATCH_NFREQ), decimation_factor=1)
         sims.append((s, f, fl))

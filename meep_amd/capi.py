@@ -183,6 +183,7 @@ class structure:
         self.dim = dim
         self.single = single
         self.a = a
+        self.pml = [tuple(p) for p in pml]
 
     @property
     def ntot(self):

@@ -1772,6 +1772,9 @@ template <class R> struct Fields : FieldsBase {
     kend();
   }
   long plan_version = 0;
+  // bumped when a step swaps the P / P_prev pointers (pswap) and when a DFT monitor is added
+  // or removed: a batch plan records device pointers and DFT chunk lists, so either makes it stale
+  long ptr_version = 0, dft_version = 0;
   void prepare_images(SrcVol &sv) {
     if (sv.img_version == plan_version) return;
     std::vector<int> ks;
@@ -1892,10 +1895,12 @@ template <class R> struct Fields : FieldsBase {
       }
     }
     // the pole updates wrote the new P into the P_prev arrays
-    if (pswap)
+    if (pswap) {
       for (auto &p : pols)
         for (int c = 0; c < NC; ++c)
           for (int cmp = 0; cmp < 2; ++cmp) std::swap(p.P[c][cmp], p.Pp[c][cmp]);
+      ++ptr_version;
+    }
     if (!plan_valid) rebuild_plan();
     boundaries(1); // D, P_E, E (step_boundaries D_stuff .. E_stuff)
     t += 1;
@@ -2867,6 +2872,7 @@ template <class R> struct Fields : FieldsBase {
       }
       out.push_back(ch.get());
       dft_chunks.push_back(std::move(ch));
+      ++dft_version;
     });
   }
   void *add_dft_flux(int d, V3 mn, V3 mxv, const std::vector<double> &freq, int dec, int *dec_out) override {
@@ -2996,6 +3002,7 @@ template <class R> struct Fields : FieldsBase {
                 break;
               }
         fluxes.erase(fluxes.begin() + q);
+        ++dft_version;
         return;
       }
   }
@@ -3089,7 +3096,7 @@ template <class R> struct Batch : BatchBase {
   double last_ms = 0;
   // plan
   bool planned = false;
-  std::vector<long> plan_versions, src_versions, plan_t;
+  std::vector<long> plan_versions, src_versions, ptr_versions, dft_versions;
   struct Group {
     int phase;
     int kind; // 0 generic, 1 vectorised, 2 vectorised with per-point uPML
@@ -3169,7 +3176,8 @@ template <class R> struct Batch : BatchBase {
   bool stale() const {
     if (!planned) return true;
     for (size_t q = 0; q < sims.size(); ++q)
-      if (sims[q]->plan_version != plan_versions[q] || sims[q]->src_version != src_versions[q] || !sims[q]->plan_valid)
+      if (sims[q]->plan_version != plan_versions[q] || sims[q]->src_version != src_versions[q] ||
+          sims[q]->ptr_version != ptr_versions[q] || sims[q]->dft_version != dft_versions[q] || !sims[q]->plan_valid)
         return true;
     return false;
   }
@@ -3277,9 +3285,13 @@ template <class R> struct Batch : BatchBase {
     }
     plan_versions.clear();
     src_versions.clear();
+    ptr_versions.clear();
+    dft_versions.clear();
     for (Fields<R> *f : sims) {
       plan_versions.push_back(f->plan_version);
       src_versions.push_back(f->src_version);
+      ptr_versions.push_back(f->ptr_version);
+      dft_versions.push_back(f->dft_version);
     }
     planned = true;
   }

@@ -24,16 +24,31 @@
  * same calls.  realnum is a template parameter (double, or float for --enable-single).
  *
  * Pinning (what holds this restatement to the reference, beyond reading it):
- *   - the whole step (curl, uPML, E/H update, Lorentz poles, Bloch/metal/magnetic walls,
- *     integrated legacy sources) and field_energy: the reference's 13 known answers
+ *   - the whole step (curl, uPML, E/H update, Lorentz poles, Bloch/metal walls, integrated
+ *     legacy sources) and field_energy: the reference's 13 known answers
  *     (tests/known_results.cpp:137-157), FP64 and FP32, in tests/test_oracle_known_results.py;
- *   - DFT accumulation, flux, volume sources with loop_in_chunks weights and field_energy:
- *     closed forms in tests/test_oracle_closed_forms.py (a lossless cavity conserves energy,
- *     a lossless line carries the same flux through two planes, a current sheet radiates a
- *     flux spectrum proportional to |J(omega)|^2);
+ *   - closed forms in tests/closed_forms.py, asserted in tests/test_oracle_closed_forms.py:
+ *     the DFT's scale, phase convention and E/H time offset, the current injection's dt and
+ *     resolution factors and the flux of a point (a 1D current sheet against the exact
+ *     frequency-domain solution of the Yee lattice, 1e-9); the flux integration weights (a
+ *     periodic 2D line current, 1e-8); the Lorentz and Drude recurrences (relaxation and free
+ *     oscillation against the continuum model, src/susceptibility.cpp:297-307); the magnetic
+ *     and metallic walls (an exact quarter-wave lattice eigenmode); energy conservation;
+ *   - tests/test_oracle_mutations.py: each planted mistake below fails one of those checks;
  *   - add_dft_fields shares the accumulation code the flux checks pin.
  * The reverse mode and batched stepping entry points are not restated (they fail here); the
  * engine's are checked against finite differences and against one-at-a-time runs.
+ *
+ * Planted mistakes (tests/test_oracle_mutations.py): building with -DMUT_<name> plants one
+ * plausible mistake, which the checks of tests/closed_forms.py must detect:
+ *   MUT_DFT_SCALE     the DFT scale loses its dt/sqrt(2 pi) factor
+ *   MUT_DFT_SIGN      the DFT phase exp(i omega t) becomes exp(-i omega t)
+ *   MUT_DFT_HTIME     H is transformed at t dt instead of t dt - dt/2
+ *   MUT_SRC_DT        a current source is injected without its factor dt
+ *   MUT_FLUX_WEIGHT   the flux loses the E chunks' integration weights
+ *   MUT_DRUDE         a Drude pole keeps the restoring term of a Lorentz pole
+ *   MUT_GAMMA         the pole damping uses gamma dt instead of pi gamma dt
+ *   MUT_MAGNETIC      a Low magnetic wall zeroes the metallic wall's plane instead
  */
 #include "../include/meep_amd.h"
 
@@ -724,7 +739,11 @@ template <class R> struct Fields {
     for (int d = 0; d < 3; ++d) {
       if (!gv.has(d)) continue;
       if (here.t[d] == 2 * gv.num[d] && boundaries[High][d] == Metallic) return true;
+#ifdef MUT_MAGNETIC
+      if (boundaries[Low][d] == Magnetic && here.t[d] == 0) return true;
+#else
       if (boundaries[Low][d] == Magnetic && here.t[d] == 1) return true;
+#endif
       if (boundaries[Low][d] == Metallic && here.t[d] == 0) return true;
     }
     return false;
@@ -899,7 +918,11 @@ template <class R> struct Fields {
       const SrcTime &st = srcs[sv.src];
       if (!st.is_integrated && f[c][0] && ((ft == D_stuff && is_electric(sv.c)) || (ft == B_stuff && is_magnetic(sv.c))))
         for (size_t j = 0; j < sv.index.size(); ++j) {
+#ifdef MUT_SRC_DT
+          cdouble A = sv.amp[j] * st.cur_current;
+#else
           cdouble A = sv.amp[j] * st.cur_current * dt;
+#endif
           ptrdiff_t i = sv.index[j];
           f[c][0][i] -= real(A);
           if (!is_real) f[c][1][i] -= imag(A);
@@ -1019,8 +1042,17 @@ template <class R> struct Fields {
       const R omega2pi = 2 * pi * R(su.omega_0), g2pi = R(su.gamma) * 2 * pi;
       const R rdt = R(dt);
       const R omega0dtsqr = omega2pi * omega2pi * rdt * rdt;
+#ifdef MUT_GAMMA
+      (void)g2pi;
+      const R gamma1inv = 1 / (1 + R(su.gamma) * rdt), gamma1 = (1 - R(su.gamma) * rdt);
+#else
       const R gamma1inv = 1 / (1 + g2pi * rdt / 2), gamma1 = (1 - g2pi * rdt / 2);
+#endif
+#ifdef MUT_DRUDE
+      const R omega0dtsqr_denom = omega0dtsqr;
+#else
       const R omega0dtsqr_denom = su.drude ? 0 : omega0dtsqr;
+#endif
       for (int c = 0; c < NC; ++c)
         for (int cmp = 0; cmp < 2; ++cmp) {
           if (!p.P[c][cmp]) continue;
@@ -1040,7 +1072,11 @@ template <class R> struct Fields {
 
   // ---- DFT ----
   void update_dfts() {
+#ifdef MUT_DFT_HTIME
+    double timeE = t * dt, timeH = t * dt;
+#else
     double timeE = t * dt, timeH = t * dt - 0.5 * dt;
+#endif
     for (auto &cp : dft_chunks) {
       DftChunk<R> &ch = *cp;
       if (t % ch.decimation) continue;
@@ -1050,13 +1086,21 @@ template <class R> struct Fields {
       std::vector<std::complex<R>> ph(Nw);
       std::vector<cdouble> ph64(Nw);
       for (int i = 0; i < Nw; ++i) {
+#ifdef MUT_DFT_SIGN
+        ph64[i] = std::polar(1.0, -ch.omega[i] * time) * ch.scale;
+#else
         ph64[i] = std::polar(1.0, ch.omega[i] * time) * ch.scale;
+#endif
         ph[i] = std::complex<R>(ph64[i]);
       }
       int numcmp = f[ch.c][1] ? 2 : 1;
       loop_ivecs_par(gv, ch.is, ch.ie, [&](ptrdiff_t idx, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
         size_t idft = (i1 * L.n[1] + i2) * L.n[2] + i3;
+#ifdef MUT_FLUX_WEIGHT
+        double w = 1.0;
+#else
         double w = ch.include_w ? loop_weight(gv, L, i1, i2, i3, ch.s0, ch.s1, ch.e0, ch.e1, ch.dV0 + ch.dV1 * i2) : 1.0;
+#endif
         R fv[2] = {0, 0};
         for (int cmp = 0; cmp < numcmp; ++cmp) {
           const R *F = f[ch.c][cmp];
@@ -1426,7 +1470,11 @@ template <class R> struct Fields {
       ch->dV0 = dV0;
       ch->dV1 = 0;
       gv.yee2cent_offsets(c, ch->avg1, ch->avg2);
+#ifdef MUT_DFT_SCALE
+      ch->scale = stored_weight * shift_phase * double(decimation);
+#else
       ch->scale = stored_weight * shift_phase * (dt / sqrt(2.0 * pi) * decimation);
+#endif
       ch->include_w = include_w;
       ch->decimation = decimation;
       for (double fr : freq) ch->omega.push_back(2 * pi * fr);

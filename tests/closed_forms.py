@@ -1,0 +1,228 @@
+"""Closed forms that pin the parts of the FDTD step the reference's golden values leave open.
+
+Each check runs one small simulation through the C-ABI of any library (the CPU oracle, a
+planted-mistake build of it, or the MI355X engine) and returns the relative error of the
+run against a value the mathematics fixes; the tests assert on these errors.
+
+* ``line_source_1d``: a current sheet in 1D vacuum.  On the infinite Yee lattice with Courant
+  number S the update equations are solved exactly in the frequency domain: with
+  theta = omega dt / 2 the field is E_j = A exp(i K |j - j0|), sin(K / 2) = sin(theta) / S, and
+  the jump at the sheet gives A = -sin(theta) dt J / (S^2 sin K), where J is the transform of
+  the current density sequence the step subtracts from D.  H between two E points follows
+  from the B update, H_{j+1/2} = S (E_{j+1} - E_j) / (2 i sin theta).  The reference's DFT of a
+  field is dt / sqrt(2 pi) times the sum of its samples times exp(i omega t) at the sample
+  times (E at t dt, H half a step earlier; src/dft.cpp:98,253,270), on the centred grid (the
+  mean of the two E points around a centred point), and the flux is Re(E conj(H)) there.
+  The continuum limit A -> -I / 2 is Maxwell's sheet radiating half its current each way.
+  This pins the DFT scale, its phase convention and the E/H time offset, the current
+  injection's dt and resolution factors, and the flux of one point.
+* ``plane_wave_2d``: the same sheet as a periodic line current in 2D TM; the flux through a
+  transverse line of length Ly is Ly times the 1D flux, which pins the integration weights.
+* ``source_spectrum``: the discrete Gaussian current's transform against the Gaussian
+  integral (the pulse's Fourier transform in closed form).
+* ``pole_free_response``: a uniform field in a periodic cell of a Lorentz or Drude medium
+  with D held fixed.  The reference's continuum model, chi(omega) = sigma omega0^2 /
+  (omega0^2 - omega^2 - i gamma omega) with omega0 = 2 pi f0 and gamma in 2 pi units
+  (src/susceptibility.cpp:297-307), makes P relax to sigma D / (1 + sigma) (Lorentz) or to D
+  (Drude: no restoring term), oscillating at sqrt(omega0^2 (1 + sigma) - (pi gamma)^2)
+  (Lorentz) or sqrt(omega0^2 sigma - (pi gamma)^2) (Drude) with amplitude decay rate
+  pi gamma; the discrete recurrence approaches this as (omega dt)^2.
+* ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
+  magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
+  one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
+  exact eigenvector of the discrete update: started from it, E stays proportional to it, with
+  an amplitude a_k obeying a_{k+1} + a_{k-1} = 2 cos(omega dt) a_k at the lattice frequency
+  sin(omega dt / 2) = S sin(K / 2).
+"""
+
+import math
+
+import numpy as np
+
+from meep_amd.capi import (ARR_F, ARR_P, Dx, Ex, Ez, Hy, High, Low, Magnetic, Metallic, X, Z,
+                           continuous_src_time, fields, gaussian_src_time, structure)
+
+S = 0.5  # Courant number of every check
+
+
+def _dipole(fcen, width, t0, cutoff):
+    """gaussian_src_time::dipole (src/sources.cpp:98-108): the source's time dependence."""
+    def d(t):
+        tt = t - t0
+        v = np.exp(-tt * tt / (2 * width * width)) * np.exp(-2j * math.pi * fcen * tt) / (-2j * math.pi * fcen)
+        return np.where(np.abs(tt).astype(np.float32) > np.float32(cutoff), 0.0, v)
+    return d
+
+
+def current_transform(fcen, fwidth, dt, nsteps, freqs):
+    """sum_n Re(current((n + 1/2) dt)) exp(i omega (n + 1/2) dt): the transform of the current
+    the D update at step n subtracts (times dt), current(t) = (dipole(t + dt) - dipole(t)) / dt
+    (src/meep.hpp:989-991) taken at t = (n + 1/2) dt (src/step.cpp:92)."""
+    width = 1.0 / fwidth
+    d = _dipole(fcen, width, 5 * width, float(np.float32(5 * width)))
+    t = (np.arange(nsteps) + 0.5) * dt
+    cur = ((d(t + dt) - d(t)) / dt).real
+    return np.array([np.sum(cur * np.exp(2j * math.pi * fr * t)) for fr in freqs])
+
+
+def lattice_field(freqs, dt, J, j, j0, half=False):
+    """The exact frequency-domain field of the 1D Yee lattice driven by the current transform
+    J at point j0: E at integer j, or H at j + 1/2 (half=True)."""
+    theta = math.pi * np.asarray(freqs) * dt
+    K = 2 * np.arcsin(np.sin(theta) / S)
+    A = -np.sin(theta) * dt * J / (S * S * np.sin(K))
+
+    def E(jj):
+        return A * np.exp(1j * K * abs(jj - j0))
+    if not half:
+        return E(j)
+    return S * (E(j + 1) - E(j)) / (2j * np.sin(theta))
+
+
+def line_source_1d(lib, a=20, freqs=None):
+    """Errors of the 1D current sheet's DFT fields and flux against the lattice solution."""
+    if freqs is None:
+        freqs = np.linspace(0.88, 1.12, 7)
+    L, z0, fcen, fwidth, T = 80.0, 30.0, 1.0, 0.5, 32.0
+    s = structure(lib, dim=1, size=(0, 0, L), a=a)
+    f = fields(s)
+    f.use_real_fields()
+    f.add_volume_source(Ex, gaussian_src_time(fcen, fwidth), (0, 0, z0), (0, 0, z0))
+    # centred points (i + 1/2) / a after and before the sheet: 5 units away each side
+    zs = {"fwd": z0 + 5.0 + 0.5 / a, "bwd": z0 - 5.0 + 0.5 / a}
+    mons = {k: f.add_dft_fields([Ex, Hy], (0, 0, z), (0, 0, z), freqs, decimation_factor=1) for k, z in zs.items()}
+    flx = f.add_dft_flux(Z, (0, 0, zs["fwd"]), (0, 0, zs["fwd"]), freqs, decimation_factor=1)
+    dt = s.dt
+    nsteps = int(round(T / dt))
+    f.step_n(nsteps)
+    J = a * current_transform(fcen, fwidth, dt, nsteps, freqs)  # the sheet's density: amp * a
+    j0 = int(round(z0 * a))
+    scale = dt / math.sqrt(2 * math.pi)
+    err = {}
+    for k, z in zs.items():
+        i = int(round(z * a - 0.5))
+        e_exp = scale * 0.5 * (lattice_field(freqs, dt, J, i, j0) + lattice_field(freqs, dt, J, i + 1, j0))
+        h_exp = scale * lattice_field(freqs, dt, J, i, j0, half=True)
+        got = {}
+        for q in range(mons[k].num_chunks()):
+            d = mons[k].dft(q)
+            assert d.shape[0] == 1, "a centred point is one DFT point"
+            got[mons[k].component(q)] = d[0]
+        err[f"E_{k}"] = float(np.abs(got[Ex] - e_exp).max() / np.abs(e_exp).max())
+        err[f"H_{k}"] = float(np.abs(got[Hy] - h_exp).max() / np.abs(h_exp).max())
+        if k == "fwd":
+            flux_exp = np.real(e_exp * np.conj(h_exp))
+            err["flux"] = float(np.abs(flx.flux() - flux_exp).max() / np.abs(flux_exp).max())
+    return err
+
+
+def plane_wave_2d(lib, a=10, freqs=None):
+    """Error of the flux of a periodic line current in 2D TM through a transverse line of
+    length Ly against Ly times the 1D lattice solution's flux."""
+    if freqs is None:
+        freqs = np.linspace(0.88, 1.12, 7)
+    Lx, Ly, x0, fcen, fwidth, T = 44.0, 1.3, 14.0, 1.0, 0.5, 28.0
+    s = structure(lib, dim=2, size=(Lx, Ly), a=a)
+    f = fields(s)
+    f.use_real_fields()
+    f.use_bloch((0.0, 0.0))
+    f.set_boundary(Low, X, Metallic)
+    f.set_boundary(High, X, Metallic)
+    f.add_volume_source(Ez, gaussian_src_time(fcen, fwidth), (x0, 0, 0), (x0, Ly, 0))
+    x1 = x0 + 5.0 + 0.5 / a
+    flx = f.add_dft_flux(X, (x1, 0, 0), (x1, Ly, 0), freqs, decimation_factor=1)
+    dt = s.dt
+    nsteps = int(round(T / dt))
+    f.step_n(nsteps)
+    J = a * current_transform(fcen, fwidth, dt, nsteps, freqs)
+    j0, i = int(round(x0 * a)), int(round(x1 * a - 0.5))
+    scale = dt / math.sqrt(2 * math.pi)
+    e = scale * 0.5 * (lattice_field(freqs, dt, J, i, j0) + lattice_field(freqs, dt, J, i + 1, j0))
+    h = scale * lattice_field(freqs, dt, J, i, j0, half=True)
+    # TM along x: Ez plays 1D Ex and Hy plays -(1D Hy); the X flux is -Re(Ez conj(Hy))
+    flux_exp = Ly * np.real(e * np.conj(h))
+    return {"flux": float(np.abs(flx.flux() - flux_exp).max() / np.abs(flux_exp).max())}
+
+
+def source_spectrum(dt=0.025, fcen=1.0, fwidth=0.5, freqs=None):
+    """Error of the discrete current transform against the Gaussian integral:
+    dt * sum ~ D(omega) (exp(-i omega dt) - 1) / dt with D the dipole's transform
+    exp(i omega t0) w sqrt(2 pi) exp(-w^2 (omega - omega0)^2 / 2) / (-i omega0)."""
+    if freqs is None:
+        freqs = np.linspace(0.88, 1.12, 7)
+    w = 1.0 / fwidth
+    t0 = 5 * w
+    n = int(round(12 * w / dt))
+    got = dt * current_transform(fcen, fwidth, dt, n, freqs)
+    om, om0 = 2 * math.pi * np.asarray(freqs), 2 * math.pi * fcen
+    D = np.exp(1j * om * t0) * w * math.sqrt(2 * math.pi) * np.exp(-w * w * (om - om0) ** 2 / 2) / (-1j * om0)
+    exp = 0.5 * D * (np.exp(-1j * om * dt) - 1) / dt  # Re(): half the positive-frequency part
+    return float(np.abs(got - exp).max() / np.abs(exp).max())
+
+
+def pole_free_response(lib, drude, a=100, f0=1.0, gamma=0.3, sigma=2.0, T=20.0):
+    """Errors of a pole's relaxation against the continuum model: the final P against its
+    static value, and the oscillation frequency and decay rate of P against the model's."""
+    s = structure(lib, dim=1, size=(0, 0, 3.0 / a), a=a)
+    s.add_susceptibility(lambda x, y, z: sigma, f0, gamma, drude=drude)
+    f = fields(s)
+    f.use_real_fields()
+    f.use_bloch((0.0,))
+    # a source of zero amplitude allocates the fields without driving them
+    f.add_volume_source(Ex, continuous_src_time(1.0), (0, 0, 0.5 / a), (0, 0, 0.5 / a), amp=0.0)
+    one = np.ones(s.shape)
+    f.set_array(Dx, 0, ARR_F, one)
+    f.set_array(Ex, 0, ARR_F, one)
+    dt = s.dt
+    n = int(round(T / dt))
+    P = np.empty(n)
+    for k in range(n):
+        f.step()
+        P[k] = f.get_array(Ex, 0, ARR_P, 0)[0][1]
+    p_inf = 1.0 if drude else sigma / (1.0 + sigma)
+    om0 = 2 * math.pi * f0
+    rate = math.pi * gamma
+    om = math.sqrt(om0 * om0 * (sigma if drude else 1.0 + sigma) - rate * rate)
+    # the model's free response about p_inf, fitted as x_{k+1} = alpha x_k + beta x_{k-1}
+    x = P - p_inf
+    M = np.stack([x[1:-1], x[:-2]], axis=1)
+    alpha, beta = np.linalg.lstsq(M, x[2:], rcond=None)[0]
+    lam = np.roots([1.0, -alpha, -beta])
+    lam = lam[np.argmax(lam.imag)]
+    om_fit = abs(np.angle(lam)) / dt
+    rate_fit = -math.log(abs(lam)) / dt
+    # after T the free response has decayed by exp(-pi gamma T) (2.6e-9 here)
+    return {"p_final": float(abs(P[-1] - p_inf) / p_inf),
+            "frequency": abs(om_fit - om) / om, "decay": abs(rate_fit - rate) / rate}
+
+
+def magnetic_cavity(lib, a=20, L=2.0, m=2, nsteps=300):
+    """Errors of a quarter-wave cavity mode between a magnetic Low wall and a metallic High
+    wall: the field's departure from the mode shape and the amplitude's departure from the
+    lattice frequency's recurrence (both relative to the amplitude)."""
+    s = structure(lib, dim=1, size=(0, 0, L), a=a)
+    f = fields(s)
+    f.use_real_fields()
+    f.set_boundary(Low, Z, Magnetic)
+    f.set_boundary(High, Z, Metallic)
+    f.add_volume_source(Ex, continuous_src_time(1.0), (0, 0, 0.5), (0, 0, 0.5), amp=0.0)
+    n = int(round(L * a))
+    K = (m + 0.5) * math.pi / (n - 0.5)
+    j = np.arange(n + 1)
+    mode = np.cos(K * (j - 0.5))
+    mode[n] = 0.0
+    f.set_array(Dx, 0, ARR_F, mode)
+    f.set_array(Ex, 0, ARR_F, mode)
+    dt = s.dt
+    om = 2 * math.asin(S * math.sin(K / 2)) / dt
+    amp, resid = [], 0.0
+    owned = slice(1, n + 1)  # E's owned points (index 0 is the halo plane)
+    for _ in range(nsteps):
+        f.step()
+        e = f.get_array(Ex, 0)[0]
+        c = float(np.dot(e[owned], mode[owned]) / np.dot(mode[owned], mode[owned]))
+        amp.append(c)
+        resid = max(resid, float(np.abs(e[owned] - c * mode[owned]).max()))
+    amp = np.array(amp)
+    rec = np.abs(amp[2:] + amp[:-2] - 2 * math.cos(om * dt) * amp[1:-1]).max()
+    return {"shape": resid / np.abs(amp).max(), "frequency": float(rec / np.abs(amp).max())}

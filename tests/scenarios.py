@@ -200,10 +200,57 @@ SCENARIOS = {
 }
 
 FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)
+_DIR = {Ex: 0, Ey: 1, Ez: 2, Hx: 0, Hy: 1, Hz: 2, Dx: 0, Dy: 1, Dz: 2, Bx: 0, By: 1, Bz: 2}
+
+
+def yee_shift(s, c, d):
+    """Yee shift (0 or 1 half pixel) of component c along direction d: electric components
+    are shifted along their own direction, magnetic ones along the others."""
+    has = {1: (2,), 2: (0, 1), 3: (0, 1, 2)}[s.dim]
+    if d not in has:
+        return 0
+    electric = c in (Ex, Ey, Ez, Dx, Dy, Dz)
+    return int(d == _DIR[c]) if electric else int(d != _DIR[c])
+
+
+def pml_active(s, d):
+    """Half-pixel indices 0 .. 2n+1 along direction d where a uPML profile of the structure's
+    pml terms is non-trivial: the reference's pml_x > 0 (src/structure.cpp:625-636)."""
+    n = s.num[d]
+    act = np.zeros(2 * n + 2, dtype=bool)
+    a = s.a
+    for dd, side, dx, _R in s.pml:
+        if dd != d or dx <= 0 or n <= 1:
+            continue
+        bloc = n * (1.0 / a) if side == High else 0.0
+        for i in range(2 * n + 2):
+            here = i * 0.5 / a
+            x = 0.5 / a * (int(dx * (2 * a) + 0.5) - int(abs(bloc - here) * (2 * a) + 0.5))
+            act[i] = act[i] or x > 0
+    return act
+
+
+def aux_mask(s, c, d):
+    """Points of a component's array (reference layout) where the uPML profile along d acts:
+    the points where f_u (d = dsigu) or f_w (d = dsigw) take part in the update.  Elsewhere
+    the reference keeps no such array (its uPML lives in separate chunks) and the value is
+    never read."""
+    axes = {1: (2,), 2: (0, 1), 3: (0, 1, 2)}[s.dim]
+    if d not in axes:
+        return np.zeros(s.shape, dtype=bool)
+    act = pml_active(s, d)
+    j = np.arange(s.num[d] + 1)
+    m1 = act[2 * j + yee_shift(s, c, d)]
+    shape = [1] * len(axes)
+    shape[axes.index(d)] = m1.size
+    return np.broadcast_to(m1.reshape(shape), s.shape)
 
 
 def state(f, s, npol=0):
-    """Every allocated field and polarisation array, keyed (name, component, cmp)."""
+    """Every allocated field and polarisation array, keyed (name, component, cmp), plus the
+    uPML auxiliaries f_u (of B and D, along dsigu = the direction after the next,
+    src/step_db.cpp:56-59) and f_w (of E and H, along their own direction,
+    src/update_eh.cpp:162-163) at the points where their profile acts."""
     out = {}
     for c in FIELD_COMPONENTS:
         if not f.have_component(c):
@@ -212,6 +259,14 @@ def state(f, s, npol=0):
             arr, ok = f.get_array(c, cmp)
             if ok:
                 out[("f", c, cmp)] = arr
+            for kind, name in ((1, "f_u"), (2, "f_w")):
+                arr, ok = f.get_array(c, cmp, kind)
+                if not ok:
+                    continue
+                d = (_DIR[c] + 2) % 3 if kind == 1 else _DIR[c]
+                m = aux_mask(s, c, d)
+                if m.any():
+                    out[(name, c, cmp)] = np.where(m, arr, 0.0)
             for ip in range(npol):
                 for kind, name in ((3, "P"), (4, "Pp")):
                     arr, ok = f.get_array(c, cmp, kind, ip)

@@ -122,3 +122,53 @@ def test_batched_chunks_continue_on_their_own(gpu_lib, single):
         for x, y in zip(fl1, fl2):
             fx, fy = x.flux(), y.flux()
             assert np.abs(fx - fy).max() <= tol * max(np.abs(fx).max(), 1e-30)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_batch_after_odd_solo_steps_with_poles(gpu_lib, single):
+    """A chunk with poles stepped on its own swaps its P / P_prev arrays after every step; a
+    batch planned before such steps must re-plan, or it would read P_prev as P.  Batch, one
+    solo step, batch again, against chunks stepped on their own throughout (P arrays too)."""
+    print(f"[batch] batch, odd solo steps, batch single={single}", flush=True)
+    solo = mixed_physics(gpu_lib, single)
+    for s, f, _ in solo:
+        f.step_n(71)
+    mixed = mixed_physics(gpu_lib, single)
+    b = batch([f for _, f, _ in mixed], use_graph=True, autotune=False)
+    b.step_n(30)
+    for s, f, _ in mixed:
+        f.step_n(1)
+    b.step_n(40)
+    tol = 1e-5 if single else 1e-11
+    for (s1, f1, _), (s2, f2, _) in zip(solo, mixed):
+        assert f1.t == f2.t == 71
+        a, c = state(f1, s1, npol=2), state(f2, s2, npol=2)
+        assert set(a) == set(c)
+        for key in a:
+            scale = max(np.abs(a[key]).max(), 1e-30)
+            assert np.abs(a[key] - c[key]).max() <= tol * scale, key
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_monitor_added_between_batch_calls(gpu_lib, single):
+    """A flux monitor added to a member after a first batch call is accumulated by the next
+    one (the batch re-plans) exactly as on a chunk stepped alone."""
+    print(f"[batch] monitor added between batch calls single={single}", flush=True)
+    solo = build_all(gpu_lib, single)
+    for s, f, _ in solo:
+        f.step_n(50)
+    late_solo = [f.add_dft_flux(X if s.dim == 2 else Z, (1.5, 0.5, 0.2), (1.5, 2.5, 0.2) if s.dim == 2 else (1.5, 1.0, 0.2),
+                                linspace(0.3, 0.6, 8), decimation_factor=1) for s, f, _ in solo]
+    for s, f, _ in solo:
+        f.step_n(70)
+    mixed = build_all(gpu_lib, single)
+    b = batch([f for _, f, _ in mixed], use_graph=True, autotune=False)
+    b.step_n(50)
+    late = [f.add_dft_flux(X if s.dim == 2 else Z, (1.5, 0.5, 0.2), (1.5, 2.5, 0.2) if s.dim == 2 else (1.5, 1.0, 0.2),
+                           linspace(0.3, 0.6, 8), decimation_factor=1) for s, f, _ in mixed]
+    b.step_n(70)
+    tol = 1e-5 if single else 1e-11
+    for x, y in zip(late_solo, late):
+        fx, fy = x.flux(), y.flux()
+        assert np.abs(fx).max() > 0
+        assert np.abs(fx - fy).max() <= tol * np.abs(fx).max()

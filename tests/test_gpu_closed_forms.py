@@ -1,0 +1,27 @@
+"""The closed forms of tests/closed_forms.py on the MI355X engine in FP64, at the tolerances
+the oracle meets: the engine's DFT scale, phase and E/H time offset, current injection,
+flux weights, Lorentz and Drude poles and magnetic/metallic walls are held to the
+mathematics directly, not only to the oracle."""
+
+import pytest
+
+import test_oracle_closed_forms as T
+
+pytestmark = pytest.mark.gpu
+
+
+def test_line_source_matches_lattice_solution_gpu(gpu_lib):
+    T.check_line_source(gpu_lib)
+
+
+def test_plane_wave_flux_weights_gpu(gpu_lib):
+    T.check_plane_wave(gpu_lib)
+
+
+@pytest.mark.parametrize("drude", [False, True], ids=["lorentz", "drude"])
+def test_pole_relaxation_and_oscillation_gpu(gpu_lib, drude):
+    T.check_pole(gpu_lib, drude)
+
+
+def test_magnetic_and_metallic_walls_quarter_wave_mode_gpu(gpu_lib):
+    T.check_cavity(gpu_lib)

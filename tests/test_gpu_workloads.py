@@ -54,3 +54,31 @@ def test_pml_lorentz_dft_fp32_precision(gpu_lib, oracle_lib):
     flux_err = np.abs(gf - of).max() / np.abs(of).max()
     print(f"[workload] dft max rel err {dft_err:.2e}, flux max rel err {flux_err:.2e}", flush=True)
     assert dft_err <= 1e-5 and flux_err <= 1e-5
+
+
+def test_pml_lorentz_dft_fp32_against_fp64_fixture(gpu_lib):
+    """FP32 fields with compensated FP32 accumulators on the 3D uPML + Lorentz-Drude + DFT
+    configuration at 192 x 192 x 96 cells and 2000 steps, against the FP64 oracle's values in
+    tests/golden/pml_lorentz_dft_192x192x96.npz (written by make_pml_lorentz_dft_fixture.py):
+    the z-flux spectrum and the sampled DFT fields within 1e-5 of their maxima."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_pml_lorentz_dft_fixture as G
+    ref = np.load(G.OUT)
+    print(f"[workload] 3D uPML Lorentz-Drude DFT {G.N}, {G.STEPS} steps, FP32 + Kahan vs FP64 fixture", flush=True)
+    s, f, df = bench.build_pml_lorentz_dft(gpu_lib, True, n=G.N, npml=G.NPML, nfreq=G.NFREQ,
+                                           dft_kind=capi.DFT_KAHAN32)
+    L = [c / bench.PLD_A for c in G.N]
+    zf = G.flux_plane()
+    fl = f.add_dft_flux(capi.Z, (0, 0, zf), (L[0], L[1], zf), ref["flux_freqs"], decimation_factor=1)
+    f.step_n(G.STEPS)
+    flux = fl.flux()
+    flux_err = np.abs(flux - ref["flux"]).max() / np.abs(ref["flux"]).max()
+    dft_err = 0.0
+    for i in range(df.num_chunks()):
+        assert df.component(i) == int(ref[f"dft{i}_component"])
+        d = df.dft(i)[ref[f"dft{i}_points"]]
+        dft_err = max(dft_err, np.abs(d - ref[f"dft{i}_values"]).max() / float(ref[f"dft{i}_max"]))
+    print(f"[workload] flux max rel err {flux_err:.2e}, sampled dft max rel err {dft_err:.2e}", flush=True)
+    assert flux_err <= 1e-5 and dft_err <= 1e-5

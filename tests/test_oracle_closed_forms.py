@@ -1,14 +1,24 @@
 """Properties the mathematics fixes, checked on the oracle where the reference's own golden
-values do not reach: its DFT accumulation, flux and field-energy code.
+values do not reach (tests/closed_forms.py states each closed form):
 
-* Energy conservation: in a closed metallic cavity without loss, fields::field_energy is
-  constant once the source is off (to the leapfrog scheme's O((omega dt)^2) oscillation, far
-  below any dropped term or wrong sign, which changes it at O(1)).
-* Lossless 1D transport: a pulse crosses two flux planes unchanged, so their flux spectra agree
-  (a wrong weight, sign or E/H time offset in the flux breaks this).
-* Source spectrum: a current sheet in 1D vacuum radiates a flux proportional to |J(omega)|^2, so
-  the ratio of the flux spectrum to |J(omega)|^2 (J's discrete Fourier transform, computed
-  here from the Gaussian's closed form) is flat across the band.
+* DFT fields of a current sheet in 1D against the exact frequency-domain solution of the Yee
+  lattice: DFT scale dt / sqrt(2 pi), phase exp(+i omega t), E at t dt and H at t dt - dt / 2,
+  the current's factors dt and a (a resolution-a point source is a density a * amp), flux of a
+  point.  Observed agreement about 1e-9.
+* Flux of a periodic line current in 2D through a transverse line: Ly times the 1D flux (the
+  integration weights).  Observed about 1e-8.
+* The discrete Gaussian current's transform against the Gaussian integral.
+* Lorentz and Drude poles: relaxation to the static value and free oscillation at the
+  continuum model's frequency and damping (to O((omega dt)^2)).
+* A quarter-wave cavity between a magnetic and a metallic wall: the lattice eigenmode
+  evolves exactly (both walls' placement).
+* Energy conservation in a lossless metallic cavity (field_energy).
+* Transport and spectrum shape: equal flux through two planes of a lossless 1D line, and a
+  flux spectrum proportional to |J(omega)|^2.  These two are scale-invariant: they do not
+  pin the DFT's normalisation, phase or E/H time offset (the first three checks do).
+
+tests/test_oracle_mutations.py shows that each planted mistake in the oracle fails one of
+these checks.
 """
 
 import math
@@ -16,8 +26,62 @@ import math
 import numpy as np
 import pytest
 
+import closed_forms as CF
 from meep_amd.capi import (Ex, Ez, Z, fields, gaussian_src_time, linspace, pml, structure, volone,
                            voltwo)
+
+TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
+       "pole_decay": 1e-4, "cavity": 1e-10}
+
+
+def check_line_source(lib):
+    err = CF.line_source_1d(lib)
+    assert max(err.values()) <= TOL["line"], err
+
+
+def check_plane_wave(lib):
+    err = CF.plane_wave_2d(lib)
+    assert err["flux"] <= TOL["plane"], err
+
+
+def check_pole(lib, drude):
+    err = CF.pole_free_response(lib, drude)
+    assert err["p_final"] <= TOL["p_final"], err
+    assert err["frequency"] <= TOL["pole_freq"], err
+    assert err["decay"] <= TOL["pole_decay"], err
+
+
+def check_cavity(lib):
+    err = CF.magnetic_cavity(lib)
+    assert err["shape"] <= TOL["cavity"] and err["frequency"] <= TOL["cavity"], err
+
+
+def test_line_source_matches_lattice_solution(oracle_lib):
+    check_line_source(oracle_lib)
+
+
+def test_plane_wave_flux_weights(oracle_lib):
+    check_plane_wave(oracle_lib)
+
+
+def test_gaussian_current_spectrum():
+    assert CF.source_spectrum() <= TOL["spectrum"]
+
+
+@pytest.mark.parametrize("drude", [False, True], ids=["lorentz", "drude"])
+def test_pole_relaxation_and_oscillation(oracle_lib, drude):
+    check_pole(oracle_lib, drude)
+
+
+def test_pole_frequency_converges_as_dt_squared(oracle_lib):
+    # the recurrence's frequency error is the leapfrog's O((omega dt)^2): a quarter at half dt
+    e1 = CF.pole_free_response(oracle_lib, False, a=100)["frequency"]
+    e2 = CF.pole_free_response(oracle_lib, False, a=200, T=8.0)["frequency"]
+    assert 0.2 < e2 / e1 < 0.3, (e1, e2)
+
+
+def test_magnetic_and_metallic_walls_quarter_wave_mode(oracle_lib):
+    check_cavity(oracle_lib)
 
 
 def _cavity_energy_spread(lib, a):
@@ -71,15 +135,8 @@ def test_flux_spectrum_follows_the_source_spectrum(oracle_lib):
     n = int(70.0 / dt)
     f.step_n(n)
     flux = fl.flux()
-    # the Gaussian current of python/source.py semantics: width 1/df, peak at 5 widths, the
-    # current being the discrete time derivative of the dipole exp(-(t-t0)^2/2w^2) e^{-iwt}/(-iw)
-    w, t0, fc = 2.0, 10.0, 1.0
-    def dipole(t):
-        return np.exp(-((t - t0) ** 2) / (2 * w * w)) * np.exp(-2j * math.pi * fc * t) / (-2j * math.pi * fc)
-    t = (np.arange(n) + 0.5) * dt
-    cur = (dipole(t + dt) - dipole(t)) / dt
     freqs = linspace(0.6, 1.4, 20)
-    J = np.array([np.sum(cur.real * np.exp(2j * math.pi * fr * t)) * dt for fr in freqs])
+    J = CF.current_transform(1.0, 0.5, dt, n, freqs) * dt
     # over the band where the pulse has its energy (|J|^2 above 1e-2 of its peak; outside it
     # the spectrum is 1e-6 of the peak and truncation of the pulse dominates)
     band = np.abs(J) ** 2 > 1e-2 * np.max(np.abs(J) ** 2)

@@ -1,0 +1,41 @@
+"""Each closed-form check of tests/closed_forms.py detects the mistake it is there to catch:
+the oracle built with one planted mistake (-DMUT_<name>, oracle/Makefile `mutants`) fails
+the check, while the unmodified oracle passes it (test_oracle_closed_forms.py).  The mistakes
+are the plausible ones a restatement of the reference could make: a dropped DFT scale, a
+flipped DFT phase sign, H transformed at the E time, a current injected without dt, flux
+weights dropped, a Drude pole with a restoring term, pole damping off by 2 pi, a magnetic
+wall at the wrong plane."""
+
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+import test_oracle_closed_forms as T
+from meep_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CHECKS = {
+    "DFT_SCALE": T.check_line_source,
+    "DFT_SIGN": T.check_line_source,
+    "DFT_HTIME": T.check_line_source,
+    "SRC_DT": T.check_line_source,
+    "FLUX_WEIGHT": T.check_plane_wave,
+    "DRUDE": lambda lib: T.check_pole(lib, True),
+    "GAMMA": lambda lib: T.check_pole(lib, False),
+    "MAGNETIC": T.check_cavity,
+}
+
+
+@pytest.fixture(scope="module")
+def mutants():
+    subprocess.check_call(["make", "-s", "-j8", "-C", os.path.join(ROOT, "oracle"), "mutants"])
+    return {m: capi.bind(ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", f"libmeep_oracle_mut_{m}.so")))
+            for m in CHECKS}
+
+
+@pytest.mark.parametrize("mutant", sorted(CHECKS))
+def test_planted_mistake_is_detected(mutants, mutant):
+    with pytest.raises(AssertionError):
+        CHECKS[mutant](mutants[mutant])
