@@ -2,7 +2,7 @@
 fields for the 3D uPML + Lorentz-Drude + DFT configuration at half the benchmark's size
 (192 x 192 x 96 cells, 8-cell uPML, 2000 steps, 32 frequencies), the reference against which
 the engine's FP32 fields with compensated FP32 accumulators are held.  Run in the CPU
-container (about two minutes on 8 cores):
+container (about nine minutes on 8 cores):
 
     python tests/golden/make_pml_lorentz_dft_fixture.py
 """
