@@ -35,7 +35,8 @@ def flux_plane(n=N):
 
 
 def sample_points(npts, k=400, seed=1):
-    return np.sort(np.random.default_rng(seed).choice(npts, size=min(k, npts), replace=False))
+    """Every fourth of k seeded distinct points of a chunk, in increasing order."""
+    return np.sort(np.random.default_rng(seed).choice(npts, size=min(k, npts), replace=False))[::4]
 
 
 def main():
