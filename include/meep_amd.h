@@ -178,11 +178,14 @@ int mp_fields_set_dft_accumulator(mp_fields *f, int kind);
 /* ---- DFT flux ----------------------------------------------------------------------- */
 
 /* fields::add_dft_flux(direction, volume, const double *freq, Nfreq, use_symmetry,
-   centered_grid = true, decimation_factor) — src/meep.hpp:2056-2058, src/dft.cpp:602-650,835-842.
-   decimation_factor 0 picks the reference's automatic factor (src/dft.cpp:196-217). */
+   centered_grid, decimation_factor) — src/meep.hpp:2056-2058, src/dft.cpp:602-650,835-842.
+   With the identity symmetry use_symmetry changes nothing.  centered_grid 0 accumulates each
+   component on its own Yee grid without averaging (src/dft.cpp:91-94,233); flux() then needs
+   the E and H chunks to hold the same number of points.  decimation_factor 0 picks the
+   reference's automatic factor (src/dft.cpp:196-217). */
 mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin,
                                     const double *vmax, const double *freq, int nfreq,
-                                    int decimation_factor);
+                                    int use_symmetry, int centered_grid, int decimation_factor);
 void mp_dft_flux_delete(mp_dft_flux *fl);
 /* dft_flux::flux() — src/dft.cpp:542-556.  out has nfreq entries. */
 int mp_dft_flux_flux(mp_dft_flux *fl, double *out);
@@ -196,12 +199,14 @@ int mp_dft_flux_get_dft(mp_dft_flux *fl, int which, int i, double *out);
 /* ---- DFT fields ---------------------------------------------------------------------- */
 
 /* fields::add_dft_fields(components, num_components, volume, const double *freq, Nfreq,
-   use_centered_grid = true, decimation_factor) — src/meep.hpp:2085-2090, src/dft.cpp:913-927:
-   per component, accumulators on the centred grid without integration weights.  Chunks are
+   use_centered_grid, decimation_factor) — src/meep.hpp:2085-2090, src/dft.cpp:913-927:
+   per component, accumulators without integration weights, on the centred grid (averaged
+   from the Yee grid) or, use_centered_grid 0, on the component's own Yee grid.  Chunks are
    listed in the reference's order (each component's chunks prepended). */
 mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *components, int num_components,
                                         const double *vmin, const double *vmax,
-                                        const double *freq, int nfreq, int decimation_factor);
+                                        const double *freq, int nfreq, int use_centered_grid,
+                                        int decimation_factor);
 void mp_dft_fields_delete(mp_dft_fields *df);
 int mp_dft_fields_num_chunks(mp_dft_fields *df);
 long mp_dft_fields_chunk_points(mp_dft_fields *df, int i);

@@ -11,6 +11,9 @@
 
 #include "meep_amd.h"
 
+#include <math.h>
+
+#include <chrono>
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
@@ -21,6 +24,52 @@
 namespace meep {
 
 typedef std::complex<double> cdouble;
+
+// realnum (src/meep.hpp:43-46): float when built with MEEP_AMD_SINGLE (the reference's
+// --enable-single), which also makes every structure single precision (realnum_is_float)
+#ifdef MEEP_AMD_SINGLE
+typedef float realnum;
+#else
+typedef double realnum;
+#endif
+const double pi = 3.141592653589793238462643383276; // src/meep.hpp:53
+const double infinity = HUGE_VAL;                    // src/meep.hpp:55
+// src/meep.hpp:50-51: 0 suppresses the library's non-error messages (this layer prints none)
+inline int verbosity = 1;
+
+// ---- process helpers (src/meep/mympi.hpp:29-120) for a single process: rank 0 of 1 ----
+inline double wall_time() { // src/mympi.cpp:146-152
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+class initialize { // src/meep/mympi.hpp:31-39
+public:
+  initialize(int &argc, char **&argv) : t_start(wall_time()) { (void)argc, (void)argv; }
+  double elapsed_time() { return wall_time() - t_start; }
+
+private:
+  double t_start;
+};
+inline void all_wait() {}
+inline int count_processors() { return 1; }
+inline int my_rank() { return 0; }
+inline bool am_really_master() { return true; }
+inline int am_master() { return 1; }
+inline bool with_mpi() { return false; }
+inline void master_printf(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+inline void master_printf(const char *fmt, ...) { // src/mympi.cpp:523-533
+  va_list ap;
+  va_start(ap, fmt);
+  std::vfprintf(stdout, fmt, ap);
+  va_end(ap);
+  std::fflush(stdout);
+}
+inline void master_printf_stderr(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+inline void master_printf_stderr(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vfprintf(stderr, fmt, ap);
+  va_end(ap);
+}
 
 // src/meep/vec.hpp:31-79 and src/meep.hpp:1594 (same numeric values)
 enum component { Ex = 0, Ey, Er, Ep, Ez, Hx, Hy, Hr, Hp, Hz, Dx, Dy, Dr, Dp, Dz, Bx, By, Br, Bp, Bz };
@@ -41,13 +90,15 @@ inline void check(int rc) {
   if (rc < 0) throw std::runtime_error(mp_last_error());
 }
 
-// vec (src/meep/vec.hpp:621-750), Cartesian only
+// vec (src/meep/vec.hpp:621-750); cylindrical vectors exist as values (veccyl) but no GPU
+// path accepts them
 class vec {
 public:
   vec() : dim(D1) { t[0] = t[1] = t[2] = 0; }
   explicit vec(double zz) : dim(D1) { t[0] = t[1] = 0, t[2] = zz; }
   vec(double xx, double yy) : dim(D2) { t[0] = xx, t[1] = yy, t[2] = 0; }
   vec(double xx, double yy, double zz) : dim(D3) { t[0] = xx, t[1] = yy, t[2] = zz; }
+  friend vec veccyl(double rr, double zz);
   double x() const { return t[0]; }
   double y() const { return t[1]; }
   double z() const { return t[2]; }
@@ -59,6 +110,11 @@ public:
 private:
   double t[3];
 };
+inline vec veccyl(double rr, double zz) { // src/meep/vec.hpp:744-749 (r kept in x)
+  vec v(rr, 0.0, zz);
+  v.dim = Dcyl;
+  return v;
+}
 
 // grid_volume (src/meep/vec.hpp:1014-1177): size and resolution of a cell with its low
 // corner at the origin
@@ -244,6 +300,18 @@ private:
 };
 constexpr int E_stuff = 0;
 
+// linspace (src/dft.cpp:30-40)
+inline std::vector<double> linspace(double freq_min, double freq_max, size_t Nfreq) {
+  std::vector<double> fr(Nfreq);
+  if (Nfreq <= 1) {
+    if (Nfreq == 1) fr[0] = (freq_min + freq_max) * 0.5;
+    return fr;
+  }
+  const double df = (freq_max - freq_min) / (Nfreq - 1);
+  for (size_t i = 0; i < Nfreq; ++i) fr[i] = freq_min + double(i) * df;
+  return fr;
+}
+
 // dft_flux (src/meep.hpp:1209-1255)
 class dft_flux {
 public:
@@ -296,8 +364,18 @@ private:
 // fields (src/meep.hpp:1716-2320)
 class fields {
 public:
-  explicit fields(structure *s, double m = 0, double beta = 0) : gv(s->gv()) {
+  // fields(structure*, m, beta, zero_fields_near_cylorigin, loop_tile_base_db,
+  // loop_tile_base_eh, bfast_scaled_k) (src/meep.hpp:1740-1742): the loop tile bases tune the
+  // reference's CPU loops and have no meaning here; m, beta and bfast_scaled_k select
+  // formulations without a GPU path
+  explicit fields(structure *s, double m = 0, double beta = 0, bool zero_fields_near_cylorigin = true,
+                  int loop_tile_base_db = 0, int loop_tile_base_eh = 0,
+                  std::vector<double> bfast_scaled_k = {0, 0, 0})
+      : gv(s->gv()) {
+    (void)zero_fields_near_cylorigin, (void)loop_tile_base_db, (void)loop_tile_base_eh;
     if (m != 0 || beta != 0) meep::abort("only m = 0, beta = 0 are supported by the GPU engine");
+    for (double kq : bfast_scaled_k)
+      if (kq != 0) meep::abort("the bfast formulation (bfast_scaled_k != 0) is not supported by the GPU engine");
     p = mp_fields_new(s->handle());
     if (!p) throw std::runtime_error(mp_last_error());
   }
@@ -348,30 +426,48 @@ public:
     check(mp_fields_get_field(p, c, loc.data(), &re, &im));
     return cdouble(re, im);
   }
-  // add_dft_flux(direction, volume, freq_min, freq_max, Nfreq) (src/meep.hpp:2044-2049)
-  dft_flux add_dft_flux(direction d, const volume &where, double freq_min, double freq_max, int Nfreq) {
-    std::vector<double> fr(Nfreq);
-    double df = Nfreq <= 1 ? 0.0 : (freq_max - freq_min) / (Nfreq - 1);
-    for (int i = 0; i < Nfreq; ++i) fr[i] = Nfreq <= 1 ? (freq_min + freq_max) * 0.5 : freq_min + i * df;
-    mp_dft_flux *h = mp_fields_add_dft_flux(p, d, where.get_min_corner().data(), where.get_max_corner().data(), fr.data(),
-                                            Nfreq, 0);
+  // add_dft_flux(direction, volume, freq, Nfreq, use_symmetry, centered_grid,
+  // decimation_factor) and its overloads (src/meep.hpp:2044-2059); with the identity symmetry
+  // use_symmetry changes nothing
+  dft_flux add_dft_flux(direction d, const volume &where, const double *freq, size_t Nfreq, bool use_symmetry = true,
+                        bool centered_grid = true, int decimation_factor = 0) {
+    mp_dft_flux *h = mp_fields_add_dft_flux(p, d, where.get_min_corner().data(), where.get_max_corner().data(), freq,
+                                            int(Nfreq), use_symmetry, centered_grid, decimation_factor);
     if (!h) throw std::runtime_error(mp_last_error());
     fluxes.push_back(h);
-    return dft_flux(h, Nfreq);
+    return dft_flux(h, int(Nfreq));
   }
-  // add_dft_fields(components, num_components, volume, freq_min, freq_max, Nfreq)
-  // (src/meep.hpp:2078-2084)
-  dft_fields add_dft_fields(const component *cs, int num_components, const volume &where, double freq_min, double freq_max,
-                            int Nfreq) {
-    std::vector<double> fr(Nfreq);
-    double df = Nfreq <= 1 ? 0.0 : (freq_max - freq_min) / (Nfreq - 1);
-    for (int i = 0; i < Nfreq; ++i) fr[i] = Nfreq <= 1 ? (freq_min + freq_max) * 0.5 : freq_min + i * df;
+  dft_flux add_dft_flux(direction d, const volume &where, const std::vector<double> &freq, bool use_symmetry = true,
+                        bool centered_grid = true, int decimation_factor = 0) {
+    return add_dft_flux(d, where, freq.data(), freq.size(), use_symmetry, centered_grid, decimation_factor);
+  }
+  dft_flux add_dft_flux(direction d, const volume &where, double freq_min, double freq_max, int Nfreq,
+                        bool use_symmetry = true, bool centered_grid = true, int decimation_factor = 0) {
+    return add_dft_flux(d, where, linspace(freq_min, freq_max, Nfreq), use_symmetry, centered_grid, decimation_factor);
+  }
+  // add_dft_fields(components, num_components, volume, freq, Nfreq, use_centered_grid,
+  // decimation_factor, persist) and overloads (src/meep.hpp:2078-2093); the accumulators live
+  // as long as the fields object, so persist changes nothing
+  dft_fields add_dft_fields(const component *cs, int num_components, const volume &where, const double *freq, size_t Nfreq,
+                            bool use_centered_grid = true, int decimation_factor = 0, bool persist = false) {
+    (void)persist;
     std::vector<int> ci(cs, cs + num_components);
     mp_dft_fields *h = mp_fields_add_dft_fields(p, ci.data(), num_components, where.get_min_corner().data(),
-                                                where.get_max_corner().data(), fr.data(), Nfreq, 0);
+                                                where.get_max_corner().data(), freq, int(Nfreq), use_centered_grid,
+                                                decimation_factor);
     if (!h) throw std::runtime_error(mp_last_error());
     dft_fields_list.push_back(h);
-    return dft_fields(h, Nfreq);
+    return dft_fields(h, int(Nfreq));
+  }
+  dft_fields add_dft_fields(const component *cs, int num_components, const volume &where, const std::vector<double> &freq,
+                            bool use_centered_grid = true, int decimation_factor = 0, bool persist = false) {
+    return add_dft_fields(cs, num_components, where, freq.data(), freq.size(), use_centered_grid, decimation_factor,
+                          persist);
+  }
+  dft_fields add_dft_fields(const component *cs, int num_components, const volume &where, double freq_min, double freq_max,
+                            int Nfreq, bool use_centered_grid = true, int decimation_factor = 0, bool persist = false) {
+    return add_dft_fields(cs, num_components, where, linspace(freq_min, freq_max, Nfreq), use_centered_grid,
+                          decimation_factor, persist);
   }
   double field_energy() { // :2196
     double e = 0;

@@ -54,14 +54,18 @@ def build(verbose=False):
     return LIB
 
 
-def build_cpp_program(src, out, lib_dir, lib_name, verbose=False):
-    """Compile a C++ program written against include/meep_amd.hpp and link it to one
-    implementation of the C-ABI (``lib<lib_name>.so`` in ``lib_dir``), found at run time
-    through an rpath relative to the program."""
+def build_cpp_program(src, out, lib_dir, lib_name, verbose=False, includes=()):
+    """Compile a C++ program written against include/meep_amd.hpp (or the reference's header
+    name, include/meep.hpp) and link it to one implementation of the C-ABI
+    (``lib<lib_name>.so`` in ``lib_dir``), found at run time through an rpath relative to the
+    program.  ``includes``: further include directories."""
     root = os.path.dirname(HERE)
     os.makedirs(os.path.dirname(out), exist_ok=True)
     rel = os.path.relpath(lib_dir, os.path.dirname(out))
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"), src,
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include")]
+    for inc in includes:
+        cmd += ["-I", inc]
+    cmd += [src,
            "-L", lib_dir, "-l" + lib_name, "-Wl,-rpath,$ORIGIN/" + rel, "-o", out]
     if verbose:
         print(" ".join(cmd), flush=True)

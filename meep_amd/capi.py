@@ -99,7 +99,7 @@ _SIGNATURES = {
     "mp_fields_reset_kernel_timing": (C.c_int, [C.c_void_p]),
     "mp_fields_add_dft_flux": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                            C.c_int, C.c_int]),
+                                            C.c_int, C.c_int, C.c_int, C.c_int]),
     "mp_dft_flux_delete": (None, [C.c_void_p]),
     "mp_dft_flux_flux": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mp_dft_flux_num_chunks": (C.c_int, [C.c_void_p, C.c_int]),
@@ -120,7 +120,7 @@ _SIGNATURES = {
     "mp_adjoint_gradient": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mp_fields_add_dft_fields": (C.c_void_p, [C.c_void_p, C.POINTER(C.c_int), C.c_int,
                                               C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                              C.POINTER(C.c_double), C.c_int, C.c_int]),
+                                              C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int]),
     "mp_dft_fields_delete": (None, [C.c_void_p]),
     "mp_dft_fields_num_chunks": (C.c_int, [C.c_void_p]),
     "mp_dft_fields_chunk_points": (C.c_long, [C.c_void_p, C.c_int]),
@@ -485,22 +485,29 @@ class fields:
     def set_dft_accumulator(self, kind):
         _check(self.lib, self.lib.mp_fields_set_dft_accumulator(self.ptr, kind))
 
-    def add_dft_flux(self, d, vmin, vmax, freqs, decimation_factor=0):
+    def add_dft_flux(self, d, vmin, vmax, freqs, decimation_factor=0, use_symmetry=True,
+                     centered_grid=True):
+        """fields::add_dft_flux(direction, volume, freq, use_symmetry, centered_grid,
+        decimation_factor) (src/meep.hpp:2051-2059)."""
         freqs = np.ascontiguousarray(freqs, dtype=np.float64)
         ptr = self.lib.mp_fields_add_dft_flux(self.ptr, d, _dvec(vmin), _dvec(vmax),
-                                              _dptr(freqs), len(freqs), decimation_factor)
+                                              _dptr(freqs), len(freqs), int(use_symmetry),
+                                              int(centered_grid), decimation_factor)
         if not ptr:
             raise MeepError(self.lib.mp_last_error().decode())
         fl = dft_flux(self, ptr, len(freqs))
         self._fluxes.append(fl)
         return fl
 
-    def add_dft_fields(self, components, vmin, vmax, freqs, decimation_factor=0):
+    def add_dft_fields(self, components, vmin, vmax, freqs, decimation_factor=0,
+                       use_centered_grid=True):
+        """fields::add_dft_fields(components, num_components, volume, freq,
+        use_centered_grid, decimation_factor) (src/meep.hpp:2085-2093)."""
         freqs = np.ascontiguousarray(freqs, dtype=np.float64)
         comps = (C.c_int * len(components))(*components)
         ptr = self.lib.mp_fields_add_dft_fields(self.ptr, comps, len(components), _dvec(vmin),
                                                 _dvec(vmax), _dptr(freqs), len(freqs),
-                                                decimation_factor)
+                                                int(use_centered_grid), decimation_factor)
         if not ptr:
             raise MeepError(self.lib.mp_last_error().decode())
         df = dft_fields(self, ptr, len(freqs))

@@ -311,8 +311,9 @@ struct FieldsBase {
   virtual void set_ktiming(bool on) = 0;
   virtual void get_ktiming(int which, double *ms, long *n) = 0;
   virtual void reset_ktiming() = 0;
-  virtual void *add_dft_flux(int d, V3 mn, V3 mx, const std::vector<double> &freq, int dec, int *dec_out) = 0;
-  virtual void *add_dft_fields(const int *comps, int ncomp, V3 mn, V3 mx, const std::vector<double> &freq, int dec, int *dec_out) = 0;
+  virtual void *add_dft_flux(int d, V3 mn, V3 mx, const std::vector<double> &freq, bool centered, int dec, int *dec_out) = 0;
+  virtual void *add_dft_fields(const int *comps, int ncomp, V3 mn, V3 mx, const std::vector<double> &freq, bool centered, int dec,
+                               int *dec_out) = 0;
   virtual int flux_chunk_component(void *fl, int which, int i) = 0;
   virtual void flux(void *fl, double *out) = 0;
   virtual int flux_chunks(void *fl, int which) = 0;
@@ -2831,15 +2832,19 @@ template <class R> struct Fields : FieldsBase {
     if (freq_max > 0 && src_freq_max > 0) return std::max(1, int(std::floor(1 / (dt * (freq_max + src_freq_max)))));
     return 1;
   }
-  // fields::add_dft for one component (src/dft.cpp:175-235)
+  // fields::add_dft for one component (src/dft.cpp:175-235): on the centred grid with the
+  // Yee-to-centre average, or on the component's own grid (centered false, no average)
   void add_dft(std::vector<DftChunk<R> *> &out, int c, const V3 &mn, const V3 &mxv, const std::vector<double> &freq, bool include_w,
-               cdouble stored_weight, int decimation) {
-    loop_in_chunks(mn, mxv, 20, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &,
+               cdouble stored_weight, int decimation, bool centered) {
+    loop_in_chunks(mn, mxv, centered ? 20 : c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &,
                                    cdouble sph) {
       if (!f[c][0]) return;
       std::unique_ptr<DftChunk<R>> ch(new DftChunk<R>);
       ch->c = c;
-      gv.yee2cent(c, ch->avg1, ch->avg2);
+      if (centered)
+        gv.yee2cent(c, ch->avg1, ch->avg2);
+      else
+        ch->avg1 = ch->avg2 = 0;
       ch->scale = stored_weight * sph * (dt / std::sqrt(2.0 * pi) * decimation);
       ch->decimation = decimation;
       std::vector<int64_t> idx;
@@ -2875,7 +2880,7 @@ template <class R> struct Fields : FieldsBase {
       ++dft_version;
     });
   }
-  void *add_dft_flux(int d, V3 mn, V3 mxv, const std::vector<double> &freq, int dec, int *dec_out) override {
+  void *add_dft_flux(int d, V3 mn, V3 mxv, const std::vector<double> &freq, bool centered, int dec, int *dec_out) override {
     if (!components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
     const int decim = dec > 0 ? dec : auto_decimation(freq);
     *dec_out = decim;
@@ -2893,8 +2898,8 @@ template <class R> struct Fields : FieldsBase {
     fl->nfreq = int(freq.size());
     std::vector<DftChunk<R> *> El[2], Hl[2];
     for (int q = 0; q < 2; ++q) {
-      add_dft(El[q], cE[q], mn, mxv, freq, true, cdouble(double(1 - 2 * q), 0), decim);
-      add_dft(Hl[q], cH[q], mn, mxv, freq, false, cdouble(1.0, 0), decim);
+      add_dft(El[q], cE[q], mn, mxv, freq, true, cdouble(double(1 - 2 * q), 0), decim, centered);
+      add_dft(Hl[q], cH[q], mn, mxv, freq, false, cdouble(1.0, 0), decim, centered);
     }
     // the reference prepends each chunk to its list (src/dft.cpp:120-129,623-641)
     for (int q = 1; q >= 0; --q) {
@@ -2906,7 +2911,8 @@ template <class R> struct Fields : FieldsBase {
   }
   // fields::add_dft_fields (src/dft.cpp:913-927): weights off, stored weight 1, centred grid;
   // each component's chunks go to the front of the list as the reference prepends them
-  void *add_dft_fields(const int *comps, int ncomp, V3 mn, V3 mxv, const std::vector<double> &freq, int dec, int *dec_out) override {
+  void *add_dft_fields(const int *comps, int ncomp, V3 mn, V3 mxv, const std::vector<double> &freq, bool centered, int dec,
+                       int *dec_out) override {
     if (!components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
     const int decim = dec > 0 ? dec : auto_decimation(freq);
     *dec_out = decim;
@@ -2915,7 +2921,7 @@ template <class R> struct Fields : FieldsBase {
     for (int q = 0; q < ncomp; ++q) {
       if (comps[q] < 0 || comps[q] >= NC) fail("invalid dft component %d", comps[q]);
       std::vector<DftChunk<R> *> lst;
-      add_dft(lst, comps[q], mn, mxv, freq, false, cdouble(1.0, 0), decim);
+      add_dft(lst, comps[q], mn, mxv, freq, false, cdouble(1.0, 0), decim, centered);
       std::vector<DftChunk<R> *> front(lst.rbegin(), lst.rend());
       fl->E.insert(fl->E.begin(), front.begin(), front.end());
     }
@@ -2956,6 +2962,7 @@ template <class R> struct Fields : FieldsBase {
     const int Nf = fl->nfreq;
     for (int q = 0; q < Nf; ++q) out[q] = 0;
     for (size_t q = 0; q < fl->E.size() && q < fl->H.size(); ++q) {
+      if (fl->E[q]->npts != fl->H[q]->npts) fail("flux of E and H chunks with different points (a monitor off the centred grid)");
       std::vector<cdouble> e = read_dft(*fl->E[q]), h = read_dft(*fl->H[q]);
       const bool narrow = fl->E[q]->acc_kind == MP_DFT_REALNUM;
       for (int kk = 0; kk < fl->E[q]->npts; ++kk)
@@ -3726,13 +3733,15 @@ int mp_fields_reset_kernel_timing(mp_fields *f) {
 int mp_fields_set_dft_accumulator(mp_fields *f, int kind) {
   return guard([&] { F_->set_dft_acc(kind); });
 }
-mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin, const double *vmax, const double *freq, int nfreq, int dec) {
+mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin, const double *vmax, const double *freq, int nfreq,
+                                    int use_symmetry, int centered, int dec) {
+  (void)use_symmetry; // one chunk with the identity symmetry: nothing to reduce
   FluxHandle *h = new FluxHandle;
   int rc = guard([&] {
     std::vector<double> fr(freq, freq + nfreq);
     h->owner = F_;
     h->nfreq = nfreq;
-    h->fl = F_->add_dft_flux(d, to_v3(vmin), to_v3(vmax), fr, dec, &h->decimation);
+    h->fl = F_->add_dft_flux(d, to_v3(vmin), to_v3(vmax), fr, centered != 0, dec, &h->decimation);
   });
   if (rc) {
     delete h;
@@ -3767,12 +3776,14 @@ int mp_dft_flux_get_dft(mp_dft_flux *flp, int which, int i, double *out) {
 }
 
 mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *comps, int ncomp, const double *vmin, const double *vmax,
-                                        const double *freq, int nfreq, int dec) {
+                                        const double *freq, int nfreq, int centered, int dec) {
   FluxHandle *h = new FluxHandle;
   h->owner = reinterpret_cast<FieldsBase *>(f);
   h->nfreq = nfreq;
   std::vector<double> fr(freq, freq + nfreq);
-  int rc = guard([&] { h->fl = h->owner->add_dft_fields(comps, ncomp, to_v3(vmin), to_v3(vmax), fr, dec, &h->decimation); });
+  int rc = guard([&] {
+    h->fl = h->owner->add_dft_fields(comps, ncomp, to_v3(vmin), to_v3(vmax), fr, centered != 0, dec, &h->decimation);
+  });
   if (rc) {
     delete h;
     return nullptr;

@@ -1453,10 +1453,11 @@ template <class R> struct Fields {
     require_component(c);
   }
 
-  // fields::add_dft (src/dft.cpp:175-235) for one component
+  // fields::add_dft (src/dft.cpp:175-235) for one component, on the centred grid or (centered
+  // false) on the component's own grid without the centring average (src/dft.cpp:91-94,233)
   void add_dft(std::vector<std::unique_ptr<DftChunk<R>>> &out_list, int c, vec3 wmin, vec3 wmax, const std::vector<double> &freq,
-               bool include_w, cdouble stored_weight, int decimation) {
-    loop_in_chunks(wmin, wmax, 20, [&](const ivec &is, const ivec &ie, const vec3 &s0, const vec3 &s1, const vec3 &e0, const vec3 &e1,
+               bool include_w, cdouble stored_weight, int decimation, bool centered = true) {
+    loop_in_chunks(wmin, wmax, centered ? 20 : c, [&](const ivec &is, const ivec &ie, const vec3 &s0, const vec3 &s1, const vec3 &e0, const vec3 &e1,
                                        double dV0, const ivec &, cdouble shift_phase) {
       if (!f[c][0]) return;
       std::unique_ptr<DftChunk<R>> ch(new DftChunk<R>);
@@ -1469,7 +1470,10 @@ template <class R> struct Fields {
       ch->e1 = e1;
       ch->dV0 = dV0;
       ch->dV1 = 0;
-      gv.yee2cent_offsets(c, ch->avg1, ch->avg2);
+      if (centered)
+        gv.yee2cent_offsets(c, ch->avg1, ch->avg2);
+      else
+        ch->avg1 = ch->avg2 = 0;
 #ifdef MUT_DFT_SCALE
       ch->scale = stored_weight * shift_phase * double(decimation);
 #else
@@ -1924,7 +1928,9 @@ int mp_fields_set_dft_accumulator(mp_fields *f, int kind) {
   });
 }
 
-mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin, const double *vmax, const double *freq, int nfreq, int dec) {
+mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin, const double *vmax, const double *freq, int nfreq,
+                                    int use_symmetry, int centered, int dec) {
+  (void)use_symmetry; // the identity symmetry: nothing to reduce (src/dft.cpp:614)
   AnyFields *af = reinterpret_cast<AnyFields *>(f);
   AnyFlux *fl = new AnyFlux;
   fl->owner = af;
@@ -1949,8 +1955,8 @@ mp_dft_flux *mp_fields_add_dft_flux(mp_fields *f, int d, const double *vmin, con
       // the reference prepends each new chunk list, so the E list is [cE1 chunks, cE0 chunks]
       std::vector<std::unique_ptr<DftChunk<R>>> Elist[2], Hlist[2];
       for (int i = 0; i < 2; ++i) {
-        F.add_dft(Elist[i], cE[i], mn, mx, fl->freq, true, cdouble(double(1 - 2 * i), 0), decim);
-        F.add_dft(Hlist[i], cH[i], mn, mx, fl->freq, false, cdouble(1.0, 0), decim);
+        F.add_dft(Elist[i], cE[i], mn, mx, fl->freq, true, cdouble(double(1 - 2 * i), 0), decim, centered != 0);
+        F.add_dft(Hlist[i], cH[i], mn, mx, fl->freq, false, cdouble(1.0, 0), decim, centered != 0);
       }
       for (int i = 1; i >= 0; --i) {
         for (int j = int(Elist[i].size()) - 1; j >= 0; --j) {
@@ -1982,6 +1988,7 @@ int mp_dft_flux_flux(mp_dft_flux *flp, double *out) {
       for (size_t q = 0; q < fl->E.size() && q < fl->H.size(); ++q) {
         auto *E = reinterpret_cast<DftChunk<R> *>(fl->E[q]);
         auto *H = reinterpret_cast<DftChunk<R> *>(fl->H[q]);
+        if (E->N != H->N) fail("flux of E and H chunks with different points (a monitor off the centred grid)");
         for (size_t k = 0; k < E->N; ++k)
           for (size_t i = 0; i < Nf; ++i) {
             cdouble e = E->acc == MP_DFT_F64 ? E->dft64[k * Nf + i] : cdouble(E->dft[k * Nf + i]);
@@ -2031,7 +2038,7 @@ int mp_dft_flux_get_dft(mp_dft_flux *flp, int which, int i, double *out) {
 // fields::add_dft_fields (src/dft.cpp:913-927): one chunk list over all components, no
 // integration weights, stored weight 1, centred grid; each component's chunks are prepended.
 mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *comps, int ncomp, const double *vmin, const double *vmax,
-                                        const double *freq, int nfreq, int dec) {
+                                        const double *freq, int nfreq, int centered, int dec) {
   AnyFields *af = reinterpret_cast<AnyFields *>(f);
   AnyFlux *fl = new AnyFlux;
   fl->owner = af;
@@ -2045,7 +2052,7 @@ mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *comps, int ncom
       vec3 mn = to_vec(vmin), mx = to_vec(vmax);
       for (int q = 0; q < ncomp; ++q) {
         std::vector<std::unique_ptr<DftChunk<R>>> lst;
-        F.add_dft(lst, comps[q], mn, mx, fl->freq, false, cdouble(1.0, 0), decim);
+        F.add_dft(lst, comps[q], mn, mx, fl->freq, false, cdouble(1.0, 0), decim, centered != 0);
         std::vector<void *> front;
         for (int j = int(lst.size()) - 1; j >= 0; --j) {
           front.push_back(lst[j].get());

@@ -1,7 +1,9 @@
-// The reference's known-answer program (tests/known_results.cpp:27-157) written against the
-// C++ API layer include/meep_amd.hpp.  It builds unchanged against either implementation of
-// the C-ABI: the MI355X engine (libmeep_amd.so) or the CPU oracle (libmeep_oracle.so), which
-// is what shows that a C++ caller of meep::fields::step() is served by this library.
+// The known-answer cases of the reference (tests/known_results.cpp:27-157) as a program of
+// our own against the C++ API layer include/meep_amd.hpp, with a case filter and a --single
+// switch (float realnum at run time).  It builds against either implementation of the
+// C-ABI: the MI355X engine (libmeep_amd.so) or the CPU oracle (libmeep_oracle.so).  The
+// reference's own file, compiled unmodified against include/meep.hpp, is run by
+// tests/test_abi.py (oracle) and tests/test_gpu_cpp_api.py (engine).
 //
 // Usage: known_results [case-name-substring]   (runs every case when no argument is given)
 // Exit status 0 when every selected case matches its golden value.
@@ -13,7 +15,6 @@
 
 using namespace meep;
 
-static const double pi = 3.141592653589793238462643383276;
 static double dpml = 1.0;
 
 static double one(const vec &) { return 1.0; }

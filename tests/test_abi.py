@@ -60,6 +60,29 @@ def test_cpp_api_known_results_on_oracle(tmp_path, oracle_lib):
         assert r.stdout.count("Passed") == 13, r.stdout
 
 
+REFERENCE_KNOWN_RESULTS = "/root/reference/tests/known_results.cpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_KNOWN_RESULTS), reason="the reference tree is absent")
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_reference_known_results_program_unmodified(tmp_path, oracle_lib, single):
+    """The reference's own test program, compiled in place and unmodified against
+    include/meep.hpp (its config.h include served by tests/cpp/config.h, which defines
+    nothing), passes its 13 known answers on the oracle; with MEEP_AMD_SINGLE realnum is float
+    and the program applies its single-precision tolerance (tests/known_results.cpp:43-49)."""
+    out = str(tmp_path / "known_results_reference")
+    src = REFERENCE_KNOWN_RESULTS
+    if single:
+        cpp = tmp_path / "single.cpp"
+        cpp.write_text(f'#define MEEP_AMD_SINGLE 1\n#include "{src}"\n')
+        src = str(cpp)
+    mbuild.build_cpp_program(src, out, os.path.join(ROOT, "oracle", "_build"), "meep_oracle",
+                             includes=[os.path.join(ROOT, "tests", "cpp")])
+    r = subprocess.run([out], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Passed") == 13, r.stdout
+
+
 def test_cpp_api_errors_become_exceptions(tmp_path, oracle_lib):
     src = tmp_path / "err.cpp"
     src.write_text(r'''
