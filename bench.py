@@ -1,22 +1,32 @@
 """Throughput of the FDTD time step on one MI355X: Gcell-updates/s and HBM-roofline fraction.
 
-Workload (the 3D bandwidth configuration): a 256^3-cell Bloch-periodic box (k = 0, real
-fields), no PML, eps piecewise constant on 8^3-cell blocks drawn from numpy
-default_rng(2) as 12 with probability 0.3 and 1 otherwise (no averaging: each Yee point takes
-its block's value), a Gaussian Ex point current at the centre (fcen 0.15, fwidth 0.1, Python
-GaussianSource semantics, not integrated), Courant 0.5, resolution 10.  A step is one
-fields::step() over the whole box.  Inputs are resident in HBM before the timed region.
+Headline workload (the north-star configuration, the largest that runs on one GPU): 384 x 384 x
+192 cells at resolution 10 with 16 cells of uPML on every face; the lower half (z < 96 cells)
+eps_inf = 2 with a Lorentz pole (f0 1.0, gamma 0.05, sigma 1.5) and a Drude pole
+(lorentzian_susceptibility(1.0, 0.1, no_omega_0_denominator = true), sigma 3); 150 seeded
+dielectric spheres (radius 4-12 cells, eps 2-6) above, sampled per Yee point (no averaging);
+a z-plane of Ex current at uniform amplitude (Gaussian fcen 0.7, df 0.6, not integrated); DFT
+of Ex, Ey, Ez on a 384 x 384 z-plane at 32 frequencies with compensated FP32 accumulators;
+FP32 fields.  A step is one fields::step() over the whole cell, DFT included.  Inputs are
+resident in HBM before the timed region.
 
-Roofline: the dominant kernel's launches are timed with HIP events on the engine's stream
-during the timed region; its algorithmic bytes per launch are the reference-state words it
-moves per cell (D phase: D read + write, chi1inv read, E write = 12 words; B phase: B read +
-write, E read = 9 words) times the cells, so `achieved` is GB/s of algorithmic traffic.
-`traffic` (HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE counters, scaled by a
-device-copy calibration) is read from profiles/traffic_step_de_box256_<precision>.json.
+Roofline: the dominant kernel is the D-phase launch over the interior box (whole rows along z,
+x and y inside the uPML; the poles and the z-uPML taken point by point), timed with HIP events
+on the engine's stream around every such launch in the timed region.  Its algorithmic bytes
+per launch are the reference-state words it moves (per cell: D read + write, chi1inv read, E
+write = 12; per dispersive cell 30 more: P and P_prev read + written and sigma read, for two
+poles and three components; per z-uPML cell 4 more: f_u of Dx and f_w of Ez read + written),
+4 bytes each.  `traffic` is HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+summary of the same kernel in profiles/, scaled by a device-copy calibration.
 
-CPU baseline: the oracle (a C++ restatement of the reference's loops, OpenMP over the outer
-two loop counters as the reference's collapse(2) loops) timed on this host on a bounded
-sample of the same workload: the first steps of the same cell.
+CPU baseline: the oracle (a C++ restatement of the reference's loops) built as the survey's
+baseline plan states (-O3 -march=native -fopenmp), FP64, on a bounded sample of the same
+configuration (192 x 192 x 96 cells, 8-cell uPML, the same materials, source and monitor), the
+median of five timed steps after one untimed, at one thread and at every thread OpenMP gets,
+with the reference's per-phase breakdown.
+
+Secondary keys in the same line: the 256^3 Bloch-periodic box (bandwidth test), the 600 x 400
+reverse-mode gradient against a plain forward run, and the 64-simulation batch.
 """
 
 import argparse
@@ -24,6 +34,7 @@ import ctypes
 import json
 import os
 import platform
+import statistics
 import sys
 import time
 
@@ -40,11 +51,10 @@ A = 10.0
 BLOCK = 8
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s + % HBM roofline; adjoint-gradient time / forward time"
-# algorithmic reference-state words per cell owned by each kernel (E, D, B=H, chi1inv: 21/step)
-WORDS = {capi.K_STEP_BH: 9,   # B read + write, E read
-         capi.K_STEP_DE: 12}  # D read + write, chi1inv read, E write
-KERNEL_NAME = {capi.K_STEP_BH: "B phase: k_fast<R, V, curl, no update>",
-               capi.K_STEP_DE: "D phase: k_fast<R, V, curl, E update>"}
+TIMED = (capi.K_STEP_BH, capi.K_STEP_DE, capi.K_HALO, capi.K_DFT, capi.K_SRC, capi.K_BH_INTERIOR,
+         capi.K_DE_INTERIOR)
+PHASES = ("FieldUpdateB", "FieldUpdateH", "FieldUpdateD", "FieldUpdateE", "BoundaryStepping",
+          "FourierTransforming", "Stepping")
 
 
 def eps_blocks(seed=2, p12=0.3, n=N, block=BLOCK):
@@ -218,61 +228,6 @@ def build_adjoint2d(lib, single, chi_ez=None, rho=None):
     return s, f, mon
 
 
-def run_adjoint2d(args, lib):
-    """Forward run with recording, then the reverse-mode gradient; value is the forward
-    Gcell-updates/s, and the line reports the gradient's cost relative to the forward run."""
-    from meep_amd import autograd as madj
-    single = args.precision == "f32"
-    interval = args.checkpoint_interval
-    prob = madj.Problem(lib, lambda L, chi: build_adjoint2d(L, single, chi), (capi.Ez,) + ADJ_DESIGN, ADJ_STEPS,
-                        checkpoint_interval=interval)
-    base, lo, shape = prob.base_chi()
-    chi_design = base[lo[0]:lo[0] + shape[0], lo[1]:lo[1] + shape[1]]
-    # warm-up run (compiles nothing, but pages in the library and the allocator)
-    J, h = prob.forward(chi_design)
-    prob.gradient(h)
-    J, h = prob.forward(chi_design)
-    g = prob.gradient(h)
-    tf, tg = prob.timing["forward_s"], prob.timing["gradient_s"]
-    cells = 600 * 400
-    line = line_base(args, cells * ADJ_STEPS / tf / 1e9, tf, {
-        "workload": "2D TM 600x400 cells at resolution 50, PML 50 cells, 200x200 design "
-                    "(eps = 1 + 11 rho, rho seeded uniform blurred 5 px), Gaussian-profile Ez "
-                    "line source, J = sum |DFT Ez|^2 on a 40-cell line at 10 frequencies, "
-                    f"4000 steps; reverse mode with checkpoint interval {prob.interval}",
-        "cells": cells, "resolution": ADJ_A, "courant": 0.5, "parallelism": "none"})
-    line["steps"] = ADJ_STEPS
-    line["ms_per_step"] = round(tf / ADJ_STEPS * 1e3, 4)
-    line["adjoint"] = {"forward_s": round(tf, 4), "gradient_s": round(tg, 4),
-                       "gradient_over_forward": round(tg / tf, 3), "J": J,
-                       "grad_max_abs": float(np.abs(g).max())}
-    line["roofline"] = None
-    line["cpu_baseline"] = None
-    if args.fd_pixels:
-        rng = np.random.default_rng(7)
-        out = []
-        chi_full = prob.full_chi(chi_design)
-        for _ in range(args.fd_pixels):
-            i, j = int(rng.integers(0, shape[0])), int(rng.integers(0, shape[1]))
-            h = 1e-3 * chi_full[lo[0] + i, lo[1] + j]
-            vals = []
-            for sg in (1, -1):
-                c2 = chi_full.copy()
-                c2[lo[0] + i, lo[1] + j] += sg * h
-                s, f, mon = build_adjoint2d(lib, False, c2)
-                f.step_n(ADJ_STEPS)
-                vals.append(sum(float(np.sum(np.abs(mon.dft(q)) ** 2)) for q in range(mon.num_chunks())))
-                del f, s
-            fd = (vals[0] - vals[1]) / (2 * h)
-            out.append((i, j, float(g[i, j]), fd))
-        gm = float(np.abs(g).max())
-        line["adjoint"]["fd_check"] = {
-            "pixels": len(out), "fd": "central differences of FP64 runs, step 1e-3 chi1inv",
-            "max_rel_err": max(abs(a - b) / max(abs(b), 1e-300) for _, _, a, b in out),
-            "max_err_over_max_grad": max(abs(a - b) for _, _, a, b in out) / gm}
-    return line
-
-
 # ---- the batched 2D workload ----------------------------------------------------------------
 BATCH_N = 64
 BATCH_A = 10.0
@@ -335,82 +290,87 @@ def batch2d_bytes(sides, word=4):
     return total
 
 
-def run_batch2d(args, lib):
-    single = args.precision == "f32"
-    sims, sides = build_batch2d(lib, single)
-    b = capi.batch([f for _, f, _ in sims], use_graph=not args.batch_eager, autotune=True)
-    b.step_n(args.warmup)  # planning, the chunks' own first step, block-size tuning
-    sync = device_sync()
-    sync()
-    t0 = time.perf_counter()
-    b.step_n(args.steps)
-    sync()
-    el = time.perf_counter() - t0
-    cells = int(np.sum(sides.astype(np.int64) ** 2))
-    value = cells * args.steps / el / 1e9
-    block, ms = b.stats()
-    alg = batch2d_bytes(sides, 4 if single else 8)
-    achieved = alg * args.steps / el / 1e9
-    line = line_base(args, value, el, {
-        "workload": f"{BATCH_N} independent 2D TM simulations, square sides 128-512 cells (seeded), "
-                    "resolution 10, eps 16x16 blocks in [1, 12], 10-cell uPML, Gaussian Ez point "
-                    "source fcen in [0.1, 0.3] df = 0.1 fcen, 16-frequency flux line with FP64 "
-                    "accumulators; one launch per launch group for all simulations, "
-                    f"{'eager' if args.batch_eager else 'hipGraph'} step loop, autotuned block size {block}",
-        "cells": cells, "simulations": BATCH_N, "resolution": BATCH_A, "courant": 0.5, "parallelism": "none"})
-    line["roofline"] = {"bound": "hbm", "kernel": "whole batched step (all launches)",
-                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": alg}
-    line["cpu_baseline"] = None
-    if not args.no_cpu_baseline:
-        # the oracle steps the simulations one after another
-        olib = oracle_lib()
-        osims, _ = build_batch2d(olib, single)
-        steps, t0 = 0, time.perf_counter()
-        for _, f, _ in osims:
-            f.step()
-        el0 = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": cells / el0 / 1e9, "unit": "Gcell-updates/s",
-                                "cores": ctypes.CDLL("libgomp.so.1").omp_get_max_threads(), "kind": "port",
-                                "sample": f"oracle, one step of each of the {BATCH_N} simulations in turn "
-                                          f"(the first, so it includes lazy allocation), {el0:.1f} s"}
-    return line
+def de_interior_bytes(n=PLD_CELLS, npml=PLD_PML, word=4):
+    """Algorithmic bytes of one D-phase interior launch of the uPML + Lorentz-Drude workload:
+    its box is x, y in the quiet range (npml .. n - npml - 1) and every z; 12 words per cell,
+    30 more per dispersive cell (lower half), 4 more per z-uPML cell."""
+    nx, ny = n[0] - 2 * npml, n[1] - 2 * npml
+    cells = nx * ny * n[2]
+    disp = nx * ny * (n[2] // 2)
+    zpml = nx * ny * 2 * npml
+    return (12 * cells + 30 * disp + 4 * zpml) * word
 
 
-def load_traffic(precision):
-    path = os.path.join(ROOT, "profiles", f"traffic_step_de_box256_{precision}.json")
+def box256_de_bytes(n=N, word=4):
+    """Algorithmic bytes of one D-phase launch of the Bloch box: D read + write, chi1inv read
+    and E write, 12 words per cell."""
+    return 12 * n ** 3 * word
+
+
+def load_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 counter summary
+    (profiles/<name>.json, written from the FETCH_SIZE and WRITE_SIZE passes), or None."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         return json.load(fh).get("traffic_bytes_per_launch")
 
 
-def oracle_lib():
+def oracle_lib(baseline=False):
     import subprocess
-    opath = os.path.join(ROOT, "oracle", "_build", "libmeep_oracle.so")
+    name = "libmeep_oracle_baseline.so" if baseline else "libmeep_oracle.so"
+    opath = os.path.join(ROOT, "oracle", "_build", name)
     if not os.path.exists(opath):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return capi.bind(ctypes.CDLL(opath))
 
 
-def cpu_baseline(make, cells, what, seconds=15.0, max_steps=20):
-    """The oracle (its per-point loops over OpenMP threads) on the first steps of the same
-    workload on this host."""
-    f = make(oracle_lib())
-    f.step()  # the first step builds the halo tables and allocates lazily
-    steps, t0 = 0, time.perf_counter()
-    while True:
+def _gomp():
+    return ctypes.CDLL("libgomp.so.1")
+
+
+def cpu_rate(make, cells, repeats=5, threads=None):
+    """Median Gcell-updates/s of `repeats` single steps of the oracle after one untimed step
+    (which allocates lazily and builds the halo tables), and the median seconds per step of
+    each of the reference's time sinks."""
+    g = _gomp()
+    prev = g.omp_get_max_threads()
+    if threads:
+        g.omp_set_num_threads(threads)
+    try:
+        f = make(oracle_lib(baseline=True))
         f.step()
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > seconds or steps >= max_steps:
-            break
-    rate = cells * steps / el / 1e9
-    threads = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()
-    return {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle ({threads} OpenMP threads) on {what}, {steps} timed steps after "
-                      f"1 untimed, {el:.1f} s, host {platform.processor() or platform.machine()} "
-                      f"with {os.cpu_count()} logical CPUs"}
+        rates, phases = [], []
+        for _ in range(repeats):
+            t0 = f.times()
+            a = time.perf_counter()
+            f.step()
+            el = time.perf_counter() - a
+            t1 = f.times()
+            rates.append(cells / el / 1e9)
+            phases.append([b - c for b, c in zip(t1, t0)])
+        ph = {name: round(statistics.median(p[k] for p in phases), 5) for k, name in enumerate(PHASES)}
+        return statistics.median(rates), ph, g.omp_get_max_threads()
+    finally:
+        g.omp_set_num_threads(prev)
+
+
+def cpu_baseline_pml_lorentz_dft():
+    n, npml = (192, 192, 96), 8
+    cells = int(np.prod(n))
+
+    def make(L):
+        return build_pml_lorentz_dft(L, False, n=n, npml=npml, dft_kind=capi.DFT_F64)[1]
+    r1, ph1, _ = cpu_rate(make, cells, threads=1)
+    rall, phall, nthr = cpu_rate(make, cells)
+    return {"value": round(rall, 5), "unit": "Gcell-updates/s", "cores": nthr, "kind": "port",
+            "single_thread": round(r1, 5),
+            "phase_s_per_step": {"all_threads": phall, "one_thread": ph1},
+            "sample": f"oracle (FP64, -O3 -march=native, OpenMP) on the same configuration at "
+                      f"192x192x96 cells with 8-cell uPML (same materials, source and DFT plane); "
+                      f"median of 5 steps after 1 untimed, at 1 thread and {nthr} threads; host "
+                      f"{platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs"}
 
 
 def device_sync():
@@ -433,20 +393,20 @@ def timed_run(f, steps, warmup):
     f.step_n(steps)
     sync()
     el = time.perf_counter() - t0
-    stats = {k: f.kernel_timing(k) for k in (capi.K_STEP_BH, capi.K_STEP_DE, capi.K_HALO,
-                                              capi.K_DFT, capi.K_SRC)}
+    stats = {k: f.kernel_timing(k) for k in TIMED}
     return el, stats
 
 
-def line_base(args, value, el, config):
+def line_base(args, value, el, config, steps=None, warmup=None):
+    steps = args.steps if steps is None else steps
     return {
         "metric": METRIC,
         "value": round(value, 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "steps": steps,
+        "warmup": args.warmup if warmup is None else warmup,
+        "ms_per_step": round(el / steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -456,96 +416,213 @@ def line_base(args, value, el, config):
     }
 
 
-def run_box256(args, lib):
-    single = args.precision == "f32"
-    eps_b = eps_blocks()
-    s, f = build(lib, single, eps_b)
-    el, stats = timed_run(f, args.steps, args.warmup)
-    cells = N ** 3
-    value = cells * args.steps / el / 1e9
-    dom = max((capi.K_STEP_BH, capi.K_STEP_DE), key=lambda k: stats[k][0])
-    ms, nl = stats[dom]
-    avg_s = ms / max(nl, 1) / 1e3
-    wbytes = 4 if single else 8
-    alg_bytes = WORDS[dom] * wbytes * cells
-    achieved = alg_bytes / avg_s / 1e9
-    step_alg = 21 * wbytes * cells
-    print(f"[bench] {args.steps} steps in {el:.3f} s; per-kernel ms: "
-          + ", ".join(f"{k}:{v[0]:.1f}/{v[1]}" for k, v in stats.items())
-          + f"; whole-step algorithmic rate {step_alg * args.steps / el / 1e9:.0f} GB/s",
-          file=sys.stderr)
-    line = line_base(args, value, el, {
-        "workload": "3D Bloch-periodic box 256^3 cells, eps 8^3 blocks {1,12} p=0.3 seed 2, "
-                    "Ex Gaussian dipole at centre, no PML, real fields",
-        "cells": cells, "resolution": A, "courant": 0.5, "parallelism": "none"})
-    line["roofline"] = {"bound": "hbm", "kernel": KERNEL_NAME[dom],
-                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": load_traffic(args.precision),
-                        "alg_bytes_per_launch": alg_bytes,
-                        "avg_launch_us": round(avg_s * 1e6, 2)}
-    line["cpu_baseline"] = None
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(lambda L: build(L, single, eps_b)[1], cells,
-                                            "the same 256^3 box")
-    return line
+def roofline(name, alg_bytes, ms, launches, traffic):
+    avg = ms / max(launches, 1) / 1e3
+    achieved = alg_bytes / avg / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "alg_bytes_per_launch": int(alg_bytes), "avg_launch_us": round(avg * 1e6, 2),
+            "launches": launches}
 
 
-def run_pml_lorentz_dft(args, lib):
-    """The PML + Lorentz-Drude + DFT workload; the roofline is the whole step's algorithmic
-    bytes (every launch of the step, DFT included) over the step time."""
+def log_stats(what, steps, el, stats):
+    print(f"[bench] {what}: {steps} steps in {el:.3f} s; kernel ms per step: "
+          + ", ".join(f"{k}:{v[0] / max(steps, 1):.4f}" for k, v in stats.items()), file=sys.stderr, flush=True)
+
+
+def run_pml_lorentz_dft(args, lib, steps=None, warmup=None):
     single = args.precision == "f32"
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     kind = capi.DFT_KAHAN32 if single else capi.DFT_F64
     s, f, dft = build_pml_lorentz_dft(lib, single, dft_kind=kind)
-    el, stats = timed_run(f, args.steps, args.warmup)
+    el, stats = timed_run(f, steps, warmup)
+    log_stats("pml_lorentz_dft", steps, el, stats)
     cells = int(np.prod(PLD_CELLS))
-    value = cells * args.steps / el / 1e9
+    value = cells * steps / el / 1e9
     wbytes = 4 if single else 8
-    alg = pml_lorentz_dft_bytes(word=wbytes, dft_points=[dft.points(i) for i in range(dft.num_chunks())])
-    achieved = alg * args.steps / el / 1e9
-    print(f"[bench] pml_lorentz_dft {args.steps} steps in {el:.3f} s; per-kernel ms: "
-          + ", ".join(f"{k}:{v[0]:.1f}/{v[1]}" for k, v in stats.items()), file=sys.stderr)
+    step_alg = pml_lorentz_dft_bytes(word=wbytes, dft_points=[dft.points(i) for i in range(dft.num_chunks())],
+                                     sigma_arrays=True)
     line = line_base(args, value, el, {
         "workload": "3D 384x384x192 cells, 16-cell uPML on all faces, lower half eps_inf 2 + "
                     "Lorentz (1.0, 0.05, sigma 1.5) + Drude (1.0, 0.1, sigma 3), 150 seeded "
                     "spheres above, Ex plane current, DFT Ex/Ey/Ez on a 384x384 plane at 32 "
                     "frequencies (" + ("Kahan FP32" if single else "FP64") + " accumulators)",
-        "cells": cells, "resolution": PLD_A, "courant": 0.5, "parallelism": "none"})
-    line["roofline"] = {"bound": "hbm", "kernel": "whole step (all launches)",
-                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "alg_bytes_per_step": alg,
-                        "phase_ms": {str(k): round(v[0] / args.steps, 4) for k, v in stats.items()}}
+        "cells": cells, "resolution": PLD_A, "courant": 0.5, "parallelism": "none"}, steps, warmup)
+    ms, nl = stats[capi.K_DE_INTERIOR]
+    line["roofline"] = roofline("D phase, interior box: k_fast<" + ("float, 2" if single else "double, 2")
+                                + ", curl, E update, uPML, 2 poles>", de_interior_bytes(word=wbytes), ms, nl,
+                                load_traffic(f"traffic_de_interior_pml_lorentz_dft_{args.precision}"))
+    line["roofline"]["whole_step"] = {
+        "alg_bytes_per_step": int(step_alg),
+        "achieved": round(step_alg * steps / el / 1e9, 1),
+        "frac": round(step_alg * steps / el / 1e9 / HBM_PEAK_GBS, 4)}
+    line["phase_ms_per_step"] = {name: round(stats[k][0] / steps, 4) for name, k in (
+        ("B", capi.K_STEP_BH), ("B_interior", capi.K_BH_INTERIOR), ("D", capi.K_STEP_DE),
+        ("D_interior", capi.K_DE_INTERIOR), ("halo", capi.K_HALO), ("dft", capi.K_DFT), ("src", capi.K_SRC))}
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_pml_lorentz_dft()
+    return line
+
+
+def run_box256(args, lib, steps=None, warmup=None):
+    single = args.precision == "f32"
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    s, f = build(lib, single, eps_blocks())
+    el, stats = timed_run(f, steps, warmup)
+    log_stats("box256", steps, el, stats)
+    cells = N ** 3
+    wbytes = 4 if single else 8
+    line = line_base(args, cells * steps / el / 1e9, el, {
+        "workload": "3D Bloch-periodic box 256^3 cells, eps 8^3 blocks {1,12} p=0.3 seed 2, "
+                    "Ex Gaussian dipole at centre, no PML, real fields",
+        "cells": cells, "resolution": A, "courant": 0.5, "parallelism": "none"}, steps, warmup)
+    ms, nl = stats[capi.K_DE_INTERIOR]
+    line["roofline"] = roofline("D phase: k_fast<" + ("float, 4" if single else "double, 2") + ", curl, E update>",
+                                box256_de_bytes(word=wbytes), ms, nl, load_traffic(f"traffic_step_de_box256_{args.precision}"))
+    line["roofline"]["whole_step"] = {"alg_bytes_per_step": 21 * wbytes * cells,
+                                      "frac": round(21 * wbytes * cells * steps / el / 1e9 / HBM_PEAK_GBS, 4)}
     line["cpu_baseline"] = None
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(lambda L: build_pml_lorentz_dft(L, single)[1], cells,
-                                            "the same 384x384x192 cell (realnum DFT)",
-                                            seconds=20.0, max_steps=8)
+        n = 128
+        eps_b = eps_blocks(n=n)
+        r1, ph1, _ = cpu_rate(lambda L: build(L, False, eps_b, n=n)[1], n ** 3, threads=1)
+        rall, phall, nthr = cpu_rate(lambda L: build(L, False, eps_b, n=n)[1], n ** 3)
+        line["cpu_baseline"] = {"value": round(rall, 5), "unit": "Gcell-updates/s", "cores": nthr, "kind": "port",
+                                "single_thread": round(r1, 5),
+                                "phase_s_per_step": {"all_threads": phall, "one_thread": ph1},
+                                "sample": f"oracle (FP64, -O3 -march=native, OpenMP) on the same box "
+                                          f"generator at 128^3, median of 5 steps after 1 untimed, at 1 "
+                                          f"and {nthr} threads"}
     return line
+
+
+def run_adjoint2d(args, lib):
+    """A plain forward run, a forward run with recording, then the reverse-mode gradient; value
+    is the plain forward Gcell-updates/s; gradient_over_forward is the gradient's time over the
+    plain forward run's."""
+    from meep_amd import autograd as madj
+    single = args.precision == "f32"
+    interval = args.checkpoint_interval
+    prob = madj.Problem(lib, lambda L, chi: build_adjoint2d(L, single, chi), (capi.Ez,) + ADJ_DESIGN, ADJ_STEPS,
+                        checkpoint_interval=interval)
+    base, lo, shape = prob.base_chi()
+    chi_design = base[lo[0]:lo[0] + shape[0], lo[1]:lo[1] + shape[1]]
+    sync = device_sync()
+    plain = []
+    for _ in range(2):  # the first pages in the library and the allocator
+        s, f, mon = build_adjoint2d(lib, single, prob.full_chi(chi_design))
+        f.step_n(1)
+        sync()
+        t0 = time.perf_counter()
+        f.step_n(ADJ_STEPS - 1)
+        sync()
+        plain.append((time.perf_counter() - t0) * ADJ_STEPS / (ADJ_STEPS - 1))
+        del f, s
+    J, h = prob.forward(chi_design)
+    prob.gradient(h)
+    J, h = prob.forward(chi_design)
+    g = prob.gradient(h)
+    tp, tf, tg = plain[-1], prob.timing["forward_s"], prob.timing["gradient_s"]
+    cells = 600 * 400
+    line = line_base(args, cells * ADJ_STEPS / tp / 1e9, tp, {
+        "workload": "2D TM 600x400 cells at resolution 50, PML 50 cells, 200x200 design "
+                    "(eps = 1 + 11 rho, rho seeded uniform blurred 5 px), Gaussian-profile Ez "
+                    "line source, J = sum |DFT Ez|^2 on a 40-cell line at 10 frequencies, "
+                    f"4000 steps; reverse mode with checkpoint interval {prob.interval}",
+        "cells": cells, "resolution": ADJ_A, "courant": 0.5, "parallelism": "none"}, ADJ_STEPS, 0)
+    line["adjoint"] = {"forward_plain_s": round(tp, 4), "forward_recording_s": round(tf, 4),
+                       "gradient_s": round(tg, 4), "gradient_over_forward": round(tg / tp, 3),
+                       "J": J, "grad_max_abs": float(np.abs(g).max())}
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    return line
+
+
+def run_batch2d(args, lib, steps=None, warmup=None):
+    single = args.precision == "f32"
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    sims, sides = build_batch2d(lib, single)
+    b = capi.batch([f for _, f, _ in sims], use_graph=not args.batch_eager, autotune=True)
+    b.step_n(warmup)  # planning, the chunks' own first step, block-size tuning
+    sync = device_sync()
+    sync()
+    t0 = time.perf_counter()
+    b.step_n(steps)
+    sync()
+    el = time.perf_counter() - t0
+    cells = int(np.sum(sides.astype(np.int64) ** 2))
+    block, ms = b.stats()
+    alg = batch2d_bytes(sides, 4 if single else 8)
+    achieved = alg * steps / el / 1e9
+    line = line_base(args, cells * steps / el / 1e9, el, {
+        "workload": f"{BATCH_N} independent 2D TM simulations, square sides 128-512 cells (seeded), "
+                    "resolution 10, eps 16x16 blocks in [1, 12], 10-cell uPML, Gaussian Ez point "
+                    "source fcen in [0.1, 0.3] df = 0.1 fcen, 16-frequency flux line with FP64 "
+                    "accumulators; one launch per launch group for all simulations, "
+                    f"{'eager' if args.batch_eager else 'hipGraph'} step loop, autotuned block size {block}",
+        "cells": cells, "simulations": BATCH_N, "resolution": BATCH_A, "courant": 0.5, "parallelism": "none"},
+        steps, warmup)
+    line["roofline"] = {"bound": "hbm", "kernel": "whole batched step (all launches)",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_step": alg}
+    line["cpu_baseline"] = None
+    return line
+
+
+def run_cpu_c1(args):
+    """The 2D straight waveguide (320 x 160 cells, PML, 8000 steps, 100-frequency flux line) in
+    full on the CPU baseline build, FP64: median wall time of `repeats` whole runs at one
+    thread and at every thread, with the phase breakdown of the last run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from scenarios import straight_waveguide
+    lib = oracle_lib(baseline=True)
+    g = _gomp()
+    allt = g.omp_get_max_threads()
+    out = {}
+    for threads in (1, allt):
+        g.omp_set_num_threads(threads)
+        runs = []
+        for _ in range(args.repeats):
+            s, f, (fl,) = straight_waveguide(lib)
+            t0 = time.perf_counter()
+            f.step_n(8000)
+            runs.append(time.perf_counter() - t0)
+            print(f"[bench] C1 on {threads} threads: {runs[-1]:.2f} s", file=sys.stderr, flush=True)
+        out[f"threads_{threads}"] = {"median_s": round(statistics.median(runs), 3), "runs_s": [round(r, 3) for r in runs],
+                                     "gcell_per_s": round(320 * 160 * 8000 / statistics.median(runs) / 1e9, 5),
+                                     "phase_s": dict(zip(PHASES, [round(x, 3) for x in f.times()]))}
+    g.omp_set_num_threads(allt)
+    return {"workload": "2D TM straight waveguide 320x160 cells, PML 1, 8000 steps, 100-frequency flux "
+                        "line, FP64 (the reference's CPU configuration)", "kind": "port",
+            "host": f"{platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs", **out}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--workload", choices=["box256", "pml_lorentz_dft", "adjoint2d", "batch2d"],
-                    default="box256",
-                    help="box256: 3D Bloch-periodic 256^3 box (the headline); "
-                         "pml_lorentz_dft: 384x384x192 with PML, two poles and a DFT plane; "
-                         "adjoint2d: 600x400 forward run plus its reverse-mode gradient; "
-                         "batch2d: 64 uneven 2D simulations stepped together")
+    ap.add_argument("--workload", choices=["pml_lorentz_dft", "box256", "adjoint2d", "batch2d"],
+                    default="pml_lorentz_dft",
+                    help="pml_lorentz_dft: 384x384x192 with uPML, two poles and a DFT plane (the "
+                         "headline); box256: 3D Bloch-periodic 256^3 box; adjoint2d: 600x400 "
+                         "forward run plus its reverse-mode gradient; batch2d: 64 uneven 2D "
+                         "simulations stepped together")
     ap.add_argument("--checkpoint-interval", type=int, default=None,
                     help="adjoint2d: steps between checkpoints (default sqrt(steps); 0 records "
                          "the design history every step)")
-    ap.add_argument("--fd-pixels", type=int, default=0,
-                    help="adjoint2d: check the gradient on this many pixels by central "
-                         "differences of FP64 runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="headline workload only (no box256 / adjoint2d / batch2d keys)")
     ap.add_argument("--batch-eager", action="store_true",
                     help="batch2d: launch the step loop eagerly instead of through a hipGraph "
                          "(kernel traces do not list graph-launched kernels)")
+    ap.add_argument("--cpu-c1", action="store_true",
+                    help="only time the 2D straight waveguide in full on the CPU baseline build")
+    ap.add_argument("--repeats", type=int, default=5, help="--cpu-c1: runs per thread count")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy 5 times (the reference "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -560,6 +637,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": n}))
         return
+    if args.cpu_c1:
+        print(json.dumps(run_cpu_c1(args)))
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs one chunk on one GPU; runs on more than one GPU are "
               "not supported (the step would shard as X slabs with one halo-plane exchange "
@@ -569,6 +649,19 @@ def main():
     run = {"box256": run_box256, "pml_lorentz_dft": run_pml_lorentz_dft, "adjoint2d": run_adjoint2d,
            "batch2d": run_batch2d}[args.workload]
     line = run(args, lib)
+    if args.workload == "pml_lorentz_dft" and not args.no_secondary:
+        sub = argparse.Namespace(**vars(args))
+        sub.no_cpu_baseline = True
+        sec = {}
+        b = run_box256(sub, lib, steps=200, warmup=20)
+        sec["box256"] = {k: b[k] for k in ("value", "ms_per_step", "steps", "warmup", "roofline")}
+        sec["box256"]["workload"] = b["config"]["workload"]
+        a = run_adjoint2d(sub, lib)
+        sec["adjoint2d"] = dict(a["adjoint"], workload=a["config"]["workload"], value=a["value"])
+        c = run_batch2d(sub, lib, steps=200, warmup=20)
+        sec["batch2d"] = {k: c[k] for k in ("value", "ms_per_step", "steps", "warmup", "roofline")}
+        sec["batch2d"]["workload"] = c["config"]["workload"]
+        line["secondary"] = sec
     print(json.dumps(line))
 
 

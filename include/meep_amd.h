@@ -166,9 +166,12 @@ int mp_fields_have_component(mp_fields *f, int c);
 int mp_fields_get_times(mp_fields *f, double *times7);
 /* Kernel timing with HIP events recorded on the engine's stream around every launch (no
    counterpart in the reference; feeds the time sinks above and bench.py's roofline).
-   which: MP_K_STEP_BH (curl B + H update), MP_K_STEP_DE (curl D + E/P update), MP_K_HALO,
-   MP_K_DFT, MP_K_SRC.  Times accumulate in milliseconds until mp_fields_reset_kernel_timing. */
-enum { MP_K_STEP_BH = 0, MP_K_STEP_DE = 1, MP_K_HALO = 2, MP_K_DFT = 3, MP_K_SRC = 4 };
+   which: MP_K_STEP_BH (curl B + H update: every launch of the phase), MP_K_STEP_DE (curl D +
+   E/P update: every launch of the phase), MP_K_HALO, MP_K_DFT, MP_K_SRC, and the interior-box
+   launch alone of each phase, MP_K_BH_INTERIOR and MP_K_DE_INTERIOR.  Times accumulate in
+   milliseconds until mp_fields_reset_kernel_timing. */
+enum { MP_K_STEP_BH = 0, MP_K_STEP_DE = 1, MP_K_HALO = 2, MP_K_DFT = 3, MP_K_SRC = 4, MP_K_BH_INTERIOR = 5,
+       MP_K_DE_INTERIOR = 6 };
 int mp_fields_set_kernel_timing(mp_fields *f, int on);
 int mp_fields_get_kernel_timing(mp_fields *f, int which, double *total_ms, long *launches);
 int mp_fields_reset_kernel_timing(mp_fields *f);

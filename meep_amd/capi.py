@@ -17,7 +17,7 @@ X, Y, Z = 0, 1, 2
 High, Low = 0, 1
 Periodic, Metallic, Magnetic, NoBoundary = 0, 1, 2, 3
 ARR_F, ARR_F_U, ARR_F_W, ARR_P, ARR_P_PREV = 0, 1, 2, 3, 4
-K_STEP_BH, K_STEP_DE, K_HALO, K_DFT, K_SRC = 0, 1, 2, 3, 4
+K_STEP_BH, K_STEP_DE, K_HALO, K_DFT, K_SRC, K_BH_INTERIOR, K_DE_INTERIOR = 0, 1, 2, 3, 4, 5, 6
 DFT_REALNUM, DFT_F64, DFT_KAHAN32 = 0, 1, 2
 
 component_name = {Ex: "ex", Ey: "ey", Ez: "ez", Hx: "hx", Hy: "hy", Hz: "hz",

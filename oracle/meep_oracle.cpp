@@ -53,6 +53,7 @@
 #include "../include/meep_amd.h"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <complex>
@@ -386,10 +387,14 @@ template <class F> inline void loop_ivecs(const GV &gv, const ivec &is, const iv
 }
 // The same loop with the two outer counters shared out over OpenMP threads (the reference's
 // collapse(2) loops, src/meep/vec.hpp:201-230), for bodies that touch only their own point.
+// Loops of fewer than ORACLE_PAR_MIN points stay on one thread (waking the team costs more
+// than they take); the CPU-baseline build lowers it.
+#ifndef ORACLE_PAR_MIN
+#define ORACLE_PAR_MIN (1 << 16)
+#endif
 template <class F> inline void loop_ivecs_par(const GV &gv, const ivec &is, const ivec &ie, F body) {
   const Loop L = make_loop(gv, is, ie);
-  // small loops stay on one thread: waking the team costs more than they take
-#pragma omp parallel for collapse(2) schedule(static) if (L.n[0] * L.n[1] * L.n[2] >= (1 << 16))
+#pragma omp parallel for collapse(2) schedule(static) if (L.n[0] * L.n[1] * L.n[2] >= ORACLE_PAR_MIN)
   for (ptrdiff_t i1 = 0; i1 < L.n[0]; ++i1)
     for (ptrdiff_t i2 = 0; i2 < L.n[1]; ++i2)
       for (ptrdiff_t i3 = 0; i3 < L.n[2]; ++i3) {
@@ -1147,29 +1152,54 @@ template <class R> struct Fields {
   }
 
   // ---- fields::step (src/step.cpp:35-139) ----
+  // wall time per phase under the reference's time sinks (src/meep.hpp:1595-1618, the
+  // timing scopes of src/step.cpp:66-122): 0 FieldUpdateB, 1 FieldUpdateH, 2 FieldUpdateD,
+  // 3 FieldUpdateE (with the polarisations), 4 BoundarySteppingB..E, 5 FourierTransforming,
+  // 6 the whole step
+  static double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
   void step() {
+    const double t0 = now();
+    double ta = t0, tb;
+    auto sink = [&](int k) {
+      tb = now();
+      times[k] += tb - ta;
+      ta = tb;
+    };
     calc_sources(t * dt);
     if (step_db(B_stuff)) connections_valid = false;
     step_source(B_stuff);
+    sink(0);
     step_boundaries(B_stuff);
+    sink(4);
     calc_sources(t * dt + 0.5 * dt);
     if (update_eh(H_stuff)) connections_valid = false;
+    sink(1);
     step_boundaries(WH_stuff);
+    sink(4);
     if (update_pols(H_stuff)) connections_valid = false;
+    sink(1);
     step_boundaries(PH_stuff);
     step_boundaries(H_stuff);
+    sink(4);
     calc_sources(t * dt + 0.5 * dt);
     if (step_db(D_stuff)) connections_valid = false;
     step_source(D_stuff);
+    sink(2);
     step_boundaries(D_stuff);
+    sink(4);
     calc_sources(t * dt + dt);
     if (update_eh(E_stuff)) connections_valid = false;
+    sink(3);
     step_boundaries(WE_stuff);
+    sink(4);
     if (update_pols(E_stuff)) connections_valid = false;
+    sink(3);
     step_boundaries(PE_stuff);
     step_boundaries(E_stuff);
+    sink(4);
     t += 1;
     update_dfts();
+    sink(5);
     // get_field(D_EnergyDensity, gv.center()) must be finite (src/step.cpp:137-138)
     vec3 cen = gv.loc(gv.icenter());
     double sum = 0;
@@ -1177,6 +1207,7 @@ template <class R> struct Fields {
       if (has_field(c) && component_direction(c) != NODIR)
         sum += real(conj(get_field(c, cen)) * get_field(c + 10, cen));
     if (!std::isfinite(sum)) fail("simulation fields are NaN or Inf");
+    times[6] += now() - t0;
   }
 
   // ---- field energy (src/energy_and_flux.cpp:45-178, src/integrate.cpp:46-205) ----
