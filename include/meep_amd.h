@@ -39,7 +39,7 @@ enum { MP_High = 0, MP_Low = 1 };
 /* boundary_condition (src/meep.hpp:1594) */
 enum { MP_Periodic = 0, MP_Metallic = 1, MP_Magnetic = 2, MP_None = 3 };
 /* state arrays reachable through mp_fields_get_array / mp_fields_set_array */
-enum { MP_ARR_F = 0, MP_ARR_F_U = 1, MP_ARR_F_W = 2, MP_ARR_P = 3, MP_ARR_P_PREV = 4 };
+enum { MP_ARR_F = 0, MP_ARR_F_U = 1, MP_ARR_F_W = 2, MP_ARR_P = 3, MP_ARR_P_PREV = 4, MP_ARR_F_COND = 5 };
 /* DFT accumulator storage: the reference's complex<realnum> (src/meep.hpp:1138), FP64, or
    compensated (Kahan) complex<float>. */
 enum { MP_DFT_REALNUM = 0, MP_DFT_F64 = 1, MP_DFT_KAHAN32 = 2 };
@@ -107,6 +107,14 @@ int mp_structure_add_susceptibility(mp_structure *s, double omega_0, double gamm
 /* Overwrite sigma[c][dir(c)] of susceptibility isus on all ntot points. */
 int mp_structure_set_sigma(mp_structure *s, int isus, int c, const double *sigma);
 
+/* structure::set_conductivity(component c, double C(const vec &)) — src/meep.hpp:855-856,
+   src/structure.cpp:425-437,868-905.  c is an E, H, D or B component; the conductivity is
+   stored for the D (B) component along c's direction, sampled at that component's points,
+   and for an E (H) component multiplied by c's chi1inv as it stands at this call (so set
+   epsilon first, as in the reference).  A zero function removes it.  Applies to fields
+   objects created afterwards. */
+int mp_structure_set_conductivity(mp_structure *s, int c, mp_scalar_func C, void *user);
+
 /* ---- fields ----------------------------------------------------------------------- */
 
 /* fields::fields(structure*, m = 0, beta = 0) — src/meep.hpp:1740-1742, src/fields.cpp:32-86.
@@ -152,7 +160,8 @@ int mp_fields_field_energy(mp_fields *f, double *energy);
 /* fields::get_field(component, const vec&) — src/meep.hpp:2183-2185, src/monitor.cpp:93-108 */
 int mp_fields_get_field(mp_fields *f, int c, const double *p, double *re, double *im);
 /* Copy one state array (reference layout, ntot doubles) to the host: kind MP_ARR_F is
-   fields_chunk::f, MP_ARR_F_U f_u, MP_ARR_F_W f_w, MP_ARR_P / MP_ARR_P_PREV the
+   fields_chunk::f, MP_ARR_F_U f_u, MP_ARR_F_W f_w, MP_ARR_F_COND f_cond (the conductivity
+   auxiliary where uPML acts on f, src/step_db.cpp:66-69), MP_ARR_P / MP_ARR_P_PREV the
    polarisation of susceptibility ipol (lorentzian_data, src/susceptibility.cpp:98-139).
    Returns 1 (and zeros) when the array is not allocated. */
 int mp_fields_get_array(mp_fields *f, int c, int cmp, int kind, int ipol, double *out);

@@ -291,6 +291,10 @@ public:
     check(mp_structure_add_susceptibility(p, sus.omega_0, sus.gamma, sus.drude, detail::eps_trampoline,
                                           reinterpret_cast<void *>(sigma)));
   }
+  // set_conductivity(component, double C(const vec &)) (src/meep.hpp:856, src/structure.cpp:435-438)
+  void set_conductivity(component c, double C(const vec &)) {
+    check(mp_structure_set_conductivity(p, int(c), detail::eps_trampoline, reinterpret_cast<void *>(C)));
+  }
   const grid_volume &gv() const { return gv_; }
   mp_structure *handle() { return p; }
 

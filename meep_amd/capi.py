@@ -16,7 +16,7 @@ Dx, Dy, Dz, Bx, By, Bz = 10, 11, 14, 15, 16, 19
 X, Y, Z = 0, 1, 2
 High, Low = 0, 1
 Periodic, Metallic, Magnetic, NoBoundary = 0, 1, 2, 3
-ARR_F, ARR_F_U, ARR_F_W, ARR_P, ARR_P_PREV = 0, 1, 2, 3, 4
+ARR_F, ARR_F_U, ARR_F_W, ARR_P, ARR_P_PREV, ARR_F_COND = 0, 1, 2, 3, 4, 5
 K_STEP_BH, K_STEP_DE, K_HALO, K_DFT, K_SRC, K_BH_INTERIOR, K_DE_INTERIOR = 0, 1, 2, 3, 4, 5, 6
 DFT_REALNUM, DFT_F64, DFT_KAHAN32 = 0, 1, 2
 
@@ -63,6 +63,7 @@ _SIGNATURES = {
     "mp_structure_add_susceptibility": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int,
                                                   SCALAR_FUNC, C.c_void_p]),
     "mp_structure_set_sigma": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mp_structure_set_conductivity": (C.c_int, [C.c_void_p, C.c_int, SCALAR_FUNC, C.c_void_p]),
     "mp_fields_new": (C.c_void_p, [C.c_void_p]),
     "mp_fields_delete": (None, [C.c_void_p]),
     "mp_fields_use_real_fields": (C.c_int, [C.c_void_p]),
@@ -227,6 +228,13 @@ class structure:
         self._keep.append(cb)
         return _check(self.lib, self.lib.mp_structure_add_susceptibility(
             self.ptr, omega_0, gamma, int(drude), cb, None))
+
+    def set_conductivity(self, c, cond):
+        """structure::set_conductivity(c, double C(const vec &)) (src/meep.hpp:855-856): cond is
+        a function of (x, y, z); for an E or H component it is multiplied by chi1inv."""
+        cb = SCALAR_FUNC(lambda r, _u: float(cond(r[0], r[1], r[2])))
+        self._keep.append(cb)
+        _check(self.lib, self.lib.mp_structure_set_conductivity(self.ptr, c, cb, None))
 
     def set_sigma(self, isus, c, arr):
         arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)

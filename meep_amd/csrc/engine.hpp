@@ -69,6 +69,12 @@ template <class R> struct CurlComp {
   int64_t s1, s2;   // partner strides (negated for the D update, src/step_db.cpp:81-84)
   R dtdx;
   R *fu;     // PML auxiliary (dsigu); null when absent
+  // conductivity (null when none) and 1 / (1 + cnd dt / 2); fcnd is the conductivity
+  // auxiliary where uPML acts on f (src/step_generic.cpp:85-248, src/step_db.cpp:66-69).
+  // Chunks with conductivity run the generic kernel only.
+  const R *cnd, *cndinv;
+  R *fcnd;
+  R dt2; // dt / 2 in realnum
   int dsig;  // PML direction acting on f, -1 if none
   int dsigu; // PML direction acting through fu, -1 if none
   int lo[3], hi[3]; // owned index range per direction (inclusive)

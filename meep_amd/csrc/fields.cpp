@@ -139,6 +139,26 @@ struct HostStructure {
   std::vector<double> sig[3], kap[3];
   bool pml_dir[3] = {false, false, false};
   std::vector<HostSus> sus;
+  // conductivity[c][dir(c)] by D or B component c (src/meep.hpp:599), in realnum precision;
+  // empty when trivial
+  std::vector<double> cond[NC];
+  bool any_cond() const {
+    for (int c = 0; c < NC; ++c)
+      if (!cond[c].empty()) return true;
+    return false;
+  }
+  // condinv = 1 / (1 + conductivity dt / 2) in realnum (structure_chunk::update_condinv,
+  // src/structure.cpp:693-707), reference layout; empty when c has no conductivity
+  std::vector<double> condinv(int c) const {
+    std::vector<double> v;
+    if (cond[c].empty()) return v;
+    v.resize(cond[c].size());
+    for (size_t i = 0; i < v.size(); ++i) {
+      const double q = 1 / (1 + cond[c][i] * dt * 0.5);
+      v[i] = single ? double(float(q)) : q;
+    }
+    return v;
+  }
 };
 
 // structure_chunk::use_pml (src/structure.cpp:625-691) with the quadratic profile of pml()
@@ -348,10 +368,15 @@ template <class R> struct Fields : FieldsBase {
   bool components_allocated = false;
   hipStream_t stream = nullptr;
 
-  R *f[NC][2] = {}, *f_u[NC][2] = {}, *f_w[NC][2] = {};
+  R *f[NC][2] = {}, *f_u[NC][2] = {}, *f_w[NC][2] = {}, *f_cond[NC][2] = {};
   std::vector<std::unique_ptr<DevBuf>> bufs;
   // materials and uPML on the device
   R *d_chi1inv[NC] = {};
+  // conductivity of the D and B components and its condinv (device layout), and condinv on
+  // the host (reference layout) for the current sources; empty when absent
+  R *d_cond[NC] = {}, *d_condinv[NC] = {};
+  std::vector<double> h_condinv[NC];
+  bool any_cond = false;
   R *d_sig[3] = {}, *d_kap[3] = {}, *d_siginv[3] = {};
   struct Pol {
     int isus;
@@ -508,6 +533,13 @@ template <class R> struct Fields : FieldsBase {
     // materials
     for (int c = 0; c < 5; ++c)
       if (!s->chi1inv[c].empty()) d_chi1inv[c] = upload(to_device(s->chi1inv[c], R(1)));
+    for (int c = 10; c < NC; ++c)
+      if (!s->cond[c].empty()) {
+        h_condinv[c] = s->condinv(c);
+        d_cond[c] = upload(to_device(s->cond[c], R(0)));
+        d_condinv[c] = upload(to_device(h_condinv[c], R(1)));
+        any_cond = true;
+      }
     for (int d = 0; d < 3; ++d)
       if (s->pml_dir[d]) {
         size_t n = s->sig[d].size();
@@ -894,6 +926,8 @@ template <class R> struct Fields : FieldsBase {
         if (k.g2 && !k.g1) fail("internal: curl slot %d has g2 without g1", c);
         if (k.dsigu >= 0 && !k.fu) fail("internal: curl slot %d lacks its PML auxiliary", c);
         if ((k.dsig >= 0 || k.dsigu >= 0) && (!a.pml.sig[k.dsig >= 0 ? k.dsig : k.dsigu])) fail("internal: missing PML profile");
+        if (k.cnd && (!k.cndinv || (k.dsig >= 0 && !k.fcnd))) fail("internal: curl slot %d lacks its conductivity arrays", c);
+        if (k.cnd && fast) fail("internal: conductive slot %d in the vectorised kernel", c);
         if (fast) {
           if (a.hasp[c] && !a.F[(c + 2) % 3]) fail("internal: fast slot %d lacks its plus partner", c);
           if (a.hasm[c] && !a.F[(c + 1) % 3]) fail("internal: fast slot %d lacks its minus partner", c);
@@ -1153,7 +1187,11 @@ template <class R> struct Fields : FieldsBase {
     cur_phase = kind == MP_K_STEP_BH ? 0 : 1;
     kbegin(kind);
     StepArgs<R> b = a;
-    const bool fast = use_fast && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b));
+    // conductive curls run the generic kernel (the vectorised one carries no conductivity)
+    bool cond_curl = false;
+    if (curl)
+      for (int d = 0; d < 3; ++d) cond_curl = cond_curl || (b.cc[d].f && b.cc[d].cnd);
+    const bool fast = use_fast && !cond_curl && (!curl || fast_partners(kind == MP_K_STEP_BH ? B_ft : D_ft, cur_cmp, b));
     if (!interior_empty) {
       // backward differences read the previous plane: walk the planes downwards so it was
       // fetched just before (forward differences prefetch their neighbour naturally)
@@ -1294,8 +1332,14 @@ template <class R> struct Fields : FieldsBase {
       k.dsig = pml_in(dsig0) ? dsig0 : -1;
       k.dsigu = pml_in(dsigu0) ? dsigu0 : -1;
       if (k.dsigu >= 0 && !f_u[cc_][cmp]) f_u[cc_][cmp] = dalloc(f[cc_][cmp]);
+      // f_cond where uPML acts on f and the component conducts (src/step_db.cpp:66-69)
+      if (k.dsig >= 0 && d_cond[cc_] && !f_cond[cc_][cmp]) f_cond[cc_][cmp] = dalloc(nullptr);
       k.f = f[cc_][cmp];
       k.fu = f_u[cc_][cmp];
+      k.cnd = d_cond[cc_];
+      k.cndinv = d_condinv[cc_];
+      k.fcnd = f_cond[cc_][cmp];
+      k.dt2 = R(dt * 0.5);
       const R *g1 = have_p[cc_] ? f[plus_c[cc_]][cmp] : nullptr;
       const R *g2 = have_m[cc_] ? f[minus_c[cc_]][cmp] : nullptr;
       int64_t s1 = have_p[cc_] ? gv.st[plus_d[cc_]] : 0, s2 = have_m[cc_] ? gv.st[minus_d[cc_]] : 0;
@@ -2030,6 +2074,7 @@ template <class R> struct Fields : FieldsBase {
         save(f[c][cmp], true);
         save(f_u[c][cmp], false);
         save(f_w[c][cmp], false);
+        save(f_cond[c][cmp], false);
       }
       for (int c = 5; c < 10; ++c)
         if (f[c][cmp] && f[c][cmp] != f[c + 10][cmp]) {
@@ -2087,11 +2132,13 @@ template <class R> struct Fields : FieldsBase {
       add(f[c][0]);
       add(f_u[c][0]);
       add(f_w[c][0]);
+      add(f_cond[c][0]);
     }
     return v;
   }
   void adj_begin(void *monitor, int c, V3 mn, V3 mxv, int interval, int *box_lo, int *box_n) override {
     if (!is_real) fail("the reverse mode needs real fields (use_real_fields)");
+    if (any_cond) fail("the reverse mode does not support conductivity");
     if (!pols.empty()) fail("the reverse mode does not support dispersive materials");
     for (auto &lst : sources)
       for (auto &sv : lst)
@@ -2818,6 +2865,14 @@ template <class R> struct Fields : FieldsBase {
         if (i < 0 || size_t(i) >= gv.ntot) fail("internal: source point outside the array");
       const int ft = (is_magnetic(c) || is_B(c)) ? B_ft : D_ft;
       const int cc = (is_D(c) || is_B(c)) ? c - 10 : c;
+      // a current in a conducting medium is scaled by condinv at its point
+      // (fields_chunk::step_source, src/step.cpp:299-309); integrated sources are not
+      const int cD = dir_comp(ft == B_ft ? 15 : 10, comp_dir(c));
+      if (!srcs[isrc].is_integrated && !h_condinv[cD].empty())
+        for (size_t q = 0; q < amps.size(); ++q) {
+          const int j[3] = {jpos[3 * q], jpos[3 * q + 1], jpos[3 * q + 2]};
+          amps[q] *= h_condinv[cD][gv.ref_of(j)];
+        }
       ++src_version;
       for (SrcVol &o : sources[ft])
         if (o.c == cc && o.src == isrc && o.index == index) {
@@ -3059,6 +3114,7 @@ template <class R> struct Fields : FieldsBase {
     if (kind == MP_ARR_F) return f[c][cmp];
     if (kind == MP_ARR_F_U) return f_u[c][cmp];
     if (kind == MP_ARR_F_W) return f_w[c][cmp];
+    if (kind == MP_ARR_F_COND) return f_cond[c][cmp];
     for (auto &p : pols)
       if (p.isus == ipol && p.allocated) return kind == MP_ARR_P ? p.P[c][cmp] : p.Pp[c][cmp];
     return nullptr;
@@ -3555,6 +3611,39 @@ int mp_structure_add_susceptibility(mp_structure *sp, double omega_0, double gam
   });
   return rc ? rc : idx;
 }
+// structure_chunk::set_conductivity (src/structure.cpp:868-905): sampled at the D (B)
+// component's points over the whole array; an E (H) component's value is multiplied by its
+// chi1inv (the magnetic chi1inv is 1 here, mu = 1)
+int mp_structure_set_conductivity(mp_structure *sp, int c, mp_scalar_func C, void *user) {
+  return guard([&] {
+    HostStructure *s = reinterpret_cast<HostStructure *>(sp);
+    const Geom &gv = s->gv;
+    if (c < 0 || c >= NC || comp_dir(c) == NODIR || !C) fail("invalid component for conductivity");
+    const int d = comp_dir(c);
+    const int cC = is_electric(c) ? dir_comp(10, d) : is_magnetic(c) ? dir_comp(15, d) : c;
+    if (gv.dim == 1 && cC != 10 && cC != 16) return; // the component is absent from the grid
+    const std::vector<double> *multby = is_electric(c) && !s->chi1inv[c].empty() ? &s->chi1inv[c] : nullptr;
+    auto rn = [&](double x) { return s->single ? double(float(x)) : x; };
+    std::vector<double> v(gv.ntot, 0.0);
+    bool trivial = true;
+    for (int j0 = 0; j0 <= (gv.has(0) ? gv.num[0] : 0); ++j0)
+      for (int j1 = 0; j1 <= (gv.has(1) ? gv.num[1] : 0); ++j1)
+        for (int j2 = 0; j2 <= (gv.has(2) ? gv.num[2] : 0); ++j2) {
+          int j[3] = {j0, j1, j2};
+          IV here = gv.iloc(cC, j);
+          V3 h = gv.loc(here);
+          double r[3] = {h.t[0], h.t[1], h.t[2]};
+          const int64_t i = gv.index(cC, here);
+          const double val = multby ? rn(C(r, user) * rn((*multby)[i])) : rn(C(r, user));
+          v[i] = val;
+          if (val != 0.0) trivial = false;
+        }
+    if (trivial)
+      s->cond[cC].clear();
+    else
+      s->cond[cC] = std::move(v);
+  });
+}
 int mp_structure_set_sigma(mp_structure *sp, int isus, int c, const double *sigma) {
   return guard([&] {
     HostStructure *s = reinterpret_cast<HostStructure *>(sp);
@@ -3634,6 +3723,7 @@ int mp_fields_step(mp_fields *f) {
 int mp_fields_step_n(mp_fields *f, long n) {
   return guard([&] { F_->step_n(n, false); });
 }
+
 double mp_fields_time(const mp_fields *f) {
   auto *b = reinterpret_cast<const FieldsBase *>(f);
   return b->t_now() * b->dt_now();

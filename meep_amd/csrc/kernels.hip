@@ -131,7 +131,29 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
     const int ku = k.ksigu >= 0 ? 2 * asel(j0, j1, j2, k.ksigu) + k.ysigu : 0;
     const bool pml = pml_on(k.sig, k.kap, kk);
     const bool pmlu = pml_on(k.sigu, k.kapu, ku);
-    if (!pmlu) {
+    if (k.cnd) {
+      // conductivity: the curl term is damped by (1 - dt/2 cnd) / (1 + dt/2 cnd); where uPML
+      // acts on f it goes through fcnd, whose increment the sigma update takes
+      // (src/step_generic.cpp:85-96,108-128,158-177,198-220)
+      const R cn = k.cnd[i], ci = k.cndinv[i];
+      R fprev = R(0), g = fv;
+      if (pmlu) fprev = g = k.fu[i];
+      if (pml) {
+        const R fc = k.fcnd[i];
+        const R fcn = ((R(1) - k.dt2 * cn) * fc - k.dtdx * cu) * ci;
+        k.fcnd[i] = fcn;
+        g = ((k.kap[kk] - k.sig[kk]) * g + (fcn - fc)) * k.siginv[kk];
+      }
+      else
+        g = ((R(1) - k.dt2 * cn) * g - k.dtdx * cu) * ci;
+      if (pmlu) {
+        k.fu[i] = g;
+        fv = k.siginvu[ku] * ((k.kapu[ku] - k.sigu[ku]) * fv + g - fprev);
+      }
+      else
+        fv = g;
+    }
+    else if (!pmlu) {
       if (!pml)
         fv -= k.dtdx * cu;
       else

@@ -49,6 +49,10 @@
  *   MUT_DRUDE         a Drude pole keeps the restoring term of a Lorentz pole
  *   MUT_GAMMA         the pole damping uses gamma dt instead of pi gamma dt
  *   MUT_MAGNETIC      a Low magnetic wall zeroes the metallic wall's plane instead
+ *   MUT_COND_HALF     the conductive curl update uses dt instead of dt/2
+ *   MUT_COND_CHI      an electric conductivity is not multiplied by chi1inv
+ *   MUT_COND_SRC      a current source in a conductive medium is not scaled by condinv
+ *   MUT_COND_PML      the PML-with-conductivity branch drops its (kap - sig) factor
  */
 #include "../include/meep_amd.h"
 
@@ -504,6 +508,10 @@ template <class R> struct Structure {
   GV gv;
   double courant = 0.5, dt = 0.5, a = 1;
   std::vector<R> chi1inv[NC]; // diagonal only; empty == NULL (trivial)
+  // conductivity[c][dir(c)] by D or B component c (src/meep.hpp:599); empty == NULL; condinv
+  // = 1 / (1 + conductivity dt / 2), recomputed when stale (src/structure.cpp:693-707)
+  std::vector<R> cond[NC], condinv[NC];
+  bool condinv_stale = false;
   std::vector<R> sig[3], kap[3], siginv[3];
   int sigsize[6] = {0, 0, 0, 0, 0, 0};
   std::vector<Sus<R>> sus; // insertion order; the reference's chiP list is newest first
@@ -557,6 +565,20 @@ template <class R> void use_pml(Structure<R> &s, int d, double dx, double bloc, 
   }
 }
 
+// structure_chunk::update_condinv (src/structure.cpp:693-707)
+template <class R> void update_condinv(Structure<R> &s) {
+  if (!s.condinv_stale) return;
+  for (int c = 0; c < NC; ++c) {
+    if (!s.cond[c].empty()) {
+      s.condinv[c].assign(s.gv.ntot, R(1));
+      for (size_t i = 0; i < s.gv.ntot; ++i) s.condinv[c][i] = 1 / (1 + s.cond[c][i] * s.dt * 0.5);
+    }
+    else
+      s.condinv[c].clear();
+  }
+  s.condinv_stale = false;
+}
+
 // ---- fields --------------------------------------------------------------------------------
 struct SrcVol {
   int c;       // E or H component
@@ -587,7 +609,7 @@ template <class R> struct Fields {
   double a, dt, courant;
   int t = 0;
   bool is_real = false;
-  R *f[NC][2], *f_u[NC][2], *f_w[NC][2], *fmp[NC][2];
+  R *f[NC][2], *f_u[NC][2], *f_w[NC][2], *fmp[NC][2], *f_cond[NC][2];
   std::vector<std::unique_ptr<R[]>> pool;
   int boundaries[2][3];
   cdouble k[3], eikna[3];
@@ -630,7 +652,7 @@ template <class R> struct Fields {
     dt = s->dt;
     courant = s->courant;
     for (int c = 0; c < NC; ++c)
-      for (int cmp = 0; cmp < 2; ++cmp) f[c][cmp] = f_u[c][cmp] = f_w[c][cmp] = fmp[c][cmp] = nullptr;
+      for (int cmp = 0; cmp < 2; ++cmp) f[c][cmp] = f_u[c][cmp] = f_w[c][cmp] = fmp[c][cmp] = f_cond[c][cmp] = nullptr;
     for (int d = 0; d < 3; ++d) {
       k[d] = 0.0;
       eikna[d] = 1.0;
@@ -853,9 +875,11 @@ template <class R> struct Fields {
     }
   }
 
-  // ---- step_curl (src/step_generic.cpp:65-249), no conductivity ----
+  // ---- curl ----
+  // step_curl (src/step_generic.cpp:65-249): the eight branches {no PML, PML in f, PML via
+  // fu, both} x {conductivity cnd / cndinv (with the fcnd auxiliary when PML acts on f), none}
   void step_curl(R *fp, const R *g1, const R *g2, ptrdiff_t s1, ptrdiff_t s2, const ivec &is, const ivec &ie, R dtdx,
-                 int dsig, R *fu, int dsigu) {
+                 int dsig, R *fu, int dsigu, const R *cnd, const R *cndinv, R *fcnd) {
     if (!g1) {
       std::swap(g1, g2);
       std::swap(s1, s2);
@@ -865,9 +889,44 @@ template <class R> struct Fields {
             *siginv = dsig != NODIR ? s->siginv[dsig].data() : nullptr;
     const R *sigu = dsigu != NODIR ? s->sig[dsigu].data() : nullptr, *kapu = dsigu != NODIR ? s->kap[dsigu].data() : nullptr,
             *siginvu = dsigu != NODIR ? s->siginv[dsigu].data() : nullptr;
+#ifdef MUT_COND_HALF
+    const R dt2 = R(dt);
+#else
+    const R dt2 = R(dt * 0.5);
+#endif
     loop_ivecs_par(gv, is, ie, [&](ptrdiff_t i, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
       ivec here = loop_iloc(L, i1, i2, i3);
       R c = g2 ? (g1[i + s1] - g1[i] + g2[i] - g2[i + s2]) : (g1[i + s1] - g1[i]);
+      if (cnd) {
+        // The reference breaks the uPML regions off into chunks of their own
+        // (src/structure.cpp:128-137), so a point outside every uPML region is stepped with
+        // dsig = dsigu = NO_DIRECTION.  Without conductivity the branches agree there
+        // (kap = 1, sig = 0); with it they do not (f's increments would bypass the damping
+        // of fu and fcnd), so each point takes the branch of its region here.
+        const int kk = dsig != NODIR ? here.t[dsig] : 0, ku = dsigu != NODIR ? here.t[dsigu] : 0;
+        const bool pml = dsig != NODIR && (sig[kk] != 0 || kap[kk] != 1);
+        const bool pmlu = dsigu != NODIR && (sigu[ku] != 0 || kapu[ku] != 1);
+        if (!pml && !pmlu)
+          fp[i] = ((1 - dt2 * cnd[i]) * fp[i] - dtdx * c) * cndinv[i];
+        else if (!pml) {
+          R fprev = fu[i];
+          fu[i] = ((1 - dt2 * cnd[i]) * fprev - dtdx * c) * cndinv[i];
+          fp[i] = siginvu[ku] * ((kapu[ku] - sigu[ku]) * fp[i] + fu[i] - fprev);
+        }
+        else {
+          R *g = pmlu ? fu : fp;
+          const R gprev = pmlu ? fu[i] : R(0);
+          R fcnd_prev = fcnd[i];
+          fcnd[i] = ((1 - dt2 * cnd[i]) * fcnd[i] - dtdx * c) * cndinv[i];
+#ifdef MUT_COND_PML
+          g[i] = (g[i] + (fcnd[i] - fcnd_prev)) * siginv[kk];
+#else
+          g[i] = ((kap[kk] - sig[kk]) * g[i] + (fcnd[i] - fcnd_prev)) * siginv[kk];
+#endif
+          if (pmlu) fp[i] = siginvu[ku] * ((kapu[ku] - sigu[ku]) * fp[i] + fu[i] - gprev);
+        }
+        return;
+      }
       if (dsig == NODIR) {
         if (dsigu == NODIR)
           fp[i] -= dtdx * c;
@@ -904,6 +963,9 @@ template <class R> struct Fields {
         ptrdiff_t sp = have_p[cc] ? gv.st[plus_d[cc]] : 0, sm = have_m[cc] ? gv.st[minus_d[cc]] : 0;
         const R *fpp = have_p[cc] ? f[plus_c[cc]][cmp] : nullptr;
         const R *fmm = have_m[cc] ? f[minus_c[cc]][cmp] : nullptr;
+        const R *cnd = s->cond[cc].empty() ? nullptr : s->cond[cc].data();
+        const R *cndinv = s->condinv[cc].empty() ? nullptr : s->condinv[cc].data();
+        if (dsig != NODIR && cnd && !f_cond[cc][cmp]) f_cond[cc][cmp] = alloc(nullptr);
         if (dsigu != NODIR && !f_u[cc][cmp]) {
           f_u[cc][cmp] = alloc(f[cc][cmp]);
           allocated_u = true;
@@ -912,7 +974,8 @@ template <class R> struct Fields {
           sp = -sp;
           sm = -sm;
         }
-        step_curl(f[cc][cmp], fpp, fmm, sp, sm, gv.little_owned_corner(cc), gv.big_corner(), R(courant), dsig, f_u[cc][cmp], dsigu);
+        step_curl(f[cc][cmp], fpp, fmm, sp, sm, gv.little_owned_corner(cc), gv.big_corner(), R(courant), dsig, f_u[cc][cmp], dsigu,
+                  cnd, cndinv, f_cond[cc][cmp]);
       }
     return allocated_u;
   }
@@ -921,14 +984,19 @@ template <class R> struct Fields {
     for (const SrcVol &sv : sources[ft]) {
       int c = direction_component(first_field_component(ft), component_direction(sv.c));
       const SrcTime &st = srcs[sv.src];
+      const R *cndinv = s->condinv[c].empty() ? nullptr : s->condinv[c].data();
+#ifdef MUT_COND_SRC
+      cndinv = nullptr;
+#endif
       if (!st.is_integrated && f[c][0] && ((ft == D_stuff && is_electric(sv.c)) || (ft == B_stuff && is_magnetic(sv.c))))
         for (size_t j = 0; j < sv.index.size(); ++j) {
+          ptrdiff_t i = sv.index[j];
 #ifdef MUT_SRC_DT
           cdouble A = sv.amp[j] * st.cur_current;
 #else
           cdouble A = sv.amp[j] * st.cur_current * dt;
 #endif
-          ptrdiff_t i = sv.index[j];
+          if (cndinv) A = sv.amp[j] * st.cur_current * dt * double(cndinv[i]);
           f[c][0][i] -= real(A);
           if (!is_real) f[c][1][i] -= imag(A);
         }
@@ -1165,6 +1233,7 @@ template <class R> struct Fields {
       times[k] += tb - ta;
       ta = tb;
     };
+    update_condinv(*s); // src/step.cpp:60-62
     calc_sources(t * dt);
     if (step_db(B_stuff)) connections_valid = false;
     step_source(B_stuff);
@@ -1258,6 +1327,7 @@ template <class R> struct Fields {
         save(saved, f[c][cmp]);
         save(saved_aux, f_u[c][cmp]);
         save(saved_aux, f_w[c][cmp]);
+        save(saved_aux, f_cond[c][cmp]);
       }
       for (int c = 5; c < 10; ++c)
         if (f[c][cmp] && f[c][cmp] != f[c + 10][cmp]) {
@@ -1266,6 +1336,7 @@ template <class R> struct Fields {
           save(saved_aux, f_w[c][cmp]);
         }
     }
+    update_condinv(*s);
     calc_sources(t * dt);
     if (step_db(B_stuff)) connections_valid = false;
     step_source(B_stuff);
@@ -1766,6 +1837,43 @@ int mp_structure_set_sigma(mp_structure *s, int isus, int c, const double *sigma
   });
 }
 
+// structure_chunk::set_conductivity (src/structure.cpp:868-905): conductivity[c_C][d] of the D
+// (or B) component c_C along c's direction, sampled at c_C's points over the whole array and,
+// for an E (or H) component, multiplied by that component's chi1inv at the time of the call
+int mp_structure_set_conductivity(mp_structure *s, int c, mp_scalar_func C, void *user) {
+  return guard([&] {
+    if (c < 0 || c >= NC || component_direction(c) == NODIR) fail("invalid component for conductivity");
+    with_structure(reinterpret_cast<AnyStructure *>(s), [&](auto &st) {
+      typedef typename std::remove_reference<decltype(st.chi1inv[0][0])>::type R;
+      const GV &gv = st.gv;
+      const int d = component_direction(c);
+      const int cC = is_electric(c) ? direction_component(10, d) : is_magnetic(c) ? direction_component(15, d) : c;
+      if (gv.dim == 1 && cC != 10 && cC != 16) return 0; // !gv.has_field(c)
+      const R *multby = (is_electric(c) || is_magnetic(c)) && !st.chi1inv[c].empty() ? st.chi1inv[c].data() : nullptr;
+#ifdef MUT_COND_CHI
+      multby = nullptr;
+#endif
+      std::vector<R> arr(gv.ntot, R(0));
+      bool trivial = true;
+      ivec s0 = gv.iyee_shift(cC), e0 = gv.big_corner();
+      for (int dd = 0; dd < 3; ++dd)
+        if (gv.has(dd)) e0.t[dd] += s0.t[dd];
+      loop_ivecs(gv, s0, e0, [&](ptrdiff_t i, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
+        vec3 here = gv.loc(loop_iloc(L, i1, i2, i3));
+        double r[3] = {here.t[0], here.t[1], here.t[2]};
+        arr[i] = multby ? R(C(r, user) * multby[i]) : R(C(r, user));
+        trivial = trivial && (arr[i] == 0.0);
+      });
+      if (trivial)
+        st.cond[cC].clear();
+      else
+        st.cond[cC] = std::move(arr);
+      st.condinv_stale = true;
+      return 0;
+    });
+  });
+}
+
 mp_fields *mp_fields_new(mp_structure *s) {
   AnyStructure *as = reinterpret_cast<AnyStructure *>(s);
   AnyFields *af = nullptr;
@@ -1908,6 +2016,7 @@ int mp_fields_get_array(mp_fields *f, int c, int cmp, int kind, int ipol, double
       if (kind == MP_ARR_F) src = F.f[c][cmp];
       if (kind == MP_ARR_F_U) src = F.f_u[c][cmp];
       if (kind == MP_ARR_F_W) src = F.f_w[c][cmp];
+      if (kind == MP_ARR_F_COND) src = F.f_cond[c][cmp];
       if (kind == MP_ARR_P || kind == MP_ARR_P_PREV)
         for (auto &p : F.pols)
           if (p.isus == ipol && p.allocated) src = kind == MP_ARR_P ? p.P[c][cmp] : p.Pp[c][cmp];
@@ -1926,6 +2035,7 @@ int mp_fields_set_array(mp_fields *f, int c, int cmp, int kind, int ipol, const 
       if (kind == MP_ARR_F) dst = F.f[c][cmp];
       if (kind == MP_ARR_F_U) dst = F.f_u[c][cmp];
       if (kind == MP_ARR_F_W) dst = F.f_w[c][cmp];
+      if (kind == MP_ARR_F_COND) dst = F.f_cond[c][cmp];
       if (kind == MP_ARR_P || kind == MP_ARR_P_PREV)
         for (auto &p : F.pols)
           if (p.isus == ipol && p.allocated) dst = kind == MP_ARR_P ? p.P[c][cmp] : p.Pp[c][cmp];

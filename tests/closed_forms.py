@@ -39,7 +39,7 @@ import math
 
 import numpy as np
 
-from meep_amd.capi import (ARR_F, ARR_P, Dx, Ex, Ez, Hy, High, Low, Magnetic, Metallic, X, Z,
+from meep_amd.capi import (ARR_F, ARR_P, By, Dx, Ex, Ez, Hx, Hy, High, Low, Magnetic, Metallic, X, Y, Z,
                            continuous_src_time, fields, gaussian_src_time, structure)
 
 S = 0.5  # Courant number of every check
@@ -65,11 +65,17 @@ def current_transform(fcen, fwidth, dt, nsteps, freqs):
     return np.array([np.sum(cur * np.exp(2j * math.pi * fr * t)) for fr in freqs])
 
 
-def lattice_field(freqs, dt, J, j, j0, half=False):
+def lattice_field(freqs, dt, J, j, j0, half=False, loss=0.0):
     """The exact frequency-domain field of the 1D Yee lattice driven by the current transform
-    J at point j0: E at integer j, or H at j + 1/2 (half=True)."""
+    J at point j0: E at integer j, or H at j + 1/2 (half=True).  loss = sigma dt / 2 of a
+    uniform electric conductivity sigma (eps = 1): the D update (1 + loss) D' - (1 - loss) D =
+    -S curl H - J dt turns one factor sin(theta) of the dispersion relation into
+    sin(theta) + i loss cos(theta), so sin^2(K / 2) = sin(theta) (sin(theta) + i loss
+    cos(theta)) / S^2 with Im K > 0, while the jump at the sheet keeps its form."""
     theta = math.pi * np.asarray(freqs) * dt
-    K = 2 * np.arcsin(np.sin(theta) / S)
+    s2 = np.sin(theta) * (np.sin(theta) + 1j * loss * np.cos(theta)) / (S * S)
+    K = 2 * np.arcsin(np.sqrt(s2.astype(complex)))
+    K = np.where(K.imag < 0, -K, K)
     A = -np.sin(theta) * dt * J / (S * S * np.sin(K))
 
     def E(jj):
@@ -79,12 +85,17 @@ def lattice_field(freqs, dt, J, j, j0, half=False):
     return S * (E(j + 1) - E(j)) / (2j * np.sin(theta))
 
 
-def line_source_1d(lib, a=20, freqs=None):
-    """Errors of the 1D current sheet's DFT fields and flux against the lattice solution."""
+def line_source_1d(lib, a=20, freqs=None, sigma=0.0, pml=0.0, T=32.0):
+    """Errors of the 1D current sheet's DFT fields and flux against the lattice solution.
+    sigma: a uniform electric conductivity (set through set_conductivity, scaled by
+    chi1inv = 1); pml: uPML of that thickness at both ends (the metallic walls are far
+    enough that nothing returns from them within T = 32)."""
     if freqs is None:
         freqs = np.linspace(0.88, 1.12, 7)
-    L, z0, fcen, fwidth, T = 80.0, 30.0, 1.0, 0.5, 32.0
-    s = structure(lib, dim=1, size=(0, 0, L), a=a)
+    L, z0, fcen, fwidth = 80.0, 30.0, 1.0, 0.5
+    s = structure(lib, dim=1, size=(0, 0, L), a=a, pml=[(Z, side, pml) for side in (High, Low)] if pml else ())
+    if sigma:
+        s.set_conductivity(Ex, lambda x, y, z: sigma)
     f = fields(s)
     f.use_real_fields()
     f.add_volume_source(Ex, gaussian_src_time(fcen, fwidth), (0, 0, z0), (0, 0, z0))
@@ -98,11 +109,12 @@ def line_source_1d(lib, a=20, freqs=None):
     J = a * current_transform(fcen, fwidth, dt, nsteps, freqs)  # the sheet's density: amp * a
     j0 = int(round(z0 * a))
     scale = dt / math.sqrt(2 * math.pi)
+    loss = sigma * dt / 2
     err = {}
     for k, z in zs.items():
         i = int(round(z * a - 0.5))
-        e_exp = scale * 0.5 * (lattice_field(freqs, dt, J, i, j0) + lattice_field(freqs, dt, J, i + 1, j0))
-        h_exp = scale * lattice_field(freqs, dt, J, i, j0, half=True)
+        e_exp = scale * 0.5 * (lattice_field(freqs, dt, J, i, j0, loss=loss) + lattice_field(freqs, dt, J, i + 1, j0, loss=loss))
+        h_exp = scale * lattice_field(freqs, dt, J, i, j0, half=True, loss=loss)
         got = {}
         for q in range(mons[k].num_chunks()):
             d = mons[k].dft(q)
@@ -226,3 +238,56 @@ def magnetic_cavity(lib, a=20, L=2.0, m=2, nsteps=300):
     amp = np.array(amp)
     rec = np.abs(amp[2:] + amp[:-2] - 2 * math.cos(om * dt) * amp[1:-1]).max()
     return {"shape": resid / np.abs(amp).max(), "frequency": float(rec / np.abs(amp).max())}
+
+
+def conductive_decay(lib, eps=3.0, sigma=0.7, sigma_m=0.4, nsteps=200, a=20):
+    """Errors of uniform fields in a periodic 1D cell with electric conductivity sigma (in a
+    medium of permittivity eps) and magnetic conductivity sigma_m.  The curl of a uniform
+    field vanishes, so each step multiplies D by (1 - s dt / 2) / (1 + s dt / 2) with s =
+    sigma / eps (the reference scales an electric conductivity by chi1inv,
+    src/structure.cpp:878-888) and B likewise with s = sigma_m; E = D / eps."""
+    s = structure(lib, dim=1, size=(0, 0, 3.0 / a), a=a, eps=lambda x, y, z: eps)
+    s.set_conductivity(Ex, lambda x, y, z: sigma)
+    s.set_conductivity(Hy, lambda x, y, z: sigma_m)
+    f = fields(s)
+    f.use_real_fields()
+    f.use_bloch((0.0,))
+    f.add_volume_source(Ex, continuous_src_time(1.0), (0, 0, 0.5 / a), (0, 0, 0.5 / a), amp=0.0)
+    one = np.ones(s.shape)
+    f.set_array(Dx, 0, ARR_F, one)
+    f.set_array(Ex, 0, ARR_F, one / eps)
+    f.set_array(By, 0, ARR_F, one)
+    f.step_n(nsteps)
+    dt = s.dt
+    def rate(c):
+        return (1 - c * dt / 2) / (1 + c * dt / 2)
+    d_exp = rate(sigma / eps) ** nsteps
+    b_exp = rate(sigma_m) ** nsteps
+    d = f.get_array(Dx, 0)[0]
+    e = f.get_array(Ex, 0)[0]
+    b = f.get_array(By, 0)[0]
+    return {"D": float(np.abs(d - d_exp).max() / d_exp), "E": float(np.abs(e * eps - d_exp).max() / d_exp),
+            "B": float(np.abs(b - b_exp).max() / b_exp)}
+
+
+def conductive_pml_limit(lib, sigma=1e-9, a=10, freqs=None):
+    """Relative change of the DFT fields of a 2D TM point source in a cell with uPML on
+    every face when a vanishing conductivity sigma is added to Ez and Hx, Hy.  The
+    conductive branches of the curl update (the fcnd auxiliary where uPML acts on f,
+    src/step_generic.cpp:158-248) must reduce to the non-conductive ones as sigma -> 0, so
+    the fields may move by O(sigma) only."""
+    if freqs is None:
+        freqs = np.linspace(0.9, 1.1, 5)
+    out = []
+    for sg in (0.0, sigma):
+        s = structure(lib, dim=2, size=(5.0, 4.0), a=a, pml=[(d, side, 1.0) for d in (X, Y) for side in (High, Low)])
+        if sg:
+            for c in (Ez, Hx, Hy):
+                s.set_conductivity(c, lambda x, y, z: sg)
+        f = fields(s)
+        f.use_real_fields()
+        f.add_volume_source(Ez, gaussian_src_time(1.0, 0.5), (2.1, 1.7, 0), (2.1, 1.7, 0))
+        mon = f.add_dft_fields([Ez, Hx, Hy], (0.3, 0.3, 0), (4.7, 3.7, 0), freqs, decimation_factor=1)
+        f.step_n(int(round(30.0 / s.dt)))
+        out.append([mon.dft(q) for q in range(mon.num_chunks())])
+    return max(float(np.abs(x - y).max() / np.abs(y).max()) for x, y in zip(out[1], out[0]))

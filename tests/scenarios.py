@@ -169,6 +169,48 @@ def xpml_periodic_3d(lib, single=False, a=8):
     return s, f, [fl]
 
 
+def conductive_2d_tm(lib, single=False, a=10, real=True):
+    """2D TM with uPML on every face and conductivity on Ez (graded, scaled by chi1inv) and on
+    Hx, Hy: every conductive branch of the curl update runs (plain, through f_u, through
+    f_cond, and both in the corners); a point current and a line current (a long list, so
+    the box path) are scaled by condinv."""
+    L = (3.0, 2.5)
+    s = structure(lib, dim=2, size=L, a=a, eps=blocks_eps(19, 4, 1.0, 6.0, (3.0, 2.5, 1.0)), pml=pml(0.5),
+                  single=single)
+    s.set_conductivity(Ez, lambda x, y, z: 0.3 + 0.2 * x)
+    s.set_conductivity(Hx, lambda x, y, z: 0.25)
+    s.set_conductivity(Hy, lambda x, y, z: 0.4 if y < 1.2 else 0.0)
+    f = fields(s)
+    if real:
+        f.use_real_fields()
+    f.add_volume_source(Ez, gaussian_src_time(0.8, 0.4), (1.3, 1.1, 0), (1.3, 1.1, 0))
+    f.add_volume_source(Ez, gaussian_src_time(0.7, 0.3), (0.3, 0.4, 0), (2.7, 0.4, 0), 0.5)
+    fl = f.add_dft_flux(X, (2.2, 0.6, 0), (2.2, 1.9, 0), linspace(0.5, 1.0, 6))
+    return s, f, [fl]
+
+
+def conductive_3d(lib, single=False, a=8, real=True):
+    """3D cell with uPML on every face, electric conductivity in the lower half together with
+    a Lorentz pole, magnetic conductivity on Bz everywhere, and an Ex plane current in the
+    conducting region."""
+    L = 2.0
+
+    def lower(x, y, z):
+        return 1.0 if z < L / 2 else 0.0
+    s = structure(lib, dim=3, size=(L, L, L), a=a, eps=blocks_eps(23, 3, 1.5, 5.0, (L, L, L)), pml=pml(0.375),
+                  single=single)
+    s.add_susceptibility(lambda x, y, z: 1.2 * lower(x, y, z), 1.0, 0.1)
+    for c in (Ex, Ey, Ez):
+        s.set_conductivity(c, lambda x, y, z: 0.5 * lower(x, y, z))
+    s.set_conductivity(Hz, lambda x, y, z: 0.2)
+    f = fields(s)
+    if real:
+        f.use_real_fields()
+    f.add_volume_source(Ex, gaussian_src_time(0.8, 0.4), (0.375, 0.375, 0.8), (L - 0.375, L - 0.375, 0.8))
+    fl = f.add_dft_flux(Z, (0.4, 0.4, 1.25), (L - 0.4, L - 0.4, 1.25), linspace(0.5, 1.1, 5))
+    return s, f, [fl]
+
+
 def straight_waveguide(lib, single=False, real=True):
     """The 2D straight-waveguide configuration at full size (python/examples/
     straight-waveguide.py's geometry): 16x8 at resolution 20 (320x160 cells), eps = 12 for
@@ -197,6 +239,8 @@ SCENARIOS = {
     "zone_edge_2d": (zone_edge_2d, 200),
     "xpml_periodic_3d": (xpml_periodic_3d, 150),
     "long_sources_3d": (long_sources_3d, 120),
+    "conductive_2d_tm": (conductive_2d_tm, 200),
+    "conductive_3d": (conductive_3d, 100),
 }
 
 FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)
@@ -249,8 +293,9 @@ def aux_mask(s, c, d):
 def state(f, s, npol=0):
     """Every allocated field and polarisation array, keyed (name, component, cmp), plus the
     uPML auxiliaries f_u (of B and D, along dsigu = the direction after the next,
-    src/step_db.cpp:56-59) and f_w (of E and H, along their own direction,
-    src/update_eh.cpp:162-163) at the points where their profile acts."""
+    src/step_db.cpp:56-59), f_cond (of a conducting B or D, along dsig = the next direction,
+    :56-57,66-69) and f_w (of E and H, along their own direction, src/update_eh.cpp:162-163)
+    at the points where their profile acts."""
     out = {}
     for c in FIELD_COMPONENTS:
         if not f.have_component(c):
@@ -259,11 +304,11 @@ def state(f, s, npol=0):
             arr, ok = f.get_array(c, cmp)
             if ok:
                 out[("f", c, cmp)] = arr
-            for kind, name in ((1, "f_u"), (2, "f_w")):
+            for kind, name in ((1, "f_u"), (2, "f_w"), (5, "f_cond")):
                 arr, ok = f.get_array(c, cmp, kind)
                 if not ok:
                     continue
-                d = (_DIR[c] + 2) % 3 if kind == 1 else _DIR[c]
+                d = {1: (_DIR[c] + 2) % 3, 2: _DIR[c], 5: (_DIR[c] + 1) % 3}[kind]
                 m = aux_mask(s, c, d)
                 if m.any():
                     out[(name, c, cmp)] = np.where(m, arr, 0.0)

@@ -1,7 +1,7 @@
 """The closed forms of tests/closed_forms.py on the MI355X engine in FP64, at the tolerances
 the oracle meets: the engine's DFT scale, phase and E/H time offset, current injection,
-flux weights, Lorentz and Drude poles and magnetic/metallic walls are held to the
-mathematics directly, not only to the oracle."""
+flux weights, Lorentz and Drude poles, magnetic/metallic walls and conductivity are held to
+the mathematics directly, not only to the oracle."""
 
 import pytest
 
@@ -25,3 +25,15 @@ def test_pole_relaxation_and_oscillation_gpu(gpu_lib, drude):
 
 def test_magnetic_and_metallic_walls_quarter_wave_mode_gpu(gpu_lib):
     T.check_cavity(gpu_lib)
+
+
+def test_conductive_decay_of_uniform_fields_gpu(gpu_lib):
+    T.check_conductive_decay(gpu_lib)
+
+
+def test_line_source_in_conductive_medium_gpu(gpu_lib):
+    T.check_conductive_line(gpu_lib)
+
+
+def test_conductive_upml_branches_reduce_to_lossless_gpu(gpu_lib):
+    T.check_conductive_pml_limit(gpu_lib)

@@ -13,6 +13,11 @@ values do not reach (tests/closed_forms.py states each closed form):
 * A quarter-wave cavity between a magnetic and a metallic wall: the lattice eigenmode
   evolves exactly (both walls' placement).
 * Energy conservation in a lossless metallic cavity (field_energy).
+* Conductivity: uniform fields decay by (1 - s dt / 2) / (1 + s dt / 2) per step with s the
+  electric conductivity times chi1inv (or the magnetic one); a current sheet in a uniformly
+  conductive 1D line against the lossy lattice solution, without uPML and with it (the
+  current scaled by condinv, the uPML of a lossy medium absorbing); and the conductive uPML
+  branches of a 2D cell reducing to the lossless ones as the conductivity vanishes.
 * Transport and spectrum shape: equal flux through two planes of a lossless 1D line, and a
   flux spectrum proportional to |J(omega)|^2.  These two are scale-invariant: they do not
   pin the DFT's normalisation, phase or E/H time offset (the first three checks do).
@@ -31,7 +36,8 @@ from meep_amd.capi import (Ex, Ez, Z, fields, gaussian_src_time, linspace, pml, 
                            voltwo)
 
 TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
-       "pole_decay": 1e-4, "cavity": 1e-10}
+       "pole_decay": 1e-4, "cavity": 1e-10, "cond_decay": 1e-12, "cond_pml_line": 1e-6,
+       "cond_limit": 1e-7}
 
 
 def check_line_source(lib):
@@ -54,6 +60,24 @@ def check_pole(lib, drude):
 def check_cavity(lib):
     err = CF.magnetic_cavity(lib)
     assert err["shape"] <= TOL["cavity"] and err["frequency"] <= TOL["cavity"], err
+
+
+def check_conductive_decay(lib):
+    err = CF.conductive_decay(lib)
+    assert max(err.values()) <= TOL["cond_decay"], err
+
+
+def check_conductive_line(lib):
+    # the lossy lattice solution, without uPML (the walls are out of reach) and with it (a
+    # long run: what the uPML of a conductive medium fails to absorb comes back)
+    err = CF.line_source_1d(lib, sigma=0.1)
+    assert max(err.values()) <= TOL["line"], err
+    err = CF.line_source_1d(lib, sigma=0.1, pml=2.0, T=200.0)
+    assert max(err.values()) <= TOL["cond_pml_line"], err
+
+
+def check_conductive_pml_limit(lib):
+    assert CF.conductive_pml_limit(lib) <= TOL["cond_limit"]
 
 
 def test_line_source_matches_lattice_solution(oracle_lib):
@@ -82,6 +106,18 @@ def test_pole_frequency_converges_as_dt_squared(oracle_lib):
 
 def test_magnetic_and_metallic_walls_quarter_wave_mode(oracle_lib):
     check_cavity(oracle_lib)
+
+
+def test_conductive_decay_of_uniform_fields(oracle_lib):
+    check_conductive_decay(oracle_lib)
+
+
+def test_line_source_in_conductive_medium(oracle_lib):
+    check_conductive_line(oracle_lib)
+
+
+def test_conductive_upml_branches_reduce_to_lossless(oracle_lib):
+    check_conductive_pml_limit(oracle_lib)
 
 
 def _cavity_energy_spread(lib, a):

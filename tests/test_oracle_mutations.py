@@ -4,7 +4,9 @@ the check, while the unmodified oracle passes it (test_oracle_closed_forms.py). 
 are the plausible ones a restatement of the reference could make: a dropped DFT scale, a
 flipped DFT phase sign, H transformed at the E time, a current injected without dt, flux
 weights dropped, a Drude pole with a restoring term, pole damping off by 2 pi, a magnetic
-wall at the wrong plane."""
+wall at the wrong plane, a conductive update with dt for dt / 2, an electric conductivity
+not scaled by chi1inv, a current in a conductive medium not scaled by condinv, the conductive
+uPML branch without its (kap - sig) factor."""
 
 import ctypes
 import os
@@ -25,6 +27,10 @@ CHECKS = {
     "DRUDE": lambda lib: T.check_pole(lib, True),
     "GAMMA": lambda lib: T.check_pole(lib, False),
     "MAGNETIC": T.check_cavity,
+    "COND_HALF": T.check_conductive_decay,
+    "COND_CHI": T.check_conductive_decay,
+    "COND_SRC": T.check_conductive_line,
+    "COND_PML": T.check_conductive_pml_limit,
 }
 
 
