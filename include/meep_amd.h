@@ -146,6 +146,18 @@ int mp_fields_step(mp_fields *f);
 /* n calls of fields::step() issued back to back with no host synchronisation between
    them; the finiteness check runs on the device and is read once at the end. */
 int mp_fields_step_n(mp_fields *f, long n);
+/* n steps as mp_fields_step_n with no device memory allocated or freed during the call.  No
+   counterpart in the reference: it is for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The per-step source values
+   are staged in a library-owned device array of 1 MiB per GPU, held until the process ends;
+   a call stages as many steps as fit at a time, and fails if one step's values (8 bytes per
+   source table entry or box component) exceed it.  Every call on one GPU shares that array:
+   calls that overlap on one GPU race.  The fields must have been stepped by mp_fields_step_n
+   since the last change of sources, monitors or boundaries, so that their arrays and launch
+   plan exist; otherwise, or with kernel timing on, the call fails before its first step.  DFT
+   monitors accumulate every step (the deferred accumulation of mp_fields_step_n keeps a
+   buffer of its own); results equal mp_fields_step_n's. */
+int mp_fields_step_n_noalloc(mp_fields *f, long n);
 /* fields::time(), round_time() — src/meep.hpp:1876-1877; fields::t and dt. */
 double mp_fields_time(const mp_fields *f);
 double mp_fields_round_time(const mp_fields *f);

@@ -80,6 +80,7 @@ _SIGNATURES = {
                                               C.c_void_p]),
     "mp_fields_step": (C.c_int, [C.c_void_p]),
     "mp_fields_step_n": (C.c_int, [C.c_void_p, C.c_long]),
+    "mp_fields_step_n_noalloc": (C.c_int, [C.c_void_p, C.c_long]),
     "mp_fields_time": (C.c_double, [C.c_void_p]),
     "mp_fields_round_time": (C.c_double, [C.c_void_p]),
     "mp_fields_t": (C.c_long, [C.c_void_p]),
@@ -429,6 +430,11 @@ class fields:
 
     def step_n(self, n):
         _check(self.lib, self.lib.mp_fields_step_n(self.ptr, int(n)))
+
+    def step_n_noalloc(self, n):
+        """n steps with no device allocation (mp_fields_step_n_noalloc): the fields must have
+        been stepped by step_n since their last change."""
+        _check(self.lib, self.lib.mp_fields_step_n_noalloc(self.ptr, int(n)))
 
     def time(self):
         return self.lib.mp_fields_time(self.ptr)

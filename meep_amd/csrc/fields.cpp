@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -269,6 +270,15 @@ struct SrcVol { // src_vol (src/meep_internals.hpp:49-82)
 };
 
 // ---- device buffers -----------------------------------------------------------------------
+// Set for the duration of mp_fields_step_n_noalloc: any device allocation or release on this
+// thread then fails instead of reaching hipMalloc / hipFree.
+thread_local bool g_no_alloc = false;
+static void alloc_allowed(const char *what) {
+  if (g_no_alloc)
+    fail("mp_fields_step_n_noalloc: the step would allocate or free device memory (%s); step the fields "
+         "with mp_fields_step_n first",
+         what);
+}
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
@@ -279,6 +289,7 @@ struct DevBuf {
     if (p) (void)hipFree(p);
   }
   void alloc(size_t n) {
+    alloc_allowed("a state, plan or staging buffer");
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = n;
@@ -314,6 +325,10 @@ struct FieldsBase {
   virtual void set_boundary(int side, int d, int cond) = 0;
   virtual void add_volume_source(int c, const SrcTime &st, V3 wmin, V3 wmax, cdouble amp, mp_complex_func A, void *Au) = 0;
   virtual void step_n(long n, bool check_each) = 0;
+  // step_n with the source rows staged in `rows` (device memory of rows_bytes) and no device
+  // allocation or release
+  virtual void step_n_noalloc(long n, double *rows, size_t rows_bytes) = 0;
+  virtual int device_id() const = 0;
   virtual long t_now() const = 0;
   virtual double dt_now() const = 0;
   virtual double last_source_time() const = 0;
@@ -367,6 +382,7 @@ template <class R> struct Fields : FieldsBase {
   bool is_real = false;
   bool components_allocated = false;
   hipStream_t stream = nullptr;
+  int device = 0; // the GPU current when the object was created (its stream's device)
 
   R *f[NC][2] = {}, *f_u[NC][2] = {}, *f_w[NC][2] = {}, *f_cond[NC][2] = {};
   std::vector<std::unique_ptr<DevBuf>> bufs;
@@ -449,6 +465,7 @@ template <class R> struct Fields : FieldsBase {
       return e;
     }
     hipEvent_t e;
+    alloc_allowed("a timing event");
     HIPCHECK(hipEventCreate(&e));
     return e;
   }
@@ -523,6 +540,7 @@ template <class R> struct Fields : FieldsBase {
     dt = st->dt;
     courant = st->courant;
     if (gv.ntot >= (size_t(1) << 32)) fail("chunk too large for 32-bit cell indexing (%zu points)", gv.ntot);
+    HIPCHECK(hipGetDevice(&device));
     HIPCHECK(hipStreamCreate(&stream));
     for (int d = 0; d < 3; ++d) {
       k[d] = 0.0;
@@ -1506,6 +1524,9 @@ template <class R> struct Fields : FieldsBase {
   // that read it may still be queued moves to `keep`, which step_n empties after it has
   // synchronised the stream; rows_free says that no queued launch reads rows_buf.
   std::unique_ptr<DevBuf> set_buf, rows_buf;
+  // source rows staged in the caller-independent scratch array by step_n_noalloc
+  const double *ext_rows = nullptr;
+  long ext_t0 = 0, ext_n = 0;
   std::vector<std::unique_ptr<DevBuf>> keep;
   bool rows_free = true;
   long rows_t0 = 0, rows_n = 0, rows_hint = 1;
@@ -1629,6 +1650,10 @@ template <class R> struct Fields : FieldsBase {
     if (srcplan_stale()) build_srcplan();
     if (splan.rec.empty()) return nullptr;
     const size_t rl = splan.rec.size();
+    if (ext_rows) { // rows staged by step_n_noalloc
+      if (tt < ext_t0 || tt >= ext_t0 + ext_n) fail("internal: step outside the staged source rows");
+      return ext_rows + size_t(tt - ext_t0) * rl;
+    }
     if (tt < rows_t0 || tt >= rows_t0 + rows_n) {
       const long n = std::max<long>(rows_hint, 1);
       std::vector<double> rows(rl * size_t(n));
@@ -1806,6 +1831,7 @@ template <class R> struct Fields : FieldsBase {
     if (!sv.dirty) return;
     const size_t n = sv.index.size();
     if (!sv.d_idx) {
+      alloc_allowed("a source list");
       HIPCHECK(hipMalloc(&sv.d_idx, sizeof(int64_t) * n));
       HIPCHECK(hipMalloc(&sv.d_amp, sizeof(double2) * n));
     }
@@ -1869,6 +1895,7 @@ template <class R> struct Fields : FieldsBase {
         sgs.push_back(int8_t(sign));
       }
     }
+    alloc_allowed("the periodic images of a source list");
     if (sv.d_img_k) (void)hipFree(sv.d_img_k);
     if (sv.d_img_dst) (void)hipFree(sv.d_img_dst);
     if (sv.d_img_sgn) (void)hipFree(sv.d_img_sgn);
@@ -2586,6 +2613,66 @@ template <class R> struct Fields : FieldsBase {
     HIPCHECK(hipMemcpy(&flag, d_flag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) fail("simulation fields are NaN or Inf");
   }
+  // n steps as step_n does them, with no device allocation or release: the per-step source
+  // rows go into `rows` (rows_bytes of device memory the caller owns) in pieces of as many
+  // steps as fit, each piece after the stream has drained the previous one; the DFT monitors
+  // accumulate every step (the deferred form keeps a ring of point values of its own).
+  // Everything else the steps use (state arrays, halo tables, source sets, the finiteness
+  // flag) must exist already: the fields must have been stepped by step_n since the last
+  // change of sources, monitors or boundaries, or an allocation would be needed and the call
+  // fails before its first step.
+  void step_n_noalloc(long n, double *rows, size_t rows_bytes) override {
+    if (n < 0) fail("negative step count");
+    if (t == 0 || !plan_valid || srcplan_stale() || batch_split || !boxes_valid)
+      fail("mp_fields_step_n_noalloc: step these fields with mp_fields_step_n first (their launch plan is "
+           "built, and their arrays allocated, by a step that may allocate)");
+    if (ktiming) fail("mp_fields_step_n_noalloc: kernel timing creates events; switch it off");
+    if (adj && adj->recording) fail("mp_fields_step_n_noalloc: the reverse mode records history; use mp_fields_step_n");
+    for (auto &cp : dft_chunks)
+      if (cp->nsnap) fail("internal: deferred DFT steps pending");
+    const size_t rl = splan.rec.size();
+    const long per = rl ? long(rows_bytes / (sizeof(double) * rl)) : n;
+    if (rl && per < 1)
+      fail("mp_fields_step_n_noalloc: one step's source row (%zu values) exceeds the %zu-byte scratch array", rl,
+           rows_bytes);
+    auto t0 = std::chrono::steady_clock::now();
+    struct Guard {
+      Fields *F;
+      Guard(Fields *f) : F(f) { g_no_alloc = true; }
+      ~Guard() {
+        g_no_alloc = false;
+        F->ext_rows = nullptr;
+        F->ext_n = 0;
+      }
+    } guard_(this);
+    HIPCHECK(hipStreamSynchronize(stream));
+    dft_defer_steps = false;
+    std::vector<double> host;
+    for (long q0 = 0; q0 < n;) {
+      const long m = std::min(per, n - q0);
+      if (rl) {
+        host.resize(rl * size_t(m));
+        for (long st = 0; st < m; ++st)
+          for (size_t q = 0; q < rl; ++q) host[size_t(st) * rl + q] = recipe_value(splan.rec[q], t + st);
+        // the previous piece's launches have finished reading the staging array (and the
+        // host vector is reused), so the stream is drained first
+        HIPCHECK(hipStreamSynchronize(stream));
+        HIPCHECK(hipMemcpyAsync(rows, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice, stream));
+        ext_rows = rows;
+        ext_t0 = t;
+        ext_n = m;
+      }
+      for (long q = 0; q < m; ++q) one_step(q0 + q == n - 1);
+      q0 += m;
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipGetLastError());
+    timers[6] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int flag = 0;
+    HIPCHECK(hipMemcpy(&flag, d_flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) fail("simulation fields are NaN or Inf");
+  }
+  int device_id() const override { return device; }
   long t_now() const override { return t; }
   double dt_now() const override { return dt; }
   double last_source_time() const override {
@@ -3724,6 +3811,42 @@ int mp_fields_step_n(mp_fields *f, long n) {
   return guard([&] { F_->step_n(n, false); });
 }
 
+// Staging array of mp_fields_step_n_noalloc: the per-step source rows of the call.  Each GPU
+// the library runs on holds its own copy until the process ends; calls on one GPU share it.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(1) << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+// the address of each GPU's copy, looked up once per GPU with that GPU current (the symbol's
+// name is a host-side stand-in and is never passed to a kernel or a copy)
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static double *scratch_on(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) fail("GPU %d beyond the %d the staging table holds", dev, NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) fail("no device address for the staging array on GPU %d", dev);
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+int mp_fields_step_n_noalloc(mp_fields *f, long n) {
+  return guard([&] {
+    FieldsBase *b = F_;
+    const int dev = b->device_id();
+    int cur = 0;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHECK(hipSetDevice(dev));
+    struct Restore {
+      int cur, dev;
+      ~Restore() {
+        if (cur != dev) (void)hipSetDevice(cur);
+      }
+    } restore{cur, dev};
+    b->step_n_noalloc(n, scratch_on(dev), NOALLOC_SCRATCH_BYTES);
+  });
+}
 double mp_fields_time(const mp_fields *f) {
   auto *b = reinterpret_cast<const FieldsBase *>(f);
   return b->t_now() * b->dt_now();

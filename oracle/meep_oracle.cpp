@@ -1975,6 +1975,8 @@ int mp_fields_step_n(mp_fields *f, long n) {
     });
   });
 }
+// the oracle has no device memory: the allocation-free entry point is step_n
+int mp_fields_step_n_noalloc(mp_fields *f, long n) { return mp_fields_step_n(f, n); }
 double mp_fields_time(const mp_fields *f) {
   return with_fields(const_cast<AnyFields *>(reinterpret_cast<const AnyFields *>(f)), [](auto &F) { return F.t * F.dt; });
 }

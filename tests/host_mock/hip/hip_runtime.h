@@ -41,6 +41,13 @@ inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms =
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t) { return "mock"; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+// a __device__ variable is an ordinary host variable here, and its "device address" its address
+#define __device__
+#define HIP_SYMBOL(x) (&(x))
+#define __align__(n) __attribute__((aligned(n)))
+template <class T> inline hipError_t hipGetSymbolAddress(void **p, const T *sym) { *p = (void *)sym; return hipSuccess; }
 inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipSuccess; }
 inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) { *g = (void *)1; return hipSuccess; }
 inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t, hipGraphNode_t *, char *, size_t) { *e = (void *)1; return hipSuccess; }

@@ -39,6 +39,23 @@ def test_scenario_plans_validate(host_lib, name, single, kernel, monkeypatch):
     assert f.t == n + 1
 
 
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+@pytest.mark.parametrize("name", sorted(SCENARIOS))
+def test_noalloc_steps_allocate_nothing(host_lib, name, single):
+    """mp_fields_step_n_noalloc after a regular call: every scenario's steps run without
+    reaching an allocation or release (the engine fails the call if one is attempted); fields
+    never stepped are refused."""
+    builder, _ = SCENARIOS[name]
+    s, f, _ = builder(host_lib, single=single)
+    with pytest.raises(capi.MeepError, match="step_n"):
+        f.step_n_noalloc(2)
+    f.step_n(3)
+    f.step_n_noalloc(9)
+    f.step_n(2)
+    f.step_n_noalloc(1)
+    assert f.t == 15
+
+
 def _batch_sims(lib, single):
     import test_gpu_batch as TB
     return TB.build_all(lib, single)
