@@ -53,6 +53,8 @@
  *   MUT_COND_CHI      an electric conductivity is not multiplied by chi1inv
  *   MUT_COND_SRC      a current source in a conductive medium is not scaled by condinv
  *   MUT_COND_PML      the PML-with-conductivity branch drops its (kap - sig) factor
+ *   MUT_PML_CHUNKWIDE a direction's uPML branch is taken at every point of the chunk, as if
+ *                     the uPML regions were not chunks of their own
  */
 #include "../include/meep_amd.h"
 
@@ -897,15 +899,21 @@ template <class R> struct Fields {
     loop_ivecs_par(gv, is, ie, [&](ptrdiff_t i, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
       ivec here = loop_iloc(L, i1, i2, i3);
       R c = g2 ? (g1[i + s1] - g1[i] + g2[i] - g2[i + s2]) : (g1[i + s1] - g1[i]);
+      // The reference breaks the uPML regions off into chunks of their own
+      // (src/structure.cpp:128-137, structure::use_pml skipping a chunk that does not meet a
+      // direction's uPML, :645-654), so a point is stepped with dsig (dsigu) only where that
+      // direction's profile acts on it.  Where it does not (kap = 1, sig = 0) the branches
+      // agree on the curl term, but not on what a source or a conductivity has put into f:
+      // through f_u or f_cond, f's own increments would bypass the damping.  Each point
+      // therefore takes the branch of its region here.
+      const int kk = dsig != NODIR ? here.t[dsig] : 0, ku = dsigu != NODIR ? here.t[dsigu] : 0;
+#ifdef MUT_PML_CHUNKWIDE
+      const bool pml = dsig != NODIR, pmlu = dsigu != NODIR;
+#else
+      const bool pml = dsig != NODIR && (sig[kk] != 0 || kap[kk] != 1);
+      const bool pmlu = dsigu != NODIR && (sigu[ku] != 0 || kapu[ku] != 1);
+#endif
       if (cnd) {
-        // The reference breaks the uPML regions off into chunks of their own
-        // (src/structure.cpp:128-137), so a point outside every uPML region is stepped with
-        // dsig = dsigu = NO_DIRECTION.  Without conductivity the branches agree there
-        // (kap = 1, sig = 0); with it they do not (f's increments would bypass the damping
-        // of fu and fcnd), so each point takes the branch of its region here.
-        const int kk = dsig != NODIR ? here.t[dsig] : 0, ku = dsigu != NODIR ? here.t[dsigu] : 0;
-        const bool pml = dsig != NODIR && (sig[kk] != 0 || kap[kk] != 1);
-        const bool pmlu = dsigu != NODIR && (sigu[ku] != 0 || kapu[ku] != 1);
         if (!pml && !pmlu)
           fp[i] = ((1 - dt2 * cnd[i]) * fp[i] - dtdx * c) * cndinv[i];
         else if (!pml) {
@@ -927,22 +935,19 @@ template <class R> struct Fields {
         }
         return;
       }
-      if (dsig == NODIR) {
-        if (dsigu == NODIR)
+      if (!pml) {
+        if (!pmlu)
           fp[i] -= dtdx * c;
         else {
-          int ku = here.t[dsigu];
           R fprev = fu[i];
           fu[i] -= dtdx * c;
           fp[i] = siginvu[ku] * ((kapu[ku] - sigu[ku]) * fp[i] + fu[i] - fprev);
         }
       }
       else {
-        int kk = here.t[dsig];
-        if (dsigu == NODIR)
+        if (!pmlu)
           fp[i] = ((kap[kk] - sig[kk]) * fp[i] - dtdx * c) * siginv[kk];
         else {
-          int ku = here.t[dsigu];
           R fprev = fu[i];
           fu[i] = ((kap[kk] - sig[kk]) * fu[i] - dtdx * c) * siginv[kk];
           fp[i] = siginvu[ku] * ((kapu[ku] - sigu[ku]) * fp[i] + fu[i] - fprev);

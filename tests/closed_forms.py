@@ -291,3 +291,24 @@ def conductive_pml_limit(lib, sigma=1e-9, a=10, freqs=None):
         f.step_n(int(round(30.0 / s.dt)))
         out.append([mon.dft(q) for q in range(mon.num_chunks())])
     return max(float(np.abs(x - y).max() / np.abs(y).max()) for x, y in zip(out[1], out[0]))
+
+
+def pml_locality(lib, a=10, steps=10):
+    """Relative change of Dx along a current plane that crosses a y uPML region when a z uPML
+    is added 25 cells away (beyond the 2 cells per step the update can reach).  The reference steps each uPML
+    region as a chunk of its own (src/structure.cpp:128-137), whose sigma arrays cover only
+    the directions whose uPML it meets (structure_chunk::use_pml, :645-654), so a point of the
+    y uPML takes the y branch alone wherever a z uPML is: the current it receives is damped by
+    the y profile, and nothing can depend on the far z uPML."""
+    out = []
+    for zpml in (False, True):
+        terms = [(Y, side, 0.8) for side in (High, Low)]
+        if zpml:
+            terms += [(Z, side, 0.5) for side in (High, Low)]
+        s = structure(lib, dim=3, size=(1.2, 2.4, 6.0), a=a, pml=terms)
+        f = fields(s)
+        f.use_real_fields()
+        f.add_volume_source(Ex, gaussian_src_time(0.8, 0.5), (0.0, 0.0, 3.0), (1.2, 2.4, 3.0))
+        f.step_n(steps)
+        out.append(f.get_array(Dx, 0)[0])
+    return float(np.abs(out[1] - out[0]).max() / np.abs(out[0]).max())

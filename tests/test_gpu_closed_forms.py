@@ -37,3 +37,7 @@ def test_line_source_in_conductive_medium_gpu(gpu_lib):
 
 def test_conductive_upml_branches_reduce_to_lossless_gpu(gpu_lib):
     T.check_conductive_pml_limit(gpu_lib)
+
+
+def test_upml_branch_is_local_to_its_region_gpu(gpu_lib):
+    T.check_pml_locality(gpu_lib)

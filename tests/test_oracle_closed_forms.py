@@ -18,6 +18,9 @@ values do not reach (tests/closed_forms.py states each closed form):
   conductive 1D line against the lossy lattice solution, without uPML and with it (the
   current scaled by condinv, the uPML of a lossy medium absorbing); and the conductive uPML
   branches of a 2D cell reducing to the lossless ones as the conductivity vanishes.
+* uPML regions as chunks of their own: a current inside a y uPML region evolves the same
+  whether or not a z uPML exists beyond the update's reach (each point takes the branch of
+  the directions whose uPML acts on it, as the reference's PML chunks do).
 * Transport and spectrum shape: equal flux through two planes of a lossless 1D line, and a
   flux spectrum proportional to |J(omega)|^2.  These two are scale-invariant: they do not
   pin the DFT's normalisation, phase or E/H time offset (the first three checks do).
@@ -37,7 +40,7 @@ from meep_amd.capi import (Ex, Ez, Z, fields, gaussian_src_time, linspace, pml, 
 
 TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
        "pole_decay": 1e-4, "cavity": 1e-10, "cond_decay": 1e-12, "cond_pml_line": 1e-6,
-       "cond_limit": 1e-7}
+       "cond_limit": 1e-7, "locality": 1e-13}
 
 
 def check_line_source(lib):
@@ -80,6 +83,10 @@ def check_conductive_pml_limit(lib):
     assert CF.conductive_pml_limit(lib) <= TOL["cond_limit"]
 
 
+def check_pml_locality(lib):
+    assert CF.pml_locality(lib) <= TOL["locality"]
+
+
 def test_line_source_matches_lattice_solution(oracle_lib):
     check_line_source(oracle_lib)
 
@@ -118,6 +125,10 @@ def test_line_source_in_conductive_medium(oracle_lib):
 
 def test_conductive_upml_branches_reduce_to_lossless(oracle_lib):
     check_conductive_pml_limit(oracle_lib)
+
+
+def test_upml_branch_is_local_to_its_region(oracle_lib):
+    check_pml_locality(oracle_lib)
 
 
 def _cavity_energy_spread(lib, a):

@@ -6,7 +6,8 @@ flipped DFT phase sign, H transformed at the E time, a current injected without 
 weights dropped, a Drude pole with a restoring term, pole damping off by 2 pi, a magnetic
 wall at the wrong plane, a conductive update with dt for dt / 2, an electric conductivity
 not scaled by chi1inv, a current in a conductive medium not scaled by condinv, the conductive
-uPML branch without its (kap - sig) factor."""
+uPML branch without its (kap - sig) factor, a uPML branch taken over the whole chunk instead
+of its region."""
 
 import ctypes
 import os
@@ -31,6 +32,7 @@ CHECKS = {
     "COND_CHI": T.check_conductive_decay,
     "COND_SRC": T.check_conductive_line,
     "COND_PML": T.check_conductive_pml_limit,
+    "PML_CHUNKWIDE": T.check_pml_locality,
 }
 
 
