@@ -324,6 +324,10 @@ template <class R> void launch_step(const StepArgs<R> &a, bool curl, bool upd, h
 // vectorised kernel (16-byte accesses); pml: the uPML branches per point (else the box must
 // have no active uPML profile)
 template <class R> void launch_fast(const StepArgs<R> &a, bool curl, bool upd, bool pml, hipStream_t s);
+// the 3D row kernel for a box with no active x or y uPML profile and no folded images; false
+// when the box or its slots do not qualify (the caller then uses launch_fast).
+// MEEP_AMD_NO_ROW3 (read once) turns it off.
+template <class R> bool launch_row3(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s);
 template <class R>
 void launch_conn(const Conn *conns, const double2 *phases, int n, R *const *slot_ptrs, int ncmp, hipStream_t s);
 // f[idx[j]] -= Re or Im of (amp[j] * scal) * mult for the n points of one source

@@ -576,8 +576,6 @@ template <class R> struct Fields : FieldsBase {
       Pol p;
       p.isus = i;
       const HostSus &su = s->sus[i];
-      for (int c = 0; c < 5; ++c)
-        if (!su.sigma[c].empty()) p.sigma[c] = upload(to_device(su.sigma[c], R(0)));
       // lorentzian_susceptibility::update_P coefficients in realnum (src/susceptibility.cpp:192-196)
       const R om = R(su.omega_0), ga = R(su.gamma), rdt = R(dt);
       const R omega2pi = R(2 * pi * om), g2pi = R(ga * 2 * pi);
@@ -626,6 +624,21 @@ template <class R> struct Fields : FieldsBase {
         if (bhi[d] < 0) p.coef.lo[k] = 1, p.coef.hi[k] = 0; // nowhere
       }
       pols.push_back(p);
+    }
+    // sigma of every pole for one component in one block of consecutive arrays (the row
+    // kernel addresses a component's poles from one base pointer)
+    for (int c = 0; c < 5; ++c) {
+      std::vector<size_t> with;
+      for (size_t q = 0; q < pols.size(); ++q)
+        if (!s->sus[pols[q].isus].sigma[c].empty()) with.push_back(q);
+      if (with.empty()) continue;
+      std::vector<R> host(gv.ntot * with.size());
+      for (size_t j = 0; j < with.size(); ++j) {
+        const std::vector<R> v = to_device(s->sus[pols[with[j]].isus].sigma[c], R(0));
+        std::copy(v.begin(), v.end(), host.begin() + j * gv.ntot);
+      }
+      R *base = upload(host);
+      for (size_t j = 0; j < with.size(); ++j) pols[with[j]].sigma[c] = base + j * gv.ntot;
     }
     d_flag.alloc(sizeof(int));
     HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
@@ -1092,8 +1105,9 @@ template <class R> struct Fields : FieldsBase {
       collector->items[cur_phase][kind * 4 + (curl ? 2 : 0) + (upd ? 1 : 0)].push_back(it);
       return;
     }
-    if (fast)
-      launch_fast<R>(b, curl, upd, pml, stream);
+    if (fast) {
+      if (!launch_row3<R>(b, curl, upd, stream)) launch_fast<R>(b, curl, upd, pml, stream);
+    }
     else
       launch_step<R>(b, curl, upd, stream);
   }
@@ -1482,18 +1496,35 @@ template <class R> struct Fields : FieldsBase {
   bool needs_P(const Pol &p, int c, int cmp) const {
     return is_electric(c) && p.sigma[c] && f[c][cmp];
   }
+  // zeroed arrays of n * ntot values (n consecutive state arrays)
+  R *dalloc_n(size_t n) {
+    bufs.emplace_back(new DevBuf);
+    bufs.back()->alloc(sizeof(R) * gv.ntot * n);
+    R *p = static_cast<R *>(bufs.back()->p);
+    HIPCHECK(hipMemsetAsync(p, 0, sizeof(R) * gv.ntot * n, stream));
+    return p;
+  }
+  // lorentzian_data (src/susceptibility.cpp:98-139): P and P_prev of the poles allocated
+  // now, for each component, in one block of consecutive arrays (P, P_prev of the first
+  // pole, then of the next), so the row kernel addresses them from one base pointer
   void alloc_pols() {
-    for (auto &p : pols)
-      if (!p.allocated) {
-        for (int c = 0; c < 5; ++c)
-          for (int cmp = 0; cmp < ncmp(); ++cmp)
-            if (needs_P(p, c, cmp)) {
-              p.P[c][cmp] = dalloc(nullptr);
-              p.Pp[c][cmp] = dalloc(nullptr);
-            }
-        p.allocated = true;
-        plan_valid = false;
+    bool any = false;
+    for (auto &p : pols) any = any || !p.allocated;
+    if (!any) return;
+    for (int c = 0; c < 5; ++c)
+      for (int cmp = 0; cmp < ncmp(); ++cmp) {
+        std::vector<Pol *> need;
+        for (auto &p : pols)
+          if (!p.allocated && needs_P(p, c, cmp)) need.push_back(&p);
+        if (need.empty()) continue;
+        R *base = dalloc_n(2 * need.size());
+        for (size_t j = 0; j < need.size(); ++j) {
+          need[j]->P[c][cmp] = base + (2 * j) * gv.ntot;
+          need[j]->Pp[c][cmp] = base + (2 * j + 1) * gv.ntot;
+        }
       }
+    for (auto &p : pols) p.allocated = true;
+    plan_valid = false;
   }
 
   // ---- source plan ----

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 
 namespace mx {
@@ -678,6 +680,498 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
 }
 
 // ---------------------------------------------------------------------------------------
+// 3D row kernel: V consecutive z points of one (x, y) row per thread, for launch boxes of a
+// 3D chunk with no active x or y uPML profile and no folded periodic images (the interior
+// box of a chunk stepped alone).  In 3D the loop axes are the directions, so each slot's Yee
+// shifts, curl partners and uPML directions are compile-time facts: slot C reads its
+// partners F[(C+2)%3] along C+1 and F[(C+1)%3] along C+2 (src/fields.cpp:428-456); the z
+// uPML acts through f_u on slot 0 (dsigu = z), on f itself for slot 1 (dsig = z) and through
+// f_w on slot 2's update (dsigw = z) (src/step_db.cpp:56-59, src/update_eh.cpp:162-163), lane
+// by lane and only in the groups outside the z quiet range.  The generic per-slot logic of
+// k_fast (runtime loop axes, periodic images, x/y profiles) is absent, so the per-point
+// instruction count is a fraction of k_fast's; sources run in a non-inlined helper entered
+// only by the groups inside the source box.  Arithmetic is the reference's, as in k_fast.
+
+struct Row3Geom {
+  uint32_t sx, sy;      // strides along x and y (z is stride 1; a chunk has < 2^32 points)
+  int nx, ny, nz;       // cells per direction
+  int x0, y0;           // launch box low corner in x and y
+  uint32_t nyb;         // rows along y in the box
+  int g0;               // first group along z
+  uint32_t ngroups;     // groups per row
+  int zlo, zhi;         // launch box along z
+  int qzlo, qzhi;       // z indices where no z uPML profile is active
+  int mhi_bits, mlo_bits; // bit d: the High wall normal to d is Metallic / the Low wall Magnetic
+  int rev;              // visit logical blocks in descending order
+  int cmp;              // real (0) or imaginary (1) parts
+};
+template <class R, int NP> struct Row3Args {
+  Row3Geom g;
+  R dtdx;
+  const R *F[3];        // curl partners by direction
+  R *f[3];              // B or D by direction (null: absent)
+  R *fu0;               // f_u of slot 0 (its dsigu is z), null when none
+  R *e[3];              // H or E (null: no separate update array)
+  const R *chi[3];      // diagonal chi1inv / mu1inv (null: 1)
+  R *fw2;               // f_w of slot 2 (its dsigw is z), null when none
+  const R *zsig, *zkap, *zsiginv; // the z uPML profile (null: no z uPML)
+  // poles: P and P_prev of pole p for component c are the arrays poff[p] and ppoff[p] of the
+  // block at pbase[c], its sigma the array soff[p] of the block at sbase[c] (pst values
+  // apart): six pointers instead of three per pole and component
+  R *pbase[3];
+  const R *sbase[3];
+  int64_t pst;
+  int poff[NP > 0 ? NP : 1], ppoff[NP > 0 ? NP : 1], soff[NP > 0 ? NP : 1];
+  int pmask;            // bit 3 p + c: pole p has a polarisation for component c
+  int smask;            // bit 3 p + c: ... and a sigma array
+  R sigval[NP > 0 ? NP : 1][3];
+  int sigone;           // bit 3 p + c: pole p's sigma of component c is sigval over its box
+  PoleCoef<R> pole[NP > 0 ? NP : 1];
+  int pswap;
+  const SrcSet *src;    // null: no sources in this launch
+  int sbox_lo[3], sbox_hi[3];
+  const double *vrow;
+};
+
+// Yee shift of slot C along direction d: electric components (the D phase, PH 1) are shifted
+// along their own direction, magnetic ones (the B phase, PH 0) along the others
+template <int PH, int C, int D> constexpr int r3_ysh() { return PH ? (C == D ? 1 : 0) : (C == D ? 0 : 1); }
+
+// lanes of a group owned by slot C and inside the box along z (x and y are checked per thread)
+template <int V, int PH, int C> __device__ __forceinline__ unsigned r3_zmask(const Row3Geom &g, int z0) {
+  constexpr int y = r3_ysh<PH, C, 2>();
+  unsigned m = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const int z = z0 + k;
+    if (z >= g.zlo && z <= g.zhi && z >= 1 - y && z <= g.nz - y) m |= 1u << k;
+  }
+  return m;
+}
+template <int PH, int C> __device__ __forceinline__ bool r3_xyown(const Row3Geom &g, int x, int y) {
+  constexpr int yx = r3_ysh<PH, C, 0>(), yy = r3_ysh<PH, C, 1>();
+  return x >= 1 - yx && x <= g.nx - yx && y >= 1 - yy && y <= g.ny - yy;
+}
+// metal planes (on_metal_boundary, src/boundaries.cpp:191-205): all lanes when x or y is on
+// one, else the z lanes on one
+template <int V, int PH, int C> __device__ __forceinline__ unsigned r3_metal(const Row3Geom &g, int x, int y, int z0) {
+  if (!(g.mhi_bits | g.mlo_bits)) return 0u;
+  constexpr int yx = r3_ysh<PH, C, 0>(), yy = r3_ysh<PH, C, 1>(), yz = r3_ysh<PH, C, 2>();
+  const bool mx_ = (((g.mhi_bits >> 0) & 1) && yx == 0 && x == g.nx) || (((g.mlo_bits >> 0) & 1) && yx == 1 && x == 0);
+  const bool my_ = (((g.mhi_bits >> 1) & 1) && yy == 0 && y == g.ny) || (((g.mlo_bits >> 1) & 1) && yy == 1 && y == 0);
+  if (mx_ || my_) return (1u << V) - 1;
+  unsigned m = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const int z = z0 + k;
+    if ((((g.mhi_bits >> 2) & 1) && yz == 0 && z == g.nz) || (((g.mlo_bits >> 2) & 1) && yz == 1 && z == 0)) m |= 1u << k;
+  }
+  return m;
+}
+
+// sources of slot C on the lanes of one group (entered only inside the launch's source box);
+// the group's values travel by value, so the caller's registers never need an address
+template <class R, int V, bool DIP, int C>
+__device__ __noinline__ Vec<R, V> r3_sources(const SrcSet *S, int cmp, const double *vrow, uint32_t i, int x, int y, int z0,
+                                             unsigned mask, Vec<R, V> v) {
+  const SrcTab &t = DIP ? S->dip : S->cur;
+  const SrcBoxes &b = DIP ? S->dipb : S->curb;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    if (!((mask >> k) & 1)) continue;
+    R w = v.v[k];
+    if (t.n && int64_t(i + k) >= t.lo && int64_t(i + k) <= t.hi) w = apply_tab(t, C, i + k, w, vrow);
+    if (b.n) w = apply_boxes(b, cmp, C, x, y, z0 + k, w, vrow);
+    v.v[k] = w;
+  }
+  return v;
+}
+
+template <class R, int V, int PH, bool UPD, int NP, bool ZP, int C>
+__device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, int x, int y, int z0, unsigned zm, const Vec<R, V> &F0,
+                                        const Vec<R, V> &F1, const Vec<R, V> &F2, bool srcs, unsigned pinxy) {
+  const Row3Geom &g = a.g;
+  R *const fp = a.f[C];
+  if (!fp || !zm || !r3_xyown<PH, C>(g, x, y)) return;
+  const unsigned mask = zm;
+  const unsigned metal = r3_metal<V, PH, C>(g, x, y, z0);
+  // curl partners: g1 = F[(C+2)%3] along dp, g2 = F[(C+1)%3] along dm; forward differences
+  // for B (i + stride), backward for D (i - stride), the D update's negated strides
+  // (src/step_db.cpp:81-84)
+  constexpr int dp = (C + 1) % 3, dm = (C + 2) % 3;
+  const Vec<R, V> &g1c = C == 0 ? F2 : (C == 1 ? F0 : F1);
+  const Vec<R, V> &g2c = C == 0 ? F1 : (C == 1 ? F2 : F0);
+  const R *g1p = a.F[(C + 2) % 3], *g2p = a.F[(C + 1) % 3];
+  constexpr int64_t sg = PH ? -1 : 1;
+  const bool zq = !ZP || (z0 >= g.qzlo && z0 + V - 1 <= g.qzhi); // no z uPML lane in the group
+  // ---- loads ----
+  Vec<R, V> g1n, g2n;
+  if (dp == 2) g1n = neighbour<R, V>(g1p, i, sg, g1c, mask);
+  else g1n = vload<R, V>(g1p + (PH ? i - (dp == 0 ? g.sx : g.sy) : i + (dp == 0 ? g.sx : g.sy)));
+  if (dm == 2) g2n = neighbour<R, V>(g2p, i, sg, g2c, mask);
+  else g2n = vload<R, V>(g2p + (PH ? i - (dm == 0 ? g.sx : g.sy) : i + (dm == 0 ? g.sx : g.sy)));
+  Vec<R, V> fv = vload<R, V>(fp + i), fuv;
+  const bool zfu = ZP && C == 0 && !zq && a.fu0;
+  if (zfu) fuv = vload<R, V>(a.fu0 + i);
+  R *const ep = UPD ? a.e[C] : nullptr;
+  Vec<R, V> uu, Pc[NP > 0 ? NP : 1], Ppv[NP > 0 ? NP : 1], sgv[NP > 0 ? NP : 1], fwv, eold;
+  bool pin[NP > 0 ? NP : 1];
+  const bool zfw = UPD && ZP && C == 2 && !zq && a.fw2 && ep;
+  if (ep) {
+    if (a.chi[C]) uu = vload<R, V>(a.chi[C] + i);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const PoleCoef<R> &q = a.pole[p];
+      pin[p] = ((a.pmask >> (3 * p + C)) & 1) && ((pinxy >> p) & 1) && z0 + V - 1 >= q.lo[2] && z0 <= q.hi[2];
+      if (pin[p]) {
+        Pc[p] = vload<R, V>(a.pbase[C] + (a.poff[p] * a.pst + i));
+        if ((a.smask >> (3 * p + C)) & 1) {
+          Ppv[p] = vload<R, V>(a.pbase[C] + (a.ppoff[p] * a.pst + i));
+          if ((a.sigone >> (3 * p + C)) & 1) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) sgv[p].v[k] = z0 + k >= q.lo[2] && z0 + k <= q.hi[2] ? a.sigval[p][C] : R(0);
+          }
+          else
+            sgv[p] = vload<R, V>(a.sbase[C] + (a.soff[p] * a.pst + i));
+        }
+      }
+    }
+    if (zfw) {
+      fwv = vload<R, V>(a.fw2 + i);
+      eold = vload<R, V>(ep + i);
+    }
+  }
+  // ---- curl and the z uPML of slots 0 and 1 ----
+  Vec<R, V> cu;
+#pragma unroll
+  for (int k = 0; k < V; ++k) cu.v[k] = g1n.v[k] - g1c.v[k] + g2c.v[k] - g2n.v[k];
+  unsigned um = 0; // lanes whose f_u takes part
+  if (ZP && C != 2 && !zq && a.zsig) {
+    // per-lane profile at half-pixel index 2 z + ysh_z (src/meep_internals.hpp:217-226)
+    constexpr int yz = r3_ysh<PH, C, 2>();
+    R sg_[V], kp_[V], si_[V];
+    unsigned on = 0;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const int idx = 2 * min(z0 + k, g.nz) + yz;
+      sg_[k] = a.zsig[idx];
+      kp_[k] = a.zkap[idx];
+      si_[k] = a.zsiginv[idx];
+      if (sg_[k] != R(0) || kp_[k] != R(1)) on |= 1u << k;
+    }
+    on &= mask;
+    if (C == 0 && zfu) {
+      // step_curl through f_u (dsigu = z, no uPML on f): src/step_generic.cpp:130-151
+      um = on;
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        if ((um >> k) & 1) {
+          const R fprev = fuv.v[k];
+          const R fun = fprev - a.dtdx * cu.v[k];
+          fuv.v[k] = fun;
+          fv.v[k] = si_[k] * ((kp_[k] - sg_[k]) * fv.v[k] + fun - fprev);
+        }
+        else
+          fv.v[k] -= a.dtdx * cu.v[k];
+      }
+    }
+    else if (C == 1) {
+      // uPML on f (dsig = z): src/step_generic.cpp:179-192
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        if ((on >> k) & 1)
+          fv.v[k] = ((kp_[k] - sg_[k]) * fv.v[k] - a.dtdx * cu.v[k]) * si_[k];
+        else
+          fv.v[k] -= a.dtdx * cu.v[k];
+      }
+    }
+    else {
+#pragma unroll
+      for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
+    }
+  }
+  else {
+#pragma unroll
+    for (int k = 0; k < V; ++k) fv.v[k] -= a.dtdx * cu.v[k];
+  }
+  if (srcs) fv = r3_sources<R, V, false, C>(a.src, g.cmp, a.vrow, i, x, y, z0, mask, fv);
+  if (metal) {
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+      if ((metal >> k) & 1) fv.v[k] = R(0);
+  }
+  // ---- update: E = chi1inv (D - sum P - dipoles), f_w of slot 2, the poles ----
+  Vec<R, V> ev, Pn[NP > 0 ? NP : 1];
+  unsigned wm = 0;
+  if (ep) {
+    Vec<R, V> gv = fv;
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (pin[p]) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) gv.v[k] -= Pc[p].v[k];
+      }
+    if (srcs) gv = r3_sources<R, V, true, C>(a.src, g.cmp, a.vrow, i, x, y, z0, mask, gv);
+    ev = gv;
+    if (a.chi[C]) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) ev.v[k] = gv.v[k] * uu.v[k];
+    }
+    const Vec<R, V> wv = ev; // W of the polarisations: E, or f_w where the profile acts
+    if (zfw) {
+      // step_update_EDHB's uPML branch (src/step_generic.cpp:682-701), dsigw = z
+      constexpr int yz = r3_ysh<PH, C, 2>();
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const int idx = 2 * min(z0 + k, g.nz) + yz;
+        const R sw = a.zsig[idx], kw = a.zkap[idx];
+        if (((mask >> k) & 1) && (sw != R(0) || kw != R(1))) {
+          wm |= 1u << k;
+          const R fwprev = fwv.v[k], fwn = ev.v[k];
+          fwv.v[k] = fwn;
+          ev.v[k] = eold.v[k] + ((kw + sw) * fwn - (kw - sw) * fwprev);
+        }
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (pin[p] && ((a.smask >> (3 * p + C)) & 1)) {
+        const PoleCoef<R> &q = a.pole[p];
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+          Pn[p].v[k] = q.gamma1inv * (Pc[p].v[k] * (R(2) - q.omega0dtsqr_denom) - q.gamma1 * Ppv[p].v[k] +
+                                      q.omega0dtsqr * (sgv[p].v[k] * wv.v[k]));
+      }
+    if (metal) {
+#pragma unroll
+      for (int k = 0; k < V; ++k)
+        if ((metal >> k) & 1) ev.v[k] = R(0);
+    }
+  }
+  // ---- stores ----
+  if (um) vstore<R, V>(a.fu0 + i, fuv, um);
+  vstore<R, V>(fp + i, fv, mask);
+  if (ep) {
+    if (wm) vstore<R, V>(a.fw2 + i, fwv, wm);
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (pin[p] && ((a.smask >> (3 * p + C)) & 1)) {
+        if (a.pswap)
+          vstore<R, V>(a.pbase[C] + (a.ppoff[p] * a.pst + i), Pn[p], mask);
+        else {
+          vstore<R, V>(a.pbase[C] + (a.poff[p] * a.pst + i), Pn[p], mask);
+          vstore<R, V>(a.pbase[C] + (a.ppoff[p] * a.pst + i), Pc[p], mask);
+        }
+      }
+    vstore<R, V>(ep + i, ev, mask);
+  }
+}
+
+template <class R, int V, int PH, bool UPD, int NP, bool ZP>
+__global__ __launch_bounds__(BLOCK) void k_row3(const Row3Args<R, NP> a, uint32_t count) {
+  const uint32_t t = xcd_block(a.g.rev) * uint32_t(BLOCK) + threadIdx.x;
+  if (t >= count) return;
+  const Row3Geom &g = a.g;
+  const uint32_t r = t / g.ngroups;
+  const int grp = int(t - r * g.ngroups);
+  const uint32_t qx = r / g.nyb;
+  const int x = g.x0 + int(qx), y = g.y0 + int(r - qx * g.nyb);
+  const int z0 = (g.g0 + grp) * V;
+  const uint32_t i = uint32_t(x) * g.sx + uint32_t(y) * g.sy + uint32_t(z0);
+  const unsigned z0m = r3_zmask<V, PH, 0>(g, z0), z1m = r3_zmask<V, PH, 1>(g, z0), z2m = r3_zmask<V, PH, 2>(g, z0);
+  if (!(z0m | z1m | z2m)) return;
+  // per-thread facts computed once, so the bounds they need are not kept for every slot
+  const bool srcs = a.src && x >= a.sbox_lo[0] && x <= a.sbox_hi[0] && y >= a.sbox_lo[1] && y <= a.sbox_hi[1] &&
+                    z0 + V - 1 >= a.sbox_lo[2] && z0 <= a.sbox_hi[2];
+  unsigned pinxy = 0;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const PoleCoef<R> &q = a.pole[p];
+    if (x >= q.lo[0] && x <= q.hi[0] && y >= q.lo[1] && y <= q.hi[1]) pinxy |= 1u << p;
+  }
+  Vec<R, V> F0 = vload<R, V>(a.F[0] + i), F1 = vload<R, V>(a.F[1] + i), F2 = vload<R, V>(a.F[2] + i);
+  r3_slot<R, V, PH, UPD, NP, ZP, 0>(a, i, x, y, z0, z0m, F0, F1, F2, srcs, pinxy);
+  r3_slot<R, V, PH, UPD, NP, ZP, 1>(a, i, x, y, z0, z1m, F0, F1, F2, srcs, pinxy);
+  r3_slot<R, V, PH, UPD, NP, ZP, 2>(a, i, x, y, z0, z2m, F0, F1, F2, srcs, pinxy);
+}
+
+// The row kernel for a's launch box when the box qualifies (3D, loop axes = directions, no
+// folded images, no x or y uPML profile active in the box, no conductivity, the 3D curl plan
+// in every slot); false leaves the launch to k_fast.
+template <class R, int NP>
+static bool r3_fill(const StepArgs<R> &a, bool curl, bool upd, Row3Args<R, NP> &b, int V) {
+  const Grid &g = a.g;
+  if (!curl || a.nc != 3) return false;
+  for (int k = 0; k < 3; ++k)
+    if (g.axdir[k] != k || g.wrap[k] || g.nax[k] < 1) return false;
+  for (int k = 0; k < 2; ++k) // x and y inside their quiet ranges
+    if (g.qlo[k] > g.qhi[k] || g.alo[k] < g.qlo[k] || g.alo[k] + g.an[k] - 1 > g.qhi[k]) return false;
+  if (a.sgn != 1 && a.sgn != -1) return false;
+  for (int d = 0; d < 3; ++d)
+    if (!a.F[d]) return false;
+  const int PH = a.sgn < 0 ? 1 : 0;
+  memset(&b, 0, sizeof b);
+  Row3Geom &q = b.g;
+  if (g.st[0] <= 0 || g.st[1] <= 0 || g.st[0] >= (int64_t(1) << 32) || g.st[1] >= (int64_t(1) << 32)) return false;
+  q.sx = uint32_t(g.st[0]);
+  q.sy = uint32_t(g.st[1]);
+  if (g.st[2] != 1) return false;
+  q.nx = g.nax[0];
+  q.ny = g.nax[1];
+  q.nz = g.nax[2];
+  q.x0 = g.alo[0];
+  q.y0 = g.alo[1];
+  q.nyb = uint32_t(g.an[1]);
+  q.zlo = g.alo[2];
+  q.zhi = g.alo[2] + g.an[2] - 1;
+  q.g0 = g.alo[2] / V;
+  q.ngroups = uint32_t((g.alo[2] + g.an[2] - 1) / V - q.g0 + 1);
+  q.qzlo = g.qlo[2];
+  q.qzhi = g.qhi[2];
+  q.mhi_bits = g.mhi_bits;
+  q.mlo_bits = g.mlo_bits;
+  q.rev = a.rev;
+  q.cmp = a.cmp;
+  b.dtdx = a.dtdx;
+  const R *zs = a.pml.sig[2], *zk = a.pml.kap[2], *zi = a.pml.siginv[2];
+  b.zsig = zs;
+  b.zkap = zk;
+  b.zsiginv = zi;
+  for (int c = 0; c < 3; ++c) {
+    const CurlComp<R> &k = a.cc[c];
+    b.F[c] = a.F[c];
+    b.f[c] = k.f;
+    if (!k.f) continue;
+    if (k.cnd || !a.hasp[c] || !a.hasm[c] || !a.F[(c + 1) % 3] || !a.F[(c + 2) % 3]) return false;
+    // the 3D uPML plan: dsig = c + 1, dsigu = c + 2 (src/step_db.cpp:56-59), with the z
+    // profile acting only through slot 0's f_u and on slot 1's f
+    if (k.dsig >= 0 && k.dsig != (c + 1) % 3) return false;
+    if (k.dsigu >= 0 && k.dsigu != (c + 2) % 3) return false;
+    if ((k.dsig == 2 || k.dsigu == 2) && !zs) return false;
+    if (c == 0 && k.dsigu == 2) b.fu0 = k.fu;
+    if (c == 0 && k.dsigu == 2 && !k.fu) return false;
+    const int ysh = PH ? 1 : 0; // electric: shifted along its own direction; magnetic: not
+    if (k.ysh[c] != ysh) return false;
+    if (upd && a.uc[c].f) {
+      const UpdComp<R> &u = a.uc[c];
+      if (u.dsigw >= 0 && u.dsigw != c) return false;
+      if (u.dsigw == 2) {
+        if (!u.fw || !zs) return false;
+        b.fw2 = u.fw;
+      }
+      b.e[c] = u.f;
+      b.chi[c] = u.u;
+      if (a.npole > NP) return false;
+      for (int p = 0; p < NP && p < a.npole; ++p) {
+        if (!u.P[p]) continue;
+        if (!u.Pp[p]) return false;
+        b.pmask |= 1 << (3 * p + c);
+        b.sigval[p][c] = u.sigval[p];
+        if ((u.sigone >> p) & 1) b.sigone |= 1 << (3 * p + c);
+        if (u.sigma[p]) b.smask |= 1 << (3 * p + c);
+      }
+    }
+  }
+  // pole arrays as offsets in per-component blocks: the base is the lowest address of the
+  // component's arrays, and every array must sit a whole number of arrays from it, with the
+  // same offsets for every component (else the launch goes to k_fast)
+  if (NP > 0) {
+    b.pst = 0;
+    for (int c = 0; c < 3; ++c) {
+      if (!b.e[c]) continue;
+      const UpdComp<R> &u = a.uc[c];
+      const R *lo = nullptr, *slo = nullptr;
+      for (int p = 0; p < NP && p < a.npole; ++p) {
+        if (u.P[p]) {
+          if (!lo || u.P[p] < lo) lo = u.P[p];
+          if (u.Pp[p] < lo) lo = u.Pp[p];
+        }
+        if (u.sigma[p] && (!slo || u.sigma[p] < slo)) slo = u.sigma[p];
+      }
+      b.pbase[c] = const_cast<R *>(lo);
+      b.sbase[c] = slo;
+    }
+    int64_t st = -1;
+    for (int c = 0; c < 3; ++c) {
+      if (!b.e[c]) continue;
+      const UpdComp<R> &u = a.uc[c];
+      for (int p = 0; p < NP && p < a.npole; ++p) {
+        const int64_t offs[3] = {u.P[p] ? int64_t(u.P[p] - b.pbase[c]) : -1, u.Pp[p] ? int64_t(u.Pp[p] - b.pbase[c]) : -1,
+                                 u.sigma[p] ? int64_t(u.sigma[p] - b.sbase[c]) : -1};
+        for (int w = 0; w < 3; ++w) {
+          if (offs[w] < 0) continue;
+          if (offs[w] > 0) {
+            if (st < 0) st = offs[w];
+            if (offs[w] % st) return false;
+          }
+        }
+      }
+    }
+    if (st < 0) st = 1;
+    b.pst = st;
+    for (int p = 0; p < NP; ++p) b.poff[p] = b.ppoff[p] = b.soff[p] = -1;
+    for (int c = 0; c < 3; ++c) {
+      if (!b.e[c]) continue;
+      const UpdComp<R> &u = a.uc[c];
+      for (int p = 0; p < NP && p < a.npole; ++p) {
+        const int64_t o[3] = {u.P[p] ? int64_t(u.P[p] - b.pbase[c]) / st : -1, u.Pp[p] ? int64_t(u.Pp[p] - b.pbase[c]) / st : -1,
+                              u.sigma[p] ? int64_t(u.sigma[p] - b.sbase[c]) / st : -1};
+        int *dst[3] = {&b.poff[p], &b.ppoff[p], &b.soff[p]};
+        for (int w = 0; w < 3; ++w) {
+          if (o[w] < 0) continue;
+          if (o[w] > 2 * NP) return false;
+          if (*dst[w] < 0)
+            *dst[w] = int(o[w]);
+          else if (*dst[w] != int(o[w]))
+            return false;
+        }
+      }
+    }
+  }
+  for (int p = 0; p < NP && p < a.npole; ++p) b.pole[p] = a.pole[p];
+  b.pswap = a.pswap;
+  b.src = a.src;
+  for (int k = 0; k < 3; ++k) {
+    b.sbox_lo[k] = a.sbox_lo[k];
+    b.sbox_hi[k] = a.sbox_hi[k];
+  }
+  b.vrow = a.vrow;
+  return true;
+}
+
+template <class R, int V, int PH, bool UPD, int NP>
+static bool r3_launch(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
+  Row3Args<R, NP> b;
+  if (!r3_fill<R, NP>(a, curl, upd, b, V)) return false;
+  const size_t n = size_t(b.g.ngroups) * size_t(a.g.an[0]) * size_t(a.g.an[1]);
+  if (n >= (size_t(1) << 32)) return false;
+  const uint32_t c32 = uint32_t(n);
+  const unsigned nb = nblocks8(n);
+  const bool zp = b.zsig != nullptr;
+  if (zp)
+    hipLaunchKernelGGL((k_row3<R, V, PH, UPD, NP, true>), dim3(nb), dim3(BLOCK), 0, s, b, c32);
+  else
+    hipLaunchKernelGGL((k_row3<R, V, PH, UPD, NP, false>), dim3(nb), dim3(BLOCK), 0, s, b, c32);
+  return true;
+}
+
+template <class R> bool launch_row3(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) {
+  static const bool off = getenv("MEEP_AMD_NO_ROW3") != nullptr;
+  if (off) return false;
+  constexpr int V = 16 / sizeof(R), VP = V > 2 ? 2 : V; // the pole-carrying updates take 8-byte groups in FP32
+  const int PH = a.sgn < 0 ? 1 : 0;
+  const bool u = upd;
+  if (PH == 0) return u ? r3_launch<R, V, 0, true, 0>(a, curl, upd, s) : r3_launch<R, V, 0, false, 0>(a, curl, upd, s);
+  if (!u) return r3_launch<R, V, 1, false, 0>(a, curl, upd, s);
+  switch (a.npole) {
+    case 0: return r3_launch<R, V, 1, true, 0>(a, curl, upd, s);
+    case 1: return r3_launch<R, VP, 1, true, 1>(a, curl, upd, s);
+    case 2: return r3_launch<R, VP, 1, true, 2>(a, curl, upd, s);
+    default: return false;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 
 template <class R> __device__ __forceinline__ void conn_one(const Conn *conns, const double2 *phases, int t, R *const *slots, int ncmp) {
   const Conn e = conns[t];
@@ -1222,6 +1716,7 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
 #define INSTANTIATE(R)                                                                             \
   template void launch_step<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_fast<R>(const StepArgs<R> &, bool, bool, bool, hipStream_t);                \
+  template bool launch_row3<R>(const StepArgs<R> &, bool, bool, hipStream_t);                      \
   template void launch_conn<R>(const Conn *, const double2 *, int, R *const *, int, hipStream_t);  \
   template void launch_src<R>(R *, const int64_t *, const double2 *, int, double2, double, int,      \
                               hipStream_t);                                                        \
