@@ -450,8 +450,8 @@ def run_pml_lorentz_dft(args, lib, steps=None, warmup=None):
                     "frequencies (" + ("Kahan FP32" if single else "FP64") + " accumulators)",
         "cells": cells, "resolution": PLD_A, "courant": 0.5, "parallelism": "none"}, steps, warmup)
     ms, nl = stats[capi.K_DE_INTERIOR]
-    line["roofline"] = roofline("D phase, interior box: k_fast<" + ("float, 2" if single else "double, 2")
-                                + ", curl, E update, uPML, 2 poles>", de_interior_bytes(word=wbytes), ms, nl,
+    line["roofline"] = roofline("D phase, interior box: k_row3<" + ("float, 2" if single else "double, 2")
+                                + ", D phase, E update, 2 poles, z uPML>", de_interior_bytes(word=wbytes), ms, nl,
                                 load_traffic(f"traffic_de_interior_pml_lorentz_dft_{args.precision}"))
     line["roofline"]["whole_step"] = {
         "alg_bytes_per_step": int(step_alg),
