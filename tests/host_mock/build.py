@@ -17,18 +17,20 @@ LIB = os.path.join(OUT, "libmeep_host.so")
 
 
 def launcher_stubs():
-    """C++ source defining every `template <class R> void launch_*(...)` declared in
-    engine.hpp with an empty body, instantiated for float and double."""
+    """C++ source defining every `template <class R> void|bool launch_*(...)` declared in
+    engine.hpp with an empty body (a bool launcher reports that it launched nothing, so the
+    host takes its general path), instantiated for float and double."""
     src = open(os.path.join(CSRC, "engine.hpp")).read()
-    decls = re.findall(r"template <class R>\s*void (launch_\w+)\(([^;]*?)\);", src, re.S)
+    decls = re.findall(r"template <class R>\s*(void|bool) (launch_\w+)\(([^;]*?)\);", src, re.S)
     if not decls:
         raise RuntimeError("no launcher declarations found in engine.hpp")
     out = ['#include "engine.hpp"', "namespace mx {"]
-    out += [f"template <class R> void {name}({args}) {{}}" for name, args in decls]
-    for name, args in decls:
+    out += [f"template <class R> {ret} {name}({args}) {{{' return false;' if ret == 'bool' else ''}}}"
+            for ret, name, args in decls]
+    for ret, name, args in decls:
         for real in ("float", "double"):
             typed = re.sub(r"\bR\b", real, args)
-            out.append(f"template void {name}<{real}>({typed});")
+            out.append(f"template {ret} {name}<{real}>({typed});")
     out.append("}")
     return "\n".join(out) + "\n"
 
