@@ -219,6 +219,19 @@ long mp_dft_flux_chunk_points(mp_dft_flux *fl, int which, int i);
 int mp_dft_flux_decimation(mp_dft_flux *fl);
 /* dft_chunk::dft of chunk i, interleaved (re, im), frequency fastest (src/meep.hpp:1138). */
 int mp_dft_flux_get_dft(mp_dft_flux *fl, int which, int i, double *out);
+/* fields::get_dft_array(dft_flux, component c, int num_freq, int *rank, size_t dims[3]) —
+   src/meep.hpp:2121-2122, src/dft.cpp:1259-1268 (process_dft_component :931-1057,1130-1256,
+   collapse_array src/array_slice.cpp:554-596).  The DFT of component c at frequency index
+   num_freq on the points of the monitor, each accumulator divided by its chunk's stored weight
+   and integration weight, summed over the directions in which the monitor volume is empty
+   with the interpolation weights along them.  Writes rank and dims (array dimensions in
+   X, Y, Z order of the non-degenerate directions, last fastest); out, if not NULL, receives
+   prod(dims) complex values, interleaved (re, im), rounded to realnum as the reference's
+   complex<realnum> array is.  Returns the number of complex values (0 where the reference
+   returns a null array: no chunk of c, or a monitor on a single grid point), -1 on error
+   (frequency index out of range). */
+long mp_dft_flux_get_dft_array(mp_dft_flux *fl, int c, int num_freq, int *rank, size_t *dims,
+                               double *out);
 
 /* ---- DFT fields ---------------------------------------------------------------------- */
 
@@ -238,6 +251,10 @@ long mp_dft_fields_chunk_points(mp_dft_fields *df, int i);
 int mp_dft_fields_chunk_component(mp_dft_fields *df, int i);
 /* dft_chunk::dft of chunk i, interleaved (re, im), frequency fastest. */
 int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out);
+/* fields::get_dft_array(dft_fields, component, num_freq, rank, dims) — src/meep.hpp:2123-2124,
+   src/dft.cpp:1281-1290; as mp_dft_flux_get_dft_array. */
+long mp_dft_fields_get_dft_array(mp_dft_fields *df, int c, int num_freq, int *rank, size_t *dims,
+                                 double *out);
 
 /* ---- reverse mode (gradient of a DFT objective) ------------------------------------- */
 

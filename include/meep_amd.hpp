@@ -332,11 +332,27 @@ public:
     return out;
   }
   int Nfreq() const { return n; }
+  mp_dft_flux *handle() const { return h; }
 
 private:
   mp_dft_flux *h;
   int n;
 };
+
+// the new[] array of fields::get_dft_array (src/dft.cpp:1259-1290), null where the reference's
+// is (no chunk of the component, or a monitor on one grid point)
+template <class H>
+std::complex<realnum> *get_dft_array_of(long (*fn)(H *, int, int, int *, size_t *, double *), H *h, component c,
+                                        int num_freq, int *rank, size_t dims[3]) {
+  long n = fn(h, int(c), num_freq, rank, dims, nullptr);
+  if (n < 0) throw std::runtime_error(mp_last_error());
+  if (n == 0) return nullptr;
+  std::vector<double> raw(size_t(2 * n));
+  if (fn(h, int(c), num_freq, rank, dims, raw.data()) < 0) throw std::runtime_error(mp_last_error());
+  std::complex<realnum> *out = new std::complex<realnum>[size_t(n)];
+  for (long q = 0; q < n; ++q) out[q] = std::complex<realnum>(realnum(raw[2 * q]), realnum(raw[2 * q + 1]));
+  return out;
+}
 
 // dft_fields (src/meep.hpp:1420-1441): accumulators per chunk, as the engine holds them
 class dft_fields {
@@ -472,6 +488,14 @@ public:
                             int Nfreq, bool use_centered_grid = true, int decimation_factor = 0, bool persist = false) {
     return add_dft_fields(cs, num_components, where, linspace(freq_min, freq_max, Nfreq), use_centered_grid,
                           decimation_factor, persist);
+  }
+  // get_dft_array(dft_flux | dft_fields, component, num_freq, rank, dims) (src/meep.hpp:2121-2124):
+  // a new[] array of prod(dims) values, or null
+  std::complex<realnum> *get_dft_array(dft_flux flux, component c, int num_freq, int *rank, size_t dims[3]) {
+    return get_dft_array_of(mp_dft_flux_get_dft_array, flux.handle(), c, num_freq, rank, dims);
+  }
+  std::complex<realnum> *get_dft_array(dft_fields fdft, component c, int num_freq, int *rank, size_t dims[3]) {
+    return get_dft_array_of(mp_dft_fields_get_dft_array, fdft.handle(), c, num_freq, rank, dims);
   }
   double field_energy() { // :2196
     double e = 0;

@@ -128,6 +128,10 @@ _SIGNATURES = {
     "mp_dft_fields_chunk_points": (C.c_long, [C.c_void_p, C.c_int]),
     "mp_dft_fields_chunk_component": (C.c_int, [C.c_void_p, C.c_int]),
     "mp_dft_fields_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_dft_flux_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
+    "mp_dft_fields_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                               C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
@@ -247,6 +251,24 @@ class structure:
             self.ptr = None
 
 
+def _get_dft_array(fn, ptr, lib, single, c, num_freq):
+    """The array of fields::get_dft_array as the reference's Python layer returns it
+    (python/meep.i:450-490): shape dims, or a 0-d array holding the single value (rank 0) or
+    zero (a null array); complex64 for single-precision fields."""
+    rank = C.c_int(0)
+    dims = (C.c_size_t * 3)()
+    n = _check(lib, fn(ptr, c, num_freq, C.byref(rank), dims, None))
+    out = np.empty(2 * max(n, 1))
+    n = _check(lib, fn(ptr, c, num_freq, C.byref(rank), dims, _dptr(out)))
+    dtype = np.complex64 if single else np.complex128
+    if n == 0:
+        return np.zeros((), dtype=dtype)
+    vals = (out[0:2 * n:2] + 1j * out[1:2 * n:2]).astype(dtype)
+    if rank.value == 0:
+        return vals.reshape(())
+    return vals.reshape(tuple(dims[r] for r in range(rank.value)))
+
+
 class dft_flux:
     """meep::dft_flux (src/meep.hpp:1209-1255)."""
 
@@ -273,6 +295,11 @@ class dft_flux:
         out = np.empty(2 * n * self.nfreq)
         _check(self.lib, self.lib.mp_dft_flux_get_dft(self.ptr, which, i, _dptr(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
+
+    def get_dft_array(self, c, num_freq):
+        """fields::get_dft_array(dft_flux, c, num_freq, ...) (src/dft.cpp:1259-1268)."""
+        return _get_dft_array(self.lib.mp_dft_flux_get_dft_array, self.ptr, self.lib,
+                              self.f.s.single, c, num_freq)
 
 
 class dft_fields:
@@ -305,6 +332,11 @@ class dft_fields:
         out = np.empty(2 * n * self.nfreq)
         _check(self.lib, self.lib.mp_dft_fields_get_dft(self.ptr, i, _dptr(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
+
+    def get_dft_array(self, c, num_freq):
+        """fields::get_dft_array(dft_fields, c, num_freq, ...) (src/dft.cpp:1281-1290)."""
+        return _get_dft_array(self.lib.mp_dft_fields_get_dft_array, self.ptr, self.lib,
+                              self.f.s.single, c, num_freq)
 
 
 class adjoint:

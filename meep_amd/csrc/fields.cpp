@@ -354,6 +354,9 @@ struct FieldsBase {
   virtual int flux_chunks(void *fl, int which) = 0;
   virtual long flux_points(void *fl, int which, int i) = 0;
   virtual void flux_dft(void *fl, int which, int i, double *out) = 0;
+  // fields::get_dft_array over the flux (both lists) or dft_fields (E list) monitor fl;
+  // returns the number of complex values
+  virtual long dft_array(void *fl, bool both, int c, int num_freq, int *rank, size_t *dims, double *out) = 0;
   virtual void flux_delete(void *fl) = 0;
 };
 
@@ -369,10 +372,19 @@ template <class R> struct DftChunk {
   int defer = 1, nsnap = 0;
   double snap_time[DFT_DEFER] = {};
   DevBuf snap;
+  // where the points sit, for get_dft_array: the box in half-pixel coordinates, the periodic
+  // image it stands for, its integration weights, and the monitor's empty directions
+  IV is, ie, shift;
+  V3 s0, s1, e0, e1;
+  double dV0 = 1;
+  bool weighted = false;
+  cdouble stored_weight = 1.0;
+  bool empty[3] = {};
 };
 template <class R> struct Flux {
   std::vector<DftChunk<R> *> E, H;
   int nfreq;
+  V3 wmin, wmax; // the monitor volume
 };
 
 template <class R> struct Fields : FieldsBase {
@@ -3022,11 +3034,17 @@ template <class R> struct Fields : FieldsBase {
   // Yee-to-centre average, or on the component's own grid (centered false, no average)
   void add_dft(std::vector<DftChunk<R> *> &out, int c, const V3 &mn, const V3 &mxv, const std::vector<double> &freq, bool include_w,
                cdouble stored_weight, int decimation, bool centered) {
-    loop_in_chunks(mn, mxv, centered ? 20 : c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &,
+    loop_in_chunks(mn, mxv, centered ? 20 : c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &shift,
                                    cdouble sph) {
       if (!f[c][0]) return;
       std::unique_ptr<DftChunk<R>> ch(new DftChunk<R>);
       ch->c = c;
+      ch->is = is, ch->ie = ie, ch->shift = shift;
+      ch->s0 = s0, ch->s1 = s1, ch->e0 = e0, ch->e1 = e1;
+      ch->dV0 = dV0;
+      ch->weighted = include_w;
+      ch->stored_weight = stored_weight;
+      for (int d = 0; d < 3; ++d) ch->empty[d] = gv.has(d) && mn.t[d] == mxv.t[d];
       if (centered)
         gv.yee2cent(c, ch->avg1, ch->avg2);
       else
@@ -3082,6 +3100,7 @@ template <class R> struct Fields : FieldsBase {
     }
     std::unique_ptr<Flux<R>> fl(new Flux<R>);
     fl->nfreq = int(freq.size());
+    fl->wmin = mn, fl->wmax = mxv;
     std::vector<DftChunk<R> *> El[2], Hl[2];
     for (int q = 0; q < 2; ++q) {
       add_dft(El[q], cE[q], mn, mxv, freq, true, cdouble(double(1 - 2 * q), 0), decim, centered);
@@ -3104,6 +3123,7 @@ template <class R> struct Fields : FieldsBase {
     *dec_out = decim;
     std::unique_ptr<Flux<R>> fl(new Flux<R>);
     fl->nfreq = int(freq.size());
+    fl->wmin = mn, fl->wmax = mxv;
     for (int q = 0; q < ncomp; ++q) {
       if (comps[q] < 0 || comps[q] >= NC) fail("invalid dft component %d", comps[q]);
       std::vector<DftChunk<R> *> lst;
@@ -3182,6 +3202,96 @@ template <class R> struct Fields : FieldsBase {
       out[2 * q] = v[q].real();
       out[2 * q + 1] = v[q].imag();
     }
+  }
+  // fields::get_dft_array (src/dft.cpp:1259-1290).  The grid spanned by the chunks of c
+  // (fields::get_dft_component_dims, :1061-1110: corners seeded from the cell's corners one
+  // step out) has a dimension per direction with more than one point.  Every point holds its
+  // accumulator at num_freq over the chunk's stored weight and, for a weighted chunk, over its
+  // nonzero integration weight, times its interpolation weight along the monitor's empty
+  // directions (dft_chunk::process_dft_component, :931-1057, retain_interp_weights).  The
+  // empty directions are then summed out in realnum (collapse_array,
+  // src/array_slice.cpp:554-596).
+  long dft_array(void *flp, bool both, int c, int nf, int *rank_out, size_t *dims_out, double *out) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    if (nf < 0 || nf >= fl->nfreq)
+      fail("process_dft_component: frequency index %d is outside the range of the frequency array of size %d", nf, fl->nfreq);
+    if (c < 0 || c >= NC) fail("get_dft_array: component %d is not a field component", c);
+    std::vector<const DftChunk<R> *> chs;
+    for (const DftChunk<R> *ch : fl->E)
+      if (ch->c == c) chs.push_back(ch);
+    if (both)
+      for (const DftChunk<R> *ch : fl->H)
+        if (ch->c == c) chs.push_back(ch);
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = 2 * gv.num[d] + 1, hi[d] = -1;
+    for (const DftChunk<R> *ch : chs)
+      for (int d = 0; d < 3; ++d)
+        if (gv.has(d)) {
+          lo[d] = std::min(lo[d], std::min(ch->is.t[d], ch->ie.t[d]) + ch->shift.t[d]);
+          hi[d] = std::max(hi[d], std::max(ch->is.t[d], ch->ie.t[d]) + ch->shift.t[d]);
+        }
+    int rank = 0, dir[3] = {0, 0, 0};
+    size_t n[3] = {1, 1, 1}, total = 1;
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d) && (hi[d] - lo[d]) / 2 + 1 > 1) {
+        dir[rank] = d;
+        n[rank++] = size_t((hi[d] - lo[d]) / 2 + 1);
+        total *= n[rank - 1];
+      }
+    *rank_out = 0;
+    dims_out[0] = dims_out[1] = dims_out[2] = 1;
+    if (rank == 0) return 0;
+    // the kept (non-empty) dimensions and their strides in the collapsed array
+    int rrank = 0;
+    bool keep[3];
+    size_t rstride[3] = {0, 0, 0}, rsize = 1;
+    for (int r = rank - 1; r >= 0; --r)
+      if ((keep[r] = !(fl->wmax.t[dir[r]] - fl->wmin.t[dir[r]] == 0.0))) {
+        rstride[r] = rsize;
+        rsize *= n[r];
+      }
+    for (int r = 0; r < rank; ++r)
+      if (keep[r]) dims_out[rrank++] = n[r];
+    *rank_out = rrank;
+    if (!out) return long(rsize);
+    std::vector<std::complex<R>> full(total, std::complex<R>(0, 0));
+    for (const DftChunk<R> *ch : chs) {
+      const std::vector<cdouble> acc = read_dft(*ch);
+      // interpolation weights: the chunk's edge weights along empty directions only
+      V3 s0i = ch->s0, s1i = ch->s1, e0i = ch->e0, e1i = ch->e1;
+      for (int d = 0; d < 3; ++d)
+        if (!ch->empty[d]) s0i.t[d] = s1i.t[d] = e0i.t[d] = e1i.t[d] = 1.0;
+      std::vector<double> wi;
+      wi.reserve(ch->npts);
+      visit_box(ch->is, ch->ie, s0i, s1i, e0i, e1i, 1.0, [&](int64_t, int64_t, const IV &, double w, const int *) { wi.push_back(w); });
+      visit_box(ch->is, ch->ie, ch->s0, ch->s1, ch->e0, ch->e1, ch->dV0, [&](int64_t, int64_t k, const IV &p, double w, const int *) {
+        cdouble v = acc[size_t(k) * ch->nfreq + nf] / ch->stored_weight;
+        if (ch->weighted && v != 0.0) v /= w;
+        size_t at = 0;
+        for (int r = 0; r < rank; ++r) {
+          const int off = p.t[dir[r]] + ch->shift.t[dir[r]] - lo[dir[r]];
+          if (off < 0 || size_t(off / 2) >= n[r]) fail("internal: DFT array point out of range");
+          at = at * n[r] + size_t(off / 2);
+        }
+        full[at] = std::complex<R>(wi[size_t(k)] * v);
+      });
+    }
+    if (rrank == rank) {
+      for (size_t q = 0; q < total; ++q) out[2 * q] = double(full[q].real()), out[2 * q + 1] = double(full[q].imag());
+      return long(total);
+    }
+    std::vector<R> red(2 * rsize, R(0));
+    for (size_t q = 0; q < total; ++q) {
+      size_t rem = q, at = 0;
+      for (int r = rank - 1; r >= 0; --r) {
+        at += rstride[r] * (rem % n[r]);
+        rem /= n[r];
+      }
+      red[2 * at] += full[q].real();
+      red[2 * at + 1] += full[q].imag();
+    }
+    for (size_t q = 0; q < 2 * rsize; ++q) out[q] = double(red[q]);
+    return long(rsize);
   }
   void flux_delete(void *flp) override {
     for (size_t q = 0; q < fluxes.size(); ++q)
@@ -4060,6 +4170,19 @@ int mp_dft_fields_chunk_component(mp_dft_fields *df, int i) {
 }
 int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out) {
   return mp_dft_flux_get_dft(reinterpret_cast<mp_dft_flux *>(df), 0, i, out);
+}
+
+static long dft_array(void *hp, bool both, int c, int nf, int *rank, size_t *dims, double *out) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(hp);
+  long n = 0;
+  const int rc = guard([&] { n = h->owner->dft_array(h->fl, both, c, nf, rank, dims, out); });
+  return rc ? -1 : n;
+}
+long mp_dft_flux_get_dft_array(mp_dft_flux *fl, int c, int num_freq, int *rank, size_t *dims, double *out) {
+  return dft_array(fl, true, c, num_freq, rank, dims, out);
+}
+long mp_dft_fields_get_dft_array(mp_dft_fields *df, int c, int num_freq, int *rank, size_t *dims, double *out) {
+  return dft_array(df, false, c, num_freq, rank, dims, out);
 }
 
 } // extern "C"

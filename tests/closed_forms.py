@@ -27,6 +27,14 @@ run against a value the mathematics fixes; the tests assert on these errors.
   (Drude: no restoring term), oscillating at sqrt(omega0^2 (1 + sigma) - (pi gamma)^2)
   (Lorentz) or sqrt(omega0^2 sigma - (pi gamma)^2) (Drude) with amplitude decay rate
   pi gamma; the discrete recurrence approaches this as (omega dt)^2.
+* ``dft_array_plane_wave``: fields::get_dft_array on the periodic line current of
+  ``plane_wave_2d``, whose field is uniform along the line.  From a flux monitor every point's
+  value is the lattice field (the E accumulators lose their stored weight, the flux sign, and
+  their integration weight; H has neither), from a field monitor on a line between two
+  centred planes it is the linear interpolation of the two planes' values (the interpolation
+  weights of the empty direction, then the sum over it), and from a monitor that crosses the
+  periodic boundary every point of both pieces holds the field (each piece placed by its
+  shift).
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -154,6 +162,57 @@ def plane_wave_2d(lib, a=10, freqs=None):
     # TM along x: Ez plays 1D Ex and Hy plays -(1D Hy); the X flux is -Re(Ez conj(Hy))
     flux_exp = Ly * np.real(e * np.conj(h))
     return {"flux": float(np.abs(flx.flux() - flux_exp).max() / np.abs(flux_exp).max())}
+
+
+def dft_array_plane_wave(lib, a=10, freqs=None):
+    """Errors of get_dft_array on a periodic 2D TM line current (a plane wave along x,
+    uniform along y) against the 1D lattice solution: the flux monitor's Ez and Hy arrays along
+    a centred line, a field monitor's Ez on a line 0.3 pixel past a centred line (the linear
+    interpolation of two centred lines), and a field monitor across the periodic y boundary.
+    Points of a weighted (flux E) chunk whose integration weight is zero hold zero, as the
+    reference's do; the others must hold the field."""
+    if freqs is None:
+        freqs = np.linspace(0.9, 1.1, 3)
+    Lx, Ly, x0, fcen, fwidth, T = 44.0, 1.3, 14.0, 1.0, 0.5, 28.0
+    s = structure(lib, dim=2, size=(Lx, Ly), a=a)
+    f = fields(s)
+    f.use_real_fields()
+    f.use_bloch((0.0, 0.0))
+    f.set_boundary(Low, X, Metallic)
+    f.set_boundary(High, X, Metallic)
+    f.add_volume_source(Ez, gaussian_src_time(fcen, fwidth), (x0, 0, 0), (x0, Ly, 0))
+    x1 = x0 + 5.0 + 0.5 / a
+    flx = f.add_dft_flux(X, (x1, 0.2, 0), (x1, 1.1, 0), freqs, decimation_factor=1)
+    mid = f.add_dft_fields([Ez], (x1 + 0.3 / a, 0.25, 0), (x1 + 0.3 / a, 1.05, 0), freqs, decimation_factor=1)
+    wrap = f.add_dft_fields([Ez], (x1, Ly - 0.35, 0), (x1, Ly + 0.35, 0), freqs, decimation_factor=1)
+    dt = s.dt
+    nsteps = int(round(T / dt))
+    f.step_n(nsteps)
+    J = a * current_transform(fcen, fwidth, dt, nsteps, freqs)
+    j0, i = int(round(x0 * a)), int(round(x1 * a - 0.5))
+    scale = dt / math.sqrt(2 * math.pi)
+    def e_at(k):
+        return scale * 0.5 * (lattice_field(freqs, dt, J, k, j0) + lattice_field(freqs, dt, J, k + 1, j0))
+    e, e1 = e_at(i), e_at(i + 1)
+    h = scale * lattice_field(freqs, dt, J, i, j0, half=True)
+    err = {"E_flux": 0.0, "H_flux": 0.0, "E_interp": 0.0, "E_wrap": 0.0}
+    for nf in range(len(freqs)):
+        ez = flx.get_dft_array(Ez, nf)
+        hy = flx.get_dft_array(Hy, nf)
+        assert ez.ndim == 1 and ez.shape == hy.shape and ez.size >= int(0.9 * a) - 1, (ez.shape, hy.shape)
+        held = np.abs(ez) > 0.5 * abs(e[nf])
+        assert held.sum() >= ez.size - 2, np.abs(ez)
+        err["E_flux"] = max(err["E_flux"], float(np.abs(ez[held] - e[nf]).max() / abs(e[nf])))
+        # TM along x: Hy is minus the 1D H of the sheet
+        err["H_flux"] = max(err["H_flux"], float(np.abs(hy + h[nf]).max() / abs(h[nf])))
+        m = mid.get_dft_array(Ez, nf)
+        assert m.ndim == 1 and m.size >= int(0.8 * a), m.shape
+        m_exp = 0.7 * e[nf] + 0.3 * e1[nf]
+        err["E_interp"] = max(err["E_interp"], float(np.abs(m - m_exp).max() / abs(m_exp)))
+        w = wrap.get_dft_array(Ez, nf)
+        assert w.ndim == 1 and w.size >= int(0.7 * a), w.shape
+        err["E_wrap"] = max(err["E_wrap"], float(np.abs(w - e[nf]).max() / abs(e[nf])))
+    return err
 
 
 def source_spectrum(dt=0.025, fcen=1.0, fwidth=0.5, freqs=None):

@@ -8,6 +8,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from meep_amd import capi
@@ -104,3 +105,65 @@ int main() {
     r = subprocess.run([out], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("meep: Can't use real fields with bloch"), r.stdout
+
+
+DFT_ARRAY_PROGRAM = r"""
+#include "meep_amd.hpp"
+#include <cstdio>
+using namespace meep;
+double eps(const vec &p) { return p.x() > 2.0 && p.x() < 3.0 ? 2.0 : 1.0; }
+int main() {
+  structure s(voltwo(6.0, 4.0, 10.0), eps, pml(1.0, X));
+  fields f(&s);
+  f.use_real_fields();
+  f.use_bloch(Y, 0.0);
+  f.add_volume_source(Ez, gaussian_src_time(1.0, 0.5), volume(vec(1.73, 1.12)));
+  double freq[3] = {0.9, 1.0, 1.1};
+  dft_flux fl = f.add_dft_flux(X, volume(vec(4.03, 0.5), vec(4.03, 3.5)), freq, 3, true, true, 1);
+  component cs[1] = {Ez};
+  dft_fields pt = f.add_dft_fields(cs, 1, volume(vec(3.05, 2.05)), freq, 3, true, 1);
+  for (int i = 0; i < 300; ++i) f.step();
+  int rank;
+  size_t dims[3];
+  std::complex<realnum> *a = f.get_dft_array(fl, Ez, 1, &rank, dims);
+  std::printf("rank %d dims %zu\n", rank, dims[0]);
+  for (size_t i = 0; i < dims[0]; ++i) std::printf("%.17g %.17g\n", double(a[i].real()), double(a[i].imag()));
+  delete[] a;
+  std::complex<realnum> *b = f.get_dft_array(pt, Ez, 0, &rank, dims);
+  std::printf("point %s rank %d\n", b ? "array" : "null", rank);
+  try { f.get_dft_array(fl, Ez, 3, &rank, dims); }
+  catch (const std::runtime_error &e) { std::printf("%s\n", e.what()); }
+  return 0;
+}
+"""
+
+
+def test_cpp_api_get_dft_array(tmp_path, oracle_lib):
+    """fields::get_dft_array through the C++ API (a new[] array, null for a monitor on one
+    centred point, an exception for a frequency index out of range) gives the values the
+    C-ABI gives."""
+    src = tmp_path / "dft_array.cpp"
+    src.write_text(DFT_ARRAY_PROGRAM)
+    out = str(tmp_path / "dft_array")
+    mbuild.build_cpp_program(str(src), out, os.path.join(ROOT, "oracle", "_build"), "meep_oracle")
+    r = subprocess.run([out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    rank, n = int(lines[0].split()[1]), int(lines[0].split()[3])
+    vals = np.array([complex(float(x), float(y)) for x, y in (ln.split() for ln in lines[1:1 + n])])
+    assert rank == 1 and n > 20
+    assert lines[1 + n] == "point null rank 0"
+    assert "frequency index 3 is outside the range" in lines[2 + n]
+    from meep_amd.capi import Ez as cEz, High, Low, X as cX, Y as cY, fields, gaussian_src_time, structure
+    s = structure(oracle_lib, dim=2, size=(6.0, 4.0), a=10, eps=lambda x, y, z: 2.0 if 2.0 < x < 3.0 else 1.0,
+                  pml=[(cX, side, 1.0) for side in (High, Low)])
+    f = fields(s)
+    f.use_real_fields()
+    f.use_bloch_dir(cY, 0.0)
+    # a C++ src_time is integrated by default (src/meep.hpp:957)
+    f.add_volume_source(cEz, gaussian_src_time(1.0, 0.5, is_integrated=True), (1.73, 1.12, 0), (1.73, 1.12, 0))
+    fl = f.add_dft_flux(cX, (4.03, 0.5, 0), (4.03, 3.5, 0), [0.9, 1.0, 1.1], decimation_factor=1)
+    f.step_n(300)
+    want = fl.get_dft_array(cEz, 1)
+    assert want.shape == (n,)
+    assert np.abs(vals - want).max() <= 1e-15 * np.abs(want).max()

@@ -1,6 +1,7 @@
 """The closed forms of tests/closed_forms.py on the MI355X engine in FP64, at the tolerances
 the oracle meets: the engine's DFT scale, phase and E/H time offset, current injection,
-flux weights, Lorentz and Drude poles, magnetic/metallic walls and conductivity are held to
+flux weights, Lorentz and Drude poles, magnetic/metallic walls, conductivity and the DFT
+arrays of get_dft_array are held to
 the mathematics directly, not only to the oracle."""
 
 import pytest
@@ -41,3 +42,7 @@ def test_conductive_upml_branches_reduce_to_lossless_gpu(gpu_lib):
 
 def test_upml_branch_is_local_to_its_region_gpu(gpu_lib):
     T.check_pml_locality(gpu_lib)
+
+
+def test_dft_array_of_a_plane_wave_gpu(gpu_lib):
+    T.check_dft_array(gpu_lib)

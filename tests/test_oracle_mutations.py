@@ -7,7 +7,9 @@ weights dropped, a Drude pole with a restoring term, pole damping off by 2 pi, a
 wall at the wrong plane, a conductive update with dt for dt / 2, an electric conductivity
 not scaled by chi1inv, a current in a conductive medium not scaled by condinv, the conductive
 uPML branch without its (kap - sig) factor, a uPML branch taken over the whole chunk instead
-of its region."""
+of its region, and a DFT array that keeps the integration weights or the stored weight, sums
+the planes around an off-grid monitor unweighted, or places a periodic piece without its
+shift."""
 
 import ctypes
 import os
@@ -33,6 +35,10 @@ CHECKS = {
     "COND_SRC": T.check_conductive_line,
     "COND_PML": T.check_conductive_pml_limit,
     "PML_CHUNKWIDE": T.check_pml_locality,
+    "DFTARR_WEIGHT": T.check_dft_array,
+    "DFTARR_SIGN": T.check_dft_array,
+    "DFTARR_INTERP": T.check_dft_array,
+    "DFTARR_SHIFT": T.check_dft_array,
 }
 
 
