@@ -48,6 +48,7 @@ typedef struct mp_structure mp_structure;
 typedef struct mp_fields mp_fields;
 typedef struct mp_dft_flux mp_dft_flux;
 typedef struct mp_dft_fields mp_dft_fields;
+typedef struct mp_flux_vol mp_flux_vol;
 typedef struct mp_adjoint mp_adjoint;
 typedef struct mp_batch mp_batch;
 
@@ -255,6 +256,31 @@ int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out);
    src/dft.cpp:1281-1290; as mp_dft_flux_get_dft_array. */
 long mp_dft_fields_get_dft_array(mp_dft_fields *df, int c, int num_freq, int *rank, size_t *dims,
                                  double *out);
+
+/* ---- time-domain flux ----------------------------------------------------------------- */
+
+/* fields::add_flux_vol(direction, const volume &) — src/meep.hpp:2205-2207,
+   src/energy_and_flux.cpp:230-235; d < 0 is fields::add_flux_plane(volume) (:238-241), whose
+   direction is the volume's normal (volume::normal_direction, src/vec.cpp:227-257).  The
+   flux of E x H through the volume, taken by every step after its H update and after its E
+   update (src/step.cpp:93,124; flux_vol::update_half / update, src/meep.hpp:2330-2337).  The
+   plane stays with the fields as the reference's list does; mp_flux_vol_delete releases the
+   handle. */
+mp_flux_vol *mp_fields_add_flux_vol(mp_fields *f, int d, const double *vmin, const double *vmax);
+void mp_flux_vol_delete(mp_flux_vol *fv);
+/* flux_vol::flux() — src/meep.hpp:2339: the mean of the last step's two fluxes. */
+int mp_flux_vol_flux(mp_flux_vol *fv, double *out);
+/* fields::flux_in_box(direction, const volume &) — src/energy_and_flux.cpp:223-228: E x H with
+   H synchronised to E's time (synchronize_magnetic_fields, :146-164); d < 0 as above. */
+int mp_fields_flux_in_box(mp_fields *f, int d, const double *vmin, const double *vmax, double *out);
+/* fields::flux_in_box_wrongH — src/energy_and_flux.cpp:186-221: E x H as they stand. */
+int mp_fields_flux_in_box_wrongH(mp_fields *f, int d, const double *vmin, const double *vmax,
+                                 double *out);
+/* fields::field_energy_in_box(const volume &) (which 0: the magnetic part with H synchronised,
+   src/energy_and_flux.cpp:54-59), electric_energy_in_box (1) and magnetic_energy_in_box (2,
+   H as it stands) (:84-95), over [vmin, vmax]. */
+int mp_fields_energy_in_box(mp_fields *f, int which, const double *vmin, const double *vmax,
+                            double *out);
 
 /* ---- reverse mode (gradient of a DFT objective) ------------------------------------- */
 

@@ -381,6 +381,24 @@ private:
   int n;
 };
 
+// flux_vol (src/meep.hpp:2322-2352): the flux of E x H through a volume, taken by every step;
+// owned by its fields as in the reference
+class flux_vol {
+public:
+  explicit flux_vol(mp_flux_vol *h) : h(h) {}
+  ~flux_vol() { mp_flux_vol_delete(h); }
+  flux_vol(const flux_vol &) = delete;
+  flux_vol &operator=(const flux_vol &) = delete;
+  double flux() {
+    double v = 0;
+    check(mp_flux_vol_flux(h, &v));
+    return v;
+  }
+
+private:
+  mp_flux_vol *h;
+};
+
 // fields (src/meep.hpp:1716-2320)
 class fields {
 public:
@@ -398,8 +416,11 @@ public:
       if (kq != 0) meep::abort("the bfast formulation (bfast_scaled_k != 0) is not supported by the GPU engine");
     p = mp_fields_new(s->handle());
     if (!p) throw std::runtime_error(mp_last_error());
+    dt = mp_fields_dt(p);
   }
+  double dt; // the time step Courant / a (fields::dt, src/meep.hpp:1725)
   ~fields() {
+    for (flux_vol *v : flux_vols) delete v;
     for (mp_dft_flux *h : fluxes) mp_dft_flux_delete(h);
     for (mp_dft_fields *h : dft_fields_list) mp_dft_fields_delete(h);
     mp_fields_delete(p);
@@ -502,6 +523,24 @@ public:
     check(mp_fields_field_energy(p, &e));
     return e;
   }
+  // the energy in a box (:2196-2201, src/energy_and_flux.cpp:54-95)
+  double field_energy_in_box(const volume &where) { return energy_in_box(0, where); }
+  double electric_energy_in_box(const volume &where) { return energy_in_box(1, where); }
+  double magnetic_energy_in_box(const volume &where) { return energy_in_box(2, where); }
+  // time-domain flux (:2203-2207, src/energy_and_flux.cpp:186-245)
+  double flux_in_box(direction d, const volume &where) {
+    double v = 0;
+    check(mp_fields_flux_in_box(p, d, where.get_min_corner().data(), where.get_max_corner().data(), &v));
+    return v;
+  }
+  double flux_in_box_wrongH(direction d, const volume &where) {
+    double v = 0;
+    check(mp_fields_flux_in_box_wrongH(p, d, where.get_min_corner().data(), where.get_max_corner().data(), &v));
+    return v;
+  }
+  flux_vol *add_flux_vol(direction d, const volume &where) { return add_flux(int(d), where); }
+  flux_vol *add_flux_plane(const volume &where) { return add_flux(-1, where); }
+  flux_vol *add_flux_plane(const vec &p1, const vec &p2) { return add_flux(-1, volume(p1, p2)); }
   void print_times() { // :1879; the phases' wall times as the engine accounts them
     double t7[7];
     check(mp_fields_get_times(p, t7));
@@ -512,10 +551,22 @@ public:
   mp_fields *handle() { return p; }
 
 private:
+  double energy_in_box(int which, const volume &where) {
+    double v = 0;
+    check(mp_fields_energy_in_box(p, which, where.get_min_corner().data(), where.get_max_corner().data(), &v));
+    return v;
+  }
+  flux_vol *add_flux(int d, const volume &where) {
+    mp_flux_vol *h = mp_fields_add_flux_vol(p, d, where.get_min_corner().data(), where.get_max_corner().data());
+    if (!h) throw std::runtime_error(mp_last_error());
+    flux_vols.push_back(new flux_vol(h));
+    return flux_vols.back();
+  }
   grid_volume gv;
   mp_fields *p;
   std::vector<mp_dft_flux *> fluxes;
   std::vector<mp_dft_fields *> dft_fields_list;
+  std::vector<flux_vol *> flux_vols;
 };
 
 } // namespace meep

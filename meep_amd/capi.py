@@ -128,6 +128,15 @@ _SIGNATURES = {
     "mp_dft_fields_chunk_points": (C.c_long, [C.c_void_p, C.c_int]),
     "mp_dft_fields_chunk_component": (C.c_int, [C.c_void_p, C.c_int]),
     "mp_dft_fields_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_fields_add_flux_vol": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mp_flux_vol_delete": (None, [C.c_void_p]),
+    "mp_flux_vol_flux": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_fields_flux_in_box": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]),
+    "mp_fields_flux_in_box_wrongH": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mp_fields_energy_in_box": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]),
     "mp_dft_flux_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                              C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
     "mp_dft_fields_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
@@ -339,6 +348,26 @@ class dft_fields:
                               self.f.s.single, c, num_freq)
 
 
+class flux_vol:
+    """meep::flux_vol (src/meep.hpp:2322-2352): the flux of E x H through a volume, taken by
+    every step; it stays with its fields."""
+
+    def __init__(self, fields, ptr):
+        self.f = fields
+        self.lib = fields.lib
+        self.ptr = ptr
+
+    def flux(self):
+        out = C.c_double()
+        _check(self.lib, self.lib.mp_flux_vol_flux(self.ptr, C.byref(out)))
+        return out.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.lib.mp_flux_vol_delete(self.ptr)
+            self.ptr = None
+
+
 class adjoint:
     """Reverse-mode gradient of J = sum |a|^2 over a dft_fields monitor's accumulators with
     respect to chi1inv of electric component c on the design box [vmin, vmax]; create it
@@ -490,6 +519,48 @@ class fields:
         e = C.c_double()
         _check(self.lib, self.lib.mp_fields_field_energy(self.ptr, C.byref(e)))
         return e.value
+
+    def _energy_in_box(self, which, vmin, vmax):
+        e = C.c_double()
+        _check(self.lib, self.lib.mp_fields_energy_in_box(self.ptr, which, _dvec(vmin), _dvec(vmax), C.byref(e)))
+        return e.value
+
+    def field_energy_in_box(self, vmin, vmax):
+        """fields::field_energy_in_box(const volume &) (src/energy_and_flux.cpp:54-59)."""
+        return self._energy_in_box(0, vmin, vmax)
+
+    def electric_energy_in_box(self, vmin, vmax):
+        """fields::electric_energy_in_box (src/energy_and_flux.cpp:84-88)."""
+        return self._energy_in_box(1, vmin, vmax)
+
+    def magnetic_energy_in_box(self, vmin, vmax):
+        """fields::magnetic_energy_in_box (src/energy_and_flux.cpp:90-95)."""
+        return self._energy_in_box(2, vmin, vmax)
+
+    def add_flux_vol(self, d, vmin, vmax):
+        """fields::add_flux_vol(direction, volume) (src/energy_and_flux.cpp:230-235)."""
+        ptr = self.lib.mp_fields_add_flux_vol(self.ptr, d, _dvec(vmin), _dvec(vmax))
+        if not ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        return flux_vol(self, ptr)
+
+    def add_flux_plane(self, vmin, vmax):
+        """fields::add_flux_plane(p1, p2) (src/energy_and_flux.cpp:238-245): the direction is
+        the volume's normal."""
+        return self.add_flux_vol(-1, vmin, vmax)
+
+    def flux_in_box(self, d, vmin, vmax):
+        """fields::flux_in_box(direction, volume) (src/energy_and_flux.cpp:223-228)."""
+        out = C.c_double()
+        _check(self.lib, self.lib.mp_fields_flux_in_box(self.ptr, d, _dvec(vmin), _dvec(vmax), C.byref(out)))
+        return out.value
+
+    def flux_in_box_wrongH(self, d, vmin, vmax):
+        """fields::flux_in_box_wrongH(direction, volume) (src/energy_and_flux.cpp:186-221)."""
+        out = C.c_double()
+        _check(self.lib, self.lib.mp_fields_flux_in_box_wrongH(self.ptr, d, _dvec(vmin), _dvec(vmax),
+                                                                C.byref(out)))
+        return out.value
 
     def get_field(self, c, p):
         re, im = C.c_double(), C.c_double()

@@ -206,6 +206,21 @@ template <class R> struct DftArgs {
   int acc_kind; // MP_DFT_* of meep_amd.h
 };
 
+// The flux of E x H through a plane at one instant (fields::flux_in_box_wrongH,
+// src/energy_and_flux.cpp:186-221, over fields::integrate, src/integrate.cpp:46-120): the sum
+// over the plane's centred-grid points of w * sign_p * Re(conj(E_p) H_p) for the component
+// pairs p, each field the mean of its four Yee points around the point.  Each block writes its
+// partial sum; the host adds the FLUX_BLOCKS partials.
+constexpr int FLUX_BLOCKS = 128;
+template <class R> struct FluxPlaneArgs {
+  const int64_t *idx;
+  const double *w;
+  int n, npair;
+  const R *e[2][2], *h[2][2]; // [pair][real, imaginary]; null where a part is absent
+  int64_t eo[2][2], ho[2][2]; // [pair][the two centring offsets]
+  double sign[2];
+};
+
 // Deferred DFT accumulation: the averaged, weighted field values of a chunk's points are kept
 // for up to DFT_DEFER accumulation steps (snap0/snap1, step-major), then one pass applies
 // every step's term to each accumulator, in step order and with the arithmetic of one step at
@@ -350,5 +365,7 @@ template <class R> void launch_dft_flush(const DftArgs<R> &a, const DftDefer<R> 
 template <class R>
 void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int n, int *flag,
                   hipStream_t s);
+// FLUX_BLOCKS partial sums of the plane flux into partial
+template <class R> void launch_flux_plane(const FluxPlaneArgs<R> &a, double *partial, hipStream_t s);
 
 } // namespace mx

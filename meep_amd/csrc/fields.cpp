@@ -357,6 +357,12 @@ struct FieldsBase {
   // fields::get_dft_array over the flux (both lists) or dft_fields (E list) monitor fl;
   // returns the number of complex values
   virtual long dft_array(void *fl, bool both, int c, int num_freq, int *rank, size_t *dims, double *out) = 0;
+  // time-domain flux: flux_vol planes, flux_in_box (sync) and flux_in_box_wrongH
+  virtual void *add_flux_vol(int d, V3 mn, V3 mx) = 0;
+  virtual double flux_vol_value(void *fv) = 0;
+  virtual double flux_in_box(int d, V3 mn, V3 mx, bool sync) = 0;
+  // field_energy_in_box (which 0), electric_ (1), magnetic_energy_in_box (2)
+  virtual double energy_in_box(int which, V3 mn, V3 mx) = 0;
   virtual void flux_delete(void *fl) = 0;
 };
 
@@ -1974,6 +1980,9 @@ template <class R> struct Fields : FieldsBase {
     if (!plan_valid) rebuild_plan();
     StepArgs<R> A = base_args();
     phase_B(A);
+    // flux_vol::update_half (src/step.cpp:93): only the last step's values are ever read
+    if (probe)
+      for (auto &fv : flux_vols) flux_launch(fv->plane, static_cast<double *>(fv->partial.p));
     // --- D, then E = chi1inv (D - P - dipoles) and the polarisations ---
     alloc_pols();
     if (!plan_valid) rebuild_plan();
@@ -2031,6 +2040,11 @@ template <class R> struct Fields : FieldsBase {
     }
     if (!plan_valid) rebuild_plan();
     boundaries(1); // D, P_E, E (step_boundaries D_stuff .. E_stuff)
+    if (probe) // flux_vol::update (src/step.cpp:124)
+      for (auto &fv : flux_vols) {
+        flux_launch(fv->plane, static_cast<double *>(fv->partial.p) + FLUX_BLOCKS);
+        fv->stepped = true;
+      }
     t += 1;
     if (adj && adj->recording) record_design();
     if (!dft_off) update_dfts();
@@ -2081,49 +2095,29 @@ template <class R> struct Fields : FieldsBase {
   // 0.5 * integral of Re(conj(F) G) over the cell for F = E (H) and G = D (B), per component,
   // in FP64 on the host from read-back arrays: E and D share a grid, so each component is
   // integrated on its own grid (the reference's four-point average has zero offsets there).
-  long double energy_of(int first) {
-    V3 wmin, wmax;
-    for (int d = 0; d < 3; ++d) wmax.t[d] = gv.has(d) ? gv.num[d] * gv.inva : 0.0;
+  // 0.5 * fields::integrate of Re(conj(F) G) (src/energy_and_flux.cpp:67-95) for F = E (H)
+  // and G = D (B) of each component over [mn, mx], on the component's own grid
+  long double energy_sum(int first, const V3 &mn, const V3 &mx) {
     long double sum = 0;
     for (int c = first; c < first + 5; ++c) {
       if (!has_field(c) || !f[c][0]) continue;
-      const int c2 = c + 10;
-      std::vector<R> h[2][2];
-      for (int q = 0; q < 2; ++q)
-        for (int k = 0; k < 2; ++k) {
-          const R *src = f[q ? c2 : c][k];
-          if (!src) continue;
-          h[q][k].resize(gv.ntot);
-          HIPCHECK(hipMemcpy(h[q][k].data(), src, sizeof(R) * gv.ntot, hipMemcpyDeviceToHost));
-        }
-      cdouble total = 0;
-      loop_in_chunks(wmin, wmax, c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1,
-                                        double dV0, const IV &, cdouble sph) {
-        std::complex<long double> part = 0;
-        visit_box(is, ie, s0, s1, e0, e1, dV0, [&](int64_t idx, int64_t, const IV &, double w, const int *) {
-          std::complex<R> fv[2];
-          for (int q = 0; q < 2; ++q) {
-            double fr[2] = {0, 0};
-            for (int k = 0; k < 2; ++k)
-              if (!h[q][k].empty()) {
-                const R a = h[q][k][idx];
-                fr[k] = 0.25 * (a + a + a + a);
-              }
-            fv[q] = std::complex<R>(cdouble(fr[0], fr[1]) * sph);
-          }
-          const cdouble integrand = std::real(std::conj(cdouble(fv[0])) * cdouble(fv[1]));
-          part += std::complex<long double>(integrand * w);
-        });
-        total += cdouble(part);
-      });
-      sum += std::real(total) * 0.5;
+      const int pair[1][2] = {{c, c + 10}};
+      sum += dot_integral(mn, mx, c, pair, 1) * 0.5;
     }
     return sum;
+  }
+  long double energy_of(int first) {
+    V3 wmin, wmax;
+    for (int d = 0; d < 3; ++d) wmax.t[d] = gv.has(d) ? gv.num[d] * gv.inva : 0.0;
+    return energy_sum(first, wmin, wmax);
   }
   // fields::field_energy with synchronize_magnetic_fields / restore_magnetic_fields
   // (src/energy_and_flux.cpp:46-62,146-178): one more B phase, B and H averaged with their
   // values half a step earlier, the magnetic energy taken, every magnetic array restored.
-  double field_energy() override {
+  // synchronize_magnetic_fields / restore_magnetic_fields (src/energy_and_flux.cpp:146-178)
+  // around fn: B and H advanced half a step on the device and averaged with their values at
+  // t - dt/2, fn evaluated, every magnetic array and its auxiliaries restored
+  template <class FN> double with_synchronized_magnetic(FN fn) {
     if (!plan_valid) rebuild_plan();
     HIPCHECK(hipStreamSynchronize(stream));
     struct Saved {
@@ -2159,11 +2153,169 @@ template <class R> struct Fields : FieldsBase {
     for (const Saved &sv : saved)
       if (sv.average) launch_average<R>(sv.p, static_cast<const R *>(sv.b->p), gv.ntot, stream);
     HIPCHECK(hipStreamSynchronize(stream));
-    const double magnetic = double(energy_of(5));
+    const double value = fn();
     for (const Saved &sv : saved)
       HIPCHECK(hipMemcpyAsync(sv.p, sv.b->p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
     HIPCHECK(hipStreamSynchronize(stream));
+    return value;
+  }
+  double field_energy() override {
+    const double magnetic = with_synchronized_magnetic([&] { return double(energy_of(5)); });
     return double(energy_of(0)) + magnetic;
+  }
+
+  // ---- time-domain flux (src/energy_and_flux.cpp:180-241, flux_vol src/meep.hpp:2322-2352) ----
+  struct FluxPlane {
+    int d = 0;
+    int grid = 20; // the integration grid: a component's, or 20 (centred)
+    int n = 0;
+    DevBuf idx, w; // the points of the volume on that grid and their weights
+  };
+  struct FluxVol {
+    FluxPlane plane;
+    DevBuf partial; // FLUX_BLOCKS partial sums after the last step's H update, then after its E update
+    bool stepped = false;
+  };
+  std::vector<std::unique_ptr<FluxVol>> flux_vols;
+  // the pairs of flux_in_box_wrongH (src/energy_and_flux.cpp:190-204): E_0 x H_0 - E_1 x H_1
+  static void flux_pairs(int d, int cE[2], int cH[2]) {
+    if (d == 0)
+      cE[0] = 1, cE[1] = 4, cH[0] = 9, cH[1] = 6;
+    else if (d == 1)
+      cE[0] = 4, cE[1] = 0, cH[0] = 5, cH[1] = 9;
+    else
+      cE[0] = 0, cE[1] = 1, cH[0] = 6, cH[1] = 5;
+  }
+  // add_flux_plane's direction (volume::normal_direction, src/vec.cpp:227-257) when d < 0
+  int flux_direction(int d, const V3 &mn, const V3 &mx) const {
+    if (d >= 0) {
+      if (d > 2) fail("invalid direction in add_flux_vol");
+      return d;
+    }
+    if (gv.dim == 1) return 2;
+    int nd = -1, nzero = 0, npos = 0;
+    for (int k = 0; k < 3; ++k)
+      if (gv.has(k)) {
+        const double ext = mx.t[k] - mn.t[k];
+        if (ext == 0) nd = k, ++nzero;
+        else if (ext > 0) ++npos;
+      }
+    if (nzero != 1 || npos != gv.dim - 1) fail("invalid direction in add_flux_vol");
+    return nd;
+  }
+  // the integration points of fields::integrate (loop_in_chunks over the volume on `grid`,
+  // src/integrate.cpp:122-165) with their weights; |shift phase|^2 folded in, as each field of
+  // a product carries the phase of its periodic image
+  void plan_points(FluxPlane &fp, int grid, const V3 &mn, const V3 &mx) {
+    fp.grid = grid;
+    std::vector<int64_t> idx;
+    std::vector<double> w;
+    loop_in_chunks(mn, mx, grid, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &,
+                                     cdouble sph) {
+      const double ph2 = std::norm(sph);
+      visit_box(is, ie, s0, s1, e0, e1, dV0, [&](int64_t i, int64_t, const IV &, double ww, const int *) {
+        idx.push_back(i);
+        w.push_back(ww * ph2);
+      });
+    });
+    // every field read sits within the arrays: the centring offsets are at most one stride up
+    int64_t reach = 0;
+    for (int d = 0; d < 3; ++d)
+      if (gv.has(d) && grid == 20) reach += gv.st[d];
+    for (int64_t i : idx)
+      if (i < 0 || size_t(i + reach) >= gv.ntot) fail("internal: integration point outside the array");
+    fp.n = int(idx.size());
+    if (idx.size() != size_t(fp.n)) fail("integration volume too large");
+    if (fp.n) {
+      fp.idx.alloc(sizeof(int64_t) * idx.size());
+      fp.w.alloc(sizeof(double) * w.size());
+      HIPCHECK(hipMemcpy(fp.idx.p, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(fp.w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    }
+  }
+  void plan_flux(FluxPlane &fp, int d, const V3 &mn, const V3 &mx) {
+    fp.d = d;
+    plan_points(fp, 20, mn, mx);
+  }
+  // sum over the points of sign_p Re(conj(F_p) G_p) w for the pairs (F_p, G_p), signs +1, -1;
+  // on the centred grid each field is the mean of its four Yee points, on a component's own
+  // grid its value (the reference's four-point mean with zero offsets)
+  void dot_launch(const FluxPlane &fp, const int (*pairs)[2], int np, double *partial) {
+    if (!fp.n) return;
+    FluxPlaneArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.idx = static_cast<const int64_t *>(fp.idx.p);
+    a.w = static_cast<const double *>(fp.w.p);
+    a.n = fp.n;
+    for (int p = 0; p < np; ++p) {
+      const int cF = pairs[p][0], cG = pairs[p][1];
+      // a component this grid lacks reads as zero (src/integrate.cpp:103-111)
+      if (!f[cF][0] || !f[cG][0]) continue;
+      for (int cmp = 0; cmp < 2; ++cmp) {
+        a.e[a.npair][cmp] = f[cF][cmp];
+        a.h[a.npair][cmp] = f[cG][cmp];
+      }
+      if (fp.grid == 20) {
+        gv.yee2cent(cF, a.eo[a.npair][0], a.eo[a.npair][1]);
+        gv.yee2cent(cG, a.ho[a.npair][0], a.ho[a.npair][1]);
+      }
+      a.sign[a.npair++] = 1 - 2 * p;
+    }
+    if (a.npair) launch_flux_plane<R>(a, partial, stream);
+  }
+  void flux_launch(const FluxPlane &fp, double *partial) {
+    int cE[2], cH[2];
+    flux_pairs(fp.d, cE, cH);
+    const int pairs[2][2] = {{cE[0], cH[0]}, {cE[1], cH[1]}};
+    dot_launch(fp, pairs, 2, partial);
+  }
+  // fields::integrate of the pairs over [mn, mx] on `grid` (a temporary plan and partial sums)
+  double dot_integral(const V3 &mn, const V3 &mx, int grid, const int (*pairs)[2], int np) {
+    FluxPlane fp;
+    plan_points(fp, grid, mn, mx);
+    DevBuf part;
+    part.alloc(FLUX_BLOCKS * sizeof(double));
+    HIPCHECK(hipMemsetAsync(part.p, 0, FLUX_BLOCKS * sizeof(double), stream));
+    dot_launch(fp, pairs, np, static_cast<double *>(part.p));
+    return flux_sum(part.p, 0);
+  }
+  double flux_sum(const void *dpartial, int slot) {
+    double h[FLUX_BLOCKS];
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipMemcpy(h, static_cast<const double *>(dpartial) + slot * FLUX_BLOCKS, sizeof h, hipMemcpyDeviceToHost));
+    long double sum = 0;
+    for (double v : h) sum += v;
+    return double(sum);
+  }
+  void *add_flux_vol(int d, V3 mn, V3 mx) override {
+    std::unique_ptr<FluxVol> fv(new FluxVol);
+    plan_flux(fv->plane, flux_direction(d, mn, mx), mn, mx);
+    fv->partial.alloc(2 * FLUX_BLOCKS * sizeof(double));
+    HIPCHECK(hipMemset(fv->partial.p, 0, 2 * FLUX_BLOCKS * sizeof(double)));
+    flux_vols.push_back(std::move(fv));
+    return flux_vols.back().get();
+  }
+  // flux_vol::flux (src/meep.hpp:2339): (the flux after E + the flux after H) / 2 of the last step
+  double flux_vol_value(void *p) override {
+    FluxVol *fv = static_cast<FluxVol *>(p);
+    if (!fv->stepped) return 0.0;
+    const double half = flux_sum(fv->partial.p, 0), full = flux_sum(fv->partial.p, 1);
+    return (full + half) * 0.5;
+  }
+  double flux_in_box(int d, V3 mn, V3 mx, bool sync) override {
+    int cE[2], cH[2];
+    flux_pairs(flux_direction(d, mn, mx), cE, cH);
+    const int pairs[2][2] = {{cE[0], cH[0]}, {cE[1], cH[1]}};
+    auto value = [&] { return dot_integral(mn, mx, 20, pairs, 2); };
+    return sync ? with_synchronized_magnetic(value) : value();
+  }
+  // fields::field_energy_in_box (src/energy_and_flux.cpp:54-59): the magnetic part with H
+  // synchronised; electric_energy_in_box / magnetic_energy_in_box (:84-95) as they stand
+  double energy_in_box(int which, V3 mn, V3 mx) override {
+    if (which == 1) return double(energy_sum(0, mn, mx));
+    if (which == 2) return double(energy_sum(5, mn, mx));
+    const double magnetic = with_synchronized_magnetic([&] { return double(energy_sum(5, mn, mx)); });
+    return double(energy_sum(0, mn, mx)) + magnetic;
   }
 
   // ---- reverse mode ----------------------------------------------------------------------
@@ -3488,6 +3640,8 @@ template <class R> struct Batch : BatchBase {
   // plan from the chunks' next step; returns the steps taken on the way (0 or 1)
   long plan() {
     long taken = 0;
+    for (Fields<R> *f : sims)
+      if (!f->flux_vols.empty()) fail("batch: time-domain flux planes (add_flux_vol) are not stepped in batches");
     for (int attempt = 0; attempt < 2; ++attempt) {
       // each chunk's source row sits at row_off[q] of the batch's row
       row_off.assign(sims.size(), 0);
@@ -4170,6 +4324,37 @@ int mp_dft_fields_chunk_component(mp_dft_fields *df, int i) {
 }
 int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out) {
   return mp_dft_flux_get_dft(reinterpret_cast<mp_dft_flux *>(df), 0, i, out);
+}
+
+struct FluxVolHandle {
+  FieldsBase *owner;
+  void *fv;
+};
+mp_flux_vol *mp_fields_add_flux_vol(mp_fields *f, int d, const double *vmin, const double *vmax) {
+  FieldsBase *F = reinterpret_cast<FieldsBase *>(f);
+  FluxVolHandle *h = new FluxVolHandle{F, nullptr};
+  if (guard([&] { h->fv = F->add_flux_vol(d, to_v3(vmin), to_v3(vmax)); })) {
+    delete h;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_flux_vol *>(h);
+}
+void mp_flux_vol_delete(mp_flux_vol *h) { delete reinterpret_cast<FluxVolHandle *>(h); }
+int mp_flux_vol_flux(mp_flux_vol *hp, double *out) {
+  FluxVolHandle *h = reinterpret_cast<FluxVolHandle *>(hp);
+  return guard([&] { *out = h->owner->flux_vol_value(h->fv); });
+}
+int mp_fields_flux_in_box(mp_fields *f, int d, const double *vmin, const double *vmax, double *out) {
+  return guard([&] { *out = reinterpret_cast<FieldsBase *>(f)->flux_in_box(d, to_v3(vmin), to_v3(vmax), true); });
+}
+int mp_fields_flux_in_box_wrongH(mp_fields *f, int d, const double *vmin, const double *vmax, double *out) {
+  return guard([&] { *out = reinterpret_cast<FieldsBase *>(f)->flux_in_box(d, to_v3(vmin), to_v3(vmax), false); });
+}
+int mp_fields_energy_in_box(mp_fields *f, int which, const double *vmin, const double *vmax, double *out) {
+  return guard([&] {
+    if (which < 0 || which > 2) fail("energy_in_box: which must be 0 (field), 1 (electric) or 2 (magnetic)");
+    *out = reinterpret_cast<FieldsBase *>(f)->energy_in_box(which, to_v3(vmin), to_v3(vmax));
+  });
 }
 
 static long dft_array(void *hp, bool both, int c, int nf, int *rank, size_t *dims, double *out) {

@@ -1546,6 +1546,38 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
   if (n > 0) hipLaunchKernelGGL(k_check<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, ptrs, idx, w, n, flag);
 }
 
+// a field's value at a centred point: its four Yee points summed in realnum, then halved twice
+// (the reference's 0.25 * (f + f + f + f), rounded back to realnum as its fvals are)
+template <class R> __device__ __forceinline__ double flux_avg(const R *p, int64_t i, int64_t o1, int64_t o2) {
+  if (!p) return 0.0;
+  const R s = p[i] + p[i + o1] + p[i + o2] + p[i + o1 + o2];
+  return double(R(0.25 * double(s)));
+}
+template <class R> __global__ __launch_bounds__(BLOCK) void k_flux_plane(FluxPlaneArgs<R> a, double *partial) {
+  __shared__ double red[BLOCK];
+  double acc = 0.0;
+  for (int j = blockIdx.x * BLOCK + threadIdx.x; j < a.n; j += gridDim.x * BLOCK) {
+    const int64_t i = a.idx[j];
+    double v = 0.0;
+    for (int p = 0; p < a.npair; ++p) {
+      const double er = flux_avg(a.e[p][0], i, a.eo[p][0], a.eo[p][1]), ei = flux_avg(a.e[p][1], i, a.eo[p][0], a.eo[p][1]);
+      const double hr = flux_avg(a.h[p][0], i, a.ho[p][0], a.ho[p][1]), hi = flux_avg(a.h[p][1], i, a.ho[p][0], a.ho[p][1]);
+      v += a.sign[p] * (er * hr + ei * hi);
+    }
+    acc += v * a.w[j];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = BLOCK / 2; h > 0; h >>= 1) {
+    if (int(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+template <class R> void launch_flux_plane(const FluxPlaneArgs<R> &a, double *partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_plane<R>, dim3(FLUX_BLOCKS), dim3(BLOCK), 0, s, a, partial);
+}
+
 // ---- batched launches -----------------------------------------------------------------
 // Logical blocks are dealt to XCDs in contiguous runs as in the single-chunk kernels (nblk is a
 // multiple of 8); each logical block belongs to one item.
@@ -1729,7 +1761,8 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
   template void launch_dft_snap<R>(const DftArgs<R> &, R *, R *, hipStream_t);                     \
   template void launch_dft_flush<R>(const DftArgs<R> &, const DftDefer<R> &, hipStream_t);         \
   template void launch_check<R>(const R *const *, const int64_t *, const double *, int, int *,     \
-                                hipStream_t);
+                                hipStream_t);                                                      \
+  template void launch_flux_plane<R>(const FluxPlaneArgs<R> &, double *, hipStream_t);
 // The build compiles this file once per field precision (MX_REAL) and part (MX_PART 1: the
 // single-chunk launchers, 2: the batched ones), in parallel.
 #ifdef MX_REAL

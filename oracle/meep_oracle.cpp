@@ -59,6 +59,10 @@
  *   MUT_DFTARR_SIGN   get_dft_array leaves a chunk's stored weight (the flux sign) in
  *   MUT_DFTARR_INTERP get_dft_array sums the planes around an off-grid monitor unweighted
  *   MUT_DFTARR_SHIFT  get_dft_array places a periodic image piece without its shift
+ *   MUT_FLUXT_SIGN    the time-domain flux adds its second E x H pair instead of subtracting it
+ *   MUT_FLUXT_CENTRE  integrals on the centred grid read each field at its own Yee point
+ *   MUT_FLUXT_HALF    a flux plane keeps the flux after the E update only, not the mean with
+ *                     the one after the H update
  */
 #include "../include/meep_amd.h"
 
@@ -1261,6 +1265,8 @@ template <class R> struct Fields {
     step_boundaries(PH_stuff);
     step_boundaries(H_stuff);
     sink(4);
+    for (auto &fv : flux_vols) // flux_vol::update_half (src/meep.hpp:2330-2333, src/step.cpp:93)
+      fv->cur_flux_half = flux_in_box_wrongH(fv->d, fv->wmin, fv->wmax);
     calc_sources(t * dt + 0.5 * dt);
     if (step_db(D_stuff)) connections_valid = false;
     step_source(D_stuff);
@@ -1277,6 +1283,12 @@ template <class R> struct Fields {
     step_boundaries(PE_stuff);
     step_boundaries(E_stuff);
     sink(4);
+    for (auto &fv : flux_vols) // flux_vol::update (src/meep.hpp:2334-2337, src/step.cpp:124)
+#ifdef MUT_FLUXT_HALF
+      fv->cur_flux = flux_in_box_wrongH(fv->d, fv->wmin, fv->wmax);
+#else
+      fv->cur_flux = (flux_in_box_wrongH(fv->d, fv->wmin, fv->wmax) + fv->cur_flux_half) * 0.5;
+#endif
     t += 1;
     update_dfts();
     sink(5);
@@ -1291,44 +1303,99 @@ template <class R> struct Fields {
   }
 
   // ---- field energy (src/energy_and_flux.cpp:45-178, src/integrate.cpp:46-205) ----
-  // 0.5 * integral of Re(conj(F) G) over the cell for F = E (H) and G = D (B), per component;
-  // E and D share a grid, so the integration grid is the component's own (no averaging).
-  long double energy_of(int first) {
-    vec3 wmin, wmax;
-    for (int d = 0; d < 3; ++d) wmax.t[d] = gv.has(d) ? gv.num[d] * gv.inva : 0.0;
+  // 0.5 * integral of Re(conj(F) G) over [wmin, wmax] for F = E (H) and G = D (B), per
+  // component (field_energy_in_box(c, where), :67-82); E and D share a grid, so the
+  // integration grid is the component's own.  Summed over components in long double (:84-95).
+  long double energy_in_box(int first, const vec3 &wmin, const vec3 &wmax) {
     long double sum = 0;
     for (int c = first; c < first + 5; ++c) {
       if (!has_field(c) || !f[c][0]) continue;
-      const int c2 = c + 10;
-      cdouble total = 0;
-      loop_in_chunks(wmin, wmax, c, [&](const ivec &is, const ivec &ie, const vec3 &s0, const vec3 &s1, const vec3 &e0,
-                                        const vec3 &e1, double dV0, const ivec &, cdouble sph) {
-        std::complex<long double> part = 0;
-        loop_ivecs(gv, is, ie, [&](ptrdiff_t idx, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
-          std::complex<R> fv[2];
-          for (int q = 0; q < 2; ++q) {
-            const int cc = q ? c2 : c;
-            double fr[2] = {0, 0};
-            for (int k = 0; k < 2; ++k)
-              if (f[cc][k]) {
-                const R a = f[cc][k][idx];
-                fr[k] = 0.25 * (a + a + a + a); // the four-point average with zero offsets
-              }
-            fv[q] = std::complex<R>(cdouble(fr[0], fr[1]) * sph);
-          }
-          const cdouble integrand = real(conj(cdouble(fv[0])) * cdouble(fv[1]));
-          part += std::complex<long double>(integrand * loop_weight(gv, L, i1, i2, i3, s0, s1, e0, e1, dV0));
-        });
-        total += cdouble(part);
-      });
-      sum += real(total) * 0.5;
+      sum += real(integrate_dot(c, c + 10, wmin, wmax)) * 0.5;
     }
     return sum;
   }
-  // fields::field_energy with synchronize_magnetic_fields / restore_magnetic_fields
-  // (src/energy_and_flux.cpp:46-62,146-178): B and H advanced half a step and averaged with
-  // their values at t - dt/2, the magnetic energy taken, every magnetic array restored.
-  double field_energy() {
+  long double energy_of(int first) {
+    vec3 wmin, wmax;
+    for (int d = 0; d < 3; ++d) wmax.t[d] = gv.has(d) ? gv.num[d] * gv.inva : 0.0;
+    return energy_in_box(first, wmin, wmax);
+  }
+  // fields::field_energy_in_box (src/energy_and_flux.cpp:54-59)
+  double field_energy_in_box(const vec3 &wmin, const vec3 &wmax) {
+    const double magnetic = with_synchronized_magnetic([&] { return double(energy_in_box(5, wmin, wmax)); });
+    return double(energy_in_box(0, wmin, wmax)) + magnetic;
+  }
+  // fields::integrate (src/integrate.cpp:46-120,122-180) of dot_integrand
+  // (src/energy_and_flux.cpp:61-65), Re(conj(F0) F1), over [wmin, wmax]: on the two
+  // components' own grid when they share it, else on the centred grid with each field the
+  // mean of its four Yee points (a missing component reads as zero); each piece of the volume
+  // summed in long double, the pieces in double.
+  cdouble integrate_dot(int c0, int c1, const vec3 &wmin, const vec3 &wmax) {
+    const ivec y0 = gv.iyee_shift(c0), y1 = gv.iyee_shift(c1);
+    bool same = true;
+    for (int d = 0; d < 3; ++d)
+      if (y0.t[d] != y1.t[d]) same = false;
+    const int cgrid = same ? c0 : 20;
+    const int cs[2] = {c0, c1};
+    cdouble total = 0;
+    loop_in_chunks(wmin, wmax, cgrid, [&](const ivec &is, const ivec &ie, const vec3 &s0, const vec3 &s1, const vec3 &e0,
+                                          const vec3 &e1, double dV0, const ivec &, cdouble sph) {
+      ptrdiff_t off[2][2] = {{0, 0}, {0, 0}};
+#ifndef MUT_FLUXT_CENTRE
+      if (cgrid == 20)
+        for (int q = 0; q < 2; ++q) gv.yee2cent_offsets(cs[q], off[q][0], off[q][1]);
+#endif
+      std::complex<long double> part = 0;
+      loop_ivecs(gv, is, ie, [&](ptrdiff_t idx, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
+        std::complex<R> fv[2];
+        for (int q = 0; q < 2; ++q) {
+          double fr[2] = {0, 0};
+          for (int k = 0; k < 2; ++k)
+            if (f[cs[q]][k]) {
+              const R *p = f[cs[q]][k];
+              fr[k] = 0.25 * (p[idx] + p[idx + off[q][0]] + p[idx + off[q][1]] + p[idx + off[q][0] + off[q][1]]);
+            }
+          fv[q] = std::complex<R>(cdouble(fr[0], fr[1]) * sph);
+        }
+        const cdouble integrand = real(conj(cdouble(fv[0])) * cdouble(fv[1]));
+        part += std::complex<long double>(integrand * loop_weight(gv, L, i1, i2, i3, s0, s1, e0, e1, dV0));
+      });
+      total += cdouble(part);
+    });
+    return total;
+  }
+  // fields::flux_in_box_wrongH (src/energy_and_flux.cpp:186-221): the flux of E x H through
+  // [wmin, wmax] along d with E and H as they stand (H half a step behind E)
+  double flux_in_box_wrongH(int d, const vec3 &wmin, const vec3 &wmax) {
+    int cE[2], cH[2];
+    if (d == X)
+      cE[0] = 1, cE[1] = 4, cH[0] = 9, cH[1] = 6;
+    else if (d == Y)
+      cE[0] = 4, cE[1] = 0, cH[0] = 5, cH[1] = 9;
+    else
+      cE[0] = 0, cE[1] = 1, cH[0] = 6, cH[1] = 5;
+    long double sum = 0;
+#ifdef MUT_FLUXT_SIGN
+    for (int i = 0; i < 2; ++i) sum += real(integrate_dot(cE[i], cH[i], wmin, wmax));
+#else
+    for (int i = 0; i < 2; ++i) sum += real(integrate_dot(cE[i], cH[i], wmin, wmax)) * (1 - 2 * i);
+#endif
+    return double(sum);
+  }
+  // fields::flux_in_box (src/energy_and_flux.cpp:223-228): the same with H synchronised
+  double flux_in_box(int d, const vec3 &wmin, const vec3 &wmax) {
+    return with_synchronized_magnetic([&] { return flux_in_box_wrongH(d, wmin, wmax); });
+  }
+  struct FluxVol { // flux_vol (src/meep.hpp:2322-2352)
+    int d;
+    vec3 wmin, wmax;
+    double cur_flux = 0, cur_flux_half = 0;
+  };
+  std::vector<std::unique_ptr<FluxVol>> flux_vols;
+
+  // synchronize_magnetic_fields / restore_magnetic_fields (src/energy_and_flux.cpp:146-178)
+  // around fn: B and H advanced half a step and averaged with their values at t - dt/2, fn
+  // evaluated, every magnetic array (and its uPML and conductivity auxiliaries) restored.
+  template <class FN> double with_synchronized_magnetic(FN fn) {
     std::vector<std::pair<R *, std::vector<R>>> saved, saved_aux;
     auto save = [&](std::vector<std::pair<R *, std::vector<R>>> &into, R *p) {
       if (p) into.push_back({p, std::vector<R>(p, p + gv.ntot)});
@@ -1359,9 +1426,15 @@ template <class R> struct Fields {
       R *fc = sv.first;
       for (size_t i = 0; i < gv.ntot; ++i) fc[i] = 0.5 * (fc[i] + sv.second[i]);
     }
-    const double magnetic = double(energy_of(5));
+    const double value = fn();
     for (auto *lst : {&saved, &saved_aux})
       for (auto &sv : *lst) std::copy(sv.second.begin(), sv.second.end(), sv.first);
+    return value;
+  }
+  // fields::field_energy (src/energy_and_flux.cpp:46-62): the magnetic part with H
+  // synchronised, the electric part as it stands
+  double field_energy() {
+    const double magnetic = with_synchronized_magnetic([&] { return double(energy_of(5)); });
     return double(energy_of(0)) + magnetic;
   }
 
@@ -2348,6 +2421,87 @@ static long dft_array(AnyFlux *fl, bool flux, int c, int num_freq, int *rank, si
 }
 long mp_dft_flux_get_dft_array(mp_dft_flux *flp, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(reinterpret_cast<AnyFlux *>(flp), true, c, num_freq, rank, dims, out);
+}
+
+// ---- time-domain flux (src/energy_and_flux.cpp:180-241, src/meep.hpp:2322-2352) ----------
+struct AnyFluxVol {
+  AnyFields *owner;
+  void *fv;
+};
+static int flux_direction(const GV &gv, int d, const vec3 &mn, const vec3 &mx) {
+  if (d >= 0) { // coordinate_mismatch rejects only the cylindrical directions here
+    if (d > Z) fail("invalid direction in add_flux_vol");
+    return d;
+  }
+  // volume::normal_direction (src/vec.cpp:227-257): Z in 1D, else the one direction of zero
+  // extent when the others have positive extent
+  if (gv.dim == 1) return Z;
+  int nd = -1, nzero = 0, npos = 0;
+  for (int k = 0; k < 3; ++k)
+    if (gv.has(k)) {
+      if (mx.t[k] - mn.t[k] == 0) nd = k, ++nzero;
+      else if (mx.t[k] - mn.t[k] > 0) ++npos;
+    }
+  if (nzero != 1 || npos != gv.dim - 1) fail("invalid direction in add_flux_vol");
+  return nd;
+}
+mp_flux_vol *mp_fields_add_flux_vol(mp_fields *f, int d, const double *vmin, const double *vmax) {
+  AnyFields *af = reinterpret_cast<AnyFields *>(f);
+  AnyFluxVol *h = new AnyFluxVol{af, nullptr};
+  int rc = guard([&] {
+    with_fields(af, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(F)>::type FT;
+      std::unique_ptr<typename FT::FluxVol> fv(new typename FT::FluxVol);
+      fv->wmin = to_vec(vmin);
+      fv->wmax = to_vec(vmax);
+      fv->d = flux_direction(F.gv, d, fv->wmin, fv->wmax);
+      h->fv = fv.get();
+      F.flux_vols.push_back(std::move(fv));
+      return 0;
+    });
+  });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_flux_vol *>(h);
+}
+void mp_flux_vol_delete(mp_flux_vol *h) { delete reinterpret_cast<AnyFluxVol *>(h); }
+int mp_flux_vol_flux(mp_flux_vol *hp, double *out) {
+  AnyFluxVol *h = reinterpret_cast<AnyFluxVol *>(hp);
+  return guard([&] {
+    with_fields(h->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(F)>::type FT;
+      *out = static_cast<typename FT::FluxVol *>(h->fv)->cur_flux;
+      return 0;
+    });
+  });
+}
+static int flux_in_box(mp_fields *f, int d, const double *vmin, const double *vmax, bool sync, double *out) {
+  return guard([&] {
+    with_fields(reinterpret_cast<AnyFields *>(f), [&](auto &F) {
+      const vec3 mn = to_vec(vmin), mx = to_vec(vmax);
+      const int dd = flux_direction(F.gv, d, mn, mx);
+      *out = sync ? F.flux_in_box(dd, mn, mx) : F.flux_in_box_wrongH(dd, mn, mx);
+      return 0;
+    });
+  });
+}
+int mp_fields_flux_in_box(mp_fields *f, int d, const double *vmin, const double *vmax, double *out) {
+  return flux_in_box(f, d, vmin, vmax, true, out);
+}
+int mp_fields_flux_in_box_wrongH(mp_fields *f, int d, const double *vmin, const double *vmax, double *out) {
+  return flux_in_box(f, d, vmin, vmax, false, out);
+}
+int mp_fields_energy_in_box(mp_fields *f, int which, const double *vmin, const double *vmax, double *out) {
+  return guard([&] {
+    with_fields(reinterpret_cast<AnyFields *>(f), [&](auto &F) {
+      if (which < 0 || which > 2) fail("energy_in_box: which must be 0 (field), 1 (electric) or 2 (magnetic)");
+      const vec3 mn = to_vec(vmin), mx = to_vec(vmax);
+      *out = which == 0 ? F.field_energy_in_box(mn, mx) : double(F.energy_in_box(which == 1 ? 0 : 5, mn, mx));
+      return 0;
+    });
+  });
 }
 
 // fields::add_dft_fields (src/dft.cpp:913-927): one chunk list over all components, no

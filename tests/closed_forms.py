@@ -35,6 +35,15 @@ run against a value the mathematics fixes; the tests assert on these errors.
   weights of the empty direction, then the sum over it), and from a monitor that crosses the
   periodic boundary every point of both pieces holds the field (each piece placed by its
   shift).
+* ``flux_time_domain``: the instantaneous flux and energy integrals (flux_in_box_wrongH,
+  flux_in_box, flux_vol, the energy in a box) on fields set by hand in a 2D cell.  The
+  integrals are sums over loop_in_chunks points with weights that add up to the volume's
+  length or area, so a uniform field gives field x extent; a field linear in x, read on the
+  centred grid as the mean of its two Yee neighbours, gives its value at the centred line.
+  Under uniform electric and magnetic conductivities the curl-free fields of a periodic cell
+  decay by fixed factors per step (``conductive_decay``), so after one step the flux plane's
+  mean of the fluxes after the H and the E updates, the half step that synchronises H, and
+  the synchronised magnetic energy all have closed forms.
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -47,8 +56,8 @@ import math
 
 import numpy as np
 
-from meep_amd.capi import (ARR_F, ARR_P, By, Dx, Ex, Ez, Hx, Hy, High, Low, Magnetic, Metallic, X, Y, Z,
-                           continuous_src_time, fields, gaussian_src_time, structure)
+from meep_amd.capi import (ARR_F, ARR_P, Bx, By, Dx, Dz, Ex, Ez, Hx, Hy, High, Low, Magnetic, Metallic, X, Y,
+                           Z, continuous_src_time, fields, gaussian_src_time, structure)
 
 S = 0.5  # Courant number of every check
 
@@ -212,6 +221,59 @@ def dft_array_plane_wave(lib, a=10, freqs=None):
         w = wrap.get_dft_array(Ez, nf)
         assert w.ndim == 1 and w.size >= int(0.7 * a), w.shape
         err["E_wrap"] = max(err["E_wrap"], float(np.abs(w - e[nf]).max() / abs(e[nf])))
+    return err
+
+
+def flux_time_domain(lib, a=10):
+    """Errors of the time-domain flux and energy integrals against their closed forms (see the
+    module docstring)."""
+    err = {}
+    # a field linear in x, read on the centred line x0 through a full y period, never stepped
+    s = structure(lib, dim=2, size=(2.0, 1.0), a=a)
+    f = fields(s)
+    f.use_real_fields()
+    f.use_bloch((0.0, 0.0))
+    f.add_volume_source(Ez, continuous_src_time(1.0), (0.55, 0.55, 0), (0.55, 0.55, 0), amp=0.0)
+    nx, ny = s.shape
+    h0 = 0.7
+    f.set_array(Ez, 0, ARR_F, np.repeat(np.arange(nx)[:, None] / a, ny, axis=1))
+    f.set_array(Hy, 0, ARR_F, np.full(s.shape, h0))
+    f.set_array(By, 0, ARR_F, np.full(s.shape, h0))
+    x0 = 12.5 / a
+    got = f.flux_in_box_wrongH(X, (x0, 0, 0), (x0, 1.0, 0))
+    err["linear"] = abs(got - (-x0 * h0 * 1.0)) / (x0 * h0)
+    # uniform fields decaying under conductivity in a periodic cell with eps = 2
+    eps, se, sm, e0, g0 = 2.0, 1.2, 0.8, 0.9, -0.4
+    s = structure(lib, dim=2, size=(1.0, 1.0), a=a, eps=lambda x, y, z: eps)
+    s.set_conductivity(Ez, lambda x, y, z: se)
+    s.set_conductivity(Hy, lambda x, y, z: sm)
+    f = fields(s)
+    f.use_real_fields()
+    f.use_bloch((0.0, 0.0))
+    f.add_volume_source(Ez, continuous_src_time(1.0), (0.55, 0.55, 0), (0.55, 0.55, 0), amp=0.0)
+    one = np.ones(s.shape)
+    for c, v in ((Ez, e0), (Dz, eps * e0), (Hx, g0), (Bx, g0), (Hy, h0), (By, h0)):
+        f.set_array(c, 0, ARR_F, v * one)
+    box = ((0.2, 0.3, 0), (0.7, 0.9, 0))
+    area = 0.5 * 0.6
+    err["energy_e"] = abs(f.electric_energy_in_box(*box) - 0.5 * e0 * eps * e0 * area) / (e0 * e0 * area)
+    err["energy_h"] = abs(f.magnetic_energy_in_box(*box) - 0.5 * (g0 * g0 + h0 * h0) * area) / (h0 * h0 * area)
+    err["flux_y"] = abs(f.flux_in_box_wrongH(Y, (0.2, 0.45, 0), (0.7, 0.45, 0)) - e0 * g0 * 0.5) / abs(e0 * g0 * 0.5)
+    plane = f.add_flux_plane((x0 - 1.0, 0, 0), (x0 - 1.0, 1.0, 0))
+    f.step()
+    dt = s.dt
+    def rate(c):
+        return (1 - c * dt / 2) / (1 + c * dt / 2)
+    r_e, r_m = rate(se / eps), rate(sm)
+    ref = e0 * h0
+    # after the step: E at t + 1 (r_e e0), H at t + 1/2 (r_m h0); the plane averaged the flux
+    # after the H update (E still e0) with the one after the E update
+    err["flux_vol"] = abs(plane.flux() - (-e0 * r_m * h0 * (1 + r_e) / 2)) / ref
+    err["wrongH"] = abs(f.flux_in_box_wrongH(X, (x0 - 1.0, 0, 0), (x0 - 1.0, 1.0, 0)) - (-r_e * e0 * r_m * h0)) / ref
+    h_sync = r_m * h0 * (1 + r_m) / 2  # H half a step on, averaged with H now
+    err["sync"] = abs(f.flux_in_box(X, (x0 - 1.0, 0, 0), (x0 - 1.0, 1.0, 0)) - (-r_e * e0 * h_sync)) / ref
+    e_exp = 0.5 * eps * (r_e * e0) ** 2 * area + 0.5 * (g0 * g0 + h_sync * h_sync) * area
+    err["energy_sync"] = abs(f.field_energy_in_box(*box) - e_exp) / e_exp
     return err
 
 

@@ -167,3 +167,51 @@ def test_cpp_api_get_dft_array(tmp_path, oracle_lib):
     want = fl.get_dft_array(cEz, 1)
     assert want.shape == (n,)
     assert np.abs(vals - want).max() <= 1e-15 * np.abs(want).max()
+
+
+FLUX_1D_PROGRAM = r"""
+#include "meep_amd.hpp"
+#include <cmath>
+#include <cstdio>
+using namespace meep;
+double bump(const vec &pt) { return (std::fabs(pt.z() - 50.0) > 20.0) ? 1.0 : 12.0; }
+int main() {
+  const double zmax = 100.0;
+  grid_volume gv = volone(zmax, 10.0);
+  structure s(gv, bump, pml(zmax / 6));
+  fields f(&s);
+  f.use_real_fields();
+  f.add_point_source(Ex, 0.25, 3.5, 0.0, 8.0, vec(zmax / 6 + 0.3), 1.0);
+  flux_vol *left = f.add_flux_plane(vec(zmax / 3.0), vec(zmax / 3.0));
+  flux_vol *right = f.add_flux_plane(vec(zmax * 2.0 / 3.0), vec(zmax * 2.0 / 3.0));
+  const double ttot = std::fmin(10.0 + 1e5 / zmax, f.last_source_time());
+  f.step();
+  volume mid(vec(zmax / 3), vec(2 * zmax / 3));
+  double delta_energy = f.field_energy_in_box(mid);
+  double flux_left = 0.0, flux_right = 0.0;
+  while (f.time() < ttot) {
+    f.step();
+    flux_left += f.dt * left->flux();
+    flux_right += f.dt * right->flux();
+  }
+  delta_energy -= f.field_energy_in_box(mid);
+  std::printf("%.17g %.17g\n", flux_left, flux_right - delta_energy);
+  return 0;
+}
+"""
+
+
+def test_cpp_api_flux_planes(tmp_path, oracle_lib):
+    """The reference's flux_1d case (tests/flux.cpp:57-90) written against the C++ API's
+    flux_vol, add_flux_plane and field_energy_in_box gives the C-ABI's numbers."""
+    import reference_flux_cases as RF
+    src = tmp_path / "flux_1d.cpp"
+    src.write_text(FLUX_1D_PROGRAM)
+    out = str(tmp_path / "flux_1d")
+    mbuild.build_cpp_program(str(src), out, os.path.join(ROOT, "oracle", "_build"), "meep_oracle")
+    r = subprocess.run([out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = [float(x) for x in r.stdout.split()]
+    want = RF.flux_1d(oracle_lib, 100.0, 20.0)
+    assert got == [want[0], want[1]], (got, want)
+

@@ -1,7 +1,7 @@
 """The closed forms of tests/closed_forms.py on the MI355X engine in FP64, at the tolerances
 the oracle meets: the engine's DFT scale, phase and E/H time offset, current injection,
-flux weights, Lorentz and Drude poles, magnetic/metallic walls, conductivity and the DFT
-arrays of get_dft_array are held to
+flux weights, Lorentz and Drude poles, magnetic/metallic walls, conductivity, the DFT
+arrays of get_dft_array and the time-domain flux and energy integrals are held to
 the mathematics directly, not only to the oracle."""
 
 import pytest
@@ -46,3 +46,7 @@ def test_upml_branch_is_local_to_its_region_gpu(gpu_lib):
 
 def test_dft_array_of_a_plane_wave_gpu(gpu_lib):
     T.check_dft_array(gpu_lib)
+
+
+def test_time_domain_flux_and_energy_integrals_gpu(gpu_lib):
+    T.check_flux_time_domain(gpu_lib)

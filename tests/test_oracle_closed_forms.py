@@ -27,6 +27,10 @@ values do not reach (tests/closed_forms.py states each closed form):
   linear interpolation (interpolation weights of the empty direction, summed out), and a
   monitor across the periodic boundary holds the field in both of its pieces.  Observed
   about 1e-8.
+* Time-domain flux and energy integrals on fields set by hand: a linear field read on the
+  centred grid, uniform fields over a box or line, and uniform fields decaying under
+  conductivity through one step (the flux plane's mean of its two fluxes, the synchronising
+  half step of H).  Observed about 1e-16.
 * Transport and spectrum shape: equal flux through two planes of a lossless 1D line, and a
   flux spectrum proportional to |J(omega)|^2.  These two are scale-invariant: they do not
   pin the DFT's normalisation, phase or E/H time offset (the first three checks do).
@@ -46,7 +50,7 @@ from meep_amd.capi import (Ex, Ez, MeepError, Z, fields, gaussian_src_time, lins
 
 TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
        "pole_decay": 1e-4, "cavity": 1e-10, "cond_decay": 1e-12, "cond_pml_line": 1e-6,
-       "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7}
+       "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12}
 
 
 def check_line_source(lib):
@@ -101,6 +105,11 @@ def check_dft_array(lib):
     assert max(err.values()) <= TOL["dft_array"], err
 
 
+def check_flux_time_domain(lib):
+    err = CF.flux_time_domain(lib)
+    assert max(err.values()) <= TOL["flux_time"], err
+
+
 def test_line_source_matches_lattice_solution(oracle_lib):
     check_line_source(oracle_lib)
 
@@ -147,6 +156,10 @@ def test_upml_branch_is_local_to_its_region(oracle_lib):
 
 def test_dft_array_of_a_plane_wave(oracle_lib):
     check_dft_array(oracle_lib)
+
+
+def test_time_domain_flux_and_energy_integrals(oracle_lib):
+    check_flux_time_domain(oracle_lib)
 
 
 def _cavity_energy_spread(lib, a):

@@ -9,7 +9,8 @@ not scaled by chi1inv, a current in a conductive medium not scaled by condinv, t
 uPML branch without its (kap - sig) factor, a uPML branch taken over the whole chunk instead
 of its region, and a DFT array that keeps the integration weights or the stored weight, sums
 the planes around an off-grid monitor unweighted, or places a periodic piece without its
-shift."""
+shift; a time-domain flux that adds its second pair, reads fields off the centred grid, or
+keeps only the flux after the E update."""
 
 import ctypes
 import os
@@ -39,6 +40,9 @@ CHECKS = {
     "DFTARR_SIGN": T.check_dft_array,
     "DFTARR_INTERP": T.check_dft_array,
     "DFTARR_SHIFT": T.check_dft_array,
+    "FLUXT_SIGN": T.check_flux_time_domain,
+    "FLUXT_CENTRE": T.check_flux_time_domain,
+    "FLUXT_HALF": T.check_flux_time_domain,
 }
 
 
