@@ -89,6 +89,25 @@ mp_structure *mp_structure_new(int dim, double xsize, double ysize, double zsize
                                const int *pml_direction, const int *pml_side,
                                const double *pml_thickness, const double *pml_R,
                                double courant, int single_precision);
+/* mp_structure_new with every parameter of a uPML term, boundary_region(PML, thickness,
+   Rasymptotic, mean_stretch, pml_profile, pml_profile_data, pml_profile_integral,
+   pml_profile_integral_u, d, side) — src/meep.hpp:665-672,732-736, structure_chunk::use_pml
+   src/structure.cpp:630-691: sigma = -ln(R) / (4 d integral) * profile(x/d) and kappa =
+   1 + (mean_stretch - 1) / integral_u * profile(x/d) * x/d at depth x into the term's layer.
+   The profile is called with r[0] = u in [0, 1]; integral and integral_u are its integrals of
+   profile(u) and u profile(u) over [0, 1] (the reference's Python PML passes 1/3 and 1/4
+   whatever the profile, python/simulation.py:4985-5000).  Each array may be NULL, and an
+   entry of profile NULL or of an integral <= 0, for the defaults: mean stretch 1, profile u^2,
+   integrals 1/3 and 1/4. */
+mp_structure *mp_structure_new_pml(int dim, double xsize, double ysize, double zsize, double a,
+                                   mp_scalar_func eps, void *eps_user, int n_pml,
+                                   const int *pml_direction, const int *pml_side,
+                                   const double *pml_thickness, const double *pml_R,
+                                   const double *pml_mean_stretch,
+                                   const mp_scalar_func *pml_profile,
+                                   void *const *pml_profile_user, const double *pml_integral,
+                                   const double *pml_integral_u, double courant,
+                                   int single_precision);
 void mp_structure_delete(mp_structure *s);
 /* grid_volume::ntot (src/meep/vec.hpp:1056) and num_direction (src/meep/vec.hpp:1028). */
 size_t mp_structure_ntot(const mp_structure *s);

@@ -156,30 +156,51 @@ private:
 };
 
 // boundary_region / pml() (src/meep.hpp:660-736, src/structure.cpp:252-273)
+typedef double (*pml_profile_func)(double u, void *func_data);
+inline double pml_quadratic_profile(double u, void *) { return u * u; }
 class boundary_region {
 public:
+  enum boundary_region_kind { NOTHING_SPECIAL, PML };
   struct term {
     int d, side;
-    double thickness, R;
+    double thickness, R, mean_stretch;
+    pml_profile_func profile;
+    void *profile_data;
+    double integral, integral_u;
   };
   std::vector<term> terms;
+  boundary_region() {}
+  boundary_region(boundary_region_kind kind, double thickness, double Rasymptotic, double mean_stretch,
+                  pml_profile_func pml_profile, void *pml_profile_data, double pml_profile_integral,
+                  double pml_profile_integral_u, direction d, boundary_side side) {
+    if (kind == PML)
+      terms.push_back({int(d), int(side), thickness, Rasymptotic, mean_stretch, pml_profile, pml_profile_data,
+                       pml_profile_integral, pml_profile_integral_u});
+  }
   boundary_region operator+(const boundary_region &o) const {
     boundary_region r(*this);
     r.terms.insert(r.terms.end(), o.terms.begin(), o.terms.end());
     return r;
   }
 };
-inline boundary_region pml(double thickness, direction d, boundary_side side, double Rasymptotic = 1e-15) {
-  boundary_region r;
-  r.terms.push_back({int(d), int(side), thickness, Rasymptotic});
-  return r;
+inline boundary_region pml(double thickness, direction d, boundary_side side, double Rasymptotic = 1e-15,
+                           double mean_stretch = 1.0) {
+  return boundary_region(boundary_region::PML, thickness, Rasymptotic, mean_stretch, pml_quadratic_profile, nullptr,
+                         1. / 3., 1. / 4., d, side);
 }
-inline boundary_region pml(double thickness, direction d, double Rasymptotic = 1e-15) {
-  return pml(thickness, d, Low, Rasymptotic) + pml(thickness, d, High, Rasymptotic);
+inline boundary_region pml(double thickness, direction d, double Rasymptotic = 1e-15, double mean_stretch = 1.0) {
+  return pml(thickness, d, Low, Rasymptotic, mean_stretch) + pml(thickness, d, High, Rasymptotic, mean_stretch);
 }
-inline boundary_region pml(double thickness, double Rasymptotic = 1e-15) {
-  return pml(thickness, X, Rasymptotic) + pml(thickness, Y, Rasymptotic) + pml(thickness, Z, Rasymptotic);
+inline boundary_region pml(double thickness, double Rasymptotic = 1e-15, double mean_stretch = 1.0) {
+  return pml(thickness, X, Rasymptotic, mean_stretch) + pml(thickness, Y, Rasymptotic, mean_stretch) +
+         pml(thickness, Z, Rasymptotic, mean_stretch);
 }
+namespace detail {
+inline double profile_trampoline(const double *r, void *user) {
+  const boundary_region::term *t = static_cast<const boundary_region::term *>(user);
+  return t->profile(r[0], t->profile_data);
+}
+} // namespace detail
 
 // lorentzian_susceptibility (src/meep.hpp:246-280)
 class lorentzian_susceptibility {
@@ -270,17 +291,25 @@ public:
     if (!sym.is_identity) meep::abort("only the identity symmetry is supported by the GPU engine");
     if (use_anisotropic_averaging) meep::abort("subpixel averaging is not supported by the GPU engine");
     std::vector<int> dirs, sides;
-    std::vector<double> th, R;
+    std::vector<double> th, R, stretch, integ, integ_u;
+    std::vector<mp_scalar_func> prof;
+    std::vector<void *> user;
     for (const auto &t : br.terms) {
       dirs.push_back(t.d);
       sides.push_back(t.side);
       th.push_back(t.thickness);
       R.push_back(t.R);
+      stretch.push_back(t.mean_stretch);
+      prof.push_back(t.profile ? detail::profile_trampoline : nullptr);
+      user.push_back(const_cast<boundary_region::term *>(&t));
+      integ.push_back(t.integral);
+      integ_u.push_back(t.integral_u);
     }
     int dim = gv.dim == D1 ? 1 : gv.dim == D2 ? 2 : 3;
-    p = mp_structure_new(dim, gv.size[0], gv.size[1], gv.size[2], gv.a, eps ? detail::eps_trampoline : nullptr,
-                         reinterpret_cast<void *>(eps), int(dirs.size()), dirs.data(), sides.data(), th.data(), R.data(),
-                         Courant, realnum_is_float() ? 1 : 0);
+    p = mp_structure_new_pml(dim, gv.size[0], gv.size[1], gv.size[2], gv.a, eps ? detail::eps_trampoline : nullptr,
+                             reinterpret_cast<void *>(eps), int(dirs.size()), dirs.data(), sides.data(), th.data(),
+                             R.data(), stretch.data(), prof.data(), user.data(), integ.data(), integ_u.data(), Courant,
+                             realnum_is_float() ? 1 : 0);
     if (!p) throw std::runtime_error(mp_last_error());
   }
   ~structure() { mp_structure_delete(p); }

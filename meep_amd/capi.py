@@ -54,6 +54,11 @@ _SIGNATURES = {
                                       SCALAR_FUNC, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.c_double, C.c_int]),
+    "mp_structure_new_pml": (C.c_void_p, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, SCALAR_FUNC,
+                                          C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(SCALAR_FUNC), C.POINTER(C.c_void_p), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.c_double, C.c_int]),
     "mp_structure_delete": (None, [C.c_void_p]),
     "mp_structure_ntot": (C.c_size_t, [C.c_void_p]),
     "mp_structure_num": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
@@ -186,13 +191,26 @@ class structure:
         eps_cb = SCALAR_FUNC(0) if eps is None else SCALAR_FUNC(
             lambda r, _u: float(eps(r[0], r[1], r[2])))
         self._keep.append(eps_cb)
+        # a term is (direction, side, thickness[, R[, mean_stretch[, profile[, integral,
+        # integral_u]]]]): boundary_region's uPML parameters (src/meep.hpp:665-672); profile
+        # is a function of u in [0, 1] (None: u^2), the integrals default to 1/3 and 1/4
         n = len(pml)
         dirs = (C.c_int * max(n, 1))(*[p[0] for p in pml])
         sides = (C.c_int * max(n, 1))(*[p[1] for p in pml])
         thick = (C.c_double * max(n, 1))(*[p[2] for p in pml])
         R = (C.c_double * max(n, 1))(*[(p[3] if len(p) > 3 else 1e-15) for p in pml])
-        self.ptr = lib.mp_structure_new(dim, size[0], size[1], size[2], a, eps_cb, None, n,
-                                        dirs, sides, thick, R, courant, int(single))
+        stretch = (C.c_double * max(n, 1))(*[(p[4] if len(p) > 4 else 1.0) for p in pml])
+        profs = (SCALAR_FUNC * max(n, 1))()
+        for i, p in enumerate(pml):
+            if len(p) > 5 and p[5] is not None:
+                cb = SCALAR_FUNC(lambda r, _u, fn=p[5]: float(fn(r[0])))
+                self._keep.append(cb)
+                profs[i] = cb
+        integ = (C.c_double * max(n, 1))(*[(p[6] if len(p) > 6 else 0.0) for p in pml])
+        integ_u = (C.c_double * max(n, 1))(*[(p[7] if len(p) > 7 else 0.0) for p in pml])
+        self.ptr = lib.mp_structure_new_pml(dim, size[0], size[1], size[2], a, eps_cb, None, n,
+                                            dirs, sides, thick, R, stretch, profs, None, integ,
+                                            integ_u, courant, int(single))
         if not self.ptr:
             raise MeepError(lib.mp_last_error().decode())
         self.dim = dim
@@ -666,8 +684,8 @@ def vol3d(xsize, ysize, zsize, a):
     return dict(dim=3, size=(xsize, ysize, zsize), a=a)
 
 
-def pml(thickness, direction=None, side=None, R=1e-15):
+def pml(thickness, direction=None, side=None, R=1e-15, mean_stretch=1.0):
     """boundary_region pml(...) terms (src/structure.cpp:252-273)."""
     dirs = [X, Y, Z] if direction is None else [direction]
     sides = [Low, High] if side is None else [side]
-    return [(d, s, thickness, R) for d in dirs for s in sides]
+    return [(d, s, thickness, R, mean_stretch) for d in dirs for s in sides]

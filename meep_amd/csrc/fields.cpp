@@ -163,11 +163,25 @@ struct HostStructure {
 };
 
 // structure_chunk::use_pml (src/structure.cpp:625-691) with the quadratic profile of pml()
-static void host_use_pml(HostStructure &s, int d, double dx, int side, double Rasym) {
+// a uPML term's mean stretch and profile (boundary_region, src/meep.hpp:665-672); the default
+// profile is u^2 (pml_quadratic_profile, src/structure.cpp:252-255)
+struct PmlShape {
+  double mean_stretch = 1.0;
+  mp_scalar_func profile = nullptr;
+  void *user = nullptr;
+  double integral = 1. / 3., integral_u = 1. / 4.;
+  double at(double u) const {
+    if (!profile) return u * u;
+    const double r[3] = {u, 0.0, 0.0};
+    return profile(r, user);
+  }
+};
+static void host_use_pml(HostStructure &s, int d, double dx, int side, double Rasym, const PmlShape &sh) {
   if (dx <= 0.0) return;
   const Geom &gv = s.gv;
   const double a = gv.a;
-  const double prefac = (-std::log(Rasym)) / (4 * dx * (1. / 3.));
+  const double prefac = (-std::log(Rasym)) / (4 * dx * sh.integral);
+  const double kprefac = (sh.mean_stretch - 1) / sh.integral_u;
   const double bloc = side == High ? gv.num[d] * gv.inva : 0.0;
   auto pml_x = [&](int i) {
     double here = i * 0.5 / a;
@@ -185,9 +199,9 @@ static void host_use_pml(HostStructure &s, int d, double dx, int side, double Ra
   for (int i = 0; i <= 2 * gv.num[d] + 1; ++i) {
     double x = pml_x(i);
     if (x > 0) {
-      double sp = (x / dx) * (x / dx);
+      const double sp = sh.at(x / dx);
       s.sig[d][i] = 0.5 * s.dt * prefac * sp;
-      s.kap[d][i] = 1.0; // 1 + (mean_stretch - 1) / (1/4) * sp * (x / dx), mean_stretch = 1
+      s.kap[d][i] = 1 + kprefac * sp * (x / dx);
     }
   }
 }
@@ -3888,6 +3902,13 @@ const char *mp_engine_name(void) { return "hip-gfx950"; }
 
 mp_structure *mp_structure_new(int dim, double xs, double ys, double zs, double a, mp_scalar_func eps, void *eu, int n_pml,
                                const int *pd, const int *ps, const double *pt, const double *pR, double courant, int single) {
+  return mp_structure_new_pml(dim, xs, ys, zs, a, eps, eu, n_pml, pd, ps, pt, pR, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              courant, single);
+}
+mp_structure *mp_structure_new_pml(int dim, double xs, double ys, double zs, double a, mp_scalar_func eps, void *eu, int n_pml,
+                                   const int *pd, const int *ps, const double *pt, const double *pR, const double *pstretch,
+                                   const mp_scalar_func *pprof, void *const *puser, const double *pint, const double *pint_u,
+                                   double courant, int single) {
   HostStructure *s = new HostStructure;
   int rc = guard([&] {
     auto np = [&](double sz) { return sz == 0 ? 1 : int(sz * a + 0.5); };
@@ -3906,7 +3927,13 @@ mp_structure *mp_structure_new(int dim, double xs, double ys, double zs, double 
     for (int i = 0; i < n_pml; ++i) {
       const int d = pd[i];
       if (d < 0 || d > 2 || !gv.has(d) || gv.num[d] <= 1) continue;
-      host_use_pml(*s, d, pt[i], ps[i], pR && pR[i] > 0 ? pR[i] : 1e-15);
+      PmlShape sh;
+      if (pstretch) sh.mean_stretch = pstretch[i];
+      if (pprof) sh.profile = pprof[i];
+      if (puser) sh.user = puser[i];
+      if (pint && pint[i] > 0) sh.integral = pint[i];
+      if (pint_u && pint_u[i] > 0) sh.integral_u = pint_u[i];
+      host_use_pml(*s, d, pt[i], ps[i], pR && pR[i] > 0 ? pR[i] : 1e-15, sh);
     }
     if (eps) // set_chi1inv without subpixel averaging (src/anisotropic_averaging.cpp:90-98)
       for (int c = 0; c < 5; ++c) {

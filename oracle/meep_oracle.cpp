@@ -63,6 +63,8 @@
  *   MUT_FLUXT_CENTRE  integrals on the centred grid read each field at its own Yee point
  *   MUT_FLUXT_HALF    a flux plane keeps the flux after the E update only, not the mean with
  *                     the one after the H update
+ *   MUT_PML_INTEGRAL  a uPML term's sigma ignores its profile's integral (takes 1/3)
+ *   MUT_PML_STRETCH   a uPML term's kappa divides by the profile's integral, not that of u profile(u)
  */
 #include "../include/meep_amd.h"
 
@@ -536,9 +538,28 @@ inline double pml_x(int i, double dx, double bloc, double a) {
 // structure_chunk::use_pml (src/structure.cpp:630-691).  The reference splits each PML slab
 // into its own chunk so no chunk sees both sides of one direction; one chunk here holds
 // both sides, so each side's profile is written into the same array.
-template <class R> void use_pml(Structure<R> &s, int d, double dx, double bloc, double Rasym) {
+// one uPML term of a boundary_region (src/meep.hpp:665-672): the profile (null: the quadratic
+// pml_quadratic_profile, src/structure.cpp:252-255) and its integrals
+struct PmlTerm {
+  double mean_stretch = 1.0;
+  mp_scalar_func profile = nullptr;
+  void *user = nullptr;
+  double integral = 1. / 3., integral_u = 1. / 4.;
+  double eval(double u) const {
+    if (!profile) return u * u;
+    const double r[3] = {u, 0, 0};
+    return profile(r, user);
+  }
+};
+template <class R> void use_pml(Structure<R> &s, int d, double dx, double bloc, double Rasym, const PmlTerm &term) {
   if (dx <= 0.0) return;
-  const double integral = 1. / 3., integral_u = 1. / 4., mean_stretch = 1.0;
+#ifdef MUT_PML_INTEGRAL
+  const double integral = 1. / 3., integral_u = term.integral_u, mean_stretch = term.mean_stretch;
+#elif defined(MUT_PML_STRETCH)
+  const double integral = term.integral, integral_u = term.integral, mean_stretch = term.mean_stretch;
+#else
+  const double integral = term.integral, integral_u = term.integral_u, mean_stretch = term.mean_stretch;
+#endif
   const double prefac = (-log(Rasym)) / (4 * dx * integral);
   const double kappa_prefac = (mean_stretch - 1) / integral_u;
   const GV &gv = s.gv;
@@ -567,7 +588,7 @@ template <class R> void use_pml(Structure<R> &s, int d, double dx, double bloc, 
   for (int i = 0; i <= 2 * gv.num[d] + 1; ++i) {
     double x = pml_x(i, dx, bloc, s.a);
     if (x > 0) {
-      double sp = (x / dx) * (x / dx); // pml_quadratic_profile
+      double sp = term.eval(x / dx);
       s.sig[d][i] = 0.5 * s.dt * prefac * sp;
       s.kap[d][i] = 1 + kappa_prefac * sp * (x / dx);
       s.siginv[d][i] = 1 / (s.kap[d][i] + s.sig[d][i]);
@@ -1863,7 +1884,8 @@ template <class F> int guard(F fn) {
 }
 
 template <class R> void fill_structure(Structure<R> &s, int dim, double xs, double ys, double zs, double a, mp_scalar_func eps, void *eu, int n_pml,
-                                       const int *pd, const int *ps, const double *pt, const double *pR, double courant) {
+                                       const int *pd, const int *ps, const double *pt, const double *pR, const PmlTerm *terms,
+                                       double courant) {
   auto np = [&](double sz) { return sz == 0 ? 1 : int(sz * a + 0.5); };
   if (dim == 1)
     s.gv.init(1, a, 0, 0, int(zs * a + 0.5));
@@ -1881,7 +1903,7 @@ template <class R> void fill_structure(Structure<R> &s, int dim, double xs, doub
   for (int i = 0; i < n_pml; ++i) {
     int d = pd[i];
     if (d < 0 || d > 2 || !gv.has(d) || gv.num[d] <= 1) continue;
-    use_pml(s, d, pt[i], gv.boundary_location(ps[i], d), pR[i] > 0 ? pR[i] : 1e-15);
+    use_pml(s, d, pt[i], gv.boundary_location(ps[i], d), pR && pR[i] > 0 ? pR[i] : 1e-15, terms[i]);
   }
   // set_chi1inv without averaging: 1/eps at the Yee point (src/anisotropic_averaging.cpp:90-98)
   if (eps)
@@ -1944,19 +1966,32 @@ extern "C" {
 const char *mp_last_error(void) { return last_error.c_str(); }
 const char *mp_engine_name(void) { return "cpu-oracle"; }
 
-mp_structure *mp_structure_new(int dim, double xsize, double ysize, double zsize, double a, mp_scalar_func eps, void *eps_user, int n_pml,
-                               const int *pml_direction, const int *pml_side, const double *pml_thickness, const double *pml_R,
-                               double courant, int single_precision) {
+mp_structure *mp_structure_new_pml(int dim, double xsize, double ysize, double zsize, double a, mp_scalar_func eps,
+                                   void *eps_user, int n_pml, const int *pml_direction, const int *pml_side,
+                                   const double *pml_thickness, const double *pml_R, const double *pml_mean_stretch,
+                                   const mp_scalar_func *pml_profile, void *const *pml_profile_user,
+                                   const double *pml_integral, const double *pml_integral_u, double courant,
+                                   int single_precision) {
   AnyStructure *s = new AnyStructure;
   s->single = single_precision != 0;
   int rc = guard([&] {
+    std::vector<PmlTerm> terms(n_pml > 0 ? n_pml : 0);
+    for (int i = 0; i < n_pml; ++i) {
+      if (pml_mean_stretch) terms[i].mean_stretch = pml_mean_stretch[i];
+      if (pml_profile) terms[i].profile = pml_profile[i];
+      if (pml_profile_user) terms[i].user = pml_profile_user[i];
+      if (pml_integral && pml_integral[i] > 0) terms[i].integral = pml_integral[i];
+      if (pml_integral_u && pml_integral_u[i] > 0) terms[i].integral_u = pml_integral_u[i];
+    }
     if (s->single) {
       s->s.reset(new Structure<float>);
-      fill_structure(*s->s, dim, xsize, ysize, zsize, a, eps, eps_user, n_pml, pml_direction, pml_side, pml_thickness, pml_R, courant);
+      fill_structure(*s->s, dim, xsize, ysize, zsize, a, eps, eps_user, n_pml, pml_direction, pml_side, pml_thickness, pml_R,
+                     terms.data(), courant);
     }
     else {
       s->d.reset(new Structure<double>);
-      fill_structure(*s->d, dim, xsize, ysize, zsize, a, eps, eps_user, n_pml, pml_direction, pml_side, pml_thickness, pml_R, courant);
+      fill_structure(*s->d, dim, xsize, ysize, zsize, a, eps, eps_user, n_pml, pml_direction, pml_side, pml_thickness, pml_R,
+                     terms.data(), courant);
     }
   });
   if (rc) {
@@ -1964,6 +1999,12 @@ mp_structure *mp_structure_new(int dim, double xsize, double ysize, double zsize
     return nullptr;
   }
   return reinterpret_cast<mp_structure *>(s);
+}
+mp_structure *mp_structure_new(int dim, double xsize, double ysize, double zsize, double a, mp_scalar_func eps, void *eps_user, int n_pml,
+                               const int *pml_direction, const int *pml_side, const double *pml_thickness, const double *pml_R,
+                               double courant, int single_precision) {
+  return mp_structure_new_pml(dim, xsize, ysize, zsize, a, eps, eps_user, n_pml, pml_direction, pml_side, pml_thickness, pml_R,
+                              nullptr, nullptr, nullptr, nullptr, nullptr, courant, single_precision);
 }
 void mp_structure_delete(mp_structure *s) { delete reinterpret_cast<AnyStructure *>(s); }
 size_t mp_structure_ntot(const mp_structure *s) {

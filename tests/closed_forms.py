@@ -44,6 +44,15 @@ run against a value the mathematics fixes; the tests assert on these errors.
   decay by fixed factors per step (``conductive_decay``), so after one step the flux plane's
   mean of the fluxes after the H and the E updates, the half step that synchronises H, and
   the synchronised magnetic energy all have closed forms.
+* ``pml_reflection``: a uPML layer of asymptotic reflection R (a custom profile u^3 with its
+  integrals 1/4 and 1/5, and the default u^2) reflects power R in the continuum: the
+  attenuation of each pass is exp(-int sigma) with int sigma = -ln(R) / 4 (structure_chunk::
+  use_pml, src/structure.cpp:630-691).  The reflected wave is the difference between a run
+  with the layer (metal behind it) and one with a thick layer of R = 1e-15.
+* ``pml_stretch``: a lossless (R = 1) layer of thickness d and mean stretch s is, in the
+  continuum, the coordinate map x -> int kappa dx, i.e. a layer of vacuum s d thick: the
+  fields in front of it equal those of a cell longer by (s - 1) d, up to the lattice's
+  O(dx^2) error.
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -275,6 +284,35 @@ def flux_time_domain(lib, a=10):
     e_exp = 0.5 * eps * (r_e * e0) ** 2 * area + 0.5 * (g0 * g0 + h_sync * h_sync) * area
     err["energy_sync"] = abs(f.field_energy_in_box(*box) - e_exp) / e_exp
     return err
+
+
+def _pml_line(lib, L, terms, a, zs=3.0, zm=5.0, T=80.0, freqs=(0.4, 0.5, 0.6)):
+    """DFT of Ex at zm after a pulse from zs on a 1D line of length L with the uPML terms."""
+    s = structure(lib, dim=1, size=(0, 0, L), a=a, pml=terms)
+    f = fields(s)
+    f.use_real_fields()
+    f.add_volume_source(Ex, gaussian_src_time(0.5, 0.4), (0, 0, zs), (0, 0, zs))
+    m = f.add_dft_fields([Ex], (0, 0, zm), (0, 0, zm), list(freqs), decimation_factor=1)
+    f.step_n(int(round(T / s.dt)))
+    return m.dft(0)[0]
+
+
+def pml_reflection(lib, a=20, R=1e-3):
+    """|r|^2 / R - 1 over three frequencies for a cubic-profile layer and a quadratic one."""
+    inc = _pml_line(lib, 14.0, [(Z, Low, 2.0, 1e-15), (Z, High, 6.0, 1e-15)], a)
+    out = {}
+    for name, term in (("cubic", (Z, High, 2.0, R, 1.0, lambda u: u ** 3, 0.25, 0.2)), ("quadratic", (Z, High, 2.0, R))):
+        tot = _pml_line(lib, 14.0, [(Z, Low, 2.0, 1e-15), term], a)
+        out[name] = float(np.abs(np.abs(tot - inc) ** 2 / np.abs(inc) ** 2 / R - 1).max())
+    return out
+
+
+def pml_stretch(lib, a=40, d=1.0, s=2.0):
+    """Relative difference of the fields in front of a lossless stretched layer and of those
+    in a cell longer by (s - 1) d."""
+    A = _pml_line(lib, 10.0, [(Z, Low, 2.0, 1e-15), (Z, High, d, 1.0, s)], a)
+    B = _pml_line(lib, 10.0 + (s - 1) * d, [(Z, Low, 2.0, 1e-15)], a)
+    return float(np.abs(A - B).max() / np.abs(B).max())
 
 
 def source_spectrum(dt=0.025, fcen=1.0, fwidth=0.5, freqs=None):

@@ -211,6 +211,26 @@ def conductive_3d(lib, single=False, a=8, real=True):
     return s, f, [fl]
 
 
+def stretched_pml_3d(lib, single=False, a=8, real=True):
+    """3D cell whose uPML terms carry the rest of boundary_region's parameters: a mean stretch
+    on x (kappa != 1 through the f_u / f_w branches), a cubic profile with its integrals on y,
+    the defaults on z; a Lorentz pole in the lower half and a point and a plane source."""
+    L = 2.0
+    terms = [(X, side, 0.5, 1e-8, 1.7) for side in (High, Low)]
+    terms += [(Y, side, 0.5, 1e-6, 1.0, lambda u: u ** 3, 0.25, 0.2) for side in (High, Low)]
+    terms += [(Z, side, 0.375) for side in (High, Low)]
+    s = structure(lib, dim=3, size=(L, L, L), a=a, eps=lambda x, y, z: 2.5 if z < L / 2 else 1.0, pml=terms,
+                  single=single)
+    s.add_susceptibility(lambda x, y, z: 1.2 if z < L / 2 else 0.0, 1.1, 0.1)
+    f = fields(s)
+    if real:
+        f.use_real_fields()
+    f.add_volume_source(Ey, gaussian_src_time(0.9, 0.4), (0.3, 0.3, 1.4), (L - 0.3, L - 0.3, 1.4))
+    f.add_volume_source(Ez, gaussian_src_time(0.7, 0.3), (0.23, 1.61, 0.8), (0.23, 1.61, 0.8))
+    fl = f.add_dft_flux(Z, (0.4, 0.4, 1.2), (L - 0.4, L - 0.4, 1.2), linspace(0.5, 1.1, 6))
+    return s, f, [fl]
+
+
 def straight_waveguide(lib, single=False, real=True):
     """The 2D straight-waveguide configuration at full size (python/examples/
     straight-waveguide.py's geometry): 16x8 at resolution 20 (320x160 cells), eps = 12 for
@@ -241,6 +261,7 @@ SCENARIOS = {
     "long_sources_3d": (long_sources_3d, 120),
     "conductive_2d_tm": (conductive_2d_tm, 200),
     "conductive_3d": (conductive_3d, 100),
+    "stretched_pml_3d": (stretched_pml_3d, 100),
 }
 
 FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)
@@ -263,7 +284,8 @@ def pml_active(s, d):
     n = s.num[d]
     act = np.zeros(2 * n + 2, dtype=bool)
     a = s.a
-    for dd, side, dx, _R in s.pml:
+    for term in s.pml:
+        dd, side, dx = term[:3]
         if dd != d or dx <= 0 or n <= 1:
             continue
         bloc = n * (1.0 / a) if side == High else 0.0

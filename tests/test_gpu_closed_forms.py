@@ -50,3 +50,11 @@ def test_dft_array_of_a_plane_wave_gpu(gpu_lib):
 
 def test_time_domain_flux_and_energy_integrals_gpu(gpu_lib):
     T.check_flux_time_domain(gpu_lib)
+
+
+def test_pml_reflects_its_asymptotic_reflection_gpu(gpu_lib):
+    T.check_pml_reflection(gpu_lib)
+
+
+def test_pml_stretch_is_a_longer_vacuum_gpu(gpu_lib):
+    T.check_pml_stretch(gpu_lib)

@@ -31,6 +31,10 @@ values do not reach (tests/closed_forms.py states each closed form):
   centred grid, uniform fields over a box or line, and uniform fields decaying under
   conductivity through one step (the flux plane's mean of its two fluxes, the synchronising
   half step of H).  Observed about 1e-16.
+* uPML terms: a layer of asymptotic reflection R reflects power R (custom cubic profile with
+  its integrals, and the default quadratic one; 1-2 % at 20 points per unit), and a lossless
+  layer of mean stretch s acts as s times its thickness of vacuum (5 % at 40 points per unit,
+  falling as dx^2).
 * Transport and spectrum shape: equal flux through two planes of a lossless 1D line, and a
   flux spectrum proportional to |J(omega)|^2.  These two are scale-invariant: they do not
   pin the DFT's normalisation, phase or E/H time offset (the first three checks do).
@@ -50,7 +54,7 @@ from meep_amd.capi import (Ex, Ez, MeepError, Z, fields, gaussian_src_time, lins
 
 TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
        "pole_decay": 1e-4, "cavity": 1e-10, "cond_decay": 1e-12, "cond_pml_line": 1e-6,
-       "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12}
+       "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12, "pml_refl": 0.03, "pml_stretch": 0.08}
 
 
 def check_line_source(lib):
@@ -110,6 +114,15 @@ def check_flux_time_domain(lib):
     assert max(err.values()) <= TOL["flux_time"], err
 
 
+def check_pml_reflection(lib):
+    err = CF.pml_reflection(lib)
+    assert max(err.values()) <= TOL["pml_refl"], err
+
+
+def check_pml_stretch(lib):
+    assert CF.pml_stretch(lib) <= TOL["pml_stretch"]
+
+
 def test_line_source_matches_lattice_solution(oracle_lib):
     check_line_source(oracle_lib)
 
@@ -160,6 +173,17 @@ def test_dft_array_of_a_plane_wave(oracle_lib):
 
 def test_time_domain_flux_and_energy_integrals(oracle_lib):
     check_flux_time_domain(oracle_lib)
+
+
+def test_pml_reflects_its_asymptotic_reflection(oracle_lib):
+    check_pml_reflection(oracle_lib)
+
+
+def test_pml_stretch_is_a_longer_vacuum(oracle_lib):
+    check_pml_stretch(oracle_lib)
+    # the lattice's error of the equivalence falls as dx^2
+    e40, e80 = CF.pml_stretch(oracle_lib, a=40), CF.pml_stretch(oracle_lib, a=80)
+    assert 0.18 < e80 / e40 < 0.32, (e40, e80)
 
 
 def _cavity_energy_spread(lib, a):

@@ -10,7 +10,8 @@ uPML branch without its (kap - sig) factor, a uPML branch taken over the whole c
 of its region, and a DFT array that keeps the integration weights or the stored weight, sums
 the planes around an off-grid monitor unweighted, or places a periodic piece without its
 shift; a time-domain flux that adds its second pair, reads fields off the centred grid, or
-keeps only the flux after the E update."""
+keeps only the flux after the E update; a uPML term whose sigma ignores its profile's integral or
+whose kappa uses the wrong one."""
 
 import ctypes
 import os
@@ -43,6 +44,8 @@ CHECKS = {
     "FLUXT_SIGN": T.check_flux_time_domain,
     "FLUXT_CENTRE": T.check_flux_time_domain,
     "FLUXT_HALF": T.check_flux_time_domain,
+    "PML_INTEGRAL": T.check_pml_reflection,
+    "PML_STRETCH": T.check_pml_stretch,
 }
 
 
