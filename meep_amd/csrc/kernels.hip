@@ -690,7 +690,36 @@ __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t co
 // by lane and only in the groups outside the z quiet range.  The generic per-slot logic of
 // k_fast (runtime loop axes, periodic images, x/y profiles) is absent, so the per-point
 // instruction count is a fraction of k_fast's; sources run in a non-inlined helper entered
-// only by the groups inside the source box.  Arithmetic is the reference's, as in k_fast.
+// only by the groups inside the source box.  Each block updates one slot (logical block mod
+// 3), so a thread carries one slot's registers and the three blocks of a tile, dealt to the
+// same XCD, share the partner rows in its L2.  Arithmetic is the reference's, as in k_fast.
+
+// t / d for every 32-bit t by a multiply-high, a subtraction and two shifts (the round-up
+// method of Granlund and Montgomery, "Division by invariant integers using multiplication",
+// PLDI 1994): with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// t / d = (h + ((t - h) >> 1)) >> (l - 1) where h = mulhi(m, t); d = 1 takes m = 0, no shifts.
+struct Div32 {
+  uint32_t m;
+  int s1, s2;
+};
+static inline Div32 div32(uint32_t d) {
+  Div32 r{0, 0, 0};
+  if (d <= 1) return r;
+  int l = 0;
+  while ((uint64_t(1) << l) < d) ++l;
+  r.m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+  r.s1 = 1;
+  r.s2 = l - 1;
+  return r;
+}
+__host__ __device__ __forceinline__ uint32_t div32_apply(const Div32 &q, uint32_t t) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint32_t h = __umulhi(q.m, t);
+#else
+  const uint32_t h = uint32_t((uint64_t(q.m) * t) >> 32);
+#endif
+  return (h + ((t - h) >> q.s1)) >> q.s2;
+}
 
 struct Row3Geom {
   uint32_t sx, sy;      // strides along x and y (z is stride 1; a chunk has < 2^32 points)
@@ -699,6 +728,7 @@ struct Row3Geom {
   uint32_t nyb;         // rows along y in the box
   int g0;               // first group along z
   uint32_t ngroups;     // groups per row
+  Div32 dgroups, dnyb;  // division by ngroups and by nyb
   int zlo, zhi;         // launch box along z
   int qzlo, qzhi;       // z indices where no z uPML profile is active
   int mhi_bits, mlo_bits; // bit d: the High wall normal to d is Metallic / the Low wall Magnetic
@@ -715,13 +745,12 @@ template <class R, int NP> struct Row3Args {
   const R *chi[3];      // diagonal chi1inv / mu1inv (null: 1)
   R *fw2;               // f_w of slot 2 (its dsigw is z), null when none
   const R *zsig, *zkap, *zsiginv; // the z uPML profile (null: no z uPML)
-  // poles: P and P_prev of pole p for component c are the arrays poff[p] and ppoff[p] of the
-  // block at pbase[c], its sigma the array soff[p] of the block at sbase[c] (pst values
-  // apart): six pointers instead of three per pole and component
+  // poles: P and P_prev of pole p for component c are at pofs[p] and ppofs[p] (values) from
+  // pbase[c], its sigma at sofs[p] from sbase[c]: six pointers instead of three per pole and
+  // component, offsets the same for every component
   R *pbase[3];
   const R *sbase[3];
-  int64_t pst;
-  int poff[NP > 0 ? NP : 1], ppoff[NP > 0 ? NP : 1], soff[NP > 0 ? NP : 1];
+  int64_t pofs[NP > 0 ? NP : 1], ppofs[NP > 0 ? NP : 1], sofs[NP > 0 ? NP : 1];
   int pmask;            // bit 3 p + c: pole p has a polarisation for component c
   int smask;            // bit 3 p + c: ... and a sigma array
   R sigval[NP > 0 ? NP : 1][3];
@@ -824,15 +853,15 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
       const PoleCoef<R> &q = a.pole[p];
       pin[p] = ((a.pmask >> (3 * p + C)) & 1) && ((pinxy >> p) & 1) && z0 + V - 1 >= q.lo[2] && z0 <= q.hi[2];
       if (pin[p]) {
-        Pc[p] = vload<R, V>(a.pbase[C] + (a.poff[p] * a.pst + i));
+        Pc[p] = vload<R, V>(a.pbase[C] + (a.pofs[p] + i));
         if ((a.smask >> (3 * p + C)) & 1) {
-          Ppv[p] = vload<R, V>(a.pbase[C] + (a.ppoff[p] * a.pst + i));
+          Ppv[p] = vload<R, V>(a.pbase[C] + (a.ppofs[p] + i));
           if ((a.sigone >> (3 * p + C)) & 1) {
 #pragma unroll
             for (int k = 0; k < V; ++k) sgv[p].v[k] = z0 + k >= q.lo[2] && z0 + k <= q.hi[2] ? a.sigval[p][C] : R(0);
           }
           else
-            sgv[p] = vload<R, V>(a.sbase[C] + (a.soff[p] * a.pst + i));
+            sgv[p] = vload<R, V>(a.sbase[C] + (a.sofs[p] + i));
         }
       }
     }
@@ -957,10 +986,10 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
     for (int p = 0; p < NP; ++p)
       if (pin[p] && ((a.smask >> (3 * p + C)) & 1)) {
         if (a.pswap)
-          vstore<R, V>(a.pbase[C] + (a.ppoff[p] * a.pst + i), Pn[p], mask);
+          vstore<R, V>(a.pbase[C] + (a.ppofs[p] + i), Pn[p], mask);
         else {
-          vstore<R, V>(a.pbase[C] + (a.poff[p] * a.pst + i), Pn[p], mask);
-          vstore<R, V>(a.pbase[C] + (a.ppoff[p] * a.pst + i), Pc[p], mask);
+          vstore<R, V>(a.pbase[C] + (a.pofs[p] + i), Pn[p], mask);
+          vstore<R, V>(a.pbase[C] + (a.ppofs[p] + i), Pc[p], mask);
         }
       }
     vstore<R, V>(ep + i, ev, mask);
@@ -969,18 +998,17 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
 
 template <class R, int V, int PH, bool UPD, int NP, bool ZP>
 __global__ __launch_bounds__(BLOCK) void k_row3(const Row3Args<R, NP> a, uint32_t count) {
-  const uint32_t t = xcd_block(a.g.rev) * uint32_t(BLOCK) + threadIdx.x;
+  const uint32_t lb = xcd_block(a.g.rev);
+  const int C = int(lb % 3u);
+  const uint32_t t = (lb / 3u) * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
   const Row3Geom &g = a.g;
-  const uint32_t r = t / g.ngroups;
+  const uint32_t r = div32_apply(g.dgroups, t);
   const int grp = int(t - r * g.ngroups);
-  const uint32_t qx = r / g.nyb;
+  const uint32_t qx = div32_apply(g.dnyb, r);
   const int x = g.x0 + int(qx), y = g.y0 + int(r - qx * g.nyb);
   const int z0 = (g.g0 + grp) * V;
   const uint32_t i = uint32_t(x) * g.sx + uint32_t(y) * g.sy + uint32_t(z0);
-  const unsigned z0m = r3_zmask<V, PH, 0>(g, z0), z1m = r3_zmask<V, PH, 1>(g, z0), z2m = r3_zmask<V, PH, 2>(g, z0);
-  if (!(z0m | z1m | z2m)) return;
-  // per-thread facts computed once, so the bounds they need are not kept for every slot
   const bool srcs = a.src && x >= a.sbox_lo[0] && x <= a.sbox_hi[0] && y >= a.sbox_lo[1] && y <= a.sbox_hi[1] &&
                     z0 + V - 1 >= a.sbox_lo[2] && z0 <= a.sbox_hi[2];
   unsigned pinxy = 0;
@@ -989,10 +1017,24 @@ __global__ __launch_bounds__(BLOCK) void k_row3(const Row3Args<R, NP> a, uint32_
     const PoleCoef<R> &q = a.pole[p];
     if (x >= q.lo[0] && x <= q.hi[0] && y >= q.lo[1] && y <= q.hi[1]) pinxy |= 1u << p;
   }
-  Vec<R, V> F0 = vload<R, V>(a.F[0] + i), F1 = vload<R, V>(a.F[1] + i), F2 = vload<R, V>(a.F[2] + i);
-  r3_slot<R, V, PH, UPD, NP, ZP, 0>(a, i, x, y, z0, z0m, F0, F1, F2, srcs, pinxy);
-  r3_slot<R, V, PH, UPD, NP, ZP, 1>(a, i, x, y, z0, z1m, F0, F1, F2, srcs, pinxy);
-  r3_slot<R, V, PH, UPD, NP, ZP, 2>(a, i, x, y, z0, z2m, F0, F1, F2, srcs, pinxy);
+  if (C == 0) {
+    const unsigned m = r3_zmask<V, PH, 0>(g, z0);
+    if (!m) return;
+    const Vec<R, V> F1 = vload<R, V>(a.F[1] + i), F2 = vload<R, V>(a.F[2] + i);
+    r3_slot<R, V, PH, UPD, NP, ZP, 0>(a, i, x, y, z0, m, F1, F1, F2, srcs, pinxy);
+  }
+  else if (C == 1) {
+    const unsigned m = r3_zmask<V, PH, 1>(g, z0);
+    if (!m) return;
+    const Vec<R, V> F0 = vload<R, V>(a.F[0] + i), F2 = vload<R, V>(a.F[2] + i);
+    r3_slot<R, V, PH, UPD, NP, ZP, 1>(a, i, x, y, z0, m, F0, F2, F2, srcs, pinxy);
+  }
+  else {
+    const unsigned m = r3_zmask<V, PH, 2>(g, z0);
+    if (!m) return;
+    const Vec<R, V> F0 = vload<R, V>(a.F[0] + i), F1 = vload<R, V>(a.F[1] + i);
+    r3_slot<R, V, PH, UPD, NP, ZP, 2>(a, i, x, y, z0, m, F0, F1, F1, srcs, pinxy);
+  }
 }
 
 // The row kernel for a's launch box when the box qualifies (3D, loop axes = directions, no
@@ -1026,6 +1068,8 @@ static bool r3_fill(const StepArgs<R> &a, bool curl, bool upd, Row3Args<R, NP> &
   q.zhi = g.alo[2] + g.an[2] - 1;
   q.g0 = g.alo[2] / V;
   q.ngroups = uint32_t((g.alo[2] + g.an[2] - 1) / V - q.g0 + 1);
+  q.dgroups = div32(q.ngroups);
+  q.dnyb = div32(q.nyb);
   q.qzlo = g.qlo[2];
   q.qzhi = g.qhi[2];
   q.mhi_bits = g.mhi_bits;
@@ -1076,7 +1120,6 @@ static bool r3_fill(const StepArgs<R> &a, bool curl, bool upd, Row3Args<R, NP> &
   // component's arrays, and every array must sit a whole number of arrays from it, with the
   // same offsets for every component (else the launch goes to k_fast)
   if (NP > 0) {
-    b.pst = 0;
     for (int c = 0; c < 3; ++c) {
       if (!b.e[c]) continue;
       const UpdComp<R> &u = a.uc[c];
@@ -1108,15 +1151,15 @@ static bool r3_fill(const StepArgs<R> &a, bool curl, bool upd, Row3Args<R, NP> &
       }
     }
     if (st < 0) st = 1;
-    b.pst = st;
-    for (int p = 0; p < NP; ++p) b.poff[p] = b.ppoff[p] = b.soff[p] = -1;
+    int poff[NP > 0 ? NP : 1], ppoff[NP > 0 ? NP : 1], soff[NP > 0 ? NP : 1];
+    for (int p = 0; p < NP; ++p) poff[p] = ppoff[p] = soff[p] = -1;
     for (int c = 0; c < 3; ++c) {
       if (!b.e[c]) continue;
       const UpdComp<R> &u = a.uc[c];
       for (int p = 0; p < NP && p < a.npole; ++p) {
         const int64_t o[3] = {u.P[p] ? int64_t(u.P[p] - b.pbase[c]) / st : -1, u.Pp[p] ? int64_t(u.Pp[p] - b.pbase[c]) / st : -1,
                               u.sigma[p] ? int64_t(u.sigma[p] - b.sbase[c]) / st : -1};
-        int *dst[3] = {&b.poff[p], &b.ppoff[p], &b.soff[p]};
+        int *dst[3] = {&poff[p], &ppoff[p], &soff[p]};
         for (int w = 0; w < 3; ++w) {
           if (o[w] < 0) continue;
           if (o[w] > 2 * NP) return false;
@@ -1126,6 +1169,11 @@ static bool r3_fill(const StepArgs<R> &a, bool curl, bool upd, Row3Args<R, NP> &
             return false;
         }
       }
+    }
+    for (int p = 0; p < NP; ++p) {
+      b.pofs[p] = poff[p] < 0 ? 0 : poff[p] * st;
+      b.ppofs[p] = ppoff[p] < 0 ? 0 : ppoff[p] * st;
+      b.sofs[p] = soff[p] < 0 ? 0 : soff[p] * st;
     }
   }
   for (int p = 0; p < NP && p < a.npole; ++p) b.pole[p] = a.pole[p];
@@ -1146,7 +1194,7 @@ static bool r3_launch(const StepArgs<R> &a, bool curl, bool upd, hipStream_t s) 
   const size_t n = size_t(b.g.ngroups) * size_t(a.g.an[0]) * size_t(a.g.an[1]);
   if (n >= (size_t(1) << 32)) return false;
   const uint32_t c32 = uint32_t(n);
-  const unsigned nb = nblocks8(n);
+  const unsigned nb = (3u * nblocks(n) + 7u) / 8u * 8u;
   const bool zp = b.zsig != nullptr;
   if (zp)
     hipLaunchKernelGGL((k_row3<R, V, PH, UPD, NP, true>), dim3(nb), dim3(BLOCK), 0, s, b, c32);
