@@ -672,11 +672,52 @@ __device__ __forceinline__ void fast_body(const StepArgs<R> &a, uint32_t t, int 
 }
 
 
+// group t of the launch box for slot C alone
+template <class R, int V, bool CURL, bool UPD, bool PML, int NP, int C>
+__device__ __forceinline__ void fast_body_slot(const StepArgs<R> &a, uint32_t t, int g0, int ngroups) {
+  const Grid &g = a.g;
+  const uint32_t r = t / uint32_t(ngroups);
+  const int grp = int(t - r * uint32_t(ngroups));
+  const int q0 = int(r / uint32_t(g.an[1]));
+  const int q1 = int(r - uint32_t(q0) * uint32_t(g.an[1]));
+  const int j0 = g.alo[0] + q0, j1 = g.alo[1] + q1;
+  const int z0 = (g0 + grp) * V;
+  const uint32_t i = uint32_t(j0 * g.ast[0] + j1 * g.ast[1] + z0);
+  unsigned boxmask = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+    if (z0 + k >= g.alo[2] && z0 + k < g.alo[2] + g.an[2]) boxmask |= 1u << k;
+  if (!(CURL ? a.cc[C].f : a.uc[C].f)) return;
+  const unsigned m = group_owned<V>(g, CURL ? a.cc[C].ax.ybits : a.uc[C].ax.ybits, j0, j1, z0, boxmask);
+  if (!m) return;
+  Vec<R, V> F0, F1, F2;
+  if (CURL) {
+    if (a.F[0]) F0 = vload<R, V>(a.F[0] + i);
+    if (a.F[1]) F1 = vload<R, V>(a.F[1] + i);
+    if (a.F[2]) F2 = vload<R, V>(a.F[2] + i);
+  }
+  fast_slot<R, V, CURL, UPD, PML, NP, C>(a, i, j0, j1, z0, m, F0, F1, F2, a.vrow);
+}
+
+// With an update (UPD), one slot per block (logical block mod 3): a thread carries one
+// slot's registers, and the three blocks of a tile, dealt to the same XCD, share the partner
+// rows in its L2.  A curl alone keeps the three slots in one thread (measured faster there:
+// its threads are light, and the shared centre rows are loaded once).
 template <class R, int V, bool CURL, bool UPD, bool PML, int NP>
 __global__ __launch_bounds__(BLOCK) void k_fast(const StepArgs<R> a, uint32_t count, int g0, int ngroups) {
-  const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
+  if (!UPD) {
+    const uint32_t t = xcd_block(a.rev) * uint32_t(BLOCK) + threadIdx.x;
+    if (t >= count) return;
+    fast_body<R, V, CURL, UPD, PML, NP>(a, t, g0, ngroups, a.vrow);
+    return;
+  }
+  const uint32_t lb = xcd_block(a.rev);
+  const int C = int(lb % 3u);
+  const uint32_t t = (lb / 3u) * uint32_t(BLOCK) + threadIdx.x;
   if (t >= count) return;
-  fast_body<R, V, CURL, UPD, PML, NP>(a, t, g0, ngroups, a.vrow);
+  if (C == 0) fast_body_slot<R, V, CURL, UPD, PML, NP, 0>(a, t, g0, ngroups);
+  else if (C == 1) fast_body_slot<R, V, CURL, UPD, PML, NP, 1>(a, t, g0, ngroups);
+  else fast_body_slot<R, V, CURL, UPD, PML, NP, 2>(a, t, g0, ngroups);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1498,7 +1539,7 @@ template <class R, int V, bool PML, int NP> void launch_fast_np(const StepArgs<R
   int g0, ngroups;
   uint32_t c32;
   fast_groups<V>(a.g, g0, ngroups, c32);
-  const unsigned nb = nblocks8(c32);
+  const unsigned nb = upd ? (3u * nblocks(c32) + 7u) / 8u * 8u : nblocks8(c32); // with an update, one slot per block
   if (curl && upd)
     hipLaunchKernelGGL((k_fast<R, V, true, true, PML, NP>), dim3(nb), dim3(BLOCK), 0, s, a, c32, g0, ngroups);
   else if (curl)
