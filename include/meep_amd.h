@@ -160,6 +160,30 @@ int mp_fields_add_point_source_legacy(mp_fields *f, int c, double freq, double w
 int mp_fields_add_volume_source(mp_fields *f, int c, const mp_src_time *src,
                                 const double *vmin, const double *vmax, double amp_re,
                                 double amp_im, mp_complex_func A, void *A_user);
+/* fields::add_volume_source(c, src, where, complex<double> *arr, dim1, dim2, dim3, amp) —
+   src/meep.hpp:1895-1896, src/sources.cpp:394-417: the amplitude at a point is the array
+   interpolated at the point's place in the volume (amp_file_func, src/sources.cpp:345-376:
+   each direction the volume spans mapped to [0, 1]; linear_interpolate with mirror
+   boundaries, src/fields.cpp:754-810).  arr holds dim1 x dim2 x dim3 complex values, row
+   major, interleaved (re, im). */
+int mp_fields_add_volume_source_array(mp_fields *f, int c, const mp_src_time *src,
+                                      const double *vmin, const double *vmax, const double *arr,
+                                      size_t dim1, size_t dim2, size_t dim3, double amp_re,
+                                      double amp_im);
+/* fields::add_srcdata(sourcedata{c, index, chunk 0, amp}, src, n, amp, needs_boundary_fix) —
+   src/meep.hpp:1907-1908, src/sources.cpp:314-334: a source on n points of the E or H
+   component c, given by their indices in c's array (reference layout) and complex
+   amplitudes (interleaved re, im) applied as the reference applies a source's (times the
+   current and dt, or the dipole).  Points the chunk does not own wait for
+   mp_fields_require_source_components when needs_boundary_fix, and are dropped otherwise
+   (the halo exchange would overwrite them). */
+int mp_fields_add_srcdata(mp_fields *f, int c, const mp_src_time *src, long n, const long *index,
+                          const double *amp, int needs_boundary_fix);
+/* fields::require_source_components — src/fields.cpp:508-530, with fix_boundary_sources
+   (src/fix_boundary_sources.cpp:51-197): every marked point the chunk does not own moves to
+   the point that owns it (its periodic image, the amplitude times the conjugate Bloch phase)
+   unless that point is on a metal boundary; the source components are allocated. */
+int mp_fields_require_source_components(mp_fields *f);
 /* fields::step() — src/meep.hpp:1871, src/step.cpp:35-139.  Returns an error (the
    reference aborts) when the fields are not finite. */
 int mp_fields_step(mp_fields *f);

@@ -410,6 +410,14 @@ private:
   int n;
 };
 
+// sourcedata (src/meep.hpp:1323-1328): points of one chunk's component array and amplitudes
+struct sourcedata {
+  component near_fd_comp;
+  std::vector<ptrdiff_t> idx_arr;
+  int fc_idx;
+  std::vector<cdouble> amp_arr;
+};
+
 // flux_vol (src/meep.hpp:2322-2352): the flux of E x H through a volume, taken by every step;
 // owned by its fields as in the reference
 class flux_vol {
@@ -487,6 +495,30 @@ public:
     check(mp_fields_add_volume_source(p, c, &d, where.get_min_corner().data(), where.get_max_corner().data(), amp.real(),
                                       amp.imag(), nullptr, nullptr));
   }
+  // add_volume_source with an amplitude array (:1895-1896, src/sources.cpp:394-417)
+  void add_volume_source(component c, const src_time &src, const volume &where, cdouble *arr, size_t dim1, size_t dim2,
+                         size_t dim3, cdouble amp) {
+    mp_src_time d = src.desc();
+    std::vector<double> flat(2 * dim1 * dim2 * dim3);
+    for (size_t q = 0; q < dim1 * dim2 * dim3; ++q) flat[2 * q] = arr[q].real(), flat[2 * q + 1] = arr[q].imag();
+    check(mp_fields_add_volume_source_array(p, c, &d, where.get_min_corner().data(), where.get_max_corner().data(),
+                                            flat.data(), dim1, dim2, dim3, amp.real(), amp.imag()));
+  }
+  // add_srcdata (:1907-1908, src/sources.cpp:314-334); one chunk, so fc_idx must be 0
+  void add_srcdata(sourcedata cur_data, src_time *src, size_t n, cdouble *amp_arr, bool needs_boundary_fix) {
+    if (cur_data.fc_idx != 0) meep::abort("fields chunk index out of range");
+    if (n == 0) {
+      n = cur_data.idx_arr.size();
+      amp_arr = cur_data.amp_arr.data();
+    }
+    if (n != cur_data.idx_arr.size()) meep::abort("add_srcdata: %zu amplitudes for %zu points", n, cur_data.idx_arr.size());
+    std::vector<long> idx(cur_data.idx_arr.begin(), cur_data.idx_arr.end());
+    std::vector<double> amp(2 * n);
+    for (size_t q = 0; q < n; ++q) amp[2 * q] = amp_arr[q].real(), amp[2 * q + 1] = amp_arr[q].imag();
+    mp_src_time d = src->desc();
+    check(mp_fields_add_srcdata(p, cur_data.near_fd_comp, &d, long(n), idx.data(), amp.data(), needs_boundary_fix));
+  }
+  void require_source_components() { check(mp_fields_require_source_components(p)); } // :1907
   void step() { check(mp_fields_step(p)); }                // :1871
   double time() const { return mp_fields_time(p); }        // :1877
   double round_time() const { return mp_fields_round_time(p); } // :1876

@@ -133,6 +133,12 @@ _SIGNATURES = {
     "mp_dft_fields_chunk_points": (C.c_long, [C.c_void_p, C.c_int]),
     "mp_dft_fields_chunk_component": (C.c_int, [C.c_void_p, C.c_int]),
     "mp_dft_fields_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_fields_add_volume_source_array": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SrcTime), C.POINTER(C.c_double),
+                                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_size_t,
+                                                    C.c_size_t, C.c_size_t, C.c_double, C.c_double]),
+    "mp_fields_add_srcdata": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SrcTime), C.c_long, C.POINTER(C.c_long),
+                                        C.POINTER(C.c_double), C.c_int]),
+    "mp_fields_require_source_components": (C.c_int, [C.c_void_p]),
     "mp_fields_add_flux_vol": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mp_flux_vol_delete": (None, [C.c_void_p]),
     "mp_flux_vol_flux": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
@@ -500,6 +506,32 @@ class fields:
         self._keep.append(cb)
         _check(self.lib, self.lib.mp_fields_add_volume_source(
             self.ptr, c, C.byref(src), _dvec(vmin), _dvec(vmax), amp.real, amp.imag, cb, None))
+
+    def add_volume_source_array(self, c, src, vmin, vmax, arr, amp=1.0):
+        """add_volume_source(c, src, where, complex<double> *arr, dim1, dim2, dim3, amp)
+        (src/sources.cpp:394-417): arr, up to three dimensions, is interpolated over the
+        volume."""
+        amp = complex(amp)
+        a = np.asarray(arr, dtype=np.complex128)
+        shape = list(a.shape) + [1] * (3 - a.ndim)
+        flat = np.ascontiguousarray(np.stack([a.real.ravel(), a.imag.ravel()], axis=1).ravel())
+        _check(self.lib, self.lib.mp_fields_add_volume_source_array(
+            self.ptr, c, C.byref(src), _dvec(vmin), _dvec(vmax), _dptr(flat), shape[0], shape[1], shape[2],
+            amp.real, amp.imag))
+
+    def add_srcdata(self, c, src, index, amps, needs_boundary_fix=False):
+        """fields::add_srcdata (src/sources.cpp:314-334): a source on the given points of c's
+        array (reference-layout indices) with the given complex amplitudes."""
+        idx = np.ascontiguousarray(index, dtype=np.int64)
+        a = np.asarray(amps, dtype=np.complex128).ravel()
+        flat = np.ascontiguousarray(np.stack([a.real, a.imag], axis=1).ravel())
+        _check(self.lib, self.lib.mp_fields_add_srcdata(
+            self.ptr, c, C.byref(src), len(idx), idx.ctypes.data_as(C.POINTER(C.c_long)), _dptr(flat),
+            int(needs_boundary_fix)))
+
+    def require_source_components(self):
+        """fields::require_source_components (src/fields.cpp:508-530), with fix_boundary_sources."""
+        _check(self.lib, self.lib.mp_fields_require_source_components(self.ptr))
 
     def add_point_source_src(self, c, src, p, amp=1.0):
         self.add_volume_source(c, src, p, p, amp)

@@ -338,6 +338,9 @@ struct FieldsBase {
   virtual void use_bloch(int d, cdouble k) = 0;
   virtual void set_boundary(int side, int d, int cond) = 0;
   virtual void add_volume_source(int c, const SrcTime &st, V3 wmin, V3 wmax, cdouble amp, mp_complex_func A, void *Au) = 0;
+  virtual void add_srcdata(int c, const SrcTime &st, const long *ref_index, long n, const double *amp, bool fix) = 0;
+  virtual void require_source_components() = 0;
+  virtual int dimension() const = 0;
   virtual void step_n(long n, bool check_each) = 0;
   // step_n with the source rows staged in `rows` (device memory of rows_bytes) and no device
   // allocation or release
@@ -3187,6 +3190,113 @@ template <class R> struct Fields : FieldsBase {
     require_component(c);
   }
 
+  // a source list of component c at the points with array coordinates j3 (3 per point), as
+  // add_volume_source stores one: device indices, a current in a conducting medium scaled by
+  // condinv at its point
+  void add_points(int c, int isrc, const std::vector<int> &j3, std::vector<cdouble> amps) {
+    const int ft = is_magnetic(c) ? B_ft : D_ft;
+    const int cD = dir_comp(ft == B_ft ? 15 : 10, comp_dir(c));
+    SrcVol sv;
+    sv.c = c;
+    sv.src = isrc;
+    for (size_t q = 0; q < amps.size(); ++q) {
+      const int j[3] = {j3[3 * q], j3[3 * q + 1], j3[3 * q + 2]};
+      sv.index.push_back(gv.dev_of(j));
+      for (int d = 0; d < 3; ++d) sv.jpos.push_back(j[d]);
+      if (!srcs[isrc].is_integrated && !h_condinv[cD].empty()) amps[q] *= h_condinv[cD][gv.ref_of(j)];
+    }
+    sv.amp = std::move(amps);
+    ++src_version;
+    sources[ft].push_back(std::move(sv));
+  }
+  // fields::add_srcdata (src/sources.cpp:314-334): a source on points of component c's array
+  // (reference-layout indices) with the amplitudes as given.  A point the chunk does not own
+  // waits for require_source_components when the list is marked for the boundary fix, and is
+  // dropped otherwise: the halo exchange after the source would overwrite it.
+  struct PendingFix {
+    int c, src;
+    std::vector<int> j3;
+    std::vector<cdouble> amp;
+  };
+  std::vector<PendingFix> pending_fix;
+  void array_coords(int64_t ref, int j[3]) const {
+    for (int d = 0; d < 3; ++d) j[d] = 0;
+    int order[3] = {0, 1, 2};
+    std::sort(order, order + 3, [&](int u, int v) { return gv.rst[u] > gv.rst[v]; });
+    for (int d : order)
+      if (gv.has(d)) {
+        j[d] = int(ref / gv.rst[d]);
+        ref %= gv.rst[d];
+      }
+  }
+  void add_srcdata(int c, const SrcTime &st, const long *ref, long n, const double *amp, bool fix) override {
+    if (c < 0 || c >= NC || !(is_electric(c) || is_magnetic(c)) || comp_dir(c) == NODIR || !has_field(c))
+      fail("add_srcdata: invalid component %d", c);
+    if (n < 0) fail("add_srcdata: negative point count");
+    const int isrc = add_src_time(st);
+    std::vector<int> jown;
+    std::vector<cdouble> aown;
+    PendingFix pf{c, isrc, {}, {}};
+    for (long q = 0; q < n; ++q) {
+      if (ref[q] < 0 || size_t(ref[q]) >= gv.rntot) fail("add_srcdata: point outside the array");
+      int j[3];
+      array_coords(ref[q], j);
+      const cdouble A(amp[2 * q], amp[2 * q + 1]);
+      if (gv.owns(gv.iloc(c, j))) {
+        jown.insert(jown.end(), j, j + 3);
+        aown.push_back(A);
+      }
+      else if (fix) {
+        pf.j3.insert(pf.j3.end(), j, j + 3);
+        pf.amp.push_back(A);
+      }
+    }
+    if (!aown.empty()) add_points(c, isrc, jown, std::move(aown));
+    if (!pf.amp.empty()) pending_fix.push_back(std::move(pf));
+  }
+  // fields::require_source_components (src/fields.cpp:508-530) with fix_boundary_sources
+  // (src/fix_boundary_sources.cpp:51-197): each waiting point with a nonzero amplitude moves to
+  // the point that owns it (its periodic image, the amplitude times the conjugate phase)
+  // unless that point is on a metal boundary; the moved points form one list per source time
+  // and component; every source component is allocated
+  int dimension() const override { return gv.dim; }
+  void require_source_components() override {
+    struct Moved {
+      int src, c;
+      int j[3];
+      cdouble A;
+    };
+    std::vector<Moved> moved;
+    for (const PendingFix &pf : pending_fix)
+      for (size_t q = 0; q < pf.amp.size(); ++q) {
+        if (pf.amp[q] == 0.0) continue;
+        IV here = gv.iloc(pf.c, &pf.j3[3 * q]);
+        cdouble ph = 1.0;
+        if (!locate(&here, &ph) || on_metal_boundary(here)) continue;
+        Moved m;
+        m.src = pf.src;
+        m.c = pf.c;
+        for (int d = 0; d < 3; ++d) m.j[d] = gv.has(d) ? (here.t[d] - gv.ysh(pf.c, d)) / 2 : 0;
+        m.A = pf.amp[q] * std::conj(ph);
+        moved.push_back(m);
+      }
+    pending_fix.clear();
+    std::stable_sort(moved.begin(), moved.end(), [](const Moved &a, const Moved &b) { return a.src != b.src ? a.src < b.src : a.c < b.c; });
+    for (size_t q0 = 0; q0 < moved.size();) {
+      std::vector<int> j3;
+      std::vector<cdouble> amps;
+      size_t q1 = q0;
+      for (; q1 < moved.size() && moved[q1].src == moved[q0].src && moved[q1].c == moved[q0].c; ++q1) {
+        j3.insert(j3.end(), moved[q1].j, moved[q1].j + 3);
+        amps.push_back(moved[q1].A);
+      }
+      add_points(moved[q0].c, moved[q0].src, j3, std::move(amps));
+      q0 = q1;
+    }
+    for (int ft : {D_ft, B_ft})
+      for (const SrcVol &sv : sources[ft]) require_component(sv.c);
+  }
+
   int auto_decimation(const std::vector<double> &freq) const {
     double src_freq_max = 0;
     for (const SrcTime &st : srcs)
@@ -4125,6 +4235,72 @@ int mp_fields_add_volume_source(mp_fields *f, int c, const mp_src_time *src, con
     st.is_integrated = src->is_integrated != 0;
     F_->add_volume_source(c, st, to_v3(vmin), to_v3(vmax), cdouble(ar, ai), A, Au);
   });
+}
+static SrcTime src_time_of(const mp_src_time *src) {
+  SrcTime st = src->kind == 0 ? SrcTime::gaussian(src->freq, src->width, src->start_time, src->end_time)
+                              : SrcTime::continuous(src->freq, src->width, src->start_time, src->end_time, src->slowness);
+  st.is_integrated = src->is_integrated != 0;
+  return st;
+}
+// amp_file_func (src/sources.cpp:345-376) over a caller's array: the point's position
+// relative to the volume's centre mapped to [0, 1] along each direction the volume spans,
+// then linearly interpolated with mirror boundaries (map_coordinates and linear_interpolate,
+// src/fields.cpp:754-810)
+struct AmpGrid {
+  const double *v; // interleaved (re, im), row major
+  int n[3];
+  double span[3];
+};
+static int mirror_at(int i, int n) { return i >= n ? 2 * n - 1 - i : (i < 0 ? -1 - i : i); }
+static double amp_grid_value(const AmpGrid &G, const double r_in[3], int part) {
+  int lo[3], hi[3];
+  double w[3];
+  for (int d = 0; d < 3; ++d) {
+    double r = r_in[d];
+    r = r < 0.0 ? -r : (r > 1.0 ? 1.0 - r : r);
+    const int n = G.n[d];
+    lo[d] = mirror_at(int(r * n), n);
+    const double off = r * n - lo[d] - 0.5;
+    hi[d] = mirror_at(off >= 0.0 ? lo[d] + 1 : lo[d] - 1, n);
+    w[d] = std::fabs(off);
+  }
+  auto at = [&](int x, int y, int z) { return G.v[2 * ((size_t(x) * G.n[1] + y) * G.n[2] + z) + part]; };
+  const double z0 = (at(lo[0], lo[1], lo[2]) * (1.0 - w[0]) + at(hi[0], lo[1], lo[2]) * w[0]) * (1.0 - w[1]) +
+                    (at(lo[0], hi[1], lo[2]) * (1.0 - w[0]) + at(hi[0], hi[1], lo[2]) * w[0]) * w[1];
+  const double z1 = (at(lo[0], lo[1], hi[2]) * (1.0 - w[0]) + at(hi[0], lo[1], hi[2]) * w[0]) * (1.0 - w[1]) +
+                    (at(lo[0], hi[1], hi[2]) * (1.0 - w[0]) + at(hi[0], hi[1], hi[2]) * w[0]) * w[1];
+  return z0 * (1.0 - w[2]) + z1 * w[2];
+}
+static void amp_grid_func(const double *rel, void *user, double *re, double *im) {
+  const AmpGrid &G = *static_cast<const AmpGrid *>(user);
+  double r[3];
+  for (int d = 0; d < 3; ++d) r[d] = G.span[d] == 0 ? 0 : 0.5 + rel[d] / G.span[d];
+  *re = amp_grid_value(G, r, 0);
+  *im = amp_grid_value(G, r, 1);
+}
+int mp_fields_add_volume_source_array(mp_fields *f, int c, const mp_src_time *src, const double *vmin, const double *vmax,
+                                      const double *arr, size_t dim1, size_t dim2, size_t dim3, double ar, double ai) {
+  return guard([&] {
+    if (!arr || dim1 == 0 || dim2 == 0 || dim3 == 0) fail("add_volume_source: empty amplitude array");
+    if (dim1 > (size_t(1) << 30) || dim2 > (size_t(1) << 30) || dim3 > (size_t(1) << 30)) fail("add_volume_source: amplitude array too large");
+    const V3 mn = to_v3(vmin), mx = to_v3(vmax);
+    AmpGrid G;
+    G.v = arr;
+    G.n[0] = int(dim1), G.n[1] = int(dim2), G.n[2] = int(dim3);
+    const int dim = F_->dimension();
+    for (int d = 0; d < 3; ++d) {
+      const bool spans = dim == 1 ? d == 2 : (dim == 2 ? d != 2 : true);
+      G.span[d] = spans ? mx.t[d] - mn.t[d] : 0.0;
+    }
+    F_->add_volume_source(c, src_time_of(src), mn, mx, cdouble(ar, ai), amp_grid_func, &G);
+  });
+}
+int mp_fields_add_srcdata(mp_fields *f, int c, const mp_src_time *src, long n, const long *index, const double *amp,
+                          int needs_boundary_fix) {
+  return guard([&] { F_->add_srcdata(c, src_time_of(src), index, n, amp, needs_boundary_fix != 0); });
+}
+int mp_fields_require_source_components(mp_fields *f) {
+  return guard([&] { F_->require_source_components(); });
 }
 int mp_fields_step(mp_fields *f) {
   return guard([&] { F_->step_n(1, true); });

@@ -65,6 +65,8 @@
  *                     the one after the H update
  *   MUT_PML_INTEGRAL  a uPML term's sigma ignores its profile's integral (takes 1/3)
  *   MUT_PML_STRETCH   a uPML term's kappa divides by the profile's integral, not that of u profile(u)
+ *   MUT_AMPARR_OFFSET an amplitude array is interpolated without its half-cell offset
+ *   MUT_FIXBND_PHASE  a source point moved to its periodic owner keeps the phase, not its conjugate
  */
 #include "../include/meep_amd.h"
 
@@ -217,6 +219,18 @@ struct GV {
     for (int d = 0; d < 3; ++d)
       if (has(d) && (p.t[d] < -inva || p.t[d] > num[d] * inva + inva)) return false;
     return true;
+  }
+  // grid_volume::iloc(c, index) (src/vec.cpp:547-556): the point of c's array at index
+  ivec iloc_of(int c, ptrdiff_t index) const {
+    ivec r, s = iyee_shift(c);
+    int order[3] = {X, Y, Z};
+    std::sort(order, order + 3, [&](int u, int v) { return st[u] > st[v]; });
+    for (int d : order)
+      if (has(d)) {
+        r.t[d] = int(index / st[d]) * 2 + s.t[d];
+        index %= st[d];
+      }
+    return r;
   }
   ptrdiff_t index(int c, const ivec &p) const {
     ivec s = iyee_shift(c);
@@ -1660,6 +1674,113 @@ template <class R> struct Fields {
     require_component(c);
   }
 
+  // fields::add_volume_source with an amplitude array (src/sources.cpp:394-417): the
+  // amplitude at a point is amp_file_func of its position relative to the volume's centre
+  // (src/sources.cpp:345-376): each direction the volume spans mapped to [0, 1], then
+  // linear_interpolate over the row-major n0 x n1 x n2 array with mirror boundaries
+  // (src/fields.cpp:754-810)
+  struct AmpArray {
+    const double *v; // interleaved (re, im)
+    int n[3];
+    double size[3];
+  };
+  static int mirrorindex(int i, int n) { return i >= n ? 2 * n - 1 - i : (i < 0 ? -1 - i : i); }
+  static double linear_interpolate(double rx, double ry, double rz, const double *data, int nx, int ny, int nz, int stride) {
+    rx = rx < 0.0 ? -rx : (rx > 1.0 ? 1.0 - rx : rx);
+    ry = ry < 0.0 ? -ry : (ry > 1.0 ? 1.0 - ry : ry);
+    rz = rz < 0.0 ? -rz : (rz > 1.0 ? 1.0 - rz : rz);
+    const int x1 = mirrorindex(int(rx * nx), nx), y1 = mirrorindex(int(ry * ny), ny), z1 = mirrorindex(int(rz * nz), nz);
+#ifdef MUT_AMPARR_OFFSET
+    double dx = rx * nx - x1, dy = ry * ny - y1, dz = rz * nz - z1;
+#else
+    double dx = rx * nx - x1 - 0.5, dy = ry * ny - y1 - 0.5, dz = rz * nz - z1 - 0.5;
+#endif
+    const int x2 = mirrorindex(dx >= 0.0 ? x1 + 1 : x1 - 1, nx), y2 = mirrorindex(dy >= 0.0 ? y1 + 1 : y1 - 1, ny),
+              z2 = mirrorindex(dz >= 0.0 ? z1 + 1 : z1 - 1, nz);
+    dx = fabs(dx);
+    dy = fabs(dy);
+    dz = fabs(dz);
+    auto D = [&](int x, int y, int z) { return data[(((size_t(x) * ny + y) * nz) + z) * stride]; };
+    return (((D(x1, y1, z1) * (1.0 - dx) + D(x2, y1, z1) * dx) * (1.0 - dy) + (D(x1, y2, z1) * (1.0 - dx) + D(x2, y2, z1) * dx) * dy) *
+                (1.0 - dz) +
+            ((D(x1, y1, z2) * (1.0 - dx) + D(x2, y1, z2) * dx) * (1.0 - dy) + (D(x1, y2, z2) * (1.0 - dx) + D(x2, y2, z2) * dx) * dy) *
+                dz);
+  }
+  static void amp_file_func(const double *rel, void *user, double *re, double *im) {
+    const AmpArray &A = *static_cast<const AmpArray *>(user);
+    double r[3];
+    for (int d = 0; d < 3; ++d) r[d] = A.size[d] == 0 ? 0 : 0.5 + rel[d] / A.size[d];
+    *re = linear_interpolate(r[0], r[1], r[2], A.v, A.n[0], A.n[1], A.n[2], 2);
+    *im = linear_interpolate(r[0], r[1], r[2], A.v + 1, A.n[0], A.n[1], A.n[2], 2);
+  }
+  void add_volume_source_array(int c, const SrcTime &st, vec3 wmin, vec3 wmax, const double *arr, const size_t *dims, cdouble amp) {
+    AmpArray A;
+    A.v = arr;
+    for (int k = 0; k < 3; ++k) A.n[k] = int(dims[k]);
+    // the volume's extent in x, y, z as amp_file_func reads it: Z alone in 1D, X and Y in 2D
+    for (int d = 0; d < 3; ++d) A.size[d] = gv.has(d) ? wmax.t[d] - wmin.t[d] : 0.0;
+    add_volume_source(c, st, wmin, wmax, amp, amp_file_func, &A);
+  }
+
+  // fields::add_srcdata (src/sources.cpp:314-334): a source on given points of component c's
+  // array with the given amplitudes; a list marked needs_boundary_fix waits for
+  // require_source_components
+  std::vector<std::pair<int, size_t>> boundary_fix; // (field type, list)
+  void add_srcdata(int c, const SrcTime &st, std::vector<ptrdiff_t> index, std::vector<cdouble> amp, bool fix) {
+    if (c < 0 || c >= NC || !(is_electric(c) || is_magnetic(c)) || component_direction(c) == NODIR || !has_field(c))
+      fail("add_srcdata: invalid component %d", c);
+    for (ptrdiff_t i : index)
+      if (i < 0 || size_t(i) >= gv.ntot) fail("add_srcdata: point outside the array");
+    const int isrc = add_src_time(st);
+    const int ft = is_magnetic(c) ? B_stuff : D_stuff;
+    sources[ft].push_back(SrcVol{c, isrc, std::move(index), std::move(amp)});
+    if (fix) boundary_fix.push_back({ft, sources[ft].size() - 1});
+  }
+  // fields::require_source_components (src/fields.cpp:508-530) and fix_boundary_sources
+  // (src/fix_boundary_sources.cpp:51-197) for one chunk: each point of a marked list that the
+  // chunk does not own, with a nonzero amplitude, moves to the point that owns it
+  // (locate_component_point: its periodic image, the amplitude times the conjugate phase)
+  // unless that point is on a metal boundary, and its amplitude in the list becomes zero.
+  // The moved points form new lists, one per source time and component, sorted by them.
+  void require_source_components() {
+    struct Moved {
+      int src, c;
+      ptrdiff_t index;
+      cdouble A;
+    };
+    std::vector<Moved> moved;
+    for (auto &fx : boundary_fix) {
+      SrcVol &sv = sources[fx.first][fx.second];
+      for (size_t j = 0; j < sv.index.size(); ++j) {
+        ivec here = gv.iloc_of(sv.c, sv.index[j]);
+        if (gv.owns(here) || sv.amp[j] == 0.0) continue;
+        cdouble phase = 1.0;
+        if (locate_point_in_user_volume(&here, &phase) && !on_metal_boundary(here))
+#ifdef MUT_FIXBND_PHASE
+          moved.push_back(Moved{sv.src, sv.c, gv.index(sv.c, here), sv.amp[j] * phase});
+#else
+          moved.push_back(Moved{sv.src, sv.c, gv.index(sv.c, here), sv.amp[j] * conj(phase)});
+#endif
+        sv.amp[j] = 0.0;
+      }
+    }
+    boundary_fix.clear();
+    std::stable_sort(moved.begin(), moved.end(), [](const Moved &a, const Moved &b) { return a.src != b.src ? a.src < b.src : a.c < b.c; });
+    for (size_t q0 = 0; q0 < moved.size();) {
+      size_t q1 = q0;
+      SrcVol sv{moved[q0].c, moved[q0].src, {}, {}};
+      while (q1 < moved.size() && moved[q1].src == sv.src && moved[q1].c == sv.c) {
+        sv.index.push_back(moved[q1].index);
+        sv.amp.push_back(moved[q1].A);
+        ++q1;
+      }
+      sources[is_magnetic(sv.c) ? B_stuff : D_stuff].push_back(std::move(sv));
+      q0 = q1;
+    }
+    for (int ft : {D_stuff, B_stuff})
+      for (const SrcVol &sv : sources[ft]) require_component(sv.c);
+  }
+
   // fields::add_dft (src/dft.cpp:175-235) for one component, on the centred grid or (centered
   // false) on the component's own grid without the centring average (src/dft.cpp:91-94,233)
   void add_dft(std::vector<std::unique_ptr<DftChunk<R>>> &out_list, int c, vec3 wmin, vec3 wmax, const std::vector<double> &freq,
@@ -2540,6 +2661,46 @@ int mp_fields_energy_in_box(mp_fields *f, int which, const double *vmin, const d
       if (which < 0 || which > 2) fail("energy_in_box: which must be 0 (field), 1 (electric) or 2 (magnetic)");
       const vec3 mn = to_vec(vmin), mx = to_vec(vmax);
       *out = which == 0 ? F.field_energy_in_box(mn, mx) : double(F.energy_in_box(which == 1 ? 0 : 5, mn, mx));
+      return 0;
+    });
+  });
+}
+
+static SrcTime src_time_of(const mp_src_time *src) {
+  SrcTime st = src->kind == 0 ? SrcTime::gaussian(src->freq, src->width, src->start_time, src->end_time)
+                              : SrcTime::continuous(src->freq, src->width, src->start_time, src->end_time, src->slowness);
+  st.is_integrated = src->is_integrated != 0;
+  return st;
+}
+int mp_fields_add_volume_source_array(mp_fields *f, int c, const mp_src_time *src, const double *vmin, const double *vmax,
+                                      const double *arr, size_t dim1, size_t dim2, size_t dim3, double amp_re,
+                                      double amp_im) {
+  return guard([&] {
+    if (!arr || dim1 == 0 || dim2 == 0 || dim3 == 0) fail("add_volume_source: empty amplitude array");
+    const size_t dims[3] = {dim1, dim2, dim3};
+    with_fields(reinterpret_cast<AnyFields *>(f), [&](auto &F) {
+      F.add_volume_source_array(c, src_time_of(src), to_vec(vmin), to_vec(vmax), arr, dims, cdouble(amp_re, amp_im));
+      return 0;
+    });
+  });
+}
+int mp_fields_add_srcdata(mp_fields *f, int c, const mp_src_time *src, long n, const long *index, const double *amp,
+                          int needs_boundary_fix) {
+  return guard([&] {
+    if (n < 0) fail("add_srcdata: negative point count");
+    std::vector<ptrdiff_t> idx(index, index + n);
+    std::vector<cdouble> a(static_cast<size_t>(n));
+    for (long q = 0; q < n; ++q) a[size_t(q)] = cdouble(amp[2 * q], amp[2 * q + 1]);
+    with_fields(reinterpret_cast<AnyFields *>(f), [&](auto &F) {
+      F.add_srcdata(c, src_time_of(src), std::move(idx), std::move(a), needs_boundary_fix != 0);
+      return 0;
+    });
+  });
+}
+int mp_fields_require_source_components(mp_fields *f) {
+  return guard([&] {
+    with_fields(reinterpret_cast<AnyFields *>(f), [&](auto &F) {
+      F.require_source_components();
       return 0;
     });
   });

@@ -53,6 +53,16 @@ run against a value the mathematics fixes; the tests assert on these errors.
   continuum, the coordinate map x -> int kappa dx, i.e. a layer of vacuum s d thick: the
   fields in front of it equal those of a cell longer by (s - 1) d, up to the lattice's
   O(dx^2) error.
+* ``source_array``: a line source whose amplitudes come from an array (add_volume_source with
+  arr, src/sources.cpp:394-417) of values 0, 1, ..., n - 1 equals the same line with the
+  amplitude function the array's linear interpolation defines: the position mapped to
+  r = 0.5 + rel / size, the value clamp(n r - 1/2, 0, n - 1) (cell-centred samples, mirror
+  boundaries).
+* ``srcdata_moves``: add_srcdata at the point and amplitude add_volume_source gives a point
+  source (amplitude a^2 in 2D) reproduces it; a point on the periodic halo marked for the
+  boundary fix equals the point that owns it with the amplitude times exp(i k L) (the
+  conjugate of the phase that locates it), and one not marked, or with no owner behind a
+  metal wall, is dropped.
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -313,6 +323,66 @@ def pml_stretch(lib, a=40, d=1.0, s=2.0):
     A = _pml_line(lib, 10.0, [(Z, Low, 2.0, 1e-15), (Z, High, d, 1.0, s)], a)
     B = _pml_line(lib, 10.0 + (s - 1) * d, [(Z, Low, 2.0, 1e-15)], a)
     return float(np.abs(A - B).max() / np.abs(B).max())
+
+
+def _src_run(lib, setup, nsteps=120, k=None, metal_x=False):
+    s = structure(lib, dim=2, size=(3.0, 2.5), a=10)
+    f = fields(s)
+    if k is None:
+        f.use_real_fields()
+        f.use_bloch((0.0, 0.0))
+    else:
+        f.use_bloch((0.0, k))
+    if metal_x:
+        f.set_boundary(Low, X, Metallic)
+        f.set_boundary(High, X, Metallic)
+    setup(s, f)
+    f.step_n(nsteps)
+    return f.get_array(Ez, 0)[0], (f.get_array(Ez, 1)[0] if k is not None else np.zeros(s.shape))
+
+
+def source_array(lib, n=7):
+    """Relative difference between a line source with an amplitude array and the same line
+    with the amplitude function the array's interpolation defines."""
+    src = gaussian_src_time(1.0, 0.5)
+    # ends on grid points, so every source point lies inside the volume (0 <= r <= 1)
+    lo, hi = (1.3, 0.3, 0), (1.3, 2.2, 0)
+    data = np.arange(n, dtype=float) * (1.0 + 0.5j)
+    def fn(x, y, z):
+        r = 0.5 + y / (hi[1] - lo[1])
+        return min(max(n * r - 0.5, 0.0), n - 1.0) * (1.0 + 0.5j)
+    A = _src_run(lib, lambda s, f: f.add_volume_source_array(Ez, src, lo, hi, data.reshape(1, n, 1)), k=0.0)
+    B = _src_run(lib, lambda s, f: f.add_volume_source(Ez, src, lo, hi, amp_func=fn), k=0.0)
+    scale = max(np.abs(B[0]).max(), np.abs(B[1]).max())
+    return float(max(np.abs(A[0] - B[0]).max(), np.abs(A[1] - B[1]).max()) / scale)
+
+
+def srcdata_moves(lib, k=0.3):
+    """Errors of add_srcdata against the sources it stands for (see the module docstring)."""
+    src = gaussian_src_time(1.0, 0.5)
+    a = 10
+    err = {}
+    def ref_index(s, i, j):
+        return i * s.shape[1] + j  # Ez sits at integer points; Y is the stride-1 direction
+    # a point source at grid point (13, 11) / a: one point of amplitude a^2
+    P = _src_run(lib, lambda s, f: f.add_volume_source(Ez, src, (1.3, 1.1, 0), (1.3, 1.1, 0)), k=k)
+    Q = _src_run(lib, lambda s, f: (f.add_srcdata(Ez, src, [ref_index(s, 13, 11)], [a * a]), f.require_source_components()), k=k)
+    scale = max(np.abs(P[0]).max(), np.abs(P[1]).max())
+    err["point"] = float(max(np.abs(P[0] - Q[0]).max(), np.abs(P[1] - Q[1]).max()) / scale)
+    # the halo point y = 0 of a periodic y with Bloch k: its owner is y = L with exp(i k L)
+    L = 2.5
+    ph = np.exp(2j * math.pi * k * L)
+    H = _src_run(lib, lambda s, f: (f.add_srcdata(Ez, src, [ref_index(s, 13, 0)], [a * a], True), f.require_source_components()), k=k)
+    O = _src_run(lib, lambda s, f: (f.add_srcdata(Ez, src, [ref_index(s, 13, 25)], [a * a * ph]), f.require_source_components()), k=k)
+    scale = max(np.abs(O[0]).max(), np.abs(O[1]).max())
+    err["moved"] = float(max(np.abs(H[0] - O[0]).max(), np.abs(H[1] - O[1]).max()) / scale)
+    # not marked: dropped; behind a metal wall (x = 0, no periodic owner): dropped
+    D = _src_run(lib, lambda s, f: (f.add_srcdata(Ez, src, [ref_index(s, 13, 0)], [a * a]), f.require_source_components()), k=k)
+    M = _src_run(lib, lambda s, f: (f.add_srcdata(Ez, src, [ref_index(s, 0, 11)], [a * a], True), f.require_source_components()),
+                 k=k, metal_x=True)
+    err["unmarked"] = float(max(np.abs(D[0]).max(), np.abs(D[1]).max()) / scale)
+    err["metal"] = float(max(np.abs(M[0]).max(), np.abs(M[1]).max()) / scale)
+    return err
 
 
 def source_spectrum(dt=0.025, fcen=1.0, fwidth=0.5, freqs=None):

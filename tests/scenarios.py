@@ -231,6 +231,27 @@ def stretched_pml_3d(lib, single=False, a=8, real=True):
     return s, f, [fl]
 
 
+def data_sources_3d(lib, single=False, a=8, real=False):
+    """Complex Bloch fields in a periodic 3D cell driven by sources given as data: a z-plane of
+    Ex whose amplitudes come from a 5 x 4 array (add_volume_source with arr), and Ey points
+    given by add_srcdata on owned points and on halo points moved to their owners by
+    require_source_components."""
+    L = 1.5
+    s = structure(lib, dim=3, size=(L, L, L), a=a, eps=lambda x, y, z: 2.0 if z < 0.7 else 1.0, single=single)
+    f = fields(s)
+    f.use_bloch((0.2, 0.0, 0.35))
+    arr = (np.arange(20).reshape(5, 4) * (0.3 + 0.1j) + 1.0)[:, :, None]
+    f.add_volume_source_array(Ex, gaussian_src_time(0.9, 0.4), (0.0, 0.0, 0.9), (L, L, 0.9), arr)
+    n = s.num
+    def idx(i, j, k):  # Ey (reference layout: X slowest, Z stride 1)
+        return (i * (n[1] + 1) + j) * (n[2] + 1) + k
+    f.add_srcdata(Ey, gaussian_src_time(1.1, 0.5), [idx(3, 4, 5), idx(0, 5, 6), idx(7, 2, 0)],
+                  [64.0, 32.0 + 8.0j, -16.0], True)
+    f.require_source_components()
+    fl = f.add_dft_flux(Z, (0.0, 0.0, 1.2), (L, L, 1.2), linspace(0.7, 1.2, 4))
+    return s, f, [fl]
+
+
 def straight_waveguide(lib, single=False, real=True):
     """The 2D straight-waveguide configuration at full size (python/examples/
     straight-waveguide.py's geometry): 16x8 at resolution 20 (320x160 cells), eps = 12 for
@@ -262,6 +283,7 @@ SCENARIOS = {
     "conductive_2d_tm": (conductive_2d_tm, 200),
     "conductive_3d": (conductive_3d, 100),
     "stretched_pml_3d": (stretched_pml_3d, 100),
+    "data_sources_3d": (data_sources_3d, 80),
 }
 
 FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)

@@ -215,3 +215,50 @@ def test_cpp_api_flux_planes(tmp_path, oracle_lib):
     want = RF.flux_1d(oracle_lib, 100.0, 20.0)
     assert got == [want[0], want[1]], (got, want)
 
+
+DATA_SOURCES_PROGRAM = r"""
+#include "meep_amd.hpp"
+#include <cstdio>
+using namespace meep;
+int main() {
+  structure s(voltwo(3.0, 2.5, 10.0), nullptr);
+  fields f(&s);
+  f.use_bloch(vec(0.0, 0.3));
+  gaussian_src_time src(1.0, 0.5);
+  src.is_integrated = false;
+  std::complex<double> arr[7];
+  for (int i = 0; i < 7; ++i) arr[i] = std::complex<double>(i, 0.5 * i);
+  f.add_volume_source(Ez, src, volume(vec(1.3, 0.3), vec(1.3, 2.2)), arr, 1, 7, 1, 1.0);
+  sourcedata sd{Ez, {13 * 26 + 0}, 0, {std::complex<double>(100.0, 0.0)}};
+  f.add_srcdata(sd, &src, 0, nullptr, true);
+  f.require_source_components();
+  for (int i = 0; i < 120; ++i) f.step();
+  std::printf("%.17g %.17g\n", f.get_field(Ez, vec(1.7, 1.1)).real(), f.get_field(Ez, vec(1.7, 1.1)).imag());
+  return 0;
+}
+"""
+
+
+def test_cpp_api_data_sources(tmp_path, oracle_lib):
+    """The C++ API's amplitude-array add_volume_source, add_srcdata and
+    require_source_components give the C-ABI's fields."""
+    src = tmp_path / "data_sources.cpp"
+    src.write_text(DATA_SOURCES_PROGRAM)
+    out = str(tmp_path / "data_sources")
+    mbuild.build_cpp_program(str(src), out, os.path.join(ROOT, "oracle", "_build"), "meep_oracle")
+    r = subprocess.run([out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = complex(*[float(x) for x in r.stdout.split()])
+    from meep_amd.capi import Ez as cEz, fields, gaussian_src_time, structure
+    s = structure(oracle_lib, dim=2, size=(3.0, 2.5), a=10)
+    f = fields(s)
+    f.use_bloch((0.0, 0.3))
+    g = gaussian_src_time(1.0, 0.5)
+    f.add_volume_source_array(cEz, g, (1.3, 0.3, 0), (1.3, 2.2, 0), (np.arange(7) * (1 + 0.5j)).reshape(1, 7, 1))
+    f.add_srcdata(cEz, g, [13 * 26 + 0], [100.0], True)
+    f.require_source_components()
+    for _ in range(120):
+        f.step()
+    want = f.get_field(cEz, (1.7, 1.1, 0))
+    assert got != 0 and abs(got - want) <= 1e-15 * abs(want), (got, want)
+

@@ -58,3 +58,11 @@ def test_pml_reflects_its_asymptotic_reflection_gpu(gpu_lib):
 
 def test_pml_stretch_is_a_longer_vacuum_gpu(gpu_lib):
     T.check_pml_stretch(gpu_lib)
+
+
+def test_amplitude_array_source_gpu(gpu_lib):
+    T.check_source_array(gpu_lib)
+
+
+def test_srcdata_and_boundary_fix_gpu(gpu_lib):
+    T.check_srcdata(gpu_lib)

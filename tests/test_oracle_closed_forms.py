@@ -35,6 +35,10 @@ values do not reach (tests/closed_forms.py states each closed form):
   its integrals, and the default quadratic one; 1-2 % at 20 points per unit), and a lossless
   layer of mean stretch s acts as s times its thickness of vacuum (5 % at 40 points per unit,
   falling as dx^2).
+* Sources given as data: an amplitude array equals the amplitude function of its linear
+  interpolation, and add_srcdata reproduces a point source, moves a marked halo point to its
+  periodic owner with the conjugate phase, and drops unmarked or ownerless halo points
+  (exact).
 * Transport and spectrum shape: equal flux through two planes of a lossless 1D line, and a
   flux spectrum proportional to |J(omega)|^2.  These two are scale-invariant: they do not
   pin the DFT's normalisation, phase or E/H time offset (the first three checks do).
@@ -54,7 +58,7 @@ from meep_amd.capi import (Ex, Ez, MeepError, Z, fields, gaussian_src_time, lins
 
 TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
        "pole_decay": 1e-4, "cavity": 1e-10, "cond_decay": 1e-12, "cond_pml_line": 1e-6,
-       "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12, "pml_refl": 0.03, "pml_stretch": 0.08}
+       "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12, "pml_refl": 0.03, "pml_stretch": 0.08, "sources": 1e-13}
 
 
 def check_line_source(lib):
@@ -123,6 +127,15 @@ def check_pml_stretch(lib):
     assert CF.pml_stretch(lib) <= TOL["pml_stretch"]
 
 
+def check_source_array(lib):
+    assert CF.source_array(lib) <= TOL["sources"]
+
+
+def check_srcdata(lib):
+    err = CF.srcdata_moves(lib)
+    assert max(err.values()) <= TOL["sources"], err
+
+
 def test_line_source_matches_lattice_solution(oracle_lib):
     check_line_source(oracle_lib)
 
@@ -184,6 +197,14 @@ def test_pml_stretch_is_a_longer_vacuum(oracle_lib):
     # the lattice's error of the equivalence falls as dx^2
     e40, e80 = CF.pml_stretch(oracle_lib, a=40), CF.pml_stretch(oracle_lib, a=80)
     assert 0.18 < e80 / e40 < 0.32, (e40, e80)
+
+
+def test_amplitude_array_source(oracle_lib):
+    check_source_array(oracle_lib)
+
+
+def test_srcdata_and_boundary_fix(oracle_lib):
+    check_srcdata(oracle_lib)
 
 
 def _cavity_energy_spread(lib, a):

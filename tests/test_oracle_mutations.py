@@ -11,7 +11,8 @@ of its region, and a DFT array that keeps the integration weights or the stored 
 the planes around an off-grid monitor unweighted, or places a periodic piece without its
 shift; a time-domain flux that adds its second pair, reads fields off the centred grid, or
 keeps only the flux after the E update; a uPML term whose sigma ignores its profile's integral or
-whose kappa uses the wrong one."""
+whose kappa uses the wrong one; an amplitude array interpolated without its half-cell offset, a
+source point moved to its periodic owner with the phase instead of its conjugate."""
 
 import ctypes
 import os
@@ -46,6 +47,8 @@ CHECKS = {
     "FLUXT_HALF": T.check_flux_time_domain,
     "PML_INTEGRAL": T.check_pml_reflection,
     "PML_STRETCH": T.check_pml_stretch,
+    "AMPARR_OFFSET": T.check_source_array,
+    "FIXBND_PHASE": T.check_srcdata,
 }
 
 
