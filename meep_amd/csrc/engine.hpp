@@ -68,12 +68,12 @@ template <class R> struct CurlComp {
   const R *g1, *g2; // curl partners after the reference's swap (g1 never null unless both are)
   int64_t s1, s2;   // partner strides (negated for the D update, src/step_db.cpp:81-84)
   R dtdx;
-  R *fu;     // PML auxiliary (dsigu); null when absent
+  double *fu; // PML auxiliary (dsigu), FP64 whatever R is; null when absent
   // conductivity (null when none) and 1 / (1 + cnd dt / 2); fcnd is the conductivity
   // auxiliary where uPML acts on f (src/step_generic.cpp:85-248, src/step_db.cpp:66-69).
   // Chunks with conductivity run the generic kernel only.
   const R *cnd, *cndinv;
-  R *fcnd;
+  double *fcnd; // FP64 whatever R is
   R dt2; // dt / 2 in realnum
   int dsig;  // PML direction acting on f, -1 if none
   int dsigu; // PML direction acting through fu, -1 if none
@@ -92,7 +92,7 @@ template <class R> struct UpdComp {
   R *f;       // E (or H)
   const R *g; // D (or B)
   const R *u; // diagonal chi1inv; null means 1
-  R *fw;      // PML auxiliary W; null when this direction has no PML
+  double *fw; // PML auxiliary W, FP64 whatever R is; null when this direction has no PML
   int dsigw;  // -1 when none
   int lo[3], hi[3], ysh[3];
   R *P[MAXPOLE], *Pp[MAXPOLE];
@@ -351,6 +351,8 @@ void launch_src(R *f, const int64_t *idx, const double2 *amp, int n, double2 sca
                 hipStream_t s);
 // f[i] = (f[i] + b[i]) / 2 over n points (fields_chunk::average_with_backup)
 template <class R> void launch_average(R *f, const R *b, size_t n, hipStream_t s);
+// dst[i] = src[i] over n points, realnum to FP64 (an auxiliary array's first values)
+template <class R> void launch_widen(double *dst, const R *src, size_t n, hipStream_t s);
 // f[dst[q]] = sgn[q] * f[idx[k[q]]]: the folded periodic images of source points
 template <class R>
 void launch_src_images(R *f, const int64_t *idx, const int *k, const int64_t *dst, const int8_t *sgn, int n, hipStream_t s);

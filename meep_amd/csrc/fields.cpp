@@ -419,7 +419,9 @@ template <class R> struct Fields : FieldsBase {
   hipStream_t stream = nullptr;
   int device = 0; // the GPU current when the object was created (its stream's device)
 
-  R *f[NC][2] = {}, *f_u[NC][2] = {}, *f_w[NC][2] = {}, *f_cond[NC][2] = {};
+  R *f[NC][2] = {};
+  // the uPML and conductivity auxiliaries are FP64 arrays whatever realnum is
+  double *f_u[NC][2] = {}, *f_w[NC][2] = {}, *f_cond[NC][2] = {};
   std::vector<std::unique_ptr<DevBuf>> bufs;
   // materials and uPML on the device
   R *d_chi1inv[NC] = {};
@@ -550,6 +552,18 @@ template <class R> struct Fields : FieldsBase {
       HIPCHECK(hipMemcpyAsync(p, copy_from, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
     else
       HIPCHECK(hipMemsetAsync(p, 0, sizeof(R) * gv.ntot, stream));
+    return p;
+  }
+  // an FP64 auxiliary array: a copy of the realnum array copy_from (f_u and f_w start as
+  // copies of f, src/step_db.cpp:61-65, src/update_eh.cpp:174-178), or zero
+  double *daux(const R *copy_from) {
+    bufs.emplace_back(new DevBuf);
+    bufs.back()->alloc(sizeof(double) * gv.ntot);
+    double *p = static_cast<double *>(bufs.back()->p);
+    if (copy_from)
+      launch_widen<R>(p, copy_from, gv.ntot, stream);
+    else
+      HIPCHECK(hipMemsetAsync(p, 0, sizeof(double) * gv.ntot, stream));
     return p;
   }
   template <class T> T *upload(const std::vector<T> &v) {
@@ -764,7 +778,10 @@ template <class R> struct Fields : FieldsBase {
     for (int d = 0; d < 3; ++d)
       if (bc[High][d] == Periodic && k[d] != 0.0) fail("Can't use real fields with bloch boundary conditions!");
     is_real = true;
-    for (int c = 0; c < NC; ++c) f[c][1] = f_u[c][1] = f_w[c][1] = nullptr;
+    for (int c = 0; c < NC; ++c) {
+      f[c][1] = nullptr;
+      f_u[c][1] = f_w[c][1] = nullptr;
+    }
     plan_valid = false;
   }
   void set_boundary(int b, int d, int cond) override {
@@ -1398,9 +1415,9 @@ template <class R> struct Fields : FieldsBase {
       const int dsig0 = cycle_dir(dc, 1), dsigu0 = cycle_dir(dc, 2);
       k.dsig = pml_in(dsig0) ? dsig0 : -1;
       k.dsigu = pml_in(dsigu0) ? dsigu0 : -1;
-      if (k.dsigu >= 0 && !f_u[cc_][cmp]) f_u[cc_][cmp] = dalloc(f[cc_][cmp]);
+      if (k.dsigu >= 0 && !f_u[cc_][cmp]) f_u[cc_][cmp] = daux(f[cc_][cmp]);
       // f_cond where uPML acts on f and the component conducts (src/step_db.cpp:66-69)
-      if (k.dsig >= 0 && d_cond[cc_] && !f_cond[cc_][cmp]) f_cond[cc_][cmp] = dalloc(nullptr);
+      if (k.dsig >= 0 && d_cond[cc_] && !f_cond[cc_][cmp]) f_cond[cc_][cmp] = daux(nullptr);
       k.f = f[cc_][cmp];
       k.fu = f_u[cc_][cmp];
       k.cnd = d_cond[cc_];
@@ -1494,7 +1511,7 @@ template <class R> struct Fields : FieldsBase {
         f[ec][cmp] = dalloc(f[dc][cmp]);
         plan_valid = false;
       }
-      if (dsigw >= 0 && !f_w[ec][cmp]) f_w[ec][cmp] = dalloc(f[ec][cmp]);
+      if (dsigw >= 0 && !f_w[ec][cmp]) f_w[ec][cmp] = daux(f[ec][cmp]);
       if (f[ec][cmp] == f[dc][cmp]) continue;
       e.f = f[ec][cmp];
       e.g = f[dc][cmp];
@@ -2138,18 +2155,20 @@ template <class R> struct Fields : FieldsBase {
     if (!plan_valid) rebuild_plan();
     HIPCHECK(hipStreamSynchronize(stream));
     struct Saved {
-      R *p;
-      bool average;
+      void *p;
+      size_t bytes;
+      bool average; // a realnum field array, averaged with its saved values
       std::unique_ptr<DevBuf> b;
     };
     std::vector<Saved> saved;
-    auto save = [&](R *p, bool average) {
+    auto save_bytes = [&](void *p, size_t esz, bool average) {
       if (!p) return;
-      Saved sv{p, average, std::unique_ptr<DevBuf>(new DevBuf)};
-      sv.b->alloc(sizeof(R) * gv.ntot);
-      HIPCHECK(hipMemcpyAsync(sv.b->p, p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+      Saved sv{p, esz * gv.ntot, average, std::unique_ptr<DevBuf>(new DevBuf)};
+      sv.b->alloc(sv.bytes);
+      HIPCHECK(hipMemcpyAsync(sv.b->p, p, sv.bytes, hipMemcpyDeviceToDevice, stream));
       saved.push_back(std::move(sv));
     };
+    auto save = [&](auto *p, bool average) { save_bytes(p, sizeof(*p), average); };
     for (int cmp = 0; cmp < 2; ++cmp) {
       for (int c = 15; c < 20; ++c) {
         save(f[c][cmp], true);
@@ -2168,11 +2187,10 @@ template <class R> struct Fields : FieldsBase {
     rows_hint = 1;
     phase_B(A);
     for (const Saved &sv : saved)
-      if (sv.average) launch_average<R>(sv.p, static_cast<const R *>(sv.b->p), gv.ntot, stream);
+      if (sv.average) launch_average<R>(static_cast<R *>(sv.p), static_cast<const R *>(sv.b->p), gv.ntot, stream);
     HIPCHECK(hipStreamSynchronize(stream));
     const double value = fn();
-    for (const Saved &sv : saved)
-      HIPCHECK(hipMemcpyAsync(sv.p, sv.b->p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+    for (const Saved &sv : saved) HIPCHECK(hipMemcpyAsync(sv.p, sv.b->p, sv.bytes, hipMemcpyDeviceToDevice, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     return value;
   }
@@ -2361,17 +2379,25 @@ template <class R> struct Fields : FieldsBase {
   std::unique_ptr<Adj> adj;
   bool dft_off = false; // recomputation steps leave the DFT accumulators alone
 
-  // every distinct state array (real parts; the reverse mode needs real fields)
-  std::vector<R *> state_arrays() const {
-    std::vector<R *> v;
-    auto add = [&](R *p) {
-      if (p && std::find(v.begin(), v.end(), p) == v.end()) v.push_back(p);
+  // every distinct state array (real parts; the reverse mode needs real fields) with its
+  // element size: realnum fields, FP64 auxiliaries
+  struct StateArr {
+    void *p;
+    size_t esz;
+  };
+  std::vector<StateArr> state_arrays() const {
+    std::vector<StateArr> v;
+    auto add = [&](void *p, size_t esz) {
+      if (!p) return;
+      for (const StateArr &a : v)
+        if (a.p == p) return;
+      v.push_back(StateArr{p, esz});
     };
     for (int c = 0; c < NC; ++c) {
-      add(f[c][0]);
-      add(f_u[c][0]);
-      add(f_w[c][0]);
-      add(f_cond[c][0]);
+      add(f[c][0], sizeof(R));
+      add(f_u[c][0], sizeof(double));
+      add(f_w[c][0], sizeof(double));
+      add(f_cond[c][0], sizeof(double));
     }
     return v;
   }
@@ -2424,10 +2450,10 @@ template <class R> struct Fields : FieldsBase {
     HIPCHECK(hipStreamSynchronize(stream));
     typename Adj::Checkpoint cp;
     cp.t = t;
-    for (R *p : state_arrays()) {
+    for (const StateArr &a : state_arrays()) {
       cp.data.emplace_back(new DevBuf);
-      cp.data.back()->alloc(sizeof(R) * gv.ntot);
-      HIPCHECK(hipMemcpyAsync(cp.data.back()->p, p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+      cp.data.back()->alloc(a.esz * gv.ntot);
+      HIPCHECK(hipMemcpyAsync(cp.data.back()->p, a.p, a.esz * gv.ntot, hipMemcpyDeviceToDevice, stream));
     }
     adj->cps.push_back(std::move(cp));
   }
@@ -2481,16 +2507,16 @@ template <class R> struct Fields : FieldsBase {
     if (!plan_valid) rebuild_plan();
     const long N = A.nrec;
     // cotangent arrays, aliased as the forward arrays are
-    std::vector<R *> fwd = state_arrays();
+    std::vector<StateArr> fwd = state_arrays();
     std::vector<std::unique_ptr<DevBuf>> barbuf;
-    std::map<const R *, R *> bar;
-    for (R *p : fwd) {
+    std::map<const void *, R *> bar; // cotangents in realnum, of the FP64 auxiliaries too
+    for (const StateArr &a : fwd) {
       barbuf.emplace_back(new DevBuf);
       barbuf.back()->alloc(sizeof(R) * gv.ntot);
       HIPCHECK(hipMemsetAsync(barbuf.back()->p, 0, sizeof(R) * gv.ntot, stream));
-      bar[p] = static_cast<R *>(barbuf.back()->p);
+      bar[a.p] = static_cast<R *>(barbuf.back()->p);
     }
-    auto B = [&](const R *p) -> R * { return p ? bar.at(p) : nullptr; };
+    auto B = [&](const void *p) -> R * { return p ? bar.at(p) : nullptr; };
     DevBuf cbar[3];
     for (int q = 0; q < 3; ++q) {
       cbar[q].alloc(sizeof(R) * gv.ntot);
@@ -2655,10 +2681,10 @@ template <class R> struct Fields : FieldsBase {
       const long tend = t;
       typename Adj::Checkpoint fin;
       fin.t = t;
-      for (R *p : fwd) {
+      for (const StateArr &a : fwd) {
         fin.data.emplace_back(new DevBuf);
-        fin.data.back()->alloc(sizeof(R) * gv.ntot);
-        HIPCHECK(hipMemcpyAsync(fin.data.back()->p, p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+        fin.data.back()->alloc(a.esz * gv.ntot);
+        HIPCHECK(hipMemcpyAsync(fin.data.back()->p, a.p, a.esz * gv.ntot, hipMemcpyDeviceToDevice, stream));
       }
       A.seg.alloc(sizeof(R) * A.npts * size_t(A.interval));
       const long nrec_total = N;
@@ -2690,14 +2716,14 @@ template <class R> struct Fields : FieldsBase {
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpy(out, grad.p, sizeof(double) * A.npts, hipMemcpyDeviceToHost));
   }
-  void restore_state(const typename Adj::Checkpoint &cp, const std::vector<R *> &arrays) {
+  void restore_state(const typename Adj::Checkpoint &cp, const std::vector<StateArr> &arrays) {
     if (cp.data.empty()) {
-      for (R *p : arrays) HIPCHECK(hipMemsetAsync(p, 0, sizeof(R) * gv.ntot, stream));
+      for (const StateArr &a : arrays) HIPCHECK(hipMemsetAsync(a.p, 0, a.esz * gv.ntot, stream));
       return;
     }
     if (cp.data.size() != arrays.size()) fail("internal: the state arrays changed since the checkpoint");
     for (size_t q = 0; q < arrays.size(); ++q)
-      HIPCHECK(hipMemcpyAsync(arrays[q], cp.data[q]->p, sizeof(R) * gv.ntot, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(arrays[q].p, cp.data[q]->p, arrays[q].esz * gv.ntot, hipMemcpyDeviceToDevice, stream));
   }
   // Halo transposes: every halo point of every allocated array (the kernels' folded planes
   // included), grouped by owned source index so one thread owns each source.
@@ -2705,7 +2731,7 @@ template <class R> struct Fields : FieldsBase {
     int nsrc = 0;
     DevBuf first, src, dst, sgn, slot, bars;
   };
-  void build_adj_halo(AdjHalo hal[2], const std::map<const R *, R *> &bar) {
+  void build_adj_halo(AdjHalo hal[2], const std::map<const void *, R *> &bar) {
     struct E {
       uint32_t src, dst;
       int8_t sgn;
@@ -3616,33 +3642,55 @@ template <class R> struct Fields : FieldsBase {
   R *array_ptr(int c, int cmp, int kind, int ipol) {
     if (c < 0 || c >= NC || cmp < 0 || cmp > 1) fail("invalid component");
     if (kind == MP_ARR_F) return f[c][cmp];
-    if (kind == MP_ARR_F_U) return f_u[c][cmp];
-    if (kind == MP_ARR_F_W) return f_w[c][cmp];
-    if (kind == MP_ARR_F_COND) return f_cond[c][cmp];
     for (auto &p : pols)
       if (p.isus == ipol && p.allocated) return kind == MP_ARR_P ? p.P[c][cmp] : p.Pp[c][cmp];
     return nullptr;
   }
+  double *aux_ptr(int c, int cmp, int kind) {
+    if (c < 0 || c >= NC || cmp < 0 || cmp > 1) fail("invalid component");
+    if (kind == MP_ARR_F_U) return f_u[c][cmp];
+    if (kind == MP_ARR_F_W) return f_w[c][cmp];
+    return f_cond[c][cmp];
+  }
+  static bool is_aux(int kind) { return kind == MP_ARR_F_U || kind == MP_ARR_F_W || kind == MP_ARR_F_COND; }
+  template <class T> void read_layout(const T *p, double *out) {
+    std::vector<T> v(gv.ntot);
+    HIPCHECK(hipMemcpy(v.data(), p, sizeof(T) * gv.ntot, hipMemcpyDeviceToHost));
+    gv.for_each_point([&](int64_t r, int64_t d) { out[r] = double(v[d]); });
+  }
+  template <class T> void write_layout(T *p, const double *in) {
+    std::vector<T> v(gv.ntot);
+    HIPCHECK(hipMemcpy(v.data(), p, sizeof(T) * gv.ntot, hipMemcpyDeviceToHost));
+    gv.for_each_point([&](int64_t r, int64_t d) { v[d] = T(in[r]); });
+    HIPCHECK(hipMemcpy(p, v.data(), sizeof(T) * gv.ntot, hipMemcpyHostToDevice));
+  }
   int get_array(int c, int cmp, int kind, int ipol, double *out) override {
     HIPCHECK(hipStreamSynchronize(stream));
-    R *p = array_ptr(c, cmp, kind, ipol);
-    if (!p) {
-      for (size_t i = 0; i < gv.rntot; ++i) out[i] = 0;
-      return 1;
+    if (is_aux(kind)) {
+      double *p = aux_ptr(c, cmp, kind);
+      if (p) {
+        read_layout(p, out);
+        return 0;
+      }
     }
-    std::vector<R> v(gv.ntot);
-    HIPCHECK(hipMemcpy(v.data(), p, sizeof(R) * gv.ntot, hipMemcpyDeviceToHost));
-    gv.for_each_point([&](int64_t r, int64_t d) { out[r] = double(v[d]); });
-    return 0;
+    else if (R *p = array_ptr(c, cmp, kind, ipol)) {
+      read_layout(p, out);
+      return 0;
+    }
+    for (size_t i = 0; i < gv.rntot; ++i) out[i] = 0;
+    return 1;
   }
   void set_array(int c, int cmp, int kind, int ipol, const double *in) override {
     HIPCHECK(hipStreamSynchronize(stream));
+    if (is_aux(kind)) {
+      double *p = aux_ptr(c, cmp, kind);
+      if (!p) fail("array not allocated");
+      write_layout(p, in);
+      return;
+    }
     R *p = array_ptr(c, cmp, kind, ipol);
     if (!p) fail("array not allocated");
-    std::vector<R> v(gv.ntot);
-    HIPCHECK(hipMemcpy(v.data(), p, sizeof(R) * gv.ntot, hipMemcpyDeviceToHost));
-    gv.for_each_point([&](int64_t r, int64_t d) { v[d] = R(in[r]); });
-    HIPCHECK(hipMemcpy(p, v.data(), sizeof(R) * gv.ntot, hipMemcpyHostToDevice));
+    write_layout(p, in);
   }
   bool have_component(int c) const override { return c >= 0 && c < NC && f[c][0]; }
   void times(double *t7) const override {

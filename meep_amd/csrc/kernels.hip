@@ -137,23 +137,24 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
       // conductivity: the curl term is damped by (1 - dt/2 cnd) / (1 + dt/2 cnd); where uPML
       // acts on f it goes through fcnd, whose increment the sigma update takes
       // (src/step_generic.cpp:85-96,108-128,158-177,198-220)
-      const R cn = k.cnd[i], ci = k.cndinv[i];
-      R fprev = R(0), g = fv;
+      // the auxiliaries f_u and fcnd are FP64 arrays, updated in FP64
+      const double cn = k.cnd[i], ci = k.cndinv[i], dt2 = k.dt2, dtdx = k.dtdx, cud = cu;
+      double fprev = 0.0, g = fv;
       if (pmlu) fprev = g = k.fu[i];
       if (pml) {
-        const R fc = k.fcnd[i];
-        const R fcn = ((R(1) - k.dt2 * cn) * fc - k.dtdx * cu) * ci;
+        const double fc = k.fcnd[i];
+        const double fcn = ((1.0 - dt2 * cn) * fc - dtdx * cud) * ci;
         k.fcnd[i] = fcn;
-        g = ((k.kap[kk] - k.sig[kk]) * g + (fcn - fc)) * k.siginv[kk];
+        g = ((double(k.kap[kk]) - double(k.sig[kk])) * g + (fcn - fc)) * double(k.siginv[kk]);
       }
       else
-        g = ((R(1) - k.dt2 * cn) * g - k.dtdx * cu) * ci;
+        g = ((1.0 - dt2 * cn) * g - dtdx * cud) * ci;
       if (pmlu) {
         k.fu[i] = g;
-        fv = k.siginvu[ku] * ((k.kapu[ku] - k.sigu[ku]) * fv + g - fprev);
+        fv = R(double(k.siginvu[ku]) * ((double(k.kapu[ku]) - double(k.sigu[ku])) * double(fv) + g - fprev));
       }
       else
-        fv = g;
+        fv = R(g);
     }
     else if (!pmlu) {
       if (!pml)
@@ -162,14 +163,15 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
         fv = ((k.kap[kk] - k.sig[kk]) * fv - k.dtdx * cu) * k.siginv[kk];
     }
     else {
-      const R fprev = k.fu[i];
-      R fun;
+      // the uPML auxiliary f_u is an FP64 array, updated in FP64
+      const double fprev = k.fu[i], dtdx = k.dtdx, cud = cu;
+      double fun;
       if (!pml)
-        fun = fprev - k.dtdx * cu;
+        fun = fprev - dtdx * cud;
       else
-        fun = ((k.kap[kk] - k.sig[kk]) * fprev - k.dtdx * cu) * k.siginv[kk];
+        fun = ((double(k.kap[kk]) - double(k.sig[kk])) * fprev - dtdx * cud) * double(k.siginv[kk]);
       k.fu[i] = fun;
-      fv = k.siginvu[ku] * ((k.kapu[ku] - k.sigu[ku]) * fv + fun - fprev);
+      fv = R(double(k.siginvu[ku]) * ((double(k.kapu[ku]) - double(k.sigu[ku])) * double(fv) + fun - fprev));
     }
     if (a.src && in_src_box<1>(a, j0, j1, j2)) fv = apply_sources<R, false, C>(a, i, j0, j1, j2, fv, vrow);
     if (slot_metal(k.ax, j0, j1, j2)) fv = R(0);
@@ -189,12 +191,14 @@ __device__ __forceinline__ void cell_update(const StepArgs<R> &a, uint32_t i, in
     R ev, wv;
     const int kw = u.ksigw >= 0 ? 2 * asel(j0, j1, j2, u.ksigw) + u.ysigw : 0;
     if (pml_on(u.sigw, u.kapw, kw)) {
-      const R kapw = u.kapw[kw], sigw = u.sigw[kw];
-      const R fwprev = u.fw[i];
-      const R fwn = u.u ? gv * u.u[i] : gv;
+      // the uPML auxiliary f_w is an FP64 array, updated in FP64
+      const double kapw = u.kapw[kw], sigw = u.sigw[kw];
+      const double fwprev = u.fw[i];
+      const R fwr = u.u ? gv * u.u[i] : gv;
+      const double fwn = fwr;
       u.fw[i] = fwn;
-      ev = u.f[i] + ((kapw + sigw) * fwn - (kapw - sigw) * fwprev);
-      wv = fwn;
+      ev = R(double(u.f[i]) + ((kapw + sigw) * fwn - (kapw - sigw) * fwprev));
+      wv = fwr;
     }
     else {
       ev = u.u ? gv * u.u[i] : gv;
@@ -288,6 +292,33 @@ template <class R, int V> __device__ __forceinline__ void vstore(R *p, const Vec
     for (int k = 0; k < V; ++k)
       if (mask & (1u << k)) p[k] = x.v[k];
   }
+}
+
+// the FP64 auxiliary arrays (f_u, f_w) of a group: V doubles in 16-byte pieces (groups are
+// aligned to V values, so to 8 V bytes)
+template <int V> __device__ __forceinline__ Vec<double, V> vload_aux(const double *p) {
+  Vec<double, V> r;
+  if (V == 1) {
+    r.v[0] = p[0];
+    return r;
+  }
+#pragma unroll
+  for (int h = 0; h < V / 2; ++h) {
+    const double2 q = reinterpret_cast<const double2 *>(p)[h];
+    r.v[2 * h] = q.x;
+    r.v[2 * h + 1] = q.y;
+  }
+  return r;
+}
+template <int V> __device__ __forceinline__ void vstore_aux(double *p, const Vec<double, V> &x, unsigned mask) {
+  if (V > 1 && mask == (1u << V) - 1) {
+#pragma unroll
+    for (int h = 0; h < V / 2; ++h) reinterpret_cast<double2 *>(p)[h] = make_double2(x.v[2 * h], x.v[2 * h + 1]);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+    if (mask & (1u << k)) p[k] = x.v[k];
 }
 
 // partner values at i + s for the group starting at i (s = +-1 along the row: a shift of the
@@ -449,8 +480,9 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
   const Vec<R, V> &g2c = C == 0 ? F1 : (C == 1 ? F2 : F0);
 
   // ---- loads ----
-  Vec<R, V> fv, g1n, g2n, fuv;
-  unsigned upot = 0; // lanes where the f_u profile may be active
+  Vec<R, V> fv, g1n, g2n;
+  Vec<double, V> fuv; // f_u is FP64 whatever R is
+  unsigned upot = 0;  // lanes where the f_u profile may be active
   if (CURL) {
     const CurlComp<R> &kc = a.cc[C];
     if (a.hasp[C]) g1n = neighbour<R, V>(a.F[(C + 2) % 3], i, a.sgn * g.st[dp], g1c, mask);
@@ -458,10 +490,11 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
     fv = vload<R, V>(kc.f + i);
     if (PML && kc.sigu) {
       upot = prof_lanes<V>(g, kc.ksigu, j0, j1, z0) & mask;
-      if (upot) fuv = vload<R, V>(kc.fu + i);
+      if (upot) fuv = vload_aux<V>(kc.fu + i);
     }
   }
-  Vec<R, V> gv, uu, Pc[NP > 0 ? NP : 1], Ppv[NP > 0 ? NP : 1], sgv[NP > 0 ? NP : 1], fwv, eold;
+  Vec<R, V> gv, uu, Pc[NP > 0 ? NP : 1], Ppv[NP > 0 ? NP : 1], sgv[NP > 0 ? NP : 1], eold;
+  Vec<double, V> fwv; // f_w is FP64 whatever R is
   bool pin[NP > 0 ? NP : 1]; // the group meets pole p's box
   unsigned wpot = 0;         // lanes where the f_w profile may be active
   if (upd) {
@@ -489,7 +522,7 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
     if (PML && u.sigw) {
       wpot = prof_lanes<V>(g, u.ksigw, j0, j1, z0) & mask;
       if (wpot) {
-        fwv = vload<R, V>(u.fw + i);
+        fwv = vload_aux<V>(u.fw + i);
         eold = vload<R, V>(u.f + i);
       }
     }
@@ -543,14 +576,15 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
             fv.v[k] = ((P.kap[k] - P.sig[k]) * fv.v[k] - a.dtdx * cu.v[k]) * P.sinv[k];
         }
         else {
-          const R fprev = fuv.v[k];
-          R fun;
+          // f_u in FP64
+          const double fprev = fuv.v[k], dtdx = a.dtdx, cud = cu.v[k];
+          double fun;
           if (!pml)
-            fun = fprev - a.dtdx * cu.v[k];
+            fun = fprev - dtdx * cud;
           else
-            fun = ((P.kap[k] - P.sig[k]) * fprev - a.dtdx * cu.v[k]) * P.sinv[k];
+            fun = ((double(P.kap[k]) - double(P.sig[k])) * fprev - dtdx * cud) * double(P.sinv[k]);
           fuv.v[k] = fun;
-          fv.v[k] = U.sinv[k] * ((U.kap[k] - U.sig[k]) * fv.v[k] + fun - fprev);
+          fv.v[k] = R(double(U.sinv[k]) * ((double(U.kap[k]) - double(U.sig[k])) * double(fv.v[k]) + fun - fprev));
         }
       }
     }
@@ -593,9 +627,10 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
 #pragma unroll
       for (int k = 0; k < V; ++k)
         if ((wm >> k) & 1) {
-          const R fwprev = fwv.v[k], fwn = ev.v[k];
+          // f_w in FP64
+          const double fwprev = fwv.v[k], fwn = ev.v[k], kw = W.kap[k], sw = W.sig[k];
           fwv.v[k] = fwn;
-          ev.v[k] = eold.v[k] + ((W.kap[k] + W.sig[k]) * fwn - (W.kap[k] - W.sig[k]) * fwprev);
+          ev.v[k] = R(double(eold.v[k]) + ((kw + sw) * fwn - (kw - sw) * fwprev));
         }
     }
 #pragma unroll
@@ -617,12 +652,12 @@ __device__ __forceinline__ void fast_slot(const StepArgs<R> &a, uint32_t i, int 
   // ---- stores ----
   if (CURL) {
     const CurlComp<R> &kc = a.cc[C];
-    if (PML && um) vstore<R, V>(kc.fu + i, fuv, um);
+    if (PML && um) vstore_aux<V>(kc.fu + i, fuv, um);
     put_row<R, V>(kc.f, i, fv, mask, g, ybits, j0, j1, z0);
   }
   if (upd) {
     const UpdComp<R> &u = a.uc[C];
-    if (PML && wm) vstore<R, V>(u.fw + i, fwv, wm);
+    if (PML && wm) vstore_aux<V>(u.fw + i, fwv, wm);
 #pragma unroll
     for (int p = 0; p < NP; ++p)
       if (pin[p] && u.sigma[p]) {
@@ -781,10 +816,10 @@ template <class R, int NP> struct Row3Args {
   R dtdx;
   const R *F[3];        // curl partners by direction
   R *f[3];              // B or D by direction (null: absent)
-  R *fu0;               // f_u of slot 0 (its dsigu is z), null when none
+  double *fu0;          // f_u of slot 0 (its dsigu is z, FP64), null when none
   R *e[3];              // H or E (null: no separate update array)
   const R *chi[3];      // diagonal chi1inv / mu1inv (null: 1)
-  R *fw2;               // f_w of slot 2 (its dsigw is z), null when none
+  double *fw2;          // f_w of slot 2 (its dsigw is z, FP64), null when none
   const R *zsig, *zkap, *zsiginv; // the z uPML profile (null: no z uPML)
   // poles: P and P_prev of pole p for component c are at pofs[p] and ppofs[p] (values) from
   // pbase[c], its sigma at sofs[p] from sbase[c]: six pointers instead of three per pole and
@@ -880,11 +915,12 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
   else g1n = vload<R, V>(g1p + (PH ? i - (dp == 0 ? g.sx : g.sy) : i + (dp == 0 ? g.sx : g.sy)));
   if (dm == 2) g2n = neighbour<R, V>(g2p, i, sg, g2c, mask);
   else g2n = vload<R, V>(g2p + (PH ? i - (dm == 0 ? g.sx : g.sy) : i + (dm == 0 ? g.sx : g.sy)));
-  Vec<R, V> fv = vload<R, V>(fp + i), fuv;
+  Vec<R, V> fv = vload<R, V>(fp + i);
+  Vec<double, V> fuv, fwv; // the FP64 uPML auxiliaries
   const bool zfu = ZP && C == 0 && !zq && a.fu0;
-  if (zfu) fuv = vload<R, V>(a.fu0 + i);
+  if (zfu) fuv = vload_aux<V>(a.fu0 + i);
   R *const ep = UPD ? a.e[C] : nullptr;
-  Vec<R, V> uu, Pc[NP > 0 ? NP : 1], Ppv[NP > 0 ? NP : 1], sgv[NP > 0 ? NP : 1], fwv, eold;
+  Vec<R, V> uu, Pc[NP > 0 ? NP : 1], Ppv[NP > 0 ? NP : 1], sgv[NP > 0 ? NP : 1], eold;
   bool pin[NP > 0 ? NP : 1];
   const bool zfw = UPD && ZP && C == 2 && !zq && a.fw2 && ep;
   if (ep) {
@@ -907,7 +943,7 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
       }
     }
     if (zfw) {
-      fwv = vload<R, V>(a.fw2 + i);
+      fwv = vload_aux<V>(a.fw2 + i);
       eold = vload<R, V>(ep + i);
     }
   }
@@ -936,10 +972,11 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
 #pragma unroll
       for (int k = 0; k < V; ++k) {
         if ((um >> k) & 1) {
-          const R fprev = fuv.v[k];
-          const R fun = fprev - a.dtdx * cu.v[k];
+          // f_u in FP64
+          const double fprev = fuv.v[k], dtdx = a.dtdx, cud = cu.v[k];
+          const double fun = fprev - dtdx * cud;
           fuv.v[k] = fun;
-          fv.v[k] = si_[k] * ((kp_[k] - sg_[k]) * fv.v[k] + fun - fprev);
+          fv.v[k] = R(double(si_[k]) * ((double(kp_[k]) - double(sg_[k])) * double(fv.v[k]) + fun - fprev));
         }
         else
           fv.v[k] -= a.dtdx * cu.v[k];
@@ -997,9 +1034,10 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
         const R sw = a.zsig[idx], kw = a.zkap[idx];
         if (((mask >> k) & 1) && (sw != R(0) || kw != R(1))) {
           wm |= 1u << k;
-          const R fwprev = fwv.v[k], fwn = ev.v[k];
+          // f_w in FP64
+          const double fwprev = fwv.v[k], fwn = ev.v[k], kwd = kw, swd = sw;
           fwv.v[k] = fwn;
-          ev.v[k] = eold.v[k] + ((kw + sw) * fwn - (kw - sw) * fwprev);
+          ev.v[k] = R(double(eold.v[k]) + ((kwd + swd) * fwn - (kwd - swd) * fwprev));
         }
       }
     }
@@ -1019,10 +1057,10 @@ __device__ __forceinline__ void r3_slot(const Row3Args<R, NP> &a, uint32_t i, in
     }
   }
   // ---- stores ----
-  if (um) vstore<R, V>(a.fu0 + i, fuv, um);
+  if (um) vstore_aux<V>(a.fu0 + i, fuv, um);
   vstore<R, V>(fp + i, fv, mask);
   if (ep) {
-    if (wm) vstore<R, V>(a.fw2 + i, fwv, wm);
+    if (wm) vstore_aux<V>(a.fw2 + i, fwv, wm);
 #pragma unroll
     for (int p = 0; p < NP; ++p)
       if (pin[p] && ((a.smask >> (3 * p + C)) & 1)) {
@@ -1596,6 +1634,13 @@ template <class R> __global__ __launch_bounds__(BLOCK) void k_average(R *f, cons
 template <class R> void launch_average(R *f, const R *b, size_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_average<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, f, b, n);
 }
+template <class R> __global__ __launch_bounds__(BLOCK) void k_widen(double *dst, const R *src, size_t n) {
+  const size_t t = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (t < n) dst[t] = double(src[t]);
+}
+template <class R> void launch_widen(double *dst, const R *src, size_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_widen<R>, dim3(nblocks(n)), dim3(BLOCK), 0, s, dst, src, n);
+}
 template <class R> __global__ __launch_bounds__(BLOCK) void k_gather(R *dst, const R *f, const int64_t *idx, int n) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   if (t < n) dst[t] = f[idx[t]];
@@ -1842,6 +1887,7 @@ void launch_conn_batch(const BatchConn *items, const uint32_t *blk_item, const u
   template void launch_src<R>(R *, const int64_t *, const double2 *, int, double2, double, int,      \
                               hipStream_t);                                                        \
   template void launch_average<R>(R *, const R *, size_t, hipStream_t);                            \
+  template void launch_widen<R>(double *, const R *, size_t, hipStream_t);                         \
   template void launch_src_images<R>(R *, const int64_t *, const int *, const int64_t *, const int8_t *, int, \
                                      hipStream_t);                                                 \
   template void launch_gather<R>(R *, const R *, const int64_t *, int, hipStream_t);               \
