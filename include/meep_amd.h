@@ -49,6 +49,7 @@ typedef struct mp_fields mp_fields;
 typedef struct mp_dft_flux mp_dft_flux;
 typedef struct mp_dft_fields mp_dft_fields;
 typedef struct mp_flux_vol mp_flux_vol;
+typedef struct mp_near2far mp_near2far;
 typedef struct mp_adjoint mp_adjoint;
 typedef struct mp_batch mp_batch;
 
@@ -233,10 +234,11 @@ int mp_fields_get_times(mp_fields *f, double *times7);
    counterpart in the reference; feeds the time sinks above and bench.py's roofline).
    which: MP_K_STEP_BH (curl B + H update: every launch of the phase), MP_K_STEP_DE (curl D +
    E/P update: every launch of the phase), MP_K_HALO, MP_K_DFT, MP_K_SRC, and the interior-box
-   launch alone of each phase, MP_K_BH_INTERIOR and MP_K_DE_INTERIOR.  Times accumulate in
+   launch alone of each phase, MP_K_BH_INTERIOR and MP_K_DE_INTERIOR, and the near-to-far field
+   sums, MP_K_FARFIELD.  Times accumulate in
    milliseconds until mp_fields_reset_kernel_timing. */
 enum { MP_K_STEP_BH = 0, MP_K_STEP_DE = 1, MP_K_HALO = 2, MP_K_DFT = 3, MP_K_SRC = 4, MP_K_BH_INTERIOR = 5,
-       MP_K_DE_INTERIOR = 6 };
+       MP_K_DE_INTERIOR = 6, MP_K_FARFIELD = 7 };
 int mp_fields_set_kernel_timing(mp_fields *f, int on);
 int mp_fields_get_kernel_timing(mp_fields *f, int which, double *total_ms, long *launches);
 int mp_fields_reset_kernel_timing(mp_fields *f);
@@ -324,6 +326,45 @@ int mp_fields_flux_in_box_wrongH(mp_fields *f, int d, const double *vmin, const 
    H as it stands) (:84-95), over [vmin, vmax]. */
 int mp_fields_energy_in_box(mp_fields *f, int which, const double *vmin, const double *vmax,
                             double *out);
+
+/* ---- near-to-far transformation --------------------------------------------------------- */
+
+/* fields::add_dft_near2far(const volume_list *, const double *freq, size_t Nfreq,
+   int decimation_factor, int Nperiods) — src/meep.hpp:2161-2168, src/near2far.cpp:569-654.
+   Surface w is [vmins + 3w, vmaxs + 3w] with the complex weight (weights[2w], weights[2w + 1])
+   (volume_list::weight) and normal dirs[w] (X, Y, Z, or
+   -1 for the volume's normal direction; the reference's volume_list::c is the flux component
+   Sx, Sy or Sz of that direction).  Its tangential E and H are accumulated off the centred grid
+   with integration weights as the equivalent currents of the other kind.  The medium is
+   eps, mu at each surface's centre and must agree between surfaces.  dec <= 0 picks the
+   decimation as add_dft_flux does. */
+mp_near2far *mp_fields_add_dft_near2far(mp_fields *f, int nsurf, const int *dirs, const double *vmins,
+                                        const double *vmaxs, const double *weights, const double *freq,
+                                        int nfreq, int dec, int nperiods);
+void mp_near2far_delete(mp_near2far *n);
+/* dft_near2far::eps, mu (src/meep.hpp:1468) into eps_mu[2], and periodic_d / periodic_n of
+   the two tangential directions (-1 when not periodic) into periodic[4]; returns Nfreq. */
+int mp_near2far_info(const mp_near2far *n, double *eps_mu, int *periodic);
+/* dft_near2far::farfield(const vec &x) — src/near2far.cpp:399-408: EH[12 * Nfreq] as
+   (re, im) of Ex, Ey, Ez, Hx, Hy, Hz for each frequency in turn. */
+int mp_near2far_farfield(mp_near2far *n, const double *x, double *EH);
+/* dft_near2far::get_farfields_array(const volume &, int &rank, size_t *dims, size_t &N,
+   double resolution) — src/near2far.cpp:411-483: a 6 x 2 x N x Nfreq row-major array
+   (component, real/imaginary part, point, frequency) on floor(extent * resolution) points per
+   direction.  Returns N; out == NULL returns N, rank and dims only. */
+long mp_near2far_get_farfields_array(mp_near2far *n, const double *wmin, const double *wmax,
+                                     double resolution, int *rank, size_t *dims, double *out);
+/* dft_near2far::flux(direction, const volume &, double resolution) — src/near2far.cpp:517-565. */
+int mp_near2far_flux(mp_near2far *n, int d, const double *wmin, const double *wmax, double resolution,
+                     double *out);
+/* fields::get_dft_array(dft_near2far, component, num_freq, rank, dims) — src/dft.cpp:1282-1290;
+   as mp_dft_flux_get_dft_array, collapsed over the first surface's empty directions. */
+long mp_near2far_get_dft_array(mp_near2far *n, int c, int num_freq, int *rank, size_t *dims,
+                               double *out);
+/* green2d / green3d (src/near2far.cpp:133-186, 207-267): EH[12] of a point current of
+   component c0 and amplitude f0[2] at x0, seen at x, in the medium eps, mu; dim is 2 or 3. */
+int mp_near2far_green(int dim, const double *x, double freq, double eps, double mu, const double *x0,
+                      int c0, const double *f0, double *EH);
 
 /* ---- reverse mode (gradient of a DFT objective) ------------------------------------- */
 

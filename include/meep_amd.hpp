@@ -13,6 +13,7 @@
 
 #include <math.h>
 
+#include <algorithm>
 #include <chrono>
 #include <complex>
 #include <cstdarg>
@@ -143,10 +144,17 @@ inline grid_volume vol3d(double xsize, double ysize, double zsize, double a) {
   return grid_volume{D3, a, {xsize, ysize, zsize}};
 }
 
-// volume (src/meep/vec.hpp:948-1004): a box between two corners
+// volume (src/meep/vec.hpp:948-1004): a box between two corners, ordered per direction as
+// volume::volume does (src/vec.cpp:163-167)
 class volume {
 public:
-  volume(const vec &a, const vec &b) : lo(a), hi(b) {}
+  volume(const vec &a, const vec &b) : lo(a), hi(b) {
+    for (int d = 0; d < 3; ++d) {
+      const direction dd = direction(d);
+      lo.set_direction(dd, std::min(a.in_direction(dd), b.in_direction(dd)));
+      hi.set_direction(dd, std::max(a.in_direction(dd), b.in_direction(dd)));
+    }
+  }
   explicit volume(const vec &p) : lo(p), hi(p) {}
   vec get_min_corner() const { return lo; }
   vec get_max_corner() const { return hi; }
@@ -436,6 +444,89 @@ private:
   mp_flux_vol *h;
 };
 
+// derived_component (src/meep/vec.hpp:57-66): the flux components name a surface's normal
+enum derived_component { Sx = 100, Sy, Sr, Sp, Sz, EnergyDensity, D_EnergyDensity, H_EnergyDensity };
+
+// volume_list (src/meep/vec.hpp:1233-1255): surfaces with a component (Sx, Sy, Sz or a field
+// component naming the normal direction) and a complex weight; owns the rest of its list
+class volume_list {
+public:
+  volume_list(const volume &v, int c, std::complex<double> weight = 1.0, volume_list *next = 0)
+      : v(v), c(c), weight(weight), next(next) {}
+  ~volume_list() { delete next; }
+  volume_list(const volume_list &) = delete;
+  volume_list &operator=(const volume_list &) = delete;
+  volume v;
+  int c;
+  std::complex<double> weight;
+  volume_list *next;
+};
+
+// dft_near2far (src/meep.hpp:1331-1395): owned by its fields as the other monitors are
+class dft_near2far {
+public:
+  dft_near2far(mp_near2far *h, int n) : h(h), n(n) {}
+  // farfield(x): a new[] array of 6 * Nfreq values, Ex Ey Ez Hx Hy Hz per frequency
+  std::complex<double> *farfield(const vec &x) {
+    std::vector<double> raw(size_t(12 * n));
+    check(mp_near2far_farfield(h, x.data(), raw.data()));
+    std::complex<double> *out = new std::complex<double>[size_t(6 * n)];
+    for (int q = 0; q < 6 * n; ++q) out[q] = std::complex<double>(raw[2 * q], raw[2 * q + 1]);
+    return out;
+  }
+  // farfield_lowlevel(EH, x): the same into EH[6 * Nfreq]
+  void farfield_lowlevel(std::complex<double> *EH, const vec &x) {
+    std::complex<double> *v = farfield(x);
+    std::copy(v, v + 6 * n, EH);
+    delete[] v;
+  }
+  // get_farfields_array(where, rank, dims, N, resolution): a new[] 6 x 2 x N x Nfreq array
+  double *get_farfields_array(const volume &where, int &rank, size_t *dims, size_t &N, double resolution) {
+    long np = mp_near2far_get_farfields_array(h, where.get_min_corner().data(), where.get_max_corner().data(),
+                                              resolution, &rank, dims, nullptr);
+    if (np < 0) throw std::runtime_error(mp_last_error());
+    N = size_t(np);
+    double *out = new double[12 * N * size_t(n)];
+    if (mp_near2far_get_farfields_array(h, where.get_min_corner().data(), where.get_max_corner().data(), resolution,
+                                        &rank, dims, out) < 0) {
+      delete[] out;
+      throw std::runtime_error(mp_last_error());
+    }
+    return out;
+  }
+  // flux(direction, where, resolution): a new[] array of Nfreq values
+  double *flux(direction df, const volume &where, double resolution) {
+    double *out = new double[size_t(n)];
+    if (mp_near2far_flux(h, int(df), where.get_min_corner().data(), where.get_max_corner().data(), resolution, out)) {
+      delete[] out;
+      throw std::runtime_error(mp_last_error());
+    }
+    return out;
+  }
+  int Nfreq() const { return n; }
+  mp_near2far *handle() const { return h; }
+
+private:
+  mp_near2far *h;
+  int n;
+};
+
+// green2d / green3d (src/meep.hpp:2389-2392): EH[6] of a point current c0 of amplitude f0 at x0
+inline void green_of(int dim, std::complex<double> *EH, const vec &x, double freq, double eps, double mu, const vec &x0,
+                     component c0, std::complex<double> f0) {
+  double amp[2] = {f0.real(), f0.imag()}, raw[12];
+  check(mp_near2far_green(dim, x.data(), freq, eps, mu, x0.data(), int(c0), amp, raw));
+  for (int q = 0; q < 6; ++q) EH[q] = std::complex<double>(raw[2 * q], raw[2 * q + 1]);
+}
+inline void green2d(std::complex<double> *EH, const vec &x, double freq, double eps, double mu, const vec &x0,
+                    component c0, std::complex<double> f0) {
+  green_of(2, EH, x, freq, eps, mu, x0, c0, f0);
+}
+inline void green3d(std::complex<double> *EH, const vec &x, double freq, double eps, double mu, const vec &x0,
+                    component c0, std::complex<double> f0) {
+  green_of(3, EH, x, freq, eps, mu, x0, c0, f0);
+}
+
 // fields (src/meep.hpp:1716-2320)
 class fields {
 public:
@@ -460,6 +551,7 @@ public:
     for (flux_vol *v : flux_vols) delete v;
     for (mp_dft_flux *h : fluxes) mp_dft_flux_delete(h);
     for (mp_dft_fields *h : dft_fields_list) mp_dft_fields_delete(h);
+    for (mp_near2far *h : near2far_list) mp_near2far_delete(h);
     mp_fields_delete(p);
   }
   fields(const fields &) = delete;
@@ -579,6 +671,42 @@ public:
   std::complex<realnum> *get_dft_array(dft_fields fdft, component c, int num_freq, int *rank, size_t dims[3]) {
     return get_dft_array_of(mp_dft_fields_get_dft_array, fdft.handle(), c, num_freq, rank, dims);
   }
+  std::complex<realnum> *get_dft_array(dft_near2far n2f, component c, int num_freq, int *rank, size_t dims[3]) {
+    return get_dft_array_of(mp_near2far_get_dft_array, n2f.handle(), c, num_freq, rank, dims);
+  }
+  // add_dft_near2far(where, freq, Nfreq, decimation_factor, Nperiods) (src/meep.hpp:2161-2168):
+  // each surface's normal is the direction of its component (Sx, Sy, Sz or a field component),
+  // else the volume's normal
+  dft_near2far add_dft_near2far(const volume_list *where, const double *freq, size_t Nfreq, int decimation_factor = 0,
+                                int Nperiods = 1) {
+    std::vector<int> dirs;
+    std::vector<double> mins, maxs, weights;
+    for (const volume_list *w = where; w; w = w->next) {
+      int d = -1;
+      if (w->c == Sx || w->c == Ex || w->c == Hx || w->c == Dx || w->c == Bx) d = X;
+      if (w->c == Sy || w->c == Ey || w->c == Hy || w->c == Dy || w->c == By) d = Y;
+      if (w->c == Sz || w->c == Ez || w->c == Hz || w->c == Dz || w->c == Bz) d = Z;
+      dirs.push_back(d);
+      const vec lo = w->v.get_min_corner(), hi = w->v.get_max_corner();
+      mins.insert(mins.end(), lo.data(), lo.data() + 3);
+      maxs.insert(maxs.end(), hi.data(), hi.data() + 3);
+      weights.push_back(w->weight.real());
+      weights.push_back(w->weight.imag());
+    }
+    mp_near2far *h = mp_fields_add_dft_near2far(p, int(dirs.size()), dirs.data(), mins.data(), maxs.data(), weights.data(),
+                                                freq, int(Nfreq), decimation_factor, Nperiods);
+    if (!h) throw std::runtime_error(mp_last_error());
+    near2far_list.push_back(h);
+    return dft_near2far(h, int(Nfreq));
+  }
+  dft_near2far add_dft_near2far(const volume_list *where, const std::vector<double> &freq, int decimation_factor = 0,
+                                int Nperiods = 1) {
+    return add_dft_near2far(where, freq.data(), freq.size(), decimation_factor, Nperiods);
+  }
+  dft_near2far add_dft_near2far(const volume_list *where, double freq_min, double freq_max, int Nfreq,
+                                int decimation_factor = 0, int Nperiods = 1) {
+    return add_dft_near2far(where, linspace(freq_min, freq_max, Nfreq), decimation_factor, Nperiods);
+  }
   double field_energy() { // :2196
     double e = 0;
     check(mp_fields_field_energy(p, &e));
@@ -627,6 +755,7 @@ private:
   mp_fields *p;
   std::vector<mp_dft_flux *> fluxes;
   std::vector<mp_dft_fields *> dft_fields_list;
+  std::vector<mp_near2far *> near2far_list;
   std::vector<flux_vol *> flux_vols;
 };
 

@@ -152,6 +152,22 @@ _SIGNATURES = {
                                              C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
     "mp_dft_fields_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                                C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
+    "mp_fields_add_dft_near2far": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int]),
+    "mp_near2far_delete": (None, [C.c_void_p]),
+    "mp_near2far_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "mp_near2far_farfield": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mp_near2far_get_farfields_array": (C.c_long, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                   C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
+                                                   C.POINTER(C.c_double)]),
+    "mp_near2far_flux": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.c_double, C.POINTER(C.c_double)]),
+    "mp_near2far_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
+    "mp_near2far_green": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
@@ -390,6 +406,73 @@ class flux_vol:
         if getattr(self, "ptr", None):
             self.lib.mp_flux_vol_delete(self.ptr)
             self.ptr = None
+
+
+class near2far:
+    """meep::dft_near2far (src/meep.hpp:1455-1520): the tangential fields on closed or open
+    surfaces, transformed to the fields anywhere in the surrounding homogeneous medium."""
+
+    def __init__(self, fields, ptr, nfreq):
+        self.f = fields
+        self.lib = fields.lib
+        self.ptr = ptr
+        self.nfreq = nfreq
+        em = (C.c_double * 2)()
+        per = (C.c_int * 4)()
+        self.lib.mp_near2far_info(ptr, em, per)
+        self.eps, self.mu = em[0], em[1]
+        self.periodic_d = (per[0], per[2])
+        self.periodic_n = (per[1], per[3])
+
+    def farfield(self, x):
+        """dft_near2far::farfield(x) (src/near2far.cpp:399-408): array (nfreq, 6) of
+        Ex, Ey, Ez, Hx, Hy, Hz."""
+        out = np.empty(12 * self.nfreq)
+        _check(self.lib, self.lib.mp_near2far_farfield(self.ptr, _dvec(x), _dptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(self.nfreq, 6)
+
+    def get_farfields_array(self, vmin, vmax, resolution):
+        """dft_near2far::get_farfields_array (src/near2far.cpp:411-483): array
+        (6, 2, N, nfreq) and the grid dims with singletons dropped."""
+        rank = C.c_int(0)
+        dims = (C.c_size_t * 3)()
+        n = self.lib.mp_near2far_get_farfields_array(self.ptr, _dvec(vmin), _dvec(vmax), resolution,
+                                                     C.byref(rank), dims, None)
+        n = _check(self.lib, n)
+        out = np.empty(12 * n * self.nfreq)
+        _check(self.lib, self.lib.mp_near2far_get_farfields_array(self.ptr, _dvec(vmin), _dvec(vmax),
+                                                                  resolution, C.byref(rank), dims,
+                                                                  _dptr(out)))
+        return out.reshape(6, 2, n, self.nfreq), tuple(dims[r] for r in range(rank.value))
+
+    def get_farfields(self, vmin, vmax, resolution):
+        """The Python layer's get_farfields (python/simulation.py): a dict of complex arrays
+        of shape dims (+ nfreq when nfreq > 1) per component."""
+        EH, dims = self.get_farfields_array(vmin, vmax, resolution)
+        shape = dims + ((self.nfreq,) if self.nfreq > 1 else ())
+        names = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+        return {names[k]: (EH[k, 0] + 1j * EH[k, 1]).reshape(shape) for k in range(6)}
+
+    def flux(self, d, vmin, vmax, resolution):
+        """dft_near2far::flux(direction, where, resolution) (src/near2far.cpp:517-565)."""
+        out = np.empty(self.nfreq)
+        _check(self.lib, self.lib.mp_near2far_flux(self.ptr, d, _dvec(vmin), _dvec(vmax), resolution,
+                                                   _dptr(out)))
+        return out
+
+    def get_dft_array(self, c, num_freq):
+        """fields::get_dft_array(dft_near2far, c, num_freq, ...) (src/dft.cpp:1282-1290)."""
+        return _get_dft_array(self.lib.mp_near2far_get_dft_array, self.ptr, self.lib,
+                              self.f.s.single, c, num_freq)
+
+
+def green(lib, dim, x, freq, eps, mu, x0, c0, f0=1.0):
+    """green2d / green3d (src/near2far.cpp:133-267): the six fields at x of a point current
+    c0 of amplitude f0 at x0 in the medium eps, mu."""
+    out = np.empty(12)
+    f = _dvec([complex(f0).real, complex(f0).imag])
+    _check(lib, lib.mp_near2far_green(dim, _dvec(x), freq, eps, mu, _dvec(x0), c0, f, _dptr(out)))
+    return out[0::2] + 1j * out[1::2]
 
 
 class adjoint:
@@ -681,13 +764,33 @@ class fields:
         self._fluxes.append(df)
         return df
 
+    def add_dft_near2far(self, surfaces, freqs, decimation_factor=0, nperiods=1):
+        """fields::add_dft_near2far(volume_list, freq, Nfreq, decimation_factor, Nperiods)
+        (src/near2far.cpp:569-654).  surfaces: (vmin, vmax, weight) or (vmin, vmax, weight,
+        normal direction) tuples, the direction defaulting to the volume's normal."""
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        n = len(surfaces)
+        dirs = (C.c_int * n)(*[int(sf[3]) if len(sf) > 3 else -1 for sf in surfaces])
+        vmins = np.ascontiguousarray([[float(v) for v in (list(sf[0]) + [0.0] * 3)[:3]] for sf in surfaces])
+        vmaxs = np.ascontiguousarray([[float(v) for v in (list(sf[1]) + [0.0] * 3)[:3]] for sf in surfaces])
+        weights = np.ascontiguousarray([[complex(sf[2]).real, complex(sf[2]).imag] for sf in surfaces])
+        ptr = self.lib.mp_fields_add_dft_near2far(self.ptr, n, dirs, _dptr(vmins), _dptr(vmaxs), _dptr(weights),
+                                                  _dptr(freqs), len(freqs), decimation_factor, nperiods)
+        if not ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        nf = near2far(self, ptr, len(freqs))
+        self._fluxes.append(nf)
+        return nf
+
     def __del__(self):
         if getattr(self, "ptr", None):
             for adj in self._adjoints:
                 adj.close()
             for fl in self._fluxes:
                 if fl.ptr:
-                    if isinstance(fl, dft_fields):
+                    if isinstance(fl, near2far):
+                        self.lib.mp_near2far_delete(fl.ptr)
+                    elif isinstance(fl, dft_fields):
                         self.lib.mp_dft_fields_delete(fl.ptr)
                     else:
                         self.lib.mp_dft_flux_delete(fl.ptr)

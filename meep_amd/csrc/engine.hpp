@@ -370,4 +370,24 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
 // FLUX_BLOCKS partial sums of the plane flux into partial
 template <class R> void launch_flux_plane(const FluxPlaneArgs<R> &a, double *partial, hipStream_t s);
 
+// The near-to-far field sums (near2far.hip): nsrc equivalent currents at (sx, sy, sz) of
+// component comp with amplitudes f0[frequency][current], seen at npts far points (px, py, pz),
+// in the medium eps, mu of dimension dim (2 or 3).  Each (point, frequency) pair is split into
+// segs segments of the current list.
+struct FarArgs {
+  const double *sx, *sy, *sz;
+  const int *comp;
+  const double2 *f0;
+  int nsrc;
+  const double *px, *py, *pz;
+  int npts;
+  const double *freq;
+  int nfreq;
+  double eps, mu;
+  int dim, segs;
+};
+// partial: npts * nfreq * segs * 12 doubles of scratch; out: npts * nfreq * 12 doubles, the
+// (re, im) of Ex Ey Ez Hx Hy Hz per (point, frequency), point-major
+void launch_farfield(const FarArgs &a, double *partial, double *out, hipStream_t s);
+
 } // namespace mx

@@ -374,6 +374,12 @@ struct FieldsBase {
   // fields::get_dft_array over the flux (both lists) or dft_fields (E list) monitor fl;
   // returns the number of complex values
   virtual long dft_array(void *fl, bool both, int c, int num_freq, int *rank, size_t *dims, double *out) = 0;
+  // near2far: the surfaces (normal direction or -1, box, weight), and the six fields of every
+  // frequency at np far points (out: np * nfreq * 12 doubles, point-major)
+  virtual void *add_dft_near2far(int nsurf, const int *dirs, const double *vmins, const double *vmaxs, const double *weights,
+                                 const std::vector<double> &freq, int dec, int nperiods, int *dec_out) = 0;
+  virtual void n2f_info(void *n, double *eps_mu, int *periodic) = 0;
+  virtual void n2f_eval(void *n, const double *pts, long np, double *out) = 0;
   // time-domain flux: flux_vol planes, flux_in_box (sync) and flux_in_box_wrongH
   virtual void *add_flux_vol(int d, V3 mn, V3 mx) = 0;
   virtual double flux_vol_value(void *fv) = 0;
@@ -403,11 +409,26 @@ template <class R> struct DftChunk {
   bool weighted = false;
   cdouble stored_weight = 1.0;
   bool empty[3] = {};
+  int vc = -1; // near2far: the equivalent current's component
 };
 template <class R> struct Flux {
+  virtual ~Flux() = default;
   std::vector<DftChunk<R> *> E, H;
   int nfreq;
   V3 wmin, wmax; // the monitor volume
+};
+// dft_near2far (src/meep.hpp:1455-1520): the accumulators of the tangential fields (the E list)
+// with the medium and the periodic copies, and the equivalent-current list the far-field
+// kernel reads, kept on the device until the fields step again
+template <class R> struct N2F : Flux<R> {
+  std::vector<double> freq;
+  double eps = 0, mu = 0;
+  int periodic_d[2] = {-1, -1}, periodic_n[2] = {0, 0};
+  double periodic_k[2] = {0, 0}, period[2] = {0, 0};
+  long built_t = -1;
+  unsigned built_version = 0;
+  int nsrc = 0;
+  DevBuf sx, sy, sz, comp, f0, d_freq;
 };
 
 template <class R> struct Fields : FieldsBase {
@@ -490,9 +511,9 @@ template <class R> struct Fields : FieldsBase {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
-  static constexpr int NKIND = 7; // the MP_K_* classes
-  double k_ms[NKIND] = {0, 0, 0, 0, 0, 0, 0};
-  long k_n[NKIND] = {0, 0, 0, 0, 0, 0, 0};
+  static constexpr int NKIND = 8; // the MP_K_* classes
+  double k_ms[NKIND] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long k_n[NKIND] = {0, 0, 0, 0, 0, 0, 0, 0};
   int open_kind = -1;
   hipEvent_t open_ev = nullptr;
   hipEvent_t get_event() {
@@ -3335,12 +3356,13 @@ template <class R> struct Fields : FieldsBase {
   // fields::add_dft for one component (src/dft.cpp:175-235): on the centred grid with the
   // Yee-to-centre average, or on the component's own grid (centered false, no average)
   void add_dft(std::vector<DftChunk<R> *> &out, int c, const V3 &mn, const V3 &mxv, const std::vector<double> &freq, bool include_w,
-               cdouble stored_weight, int decimation, bool centered) {
+               cdouble stored_weight, int decimation, bool centered, int vc = -1) {
     loop_in_chunks(mn, mxv, centered ? 20 : c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &shift,
                                    cdouble sph) {
       if (!f[c][0]) return;
       std::unique_ptr<DftChunk<R>> ch(new DftChunk<R>);
       ch->c = c;
+      ch->vc = vc;
       ch->is = is, ch->ie = ie, ch->shift = shift;
       ch->s0 = s0, ch->s1 = s1, ch->e0 = e0, ch->e1 = e1;
       ch->dV0 = dV0;
@@ -3594,6 +3616,196 @@ template <class R> struct Fields : FieldsBase {
     }
     for (size_t q = 0; q < 2 * rsize; ++q) out[q] = double(red[q]);
     return long(rsize);
+  }
+  // fields::get_eps / get_mu at frequency 0 (src/monitor.cpp:135-195): the number of electric
+  // (magnetic) components over the sum of their chi1inv (in realnum) interpolated at p, each
+  // point located in the cell, vacuum outside it; the magnetic chi1inv is 1 here
+  double eps_mu_at(const V3 &p, bool magnetic) {
+    double tr = 0;
+    int nc = 0;
+    for (int c = magnetic ? 5 : 0; c < (magnetic ? 10 : 5); ++c) {
+      const bool has = gv.dim == 1 ? (c == 0 || c == 6) : comp_dir(c) != NODIR;
+      if (!has) continue;
+      IV locs[8];
+      double w[8];
+      gv.interpolate(c, p, locs, w);
+      double v = 0;
+      for (int q = 0; q < 8 && w[q] != 0; ++q) {
+        IV il = locs[q];
+        cdouble ph = 1.0;
+        locate(&il, &ph);
+        double u = 1.0;
+        if (gv.owns(il) && !magnetic && !s->chi1inv[c].empty()) u = double(R(s->chi1inv[c][size_t(gv.index(c, il))]));
+        v += w[q] * u;
+      }
+      tr += v;
+      ++nc;
+    }
+    return nc / tr;
+  }
+  // fields::add_dft_near2far (src/near2far.cpp:569-654): for each surface, the tangential E and
+  // H of the two directions transverse to its normal (cyclic order) accumulate with their
+  // integration weights, off the centred grid, as the equivalent currents n x H and -n x E
+  // (the other kind's component along the other tangential direction; sign +1 for the first
+  // direction, -1 for the second, negated for E; times the surface's complex weight, weights
+  // holding (re, im) pairs).  Each call's
+  // chunks go in front of the list, as the reference prepends them.
+  void *add_dft_near2far(int nsurf, const int *dirs, const double *vmins, const double *vmaxs, const double *weights,
+                         const std::vector<double> &freq, int dec, int nperiods, int *dec_out) override {
+    if (!components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+    if (nsurf < 1) fail("add_dft_near2far: no surfaces");
+    const int decim = dec > 0 ? dec : auto_decimation(freq);
+    *dec_out = decim;
+    std::unique_ptr<N2F<R>> n(new N2F<R>);
+    n->nfreq = int(freq.size());
+    n->freq = freq;
+    // dft_near2far::where is the first surface's volume (the union in the loop takes where->v)
+    auto v3 = [](const double *q) {
+      V3 v;
+      for (int d = 0; d < 3; ++d) v.t[d] = q[d];
+      return v;
+    };
+    n->wmin = v3(vmins), n->wmax = v3(vmaxs);
+    std::vector<DftChunk<R> *> front;
+    for (int w = 0; w < nsurf; ++w) {
+      const V3 mn = v3(vmins + 3 * w), mx = v3(vmaxs + 3 * w);
+      int nd = dirs[w];
+      if (nd < 0) nd = flux_direction(-1, mn, mx);
+      if (nd > 2) fail("unknown dft_near2far normal");
+      V3 cen;
+      for (int d = 0; d < 3; ++d) cen.t[d] = 0.5 * (mn.t[d] + mx.t[d]);
+      const double weps = eps_mu_at(cen, false), wmu = eps_mu_at(cen, true);
+      auto approxeq = [](double a, double b) { return std::fabs(a - b) < 0.5e-11 * (std::fabs(a) + std::fabs(b)); };
+      if (w > 0 && !(approxeq(n->eps, weps) && approxeq(n->mu, wmu))) fail("dft_near2far requires surfaces in a homogeneous medium");
+      n->eps = weps, n->mu = wmu;
+      const int fd[2] = {(nd + 1) % 3, (nd + 2) % 3};
+      if (nperiods > 1)
+        for (int i = 0; i < 2; ++i) {
+          const double user_width = gv.num[fd[i]] * gv.inva;
+          if (gv.has(fd[i]) && bc[High][fd[i]] == Periodic && bc[Low][fd[i]] == Periodic &&
+              float(mx.t[fd[i]] - mn.t[fd[i]]) >= float(user_width)) {
+            n->periodic_d[i] = fd[i];
+            n->periodic_n[i] = nperiods;
+            n->period[i] = user_width;
+            n->periodic_k[i] = 2 * pi * k[fd[i]].real() * user_width;
+          }
+        }
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+          const int dc[3] = {0, 1, 4};
+          const int c = (i == 0 ? 0 : 5) + dc[fd[j]], c0 = (i == 0 ? 5 : 0) + dc[fd[1 - j]];
+          const double sgn = (j == 0 ? 1.0 : -1.0) * (i == 0 ? -1.0 : 1.0);
+          std::vector<DftChunk<R> *> lst;
+          add_dft(lst, c, mn, mx, freq, true, sgn * cdouble(weights[2 * w], weights[2 * w + 1]), decim, false, c0);
+          front.insert(front.begin(), lst.rbegin(), lst.rend());
+        }
+    }
+    n->E = front;
+    fluxes.push_back(std::move(n));
+    return fluxes.back().get();
+  }
+  void n2f_info(void *h, double *eps_mu, int *periodic) override {
+    N2F<R> *n = static_cast<N2F<R> *>(h);
+    eps_mu[0] = n->eps, eps_mu[1] = n->mu;
+    for (int i = 0; i < 2; ++i) periodic[2 * i] = n->periodic_d[i], periodic[2 * i + 1] = n->periodic_n[i];
+  }
+  // The equivalent currents of dft_near2far::farfield_lowlevel (src/near2far.cpp:360-393): every
+  // accumulator point of every chunk at its location plus the chunk's periodic-image shift,
+  // and its periodic copies within periodic_n periods each way with the Bloch phase of their
+  // offset folded into the amplitude; uploaded frequency-major.
+  void n2f_sources(N2F<R> &n) {
+    if (n.built_t == t && n.built_version == dft_version && n.nsrc > 0) return;
+    const int Nf = n.nfreq;
+    std::vector<double> x, y, z;
+    std::vector<int> cc;
+    std::vector<cdouble> amp; // source-major here, transposed on upload
+    for (DftChunk<R> *ch : n.E) {
+      const std::vector<cdouble> acc = read_dft(*ch);
+      std::vector<V3> locs;
+      visit_box(ch->is, ch->ie, ch->s0, ch->s1, ch->e0, ch->e1, 1.0, [&](int64_t, int64_t, const IV &p, double, const int *) {
+        IV q = p;
+        for (int d = 0; d < 3; ++d) q.t[d] += ch->shift.t[d];
+        locs.push_back(gv.loc(q));
+      });
+      for (size_t kk = 0; kk < locs.size(); ++kk)
+        for (int i0 = -n.periodic_n[0]; i0 <= n.periodic_n[0]; ++i0)
+          for (int i1 = -n.periodic_n[1]; i1 <= n.periodic_n[1]; ++i1) {
+            V3 xs = locs[kk];
+            if (n.periodic_d[0] >= 0) xs.t[n.periodic_d[0]] += i0 * n.period[0];
+            if (n.periodic_d[1] >= 0) xs.t[n.periodic_d[1]] += i1 * n.period[1];
+            const cdouble ph = std::polar(1.0, i0 * n.periodic_k[0] + i1 * n.periodic_k[1]);
+            x.push_back(xs.t[0]), y.push_back(xs.t[1]), z.push_back(gv.dim == 2 ? 0.0 : xs.t[2]);
+            cc.push_back(ch->vc);
+            for (int i = 0; i < Nf; ++i) amp.push_back(acc[kk * Nf + i] * ph);
+          }
+    }
+    const size_t ns = x.size();
+    if (ns > size_t(INT32_MAX)) fail("near2far: too many surface points");
+    std::vector<double> fm(2 * ns * Nf);
+    for (size_t j = 0; j < ns; ++j)
+      for (int i = 0; i < Nf; ++i) {
+        fm[2 * (size_t(i) * ns + j)] = amp[j * Nf + i].real();
+        fm[2 * (size_t(i) * ns + j) + 1] = amp[j * Nf + i].imag();
+      }
+    n.nsrc = int(ns);
+    if (ns) {
+      auto put = [&](DevBuf &b, const void *src, size_t bytes) {
+        if (b.bytes != bytes) b.alloc(bytes);
+        HIPCHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+      };
+      put(n.sx, x.data(), 8 * ns), put(n.sy, y.data(), 8 * ns), put(n.sz, z.data(), 8 * ns);
+      put(n.comp, cc.data(), sizeof(int) * ns);
+      put(n.f0, fm.data(), 8 * fm.size());
+      put(n.d_freq, n.freq.data(), 8 * n.freq.size());
+    }
+    n.built_t = t, n.built_version = dft_version;
+  }
+  // dft_near2far::farfield_lowlevel (src/near2far.cpp:350-397) at np points on the device: the
+  // Green's function sums of near2far.hip, in batches of at most 2^20 (point, frequency) pairs
+  void n2f_eval(void *h, const double *pts, long np, double *out) override {
+    N2F<R> &n = *static_cast<N2F<R> *>(h);
+    if (gv.dim != 2 && gv.dim != 3) fail("only 2d or 3d or cylindrical far-field computation is supported");
+    HIPCHECK(hipStreamSynchronize(stream));
+    n2f_sources(n);
+    const int Nf = n.nfreq;
+    if (np <= 0) return;
+    if (n.nsrc == 0 || Nf == 0) {
+      std::fill(out, out + size_t(np) * Nf * 12, 0.0);
+      return;
+    }
+    const long batch = std::max(1L, (1L << 20) / Nf);
+    const long nb = std::min(np, batch);
+    const long pairs_max = nb * Nf;
+    const int segs = int(std::max(1L, std::min<long>((n.nsrc + 511) / 512, (4096 + pairs_max - 1) / pairs_max)));
+    DevBuf dp, partial, dout;
+    dp.alloc(8 * 3 * size_t(nb));
+    partial.alloc(8 * 12 * size_t(pairs_max) * segs);
+    dout.alloc(8 * 12 * size_t(pairs_max));
+    std::vector<double> soa(3 * size_t(nb));
+    for (long b0 = 0; b0 < np; b0 += nb) {
+      const long m = std::min(nb, np - b0);
+      for (long q = 0; q < m; ++q)
+        for (int d = 0; d < 3; ++d) soa[size_t(d) * nb + q] = (gv.dim == 2 && d == 2) ? 0.0 : pts[3 * (b0 + q) + d];
+      HIPCHECK(hipMemcpyAsync(dp.p, soa.data(), 8 * 3 * size_t(nb), hipMemcpyHostToDevice, stream));
+      FarArgs a;
+      a.sx = static_cast<const double *>(n.sx.p), a.sy = static_cast<const double *>(n.sy.p), a.sz = static_cast<const double *>(n.sz.p);
+      a.comp = static_cast<const int *>(n.comp.p);
+      a.f0 = static_cast<const double2 *>(n.f0.p);
+      a.nsrc = n.nsrc;
+      a.px = static_cast<const double *>(dp.p), a.py = a.px + nb, a.pz = a.px + 2 * nb;
+      a.npts = int(m);
+      a.freq = static_cast<const double *>(n.d_freq.p);
+      a.nfreq = Nf;
+      a.eps = n.eps, a.mu = n.mu;
+      a.dim = gv.dim;
+      a.segs = segs;
+      kbegin(7);
+      launch_farfield(a, static_cast<double *>(partial.p), static_cast<double *>(dout.p), stream);
+      HIPCHECK(hipGetLastError());
+      kend();
+      HIPCHECK(hipMemcpyAsync(out + size_t(b0) * Nf * 12, dout.p, 8 * 12 * size_t(m) * Nf, hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
   }
   void flux_delete(void *flp) override {
     for (size_t q = 0; q < fluxes.size(); ++q)
@@ -4048,6 +4260,52 @@ struct FluxHandle {
   void *fl;
   int nfreq, decimation;
 };
+
+// The output grid of dft_near2far::get_farfields_array (src/near2far.cpp:411-427):
+// floor(extent * resolution) points along each direction of the cell's dimensionality, spaced
+// extent / (n - 1) from the volume's low corner (one point at the low corner when n <= 1).
+struct FarGrid {
+  int rank = 0, dirs[3] = {0, 1, 2};
+  size_t dims[3] = {1, 1, 1}, N = 1;
+  double dx[3] = {0, 0, 0};
+};
+static FarGrid far_grid(int dim, const V3 &mn, const V3 &mx, double resolution) {
+  FarGrid g;
+  for (int d = 0; d < 3; ++d) {
+    const bool has = dim == 1 ? d == 2 : (dim == 2 ? d != 2 : true);
+    if (!has) continue;
+    const double ext = mx.t[d] - mn.t[d];
+    size_t n = size_t(std::max(0, int(std::floor(ext * resolution))));
+    if (n <= 1)
+      n = 1;
+    else
+      g.dx[g.rank] = ext / double(n - 1);
+    g.dims[g.rank] = n;
+    g.N *= n;
+    g.dirs[g.rank++] = d;
+  }
+  return g;
+}
+// dft_near2far::get_farfields_array (src/near2far.cpp:411-483): every grid point's fields in one
+// far-field evaluation, rearranged to the 6 x 2 x N x Nfreq row-major array
+static std::vector<double> farfields_array(FluxHandle *h, const FarGrid &g, const V3 &mn) {
+  const size_t N = g.N, Nf = size_t(h->nfreq);
+  std::vector<double> pts(3 * N, 0.0);
+  for (size_t i0 = 0; i0 < g.dims[0]; ++i0)
+    for (size_t i1 = 0; i1 < g.dims[1]; ++i1)
+      for (size_t i2 = 0; i2 < g.dims[2]; ++i2) {
+        const size_t idx = (i0 * g.dims[1] + i1) * g.dims[2] + i2, ii[3] = {i0, i1, i2};
+        for (int r = 0; r < g.rank; ++r) pts[3 * idx + g.dirs[r]] = mn.t[g.dirs[r]] + double(ii[r]) * g.dx[r];
+      }
+  std::vector<double> raw(N * Nf * 12);
+  h->owner->n2f_eval(h->fl, pts.data(), long(N), raw.data());
+  std::vector<double> out(12 * N * Nf);
+  for (size_t idx = 0; idx < N; ++idx)
+    for (size_t i = 0; i < Nf; ++i)
+      for (int k = 0; k < 6; ++k)
+        for (int ri = 0; ri < 2; ++ri) out[((k * 2 + ri) * N + idx) * Nf + i] = raw[(idx * Nf + i) * 12 + 2 * k + ri];
+  return out;
+}
 
 } // namespace mx
 
@@ -4619,6 +4877,109 @@ long mp_dft_flux_get_dft_array(mp_dft_flux *fl, int c, int num_freq, int *rank, 
 }
 long mp_dft_fields_get_dft_array(mp_dft_fields *df, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(df, false, c, num_freq, rank, dims, out);
+}
+
+mp_near2far *mp_fields_add_dft_near2far(mp_fields *f, int nsurf, const int *dirs, const double *vmins, const double *vmaxs,
+                                        const double *weights, const double *freq, int nfreq, int dec, int nperiods) {
+  FluxHandle *h = new FluxHandle;
+  int rc = guard([&] {
+    std::vector<double> fr(freq, freq + nfreq);
+    h->owner = F_;
+    h->nfreq = nfreq;
+    h->fl = F_->add_dft_near2far(nsurf, dirs, vmins, vmaxs, weights, fr, dec, nperiods, &h->decimation);
+  });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_near2far *>(h);
+}
+void mp_near2far_delete(mp_near2far *n) { mp_dft_flux_delete(reinterpret_cast<mp_dft_flux *>(n)); }
+int mp_near2far_info(const mp_near2far *n, double *eps_mu, int *periodic) {
+  const FluxHandle *h = reinterpret_cast<const FluxHandle *>(n);
+  if (guard([&] { h->owner->n2f_info(h->fl, eps_mu, periodic); })) return -1;
+  return h->nfreq;
+}
+int mp_near2far_farfield(mp_near2far *n, const double *x, double *EH) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(n);
+  return guard([&] { h->owner->n2f_eval(h->fl, x, 1, EH); });
+}
+long mp_near2far_get_farfields_array(mp_near2far *n, const double *wmin, const double *wmax, double resolution, int *rank,
+                                     size_t *dims, double *out) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(n);
+  long count = -1;
+  const int rc = guard([&] {
+    const V3 mn = to_v3(wmin), mx = to_v3(wmax);
+    const FarGrid g = far_grid(h->owner->dimension(), mn, mx, resolution);
+    count = long(g.N);
+    int rr = 0;
+    dims[0] = dims[1] = dims[2] = 1;
+    for (int r = 0; r < g.rank; ++r)
+      if (g.dims[r] > 1) dims[rr++] = g.dims[r];
+    *rank = rr;
+    if (out) {
+      const std::vector<double> a = farfields_array(h, g, mn);
+      std::copy(a.begin(), a.end(), out);
+    }
+  });
+  return rc ? -1 : count;
+}
+// dft_near2far::flux (src/near2far.cpp:517-565): Re(E1 conj(H1)) - Re(E2 conj(H2)) of the
+// transverse pair in cyclic order, summed over the grid, times the grid's cell volume
+int mp_near2far_flux(mp_near2far *n, int d, const double *wmin, const double *wmax, double resolution, double *out) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(n);
+  return guard([&] {
+    const int dim = h->owner->dimension();
+    const bool has = dim == 1 ? d == 2 : (dim == 2 ? d == 0 || d == 1 : d >= 0 && d <= 2);
+    if (!has) fail("cannot get flux for near2far: co-ordinate mismatch");
+    const V3 mn = to_v3(wmin), mx = to_v3(wmax);
+    const FarGrid g = far_grid(dim, mn, mx, resolution);
+    const std::vector<double> EH = farfields_array(h, g, mn);
+    const size_t N = g.N, Nf = size_t(h->nfreq);
+    const int cE[3][2] = {{1, 2}, {2, 0}, {0, 1}}, cH[3][2] = {{5, 4}, {3, 5}, {4, 3}};
+    for (size_t i = 0; i < Nf; ++i) out[i] = 0;
+    auto at = [&](int k, size_t idx, size_t i) {
+      return cdouble(EH[((k * 2 + 0) * N + idx) * Nf + i], EH[((k * 2 + 1) * N + idx) * Nf + i]);
+    };
+    for (size_t idx = 0; idx < N; ++idx)
+      for (size_t i = 0; i < Nf; ++i)
+        for (int j = 0; j < 2; ++j) out[i] += std::real(at(cE[d][j], idx, i) * std::conj(at(cH[d][j], idx, i))) * (1 - 2 * j);
+    double dV = 1;
+    for (int q = 0; q < 3; ++q) {
+      const bool hq = dim == 1 ? q == 2 : (dim == 2 ? q != 2 : true);
+      if (!hq) continue;
+      const int nn = int(std::floor((mx.t[q] - mn.t[q]) * resolution));
+      if (nn > 1) dV *= (mx.t[q] - mn.t[q]) / (nn - 1);
+    }
+    for (size_t i = 0; i < Nf; ++i) out[i] *= dV;
+  });
+}
+long mp_near2far_get_dft_array(mp_near2far *n, int c, int num_freq, int *rank, size_t *dims, double *out) {
+  return dft_array(n, false, c, num_freq, rank, dims, out);
+}
+// green2d / green3d on the device: the far-field kernel with one current and one point
+int mp_near2far_green(int dim, const double *x, double freq, double eps, double mu, const double *x0, int c0, const double *f0,
+                      double *EH) {
+  return guard([&] {
+    ensure_gpu();
+    if (dim != 2 && dim != 3) fail("only 2d or 3d Green's functions");
+    if (c0 < 0 || c0 >= 10 || comp_dir(c0) == NODIR) fail("unrecognized source type");
+    // [sx sy sz px py pz freq f0.re f0.im] then the component, the partial and the output
+    double host[9] = {x0[0], x0[1], dim == 3 ? x0[2] : 0.0, x[0], x[1], dim == 3 ? x[2] : 0.0, freq, f0[0], f0[1]};
+    DevBuf buf, cb, part, outb;
+    buf.alloc(sizeof host), cb.alloc(sizeof(int)), part.alloc(12 * 8), outb.alloc(12 * 8);
+    HIPCHECK(hipMemcpy(buf.p, host, sizeof host, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(cb.p, &c0, sizeof(int), hipMemcpyHostToDevice));
+    const double *d = static_cast<const double *>(buf.p);
+    FarArgs a;
+    a.sx = d, a.sy = d + 1, a.sz = d + 2, a.px = d + 3, a.py = d + 4, a.pz = d + 5, a.freq = d + 6;
+    a.f0 = reinterpret_cast<const double2 *>(d + 7);
+    a.comp = static_cast<const int *>(cb.p);
+    a.nsrc = 1, a.npts = 1, a.nfreq = 1, a.eps = eps, a.mu = mu, a.dim = dim, a.segs = 1;
+    launch_farfield(a, static_cast<double *>(part.p), static_cast<double *>(outb.p), nullptr);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpy(EH, outb.p, 12 * 8, hipMemcpyDeviceToHost));
+  });
 }
 
 } // extern "C"

@@ -67,6 +67,8 @@
  *   MUT_PML_STRETCH   a uPML term's kappa divides by the profile's integral, not that of u profile(u)
  *   MUT_AMPARR_OFFSET an amplitude array is interpolated without its half-cell offset
  *   MUT_FIXBND_PHASE  a source point moved to its periodic owner keeps the phase, not its conjugate
+ *   MUT_N2F_SIGN      near2far's equivalent magnetic current takes +n x E instead of -n x E
+ *   MUT_N2F_PHASE     near2far's periodic copies add without their Bloch phase
  */
 #include "../include/meep_amd.h"
 
@@ -641,6 +643,7 @@ template <class R> struct DftChunk {
   cdouble scale, stored_weight = 1.0;
   ivec shift;                   // the periodic image this piece of the monitor stands for
   bool empty_dim[3] = {};       // the monitor volume has no extent in this direction
+  int vc = -1;                  // near2far: the equivalent source component
   bool include_w;
   std::vector<double> omega;
   int decimation = 1;
@@ -1260,6 +1263,31 @@ template <class R> struct Fields {
     }
     return 0.0;
   }
+  // fields::get_eps / get_mu at frequency 0 (src/monitor.cpp:135-195): the number of electric
+  // (magnetic) components over the sum of their chi1inv interpolated at p, each point located
+  // in the cell, vacuum outside it
+  double get_eps_mu(const vec3 &p, bool magnetic) const {
+    double tr = 0;
+    int nc = 0;
+    for (int c = magnetic ? 5 : 0; c < (magnetic ? 10 : 5); ++c) {
+      if (!has_field(c)) continue;
+      ivec locs[8];
+      double w[8];
+      gv.interpolate(c, p, locs, w);
+      double v = 0;
+      for (int q = 0; q < 8 && w[q] != 0; ++q) {
+        ivec il = locs[q];
+        cdouble ph = 1.0;
+        locate_point_in_user_volume(&il, &ph);
+        double u = 1.0;
+        if (gv.owns(il)) u = s->chi1inv[c].empty() ? 1.0 : double(s->chi1inv[c][gv.index(c, il)]);
+        v += w[q] * u;
+      }
+      tr += v;
+      ++nc;
+    }
+    return nc / tr;
+  }
   cdouble get_field(int c, const vec3 &p) const {
     ivec locs[8];
     double w[8];
@@ -1784,12 +1812,13 @@ template <class R> struct Fields {
   // fields::add_dft (src/dft.cpp:175-235) for one component, on the centred grid or (centered
   // false) on the component's own grid without the centring average (src/dft.cpp:91-94,233)
   void add_dft(std::vector<std::unique_ptr<DftChunk<R>>> &out_list, int c, vec3 wmin, vec3 wmax, const std::vector<double> &freq,
-               bool include_w, cdouble stored_weight, int decimation, bool centered = true) {
+               bool include_w, cdouble stored_weight, int decimation, bool centered = true, int vc = -1) {
     loop_in_chunks(wmin, wmax, centered ? 20 : c, [&](const ivec &is, const ivec &ie, const vec3 &s0, const vec3 &s1, const vec3 &e0, const vec3 &e1,
                                        double dV0, const ivec &shift, cdouble shift_phase) {
       if (!f[c][0]) return;
       std::unique_ptr<DftChunk<R>> ch(new DftChunk<R>);
       ch->c = c;
+      ch->vc = vc;
       ch->stored_weight = stored_weight;
       ch->shift = shift;
       for (int d = 0; d < 3; ++d) ch->empty_dim[d] = gv.has(d) && wmin.t[d] == wmax.t[d];
@@ -2764,6 +2793,398 @@ int mp_dft_fields_get_dft(mp_dft_fields *h, int i, double *out) {
 
 long mp_dft_fields_get_dft_array(mp_dft_fields *h, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(reinterpret_cast<AnyFlux *>(h), false, c, num_freq, rank, dims, out);
+}
+
+} // extern "C"
+
+namespace {
+
+// ---- near-to-far transformation (src/near2far.cpp) ----------------------------------------
+// The fields EH = (Ex, Ey, Ez, Hx, Hy, Hz) at x radiated by a point current f0 of component c0
+// at x0 in the homogeneous medium (eps, mu), at frequency freq, for the e^{-i omega t}
+// convention of the DFT accumulators.
+
+// green3d (src/near2far.cpp:133-186): the dipole fields written for a current rather than a
+// dipole moment (the moment's fields divided by -i omega).  The radiated field of a current
+// of one kind (E for J, H for a magnetic current) is pre (t1 p + t2 (p.r) r), the other kind
+// is pre t3 (r x p) divided (J) or multiplied (magnetic current, negated) by the impedance.
+void near2far_green3d(cdouble EH[6], const vec3 &x, double freq, double eps, double mu, const vec3 &x0, int c0,
+                      cdouble f0) {
+  double rh[3], r2 = 0;
+  for (int d = 0; d < 3; ++d) {
+    rh[d] = x.t[d] - x0.t[d];
+    r2 += rh[d] * rh[d];
+  }
+  const double r = std::sqrt(r2);
+  for (int d = 0; d < 3; ++d) rh[d] /= r;
+  const double n = std::sqrt(eps * mu), k = 2 * pi * freq * n, kr = k * r, imp = std::sqrt(mu / eps);
+  const cdouble ikr(0.0, kr);
+  const double ikr2 = -kr * kr;
+  cdouble pre = f0 * std::polar(k * n / (4 * pi * r), kr + 0.5 * pi);
+  const int pd = component_direction(c0);
+  if (pd == NODIR) fail("unrecognized source type");
+  const double p[3] = {pd == X ? 1.0 : 0.0, pd == Y ? 1.0 : 0.0, pd == Z ? 1.0 : 0.0};
+  const double prh = rh[pd];
+  const double rxp[3] = {rh[1] * p[2] - rh[2] * p[1], rh[2] * p[0] - rh[0] * p[2], rh[0] * p[1] - rh[1] * p[0]};
+  const cdouble t1 = 1.0 - 1.0 / ikr + 1.0 / ikr2, t2 = (-1.0 + 3.0 / ikr - 3.0 / ikr2) * prh, t3 = 1.0 - 1.0 / ikr;
+  const bool electric = is_electric(c0);
+  if (!electric && !is_magnetic(c0)) fail("unrecognized source type");
+  pre /= electric ? eps : mu;
+  cdouble *same = electric ? EH : EH + 3, *other = electric ? EH + 3 : EH;
+  for (int d = 0; d < 3; ++d) {
+    same[d] = pre * (t1 * p[d] + t2 * rh[d]);
+    other[d] = electric ? pre * t3 * rxp[d] / imp : -pre * t3 * rxp[d] * imp;
+  }
+}
+
+// hankel (src/near2far.cpp:188-204): H_n^(1)(x) = J_n(x) + i Y_n(x), from the C library's jn/yn
+cdouble near2far_hankel(int n, double x) { return cdouble(jn(n, x), yn(n, x)); }
+
+// green2d (src/near2far.cpp:207-267): the line-current fields from the Hankel functions of
+// order 0, 1 and 2 of k r.  An out-of-plane current (Ez or Hz) radiates its own component
+// -omega (mu or eps) H0 / 4 and the in-plane field of the other kind along z x r; an in-plane
+// current radiates the out-of-plane field of the other kind (r x p) i k H1 / 4 and its own
+// in-plane field from H1 (along r) and H0 - H2 (across r).
+void near2far_green2d(cdouble EH[6], const vec3 &x, double freq, double eps, double mu, const vec3 &x0, int c0,
+                      cdouble f0) {
+  const double dx = x.t[0] - x0.t[0], dy = x.t[1] - x0.t[1], r = std::sqrt(dx * dx + dy * dy);
+  const double rx = dx / r, ry = dy / r;
+  const double omega = 2 * pi * freq, k = omega * std::sqrt(eps * mu), kr = k * r, imp = std::sqrt(mu / eps);
+  const cdouble H0 = near2far_hankel(0, kr) * f0, H1 = near2far_hankel(1, kr) * f0;
+  const cdouble ikH1 = 0.25 * cdouble(0.0, k) * H1;
+  for (int j = 0; j < 6; ++j) EH[j] = 0.0;
+  const int pd = component_direction(c0);
+  const bool electric = is_electric(c0);
+  if (pd == Z) {
+    if (electric) {
+      EH[2] = (-0.25 * omega * mu) * H0;
+      EH[3] = -ry * ikH1;
+      EH[4] = rx * ikH1;
+    }
+    else {
+      EH[0] = ry * ikH1;
+      EH[1] = -rx * ikH1;
+      EH[5] = (-0.25 * omega * eps) * H0;
+    }
+    return;
+  }
+  const cdouble H2 = near2far_hankel(2, kr) * f0;
+  const double px = pd == X ? 1.0 : 0.0, py = pd == Y ? 1.0 : 0.0;
+  const double prh = px * rx + py * ry, rxp = rx * py - ry * px;
+  // the in-plane pair of the source's own kind, and the out-of-plane field of the other kind
+  const double radial = prh / r * 0.25 * (electric ? imp : 1.0 / imp), across = rxp * omega * (electric ? mu : eps) * 0.125;
+  cdouble *own = electric ? EH : EH + 3;
+  own[0] = -(rx * radial) * H1 + (ry * across) * (H0 - H2);
+  own[1] = -(ry * radial) * H1 - (rx * across) * (H0 - H2);
+  if (electric)
+    EH[5] = -rxp * ikH1;
+  else
+    EH[2] = rxp * ikH1;
+}
+
+struct AnyN2F : AnyFlux {
+  double eps = 0, mu = 0;
+  int periodic_d[2] = {NODIR, NODIR}, periodic_n[2] = {0, 0};
+  double periodic_k[2] = {0, 0}, period[2] = {0, 0};
+};
+
+// dft_near2far::farfield_lowlevel (src/near2far.cpp:350-397): every accumulator point of every
+// chunk (its location plus the chunk's periodic-image shift) is a point current of the chunk's
+// equivalent-source component with the accumulator as amplitude; the periodic copies within
+// periodic_n periods add with the Bloch phase of their offset.
+template <class R> void near2far_point(const AnyN2F &n2f, const GV &gv, const vec3 &xin, cdouble *EH) {
+  if (gv.dim != 2 && gv.dim != 3) fail("only 2d or 3d or cylindrical far-field computation is supported");
+  vec3 x = xin;
+  if (gv.dim == 2) x.t[Z] = 0;
+  const size_t Nf = n2f.freq.size();
+  for (size_t i = 0; i < 6 * Nf; ++i) EH[i] = 0.0;
+  for (void *p : n2f.E) {
+    const DftChunk<R> &ch = *reinterpret_cast<const DftChunk<R> *>(p);
+    std::vector<vec3> locs;
+    loop_ivecs(gv, ch.is, ch.ie, [&](ptrdiff_t, const Loop &L, ptrdiff_t i1, ptrdiff_t i2, ptrdiff_t i3) {
+      ivec at = ch.shift;
+      const ptrdiff_t ik[3] = {i1, i2, i3};
+      for (int k = 0; k < 3; ++k)
+        if (L.pres[k]) at.t[L.d[k]] += int(L.is[k] + 2 * ik[k]);
+      locs.push_back(gv.loc(at));
+    });
+    for (size_t i = 0; i < Nf; ++i)
+      for (size_t q = 0; q < locs.size(); ++q) {
+        const cdouble f0 = ch.acc == MP_DFT_F64 ? ch.dft64[Nf * q + i] : cdouble(ch.dft[Nf * q + i]);
+        vec3 xs = locs[q];
+        for (int i0 = -n2f.periodic_n[0]; i0 <= n2f.periodic_n[0]; ++i0) {
+          if (n2f.periodic_d[0] != NODIR) xs.t[n2f.periodic_d[0]] = locs[q].t[n2f.periodic_d[0]] + i0 * n2f.period[0];
+          for (int i1 = -n2f.periodic_n[1]; i1 <= n2f.periodic_n[1]; ++i1) {
+            if (n2f.periodic_d[1] != NODIR) xs.t[n2f.periodic_d[1]] = locs[q].t[n2f.periodic_d[1]] + i1 * n2f.period[1];
+            const cdouble cphase = std::polar(1.0, i0 * n2f.periodic_k[0] + i1 * n2f.periodic_k[1]);
+            cdouble EH6[6];
+            if (gv.dim == 2)
+              near2far_green2d(EH6, x, n2f.freq[i], n2f.eps, n2f.mu, xs, ch.vc, f0);
+            else
+              near2far_green3d(EH6, x, n2f.freq[i], n2f.eps, n2f.mu, xs, ch.vc, f0);
+#ifdef MUT_N2F_PHASE
+            for (int j = 0; j < 6; ++j) EH[i * 6 + j] += EH6[j];
+#else
+            for (int j = 0; j < 6; ++j) EH[i * 6 + j] += EH6[j] * cphase;
+#endif
+          }
+        }
+      }
+  }
+}
+
+// The output grid of dft_near2far::get_farfields_array (src/near2far.cpp:411-427): floor(extent
+// resolution) points along each direction of the cell's dimensionality, spaced extent / (n - 1)
+// from the volume's low corner (one point at the low corner when n <= 1).
+struct FarGrid {
+  int rank = 0, dirs[3] = {X, Y, Z};
+  size_t dims[3] = {1, 1, 1}, N = 1;
+  double dx[3] = {0, 0, 0};
+};
+FarGrid far_grid(const GV &gv, const vec3 &mn, const vec3 &mx, double resolution) {
+  FarGrid g;
+  for (int d = 0; d < 3; ++d) {
+    if (!gv.has(d)) continue;
+    const double ext = mx.t[d] - mn.t[d];
+    size_t n = size_t(std::max(0, int(std::floor(ext * resolution))));
+    if (n <= 1)
+      n = 1;
+    else
+      g.dx[g.rank] = ext / double(n - 1);
+    g.dims[g.rank] = n;
+    g.N *= n;
+    g.dirs[g.rank++] = d;
+  }
+  return g;
+}
+
+// dft_near2far::get_farfields_array (src/near2far.cpp:411-483): the fields at every grid point
+// as a 6 x 2 x N x Nfreq row-major array (component, real/imaginary part, point, frequency),
+// the singleton dimensions then dropped from dims.
+template <class R> std::vector<double> farfields_array(const AnyN2F &n2f, const GV &gv, const vec3 &mn, const vec3 &mx,
+                                                       double resolution, int *rank, size_t *dims) {
+  const FarGrid g = far_grid(gv, mn, mx, resolution);
+  const size_t Nf = n2f.freq.size(), N = g.N;
+  std::vector<double> out(6 * 2 * N * Nf, 0.0);
+  std::vector<cdouble> EH1(6 * Nf);
+  for (size_t i0 = 0; i0 < g.dims[0]; ++i0)
+    for (size_t i1 = 0; i1 < g.dims[1]; ++i1)
+      for (size_t i2 = 0; i2 < g.dims[2]; ++i2) {
+        vec3 x;
+        const size_t ii[3] = {i0, i1, i2};
+        for (int r = 0; r < g.rank; ++r) x.t[g.dirs[r]] = mn.t[g.dirs[r]] + double(ii[r]) * g.dx[r];
+        near2far_point<R>(n2f, gv, x, EH1.data());
+        const size_t idx = (i0 * g.dims[1] + i1) * g.dims[2] + i2;
+        for (size_t i = 0; i < Nf; ++i)
+          for (int k = 0; k < 6; ++k) {
+            out[((k * 2 + 0) * N + idx) * Nf + i] = EH1[i * 6 + k].real();
+            out[((k * 2 + 1) * N + idx) * Nf + i] = EH1[i * 6 + k].imag();
+          }
+      }
+  int rr = 0;
+  dims[0] = dims[1] = dims[2] = 1;
+  for (int r = 0; r < g.rank; ++r)
+    if (g.dims[r] > 1) dims[rr++] = g.dims[r];
+  *rank = rr;
+  return out;
+}
+
+// dft_near2far::flux (src/near2far.cpp:517-565): the far-field Poynting flux along df summed
+// over the grid, Re(E1 conj(H1)) - Re(E2 conj(H2)) with the pair of transverse components in
+// cyclic order, times the grid's cell volume (the spacing of each direction of more than one
+// point).
+void near2far_flux_sum(const std::vector<double> &EH, const FarGrid &g, size_t Nf, int df, const GV &gv, const vec3 &mn,
+                       const vec3 &mx, double resolution, double *F) {
+  int cE[2], cH[2];
+  if (df == X)
+    cE[0] = 1, cE[1] = 2, cH[0] = 5, cH[1] = 4;
+  else if (df == Y)
+    cE[0] = 2, cE[1] = 0, cH[0] = 3, cH[1] = 5;
+  else
+    cE[0] = 0, cE[1] = 1, cH[0] = 4, cH[1] = 3;
+  const size_t N = g.N;
+  for (size_t i = 0; i < Nf; ++i) F[i] = 0;
+  auto at = [&](int k, size_t idx, size_t i) {
+    return cdouble(EH[((k * 2 + 0) * N + idx) * Nf + i], EH[((k * 2 + 1) * N + idx) * Nf + i]);
+  };
+  for (size_t idx = 0; idx < N; ++idx)
+    for (size_t i = 0; i < Nf; ++i)
+      for (int j = 0; j < 2; ++j) F[i] += std::real(at(cE[j], idx, i) * std::conj(at(cH[j], idx, i))) * (1 - 2 * j);
+  double dV = 1;
+  for (int d = 0; d < 3; ++d)
+    if (gv.has(d)) {
+      const int n = int(std::floor((mx.t[d] - mn.t[d]) * resolution));
+      if (n > 1) dV *= (mx.t[d] - mn.t[d]) / (n - 1);
+    }
+  for (size_t i = 0; i < Nf; ++i) F[i] *= dV;
+}
+
+bool n2f_approxeq(double a, double b) { return std::fabs(a - b) < 0.5e-11 * (std::fabs(a) + std::fabs(b)); }
+
+}  // namespace
+
+extern "C" {
+
+// fields::add_dft_near2far (src/near2far.cpp:569-654): for each surface (normal nd from its
+// direction or, when that is -1, volume::normal_direction) the tangential E and H components
+// of the two directions transverse to nd in cyclic order are accumulated with their
+// integration weights, off the centred grid, as the equivalent currents n x H (electric,
+// component of the other tangential direction) and -n x E (magnetic): sign +1 for the first
+// tangential direction and -1 for the second, negated for E, times the surface's (complex)
+// weight, given as (re, im) pairs.  The
+// medium is eps, mu at the surface's centre and must be the same for all surfaces; with
+// nperiods > 1 a tangential direction that is periodic and spanned by the surface is summed
+// over nperiods copies each way.
+mp_near2far *mp_fields_add_dft_near2far(mp_fields *f, int nsurf, const int *dirs, const double *vmins,
+                                        const double *vmaxs, const double *weights, const double *freq, int nfreq,
+                                        int dec, int nperiods) {
+  AnyFields *af = reinterpret_cast<AnyFields *>(f);
+  AnyN2F *n2f = new AnyN2F;
+  n2f->owner = af;
+  n2f->freq.assign(freq, freq + nfreq);
+  int rc = guard([&] {
+    with_fields(af, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      if (!F.components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+      if (nsurf < 1) fail("add_dft_near2far: no surfaces");
+      const int decim = dec > 0 ? dec : F.auto_decimation(n2f->freq);
+      n2f->decimation = decim;
+      // dft_near2far::where is the first surface's volume (the union in the loop takes where->v)
+      n2f->wmin = to_vec(vmins);
+      n2f->wmax = to_vec(vmaxs);
+      std::vector<void *> front;
+      for (int w = 0; w < nsurf; ++w) {
+        const vec3 mn = to_vec(vmins + 3 * w), mx = to_vec(vmaxs + 3 * w);
+        int nd = dirs[w];
+        if (nd < 0) nd = flux_direction(F.gv, -1, mn, mx);
+        if (nd > Z) fail("unknown dft_near2far normal");
+        vec3 cen;
+        for (int d = 0; d < 3; ++d) cen.t[d] = 0.5 * (mn.t[d] + mx.t[d]);
+        const double weps = F.get_eps_mu(cen, false), wmu = F.get_eps_mu(cen, true);
+        if (w > 0 && !(n2f_approxeq(n2f->eps, weps) && n2f_approxeq(n2f->mu, wmu)))
+          fail("dft_near2far requires surfaces in a homogeneous medium");
+        n2f->eps = weps;
+        n2f->mu = wmu;
+        const int fd[2] = {(nd + 1) % 3, (nd + 2) % 3};
+        if (nperiods > 1)
+          for (int i = 0; i < 2; ++i) {
+            const double user_width = F.gv.num[fd[i]] * F.gv.inva;
+            if (F.gv.has(fd[i]) && F.boundaries[High][fd[i]] == Periodic && F.boundaries[Low][fd[i]] == Periodic &&
+                float(mx.t[fd[i]] - mn.t[fd[i]]) >= float(user_width)) {
+              n2f->periodic_d[i] = fd[i];
+              n2f->periodic_n[i] = nperiods;
+              n2f->period[i] = user_width;
+              n2f->periodic_k[i] = 2 * pi * F.k[fd[i]].real() * user_width;
+            }
+          }
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) {
+            const int c = direction_component(i == 0 ? 0 : 5, fd[j]);
+            const int c0 = direction_component(i == 0 ? 5 : 0, fd[1 - j]);
+            double s = j == 0 ? 1 : -1;
+#ifndef MUT_N2F_SIGN
+            if (is_electric(c)) s = -s;
+#endif
+            std::vector<std::unique_ptr<DftChunk<R>>> lst;
+            F.add_dft(lst, c, mn, mx, n2f->freq, true, s * cdouble(weights[2 * w], weights[2 * w + 1]), decim, false, c0);
+            // each call's chunks go in front of the list built so far
+            std::vector<void *> mine;
+            for (int q = int(lst.size()) - 1; q >= 0; --q) {
+              mine.push_back(lst[q].get());
+              F.dft_chunks.push_back(std::move(lst[q]));
+            }
+            front.insert(front.begin(), mine.begin(), mine.end());
+          }
+      }
+      n2f->E = front;
+      return 0;
+    });
+  });
+  if (rc) {
+    delete n2f;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_near2far *>(n2f);
+}
+void mp_near2far_delete(mp_near2far *h) { delete reinterpret_cast<AnyN2F *>(h); }
+int mp_near2far_info(const mp_near2far *h, double *eps_mu, int *periodic) {
+  const AnyN2F *n = reinterpret_cast<const AnyN2F *>(h);
+  eps_mu[0] = n->eps, eps_mu[1] = n->mu;
+  for (int i = 0; i < 2; ++i) periodic[2 * i] = n->periodic_d[i] == NODIR ? -1 : n->periodic_d[i], periodic[2 * i + 1] = n->periodic_n[i];
+  return int(n->freq.size());
+}
+int mp_near2far_farfield(mp_near2far *h, const double *x, double *EH) {
+  AnyN2F *n = reinterpret_cast<AnyN2F *>(h);
+  return guard([&] {
+    with_fields(n->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      std::vector<cdouble> v(6 * n->freq.size());
+      near2far_point<R>(*n, F.gv, to_vec(x), v.data());
+      for (size_t q = 0; q < v.size(); ++q) EH[2 * q] = v[q].real(), EH[2 * q + 1] = v[q].imag();
+      return 0;
+    });
+  });
+}
+long mp_near2far_get_farfields_array(mp_near2far *h, const double *wmin, const double *wmax, double resolution, int *rank,
+                                     size_t *dims, double *out) {
+  AnyN2F *n = reinterpret_cast<AnyN2F *>(h);
+  long count = -1;
+  const int rc = guard([&] {
+    with_fields(n->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      const vec3 mn = to_vec(wmin), mx = to_vec(wmax);
+      const FarGrid g = far_grid(F.gv, mn, mx, resolution);
+      count = long(g.N);
+      if (!out) {
+        int rr = 0;
+        dims[0] = dims[1] = dims[2] = 1;
+        for (int r = 0; r < g.rank; ++r)
+          if (g.dims[r] > 1) dims[rr++] = g.dims[r];
+        *rank = rr;
+        return 0;
+      }
+      std::vector<double> a = farfields_array<R>(*n, F.gv, mn, mx, resolution, rank, dims);
+      std::copy(a.begin(), a.end(), out);
+      return 0;
+    });
+  });
+  return rc ? -1 : count;
+}
+int mp_near2far_flux(mp_near2far *h, int d, const double *wmin, const double *wmax, double resolution, double *out) {
+  AnyN2F *n = reinterpret_cast<AnyN2F *>(h);
+  return guard([&] {
+    with_fields(n->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      if (d < 0 || d > Z || !F.gv.has(d)) fail("cannot get flux for near2far: co-ordinate mismatch");
+      const vec3 mn = to_vec(wmin), mx = to_vec(wmax);
+      int rank;
+      size_t dims[3];
+      std::vector<double> a = farfields_array<R>(*n, F.gv, mn, mx, resolution, &rank, dims);
+      near2far_flux_sum(a, far_grid(F.gv, mn, mx, resolution), n->freq.size(), d, F.gv, mn, mx, resolution, out);
+      return 0;
+    });
+  });
+}
+long mp_near2far_get_dft_array(mp_near2far *h, int c, int num_freq, int *rank, size_t *dims, double *out) {
+  return dft_array(reinterpret_cast<AnyFlux *>(h), false, c, num_freq, rank, dims, out);
+}
+// green2d / green3d as library routines (fields of a point current c0 of amplitude f0 at x0)
+int mp_near2far_green(int dim, const double *x, double freq, double eps, double mu, const double *x0, int c0,
+                      const double *f0, double *EH) {
+  return guard([&] {
+    cdouble v[6];
+    vec3 a = to_vec(x), b = to_vec(x0);
+    if (dim == 2) {
+      a.t[Z] = b.t[Z] = 0;
+      near2far_green2d(v, a, freq, eps, mu, b, c0, cdouble(f0[0], f0[1]));
+    }
+    else if (dim == 3)
+      near2far_green3d(v, a, freq, eps, mu, b, c0, cdouble(f0[0], f0[1]));
+    else
+      fail("only 2d or 3d Green's functions");
+    for (int j = 0; j < 6; ++j) EH[2 * j] = v[j].real(), EH[2 * j + 1] = v[j].imag();
+    return 0;
+  });
 }
 
 // The reverse mode is a product feature with no counterpart in the reference's C++ library;

@@ -63,6 +63,22 @@ run against a value the mathematics fixes; the tests assert on these errors.
   boundary fix equals the point that owns it with the amplitude times exp(i k L) (the
   conjugate of the phase that locates it), and one not marked, or with no owner behind a
   metal wall, is dropped.
+* ``near2far_green``: green2d / green3d (src/near2far.cpp:133-267) against the dyadic Green's
+  function of the homogeneous medium.  With G the scalar Green's function ((i/4) H0(k r) in 2D,
+  exp(i k r) / (4 pi r) in 3D) and p the current's direction, an electric current radiates
+  E = i omega mu (G p + grad grad.(G p) / k^2) and H = G'(r) r x p, a magnetic current
+  H = i omega eps (G p + grad grad.(G p) / k^2) and E = -G'(r) r x p (Maxwell's equations with
+  e^{-i omega t}); the Hessian of the radial G is G'' r r + G' / r (P - r r), P the projector
+  onto the directions G varies in.
+* ``near2far_equivalence``: the equivalence principle on the lattice.  A point current inside
+  a closed box of near2far surfaces in a homogeneous medium: the far-field transform of the
+  box's tangential fields at points outside the box equals the fields the simulation itself
+  accumulates there (dft_fields at those points), and the transform's flux through a large
+  box far away equals the DFT flux through the near box, both to the lattice's discretisation
+  error (the reference's own near2far test allows 0.05 * 30 / resolution).
+* ``near2far_periodic``: with Nperiods copies of a surface that spans a Bloch-periodic cell,
+  the transform at x is the sum over n of exp(i n 2 pi k L) times the single-period transform
+  at x - n L (a copy of every current sits n L away with the Bloch phase of its offset).
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -75,7 +91,7 @@ import math
 
 import numpy as np
 
-from meep_amd.capi import (ARR_F, ARR_P, Bx, By, Dx, Dz, Ex, Ez, Hx, Hy, High, Low, Magnetic, Metallic, X, Y,
+from meep_amd.capi import (ARR_F, ARR_P, Bx, By, Dx, Dz, Ex, Ey, Ez, Hx, Hy, Hz, High, Low, Magnetic, Metallic, X, Y,
                            Z, continuous_src_time, fields, gaussian_src_time, structure)
 
 S = 0.5  # Courant number of every check
@@ -541,3 +557,121 @@ def pml_locality(lib, a=10, steps=10):
         f.step_n(steps)
         out.append(f.get_array(Dx, 0)[0])
     return float(np.abs(out[1] - out[0]).max() / np.abs(out[0]).max())
+
+
+def _dyadic_green(dim, x, freq, eps, mu, x0, c0, f0):
+    """The six fields of a point current from the scalar Green's function (module docstring)."""
+    from scipy.special import hankel1
+    d = np.array(x, float) - np.array(x0, float)
+    if dim == 2:
+        d[2] = 0.0
+    r = np.linalg.norm(d)
+    rh = d / r
+    w = 2 * math.pi * freq
+    k = w * math.sqrt(eps * mu)
+    if dim == 3:
+        G = np.exp(1j * k * r) / (4 * math.pi * r)
+        G1 = G * (1j * k - 1 / r)
+        G2 = G * ((1j * k - 1 / r) ** 2 + 1 / r ** 2)
+    else:
+        G = 0.25j * hankel1(0, k * r)
+        G1 = -0.25j * k * hankel1(1, k * r)
+        G2 = -0.25j * k * k * (hankel1(0, k * r) - hankel1(1, k * r) / (k * r))
+    p = np.zeros(3)
+    p[{0: 0, 1: 1, 4: 2}[c0 % 5]] = 1.0
+    proj = p.copy()
+    if dim == 2:
+        proj[2] = 0.0
+    own = G * p + (G2 * rh * (rh @ p) + G1 / r * (proj - rh * (rh @ p))) / k ** 2
+    cross = G1 * np.cross(rh, p)
+    if c0 < 5:
+        EH = np.concatenate([1j * w * mu * own, cross])
+    else:
+        EH = np.concatenate([-cross, 1j * w * eps * own])
+    return EH * f0
+
+
+def near2far_green(lib):
+    """Largest error of green2d / green3d relative to the largest field, over both
+    dimensions, the six current components and a few geometries and media."""
+    from meep_amd.capi import green
+    cases = [((0.9, -0.4, 0.55), (0.1, 0.2, -0.3), 0.4, 2.0, 1.3),
+             ((3.1, 2.2, -1.7), (0.0, 0.0, 0.0), 1.1, 1.0, 1.0),
+             ((-0.2, 0.35, 0.1), (0.15, -0.05, 0.3), 0.25, 12.0, 1.0)]
+    err = 0.0
+    for dim in (2, 3):
+        for c0 in (Ex, Ey, Ez, Hx, Hy, Hz):
+            for x, x0, fr, eps, mu in cases:
+                got = green(lib, dim, x, fr, eps, mu, x0, c0, 0.3 + 0.7j)
+                want = _dyadic_green(dim, x, fr, eps, mu, x0, c0, 0.3 + 0.7j)
+                err = max(err, float(np.abs(got - want).max() / np.abs(want).max()))
+    return err
+
+
+def _n2f_box(cen, L, dim):
+    """The closed box of half-width L around cen as near2far / flux surfaces (vmin, vmax,
+    weight = the outward sign)."""
+    out = []
+    for d in range(dim):
+        for sgn in (1.0, -1.0):
+            lo = [c - L for c in cen]
+            hi = [c + L for c in cen]
+            lo[d] = hi[d] = cen[d] + sgn * L
+            out.append((tuple(lo) + (0.0,) * (3 - dim), tuple(hi) + (0.0,) * (3 - dim), sgn, d))
+    return out
+
+
+def near2far_equivalence(lib, dim=2, c0=Ez, a=12, sz=6.0, L=0.8, freq=0.5, single=False, far_R=12.0):
+    """Errors of the far-field transform of a closed box around a point current (module
+    docstring): 'fields', relative to the directly accumulated fields at six points outside
+    the box, and 'flux', the transform's flux out of a box of half-width far_R relative to the
+    DFT flux out of the near box."""
+    size = (sz,) * dim
+    s = structure(lib, dim=dim, size=size, a=a, eps=lambda x, y, z: 2.0,
+                  pml=[(d, side, 1.0) for d in range(dim) for side in (High, Low)], single=single)
+    f = fields(s)
+    cen = [sz / 2] * dim
+    f.add_volume_source(c0, gaussian_src_time(freq, 0.4), tuple(cen) + (0.0,) * (3 - dim),
+                        tuple(cen) + (0.0,) * (3 - dim))
+    box = _n2f_box(cen, L, dim)
+    n2f = f.add_dft_near2far([(lo, hi, w) for lo, hi, w, _ in box], [freq], decimation_factor=1)
+    near = [(f.add_dft_flux(d, lo, hi, [freq], decimation_factor=1), w) for lo, hi, w, d in box]
+    pts = []
+    for i in range(6):
+        r = L + 0.3 + 0.1 * i
+        v = np.array([1.0, 0.5, 0.3][:dim])
+        p = np.array(cen) + r * v / np.linalg.norm(v)
+        pts.append(tuple(p) + (0.0,) * (3 - dim))
+    comps = (Ex, Ey, Ez, Hx, Hy, Hz)
+    mons = [f.add_dft_fields(list(comps), p, p, [freq], decimation_factor=1) for p in pts]
+    f.step_n(int(round(40.0 / s.dt)))
+    direct = np.array([[complex(m.get_dft_array(c, 0)) for c in comps] for m in mons])
+    far = np.array([n2f.farfield(p)[0] for p in pts])
+    fields_err = float(np.linalg.norm(far - direct) / np.linalg.norm(direct))
+    flux_near = sum(w * fl.flux()[0] for fl, w in near)
+    flux_far = 0.0
+    for lo, hi, w, d in _n2f_box(cen, far_R, dim):
+        flux_far += w * n2f.flux(d, lo, hi, 4.0)[0]
+    return {"fields": fields_err, "flux": abs(flux_far - flux_near) / abs(flux_near)}
+
+
+def near2far_periodic(lib, nperiods=3, ky=0.27, a=10):
+    """Largest difference, relative to the field, between the transform of a surface spanning
+    a Bloch-periodic cell with nperiods copies and the phased sum of single-period transforms
+    at the shifted points (module docstring)."""
+    sx, sy = 4.0, 2.0
+    s = structure(lib, dim=2, size=(sx, sy), a=a, pml=[(X, side, 1.0) for side in (High, Low)])
+    f = fields(s)
+    f.use_bloch((0.0, ky))
+    f.add_volume_source(Ez, gaussian_src_time(0.6, 0.4), (1.5, 0.7, 0), (1.5, 0.7, 0))
+    surf = [((2.6, 0.0, 0), (2.6, sy, 0), 1.0)]
+    many = f.add_dft_near2far(surf, [0.6], decimation_factor=1, nperiods=nperiods)
+    one = f.add_dft_near2far(surf, [0.6], decimation_factor=1, nperiods=1)
+    f.step_n(int(round(20.0 / s.dt)))
+    err = 0.0
+    for x in ((5.0, 0.4, 0.0), (7.5, 1.9, 0.0)):
+        got = many.farfield(x)[0]
+        want = sum(np.exp(2j * math.pi * ky * sy * n) * one.farfield((x[0], x[1] - n * sy, 0.0))[0]
+                   for n in range(-nperiods, nperiods + 1))
+        err = max(err, float(np.abs(got - want).max() / np.abs(want).max()))
+    return err, many.periodic_d, many.periodic_n

@@ -17,9 +17,9 @@ LIB = os.path.join(OUT, "libmeep_host.so")
 
 
 def launcher_stubs():
-    """C++ source defining every `template <class R> void|bool launch_*(...)` declared in
-    engine.hpp with an empty body (a bool launcher reports that it launched nothing, so the
-    host takes its general path), instantiated for float and double."""
+    """C++ source defining every `void|bool launch_*(...)` declared in engine.hpp with an empty
+    body (a bool launcher reports that it launched nothing, so the host takes its general
+    path), the templates instantiated for float and double."""
     src = open(os.path.join(CSRC, "engine.hpp")).read()
     decls = re.findall(r"template <class R>\s*(void|bool) (launch_\w+)\(([^;]*?)\);", src, re.S)
     if not decls:
@@ -31,6 +31,11 @@ def launcher_stubs():
         for real in ("float", "double"):
             typed = re.sub(r"\bR\b", real, args)
             out.append(f"template {ret} {name}<{real}>({typed});")
+    # launchers that do not depend on the field precision
+    templated = {name for _, name, _ in decls}
+    for ret, name, args in re.findall(r"^(void|bool) (launch_\w+)\(([^;]*?)\);", src, re.M | re.S):
+        if name not in templated:
+            out.append(f"{ret} {name}({args}) {{{' return false;' if ret == 'bool' else ''}}}")
     out.append("}")
     return "\n".join(out) + "\n"
 

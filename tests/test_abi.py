@@ -262,3 +262,68 @@ def test_cpp_api_data_sources(tmp_path, oracle_lib):
     want = f.get_field(cEz, (1.7, 1.1, 0))
     assert got != 0 and abs(got - want) <= 1e-15 * abs(want), (got, want)
 
+
+
+NEAR2FAR_PROGRAM = r"""
+#include "meep_amd.hpp"
+#include <cstdio>
+using namespace meep;
+double two(const vec &) { return 2.0; }
+int main() {
+  structure s(voltwo(6.0, 6.0, 10.0), two, pml(1.0));
+  fields f(&s);
+  gaussian_src_time src(0.5, 0.4);
+  src.is_integrated = false;
+  f.add_point_source(Ez, src, vec(3.0, 3.0));
+  const double L = 0.8, c = 3.0;
+  // the reference's box (tests/near2far.cpp:182-190), corners given in either order
+  volume_list vl(volume(vec(c - L, c + L), vec(c + L, c + L)), Sy, 1.0,
+                 new volume_list(volume(vec(c + L, c + L), vec(c + L, c - L)), Sx, 1.0,
+                                 new volume_list(volume(vec(c - L, c - L), vec(c + L, c - L)), Sy, -1.0,
+                                                 new volume_list(volume(vec(c - L, c - L), vec(c - L, c + L)), Sx, -1.0))));
+  dft_near2far n2f = f.add_dft_near2far(&vl, 0.5, 0.5, 1);
+  while (f.time() < 30.0) f.step();
+  std::complex<double> EH[6], G[6];
+  n2f.farfield_lowlevel(EH, vec(5.1, 4.2));
+  green2d(G, vec(5.1, 4.2), 0.5, 2.0, 1.0, vec(3.0, 3.0), Ez, 1.0);
+  int rank;
+  size_t dims[3], N;
+  double *arr = n2f.get_farfields_array(volume(vec(8.0, -2.0), vec(8.0, 8.0)), rank, dims, N, 2.0);
+  double *fl = n2f.flux(X, volume(vec(8.0, -2.0), vec(8.0, 8.0)), 2.0);
+  for (int q = 0; q < 6; ++q) std::printf("%.17g %.17g ", EH[q].real(), EH[q].imag());
+  std::printf("%.17g %.17g %d %zu %zu %.17g %.17g\n", G[2].real(), G[2].imag(), rank, dims[0], N, arr[5 * 2 * N], fl[0]);
+  delete[] arr;
+  delete[] fl;
+  return 0;
+}
+"""
+
+
+def test_cpp_api_near2far(tmp_path, oracle_lib):
+    """The C++ API's volume_list / add_dft_near2far / dft_near2far give the C-ABI's far fields,
+    get_farfields_array and flux, and green2d the library's Green's function."""
+    src = tmp_path / "near2far.cpp"
+    src.write_text(NEAR2FAR_PROGRAM)
+    out = str(tmp_path / "near2far")
+    mbuild.build_cpp_program(str(src), out, os.path.join(ROOT, "oracle", "_build"), "meep_oracle")
+    r = subprocess.run([out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    v = [float(x) for x in r.stdout.split()]
+    from meep_amd.capi import X as cX, Ez as cEz, fields, gaussian_src_time, green, pml, structure
+    s = structure(oracle_lib, dim=2, size=(6.0, 6.0), a=10, eps=lambda x, y, z: 2.0, pml=pml(1.0, None)[:4])
+    f = fields(s)
+    f.add_point_source_src(cEz, gaussian_src_time(0.5, 0.4), (3.0, 3.0, 0))
+    c, L = 3.0, 0.8
+    n2f = f.add_dft_near2far([((c - L, c + L), (c + L, c + L), 1.0, 1), ((c + L, c - L), (c + L, c + L), 1.0, 0),
+                              ((c - L, c - L), (c + L, c - L), -1.0, 1), ((c - L, c - L), (c - L, c + L), -1.0, 0)],
+                             [0.5])
+    while f.time() < 30.0:
+        f.step()
+    EH = n2f.farfield((5.1, 4.2))[0]
+    np.testing.assert_array_equal(np.array(v[0:12:2]) + 1j * np.array(v[1:12:2]), EH)
+    G = green(oracle_lib, 2, (5.1, 4.2), 0.5, 2.0, 1.0, (3.0, 3.0), cEz)
+    assert complex(v[12], v[13]) == G[2]
+    arr, dims = n2f.get_farfields_array((8.0, -2.0), (8.0, 8.0), 2.0)
+    assert (int(v[14]), int(v[15]), int(v[16])) == (1, 20, 20) and dims == (20,)
+    assert v[17] == arr[5, 0, 0, 0]
+    assert v[18] == n2f.flux(cX, (8.0, -2.0), (8.0, 8.0), 2.0)[0]

@@ -49,6 +49,8 @@ CHECKS = {
     "PML_STRETCH": T.check_pml_stretch,
     "AMPARR_OFFSET": T.check_source_array,
     "FIXBND_PHASE": T.check_srcdata,
+    "N2F_SIGN": T.check_n2f_equivalence,
+    "N2F_PHASE": T.check_n2f_periodic,
 }
 
 
