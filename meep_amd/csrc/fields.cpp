@@ -4964,16 +4964,17 @@ int mp_near2far_green(int dim, const double *x, double freq, double eps, double 
     ensure_gpu();
     if (dim != 2 && dim != 3) fail("only 2d or 3d Green's functions");
     if (c0 < 0 || c0 >= 10 || comp_dir(c0) == NODIR) fail("unrecognized source type");
-    // [sx sy sz px py pz freq f0.re f0.im] then the component, the partial and the output
-    double host[9] = {x0[0], x0[1], dim == 3 ? x0[2] : 0.0, x[0], x[1], dim == 3 ? x[2] : 0.0, freq, f0[0], f0[1]};
+    // [f0.re f0.im sx sy sz px py pz freq] (the amplitude first, 16-byte aligned), then the
+    // component, the partial and the output
+    double host[9] = {f0[0], f0[1], x0[0], x0[1], dim == 3 ? x0[2] : 0.0, x[0], x[1], dim == 3 ? x[2] : 0.0, freq};
     DevBuf buf, cb, part, outb;
     buf.alloc(sizeof host), cb.alloc(sizeof(int)), part.alloc(12 * 8), outb.alloc(12 * 8);
     HIPCHECK(hipMemcpy(buf.p, host, sizeof host, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(cb.p, &c0, sizeof(int), hipMemcpyHostToDevice));
     const double *d = static_cast<const double *>(buf.p);
     FarArgs a;
-    a.sx = d, a.sy = d + 1, a.sz = d + 2, a.px = d + 3, a.py = d + 4, a.pz = d + 5, a.freq = d + 6;
-    a.f0 = reinterpret_cast<const double2 *>(d + 7);
+    a.f0 = reinterpret_cast<const double2 *>(d);
+    a.sx = d + 2, a.sy = d + 3, a.sz = d + 4, a.px = d + 5, a.py = d + 6, a.pz = d + 7, a.freq = d + 8;
     a.comp = static_cast<const int *>(cb.p);
     a.nsrc = 1, a.npts = 1, a.nfreq = 1, a.eps = eps, a.mu = mu, a.dim = dim, a.segs = 1;
     launch_farfield(a, static_cast<double *>(part.p), static_cast<double *>(outb.p), nullptr);
