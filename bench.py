@@ -571,6 +571,94 @@ def run_batch2d(args, lib, steps=None, warmup=None):
     return line
 
 
+# ---- the near-to-far workload ------------------------------------------------------------
+FF_A = 10.0
+FF_SIZE = 6.4        # 64^3 cells
+FF_L = 2.0           # half-width of the near2far box around the source
+FF_FREQS = 10
+FF_PLANE = 64        # far points per side of the far plane
+FF_FLOPS_PER_EVAL = 85  # FP64 operations of one green3d term (division, sqrt, sin and cos counted once)
+
+
+def build_farfield(lib, single):
+    """A z-polarised point current at the centre of a 64^3 vacuum cell with 1-unit uPML,
+    enclosed by a 4 x 4 x 4 near2far box (six faces, four tangential components each) at 10
+    frequencies, stepped 100 times so every accumulator holds a value."""
+    pml = [(d, side, 1.0) for d in (capi.X, capi.Y, capi.Z) for side in (capi.High, capi.Low)]
+    s = capi.structure(lib, dim=3, size=(FF_SIZE,) * 3, a=FF_A, pml=pml, single=single)
+    f = capi.fields(s)
+    c = FF_SIZE / 2
+    f.add_volume_source(capi.Ez, capi.gaussian_src_time(0.5, 0.4), (c, c, c), (c, c, c))
+    box = []
+    for d in range(3):
+        for sgn in (1.0, -1.0):
+            lo, hi = [c - FF_L] * 3, [c + FF_L] * 3
+            lo[d] = hi[d] = c + sgn * FF_L
+            box.append((tuple(lo), tuple(hi), sgn))
+    n2f = f.add_dft_near2far(box, np.linspace(0.4, 0.6, FF_FREQS), decimation_factor=1)
+    f.step_n(100)
+    return s, f, n2f
+
+
+def cpu_baseline_farfield(npoints_eval, single):
+    """The oracle's farfield (one thread, as the reference's loop over a chunk's points) at
+    16 far points of the same monitor: Green's-function evaluations per second."""
+    lib = oracle_lib(baseline=True)
+    s, f, n2f = build_farfield(lib, single)
+    pts = [(-20.0 + 40.0 * q / 15, 3.0, 30.0) for q in range(16)]
+    n2f.farfield(pts[0])
+    t0 = time.perf_counter()
+    for p in pts:
+        n2f.farfield(p)
+    el = time.perf_counter() - t0
+    evals = len(pts) * FF_FREQS * n2f.npoints
+    return {"value": round(evals / el / 1e9, 5), "unit": "G Green's-function evaluations/s", "cores": 1, "kind": "port",
+            "sample": f"oracle farfield (-O3 -march=x86-64-v3, FP64 Green's functions) at 16 far points of the same "
+                      f"monitor ({n2f.npoints} currents x {FF_FREQS} frequencies each), {el:.2f} s"}
+
+
+def run_farfield(args, lib):
+    """get_farfields_array over a 64 x 64 far plane 28 units above the box; value is Green's-
+    function evaluations (far points x frequencies x surface currents) per second."""
+    single = args.precision == "f32"
+    s, f, n2f = build_farfield(lib, single)
+    c = FF_SIZE / 2
+    lo, hi = (c - 20.0, c - 20.0, c + 28.0), (c + 20.0, c + 20.0, c + 28.0)
+    res = FF_PLANE / 40.0
+    n2f.get_farfields_array(lo, hi, res)  # uploads the currents, warms the kernel
+    calls = max(1, min(args.steps, 20))
+    sync = device_sync()
+    f.set_kernel_timing(True)
+    f.reset_kernel_timing()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        EH, dims = n2f.get_farfields_array(lo, hi, res)
+    sync()
+    el = time.perf_counter() - t0
+    ms, nl = f.kernel_timing(capi.K_FARFIELD)
+    npts = int(np.prod(dims))
+    evals = npts * FF_FREQS * n2f.npoints
+    value = evals * calls / el / 1e9
+    line = line_base(args, value, el, {
+        "workload": f"near2far: 64^3 vacuum cell at resolution 10, Ez point current, 4x4x4 box of six "
+                    f"surfaces ({n2f.npoints} surface currents) at {FF_FREQS} frequencies; get_farfields_array "
+                    f"on a {FF_PLANE}x{FF_PLANE} plane 28 units away (3D Green's function, FP64)",
+        "far_points": npts, "surface_currents": n2f.npoints, "frequencies": FF_FREQS, "parallelism": "none"},
+        calls, 1)
+    line["unit"] = "G Green's-function evaluations/s"
+    line["metric"] = "near2far far-field evaluations/s"
+    line["dtype"] = "f64"
+    avg = ms / max(nl, 1) / 1e3
+    achieved = evals * FF_FLOPS_PER_EVAL / avg / 1e12
+    line["roofline"] = {"bound": "valu", "kernel": "k_farfield<3> + k_farfield_sum", "achieved": round(achieved, 2),
+                        "peak": 78.6, "unit": "TFLOP/s", "frac": round(achieved / 78.6, 4), "traffic": None,
+                        "flops_per_launch": evals * FF_FLOPS_PER_EVAL, "avg_launch_us": round(avg * 1e6, 1),
+                        "launches": nl}
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_farfield(n2f.npoints, single)
+    return line
+
+
 def run_cpu_c1(args):
     """The 2D straight waveguide (320 x 160 cells, PML, 8000 steps, 100-frequency flux line) in
     full on the CPU baseline build, FP64: median wall time of `repeats` whole runs at one
@@ -605,12 +693,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--workload", choices=["pml_lorentz_dft", "box256", "adjoint2d", "batch2d"],
+    ap.add_argument("--workload", choices=["pml_lorentz_dft", "box256", "adjoint2d", "batch2d", "farfield"],
                     default="pml_lorentz_dft",
                     help="pml_lorentz_dft: 384x384x192 with uPML, two poles and a DFT plane (the "
                          "headline); box256: 3D Bloch-periodic 256^3 box; adjoint2d: 600x400 "
                          "forward run plus its reverse-mode gradient; batch2d: 64 uneven 2D "
-                         "simulations stepped together")
+                         "simulations stepped together; farfield: the near-to-far transform of a 3D "
+                         "box on a 64x64 far plane")
     ap.add_argument("--checkpoint-interval", type=int, default=None,
                     help="adjoint2d: steps between checkpoints (default sqrt(steps); 0 records "
                          "the design history every step)")
@@ -647,7 +736,7 @@ def main():
         sys.exit(2)
     lib = meep_amd.load_library()
     run = {"box256": run_box256, "pml_lorentz_dft": run_pml_lorentz_dft, "adjoint2d": run_adjoint2d,
-           "batch2d": run_batch2d}[args.workload]
+           "batch2d": run_batch2d, "farfield": run_farfield}[args.workload]
     line = run(args, lib)
     if args.workload == "pml_lorentz_dft" and not args.no_secondary:
         sub = argparse.Namespace(**vars(args))

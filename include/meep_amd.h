@@ -342,9 +342,10 @@ mp_near2far *mp_fields_add_dft_near2far(mp_fields *f, int nsurf, const int *dirs
                                         const double *vmaxs, const double *weights, const double *freq,
                                         int nfreq, int dec, int nperiods);
 void mp_near2far_delete(mp_near2far *n);
-/* dft_near2far::eps, mu (src/meep.hpp:1468) into eps_mu[2], and periodic_d / periodic_n of
-   the two tangential directions (-1 when not periodic) into periodic[4]; returns Nfreq. */
-int mp_near2far_info(const mp_near2far *n, double *eps_mu, int *periodic);
+/* dft_near2far::eps, mu (src/meep.hpp:1468) into eps_mu[2], periodic_d / periodic_n of the
+   two tangential directions (-1 when not periodic) into periodic[4], and the number of
+   accumulator points over its chunks into *npoints; returns Nfreq. */
+int mp_near2far_info(const mp_near2far *n, double *eps_mu, int *periodic, long *npoints);
 /* dft_near2far::farfield(const vec &x) — src/near2far.cpp:399-408: EH[12 * Nfreq] as
    (re, im) of Ex, Ey, Ez, Hx, Hy, Hz for each frequency in turn. */
 int mp_near2far_farfield(mp_near2far *n, const double *x, double *EH);

@@ -17,7 +17,7 @@ X, Y, Z = 0, 1, 2
 High, Low = 0, 1
 Periodic, Metallic, Magnetic, NoBoundary = 0, 1, 2, 3
 ARR_F, ARR_F_U, ARR_F_W, ARR_P, ARR_P_PREV, ARR_F_COND = 0, 1, 2, 3, 4, 5
-K_STEP_BH, K_STEP_DE, K_HALO, K_DFT, K_SRC, K_BH_INTERIOR, K_DE_INTERIOR = 0, 1, 2, 3, 4, 5, 6
+K_STEP_BH, K_STEP_DE, K_HALO, K_DFT, K_SRC, K_BH_INTERIOR, K_DE_INTERIOR, K_FARFIELD = 0, 1, 2, 3, 4, 5, 6, 7
 DFT_REALNUM, DFT_F64, DFT_KAHAN32 = 0, 1, 2
 
 component_name = {Ex: "ex", Ey: "ey", Ez: "ez", Hx: "hx", Hy: "hy", Hz: "hz",
@@ -156,7 +156,7 @@ _SIGNATURES = {
                                                 C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int]),
     "mp_near2far_delete": (None, [C.c_void_p]),
-    "mp_near2far_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "mp_near2far_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)]),
     "mp_near2far_farfield": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mp_near2far_get_farfields_array": (C.c_long, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                    C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
@@ -419,7 +419,9 @@ class near2far:
         self.nfreq = nfreq
         em = (C.c_double * 2)()
         per = (C.c_int * 4)()
-        self.lib.mp_near2far_info(ptr, em, per)
+        npts = C.c_long()
+        self.lib.mp_near2far_info(ptr, em, per, C.byref(npts))
+        self.npoints = npts.value
         self.eps, self.mu = em[0], em[1]
         self.periodic_d = (per[0], per[2])
         self.periodic_n = (per[1], per[3])

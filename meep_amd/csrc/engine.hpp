@@ -370,13 +370,14 @@ void launch_check(const R *const *ptrs, const int64_t *idx, const double *w, int
 // FLUX_BLOCKS partial sums of the plane flux into partial
 template <class R> void launch_flux_plane(const FluxPlaneArgs<R> &a, double *partial, hipStream_t s);
 
-// The near-to-far field sums (near2far.hip): nsrc equivalent currents at (sx, sy, sz) of
-// component comp with amplitudes f0[frequency][current], seen at npts far points (px, py, pz),
-// in the medium eps, mu of dimension dim (2 or 3).  Each (point, frequency) pair is split into
-// segs segments of the current list.
+// The near-to-far field sums (near2far.hip): nsrc equivalent currents at (sx, sy, sz) with
+// amplitudes f0[frequency][current], sorted by class (Ex Ey Ez Hx Hy Hz: class c holds
+// [cls_lo[c], cls_lo[c + 1])), seen at npts far points (px, py, pz), in the medium eps, mu of
+// dimension dim (2 or 3).  Each (point, frequency group) is split into segs segments of the
+// current list.
 struct FarArgs {
   const double *sx, *sy, *sz;
-  const int *comp;
+  int cls_lo[7];
   const double2 *f0;
   int nsrc;
   const double *px, *py, *pz;
@@ -385,7 +386,12 @@ struct FarArgs {
   int nfreq;
   double eps, mu;
   int dim, segs;
+  int uniform; // freq[i] = freq[0] + i (freq[1] - freq[0]) to rounding
 };
+// frequencies a block of the far-field kernel takes together in 3D: 2 and 3 measured alike
+// (4 runs out of registers and halves the rate); 2 divides the usual even frequency counts
+constexpr int FF_GROUP3 = 2;
+inline int farfield_group(int dim) { return dim == 3 ? FF_GROUP3 : 1; }
 // partial: npts * nfreq * segs * 12 doubles of scratch; out: npts * nfreq * 12 doubles, the
 // (re, im) of Ex Ey Ez Hx Hy Hz per (point, frequency), point-major
 void launch_farfield(const FarArgs &a, double *partial, double *out, hipStream_t s);

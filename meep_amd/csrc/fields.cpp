@@ -17,6 +17,7 @@
 #include <chrono>
 #include <complex>
 #include <cstdarg>
+#include <cfloat>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -378,7 +379,7 @@ struct FieldsBase {
   // frequency at np far points (out: np * nfreq * 12 doubles, point-major)
   virtual void *add_dft_near2far(int nsurf, const int *dirs, const double *vmins, const double *vmaxs, const double *weights,
                                  const std::vector<double> &freq, int dec, int nperiods, int *dec_out) = 0;
-  virtual void n2f_info(void *n, double *eps_mu, int *periodic) = 0;
+  virtual void n2f_info(void *n, double *eps_mu, int *periodic, long *npoints) = 0;
   virtual void n2f_eval(void *n, const double *pts, long np, double *out) = 0;
   // time-domain flux: flux_vol planes, flux_in_box (sync) and flux_in_box_wrongH
   virtual void *add_flux_vol(int d, V3 mn, V3 mx) = 0;
@@ -427,8 +428,8 @@ template <class R> struct N2F : Flux<R> {
   double periodic_k[2] = {0, 0}, period[2] = {0, 0};
   long built_t = -1;
   unsigned built_version = 0;
-  int nsrc = 0;
-  DevBuf sx, sy, sz, comp, f0, d_freq;
+  int nsrc = 0, cls_lo[7] = {};
+  DevBuf sx, sy, sz, f0, d_freq;
 };
 
 template <class R> struct Fields : FieldsBase {
@@ -3704,10 +3705,12 @@ template <class R> struct Fields : FieldsBase {
     fluxes.push_back(std::move(n));
     return fluxes.back().get();
   }
-  void n2f_info(void *h, double *eps_mu, int *periodic) override {
+  void n2f_info(void *h, double *eps_mu, int *periodic, long *npoints) override {
     N2F<R> *n = static_cast<N2F<R> *>(h);
     eps_mu[0] = n->eps, eps_mu[1] = n->mu;
     for (int i = 0; i < 2; ++i) periodic[2 * i] = n->periodic_d[i], periodic[2 * i + 1] = n->periodic_n[i];
+    *npoints = 0;
+    for (const DftChunk<R> *ch : n->E) *npoints += ch->npts;
   }
   // The equivalent currents of dft_near2far::farfield_lowlevel (src/near2far.cpp:360-393): every
   // accumulator point of every chunk at its location plus the chunk's periodic-image shift,
@@ -3741,20 +3744,30 @@ template <class R> struct Fields : FieldsBase {
     }
     const size_t ns = x.size();
     if (ns > size_t(INT32_MAX)) fail("near2far: too many surface points");
-    std::vector<double> fm(2 * ns * Nf);
-    for (size_t j = 0; j < ns; ++j)
+    // sorted by class (Ex Ey Ez Hx Hy Hz), keeping the list order within a class
+    auto cls = [](int c) { return c < 5 ? (c == 4 ? 2 : c) : (c == 9 ? 5 : c - 2); };
+    std::vector<size_t> order(ns);
+    for (size_t j = 0; j < ns; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](size_t u, size_t v) { return cls(cc[u]) < cls(cc[v]); });
+    for (int c = 0; c <= 6; ++c) n.cls_lo[c] = 0;
+    for (size_t j = 0; j < ns; ++j) n.cls_lo[cls(cc[j]) + 1]++;
+    for (int c = 0; c < 6; ++c) n.cls_lo[c + 1] += n.cls_lo[c];
+    std::vector<double> xs(ns), ys(ns), zs(ns), fm(2 * ns * Nf);
+    for (size_t q = 0; q < ns; ++q) {
+      const size_t j = order[q];
+      xs[q] = x[j], ys[q] = y[j], zs[q] = z[j];
       for (int i = 0; i < Nf; ++i) {
-        fm[2 * (size_t(i) * ns + j)] = amp[j * Nf + i].real();
-        fm[2 * (size_t(i) * ns + j) + 1] = amp[j * Nf + i].imag();
+        fm[2 * (size_t(i) * ns + q)] = amp[j * Nf + i].real();
+        fm[2 * (size_t(i) * ns + q) + 1] = amp[j * Nf + i].imag();
       }
+    }
     n.nsrc = int(ns);
     if (ns) {
       auto put = [&](DevBuf &b, const void *src, size_t bytes) {
         if (b.bytes != bytes) b.alloc(bytes);
         HIPCHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
       };
-      put(n.sx, x.data(), 8 * ns), put(n.sy, y.data(), 8 * ns), put(n.sz, z.data(), 8 * ns);
-      put(n.comp, cc.data(), sizeof(int) * ns);
+      put(n.sx, xs.data(), 8 * ns), put(n.sy, ys.data(), 8 * ns), put(n.sz, zs.data(), 8 * ns);
       put(n.f0, fm.data(), 8 * fm.size());
       put(n.d_freq, n.freq.data(), 8 * n.freq.size());
     }
@@ -3776,7 +3789,16 @@ template <class R> struct Fields : FieldsBase {
     const long batch = std::max(1L, (1L << 20) / Nf);
     const long nb = std::min(np, batch);
     const long pairs_max = nb * Nf;
-    const int segs = int(std::max(1L, std::min<long>((n.nsrc + 511) / 512, (4096 + pairs_max - 1) / pairs_max)));
+    // blocks per segment: the points times the frequency groups a block takes together
+    const long fb = farfield_group(gv.dim), bps = nb * ((Nf + fb - 1) / fb);
+    const int segs = int(std::max(1L, std::min<long>((n.nsrc + 511) / 512, (4096 + bps - 1) / bps)));
+    // uniformly spaced frequencies (to a few ulps, as linspace makes them) let the kernel step
+    // e^{ikr} from one frequency to the next
+    bool uniform = Nf > 1;
+    for (int i = 2; i < Nf && uniform; ++i) {
+      const double want = n.freq[0] + i * (n.freq[1] - n.freq[0]);
+      uniform = std::fabs(n.freq[i] - want) <= 8 * DBL_EPSILON * std::max(std::fabs(n.freq[i]), std::fabs(n.freq[0]));
+    }
     DevBuf dp, partial, dout;
     dp.alloc(8 * 3 * size_t(nb));
     partial.alloc(8 * 12 * size_t(pairs_max) * segs);
@@ -3789,7 +3811,7 @@ template <class R> struct Fields : FieldsBase {
       HIPCHECK(hipMemcpyAsync(dp.p, soa.data(), 8 * 3 * size_t(nb), hipMemcpyHostToDevice, stream));
       FarArgs a;
       a.sx = static_cast<const double *>(n.sx.p), a.sy = static_cast<const double *>(n.sy.p), a.sz = static_cast<const double *>(n.sz.p);
-      a.comp = static_cast<const int *>(n.comp.p);
+      for (int c = 0; c <= 6; ++c) a.cls_lo[c] = n.cls_lo[c];
       a.f0 = static_cast<const double2 *>(n.f0.p);
       a.nsrc = n.nsrc;
       a.px = static_cast<const double *>(dp.p), a.py = a.px + nb, a.pz = a.px + 2 * nb;
@@ -3799,6 +3821,7 @@ template <class R> struct Fields : FieldsBase {
       a.eps = n.eps, a.mu = n.mu;
       a.dim = gv.dim;
       a.segs = segs;
+      a.uniform = uniform ? 1 : 0;
       kbegin(7);
       launch_farfield(a, static_cast<double *>(partial.p), static_cast<double *>(dout.p), stream);
       HIPCHECK(hipGetLastError());
@@ -4895,9 +4918,9 @@ mp_near2far *mp_fields_add_dft_near2far(mp_fields *f, int nsurf, const int *dirs
   return reinterpret_cast<mp_near2far *>(h);
 }
 void mp_near2far_delete(mp_near2far *n) { mp_dft_flux_delete(reinterpret_cast<mp_dft_flux *>(n)); }
-int mp_near2far_info(const mp_near2far *n, double *eps_mu, int *periodic) {
+int mp_near2far_info(const mp_near2far *n, double *eps_mu, int *periodic, long *npoints) {
   const FluxHandle *h = reinterpret_cast<const FluxHandle *>(n);
-  if (guard([&] { h->owner->n2f_info(h->fl, eps_mu, periodic); })) return -1;
+  if (guard([&] { h->owner->n2f_info(h->fl, eps_mu, periodic, npoints); })) return -1;
   return h->nfreq;
 }
 int mp_near2far_farfield(mp_near2far *n, const double *x, double *EH) {
@@ -4967,16 +4990,16 @@ int mp_near2far_green(int dim, const double *x, double freq, double eps, double 
     // [f0.re f0.im sx sy sz px py pz freq] (the amplitude first, 16-byte aligned), then the
     // component, the partial and the output
     double host[9] = {f0[0], f0[1], x0[0], x0[1], dim == 3 ? x0[2] : 0.0, x[0], x[1], dim == 3 ? x[2] : 0.0, freq};
-    DevBuf buf, cb, part, outb;
-    buf.alloc(sizeof host), cb.alloc(sizeof(int)), part.alloc(12 * 8), outb.alloc(12 * 8);
+    DevBuf buf, part, outb;
+    buf.alloc(sizeof host), part.alloc(12 * 8), outb.alloc(12 * 8);
     HIPCHECK(hipMemcpy(buf.p, host, sizeof host, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(cb.p, &c0, sizeof(int), hipMemcpyHostToDevice));
     const double *d = static_cast<const double *>(buf.p);
     FarArgs a;
     a.f0 = reinterpret_cast<const double2 *>(d);
     a.sx = d + 2, a.sy = d + 3, a.sz = d + 4, a.px = d + 5, a.py = d + 6, a.pz = d + 7, a.freq = d + 8;
-    a.comp = static_cast<const int *>(cb.p);
-    a.nsrc = 1, a.npts = 1, a.nfreq = 1, a.eps = eps, a.mu = mu, a.dim = dim, a.segs = 1;
+    const int cls = c0 < 5 ? (c0 == 4 ? 2 : c0) : (c0 == 9 ? 5 : c0 - 2);
+    for (int c = 0; c <= 6; ++c) a.cls_lo[c] = c > cls ? 1 : 0;
+    a.nsrc = 1, a.npts = 1, a.nfreq = 1, a.eps = eps, a.mu = mu, a.dim = dim, a.segs = 1, a.uniform = 0;
     launch_farfield(a, static_cast<double *>(part.p), static_cast<double *>(outb.p), nullptr);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpy(EH, outb.p, 12 * 8, hipMemcpyDeviceToHost));

@@ -3107,10 +3107,16 @@ mp_near2far *mp_fields_add_dft_near2far(mp_fields *f, int nsurf, const int *dirs
   return reinterpret_cast<mp_near2far *>(n2f);
 }
 void mp_near2far_delete(mp_near2far *h) { delete reinterpret_cast<AnyN2F *>(h); }
-int mp_near2far_info(const mp_near2far *h, double *eps_mu, int *periodic) {
+int mp_near2far_info(const mp_near2far *h, double *eps_mu, int *periodic, long *npoints) {
   const AnyN2F *n = reinterpret_cast<const AnyN2F *>(h);
   eps_mu[0] = n->eps, eps_mu[1] = n->mu;
   for (int i = 0; i < 2; ++i) periodic[2 * i] = n->periodic_d[i] == NODIR ? -1 : n->periodic_d[i], periodic[2 * i + 1] = n->periodic_n[i];
+  *npoints = with_fields(n->owner, [&](auto &F) {
+    typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+    long np = 0;
+    for (void *p : n->E) np += long(reinterpret_cast<const DftChunk<R> *>(p)->N);
+    return np;
+  });
   return int(n->freq.size());
 }
 int mp_near2far_farfield(mp_near2far *h, const double *x, double *EH) {
