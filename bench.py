@@ -643,7 +643,8 @@ def run_farfield(args, lib):
     line = line_base(args, value, el, {
         "workload": f"near2far: 64^3 vacuum cell at resolution 10, Ez point current, 4x4x4 box of six "
                     f"surfaces ({n2f.npoints} surface currents) at {FF_FREQS} frequencies; get_farfields_array "
-                    f"on a {FF_PLANE}x{FF_PLANE} plane 28 units away (3D Green's function, FP64)",
+                    f"on a {FF_PLANE}x{FF_PLANE} plane 28 units away (FP64 3D Green's functions; "
+                    f"{'FP32' if single else 'FP64'} fields)",
         "far_points": npts, "surface_currents": n2f.npoints, "frequencies": FF_FREQS, "parallelism": "none"},
         calls, 1)
     line["unit"] = "G Green's-function evaluations/s"
