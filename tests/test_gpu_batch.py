@@ -4,7 +4,9 @@ autotuned block size) give the same fields and DFT accumulators as the same simu
 stepped one at a time (which the parity tests hold to the oracle).  The batched kernels run
 the same per-point code; the bound allows for the compiler contracting a multiply-add
 differently in the batched instantiation.  A second batch mixes dispersive poles, complex
-fields and Bloch-phase halos."""
+fields and Bloch-phase halos.  Both batches are also held to the oracle directly (each
+simulation stepped there on its own), at the state-parity bounds (FP64 1e-10, FP32 1e-4 of
+each array's largest value)."""
 
 import numpy as np
 import pytest
@@ -172,3 +174,34 @@ def test_monitor_added_between_batch_calls(gpu_lib, single):
         fx, fy = x.flux(), y.flux()
         assert np.abs(fx).max() > 0
         assert np.abs(fx - fy).max() <= tol * np.abs(fx).max()
+
+
+@pytest.mark.parametrize("which", ["uneven_2d_3d", "poles_bloch"])
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_batch_matches_oracle(gpu_lib, oracle_lib, single, which):
+    nsteps = 90
+    print(f"[batch] against the oracle: {which} single={single}", flush=True)
+    make = build_all if which == "uneven_2d_3d" else mixed_physics
+    npol = 0 if which == "uneven_2d_3d" else 2
+    ref = make(oracle_lib, single)
+    for s, f, _ in ref:
+        f.step_n(nsteps)
+    together = make(gpu_lib, single)
+    b = batch([f for _, f, _ in together], use_graph=True, autotune=True)
+    b.step_n(nsteps)
+    tol = 1e-4 if single else 1e-10
+    for (s1, f1, fl1), (s2, f2, fl2) in zip(ref, together):
+        assert f1.t == f2.t == nsteps
+        a, c = state(f1, s1, npol=npol), state(f2, s2, npol=npol)
+        assert set(a) == set(c)
+        for key in a:
+            scale = max(np.abs(a[key]).max(), 1e-30)
+            assert np.abs(a[key] - c[key]).max() <= tol * scale, key
+        for x, y in zip(fl1, fl2):
+            if hasattr(x, "flux"):
+                fx, fy = np.asarray(x.flux()), np.asarray(y.flux())
+                assert np.abs(fx - fy).max() <= tol * max(np.abs(fx).max(), 1e-30)
+            else:
+                for i in range(x.num_chunks()):
+                    dx, dy = x.dft(i), y.dft(i)
+                    assert np.abs(dx - dy).max() <= tol * max(np.abs(dx).max(), 1e-30)
