@@ -125,6 +125,12 @@ int mp_structure_get_chi1inv(const mp_structure *s, int c, double *chi1inv);
 int mp_structure_add_susceptibility(mp_structure *s, double omega_0, double gamma,
                                     int no_omega_0_denominator, mp_scalar_func sigma,
                                     void *sigma_user);
+/* structure::add_susceptibility(double sigma(const vec &), field_type ft, const
+   susceptibility &) — src/meep.hpp:864, src/structure.cpp:471-507 — for ft E_stuff (0) or
+   H_stuff (1, a magnetic Lorentz/Drude pole: sigma on the H components, P subtracted from B
+   and updated from H, src/step.cpp:76-88); mp_structure_add_susceptibility is ft = E_stuff. */
+int mp_structure_add_susceptibility_ft(mp_structure *s, int ft, double omega_0, double gamma, int drude,
+                                       mp_scalar_func sigma, void *user);
 /* Overwrite sigma[c][dir(c)] of susceptibility isus on all ntot points. */
 int mp_structure_set_sigma(mp_structure *s, int isus, int c, const double *sigma);
 

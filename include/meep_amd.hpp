@@ -323,10 +323,10 @@ public:
   ~structure() { mp_structure_delete(p); }
   structure(const structure &) = delete;
   structure &operator=(const structure &) = delete;
-  // add_susceptibility(sigma, E_stuff, sus) (src/meep.hpp:864)
-  void add_susceptibility(double sigma(const vec &), int /* field_type: E_stuff */, const lorentzian_susceptibility &sus) {
-    check(mp_structure_add_susceptibility(p, sus.omega_0, sus.gamma, sus.drude, detail::eps_trampoline,
-                                          reinterpret_cast<void *>(sigma)));
+  // add_susceptibility(sigma, ft, sus) (src/meep.hpp:864), ft E_stuff (0) or H_stuff (1)
+  void add_susceptibility(double sigma(const vec &), int ft, const lorentzian_susceptibility &sus) {
+    check(mp_structure_add_susceptibility_ft(p, ft, sus.omega_0, sus.gamma, sus.drude, detail::eps_trampoline,
+                                             reinterpret_cast<void *>(sigma)));
   }
   // set_conductivity(component, double C(const vec &)) (src/meep.hpp:856, src/structure.cpp:435-438)
   void set_conductivity(component c, double C(const vec &)) {
@@ -340,6 +340,7 @@ private:
   mp_structure *p;
 };
 constexpr int E_stuff = 0;
+constexpr int H_stuff = 1;
 
 // linspace (src/dft.cpp:30-40)
 inline std::vector<double> linspace(double freq_min, double freq_max, size_t Nfreq) {

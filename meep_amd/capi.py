@@ -12,6 +12,7 @@ import numpy as np
 
 # reference enums (src/meep/vec.hpp:31-79, src/meep.hpp:1594)
 Ex, Ey, Ez, Hx, Hy, Hz = 0, 1, 4, 5, 6, 9
+E_stuff, H_stuff = 0, 1
 Dx, Dy, Dz, Bx, By, Bz = 10, 11, 14, 15, 16, 19
 X, Y, Z = 0, 1, 2
 High, Low = 0, 1
@@ -67,6 +68,8 @@ _SIGNATURES = {
     "mp_structure_get_chi1inv": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "mp_structure_add_susceptibility": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int,
                                                   SCALAR_FUNC, C.c_void_p]),
+    "mp_structure_add_susceptibility_ft": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int,
+                                                     SCALAR_FUNC, C.c_void_p]),
     "mp_structure_set_sigma": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "mp_structure_set_conductivity": (C.c_int, [C.c_void_p, C.c_int, SCALAR_FUNC, C.c_void_p]),
     "mp_fields_new": (C.c_void_p, [C.c_void_p]),
@@ -274,14 +277,15 @@ class structure:
         _check(self.lib, self.lib.mp_structure_get_chi1inv(self.ptr, c, _dptr(out)))
         return out.reshape(self.shape)
 
-    def add_susceptibility(self, sigma, omega_0, gamma, drude=False):
-        """add_susceptibility(sigma, E_stuff, lorentzian_susceptibility(omega_0, gamma, drude)).
-        sigma None leaves sigma zero everywhere (fill it with set_sigma)."""
+    def add_susceptibility(self, sigma, omega_0, gamma, drude=False, ft=E_stuff):
+        """add_susceptibility(sigma, ft, lorentzian_susceptibility(omega_0, gamma, drude)),
+        ft E_stuff or H_stuff (a magnetic pole).  sigma None leaves sigma zero everywhere
+        (fill it with set_sigma)."""
         cb = SCALAR_FUNC(0) if sigma is None else SCALAR_FUNC(
             lambda r, _u: float(sigma(r[0], r[1], r[2])))
         self._keep.append(cb)
-        return _check(self.lib, self.lib.mp_structure_add_susceptibility(
-            self.ptr, omega_0, gamma, int(drude), cb, None))
+        return _check(self.lib, self.lib.mp_structure_add_susceptibility_ft(
+            self.ptr, ft, omega_0, gamma, int(drude), cb, None))
 
     def set_conductivity(self, c, cond):
         """structure::set_conductivity(c, double C(const vec &)) (src/meep.hpp:855-856): cond is

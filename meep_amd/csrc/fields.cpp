@@ -129,6 +129,7 @@ bool Geom::interpolate(int c, const V3 &pc, IV locs[8], double w[8]) const {
 struct HostSus {
   double omega_0, gamma; // stored as realnum by the reference (src/meep.hpp:276)
   bool drude;
+  int ft = 0; // the field type it polarises: 0 E_stuff, 1 H_stuff
   std::vector<double> sigma[NC]; // diagonal sigma[c][dir(c)], empty when trivial
 };
 
@@ -656,7 +657,7 @@ template <class R> struct Fields : FieldsBase {
       p.coef.omega0dtsqr_denom = su.drude ? R(0) : p.coef.omega0dtsqr;
       // where sigma is non-zero, by loop axis
       int blo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, bhi[3] = {-1, -1, -1};
-      for (int c = 0; c < 5; ++c) {
+      for (int c = 0; c < 10; ++c) {
         if (su.sigma[c].empty()) continue;
         int j[3], hi[3];
         for (int d = 0; d < 3; ++d) hi[d] = gv.has(d) ? gv.num[d] : 0;
@@ -671,7 +672,7 @@ template <class R> struct Fields : FieldsBase {
       }
       // components whose sigma (in realnum) takes one value over the whole box
       if (bhi[0] >= 0 || bhi[1] >= 0 || bhi[2] >= 0)
-        for (int c = 0; c < 5; ++c) {
+        for (int c = 0; c < 10; ++c) {
           if (su.sigma[c].empty()) continue;
           int j[3], lo[3], hi[3];
           for (int d = 0; d < 3; ++d) {
@@ -698,7 +699,7 @@ template <class R> struct Fields : FieldsBase {
     }
     // sigma of every pole for one component in one block of consecutive arrays (the row
     // kernel addresses a component's poles from one base pointer)
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < 10; ++c) {
       std::vector<size_t> with;
       for (size_t q = 0; q < pols.size(); ++q)
         if (!s->sus[pols[q].isus].sigma[c].empty()) with.push_back(q);
@@ -1199,6 +1200,7 @@ template <class R> struct Fields : FieldsBase {
   bool collect_step(Collect &C) {
     for (auto &sv : sources[B_ft])
       if (srcs[sv.src].is_integrated) fail("integrated magnetic sources are not supported by the GPU engine");
+    if (have_fmp_H()) fail("magnetic susceptibilities are not supported in batched stepping");
     bool have_fmp_E = false;
     for (auto &sv : sources[D_ft])
       if (srcs[sv.src].is_integrated) have_fmp_E = true;
@@ -1541,7 +1543,7 @@ template <class R> struct Fields : FieldsBase {
       e.fw = f_w[ec][cmp];
       e.dsigw = dsigw;
       for (int p = 0; p < MAXPOLE; ++p)
-        if (ft == E_ft && p < int(pols.size()) && pols[p].allocated) {
+        if (p < int(pols.size()) && pols[p].allocated) {
           e.P[p] = pols[p].P[ec][cmp];
           e.Pp[p] = pols[p].Pp[ec][cmp];
           e.sigma[p] = pols[p].sigma[ec];
@@ -1568,7 +1570,16 @@ template <class R> struct Fields : FieldsBase {
   }
 
   bool needs_P(const Pol &p, int c, int cmp) const {
-    return is_electric(c) && p.sigma[c] && f[c][cmp];
+    return (is_electric(c) || is_magnetic(c)) && p.sigma[c] && f[c][cmp];
+  }
+  // some magnetic susceptibility polarises an H component (then H is its own array and the
+  // B phase subtracts and updates P)
+  bool have_fmp_H() const {
+    for (auto &p : pols)
+      for (int c = 5; c < 10; ++c)
+        for (int cmp = 0; cmp < ncmp(); ++cmp)
+          if (needs_P(p, c, cmp)) return true;
+    return false;
   }
   // zeroed arrays of n * ntot values (n consecutive state arrays)
   R *dalloc_n(size_t n) {
@@ -1585,7 +1596,7 @@ template <class R> struct Fields : FieldsBase {
     bool any = false;
     for (auto &p : pols) any = any || !p.allocated;
     if (!any) return;
-    for (int c = 0; c < 5; ++c)
+    for (int c = 0; c < 10; ++c)
       for (int cmp = 0; cmp < ncmp(); ++cmp) {
         std::vector<Pol *> need;
         for (auto &p : pols)
@@ -2033,6 +2044,9 @@ template <class R> struct Fields : FieldsBase {
         for (int cmp = 0; cmp < ncmp(); ++cmp)
           if (needs_P(p, c, cmp)) have_fmp_E = true;
 
+    // the polarisations exist from the first step's B phase on (update_pols(H_stuff) allocates
+    // them there, src/step.cpp:85; the E ones are zero until the D phase either way)
+    alloc_pols();
     if (!plan_valid) rebuild_plan();
     StepArgs<R> A = base_args();
     phase_B(A);
@@ -2087,10 +2101,10 @@ template <class R> struct Fields : FieldsBase {
         off += ls.sv->index.size();
       }
     }
-    // the pole updates wrote the new P into the P_prev arrays
+    // the E pole updates wrote the new P into the P_prev arrays (the B phase does not swap)
     if (pswap) {
       for (auto &p : pols)
-        for (int c = 0; c < NC; ++c)
+        for (int c = 0; c < 5; ++c)
           for (int cmp = 0; cmp < 2; ++cmp) std::swap(p.P[c][cmp], p.Pp[c][cmp]);
       ++ptr_version;
     }
@@ -2123,17 +2137,23 @@ template <class R> struct Fields : FieldsBase {
   // B = B - curl E (+ B sources), then H = mu^-1 B where H is its own array, then the B and
   // H halos: step_db(B_stuff), step_source(B_stuff), step_boundaries(B_stuff),
   // update_eh(H_stuff), step_boundaries(H_stuff) (src/step.cpp:64-80)
+  // With magnetic susceptibilities the update is H = mu^-1 (B - P) and update_P from the new H
+  // (update_eh(H_stuff), update_pols(H_stuff), src/step.cpp:76-88), with P_prev = P; P = new P
+  // (no pointer swap in this phase).
   void phase_B(StepArgs<R> &A) {
+    const bool fmp_H = have_fmp_H();
     for (int cmp = 0; cmp < ncmp(); ++cmp) {
       cur_cmp = cmp;
       int which[3];
       A.nc = curl_comps(B_ft, cmp, A.cc, which);
       A.cmp = cmp;
-      A.npole = 0;
+      A.npole = fmp_H ? int(pols.size()) : 0;
+      for (int p = 0; p < A.npole; ++p) A.pole[p] = pols[p].coef;
+      A.pswap = 0;
       set_sources(A, 0, cmp, t);
       const SrcLaunch &L = splan.L[0][cmp];
-      if (L.rest_cur.empty() && !upd_pending(cmp, which, A.nc, false)) {
-        const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, false);
+      if (L.rest_cur.empty() && !upd_pending(cmp, which, A.nc, fmp_H)) {
+        const int nu = upd_comps(H_ft, cmp, which, A.nc, A.uc, fmp_H);
         step_kernel(A, true, nu > 0, MP_K_STEP_BH);
         continue;
       }
@@ -2141,16 +2161,13 @@ template <class R> struct Fields : FieldsBase {
       for (const LongSrc &ls : L.rest_cur) apply_long(ls, f[dir_comp(15, comp_dir(ls.sv->c))][cmp], cmp);
       StepArgs<R> U = A;
       U.src = nullptr;
-      if (upd_comps(H_ft, cmp, which, U.nc, U.uc, false) > 0) step_kernel(U, false, true, MP_K_STEP_BH);
+      if (upd_comps(H_ft, cmp, which, U.nc, U.uc, fmp_H) > 0) step_kernel(U, false, true, MP_K_STEP_BH);
     }
     if (!plan_valid) rebuild_plan();
     boundaries(0); // B, H (step_boundaries B_stuff .. H_stuff)
   }
 
   // ---- field energy (src/energy_and_flux.cpp:45-178, src/integrate.cpp:46-205) ----
-  // 0.5 * integral of Re(conj(F) G) over the cell for F = E (H) and G = D (B), per component,
-  // in FP64 on the host from read-back arrays: E and D share a grid, so each component is
-  // integrated on its own grid (the reference's four-point average has zero offsets there).
   // 0.5 * fields::integrate of Re(conj(F) G) (src/energy_and_flux.cpp:67-95) for F = E (H)
   // and G = D (B) of each component over [mn, mx], on the component's own grid
   long double energy_sum(int first, const V3 &mn, const V3 &mx) {
@@ -4429,16 +4446,23 @@ int mp_structure_get_chi1inv(const mp_structure *sp, int c, double *arr) {
   });
 }
 int mp_structure_add_susceptibility(mp_structure *sp, double omega_0, double gamma, int drude, mp_scalar_func sigma, void *su) {
+  return mp_structure_add_susceptibility_ft(sp, 0, omega_0, gamma, drude, sigma, su);
+}
+int mp_structure_add_susceptibility_ft(mp_structure *sp, int ft, double omega_0, double gamma, int drude, mp_scalar_func sigma,
+                                       void *su) {
   int idx = -1;
   int rc = guard([&] {
+    if (ft != 0 && ft != 1) fail("add_susceptibility: field type must be E_stuff or H_stuff");
     HostStructure *s = reinterpret_cast<HostStructure *>(sp);
     const Geom &gv = s->gv;
     HostSus sus;
     sus.omega_0 = s->single ? double(float(omega_0)) : omega_0;
     sus.gamma = s->single ? double(float(gamma)) : gamma;
     sus.drude = drude != 0;
-    for (int c = 0; c < 5; ++c) {
-      if (comp_dir(c) == NODIR || (gv.dim == 1 && c != 0)) continue;
+    sus.ft = ft;
+    const int c0 = ft == 0 ? 0 : 5;
+    for (int c = c0; c < c0 + 5; ++c) {
+      if (comp_dir(c) == NODIR || (gv.dim == 1 && c != (ft == 0 ? 0 : 6))) continue;
       std::vector<double> v(gv.ntot, 0.0);
       bool trivial = true;
       for (int j0 = 0; j0 <= (gv.has(0) ? gv.num[0] : 0); ++j0)

@@ -69,6 +69,7 @@
  *   MUT_FIXBND_PHASE  a source point moved to its periodic owner keeps the phase, not its conjugate
  *   MUT_N2F_SIGN      near2far's equivalent magnetic current takes +n x E instead of -n x E
  *   MUT_N2F_PHASE     near2far's periodic copies add without their Bloch phase
+ *   MUT_HPOLE_SUB     a magnetic susceptibility's P is not subtracted from B (H = mu^-1 B)
  */
 #include "../include/meep_amd.h"
 
@@ -528,6 +529,7 @@ struct SrcTime {
 template <class R> struct Sus {
   double omega_0 = 0, gamma = 0;
   bool drude = false;
+  int ft = E_stuff; // the field type it polarises (E_stuff or H_stuff)
   std::vector<R> sigma[NC];
   bool trivial[NC];
 };
@@ -1078,8 +1080,7 @@ template <class R> struct Fields {
         bool need = false;
         if (f[ec][cmp]) {
           need = have_int;
-          if (ft == E_stuff) // only electric susceptibilities are supported
-            for (auto &p : pols) need = need || needs_P(s->sus[p.isus], ec, cmp);
+          for (auto &p : pols) need = need || needs_P(s->sus[p.isus], ec, cmp);
         }
         if (need) {
           if (!fmp[dc][cmp]) fmp[dc][cmp] = alloc(nullptr);
@@ -1098,9 +1099,13 @@ template <class R> struct Fields {
         for (int cmp = 0; cmp < ncmp(); ++cmp)
           if (fmp[dc][cmp]) memcpy(fmp[dc][cmp], f[dc][cmp], sizeof(R) * ntot);
       }
-    if (ft == E_stuff)
-      for (auto &p : pols)
-        if (p.allocated) // subtract_P (src/susceptibility.cpp:264-281)
+#ifdef MUT_HPOLE_SUB
+    for (auto &p : pols)
+      if (p.allocated && s->sus[p.isus].ft == ft && ft == E_stuff)
+#else
+    for (auto &p : pols)
+      if (p.allocated && s->sus[p.isus].ft == ft) // subtract_P (src/susceptibility.cpp:264-281)
+#endif
           for (int ec = c0; ec < c0 + 5; ++ec)
             for (int cmp = 0; cmp < 2; ++cmp)
               if (p.P[ec][cmp]) {
@@ -1149,12 +1154,13 @@ template <class R> struct Fields {
       }
     return allocated_eh;
   }
-  // fields_chunk::update_pols (src/update_pols.cpp:40-62) + lorentzian update_P
+  // fields_chunk::update_pols (src/update_pols.cpp:40-62) + lorentzian update_P, for the
+  // susceptibilities of field type ft (the chiP[ft] list)
   bool update_pols(int ft) {
-    if (ft != E_stuff) return false;
     bool allocated = false;
     for (auto &p : pols) {
       const Sus<R> &su = s->sus[p.isus];
+      if (su.ft != ft) continue;
       if (!p.allocated) {
         for (int c = 0; c < NC; ++c)
           for (int cmp = 0; cmp < 2; ++cmp)
@@ -2192,20 +2198,28 @@ int mp_structure_get_chi1inv(const mp_structure *s, int c, double *arr) {
   });
 }
 int mp_structure_add_susceptibility(mp_structure *s, double omega_0, double gamma, int drude, mp_scalar_func sigma, void *su) {
+  return mp_structure_add_susceptibility_ft(s, E_stuff, omega_0, gamma, drude, sigma, su);
+}
+int mp_structure_add_susceptibility_ft(mp_structure *s, int ft, double omega_0, double gamma, int drude, mp_scalar_func sigma,
+                                       void *su) {
   int idx = -1;
   int rc = guard([&] {
+    if (ft != E_stuff && ft != H_stuff) fail("add_susceptibility: field type must be E_stuff or H_stuff");
     with_structure(reinterpret_cast<AnyStructure *>(s), [&](auto &st) {
       typedef typename std::remove_reference<decltype(st.chi1inv[0][0])>::type R;
       Sus<R> sus;
       sus.omega_0 = R(omega_0);
       sus.gamma = R(gamma);
       sus.drude = drude != 0;
+      sus.ft = ft;
       const GV &gv = st.gv;
       for (int c = 0; c < NC; ++c) sus.trivial[c] = true;
-      // structure_chunk::add_susceptibility (src/anisotropic_averaging.cpp:330-427), E_stuff
-      for (int c = 0; c < 5; ++c) {
+      // structure_chunk::add_susceptibility (src/anisotropic_averaging.cpp:330-427): sigma on
+      // the components of field type ft
+      const int c0 = ft == E_stuff ? 0 : 5;
+      for (int c = c0; c < c0 + 5; ++c) {
         if (component_direction(c) == NODIR) continue;
-        if (gv.dim == 1 && c != 0) continue;
+        if (gv.dim == 1 && c != (ft == E_stuff ? 0 : 6)) continue;
         std::vector<R> arr(gv.ntot, R(0));
         bool trivial = true;
         ivec s0 = gv.iyee_shift(c), e0 = gv.big_corner();

@@ -21,7 +21,7 @@ run against a value the mathematics fixes; the tests assert on these errors.
 * ``source_spectrum``: the discrete Gaussian current's transform against the Gaussian
   integral (the pulse's Fourier transform in closed form).
 * ``pole_free_response``: a uniform field in a periodic cell of a Lorentz or Drude medium
-  with D held fixed.  The reference's continuum model, chi(omega) = sigma omega0^2 /
+  with D held fixed (or, for a magnetic pole, the same on H with B held fixed).  The reference's continuum model, chi(omega) = sigma omega0^2 /
   (omega0^2 - omega^2 - i gamma omega) with omega0 = 2 pi f0 and gamma in 2 pi units
   (src/susceptibility.cpp:297-307), makes P relax to sigma D / (1 + sigma) (Lorentz) or to D
   (Drude: no restoring term), oscillating at sqrt(omega0^2 (1 + sigma) - (pi gamma)^2)
@@ -91,7 +91,7 @@ import math
 
 import numpy as np
 
-from meep_amd.capi import (ARR_F, ARR_P, Bx, By, Dx, Dz, Ex, Ey, Ez, Hx, Hy, Hz, High, Low, Magnetic, Metallic, X, Y,
+from meep_amd.capi import (ARR_F, ARR_P, Bx, By, Dx, Dz, E_stuff, Ex, Ey, Ez, H_stuff, Hx, Hy, Hz, High, Low, Magnetic, Metallic, X, Y,
                            Z, continuous_src_time, fields, gaussian_src_time, structure)
 
 S = 0.5  # Courant number of every check
@@ -417,25 +417,27 @@ def source_spectrum(dt=0.025, fcen=1.0, fwidth=0.5, freqs=None):
     return float(np.abs(got - exp).max() / np.abs(exp).max())
 
 
-def pole_free_response(lib, drude, a=100, f0=1.0, gamma=0.3, sigma=2.0, T=20.0):
+def pole_free_response(lib, drude, a=100, f0=1.0, gamma=0.3, sigma=2.0, T=20.0, magnetic=False):
     """Errors of a pole's relaxation against the continuum model: the final P against its
-    static value, and the oscillation frequency and decay rate of P against the model's."""
+    static value, and the oscillation frequency and decay rate of P against the model's.
+    magnetic: the same pole on H (add_susceptibility with H_stuff), with B held fixed."""
     s = structure(lib, dim=1, size=(0, 0, 3.0 / a), a=a)
-    s.add_susceptibility(lambda x, y, z: sigma, f0, gamma, drude=drude)
+    s.add_susceptibility(lambda x, y, z: sigma, f0, gamma, drude=drude, ft=H_stuff if magnetic else E_stuff)
     f = fields(s)
     f.use_real_fields()
     f.use_bloch((0.0,))
     # a source of zero amplitude allocates the fields without driving them
     f.add_volume_source(Ex, continuous_src_time(1.0), (0, 0, 0.5 / a), (0, 0, 0.5 / a), amp=0.0)
     one = np.ones(s.shape)
-    f.set_array(Dx, 0, ARR_F, one)
-    f.set_array(Ex, 0, ARR_F, one)
+    fc, gc = (Hy, By) if magnetic else (Ex, Dx)
+    f.set_array(gc, 0, ARR_F, one)
+    f.set_array(fc, 0, ARR_F, one)
     dt = s.dt
     n = int(round(T / dt))
     P = np.empty(n)
     for k in range(n):
         f.step()
-        P[k] = f.get_array(Ex, 0, ARR_P, 0)[0][1]
+        P[k] = f.get_array(fc, 0, ARR_P, 0)[0][1]
     p_inf = 1.0 if drude else sigma / (1.0 + sigma)
     om0 = 2 * math.pi * f0
     rate = math.pi * gamma

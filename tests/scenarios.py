@@ -8,7 +8,7 @@ import math
 
 import numpy as np
 
-from meep_amd.capi import (Bx, By, Bz, Dx, Dy, Dz, Ex, Ey, Ez, Hx, Hy, Hz, High, Low, X, Y, Z,
+from meep_amd.capi import (Bx, By, Bz, Dx, Dy, Dz, Ex, Ey, Ez, H_stuff, Hx, Hy, Hz, High, Low, X, Y, Z,
                            Magnetic, Metallic, continuous_src_time, fields, gaussian_src_time,
                            linspace, pml, structure)
 
@@ -270,6 +270,26 @@ def straight_waveguide(lib, single=False, real=True):
     return s, f, [fl]
 
 
+def magnetic_poles_3d(lib, single=False, a=8):
+    """A magnetic Lorentz pole (add_susceptibility with H_stuff: P subtracted from B and updated
+    from H) with sigma graded along y in the upper half, an electric Lorentz pole in the lower
+    half, z uPML, complex fields under a Bloch phase along x and y, an Ex current plane and an
+    Hx point current, a flux plane through the magnetic medium."""
+    L = 2.0
+    s = structure(lib, dim=3, size=(1.5, 1.25, L), a=a, eps=lambda x, y, z: 2.0 if z < L / 2 else 1.0,
+                  pml=pml(0.375, Z), single=single)
+    s.add_susceptibility(lambda x, y, z: 1.5 if z < L / 2 else 0.0, 1.0, 0.05)
+    s.add_susceptibility(lambda x, y, z: (0.8 + 0.4 * y) if z > L / 2 else 0.0, 0.9, 0.08, ft=H_stuff)
+    f = fields(s)
+    f.use_bloch_dir(X, 0.3)
+    f.use_bloch_dir(Y, 0.2)
+    src = gaussian_src_time(0.8, 0.4)
+    f.add_volume_source(Ex, src, (0.0, 0.0, 0.6), (1.5, 1.25, 0.6))
+    f.add_volume_source(Hx, gaussian_src_time(0.7, 0.4), (0.7, 0.6, 1.4), (0.7, 0.6, 1.4), amp=0.5)
+    fl = f.add_dft_flux(Z, (0.0, 0.0, 1.3), (1.5, 1.25, 1.3), linspace(0.6, 1.0, 6))
+    return s, f, [fl]
+
+
 SCENARIOS = {
     "waveguide_2d": (waveguide_2d, 1600),
     "box_3d_pml_lorentz": (box_3d_pml_lorentz, 120),
@@ -284,6 +304,7 @@ SCENARIOS = {
     "conductive_3d": (conductive_3d, 100),
     "stretched_pml_3d": (stretched_pml_3d, 100),
     "data_sources_3d": (data_sources_3d, 80),
+    "magnetic_poles_3d": (magnetic_poles_3d, 100),
 }
 
 FIELD_COMPONENTS = (Ex, Ey, Ez, Hx, Hy, Hz, Dx, Dy, Dz, Bx, By, Bz)

@@ -19,9 +19,10 @@ def test_plane_wave_flux_weights_gpu(gpu_lib):
     T.check_plane_wave(gpu_lib)
 
 
+@pytest.mark.parametrize("magnetic", [False, True], ids=["E", "H"])
 @pytest.mark.parametrize("drude", [False, True], ids=["lorentz", "drude"])
-def test_pole_relaxation_and_oscillation_gpu(gpu_lib, drude):
-    T.check_pole(gpu_lib, drude)
+def test_pole_relaxation_and_oscillation_gpu(gpu_lib, drude, magnetic):
+    T.check_pole(gpu_lib, drude, magnetic)
 
 
 def test_magnetic_and_metallic_walls_quarter_wave_mode_gpu(gpu_lib):

@@ -8,8 +8,8 @@ values do not reach (tests/closed_forms.py states each closed form):
 * Flux of a periodic line current in 2D through a transverse line: Ly times the 1D flux (the
   integration weights).  Observed about 1e-8.
 * The discrete Gaussian current's transform against the Gaussian integral.
-* Lorentz and Drude poles: relaxation to the static value and free oscillation at the
-  continuum model's frequency and damping (to O((omega dt)^2)).
+* Lorentz and Drude poles, electric and magnetic (H_stuff): relaxation to the static value and
+  free oscillation at the continuum model's frequency and damping (to O((omega dt)^2)).
 * A quarter-wave cavity between a magnetic and a metallic wall: the lattice eigenmode
   evolves exactly (both walls' placement).
 * Energy conservation in a lossless metallic cavity (field_energy).
@@ -78,8 +78,8 @@ def check_plane_wave(lib):
     assert err["flux"] <= TOL["plane"], err
 
 
-def check_pole(lib, drude):
-    err = CF.pole_free_response(lib, drude)
+def check_pole(lib, drude, magnetic=False):
+    err = CF.pole_free_response(lib, drude, magnetic=magnetic)
     assert err["p_final"] <= TOL["p_final"], err
     assert err["frequency"] <= TOL["pole_freq"], err
     assert err["decay"] <= TOL["pole_decay"], err
@@ -171,9 +171,10 @@ def test_gaussian_current_spectrum():
     assert CF.source_spectrum() <= TOL["spectrum"]
 
 
+@pytest.mark.parametrize("magnetic", [False, True], ids=["E", "H"])
 @pytest.mark.parametrize("drude", [False, True], ids=["lorentz", "drude"])
-def test_pole_relaxation_and_oscillation(oracle_lib, drude):
-    check_pole(oracle_lib, drude)
+def test_pole_relaxation_and_oscillation(oracle_lib, drude, magnetic):
+    check_pole(oracle_lib, drude, magnetic)
 
 
 def test_pole_frequency_converges_as_dt_squared(oracle_lib):

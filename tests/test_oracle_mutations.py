@@ -51,6 +51,7 @@ CHECKS = {
     "FIXBND_PHASE": T.check_srcdata,
     "N2F_SIGN": T.check_n2f_equivalence,
     "N2F_PHASE": T.check_n2f_periodic,
+    "HPOLE_SUB": lambda lib: T.check_pole(lib, False, magnetic=True),
 }
 
 
