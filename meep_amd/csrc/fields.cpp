@@ -1137,8 +1137,9 @@ template <class R> struct Fields : FieldsBase {
     for (int c = 0; c < 3; ++c) {
       CurlComp<R> &k = a.cc[c];
       axis_slot(k.ax, k.lo, k.hi, k.ysh);
-      k.ksig = axis_of(k.dsig);
-      k.ksigu = axis_of(k.dsigu);
+      // a slot the launch does not move may hold zeroed indices (direction 0, absent in 1D)
+      k.ksig = k.f ? axis_of(k.dsig) : -1;
+      k.ksigu = k.f ? axis_of(k.dsigu) : -1;
       k.ysig = k.dsig >= 0 ? k.ysh[k.dsig] : 0;
       k.ysigu = k.dsigu >= 0 ? k.ysh[k.dsigu] : 0;
       k.sig = k.dsig >= 0 ? a.pml.sig[k.dsig] : nullptr;
@@ -1149,7 +1150,7 @@ template <class R> struct Fields : FieldsBase {
       k.siginvu = k.dsigu >= 0 ? a.pml.siginv[k.dsigu] : nullptr;
       UpdComp<R> &u = a.uc[c];
       axis_slot(u.ax, u.lo, u.hi, u.ysh);
-      u.ksigw = axis_of(u.dsigw);
+      u.ksigw = u.f ? axis_of(u.dsigw) : -1;
       u.ysigw = u.dsigw >= 0 ? u.ysh[u.dsigw] : 0;
       u.sigw = u.dsigw >= 0 ? a.pml.sig[u.dsigw] : nullptr;
       u.kapw = u.dsigw >= 0 ? a.pml.kap[u.dsigw] : nullptr;

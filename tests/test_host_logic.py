@@ -104,3 +104,21 @@ def test_batch_plan_poles_and_bloch_phases(host_lib):
     b = capi.batch([f for _, f, _ in sims], use_graph=True, autotune=False)
     b.step_n(5)
     assert all(f.t == 5 for _, f, _ in sims)
+
+
+@pytest.mark.parametrize("magnetic", [False, True], ids=["E", "H"])
+def test_1d_pole_plans(host_lib, magnetic):
+    """The 1D pole set-up of the closed forms plans on the host: with a magnetic pole the
+    first B phase runs its curl and its update as two launches (H is split from B there), and
+    the curl-only launch's unused update slots name no direction the 1D grid lacks."""
+    import closed_forms as CF
+    CF.pole_free_response(host_lib, False, magnetic=magnetic, T=0.05)
+
+
+def test_batch_refuses_magnetic_poles(host_lib):
+    from meep_amd.capi import MeepError, batch
+    from scenarios import magnetic_poles_3d
+    s, f, _ = magnetic_poles_3d(host_lib)
+    b = batch([f], use_graph=False, autotune=False)
+    with pytest.raises(MeepError, match="magnetic susceptibilities"):
+        b.step_n(1)
