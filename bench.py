@@ -162,11 +162,13 @@ def build_pml_lorentz_dft(lib, single, n=PLD_CELLS, npml=PLD_PML, a=PLD_A, nfreq
 def pml_lorentz_dft_bytes(n=PLD_CELLS, npml=PLD_PML, word=4, nfreq=PLD_NFREQ, acc_bytes=16, dft_points=None,
                           sigma_arrays=False):
     """Reference-state bytes per step: 21 words per cell, +10 per PML face cell, +20 per edge
-    cell, +30 per corner cell, +12 per pole over dispersive cells (P and P_prev read and
-    written), +3 more where sigma is read as an array (sigma_arrays; the workload's sigma is
-    one value over the lower half, which the kernels take as a scalar), plus each DFT
-    accumulator read and written once (dft_points: the monitor's points over its chunks; a
-    plane between grid planes has two planes of points for the interpolation)."""
+    cell, +30 per corner cell (of each 10, the 8 uPML auxiliary words f_u / f_w of one B, H,
+    D and E component read and written are FP64, 8 bytes, whatever the field word), +12 per
+    pole over dispersive cells (P and P_prev read and written), +3 more where sigma is read
+    as an array (sigma_arrays; the workload's sigma is one value over the lower half, which
+    the kernels take as a scalar), plus each DFT accumulator read and written once
+    (dft_points: the monitor's points over its chunks; a plane between grid planes has two
+    planes of points for the interpolation)."""
     cnt = np.zeros(4)
     regions = []
     for d in range(3):
@@ -179,10 +181,11 @@ def pml_lorentz_dft_bytes(n=PLD_CELLS, npml=PLD_PML, word=4, nfreq=PLD_NFREQ, ac
         cnt[q] = np.count_nonzero(k == q)
     cells = np.prod(n)
     disp = n[0] * n[1] * (n[2] // 2)
-    words = 21 * cells + 10 * cnt[1] + 20 * cnt[2] + 30 * cnt[3] + 2 * (15 if sigma_arrays else 12) * disp
+    pml_units = cnt[1] + 2 * cnt[2] + 3 * cnt[3]
+    words = 21 * cells + 2 * pml_units + 2 * (15 if sigma_arrays else 12) * disp
     npts = sum(dft_points) if dft_points is not None else 3 * (n[0] + 1) * (n[1] + 1)
     dft = npts * nfreq * 2 * acc_bytes
-    return words * word + dft
+    return words * word + 8 * pml_units * 8 + dft
 
 
 # ---- the 2D differentiation workload -----------------------------------------------------
@@ -293,12 +296,13 @@ def batch2d_bytes(sides, word=4):
 def de_interior_bytes(n=PLD_CELLS, npml=PLD_PML, word=4):
     """Algorithmic bytes of one D-phase interior launch of the uPML + Lorentz-Drude workload:
     its box is x, y in the quiet range (npml .. n - npml - 1) and every z; 12 words per cell,
-    30 more per dispersive cell (lower half), 4 more per z-uPML cell."""
+    30 more per dispersive cell (lower half), and per z-uPML cell f_u of Dx and f_w of Ez read
+    and written: 4 auxiliary words, FP64 (8 bytes) whatever the field word."""
     nx, ny = n[0] - 2 * npml, n[1] - 2 * npml
     cells = nx * ny * n[2]
     disp = nx * ny * (n[2] // 2)
     zpml = nx * ny * 2 * npml
-    return (12 * cells + 30 * disp + 4 * zpml) * word
+    return (12 * cells + 30 * disp) * word + 4 * zpml * 8
 
 
 def box256_de_bytes(n=N, word=4):
