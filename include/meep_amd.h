@@ -308,6 +308,14 @@ int mp_dft_fields_get_dft(mp_dft_fields *df, int i, double *out);
 long mp_dft_fields_get_dft_array(mp_dft_fields *df, int c, int num_freq, int *rank, size_t *dims,
                                  double *out);
 
+/* Replace chunk i's accumulators with in (interleaved (re, im), frequency fastest, as
+   mp_dft_flux_get_dft gives them): the per-chunk load of the Python layer's load_flux_data
+   (python/simulation.py:3638-3647, mp._load_dft_data over flux.E and flux.H). */
+int mp_dft_flux_set_dft(mp_dft_flux *fl, int which, int i, const double *in);
+/* dft_flux::scale_dfts(complex<double>) — src/dft.cpp:597-600, dft_chunk::scale_dft
+   :402-406 (load_minus_flux_data scales by -1, python/simulation.py:3651-3659). */
+int mp_dft_flux_scale_dfts(mp_dft_flux *fl, double re, double im);
+
 /* ---- time-domain flux ----------------------------------------------------------------- */
 
 /* fields::add_flux_vol(direction, const volume &) — src/meep.hpp:2205-2207,
@@ -368,6 +376,14 @@ int mp_near2far_flux(mp_near2far *n, int d, const double *wmin, const double *wm
    as mp_dft_flux_get_dft_array, collapsed over the first surface's empty directions. */
 long mp_near2far_get_dft_array(mp_near2far *n, int c, int num_freq, int *rank, size_t *dims,
                                double *out);
+/* The near2far monitor's chunk list (dft_near2far::F) as mp_dft_flux_* give a flux's E list:
+   get_near2far_data / load_near2far_data (python/simulation.py:3355-3371), and
+   dft_near2far::scale_dfts (src/near2far.cpp:124-126; load_minus_near2far_data scales by -1). */
+int mp_near2far_num_chunks(mp_near2far *n);
+long mp_near2far_chunk_points(mp_near2far *n, int i);
+int mp_near2far_get_dft(mp_near2far *n, int i, double *out);
+int mp_near2far_set_dft(mp_near2far *n, int i, const double *in);
+int mp_near2far_scale_dfts(mp_near2far *n, double re, double im);
 /* green2d / green3d (src/near2far.cpp:133-186, 207-267): EH[12] of a point current of
    component c0 and amplitude f0[2] at x0, seen at x, in the medium eps, mu; dim is 2 or 3. */
 int mp_near2far_green(int dim, const double *x, double freq, double eps, double mu, const double *x0,

@@ -369,6 +369,8 @@ public:
     check(mp_dft_flux_flux(h, out));
     return out;
   }
+  // scale_dfts (src/dft.cpp:597-600)
+  void scale_dfts(std::complex<double> scale) { check(mp_dft_flux_scale_dfts(h, scale.real(), scale.imag())); }
   int Nfreq() const { return n; }
   mp_dft_flux *handle() const { return h; }
 
@@ -504,6 +506,8 @@ public:
     }
     return out;
   }
+  // scale_dfts (src/near2far.cpp:124-126)
+  void scale_dfts(std::complex<double> scale) { check(mp_near2far_scale_dfts(h, scale.real(), scale.imag())); }
   int Nfreq() const { return n; }
   mp_near2far *handle() const { return h; }
 

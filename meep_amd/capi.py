@@ -168,6 +168,13 @@ _SIGNATURES = {
                                    C.c_double, C.POINTER(C.c_double)]),
     "mp_near2far_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                              C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
+    "mp_dft_flux_set_dft": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mp_dft_flux_scale_dfts": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "mp_near2far_num_chunks": (C.c_int, [C.c_void_p]),
+    "mp_near2far_chunk_points": (C.c_long, [C.c_void_p, C.c_int]),
+    "mp_near2far_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_near2far_set_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_near2far_scale_dfts": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "mp_near2far_green": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
                                     C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
@@ -349,6 +356,38 @@ class dft_flux:
         _check(self.lib, self.lib.mp_dft_flux_get_dft(self.ptr, which, i, _dptr(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
 
+    def set_dft(self, which, i, values):
+        """Replace chunk i's accumulators (shape (points, nfreq))."""
+        v = np.ascontiguousarray(values, dtype=np.complex128).reshape(-1)
+        raw = np.empty(2 * v.size)
+        raw[0::2], raw[1::2] = v.real, v.imag
+        _check(self.lib, self.lib.mp_dft_flux_set_dft(self.ptr, which, i, _dptr(raw)))
+
+    def scale_dfts(self, scale):
+        """dft_flux::scale_dfts (src/dft.cpp:597-600)."""
+        scale = complex(scale)
+        _check(self.lib, self.lib.mp_dft_flux_scale_dfts(self.ptr, scale.real, scale.imag))
+
+    def get_data(self):
+        """The accumulators of the E and H lists (the Python layer's get_flux_data,
+        python/simulation.py:3628-3636), per chunk."""
+        return ([self.dft(0, i) for i in range(self.num_chunks(0))],
+                [self.dft(1, i) for i in range(self.num_chunks(1))])
+
+    def load_data(self, data):
+        """load_flux_data (python/simulation.py:3638-3647): replace the accumulators."""
+        E, H = data
+        if len(E) != self.num_chunks(0) or len(H) != self.num_chunks(1):
+            raise MeepError("meep: flux data from a monitor with a different chunk layout")
+        for which, lst in ((0, E), (1, H)):
+            for i, v in enumerate(lst):
+                self.set_dft(which, i, v)
+
+    def load_minus_data(self, data):
+        """load_minus_flux_data (python/simulation.py:3651-3659): load, then scale by -1."""
+        self.load_data(data)
+        self.scale_dfts(-1.0)
+
     def get_dft_array(self, c, num_freq):
         """fields::get_dft_array(dft_flux, c, num_freq, ...) (src/dft.cpp:1259-1268)."""
         return _get_dft_array(self.lib.mp_dft_flux_get_dft_array, self.ptr, self.lib,
@@ -465,6 +504,39 @@ class near2far:
         _check(self.lib, self.lib.mp_near2far_flux(self.ptr, d, _dvec(vmin), _dvec(vmax), resolution,
                                                    _dptr(out)))
         return out
+
+    def num_chunks(self):
+        return self.lib.mp_near2far_num_chunks(self.ptr)
+
+    def dft(self, i):
+        n = self.lib.mp_near2far_chunk_points(self.ptr, i)
+        out = np.empty(2 * n * self.nfreq)
+        _check(self.lib, self.lib.mp_near2far_get_dft(self.ptr, i, _dptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(n, self.nfreq)
+
+    def scale_dfts(self, scale):
+        """dft_near2far::scale_dfts (src/near2far.cpp:124-126)."""
+        scale = complex(scale)
+        _check(self.lib, self.lib.mp_near2far_scale_dfts(self.ptr, scale.real, scale.imag))
+
+    def get_data(self):
+        """get_near2far_data (python/simulation.py:3355-3362), per chunk."""
+        return [self.dft(i) for i in range(self.num_chunks())]
+
+    def load_data(self, data):
+        """load_near2far_data (python/simulation.py:3364-3371)."""
+        if len(data) != self.num_chunks():
+            raise MeepError("meep: near2far data from a monitor with a different chunk layout")
+        for i, vals in enumerate(data):
+            v = np.ascontiguousarray(vals, dtype=np.complex128).reshape(-1)
+            raw = np.empty(2 * v.size)
+            raw[0::2], raw[1::2] = v.real, v.imag
+            _check(self.lib, self.lib.mp_near2far_set_dft(self.ptr, i, _dptr(raw)))
+
+    def load_minus_data(self, data):
+        """load_minus_near2far_data (python/simulation.py:3373-3380)."""
+        self.load_data(data)
+        self.scale_dfts(-1.0)
 
     def get_dft_array(self, c, num_freq):
         """fields::get_dft_array(dft_near2far, c, num_freq, ...) (src/dft.cpp:1282-1290)."""

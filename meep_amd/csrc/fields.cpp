@@ -373,6 +373,10 @@ struct FieldsBase {
   virtual int flux_chunks(void *fl, int which) = 0;
   virtual long flux_points(void *fl, int which, int i) = 0;
   virtual void flux_dft(void *fl, int which, int i, double *out) = 0;
+  // replace chunk i's accumulators with in (re, im pairs); multiply every accumulator of the
+  // monitor (both lists) by s
+  virtual void flux_set_dft(void *fl, int which, int i, const double *in) = 0;
+  virtual void flux_scale(void *fl, cdouble s) = 0;
   // fields::get_dft_array over the flux (both lists) or dft_fields (E list) monitor fl;
   // returns the number of complex values
   virtual long dft_array(void *fl, bool both, int c, int num_freq, int *rank, size_t *dims, double *out) = 0;
@@ -3546,6 +3550,51 @@ template <class R> struct Fields : FieldsBase {
       out[2 * q + 1] = v[q].imag();
     }
   }
+  // the accumulators of one chunk from host values (a load of data saved from the chunk's
+  // get_dft), stored as the chunk keeps them; the compensation of compensated ones restarts
+  void write_dft(DftChunk<R> &ch, const std::vector<cdouble> &v) {
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (ch.nsnap) fail("internal: deferred DFT steps pending");
+    const size_t n = size_t(ch.npts) * ch.nfreq;
+    if (v.size() != n) fail("internal: DFT data of the wrong size");
+    if (ch.acc_kind == MP_DFT_F64) {
+      HIPCHECK(hipMemcpy(ch.acc.p, v.data(), 16 * n, hipMemcpyHostToDevice));
+    }
+    else if (ch.acc_kind == MP_DFT_KAHAN32) {
+      std::vector<float> a(2 * n), c(2 * n, 0.0f);
+      for (size_t q = 0; q < n; ++q) a[2 * q] = float(v[q].real()), a[2 * q + 1] = float(v[q].imag());
+      HIPCHECK(hipMemcpy(ch.acc.p, a.data(), 8 * n, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(ch.comp.p, c.data(), 8 * n, hipMemcpyHostToDevice));
+    }
+    else {
+      std::vector<R> a(2 * n);
+      for (size_t q = 0; q < n; ++q) a[2 * q] = R(v[q].real()), a[2 * q + 1] = R(v[q].imag());
+      HIPCHECK(hipMemcpy(ch.acc.p, a.data(), 2 * sizeof(R) * n, hipMemcpyHostToDevice));
+    }
+  }
+  void flux_set_dft(void *flp, int which, int i, const double *in) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    auto &l = which ? fl->H : fl->E;
+    if (i < 0 || i >= int(l.size())) fail("no such dft chunk");
+    const size_t n = size_t(l[i]->npts) * l[i]->nfreq;
+    std::vector<cdouble> v(n);
+    for (size_t q = 0; q < n; ++q) v[q] = cdouble(in[2 * q], in[2 * q + 1]);
+    write_dft(*l[i], v);
+  }
+  // dft_chunk::scale_dft (src/dft.cpp:402-406): dft[i] *= scale, in the accumulator's type
+  void flux_scale(void *flp, cdouble sc) override {
+    Flux<R> *fl = static_cast<Flux<R> *>(flp);
+    for (auto *lst : {&fl->E, &fl->H})
+      for (DftChunk<R> *ch : *lst) {
+        std::vector<cdouble> v = read_dft(*ch);
+        for (cdouble &x : v)
+          if (ch->acc_kind == MP_DFT_REALNUM)
+            x = cdouble(std::complex<R>(R(x.real()), R(x.imag())) * std::complex<R>(R(sc.real()), R(sc.imag())));
+          else
+            x *= sc;
+        write_dft(*ch, v);
+      }
+  }
   // fields::get_dft_array (src/dft.cpp:1259-1290).  The grid spanned by the chunks of c
   // (fields::get_dft_component_dims, :1061-1110: corners seeded from the cell's corners one
   // step out) has a dimension per direction with more than one point.  Every point holds its
@@ -4852,6 +4901,14 @@ int mp_dft_flux_get_dft(mp_dft_flux *flp, int which, int i, double *out) {
   FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
   return guard([&] { h->owner->flux_dft(h->fl, which, i, out); });
 }
+int mp_dft_flux_set_dft(mp_dft_flux *flp, int which, int i, const double *in) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
+  return guard([&] { h->owner->flux_set_dft(h->fl, which, i, in); });
+}
+int mp_dft_flux_scale_dfts(mp_dft_flux *flp, double re, double im) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(flp);
+  return guard([&] { h->owner->flux_scale(h->fl, cdouble(re, im)); });
+}
 
 mp_dft_fields *mp_fields_add_dft_fields(mp_fields *f, const int *comps, int ncomp, const double *vmin, const double *vmax,
                                         const double *freq, int nfreq, int centered, int dec) {
@@ -5004,6 +5061,19 @@ int mp_near2far_flux(mp_near2far *n, int d, const double *wmin, const double *wm
 }
 long mp_near2far_get_dft_array(mp_near2far *n, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(n, false, c, num_freq, rank, dims, out);
+}
+int mp_near2far_num_chunks(mp_near2far *n) { return mp_dft_flux_num_chunks(reinterpret_cast<mp_dft_flux *>(n), 0); }
+long mp_near2far_chunk_points(mp_near2far *n, int i) {
+  return mp_dft_flux_chunk_points(reinterpret_cast<mp_dft_flux *>(n), 0, i);
+}
+int mp_near2far_get_dft(mp_near2far *n, int i, double *out) {
+  return mp_dft_flux_get_dft(reinterpret_cast<mp_dft_flux *>(n), 0, i, out);
+}
+int mp_near2far_set_dft(mp_near2far *n, int i, const double *in) {
+  return mp_dft_flux_set_dft(reinterpret_cast<mp_dft_flux *>(n), 0, i, in);
+}
+int mp_near2far_scale_dfts(mp_near2far *n, double re, double im) {
+  return mp_dft_flux_scale_dfts(reinterpret_cast<mp_dft_flux *>(n), re, im);
 }
 // green2d / green3d on the device: the far-field kernel with one current and one point
 int mp_near2far_green(int dim, const double *x, double freq, double eps, double mu, const double *x0, int c0, const double *f0,

@@ -70,6 +70,7 @@
  *   MUT_N2F_SIGN      near2far's equivalent magnetic current takes +n x E instead of -n x E
  *   MUT_N2F_PHASE     near2far's periodic copies add without their Bloch phase
  *   MUT_HPOLE_SUB     a magnetic susceptibility's P is not subtracted from B (H = mu^-1 B)
+ *   MUT_SCALE_DFTS    scale_dfts scales a flux's E accumulators only
  */
 #include "../include/meep_amd.h"
 
@@ -2603,6 +2604,47 @@ int mp_dft_flux_get_dft(mp_dft_flux *flp, int which, int i, double *out) {
   });
 }
 
+int mp_dft_flux_set_dft(mp_dft_flux *flp, int which, int i, const double *in) {
+  AnyFlux *fl = reinterpret_cast<AnyFlux *>(flp);
+  return guard([&] {
+    with_fields(fl->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      auto &lst = which ? fl->H : fl->E;
+      if (i < 0 || i >= int(lst.size())) fail("no such dft chunk");
+      auto *ch = reinterpret_cast<DftChunk<R> *>(lst[i]);
+      size_t n = ch->N * fl->freq.size();
+      for (size_t j = 0; j < n; ++j) {
+        if (ch->acc == MP_DFT_F64)
+          ch->dft64[j] = cdouble(in[2 * j], in[2 * j + 1]);
+        else
+          ch->dft[j] = std::complex<R>(R(in[2 * j]), R(in[2 * j + 1]));
+      }
+      return 0;
+    });
+  });
+}
+// dft_flux::scale_dfts (src/dft.cpp:597-600) over dft_chunk::scale_dft (:402-406): every
+// accumulator of both lists times scale, in the accumulator's type
+int mp_dft_flux_scale_dfts(mp_dft_flux *flp, double re, double im) {
+  AnyFlux *fl = reinterpret_cast<AnyFlux *>(flp);
+  return guard([&] {
+    with_fields(fl->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+#ifdef MUT_SCALE_DFTS
+      for (auto *lst : {&fl->E})
+#else
+      for (auto *lst : {&fl->E, &fl->H})
+#endif
+        for (void *p : *lst) {
+          auto *ch = reinterpret_cast<DftChunk<R> *>(p);
+          for (auto &x : ch->dft) x *= std::complex<R>(R(re), R(im));
+          for (auto &x : ch->dft64) x *= cdouble(re, im);
+        }
+      return 0;
+    });
+  });
+}
+
 static long dft_array(AnyFlux *fl, bool flux, int c, int num_freq, int *rank, size_t *dims, double *out) {
   long count = 0;
   const int rc = guard([&] {
@@ -3187,6 +3229,20 @@ int mp_near2far_flux(mp_near2far *h, int d, const double *wmin, const double *wm
 }
 long mp_near2far_get_dft_array(mp_near2far *h, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(reinterpret_cast<AnyFlux *>(h), false, c, num_freq, rank, dims, out);
+}
+int mp_near2far_num_chunks(mp_near2far *n) { return mp_dft_flux_num_chunks(reinterpret_cast<mp_dft_flux *>(n), 0); }
+long mp_near2far_chunk_points(mp_near2far *n, int i) {
+  return mp_dft_flux_chunk_points(reinterpret_cast<mp_dft_flux *>(n), 0, i);
+}
+int mp_near2far_get_dft(mp_near2far *n, int i, double *out) {
+  return mp_dft_flux_get_dft(reinterpret_cast<mp_dft_flux *>(n), 0, i, out);
+}
+int mp_near2far_set_dft(mp_near2far *n, int i, const double *in) {
+  return mp_dft_flux_set_dft(reinterpret_cast<mp_dft_flux *>(n), 0, i, in);
+}
+// dft_near2far::scale_dfts (src/near2far.cpp:124-126)
+int mp_near2far_scale_dfts(mp_near2far *n, double re, double im) {
+  return mp_dft_flux_scale_dfts(reinterpret_cast<mp_dft_flux *>(n), re, im);
 }
 // green2d / green3d as library routines (fields of a point current c0 of amplitude f0 at x0)
 int mp_near2far_green(int dim, const double *x, double freq, double eps, double mu, const double *x0, int c0,

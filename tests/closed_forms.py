@@ -79,6 +79,11 @@ run against a value the mathematics fixes; the tests assert on these errors.
 * ``near2far_periodic``: with Nperiods copies of a surface that spans a Bloch-periodic cell,
   the transform at x is the sum over n of exp(i n 2 pi k L) times the single-period transform
   at x - n L (a copy of every current sits n L away with the Bloch phase of its offset).
+* ``dft_data_reload``: the normalisation-run pattern of the Python layer (get_flux_data, then
+  load_minus_flux_data into an identical run before it steps): the second run's flux and
+  near2far transform are zero up to roundoff, since its accumulators are the first run's
+  minus themselves; and scale_dfts(s) multiplies a flux by |s|^2 (E and H both scaled) and a
+  transform by s.
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -677,3 +682,37 @@ def near2far_periodic(lib, nperiods=3, ky=0.27, a=10):
                    for n in range(-nperiods, nperiods + 1))
         err = max(err, float(np.abs(got - want).max() / np.abs(want).max()))
     return err, many.periodic_d, many.periodic_n
+
+
+def _reload_run(lib, single, before_steps=None, nsteps=150):
+    s = structure(lib, dim=2, size=(5.0, 4.0), a=10, pml=[(X, side, 1.0) for side in (High, Low)],
+                  single=single)
+    f = fields(s)
+    f.add_volume_source(Ez, gaussian_src_time(0.7, 0.4), (1.6, 2.05, 0), (1.6, 2.05, 0))
+    fl = f.add_dft_flux(X, (3.3, 0.2, 0), (3.3, 3.8, 0), np.linspace(0.5, 0.9, 5), decimation_factor=1)
+    n2f = f.add_dft_near2far([((2.9, 0.5, 0), (2.9, 3.5, 0), 1.0)], np.linspace(0.5, 0.9, 5), decimation_factor=1)
+    if before_steps:
+        before_steps(fl, n2f)
+    f.step_n(nsteps)
+    return f, fl, n2f
+
+
+def dft_data_reload(lib, single=False):
+    """Errors of the flux-data reload pattern and of scale_dfts (module docstring)."""
+    x = (8.0, 1.0, 0.0)
+    f1, fl1, n1 = _reload_run(lib, single)
+    data, ndata = fl1.get_data(), n1.get_data()
+    flux1, far1 = np.asarray(fl1.flux()), n1.farfield(x)
+
+    def load(fl, n2f):
+        fl.load_minus_data(data)
+        n2f.load_minus_data(ndata)
+    f2, fl2, n2 = _reload_run(lib, single, load)
+    out = {"minus_flux": float(np.abs(fl2.flux()).max() / np.abs(flux1).max()),
+           "minus_far": float(np.abs(n2.farfield(x)).max() / np.abs(far1).max())}
+    sc = 2.0 + 1.0j
+    fl1.scale_dfts(sc)
+    n1.scale_dfts(sc)
+    out["scale_flux"] = float(np.abs(np.asarray(fl1.flux()) - abs(sc) ** 2 * flux1).max() / np.abs(flux1).max() / 5)
+    out["scale_far"] = float(np.abs(n1.farfield(x) - sc * far1).max() / np.abs(far1).max())
+    return out

@@ -67,3 +67,8 @@ def test_amplitude_array_source_gpu(gpu_lib):
 
 def test_srcdata_and_boundary_fix_gpu(gpu_lib):
     T.check_srcdata(gpu_lib)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_flux_data_reload_and_scale_gpu(gpu_lib, single):
+    T.check_dft_reload(gpu_lib, single)

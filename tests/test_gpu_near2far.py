@@ -61,6 +61,7 @@ def _parity_run(lib, single):
     for c in (Ex, Ey, Ez, Hx, Hy, Hz):
         out[f"dft_{c}"] = n2f.get_dft_array(c, 1)
     out["eps_mu"] = (n2f.eps, n2f.mu)
+    out["data"] = np.concatenate([d.reshape(-1) for d in n2f.get_data()])
     return out
 
 

@@ -45,6 +45,9 @@ values do not reach (tests/closed_forms.py states each closed form):
   near box's DFT flux (1-6 % at 12 points per unit; the reference's own test allows
   0.05 * 30 / resolution); and the periodic copies of a surface spanning a Bloch-periodic
   cell against the phased sum of shifted single-period transforms (observed about 4e-16).
+* Flux and near2far data: loading a run's accumulators negated into an identical run before
+  it steps leaves a zero flux and transform, and scale_dfts(s) multiplies a flux by |s|^2 and
+  a transform by s (roundoff).
 * Transport and spectrum shape: equal flux through two planes of a lossless 1D line, and a
   flux spectrum proportional to |J(omega)|^2.  These two are scale-invariant: they do not
   pin the DFT's normalisation, phase or E/H time offset (the first three checks do).
@@ -65,7 +68,8 @@ from meep_amd.capi import (Ex, Ez, Hz, MeepError, Z, fields, gaussian_src_time, 
 TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
        "pole_decay": 1e-4, "cavity": 1e-10, "cond_decay": 1e-12, "cond_pml_line": 1e-6,
        "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12, "pml_refl": 0.03, "pml_stretch": 0.08, "sources": 1e-13,
-       "n2f_green": 1e-12, "n2f_fields": 0.05 * 30 / 12, "n2f_flux": 0.08, "n2f_periodic": 1e-12}
+       "n2f_green": 1e-12, "n2f_fields": 0.05 * 30 / 12, "n2f_flux": 0.08, "n2f_periodic": 1e-12,
+       "reload": 1e-12, "reload_f32": 1e-6}
 
 
 def check_line_source(lib):
@@ -141,6 +145,11 @@ def check_source_array(lib):
 def check_srcdata(lib):
     err = CF.srcdata_moves(lib)
     assert max(err.values()) <= TOL["sources"], err
+
+
+def check_dft_reload(lib, single=False):
+    err = CF.dft_data_reload(lib, single)
+    assert max(err.values()) <= TOL["reload_f32" if single else "reload"], err
 
 
 def check_n2f_green(lib):
@@ -324,3 +333,8 @@ def test_near2far_arrays_and_errors(oracle_lib):
         np.testing.assert_array_equal(EH[:, 1, q, :], v.imag.T)
     with pytest.raises(MeepError, match="homogeneous"):
         f.add_dft_near2far([((1.5, 2.5, 0), (2.5, 2.5, 0), 1.0), ((3.7, 1.5, 0), (3.7, 2.5, 0), 1.0)], [0.5])
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_flux_data_reload_and_scale(oracle_lib, single):
+    check_dft_reload(oracle_lib, single)
