@@ -432,7 +432,7 @@ template <class R> struct N2F : Flux<R> {
   int periodic_d[2] = {-1, -1}, periodic_n[2] = {0, 0};
   double periodic_k[2] = {0, 0}, period[2] = {0, 0};
   long built_t = -1;
-  unsigned built_version = 0;
+  unsigned built_version = 0, built_acc = 0;
   int nsrc = 0, cls_lo[7] = {};
   DevBuf sx, sy, sz, f0, d_freq;
 };
@@ -3552,8 +3552,10 @@ template <class R> struct Fields : FieldsBase {
   }
   // the accumulators of one chunk from host values (a load of data saved from the chunk's
   // get_dft), stored as the chunk keeps them; the compensation of compensated ones restarts
+  unsigned acc_version = 0; // accumulators written from the host (loads, scale_dfts)
   void write_dft(DftChunk<R> &ch, const std::vector<cdouble> &v) {
     HIPCHECK(hipStreamSynchronize(stream));
+    ++acc_version;
     if (ch.nsnap) fail("internal: deferred DFT steps pending");
     const size_t n = size_t(ch.npts) * ch.nfreq;
     if (v.size() != n) fail("internal: DFT data of the wrong size");
@@ -3784,7 +3786,7 @@ template <class R> struct Fields : FieldsBase {
   // and its periodic copies within periodic_n periods each way with the Bloch phase of their
   // offset folded into the amplitude; uploaded frequency-major.
   void n2f_sources(N2F<R> &n) {
-    if (n.built_t == t && n.built_version == dft_version && n.nsrc > 0) return;
+    if (n.built_t == t && n.built_version == dft_version && n.built_acc == acc_version && n.nsrc > 0) return;
     const int Nf = n.nfreq;
     std::vector<double> x, y, z;
     std::vector<int> cc;
@@ -3838,7 +3840,7 @@ template <class R> struct Fields : FieldsBase {
       put(n.f0, fm.data(), 8 * fm.size());
       put(n.d_freq, n.freq.data(), 8 * n.freq.size());
     }
-    n.built_t = t, n.built_version = dft_version;
+    n.built_t = t, n.built_version = dft_version, n.built_acc = acc_version;
   }
   // dft_near2far::farfield_lowlevel (src/near2far.cpp:350-397) at np points on the device: the
   // Green's function sums of near2far.hip, in batches of at most 2^20 (point, frequency) pairs
