@@ -50,6 +50,7 @@ typedef struct mp_dft_flux mp_dft_flux;
 typedef struct mp_dft_fields mp_dft_fields;
 typedef struct mp_flux_vol mp_flux_vol;
 typedef struct mp_near2far mp_near2far;
+typedef struct mp_dft_energy mp_dft_energy;
 typedef struct mp_adjoint mp_adjoint;
 typedef struct mp_batch mp_batch;
 
@@ -315,6 +316,15 @@ int mp_dft_flux_set_dft(mp_dft_flux *fl, int which, int i, const double *in);
 /* dft_flux::scale_dfts(complex<double>) — src/dft.cpp:597-600, dft_chunk::scale_dft
    :402-406 (load_minus_flux_data scales by -1, python/simulation.py:3651-3659). */
 int mp_dft_flux_scale_dfts(mp_dft_flux *fl, double re, double im);
+
+/* fields::add_dft_energy(const volume_list *, freq, Nfreq, decimation_factor) —
+   src/dft.cpp:725-750: volume w is [vmins + 3w, vmaxs + 3w]; E and H accumulate with
+   integration weights, D and B without, on the centred grid. */
+mp_dft_energy *mp_fields_add_dft_energy(mp_fields *f, int nvol, const double *vmins, const double *vmaxs,
+                                        const double *freq, int nfreq, int decimation_factor);
+void mp_dft_energy_delete(mp_dft_energy *e);
+/* dft_energy::total (which 0), electric (1), magnetic (2) — src/dft.cpp:681-723: out[Nfreq]. */
+int mp_dft_energy_get(mp_dft_energy *e, int which, double *out);
 
 /* ---- time-domain flux ----------------------------------------------------------------- */
 

@@ -465,6 +465,29 @@ public:
   volume_list *next;
 };
 
+// dft_energy (src/meep.hpp:1258-1290): electric / magnetic / total energy spectra, new[]
+// arrays of Nfreq values
+class dft_energy {
+public:
+  dft_energy(mp_dft_energy *h, int n) : h(h), n(n) {}
+  double *electric() { return get(1); }
+  double *magnetic() { return get(2); }
+  double *total() { return get(0); }
+  int Nfreq() const { return n; }
+
+private:
+  double *get(int which) {
+    double *out = new double[size_t(n)];
+    if (mp_dft_energy_get(h, which, out)) {
+      delete[] out;
+      throw std::runtime_error(mp_last_error());
+    }
+    return out;
+  }
+  mp_dft_energy *h;
+  int n;
+};
+
 // dft_near2far (src/meep.hpp:1331-1395): owned by its fields as the other monitors are
 class dft_near2far {
 public:
@@ -557,6 +580,7 @@ public:
     for (mp_dft_flux *h : fluxes) mp_dft_flux_delete(h);
     for (mp_dft_fields *h : dft_fields_list) mp_dft_fields_delete(h);
     for (mp_near2far *h : near2far_list) mp_near2far_delete(h);
+    for (mp_dft_energy *h : energy_list) mp_dft_energy_delete(h);
     mp_fields_delete(p);
   }
   fields(const fields &) = delete;
@@ -679,6 +703,25 @@ public:
   std::complex<realnum> *get_dft_array(dft_near2far n2f, component c, int num_freq, int *rank, size_t dims[3]) {
     return get_dft_array_of(mp_near2far_get_dft_array, n2f.handle(), c, num_freq, rank, dims);
   }
+  // add_dft_energy(where, freq, Nfreq, decimation_factor) (src/dft.cpp:725-750)
+  dft_energy add_dft_energy(const volume_list *where, const double *freq, size_t Nfreq, int decimation_factor = 0) {
+    std::vector<double> mins, maxs;
+    for (const volume_list *w = where; w; w = w->next) {
+      const vec lo = w->v.get_min_corner(), hi = w->v.get_max_corner();
+      mins.insert(mins.end(), lo.data(), lo.data() + 3);
+      maxs.insert(maxs.end(), hi.data(), hi.data() + 3);
+    }
+    mp_dft_energy *h = mp_fields_add_dft_energy(p, int(mins.size() / 3), mins.data(), maxs.data(), freq, int(Nfreq),
+                                                decimation_factor);
+    if (!h) throw std::runtime_error(mp_last_error());
+    energy_list.push_back(h);
+    return dft_energy(h, int(Nfreq));
+  }
+  dft_energy add_dft_energy(const volume_list *where, double freq_min, double freq_max, int Nfreq,
+                            int decimation_factor = 0) {
+    const std::vector<double> fr = linspace(freq_min, freq_max, Nfreq);
+    return add_dft_energy(where, fr.data(), fr.size(), decimation_factor);
+  }
   // add_dft_near2far(where, freq, Nfreq, decimation_factor, Nperiods) (src/meep.hpp:2161-2168):
   // each surface's normal is the direction of its component (Sx, Sy, Sz or a field component),
   // else the volume's normal
@@ -761,6 +804,7 @@ private:
   std::vector<mp_dft_flux *> fluxes;
   std::vector<mp_dft_fields *> dft_fields_list;
   std::vector<mp_near2far *> near2far_list;
+  std::vector<mp_dft_energy *> energy_list;
   std::vector<flux_vol *> flux_vols;
 };
 

@@ -175,6 +175,10 @@ _SIGNATURES = {
     "mp_near2far_get_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "mp_near2far_set_dft": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "mp_near2far_scale_dfts": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "mp_fields_add_dft_energy": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double), C.c_int, C.c_int]),
+    "mp_dft_energy_delete": (None, [C.c_void_p]),
+    "mp_dft_energy_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "mp_near2far_green": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
                                     C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
@@ -449,6 +453,33 @@ class flux_vol:
         if getattr(self, "ptr", None):
             self.lib.mp_flux_vol_delete(self.ptr)
             self.ptr = None
+
+
+class dft_energy:
+    """meep::dft_energy (src/meep.hpp:1258-1290): energy-density spectra in volumes."""
+
+    def __init__(self, fields, ptr, nfreq):
+        self.f = fields
+        self.lib = fields.lib
+        self.ptr = ptr
+        self.nfreq = nfreq
+
+    def _get(self, which):
+        out = np.empty(self.nfreq)
+        _check(self.lib, self.lib.mp_dft_energy_get(self.ptr, which, _dptr(out)))
+        return out
+
+    def total(self):
+        """dft_energy::total (src/dft.cpp:713-723)."""
+        return self._get(0)
+
+    def electric(self):
+        """dft_energy::electric (src/dft.cpp:681-695)."""
+        return self._get(1)
+
+    def magnetic(self):
+        """dft_energy::magnetic (src/dft.cpp:697-711)."""
+        return self._get(2)
 
 
 class near2far:
@@ -842,6 +873,20 @@ class fields:
         self._fluxes.append(df)
         return df
 
+    def add_dft_energy(self, volumes, freqs, decimation_factor=0):
+        """fields::add_dft_energy(volume_list, freq, Nfreq, decimation_factor)
+        (src/dft.cpp:725-750); volumes: (vmin, vmax) pairs."""
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        vmins = np.ascontiguousarray([[float(v) for v in (list(lo) + [0.0] * 3)[:3]] for lo, _ in volumes])
+        vmaxs = np.ascontiguousarray([[float(v) for v in (list(hi) + [0.0] * 3)[:3]] for _, hi in volumes])
+        ptr = self.lib.mp_fields_add_dft_energy(self.ptr, len(volumes), _dptr(vmins), _dptr(vmaxs), _dptr(freqs),
+                                                len(freqs), decimation_factor)
+        if not ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        en = dft_energy(self, ptr, len(freqs))
+        self._fluxes.append(en)
+        return en
+
     def add_dft_near2far(self, surfaces, freqs, decimation_factor=0, nperiods=1):
         """fields::add_dft_near2far(volume_list, freq, Nfreq, decimation_factor, Nperiods)
         (src/near2far.cpp:569-654).  surfaces: (vmin, vmax, weight) or (vmin, vmax, weight,
@@ -868,6 +913,8 @@ class fields:
                 if fl.ptr:
                     if isinstance(fl, near2far):
                         self.lib.mp_near2far_delete(fl.ptr)
+                    elif isinstance(fl, dft_energy):
+                        self.lib.mp_dft_energy_delete(fl.ptr)
                     elif isinstance(fl, dft_fields):
                         self.lib.mp_dft_fields_delete(fl.ptr)
                     else:

@@ -385,6 +385,10 @@ struct FieldsBase {
   virtual void *add_dft_near2far(int nsurf, const int *dirs, const double *vmins, const double *vmaxs, const double *weights,
                                  const std::vector<double> &freq, int dec, int nperiods, int *dec_out) = 0;
   virtual void n2f_info(void *n, double *eps_mu, int *periodic, long *npoints) = 0;
+  // dft_energy: the volumes, and the total (which 0), electric (1) or magnetic (2) spectrum
+  virtual void *add_dft_energy(int nvol, const double *vmins, const double *vmaxs, const std::vector<double> &freq, int dec,
+                               int *dec_out) = 0;
+  virtual void energy_spectrum(void *e, int which, double *out) = 0;
   virtual void n2f_eval(void *n, const double *pts, long np, double *out) = 0;
   // time-domain flux: flux_vol planes, flux_in_box (sync) and flux_in_box_wrongH
   virtual void *add_flux_vol(int d, V3 mn, V3 mx) = 0;
@@ -426,6 +430,10 @@ template <class R> struct Flux {
 // dft_near2far (src/meep.hpp:1455-1520): the accumulators of the tangential fields (the E list)
 // with the medium and the periodic copies, and the equivalent-current list the far-field
 // kernel reads, kept on the device until the fields step again
+// dft_energy (src/dft.cpp:654-750): the E and H lists of the base, with D and B beside them
+template <class R> struct Energy : Flux<R> {
+  std::vector<DftChunk<R> *> D, B;
+};
 template <class R> struct N2F : Flux<R> {
   std::vector<double> freq;
   double eps = 0, mu = 0;
@@ -3481,6 +3489,59 @@ template <class R> struct Fields : FieldsBase {
     fluxes.push_back(std::move(fl));
     return fluxes.back().get();
   }
+  // fields::add_dft_energy (src/dft.cpp:725-750): for every volume and field direction, E and
+  // H with integration weights and D and B without, centred grid, each call's chunks in front
+  void *add_dft_energy(int nvol, const double *vmins, const double *vmaxs, const std::vector<double> &freq, int dec,
+                       int *dec_out) override {
+    if (!components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+    if (nvol < 1) fail("add_dft_energy: no volumes");
+    const int decim = dec > 0 ? dec : auto_decimation(freq);
+    *dec_out = decim;
+    std::unique_ptr<Energy<R>> en(new Energy<R>);
+    en->nfreq = int(freq.size());
+    for (int d = 0; d < 3; ++d) en->wmin.t[d] = vmins[d], en->wmax.t[d] = vmaxs[d];
+    std::vector<DftChunk<R> *> *lists[4] = {&en->E, &en->D, &en->H, &en->B};
+    const int base[4] = {0, 10, 5, 15};
+    for (int w = 0; w < nvol; ++w) {
+      V3 mn, mx;
+      for (int d = 0; d < 3; ++d) mn.t[d] = vmins[3 * w + d], mx.t[d] = vmaxs[3 * w + d];
+      for (int d = 0; d < 3; ++d)
+        for (int q = 0; q < 4; ++q) {
+          std::vector<DftChunk<R> *> lst;
+          add_dft(lst, dir_comp(base[q], d), mn, mx, freq, q == 0 || q == 2, cdouble(1.0, 0), decim, true);
+          lists[q]->insert(lists[q]->begin(), lst.rbegin(), lst.rend());
+        }
+    }
+    fluxes.push_back(std::move(en));
+    return fluxes.back().get();
+  }
+  // dft_energy::electric / magnetic / total (src/dft.cpp:681-723): 0.5 Re(conj(F) G) over the
+  // paired chunks, the product in the accumulator's type
+  void energy_spectrum(void *ep, int which, double *out) override {
+    Energy<R> *en = static_cast<Energy<R> *>(ep);
+    const int Nf = en->nfreq;
+    std::vector<double> e(Nf, 0.0), m(Nf, 0.0);
+    auto part = [&](const std::vector<DftChunk<R> *> &A, const std::vector<DftChunk<R> *> &B, std::vector<double> &acc) {
+      for (size_t q = 0; q < A.size() && q < B.size(); ++q) {
+        if (A[q]->npts != B[q]->npts) fail("internal: dft_energy chunks of different sizes");
+        const std::vector<cdouble> a = read_dft(*A[q]), b = read_dft(*B[q]);
+        const bool narrow = A[q]->acc_kind == MP_DFT_REALNUM;
+        for (size_t k = 0; k < size_t(A[q]->npts); ++k)
+          for (int i = 0; i < Nf; ++i) {
+            const size_t o = k * Nf + i;
+            if (narrow) {
+              const std::complex<R> aa(R(a[o].real()), R(a[o].imag())), bb(R(b[o].real()), R(b[o].imag()));
+              acc[i] += 0.5 * double(std::real(std::conj(aa) * bb));
+            }
+            else
+              acc[i] += 0.5 * std::real(std::conj(a[o]) * b[o]);
+          }
+      }
+    };
+    part(en->E, en->D, e);
+    part(en->H, en->B, m);
+    for (int i = 0; i < Nf; ++i) out[i] = which == 1 ? e[i] : which == 2 ? m[i] : e[i] + m[i];
+  }
   int flux_chunk_component(void *flp, int which, int i) override {
     Flux<R> *fl = static_cast<Flux<R> *>(flp);
     auto &lst = which ? fl->H : fl->E;
@@ -3903,7 +3964,9 @@ template <class R> struct Fields : FieldsBase {
     for (size_t q = 0; q < fluxes.size(); ++q)
       if (fluxes[q].get() == flp) {
         Flux<R> *fl = fluxes[q].get();
-        for (auto *lst : {&fl->E, &fl->H})
+        std::vector<std::vector<DftChunk<R> *> *> all{&fl->E, &fl->H};
+        if (Energy<R> *en = dynamic_cast<Energy<R> *>(fl)) all.push_back(&en->D), all.push_back(&en->B);
+        for (auto *lst : all)
           for (DftChunk<R> *ch : *lst)
             for (size_t z = 0; z < dft_chunks.size(); ++z)
               if (dft_chunks[z].get() == ch) {
@@ -5063,6 +5126,29 @@ int mp_near2far_flux(mp_near2far *n, int d, const double *wmin, const double *wm
 }
 long mp_near2far_get_dft_array(mp_near2far *n, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(n, false, c, num_freq, rank, dims, out);
+}
+mp_dft_energy *mp_fields_add_dft_energy(mp_fields *f, int nvol, const double *vmins, const double *vmaxs, const double *freq,
+                                        int nfreq, int dec) {
+  FluxHandle *h = new FluxHandle;
+  int rc = guard([&] {
+    std::vector<double> fr(freq, freq + nfreq);
+    h->owner = F_;
+    h->nfreq = nfreq;
+    h->fl = F_->add_dft_energy(nvol, vmins, vmaxs, fr, dec, &h->decimation);
+  });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_dft_energy *>(h);
+}
+void mp_dft_energy_delete(mp_dft_energy *e) { mp_dft_flux_delete(reinterpret_cast<mp_dft_flux *>(e)); }
+int mp_dft_energy_get(mp_dft_energy *e, int which, double *out) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(e);
+  return guard([&] {
+    if (which < 0 || which > 2) fail("dft_energy: which must be 0 (total), 1 (electric) or 2 (magnetic)");
+    h->owner->energy_spectrum(h->fl, which, out);
+  });
 }
 int mp_near2far_num_chunks(mp_near2far *n) { return mp_dft_flux_num_chunks(reinterpret_cast<mp_dft_flux *>(n), 0); }
 long mp_near2far_chunk_points(mp_near2far *n, int i) {

@@ -71,6 +71,7 @@
  *   MUT_N2F_PHASE     near2far's periodic copies add without their Bloch phase
  *   MUT_HPOLE_SUB     a magnetic susceptibility's P is not subtracted from B (H = mu^-1 B)
  *   MUT_SCALE_DFTS    scale_dfts scales a flux's E accumulators only
+ *   MUT_DFTENERGY_WEIGHT  dft_energy weights D and B as well as E and H (volume counted twice)
  */
 #include "../include/meep_amd.h"
 
@@ -3310,4 +3311,91 @@ int mp_fields_field_energy(mp_fields *f, double *out) {
   });
 }
 
+
+// ---- dft_energy (src/dft.cpp:654-750) ------------------------------------------------------
+struct AnyEnergy : AnyFlux {
+  std::vector<void *> D, B;
+};
+// fields::add_dft_energy (src/dft.cpp:725-750): for every volume and every field direction of
+// the grid, E and H with integration weights and D and B without, on the centred grid, each
+// call's chunks in front of its list
+mp_dft_energy *mp_fields_add_dft_energy(mp_fields *f, int nvol, const double *vmins, const double *vmaxs, const double *freq,
+                                        int nfreq, int dec) {
+  AnyFields *af = reinterpret_cast<AnyFields *>(f);
+  AnyEnergy *en = new AnyEnergy;
+  en->owner = af;
+  en->freq.assign(freq, freq + nfreq);
+  int rc = guard([&] {
+    with_fields(af, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      if (!F.components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+      if (nvol < 1) fail("add_dft_energy: no volumes");
+      const int decim = dec > 0 ? dec : F.auto_decimation(en->freq);
+      en->decimation = decim;
+      en->wmin = to_vec(vmins);
+      en->wmax = to_vec(vmaxs);
+      std::vector<void *> *lists[4] = {&en->E, &en->D, &en->H, &en->B};
+      const int base[4] = {0, 10, 5, 15};
+      for (int w = 0; w < nvol; ++w) {
+        const vec3 mn = to_vec(vmins + 3 * w), mx = to_vec(vmaxs + 3 * w);
+        // LOOP_OVER_FIELD_DIRECTIONS (src/meep/vec.hpp:147-149): X, Y, Z; absent components add
+        // no chunks
+        for (int d = 0; d < 3; ++d) {
+          for (int q = 0; q < 4; ++q) {
+            const int c = direction_component(base[q], d);
+            std::vector<std::unique_ptr<DftChunk<R>>> lst;
+#ifdef MUT_DFTENERGY_WEIGHT
+            F.add_dft(lst, c, mn, mx, en->freq, true, cdouble(1.0, 0), decim, true);
+#else
+            F.add_dft(lst, c, mn, mx, en->freq, q == 0 || q == 2, cdouble(1.0, 0), decim, true);
+#endif
+            std::vector<void *> mine;
+            for (int j = int(lst.size()) - 1; j >= 0; --j) {
+              mine.push_back(lst[j].get());
+              F.dft_chunks.push_back(std::move(lst[j]));
+            }
+            lists[q]->insert(lists[q]->begin(), mine.begin(), mine.end());
+          }
+        }
+      }
+      return 0;
+    });
+  });
+  if (rc) {
+    delete en;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_dft_energy *>(en);
+}
+void mp_dft_energy_delete(mp_dft_energy *e) { delete reinterpret_cast<AnyEnergy *>(e); }
+// dft_energy::electric / magnetic / total (src/dft.cpp:681-723): 0.5 Re(conj(F) G) summed over
+// the paired chunks of the E and D (H and B) lists, in the accumulator's type
+int mp_dft_energy_get(mp_dft_energy *ep, int which, double *out) {
+  AnyEnergy *en = reinterpret_cast<AnyEnergy *>(ep);
+  return guard([&] {
+    if (which < 0 || which > 2) fail("dft_energy: which must be 0 (total), 1 (electric) or 2 (magnetic)");
+    with_fields(en->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      const size_t Nf = en->freq.size();
+      std::vector<double> e(Nf, 0.0), m(Nf, 0.0);
+      auto part = [&](const std::vector<void *> &A, const std::vector<void *> &B, std::vector<double> &acc) {
+        for (size_t q = 0; q < A.size() && q < B.size(); ++q) {
+          auto *a = reinterpret_cast<DftChunk<R> *>(A[q]);
+          auto *b = reinterpret_cast<DftChunk<R> *>(B[q]);
+          for (size_t k = 0; k < a->N; ++k)
+            for (size_t i = 0; i < Nf; ++i) {
+              if (a->acc == MP_DFT_F64)
+                acc[i] += 0.5 * std::real(std::conj(a->dft64[k * Nf + i]) * b->dft64[k * Nf + i]);
+              else
+                acc[i] += 0.5 * double(std::real(std::conj(a->dft[k * Nf + i]) * b->dft[k * Nf + i]));
+            }
+        }
+      };
+      part(en->E, en->D, e);
+      part(en->H, en->B, m);
+      for (size_t i = 0; i < Nf; ++i) out[i] = which == 1 ? e[i] : which == 2 ? m[i] : e[i] + m[i];
+      return 0;
+    });
+  });
+}
 } // extern "C"

@@ -84,6 +84,10 @@ run against a value the mathematics fixes; the tests assert on these errors.
   near2far transform are zero up to roundoff, since its accumulators are the first run's
   minus themselves; and scale_dfts(s) multiplies a flux by |s|^2 (E and H both scaled) and a
   transform by s.
+* ``dft_energy_plane_wave``: dft_energy in a 1D vacuum line: a wave travelling away from a
+  current sheet carries the same electric and magnetic DFT energy, and the total over a
+  stretch of length L equals L times the DFT flux through a plane (|E| = |H| and S = u in the
+  continuum), up to the lattice's O(dx^2) dispersion, checked at two resolutions.
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -716,3 +720,21 @@ def dft_data_reload(lib, single=False):
     out["scale_flux"] = float(np.abs(np.asarray(fl1.flux()) - abs(sc) ** 2 * flux1).max() / np.abs(flux1).max() / 5)
     out["scale_far"] = float(np.abs(n1.farfield(x) - sc * far1).max() / np.abs(far1).max())
     return out
+
+
+def dft_energy_plane_wave(lib, a=20, single=False, L=1.5, spectra=False):
+    """Relative departures of a travelling wave's DFT energies from equipartition and from L
+    times its DFT flux (module docstring), over five frequencies."""
+    s = structure(lib, dim=1, size=(0, 0, 8.0), a=a, pml=[(Z, side, 1.0) for side in (High, Low)], single=single)
+    f = fields(s)
+    freqs = np.linspace(0.3, 0.5, 5)
+    f.add_volume_source(Ex, gaussian_src_time(0.4, 0.3), (0, 0, 2.0), (0, 0, 2.0))
+    fl = f.add_dft_flux(Z, (0, 0, 3.5), (0, 0, 3.5), freqs, decimation_factor=1)
+    en = f.add_dft_energy([((0, 0, 3.5), (0, 0, 3.5 + L))], freqs, decimation_factor=1)
+    f.step_n(int(round(60.0 / s.dt)))
+    flux, e, m, t = np.asarray(fl.flux()), en.electric(), en.magnetic(), en.total()
+    if spectra:
+        return e, m, t
+    return {"equipartition": float(np.abs(e - m).max() / np.abs(t).max()),
+            "energy_flux": float(np.abs(t - L * flux).max() / np.abs(t).max()),
+            "total": float(np.abs(t - e - m).max() / np.abs(t).max())}

@@ -4,7 +4,10 @@ flux weights, Lorentz and Drude poles, magnetic/metallic walls, conductivity, th
 arrays of get_dft_array and the time-domain flux and energy integrals are held to
 the mathematics directly, not only to the oracle."""
 
+import numpy as np
 import pytest
+
+import closed_forms as CF
 
 import test_oracle_closed_forms as T
 
@@ -72,3 +75,17 @@ def test_srcdata_and_boundary_fix_gpu(gpu_lib):
 @pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
 def test_flux_data_reload_and_scale_gpu(gpu_lib, single):
     T.check_dft_reload(gpu_lib, single)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_dft_energy_of_a_travelling_wave_gpu(gpu_lib, single):
+    T.check_dft_energy(gpu_lib, single)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_dft_energy_matches_oracle_gpu(gpu_lib, oracle_lib, single):
+    got = CF.dft_energy_plane_wave(gpu_lib, 20, single, spectra=True)
+    want = CF.dft_energy_plane_wave(oracle_lib, 20, single, spectra=True)
+    tol = 1e-4 if single else 1e-10
+    for g, w in zip(got, want):
+        assert np.abs(g - w).max() <= tol * np.abs(w).max()

@@ -45,6 +45,9 @@ values do not reach (tests/closed_forms.py states each closed form):
   near box's DFT flux (1-6 % at 12 points per unit; the reference's own test allows
   0.05 * 30 / resolution); and the periodic copies of a surface spanning a Bloch-periodic
   cell against the phased sum of shifted single-period transforms (observed about 4e-16).
+* dft_energy of a travelling wave in a 1D line: electric equals magnetic and the total over a
+  length L equals L times the DFT flux, to the lattice's O(dx^2) (2e-3 at 20 points per
+  unit, falling 4x at 40).
 * Flux and near2far data: loading a run's accumulators negated into an identical run before
   it steps leaves a zero flux and transform, and scale_dfts(s) multiplies a flux by |s|^2 and
   a transform by s (roundoff).
@@ -69,7 +72,8 @@ TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_fre
        "pole_decay": 1e-4, "cavity": 1e-10, "cond_decay": 1e-12, "cond_pml_line": 1e-6,
        "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12, "pml_refl": 0.03, "pml_stretch": 0.08, "sources": 1e-13,
        "n2f_green": 1e-12, "n2f_fields": 0.05 * 30 / 12, "n2f_flux": 0.08, "n2f_periodic": 1e-12,
-       "reload": 1e-12, "reload_f32": 1e-6}
+       "reload": 1e-12, "reload_f32": 1e-6,
+       "energy_equip": 4e-3, "energy_flux": 1e-4}
 
 
 def check_line_source(lib):
@@ -150,6 +154,12 @@ def check_srcdata(lib):
 def check_dft_reload(lib, single=False):
     err = CF.dft_data_reload(lib, single)
     assert max(err.values()) <= TOL["reload_f32" if single else "reload"], err
+
+
+def check_dft_energy(lib, single=False):
+    err = CF.dft_energy_plane_wave(lib, 20, single)
+    assert err["equipartition"] <= TOL["energy_equip"], err
+    assert err["energy_flux"] <= TOL["energy_flux"] and err["total"] <= 1e-6, err
 
 
 def check_n2f_green(lib):
@@ -338,3 +348,14 @@ def test_near2far_arrays_and_errors(oracle_lib):
 @pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
 def test_flux_data_reload_and_scale(oracle_lib, single):
     check_dft_reload(oracle_lib, single)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_dft_energy_of_a_travelling_wave(oracle_lib, single):
+    check_dft_energy(oracle_lib, single)
+
+
+def test_dft_energy_equipartition_converges_as_dx_squared(oracle_lib):
+    e20 = CF.dft_energy_plane_wave(oracle_lib, 20)["equipartition"]
+    e40 = CF.dft_energy_plane_wave(oracle_lib, 40)["equipartition"]
+    assert 3.0 <= e20 / e40 <= 5.0, (e20, e40)

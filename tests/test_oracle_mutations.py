@@ -53,6 +53,7 @@ CHECKS = {
     "N2F_PHASE": T.check_n2f_periodic,
     "HPOLE_SUB": lambda lib: T.check_pole(lib, False, magnetic=True),
     "SCALE_DFTS": T.check_dft_reload,
+    "DFTENERGY_WEIGHT": T.check_dft_energy,
 }
 
 
