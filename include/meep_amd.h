@@ -51,6 +51,7 @@ typedef struct mp_dft_fields mp_dft_fields;
 typedef struct mp_flux_vol mp_flux_vol;
 typedef struct mp_near2far mp_near2far;
 typedef struct mp_dft_energy mp_dft_energy;
+typedef struct mp_dft_force mp_dft_force;
 typedef struct mp_adjoint mp_adjoint;
 typedef struct mp_batch mp_batch;
 
@@ -325,6 +326,19 @@ mp_dft_energy *mp_fields_add_dft_energy(mp_fields *f, int nvol, const double *vm
 void mp_dft_energy_delete(mp_dft_energy *e);
 /* dft_energy::total (which 0), electric (1), magnetic (2) — src/dft.cpp:681-723: out[Nfreq]. */
 int mp_dft_energy_get(mp_dft_energy *e, int which, double *out);
+
+/* fields::add_dft_force(const volume_list *, freq, Nfreq, decimation_factor) —
+   src/stress.cpp:153-198: volume w is [vmins + 3w, vmaxs + 3w] with complex weight
+   (weights[2w], weights[2w + 1]); comps[w] names the force direction (an E or D component, or
+   Sx = 100, Sy = 101, Sz = 104); the normal is the volume's.  Maxwell stress-tensor terms:
+   off-diagonal E/H pairs on the centred grid, diagonal ones with square-root weights on the
+   components' own grids. */
+mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, const double *vmins,
+                                      const double *vmaxs, const double *weights, const double *freq,
+                                      int nfreq, int decimation_factor);
+void mp_dft_force_delete(mp_dft_force *fo);
+/* dft_force::force() — src/stress.cpp:101-114: out[Nfreq]. */
+int mp_dft_force_force(mp_dft_force *fo, double *out);
 
 /* ---- time-domain flux ----------------------------------------------------------------- */
 

@@ -488,6 +488,25 @@ private:
   int n;
 };
 
+// dft_force (src/meep.hpp:1292-1320): force() is a new[] array of Nfreq values
+class dft_force {
+public:
+  dft_force(mp_dft_force *h, int n) : h(h), n(n) {}
+  double *force() {
+    double *out = new double[size_t(n)];
+    if (mp_dft_force_force(h, out)) {
+      delete[] out;
+      throw std::runtime_error(mp_last_error());
+    }
+    return out;
+  }
+  int Nfreq() const { return n; }
+
+private:
+  mp_dft_force *h;
+  int n;
+};
+
 // dft_near2far (src/meep.hpp:1331-1395): owned by its fields as the other monitors are
 class dft_near2far {
 public:
@@ -581,6 +600,7 @@ public:
     for (mp_dft_fields *h : dft_fields_list) mp_dft_fields_delete(h);
     for (mp_near2far *h : near2far_list) mp_near2far_delete(h);
     for (mp_dft_energy *h : energy_list) mp_dft_energy_delete(h);
+    for (mp_dft_force *h : force_list) mp_dft_force_delete(h);
     mp_fields_delete(p);
   }
   fields(const fields &) = delete;
@@ -703,6 +723,29 @@ public:
   std::complex<realnum> *get_dft_array(dft_near2far n2f, component c, int num_freq, int *rank, size_t dims[3]) {
     return get_dft_array_of(mp_near2far_get_dft_array, n2f.handle(), c, num_freq, rank, dims);
   }
+  // add_dft_force(where, freq, Nfreq, decimation_factor) (src/stress.cpp:153-198): each
+  // volume's component names the force direction (Sx, Sy, Sz or an E component)
+  dft_force add_dft_force(const volume_list *where, const double *freq, size_t Nfreq, int decimation_factor = 0) {
+    std::vector<int> comps;
+    std::vector<double> mins, maxs, weights;
+    for (const volume_list *w = where; w; w = w->next) {
+      const vec lo = w->v.get_min_corner(), hi = w->v.get_max_corner();
+      comps.push_back(w->c);
+      mins.insert(mins.end(), lo.data(), lo.data() + 3);
+      maxs.insert(maxs.end(), hi.data(), hi.data() + 3);
+      weights.push_back(w->weight.real());
+      weights.push_back(w->weight.imag());
+    }
+    mp_dft_force *h = mp_fields_add_dft_force(p, int(comps.size()), comps.data(), mins.data(), maxs.data(), weights.data(),
+                                              freq, int(Nfreq), decimation_factor);
+    if (!h) throw std::runtime_error(mp_last_error());
+    force_list.push_back(h);
+    return dft_force(h, int(Nfreq));
+  }
+  dft_force add_dft_force(const volume_list *where, double freq_min, double freq_max, int Nfreq, int decimation_factor = 0) {
+    const std::vector<double> fr = linspace(freq_min, freq_max, Nfreq);
+    return add_dft_force(where, fr.data(), fr.size(), decimation_factor);
+  }
   // add_dft_energy(where, freq, Nfreq, decimation_factor) (src/dft.cpp:725-750)
   dft_energy add_dft_energy(const volume_list *where, const double *freq, size_t Nfreq, int decimation_factor = 0) {
     std::vector<double> mins, maxs;
@@ -805,6 +848,7 @@ private:
   std::vector<mp_dft_fields *> dft_fields_list;
   std::vector<mp_near2far *> near2far_list;
   std::vector<mp_dft_energy *> energy_list;
+  std::vector<mp_dft_force *> force_list;
   std::vector<flux_vol *> flux_vols;
 };
 

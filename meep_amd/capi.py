@@ -179,6 +179,11 @@ _SIGNATURES = {
                                               C.POINTER(C.c_double), C.c_int, C.c_int]),
     "mp_dft_energy_delete": (None, [C.c_void_p]),
     "mp_dft_energy_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mp_fields_add_dft_force": (C.c_void_p, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.c_int, C.c_int]),
+    "mp_dft_force_delete": (None, [C.c_void_p]),
+    "mp_dft_force_force": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "mp_near2far_green": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
                                     C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
@@ -480,6 +485,22 @@ class dft_energy:
     def magnetic(self):
         """dft_energy::magnetic (src/dft.cpp:697-711)."""
         return self._get(2)
+
+
+class dft_force:
+    """meep::dft_force (src/meep.hpp:1292-1320): the Maxwell-stress force spectrum."""
+
+    def __init__(self, fields, ptr, nfreq):
+        self.f = fields
+        self.lib = fields.lib
+        self.ptr = ptr
+        self.nfreq = nfreq
+
+    def force(self):
+        """dft_force::force (src/stress.cpp:101-114)."""
+        out = np.empty(self.nfreq)
+        _check(self.lib, self.lib.mp_dft_force_force(self.ptr, _dptr(out)))
+        return out
 
 
 class near2far:
@@ -887,6 +908,25 @@ class fields:
         self._fluxes.append(en)
         return en
 
+    def add_dft_force(self, volumes, freqs, decimation_factor=0):
+        """fields::add_dft_force(volume_list, freq, Nfreq, decimation_factor)
+        (src/stress.cpp:153-198); volumes: (vmin, vmax, force component[, weight]) with the
+        component an E component or Sx = 100, Sy = 101, Sz = 104."""
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        n = len(volumes)
+        comps = (C.c_int * n)(*[int(v[2]) for v in volumes])
+        vmins = np.ascontiguousarray([[float(x) for x in (list(v[0]) + [0.0] * 3)[:3]] for v in volumes])
+        vmaxs = np.ascontiguousarray([[float(x) for x in (list(v[1]) + [0.0] * 3)[:3]] for v in volumes])
+        w = np.ascontiguousarray([[complex(v[3] if len(v) > 3 else 1.0).real, complex(v[3] if len(v) > 3 else 1.0).imag]
+                                  for v in volumes])
+        ptr = self.lib.mp_fields_add_dft_force(self.ptr, n, comps, _dptr(vmins), _dptr(vmaxs), _dptr(w), _dptr(freqs),
+                                               len(freqs), decimation_factor)
+        if not ptr:
+            raise MeepError(self.lib.mp_last_error().decode())
+        fo = dft_force(self, ptr, len(freqs))
+        self._fluxes.append(fo)
+        return fo
+
     def add_dft_near2far(self, surfaces, freqs, decimation_factor=0, nperiods=1):
         """fields::add_dft_near2far(volume_list, freq, Nfreq, decimation_factor, Nperiods)
         (src/near2far.cpp:569-654).  surfaces: (vmin, vmax, weight) or (vmin, vmax, weight,
@@ -915,6 +955,8 @@ class fields:
                         self.lib.mp_near2far_delete(fl.ptr)
                     elif isinstance(fl, dft_energy):
                         self.lib.mp_dft_energy_delete(fl.ptr)
+                    elif isinstance(fl, dft_force):
+                        self.lib.mp_dft_force_delete(fl.ptr)
                     elif isinstance(fl, dft_fields):
                         self.lib.mp_dft_fields_delete(fl.ptr)
                     else:

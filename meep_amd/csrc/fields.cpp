@@ -389,6 +389,10 @@ struct FieldsBase {
   virtual void *add_dft_energy(int nvol, const double *vmins, const double *vmaxs, const std::vector<double> &freq, int dec,
                                int *dec_out) = 0;
   virtual void energy_spectrum(void *e, int which, double *out) = 0;
+  // dft_force: the volumes (force direction class per volume, complex weights) and the force
+  virtual void *add_dft_force(int nvol, const int *fdirs, const double *vmins, const double *vmaxs, const double *weights,
+                              const std::vector<double> &freq, int dec, int *dec_out) = 0;
+  virtual void force_spectrum(void *fo, double *out) = 0;
   virtual void n2f_eval(void *n, const double *pts, long np, double *out) = 0;
   // time-domain flux: flux_vol planes, flux_in_box (sync) and flux_in_box_wrongH
   virtual void *add_flux_vol(int d, V3 mn, V3 mx) = 0;
@@ -420,6 +424,7 @@ template <class R> struct DftChunk {
   cdouble stored_weight = 1.0;
   bool empty[3] = {};
   int vc = -1; // near2far: the equivalent current's component
+  cdouble extra_weight = 1.0; // dft_force: the readout's factor (src/dft.cpp:97,224)
 };
 template <class R> struct Flux {
   virtual ~Flux() = default;
@@ -430,6 +435,10 @@ template <class R> struct Flux {
 // dft_near2far (src/meep.hpp:1455-1520): the accumulators of the tangential fields (the E list)
 // with the medium and the periodic copies, and the equivalent-current list the far-field
 // kernel reads, kept on the device until the fields step again
+// dft_force (src/stress.cpp:27-114): offdiag1 and offdiag2 as the base's E and H lists, diag
+template <class R> struct Force : Flux<R> {
+  std::vector<DftChunk<R> *> D;
+};
 // dft_energy (src/dft.cpp:654-750): the E and H lists of the base, with D and B beside them
 template <class R> struct Energy : Flux<R> {
   std::vector<DftChunk<R> *> D, B;
@@ -3387,7 +3396,7 @@ template <class R> struct Fields : FieldsBase {
   // fields::add_dft for one component (src/dft.cpp:175-235): on the centred grid with the
   // Yee-to-centre average, or on the component's own grid (centered false, no average)
   void add_dft(std::vector<DftChunk<R> *> &out, int c, const V3 &mn, const V3 &mxv, const std::vector<double> &freq, bool include_w,
-               cdouble stored_weight, int decimation, bool centered, int vc = -1) {
+               cdouble stored_weight, int decimation, bool centered, int vc = -1, bool sqrt_w = false) {
     loop_in_chunks(mn, mxv, centered ? 20 : c, [&](const IV &is, const IV &ie, const V3 &s0, const V3 &s1, const V3 &e0, const V3 &e1, double dV0, const IV &shift,
                                    cdouble sph) {
       if (!f[c][0]) return;
@@ -3410,7 +3419,7 @@ template <class R> struct Fields : FieldsBase {
       std::vector<double> w;
       visit_box(is, ie, s0, s1, e0, e1, dV0, [&](int64_t i, int64_t, const IV &, double ww, const int *) {
         idx.push_back(i);
-        w.push_back(include_w ? ww : 1.0);
+        w.push_back(include_w ? (sqrt_w ? std::sqrt(ww) : ww) : 1.0); // src/dft.cpp:277-280
       });
       for (int64_t i : idx)
         if (i < 0 || size_t(i + ch->avg1 + ch->avg2) >= gv.ntot) fail("internal: DFT point outside the array");
@@ -3514,6 +3523,66 @@ template <class R> struct Fields : FieldsBase {
     }
     fluxes.push_back(std::move(en));
     return fluxes.back().get();
+  }
+  // fields::add_dft_force (src/stress.cpp:153-198): off-diagonal terms (fd != the normal) pair
+  // E and H along fd (weighted, times the volume's weight) with E and H along the normal, on
+  // the centred grid; diagonal terms take E and H of every direction with square-root weights
+  // on their own grids and the readout factor +-0.5 times the weight (+ along fd)
+  void *add_dft_force(int nvol, const int *fdirs, const double *vmins, const double *vmaxs, const double *weights,
+                      const std::vector<double> &freq, int dec, int *dec_out) override {
+    if (!components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+    if (nvol < 1) fail("add_dft_force: no volumes");
+    const int decim = dec > 0 ? dec : auto_decimation(freq);
+    *dec_out = decim;
+    std::unique_ptr<Force<R>> fo(new Force<R>);
+    fo->nfreq = int(freq.size());
+    for (int d = 0; d < 3; ++d) fo->wmin.t[d] = vmins[d], fo->wmax.t[d] = vmaxs[d];
+    auto front = [](std::vector<DftChunk<R> *> &to, std::vector<DftChunk<R> *> &lst) {
+      to.insert(to.begin(), lst.rbegin(), lst.rend());
+    };
+    for (int w = 0; w < nvol; ++w) {
+      V3 mn, mx;
+      for (int d = 0; d < 3; ++d) mn.t[d] = vmins[3 * w + d], mx.t[d] = vmaxs[3 * w + d];
+      const int nd = flux_direction(-1, mn, mx), fd = fdirs[w];
+      if (fd < 0 || fd > 2) fail("NO_DIRECTION dft_force is invalid");
+      const cdouble wt(weights[2 * w], weights[2 * w + 1]);
+      if (fd != nd) {
+        for (int base : {0, 5}) {
+          std::vector<DftChunk<R> *> l1, l2;
+          add_dft(l1, dir_comp(base, fd), mn, mx, freq, true, wt, decim, true);
+          add_dft(l2, dir_comp(base, nd), mn, mx, freq, false, cdouble(1.0, 0), decim, true);
+          front(fo->E, l1);
+          front(fo->H, l2);
+        }
+      }
+      else
+        for (int d = 0; d < 3; ++d)
+          for (int base : {0, 5}) {
+            std::vector<DftChunk<R> *> l;
+            add_dft(l, dir_comp(base, d), mn, mx, freq, true, cdouble(1.0, 0), decim, false, -1, true);
+            for (DftChunk<R> *ch : l) ch->extra_weight = wt * (d == fd ? 0.5 : -0.5);
+            front(fo->D, l);
+          }
+    }
+    fluxes.push_back(std::move(fo));
+    return fluxes.back().get();
+  }
+  // dft_force::force (src/stress.cpp:90-114): Re(extra_weight F1 conj(F2)) over the paired
+  // chunks of (offdiag1, offdiag2) and of (diag, diag), the accumulators widened to double
+  void force_spectrum(void *fp, double *out) override {
+    Force<R> *fo = static_cast<Force<R> *>(fp);
+    const int Nf = fo->nfreq;
+    for (int i = 0; i < Nf; ++i) out[i] = 0;
+    auto sum = [&](const std::vector<DftChunk<R> *> &A, const std::vector<DftChunk<R> *> &B) {
+      for (size_t q = 0; q < A.size() && q < B.size(); ++q) {
+        if (A[q]->npts != B[q]->npts) fail("internal: dft_force chunks of different sizes");
+        const std::vector<cdouble> a = read_dft(*A[q]), b = A[q] == B[q] ? a : read_dft(*B[q]);
+        for (size_t k = 0; k < size_t(A[q]->npts); ++k)
+          for (int i = 0; i < Nf; ++i) out[i] += std::real(A[q]->extra_weight * a[k * Nf + i] * std::conj(b[k * Nf + i]));
+      }
+    };
+    sum(fo->E, fo->H);
+    sum(fo->D, fo->D);
   }
   // dft_energy::electric / magnetic / total (src/dft.cpp:681-723): 0.5 Re(conj(F) G) over the
   // paired chunks, the product in the accumulator's type
@@ -3966,6 +4035,7 @@ template <class R> struct Fields : FieldsBase {
         Flux<R> *fl = fluxes[q].get();
         std::vector<std::vector<DftChunk<R> *> *> all{&fl->E, &fl->H};
         if (Energy<R> *en = dynamic_cast<Energy<R> *>(fl)) all.push_back(&en->D), all.push_back(&en->B);
+        if (Force<R> *fo = dynamic_cast<Force<R> *>(fl)) all.push_back(&fo->D);
         for (auto *lst : all)
           for (DftChunk<R> *ch : *lst)
             for (size_t z = 0; z < dft_chunks.size(); ++z)
@@ -5143,6 +5213,32 @@ mp_dft_energy *mp_fields_add_dft_energy(mp_fields *f, int nvol, const double *vm
   return reinterpret_cast<mp_dft_energy *>(h);
 }
 void mp_dft_energy_delete(mp_dft_energy *e) { mp_dft_flux_delete(reinterpret_cast<mp_dft_flux *>(e)); }
+mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, const double *vmins, const double *vmaxs,
+                                      const double *weights, const double *freq, int nfreq, int dec) {
+  FluxHandle *h = new FluxHandle;
+  int rc = guard([&] {
+    std::vector<double> fr(freq, freq + nfreq);
+    std::vector<int> fd(size_t(std::max(nvol, 0)));
+    for (int w = 0; w < nvol; ++w) {
+      const int c = comps[w];
+      fd[w] = c >= 100 ? (c == 100 ? 0 : c == 101 ? 1 : c == 104 ? 2 : -1) : (c >= 0 && c < NC ? comp_dir(c) : -1);
+      if (fd[w] == NODIR) fd[w] = -1;
+    }
+    h->owner = F_;
+    h->nfreq = nfreq;
+    h->fl = F_->add_dft_force(nvol, fd.data(), vmins, vmaxs, weights, fr, dec, &h->decimation);
+  });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_dft_force *>(h);
+}
+void mp_dft_force_delete(mp_dft_force *fo) { mp_dft_flux_delete(reinterpret_cast<mp_dft_flux *>(fo)); }
+int mp_dft_force_force(mp_dft_force *fo, double *out) {
+  FluxHandle *h = reinterpret_cast<FluxHandle *>(fo);
+  return guard([&] { h->owner->force_spectrum(h->fl, out); });
+}
 int mp_dft_energy_get(mp_dft_energy *e, int which, double *out) {
   FluxHandle *h = reinterpret_cast<FluxHandle *>(e);
   return guard([&] {

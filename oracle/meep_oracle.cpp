@@ -72,6 +72,7 @@
  *   MUT_HPOLE_SUB     a magnetic susceptibility's P is not subtracted from B (H = mu^-1 B)
  *   MUT_SCALE_DFTS    scale_dfts scales a flux's E accumulators only
  *   MUT_DFTENERGY_WEIGHT  dft_energy weights D and B as well as E and H (volume counted twice)
+ *   MUT_FORCE_DIAG    the diagonal stress terms all take +0.5 (no -1/2 delta_ij |F|^2 part)
  */
 #include "../include/meep_amd.h"
 
@@ -648,6 +649,8 @@ template <class R> struct DftChunk {
   ivec shift;                   // the periodic image this piece of the monitor stands for
   bool empty_dim[3] = {};       // the monitor volume has no extent in this direction
   int vc = -1;                  // near2far: the equivalent source component
+  bool sqrt_w = false;          // accumulate with the square root of the integration weight
+  cdouble extra_weight = 1.0;   // dft_force: the readout's factor (src/dft.cpp:97,224)
   bool include_w;
   std::vector<double> omega;
   int decimation = 1;
@@ -1234,6 +1237,7 @@ template <class R> struct Fields {
         double w = 1.0;
 #else
         double w = ch.include_w ? loop_weight(gv, L, i1, i2, i3, ch.s0, ch.s1, ch.e0, ch.e1, ch.dV0 + ch.dV1 * i2) : 1.0;
+        if (ch.sqrt_w) w = std::sqrt(w); // src/dft.cpp:277-280
 #endif
         R fv[2] = {0, 0};
         for (int cmp = 0; cmp < numcmp; ++cmp) {
@@ -3394,6 +3398,107 @@ int mp_dft_energy_get(mp_dft_energy *ep, int which, double *out) {
       part(en->E, en->D, e);
       part(en->H, en->B, m);
       for (size_t i = 0; i < Nf; ++i) out[i] = which == 1 ? e[i] : which == 2 ? m[i] : e[i] + m[i];
+      return 0;
+    });
+  });
+}
+
+// ---- dft_force (src/stress.cpp:90-200) -----------------------------------------------------
+struct AnyForce : AnyFlux {
+  std::vector<void *> D; // the diag list (E list: offdiag1, H list: offdiag2)
+};
+// fields::add_dft_force (src/stress.cpp:153-198): for each volume, normal nd from the volume
+// and force direction fd from its component; fd != nd: offdiag1 takes E and H along fd with
+// integration weights and the volume's weight, offdiag2 E and H along nd unweighted, both on
+// the centred grid; fd == nd: diag takes E and H of every field direction with square-root
+// weights on their own grids, the readout factor +-0.5 times the weight (+ along fd).
+mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, const double *vmins, const double *vmaxs,
+                                      const double *weights, const double *freq, int nfreq, int dec) {
+  AnyFields *af = reinterpret_cast<AnyFields *>(f);
+  AnyForce *fo = new AnyForce;
+  fo->owner = af;
+  fo->freq.assign(freq, freq + nfreq);
+  int rc = guard([&] {
+    with_fields(af, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      if (!F.components_allocated) fail("allocate field components (by adding sources) before adding dft objects");
+      if (nvol < 1) fail("add_dft_force: no volumes");
+      const int decim = dec > 0 ? dec : F.auto_decimation(fo->freq);
+      fo->decimation = decim;
+      fo->wmin = to_vec(vmins);
+      fo->wmax = to_vec(vmaxs);
+      auto push = [&](std::vector<void *> &to, std::vector<std::unique_ptr<DftChunk<R>>> &lst) {
+        std::vector<void *> mine;
+        for (int j = int(lst.size()) - 1; j >= 0; --j) {
+          mine.push_back(lst[j].get());
+          F.dft_chunks.push_back(std::move(lst[j]));
+        }
+        to.insert(to.begin(), mine.begin(), mine.end());
+      };
+      for (int w = 0; w < nvol; ++w) {
+        const vec3 mn = to_vec(vmins + 3 * w), mx = to_vec(vmaxs + 3 * w);
+        const int nd = flux_direction(F.gv, -1, mn, mx);
+        const int fd = comps[w] >= 100 ? (comps[w] == 100 ? X : comps[w] == 101 ? Y : comps[w] == 104 ? Z : NODIR)
+                                       : component_direction(comps[w]);
+        if (fd == NODIR) fail("NO_DIRECTION dft_force is invalid");
+        const cdouble wt(weights[2 * w], weights[2 * w + 1]);
+        if (fd != nd) {
+          for (int base : {0, 5}) {
+            std::vector<std::unique_ptr<DftChunk<R>>> l1, l2;
+            F.add_dft(l1, direction_component(base, fd), mn, mx, fo->freq, true, wt, decim, true);
+            F.add_dft(l2, direction_component(base, nd), mn, mx, fo->freq, false, cdouble(1.0, 0), decim, true);
+            push(fo->E, l1);
+            push(fo->H, l2);
+          }
+        }
+        else
+          for (int d = 0; d < 3; ++d) {
+#ifdef MUT_FORCE_DIAG
+            const cdouble w1 = wt * 0.5;
+#else
+            const cdouble w1 = wt * (d == fd ? 0.5 : -0.5);
+#endif
+            for (int base : {0, 5}) {
+              std::vector<std::unique_ptr<DftChunk<R>>> l;
+              F.add_dft(l, direction_component(base, d), mn, mx, fo->freq, true, cdouble(1.0, 0), decim, false);
+              for (auto &ch : l) ch->sqrt_w = true, ch->extra_weight = w1;
+              push(fo->D, l);
+            }
+          }
+      }
+      return 0;
+    });
+  });
+  if (rc) {
+    delete fo;
+    return nullptr;
+  }
+  return reinterpret_cast<mp_dft_force *>(fo);
+}
+void mp_dft_force_delete(mp_dft_force *h) { delete reinterpret_cast<AnyForce *>(h); }
+// dft_force::force (src/stress.cpp:90-114): stress_sum of (offdiag1, offdiag2) and (diag, diag),
+// Re(extra_weight F1 conj(F2)) with the accumulators widened to complex<double>
+int mp_dft_force_force(mp_dft_force *h, double *out) {
+  AnyForce *fo = reinterpret_cast<AnyForce *>(h);
+  return guard([&] {
+    with_fields(fo->owner, [&](auto &F) {
+      typedef typename std::remove_reference<decltype(*F.f[0][0])>::type R;
+      const size_t Nf = fo->freq.size();
+      for (size_t i = 0; i < Nf; ++i) out[i] = 0;
+      auto sum = [&](const std::vector<void *> &A, const std::vector<void *> &B) {
+        for (size_t q = 0; q < A.size() && q < B.size(); ++q) {
+          auto *a = reinterpret_cast<DftChunk<R> *>(A[q]);
+          auto *b = reinterpret_cast<DftChunk<R> *>(B[q]);
+          for (size_t k = 0; k < a->N; ++k)
+            for (size_t i = 0; i < Nf; ++i) {
+              const cdouble x = a->acc == MP_DFT_F64 ? a->dft64[k * Nf + i] : cdouble(a->dft[k * Nf + i]);
+              const cdouble y = b->acc == MP_DFT_F64 ? b->dft64[k * Nf + i] : cdouble(b->dft[k * Nf + i]);
+              out[i] += std::real(a->extra_weight * x * std::conj(y));
+            }
+        }
+      };
+      sum(fo->E, fo->H);
+      sum(fo->D, fo->D);
       return 0;
     });
   });

@@ -88,6 +88,12 @@ run against a value the mathematics fixes; the tests assert on these errors.
   current sheet carries the same electric and magnetic DFT energy, and the total over a
   stretch of length L equals L times the DFT flux through a plane (|E| = |H| and S = u in the
   continuum), up to the lattice's O(dx^2) dispersion, checked at two resolutions.
+* ``dft_force_stress``: dft_force (the Maxwell stress tensor T = E E* + H H* - |F|^2 / 2) on
+  travelling waves: in a 1D line the force through a plane along its normal is minus the DFT
+  flux (T_zz = -|E|^2 with |H| = |E|); for an oblique wave in 2D (a line current with a Bloch
+  phase k_y, at angle sin(theta) = k_y / f) the off-diagonal force along y through an x-plane
+  is -(k_y / f) times the flux and the diagonal one along x is -cos(theta) times the flux,
+  all to the lattice's O(dx^2).
 * ``magnetic_cavity``: a 1D cavity with a magnetic Low wall and a metallic High wall.  The
   magnetic wall zeroes H at the first half pixel (src/boundaries.cpp:197-199) and the metallic
   one E at the last point, so E_j = cos(K (j - 1/2)) with K (n - 1/2) = (m + 1/2) pi is an
@@ -738,3 +744,33 @@ def dft_energy_plane_wave(lib, a=20, single=False, L=1.5, spectra=False):
     return {"equipartition": float(np.abs(e - m).max() / np.abs(t).max()),
             "energy_flux": float(np.abs(t - L * flux).max() / np.abs(t).max()),
             "total": float(np.abs(t - e - m).max() / np.abs(t).max())}
+
+
+def dft_force_stress(lib, a=20, single=False, ky=0.15, spectra=False):
+    """Relative departures of dft_force from the stress of travelling waves (module
+    docstring)."""
+    s = structure(lib, dim=1, size=(0, 0, 8.0), a=a, pml=[(Z, side, 1.0) for side in (High, Low)], single=single)
+    f = fields(s)
+    freqs = np.linspace(0.3, 0.5, 5)
+    f.add_volume_source(Ex, gaussian_src_time(0.4, 0.3), (0, 0, 2.0), (0, 0, 2.0))
+    fl = f.add_dft_flux(Z, (0, 0, 3.5), (0, 0, 3.5), freqs, decimation_factor=1)
+    fz = f.add_dft_force([((0, 0, 3.5), (0, 0, 3.5), 104)], freqs, decimation_factor=1)
+    f.step_n(int(round(60.0 / s.dt)))
+    flux1, force1 = np.asarray(fl.flux()), fz.force()
+    s = structure(lib, dim=2, size=(8.0, 1.0), a=a, pml=[(X, side, 1.0) for side in (High, Low)], single=single)
+    f = fields(s)
+    f.use_bloch_dir(Y, ky)
+    freqs = np.linspace(0.4, 0.6, 5)
+    f.add_volume_source(Ez, gaussian_src_time(0.5, 0.3), (2.0, 0, 0), (2.0, 1.0, 0),
+                        amp_func=lambda x, y, z: np.exp(2j * np.pi * ky * y))
+    fl = f.add_dft_flux(X, (3.5, 0, 0), (3.5, 1.0, 0), freqs, decimation_factor=1)
+    fy = f.add_dft_force([((3.5, 0, 0), (3.5, 1.0, 0), 101)], freqs, decimation_factor=1)
+    fx = f.add_dft_force([((3.5, 0, 0), (3.5, 1.0, 0), 100)], freqs, decimation_factor=1)
+    f.step_n(int(round(60.0 / s.dt)))
+    flux2, force_y, force_x = np.asarray(fl.flux()), fy.force(), fx.force()
+    if spectra:
+        return force1, force_y, force_x
+    sin = ky / freqs
+    return {"diag_1d": float(np.abs(force1 + flux1).max() / np.abs(flux1).max()),
+            "offdiag": float(np.abs(force_y + sin * flux2).max() / np.abs(flux2).max()),
+            "diag_oblique": float(np.abs(force_x + np.sqrt(1 - sin ** 2) * flux2).max() / np.abs(flux2).max())}

@@ -89,3 +89,17 @@ def test_dft_energy_matches_oracle_gpu(gpu_lib, oracle_lib, single):
     tol = 1e-4 if single else 1e-10
     for g, w in zip(got, want):
         assert np.abs(g - w).max() <= tol * np.abs(w).max()
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_dft_force_of_travelling_waves_gpu(gpu_lib, single):
+    T.check_dft_force(gpu_lib, single)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_dft_force_matches_oracle_gpu(gpu_lib, oracle_lib, single):
+    got = CF.dft_force_stress(gpu_lib, 20, single, spectra=True)
+    want = CF.dft_force_stress(oracle_lib, 20, single, spectra=True)
+    tol = 1e-4 if single else 1e-10
+    for g, w in zip(got, want):
+        assert np.abs(g - w).max() <= tol * np.abs(w).max()

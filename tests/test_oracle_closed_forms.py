@@ -48,6 +48,9 @@ values do not reach (tests/closed_forms.py states each closed form):
 * dft_energy of a travelling wave in a 1D line: electric equals magnetic and the total over a
   length L equals L times the DFT flux, to the lattice's O(dx^2) (2e-3 at 20 points per
   unit, falling 4x at 40).
+* dft_force on travelling waves: minus the flux along the normal in 1D, and for an oblique
+  wave (Bloch k_y) -(k_y / f) and -cos(theta) times the flux for the off-diagonal and the
+  diagonal terms, to the lattice's O(dx^2) (1e-4 to 3e-3 at 20 points per unit).
 * Flux and near2far data: loading a run's accumulators negated into an identical run before
   it steps leaves a zero flux and transform, and scale_dfts(s) multiplies a flux by |s|^2 and
   a transform by s (roundoff).
@@ -73,7 +76,8 @@ TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_fre
        "cond_limit": 1e-7, "locality": 1e-13, "dft_array": 1e-7, "flux_time": 1e-12, "pml_refl": 0.03, "pml_stretch": 0.08, "sources": 1e-13,
        "n2f_green": 1e-12, "n2f_fields": 0.05 * 30 / 12, "n2f_flux": 0.08, "n2f_periodic": 1e-12,
        "reload": 1e-12, "reload_f32": 1e-6,
-       "energy_equip": 4e-3, "energy_flux": 1e-4}
+       "energy_equip": 4e-3, "energy_flux": 1e-4,
+       "force_diag": 4e-3, "force_off": 4e-4, "force_oblique": 6e-3}
 
 
 def check_line_source(lib):
@@ -160,6 +164,12 @@ def check_dft_energy(lib, single=False):
     err = CF.dft_energy_plane_wave(lib, 20, single)
     assert err["equipartition"] <= TOL["energy_equip"], err
     assert err["energy_flux"] <= TOL["energy_flux"] and err["total"] <= 1e-6, err
+
+
+def check_dft_force(lib, single=False):
+    err = CF.dft_force_stress(lib, 20, single)
+    assert err["diag_1d"] <= TOL["force_diag"] and err["offdiag"] <= TOL["force_off"], err
+    assert err["diag_oblique"] <= TOL["force_oblique"], err
 
 
 def check_n2f_green(lib):
@@ -359,3 +369,15 @@ def test_dft_energy_equipartition_converges_as_dx_squared(oracle_lib):
     e20 = CF.dft_energy_plane_wave(oracle_lib, 20)["equipartition"]
     e40 = CF.dft_energy_plane_wave(oracle_lib, 40)["equipartition"]
     assert 3.0 <= e20 / e40 <= 5.0, (e20, e40)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_dft_force_of_travelling_waves(oracle_lib, single):
+    check_dft_force(oracle_lib, single)
+
+
+def test_dft_force_converges_as_dx_squared(oracle_lib):
+    e20 = CF.dft_force_stress(oracle_lib, 20)
+    e40 = CF.dft_force_stress(oracle_lib, 40)
+    for key in e20:
+        assert 3.0 <= e20[key] / e40[key] <= 5.0, (key, e20[key], e40[key])

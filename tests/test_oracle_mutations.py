@@ -54,6 +54,7 @@ CHECKS = {
     "HPOLE_SUB": lambda lib: T.check_pole(lib, False, magnetic=True),
     "SCALE_DFTS": T.check_dft_reload,
     "DFTENERGY_WEIGHT": T.check_dft_energy,
+    "FORCE_DIAG": T.check_dft_force,
 }
 
 
