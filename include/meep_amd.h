@@ -339,6 +339,9 @@ mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, 
 void mp_dft_force_delete(mp_dft_force *fo);
 /* dft_force::force() — src/stress.cpp:101-114: out[Nfreq]. */
 int mp_dft_force_force(mp_dft_force *fo, double *out);
+/* fields::get_dft_array(dft_force, component, num_freq, rank, dims) — src/meep.hpp:2122,
+   src/dft.cpp:1270-1280; as mp_dft_flux_get_dft_array over offdiag1, offdiag2 and diag. */
+long mp_dft_force_get_dft_array(mp_dft_force *fo, int c, int num_freq, int *rank, size_t *dims, double *out);
 
 /* ---- time-domain flux ----------------------------------------------------------------- */
 

@@ -184,6 +184,8 @@ _SIGNATURES = {
                                              C.c_int, C.c_int]),
     "mp_dft_force_delete": (None, [C.c_void_p]),
     "mp_dft_force_force": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_dft_force_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                              C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
     "mp_near2far_green": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
                                     C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
@@ -501,6 +503,11 @@ class dft_force:
         out = np.empty(self.nfreq)
         _check(self.lib, self.lib.mp_dft_force_force(self.ptr, _dptr(out)))
         return out
+
+    def get_dft_array(self, c, num_freq):
+        """fields::get_dft_array(dft_force, c, num_freq, ...) (src/dft.cpp:1270-1280)."""
+        return _get_dft_array(self.lib.mp_dft_force_get_dft_array, self.ptr, self.lib,
+                              self.f.s.single, c, num_freq)
 
 
 class near2far:

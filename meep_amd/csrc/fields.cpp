@@ -425,6 +425,7 @@ template <class R> struct DftChunk {
   bool empty[3] = {};
   int vc = -1; // near2far: the equivalent current's component
   cdouble extra_weight = 1.0; // dft_force: the readout's factor (src/dft.cpp:97,224)
+  bool sqrt_w = false;        // accumulated with the square root of the integration weight
 };
 template <class R> struct Flux {
   virtual ~Flux() = default;
@@ -3407,6 +3408,7 @@ template <class R> struct Fields : FieldsBase {
       ch->s0 = s0, ch->s1 = s1, ch->e0 = e0, ch->e1 = e1;
       ch->dV0 = dV0;
       ch->weighted = include_w;
+      ch->sqrt_w = include_w && sqrt_w;
       ch->stored_weight = stored_weight;
       for (int d = 0; d < 3; ++d) ch->empty[d] = gv.has(d) && mn.t[d] == mxv.t[d];
       if (centered)
@@ -3746,6 +3748,10 @@ template <class R> struct Fields : FieldsBase {
     if (both)
       for (const DftChunk<R> *ch : fl->H)
         if (ch->c == c) chs.push_back(ch);
+    // dft_force: offdiag1, offdiag2 and diag (src/dft.cpp:1270-1280)
+    if (const Force<R> *fo = dynamic_cast<const Force<R> *>(fl))
+      for (const DftChunk<R> *ch : fo->D)
+        if (ch->c == c) chs.push_back(ch);
     int lo[3], hi[3];
     for (int d = 0; d < 3; ++d) lo[d] = 2 * gv.num[d] + 1, hi[d] = -1;
     for (const DftChunk<R> *ch : chs)
@@ -3790,7 +3796,7 @@ template <class R> struct Fields : FieldsBase {
       visit_box(ch->is, ch->ie, s0i, s1i, e0i, e1i, 1.0, [&](int64_t, int64_t, const IV &, double w, const int *) { wi.push_back(w); });
       visit_box(ch->is, ch->ie, ch->s0, ch->s1, ch->e0, ch->e1, ch->dV0, [&](int64_t, int64_t k, const IV &p, double w, const int *) {
         cdouble v = acc[size_t(k) * ch->nfreq + nf] / ch->stored_weight;
-        if (ch->weighted && v != 0.0) v /= w;
+        if (ch->weighted && v != 0.0) v /= ch->sqrt_w ? std::sqrt(w) : w; // src/dft.cpp:1013-1014
         size_t at = 0;
         for (int r = 0; r < rank; ++r) {
           const int off = p.t[dir[r]] + ch->shift.t[dir[r]] - lo[dir[r]];
@@ -5117,6 +5123,9 @@ long mp_dft_flux_get_dft_array(mp_dft_flux *fl, int c, int num_freq, int *rank, 
 }
 long mp_dft_fields_get_dft_array(mp_dft_fields *df, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(df, false, c, num_freq, rank, dims, out);
+}
+long mp_dft_force_get_dft_array(mp_dft_force *fo, int c, int num_freq, int *rank, size_t *dims, double *out) {
+  return dft_array(fo, true, c, num_freq, rank, dims, out);
 }
 
 mp_near2far *mp_fields_add_dft_near2far(mp_fields *f, int nsurf, const int *dirs, const double *vmins, const double *vmaxs,

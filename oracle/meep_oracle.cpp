@@ -73,6 +73,7 @@
  *   MUT_SCALE_DFTS    scale_dfts scales a flux's E accumulators only
  *   MUT_DFTENERGY_WEIGHT  dft_energy weights D and B as well as E and H (volume counted twice)
  *   MUT_FORCE_DIAG    the diagonal stress terms all take +0.5 (no -1/2 delta_ij |F|^2 part)
+ *   MUT_DFTARR_SQRT   get_dft_array divides square-root-weighted accumulators by the full weight
  */
 #include "../include/meep_amd.h"
 
@@ -1916,7 +1917,11 @@ void process_dft_chunk(const GV &gv, const DftChunk<R> &ch, int rank, const int 
         v /= ch.stored_weight;
 #endif
 #ifndef MUT_DFTARR_WEIGHT
+#ifdef MUT_DFTARR_SQRT
         if (ch.include_w && v != 0.0) v /= w;
+#else
+        if (ch.include_w && v != 0.0) v /= ch.sqrt_w ? std::sqrt(w) : w; // src/dft.cpp:1013-1014
+#endif
 #endif
         ivec iloc = loop_iloc(L, i1, i2, i3);
         for (int d = 0; d < 3; ++d) {
@@ -2028,6 +2033,7 @@ struct AnyFields {
 };
 struct AnyFlux {
   AnyFields *owner;
+  const std::vector<void *> *diag = nullptr; // dft_force: its diag list
   std::vector<double> freq;
   int decimation;
   // chunk lists in the reference's order: E list then H list (src/dft.cpp:623-641)
@@ -2658,6 +2664,7 @@ static long dft_array(AnyFlux *fl, bool flux, int c, int num_freq, int *rank, si
       if (c < 0 || c >= NC) fail("get_dft_array: component %d is not a field component", c);
       std::vector<std::vector<void *>> lists{fl->E};
       if (flux) lists.push_back(fl->H);
+      if (fl->diag) lists.push_back(*fl->diag); // dft_force (src/dft.cpp:1270-1280)
       std::vector<std::complex<R>> a =
           get_dft_array<R>(F.gv, lists, c, num_freq, fl->freq.size(), fl->wmin, fl->wmax, rank, dims);
       if (out)
@@ -3417,6 +3424,7 @@ mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, 
   AnyFields *af = reinterpret_cast<AnyFields *>(f);
   AnyForce *fo = new AnyForce;
   fo->owner = af;
+  fo->diag = &fo->D;
   fo->freq.assign(freq, freq + nfreq);
   int rc = guard([&] {
     with_fields(af, [&](auto &F) {
@@ -3476,6 +3484,10 @@ mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, 
   return reinterpret_cast<mp_dft_force *>(fo);
 }
 void mp_dft_force_delete(mp_dft_force *h) { delete reinterpret_cast<AnyForce *>(h); }
+// fields::get_dft_array(dft_force, component, num_freq, rank, dims) (src/dft.cpp:1270-1280)
+long mp_dft_force_get_dft_array(mp_dft_force *h, int c, int num_freq, int *rank, size_t *dims, double *out) {
+  return dft_array(reinterpret_cast<AnyFlux *>(h), true, c, num_freq, rank, dims, out);
+}
 // dft_force::force (src/stress.cpp:90-114): stress_sum of (offdiag1, offdiag2) and (diag, diag),
 // Re(extra_weight F1 conj(F2)) with the accumulators widened to complex<double>
 int mp_dft_force_force(mp_dft_force *h, double *out) {

@@ -103,3 +103,10 @@ def test_dft_force_matches_oracle_gpu(gpu_lib, oracle_lib, single):
     tol = 1e-4 if single else 1e-10
     for g, w in zip(got, want):
         assert np.abs(g - w).max() <= tol * np.abs(w).max()
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_force_dft_array_gpu(gpu_lib, oracle_lib, single):
+    got = T.check_force_dft_array(gpu_lib, single)
+    want = T.check_force_dft_array(oracle_lib, single)
+    assert np.abs(got - want).max() <= (1e-4 if single else 1e-10) * np.abs(want).max()

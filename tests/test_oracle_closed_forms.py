@@ -68,7 +68,7 @@ import numpy as np
 import pytest
 
 import closed_forms as CF
-from meep_amd.capi import (Ex, Ez, Hz, MeepError, Z, fields, gaussian_src_time, linspace, pml, structure,
+from meep_amd.capi import (Ex, Ez, Hy, Hz, MeepError, Z, fields, gaussian_src_time, linspace, pml, structure,
                            volone, voltwo)
 
 TOL = {"line": 1e-7, "plane": 1e-6, "spectrum": 1e-5, "p_final": 1e-6, "pole_freq": 1e-3,
@@ -381,3 +381,27 @@ def test_dft_force_converges_as_dx_squared(oracle_lib):
     e40 = CF.dft_force_stress(oracle_lib, 40)
     for key in e20:
         assert 3.0 <= e20[key] / e40[key] <= 5.0, (key, e20[key], e40[key])
+
+
+def check_force_dft_array(lib, single=False):
+    """get_dft_array of a dft_force monitor (square-root weights divided out by their square
+    root, src/dft.cpp:1013-1014) equals that of a field monitor on the components' own grid at
+    the same plane; returns the arrays for parity checks."""
+    s = structure(lib, eps=None, **volone(8.0, 20), single=single)
+    f = fields(s)
+    f.add_volume_source(Ex, gaussian_src_time(0.4, 0.3), (0, 0, 2.0), (0, 0, 2.0))
+    freqs = linspace(0.3, 0.5, 5)
+    fo = f.add_dft_force([((0, 0, 3.52), (0, 0, 3.52), 104)], freqs, decimation_factor=1)
+    df = f.add_dft_fields([Ex, Hy], (0, 0, 3.52), (0, 0, 3.52), freqs, decimation_factor=1, use_centered_grid=False)
+    f.step_n(600)
+    out = []
+    for c in (Ex, Hy):
+        a, b = fo.get_dft_array(c, 2), df.get_dft_array(c, 2)
+        assert abs(complex(a) - complex(b)) <= (1e-5 if single else 1e-12) * abs(complex(b)), (c, a, b)
+        out.append(complex(a))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
+def test_force_dft_array_equals_field_monitor(oracle_lib, single):
+    check_force_dft_array(oracle_lib, single)

@@ -55,6 +55,7 @@ CHECKS = {
     "SCALE_DFTS": T.check_dft_reload,
     "DFTENERGY_WEIGHT": T.check_dft_energy,
     "FORCE_DIAG": T.check_dft_force,
+    "DFTARR_SQRT": T.check_force_dft_array,
 }
 
 
