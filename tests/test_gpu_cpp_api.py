@@ -37,3 +37,17 @@ def test_cpp_known_results_on_gpu(args):
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("engine hip-gfx950")
     assert r.stdout.count("Passed") == 13, r.stdout
+
+
+NEAR2FAR_PROGRAM = os.path.join(ROOT, "tests", "cpp", "_build", "near2far_hip")
+
+
+@pytest.mark.parametrize("source", ["Ez", "Hz"])
+def test_cpp_near2far_on_gpu(source):
+    """tests/cpp/near2far.cpp: the reference's near-to-far check (tests/near2far.cpp) over the
+    C++ API on the engine, within the reference's tolerance."""
+    assert os.path.exists(NEAR2FAR_PROGRAM), "build with __graft_entry__.build() first"
+    print("running", NEAR2FAR_PROGRAM, source, flush=True)
+    r = subprocess.run([NEAR2FAR_PROGRAM, source], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "NEAR2FAR" in r.stdout, r.stdout
