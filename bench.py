@@ -26,7 +26,9 @@ median of five timed steps after one untimed, at one thread and at every thread 
 with the reference's per-phase breakdown.
 
 Secondary keys in the same line: the 256^3 Bloch-periodic box (bandwidth test), the 600 x 400
-reverse-mode gradient against a plain forward run, and the 64-simulation batch.
+reverse-mode gradient against a plain forward run, the 64-simulation batch, and the near-to-far
+transform of a 3D box on a 64 x 64 far plane (Green's-function evaluations per second, FP64
+VALU roofline, the oracle's single-thread rate beside it).
 """
 
 import argparse
@@ -755,6 +757,11 @@ def main():
         c = run_batch2d(sub, lib, steps=200, warmup=20)
         sec["batch2d"] = {k: c[k] for k in ("value", "ms_per_step", "steps", "warmup", "roofline")}
         sec["batch2d"]["workload"] = c["config"]["workload"]
+        sub2 = argparse.Namespace(**vars(args))
+        sub2.no_cpu_baseline = False
+        ff = run_farfield(sub2, lib)
+        sec["farfield"] = {k: ff[k] for k in ("value", "unit", "ms_per_step", "steps", "roofline", "cpu_baseline")}
+        sec["farfield"]["workload"] = ff["config"]["workload"]
         line["secondary"] = sec
     print(json.dumps(line))
 
