@@ -342,6 +342,10 @@ int mp_dft_force_force(mp_dft_force *fo, double *out);
 /* fields::get_dft_array(dft_force, component, num_freq, rank, dims) — src/meep.hpp:2122,
    src/dft.cpp:1270-1280; as mp_dft_flux_get_dft_array over offdiag1, offdiag2 and diag. */
 long mp_dft_force_get_dft_array(mp_dft_force *fo, int c, int num_freq, int *rank, size_t *dims, double *out);
+/* dft_force::scale_dfts — src/stress.cpp:144-148; dft_energy::scale_dfts — src/dft.cpp:785-790
+   (load_minus_force_data / load_minus_energy_data scale by -1). */
+int mp_dft_force_scale_dfts(mp_dft_force *fo, double re, double im);
+int mp_dft_energy_scale_dfts(mp_dft_energy *e, double re, double im);
 
 /* ---- time-domain flux ----------------------------------------------------------------- */
 

@@ -184,6 +184,8 @@ _SIGNATURES = {
                                              C.c_int, C.c_int]),
     "mp_dft_force_delete": (None, [C.c_void_p]),
     "mp_dft_force_force": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mp_dft_force_scale_dfts": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "mp_dft_energy_scale_dfts": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "mp_dft_force_get_dft_array": (C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                               C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
     "mp_near2far_green": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
@@ -488,6 +490,11 @@ class dft_energy:
         """dft_energy::magnetic (src/dft.cpp:697-711)."""
         return self._get(2)
 
+    def scale_dfts(self, scale):
+        """dft_energy::scale_dfts (src/dft.cpp:785-790)."""
+        scale = complex(scale)
+        _check(self.lib, self.lib.mp_dft_energy_scale_dfts(self.ptr, scale.real, scale.imag))
+
 
 class dft_force:
     """meep::dft_force (src/meep.hpp:1292-1320): the Maxwell-stress force spectrum."""
@@ -503,6 +510,11 @@ class dft_force:
         out = np.empty(self.nfreq)
         _check(self.lib, self.lib.mp_dft_force_force(self.ptr, _dptr(out)))
         return out
+
+    def scale_dfts(self, scale):
+        """dft_force::scale_dfts (src/stress.cpp:144-148)."""
+        scale = complex(scale)
+        _check(self.lib, self.lib.mp_dft_force_scale_dfts(self.ptr, scale.real, scale.imag))
 
     def get_dft_array(self, c, num_freq):
         """fields::get_dft_array(dft_force, c, num_freq, ...) (src/dft.cpp:1270-1280)."""

@@ -3718,7 +3718,10 @@ template <class R> struct Fields : FieldsBase {
   // dft_chunk::scale_dft (src/dft.cpp:402-406): dft[i] *= scale, in the accumulator's type
   void flux_scale(void *flp, cdouble sc) override {
     Flux<R> *fl = static_cast<Flux<R> *>(flp);
-    for (auto *lst : {&fl->E, &fl->H})
+    std::vector<std::vector<DftChunk<R> *> *> all{&fl->E, &fl->H};
+    if (Energy<R> *en = dynamic_cast<Energy<R> *>(fl)) all.push_back(&en->D), all.push_back(&en->B);
+    if (Force<R> *fo = dynamic_cast<Force<R> *>(fl)) all.push_back(&fo->D);
+    for (auto *lst : all)
       for (DftChunk<R> *ch : *lst) {
         std::vector<cdouble> v = read_dft(*ch);
         for (cdouble &x : v)
@@ -5244,6 +5247,12 @@ mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, 
   return reinterpret_cast<mp_dft_force *>(h);
 }
 void mp_dft_force_delete(mp_dft_force *fo) { mp_dft_flux_delete(reinterpret_cast<mp_dft_flux *>(fo)); }
+int mp_dft_force_scale_dfts(mp_dft_force *fo, double re, double im) {
+  return mp_dft_flux_scale_dfts(reinterpret_cast<mp_dft_flux *>(fo), re, im);
+}
+int mp_dft_energy_scale_dfts(mp_dft_energy *e, double re, double im) {
+  return mp_dft_flux_scale_dfts(reinterpret_cast<mp_dft_flux *>(e), re, im);
+}
 int mp_dft_force_force(mp_dft_force *fo, double *out) {
   FluxHandle *h = reinterpret_cast<FluxHandle *>(fo);
   return guard([&] { h->owner->force_spectrum(h->fl, out); });

@@ -2033,7 +2033,8 @@ struct AnyFields {
 };
 struct AnyFlux {
   AnyFields *owner;
-  const std::vector<void *> *diag = nullptr; // dft_force: its diag list
+  const std::vector<void *> *diag = nullptr;  // dft_force: its diag list; dft_energy: D
+  const std::vector<void *> *extra = nullptr; // dft_energy: B
   std::vector<double> freq;
   int decimation;
   // chunk lists in the reference's order: E list then H list (src/dft.cpp:623-641)
@@ -2644,9 +2645,9 @@ int mp_dft_flux_scale_dfts(mp_dft_flux *flp, double re, double im) {
 #ifdef MUT_SCALE_DFTS
       for (auto *lst : {&fl->E})
 #else
-      for (auto *lst : {&fl->E, &fl->H})
+      for (auto *lst : std::vector<const std::vector<void *> *>{&fl->E, &fl->H, fl->diag, fl->extra})
 #endif
-        for (void *p : *lst) {
+        for (void *p : lst ? *lst : std::vector<void *>()) {
           auto *ch = reinterpret_cast<DftChunk<R> *>(p);
           for (auto &x : ch->dft) x *= std::complex<R>(R(re), R(im));
           for (auto &x : ch->dft64) x *= cdouble(re, im);
@@ -3335,6 +3336,8 @@ mp_dft_energy *mp_fields_add_dft_energy(mp_fields *f, int nvol, const double *vm
   AnyFields *af = reinterpret_cast<AnyFields *>(f);
   AnyEnergy *en = new AnyEnergy;
   en->owner = af;
+  en->diag = &en->D;
+  en->extra = &en->B;
   en->freq.assign(freq, freq + nfreq);
   int rc = guard([&] {
     with_fields(af, [&](auto &F) {
@@ -3484,6 +3487,13 @@ mp_dft_force *mp_fields_add_dft_force(mp_fields *f, int nvol, const int *comps, 
   return reinterpret_cast<mp_dft_force *>(fo);
 }
 void mp_dft_force_delete(mp_dft_force *h) { delete reinterpret_cast<AnyForce *>(h); }
+// dft_force::scale_dfts (src/stress.cpp:144-148) and dft_energy::scale_dfts (src/dft.cpp:785-790)
+int mp_dft_force_scale_dfts(mp_dft_force *h, double re, double im) {
+  return mp_dft_flux_scale_dfts(reinterpret_cast<mp_dft_flux *>(h), re, im);
+}
+int mp_dft_energy_scale_dfts(mp_dft_energy *h, double re, double im) {
+  return mp_dft_flux_scale_dfts(reinterpret_cast<mp_dft_flux *>(h), re, im);
+}
 // fields::get_dft_array(dft_force, component, num_freq, rank, dims) (src/dft.cpp:1270-1280)
 long mp_dft_force_get_dft_array(mp_dft_force *h, int c, int num_freq, int *rank, size_t *dims, double *out) {
   return dft_array(reinterpret_cast<AnyFlux *>(h), true, c, num_freq, rank, dims, out);

@@ -110,3 +110,7 @@ def test_force_dft_array_gpu(gpu_lib, oracle_lib, single):
     got = T.check_force_dft_array(gpu_lib, single)
     want = T.check_force_dft_array(oracle_lib, single)
     assert np.abs(got - want).max() <= (1e-4 if single else 1e-10) * np.abs(want).max()
+
+
+def test_energy_and_force_scale_dfts_gpu(gpu_lib):
+    T.check_energy_force_scale(gpu_lib)

@@ -405,3 +405,24 @@ def check_force_dft_array(lib, single=False):
 @pytest.mark.parametrize("single", [False, True], ids=["f64", "f32"])
 def test_force_dft_array_equals_field_monitor(oracle_lib, single):
     check_force_dft_array(oracle_lib, single)
+
+
+def check_energy_force_scale(lib):
+    """scale_dfts(s) of dft_energy and dft_force multiplies their spectra by |s|^2 (every list
+    of each monitor scaled)."""
+    s = structure(lib, eps=None, **volone(8.0, 20))
+    f = fields(s)
+    f.add_volume_source(Ex, gaussian_src_time(0.4, 0.3), (0, 0, 2.0), (0, 0, 2.0))
+    freqs = linspace(0.3, 0.5, 5)
+    en = f.add_dft_energy([((0, 0, 3.5), (0, 0, 5.0))], freqs, decimation_factor=1)
+    fo = f.add_dft_force([((0, 0, 3.5), (0, 0, 3.5), 104)], freqs, decimation_factor=1)
+    f.step_n(600)
+    e0, m0, f0 = en.electric(), en.magnetic(), fo.force()
+    en.scale_dfts(2.0 + 1.0j)
+    fo.scale_dfts(2.0 + 1.0j)
+    for a, b in ((en.electric(), e0), (en.magnetic(), m0), (fo.force(), f0)):
+        assert np.abs(a - 5.0 * b).max() <= 1e-12 * np.abs(5.0 * b).max(), (a, b)
+
+
+def test_energy_and_force_scale_dfts(oracle_lib):
+    check_energy_force_scale(oracle_lib)
